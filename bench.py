@@ -1,0 +1,206 @@
+"""Benchmark: frames/s of track + integrate + raycast at 640x480, 1 cm voxels, on one MI355X.
+
+Workload (one step = one frame of the KinectFusion loop of the reference app, main.cpp:98-162):
+  Tracker::track against the previous frame's raycast (skipped for frame 0, which uses ground
+  truth), MapIntegrator::integrateDepth at the tracked pose, raycast_volume at that pose.
+  Sequence: 300 seeded synthetic frames of the room scene along the smooth spline trajectory
+  (<= 2 cm, <= 1 degree per frame), 10.24 m map at 1 cm, truncation 8 voxels, max weight 100,
+  pyramid iterations 10/5/4.  The first `warmup` frames are untimed; the next `steps` frames are
+  timed.  Depth frames are resident in HBM before timing starts.
+
+The FP32 path (reference arithmetic: fp32 TSDF + fp32 ICP reduction) gives `value`; the
+FP16-TSDF + compensated-ICP variant runs on the same frames and is reported beside it.
+
+Prints one JSON line (see DESIGN.md for the field definitions).
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import platform
+import subprocess
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+N_FRAMES_DEFAULT = 300
+HBM_PEAK_GBS = 8000.0
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=290)
+    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--seed", type=int, default=0)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-fp16", action="store_true")
+    ap.add_argument("--cpu-frames", type=int, default=8, help="frames of the CPU baseline sample")
+    return ap.parse_args()
+
+
+def ate(est, gt):
+    e = np.array([np.linalg.norm(a[:3, 3] - b[:3, 3]) for a, b in zip(est, gt)])
+    return float(np.sqrt(np.mean(e**2))) if len(e) else 0.0
+
+
+def run_pipeline(depth_dev, gt, n_frames, warmup, storage, compensated, torch, se, sensor):
+    """Runs the per-frame loop; returns (fps, poses, per-frame stage times of the timed frames)."""
+    H, W = sensor.height, sensor.width
+    gmap = se.TSDFMap(10.24, 0.01, storage=storage)
+    integ = se.MapIntegrator(gmap)
+    tracker = se.Tracker(gmap, sensor, se.TrackerConfig(compensated=compensated))
+    pts = [torch.zeros((H, W, 3), dtype=torch.float32, device="cuda") for _ in range(2)]
+    nrm = [torch.zeros((H, W, 3), dtype=torch.float32, device="cuda") for _ in range(2)]
+    T = gt[0].astype(np.float32)
+    poses = []
+    stages = []
+    t0 = None
+    for k in range(n_frames):
+        if k == warmup:
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+        cur, prev = k % 2, (k + 1) % 2
+        if k > 0:
+            _, T, _ = tracker.track(depth_dev[k], T, pts[prev], nrm[prev])
+        integ.integrateDepth(k, depth_dev[k], sensor, T)
+        se.raycast_volume(gmap, sensor, T, pts[cur], nrm[cur])
+        poses.append(T.copy())
+        if k >= warmup:
+            st = se.stage_times(gmap, tracker)
+            stages.append(st)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    timed = n_frames - warmup
+    fps = timed / (t1 - t0) if timed > 0 else 0.0
+    info = gmap.info()
+    tracker.close()
+    gmap.close()
+    return fps, poses, stages, (t1 - t0), int(info.num_blocks)
+
+
+def cpu_baseline(depth_host, gt, n_frames, sensor):
+    """The oracle (C restatement of the reference, OpenMP) on a bounded prefix of the workload."""
+    from oracle import oracle as O
+
+    cores = os.cpu_count() or 1
+    O.set_threads(cores)
+    pin = O.pinhole(sensor.width, sensor.height, sensor.fx, sensor.fy, sensor.cx, sensor.cy, sensor.near_plane,
+                    sensor.far_plane)
+    m = O.OracleMap(10.24, 0.01)
+    T = gt[0].astype(np.float32)
+    pts = nrm = None
+    # frame 0 (integrate + raycast) is warm-up; frames 1..n-1 are timed
+    t0 = None
+    for k in range(n_frames):
+        if k == 1:
+            t0 = time.perf_counter()
+        if k > 0:
+            _, T, _, _ = O.track(depth_host[k], pin, T, pts, nrm)
+            T = T.astype(np.float32)
+        m.integrate(k, depth_host[k], pin, T)
+        pts, nrm, _ = m.raycast(pin, T)
+    dt = time.perf_counter() - t0
+    return (n_frames - 1) / dt, cores
+
+
+def cpu_model():
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True).stdout
+        for line in out.splitlines():
+            if line.startswith("Model name"):
+                return line.split(":", 1)[1].strip()
+    except Exception:
+        pass
+    return platform.processor() or "unknown"
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only; --gpus {args.gpus} is not supported (see DESIGN.md, sharding).",
+              file=sys.stderr)
+        sys.exit(2)
+    import torch
+
+    from supereight2_amd import capi, se, synth
+
+    capi.load()
+    cam = synth.Camera()
+    sensor = se.PinholeCamera(cam.width, cam.height, cam.fx, cam.fy, cam.cx, cam.cy, cam.near_plane, cam.far_plane)
+    n_frames = min(args.warmup + args.steps, N_FRAMES_DEFAULT) if args.warmup + args.steps <= N_FRAMES_DEFAULT \
+        else args.warmup + args.steps
+    scene = synth.make_scene(args.seed)
+    gt = np.stack(synth.spline_trajectory(max(n_frames, 2), seed=args.seed + 1))[:n_frames]
+    print(f"rendering {n_frames} frames", file=sys.stderr, flush=True)
+    depth_dev = synth.render_sequence_torch(scene, cam, gt, seed=args.seed + 1000)
+    torch.cuda.synchronize()
+
+    print("fp32 pipeline", file=sys.stderr, flush=True)
+    fps32, poses32, stages32, dt32, nblk32 = run_pipeline(depth_dev, gt, n_frames, args.warmup, "f32", False, torch,
+                                                          se, sensor)
+    ate32 = ate(poses32, gt)
+    res = {}
+    if not args.no_fp16:
+        print("fp16 pipeline", file=sys.stderr, flush=True)
+        fps16, poses16, stages16, dt16, nblk16 = run_pipeline(depth_dev, gt, n_frames, args.warmup, "f16", True,
+                                                              torch, se, sensor)
+        res["fp16"] = dict(fps=fps16, ate_m=ate(poses16, gt), blocks=nblk16,
+                           update_ms=float(np.mean([s["update"] for s in stages16])) if stages16 else None,
+                           raycast_ms=float(np.mean([s["raycast"] for s in stages16])) if stages16 else None)
+
+    # roofline of the update kernel (the dominant HBM stream): algorithmic bytes per launch
+    # = 2 * s_v * 512 * N_blk + 4 * W * H with s_v = 8 B/voxel (fp32 tsdf + fp32 weight)
+    def stage_mean(stages, key):
+        return float(np.mean([s[key] for s in stages])) if stages else 0.0
+
+    upd_ms = stage_mean(stages32, "update")
+    nblk = float(np.mean([s["last_frustum_blocks"] + s["last_new_blocks"] for s in stages32])) if stages32 else 0.0
+    upd_bytes = 2 * 8 * 512 * nblk + 4 * cam.width * cam.height
+    upd_gbs = upd_bytes / (upd_ms * 1e-3) / 1e9 if upd_ms > 0 else 0.0
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        print("cpu baseline", file=sys.stderr, flush=True)
+        depth_host = depth_dev[: args.cpu_frames].cpu().numpy()
+        cfps, cores = cpu_baseline(depth_host, gt, args.cpu_frames, sensor)
+        cpu = {"value": cfps, "unit": "frames/s", "cores": cores, "kind": "port",
+               "sample": f"frames 1..{args.cpu_frames - 1} of the same sequence (frame 0 untimed), "
+                         f"track+integrate+raycast, OpenMP on {cores} threads, {cpu_model()}"}
+
+    ms_per_step = 1000.0 * dt32 / max(args.steps, 1)
+    out = {
+        "metric": "frames/s integrate+raycast+ICP @640x480,1cm; FP16-TSDF vs FP32 at stated RMSE",
+        "value": fps32,
+        "unit": "frames/s",
+        "n_gpus": 1,
+        "steps": n_frames - args.warmup,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic: seeded room + 3 spheres, analytic depth with Kinect-like noise, 1% dropout",
+        "config": {"workload": "fp32 TSDF + ICP tracking, 300-frame spline sequence, 640x480, 1 cm voxels, "
+                               "10.24 m map, mu 8 voxels, max weight 100, ICP 10/5/4",
+                   "frames": n_frames, "ate_m": ate32, "blocks": nblk32,
+                   "stage_ms": {k: stage_mean(stages32, k) for k in
+                                ("fetch_frustum", "create_list", "allocate_list", "update", "raycast", "tracking")},
+                   "update_blocks_per_frame": nblk},
+        "fp16_tsdf": res.get("fp16"),
+        "fp16_over_fp32": (res["fp16"]["fps"] / fps32) if "fp16" in res and fps32 > 0 else None,
+        "roofline": {"bound": "hbm", "kernel": "k_update", "achieved": upd_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": upd_gbs / HBM_PEAK_GBS, "traffic": None},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

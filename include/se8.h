@@ -1,0 +1,160 @@
+/*
+ * se8.h - C-ABI of the MI355X-native single-resolution TSDF pipeline (libse8.so).
+ *
+ * Every entry point is extern "C", takes plain pointers, sizes and POD structs, and returns an
+ * int status (SE8_OK = 0, negative on error).  No C++ or torch types cross this boundary.
+ *
+ * Memory convention: arguments named *_dev are device (HBM) pointers, everything else is host
+ * memory.  Poses are row-major 4x4 float matrices (rigid transforms, last row 0 0 0 1).
+ * Work is enqueued on the given HIP stream (NULL = the default stream); entry points that
+ * return host-visible results (counts, poses) synchronise that stream before returning.
+ *
+ * Each declaration names the reference interface it replaces (paths relative to the
+ * supereight2 source tree).  The header-only C++ facade in include/se8/supereight.hpp maps
+ * the reference's templated C++ API onto these functions.
+ */
+#ifndef SE8_H
+#define SE8_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SE8_OK 0
+#define SE8_ERR_INVALID -1  /* bad argument (null pointer, size mismatch, unsupported config) */
+#define SE8_ERR_OOM -2      /* device allocation failed */
+#define SE8_ERR_HIP -3      /* a HIP runtime call failed */
+#define SE8_ERR_CAPACITY -4 /* the block/node/brick pool capacity fixed at map creation is exhausted */
+
+#define SE8_STORAGE_F32 0 /* float tsdf + float weight, 8 B/voxel (reference layout minus padding) */
+#define SE8_STORAGE_F16 1 /* half tsdf + uint16 weight, 4 B/voxel */
+
+typedef void* se8_stream; /* a hipStream_t */
+typedef struct se8_map se8_map;
+typedef struct se8_tracker se8_tracker;
+
+/* se::Map::Config (include/se/map/map.hpp:42-71).  T_MW is a pure translation (the reference's
+ * raycaster assumes no rotation, include/se/map/impl/raycaster_impl.hpp:465); the default
+ * t_MW is dim/2. */
+typedef struct {
+    float dim[3];
+    float res;
+    float t_MW[3];
+} se8_map_config;
+
+/* se::FieldData<Field::TSDF>::Config (include/se/map/data_field.hpp:76-78) plus the storage
+ * precision and the block-pool capacity (blocks of 8^3 voxels) reserved up front. */
+typedef struct {
+    float truncation_boundary_factor; /* default 8 */
+    float max_weight;                 /* default 100 */
+    int storage;                      /* SE8_STORAGE_F32 or SE8_STORAGE_F16 */
+    int64_t max_blocks;               /* <= 0: a default derived from the map size */
+} se8_tsdf_config;
+
+/* se::PinholeCamera::Config (include/se/sensor/pinhole_camera.hpp:24-40, sensor.hpp:30-45);
+ * T_BS is folded into the poses passed by the caller. */
+typedef struct {
+    int width, height;
+    float fx, fy, cx, cy;
+    float near_plane, far_plane;
+} se8_pinhole;
+
+/* se::TrackerConfig (include/se/tracker/tracker.hpp:22-33).  compensated = 1 selects
+ * Kahan-compensated fp32 per-workgroup partials, an FP64 final reduction and an FP64 LDLT
+ * solve; 0 is the plain fp32 reduction + fp32 Cholesky like the reference. */
+typedef struct {
+    int iterations[3];
+    float dist_threshold;
+    float normal_threshold;
+    float track_threshold;
+    float icp_threshold;
+    int compensated;
+} se8_tracker_config;
+
+typedef struct {
+    int octree_size;    /* voxels per side (power of two) */
+    int max_scale;      /* log2(octree_size) */
+    float res;
+    float dim;          /* octree_size * res */
+    int storage;
+    int64_t num_blocks;
+    int64_t num_nodes;  /* root included */
+    int64_t max_blocks;
+    int aabb_valid;
+    int aabb_min[3], aabb_max[3]; /* voxel AABB of allocated blocks (octree_impl.hpp:267-273) */
+} se8_map_info;
+
+/* Per-stage device times of the last call of each kind, in milliseconds, using the reference's
+ * stage names (map_integrator_impl.hpp:63-75, raycast_carver_impl.hpp:58-126, main.cpp:101-217). */
+typedef struct {
+    float fetch_frustum, create_list, allocate_list, update, raycast, tracking;
+    int64_t last_frustum_blocks, last_new_blocks, last_candidates;
+} se8_stage_times;
+
+/* -------- map lifetime: se::Map ctor/dtor (include/se/map/impl/map_impl.hpp:37-55) -------- */
+int se8_default_configs(se8_map_config* map_cfg, se8_tsdf_config* tsdf_cfg, se8_pinhole* pinhole,
+                        se8_tracker_config* tracker_cfg);
+int se8_map_create(const se8_map_config* map_cfg, const se8_tsdf_config* tsdf_cfg, se8_map** out);
+int se8_map_destroy(se8_map* map);
+int se8_map_get_info(const se8_map* map, se8_map_info* out);
+
+/* -------- integration: se::MapIntegrator::integrateDepth
+ * (include/se/integrator/map_integrator.hpp:78-81, impl/map_integrator_impl.hpp:52-83):
+ * RaycastCarver allocation + single-res Updater.  depth_dev is width*height floats, row-major. */
+int se8_integrate_depth(se8_map* map, int timestamp, const float* depth_dev, const se8_pinhole* sensor,
+                        const float T_WC[16], se8_stream stream);
+
+/* -------- raycast: se::raycaster::raycast_volume (include/se/map/raycaster.hpp:56-64,
+ * impl/raycaster_impl.hpp:537-627).  points/normals are width*height*3 floats (x,y,z per pixel);
+ * scale_dev (width*height int8) may be NULL; misses write point 0 and normal 0 and leave scale. */
+int se8_raycast(const se8_map* map, const se8_pinhole* sensor, const float T_WS[16], float* points_dev,
+                float* normals_dev, int8_t* scale_dev, se8_stream stream);
+
+/* -------- tracking: se::Tracker (include/se/tracker/tracker.hpp:42-73,
+ * impl/tracker_impl.hpp:17-116) -------- */
+int se8_tracker_create(const se8_pinhole* sensor, const se8_tracker_config* cfg, se8_tracker** out);
+int se8_tracker_destroy(se8_tracker* tracker);
+/* Tracker::track(depth, T_WS, points_W, normals_W): T_WS is in/out; *ok receives the return
+ * value; sums32 (may be NULL) receives the last reduction (icp.cpp:45-204 layout). */
+int se8_track(se8_tracker* tracker, const float* depth_dev, float T_WS[16], const float* ref_points_dev,
+              const float* ref_normals_dev, int* ok, float* sums32, se8_stream stream);
+/* Tracker::track(depth, T_WS): raycasts map at T_WS first (tracker_impl.hpp:17-26). */
+int se8_track_raycast(se8_tracker* tracker, const se8_map* map, const float* depth_dev, float T_WS[16], int* ok,
+                      se8_stream stream);
+
+/* -------- map queries (host arrays) -------- */
+/* se::allocator::blocks(voxel_coords, octree, root, only_allocated)
+ * (include/se/map/octree/impl/allocator_impl.hpp:40-116): voxel_coords is n*3 ints; the block
+ * coords of the returned blocks go to out_block_coords (n*3 ints, sorted by Morton key). */
+int se8_allocate_blocks(se8_map* map, const int* voxel_coords, int64_t n, int only_new, int* out_block_coords,
+                        int64_t* out_n);
+/* Write/read the 512 voxels (x fastest, BlockSingleRes::data, block_impl.hpp:45-65) of one block. */
+int se8_write_block(se8_map* map, const int block_coord[3], const float* tsdf512, const float* weight512);
+int se8_read_block(const se8_map* map, const int block_coord[3], float* tsdf512, float* weight512);
+/* All allocated blocks sorted by Morton key (key_util_impl.hpp:74-101 layout code<<5|3), with
+ * their timestamps; tsdf/weight (512 per block) may be NULL.  cap = capacity of the arrays. */
+int se8_dump_blocks(const se8_map* map, uint64_t* keys, int* timestamps, float* tsdf, float* weight, int64_t cap,
+                    int64_t* out_n);
+/* All octree nodes (root included) sorted by key, with child masks (octant.hpp:19-72) and
+ * timestamps (propagator_impl.hpp:146-197). */
+int se8_dump_nodes(const se8_map* map, uint64_t* keys, uint8_t* child_masks, int* timestamps, int64_t cap,
+                   int64_t* out_n);
+/* Map::getData / getFieldInterp / getFieldGrad (map_impl.hpp:68-86, 154-172, 239-268) for n
+ * world points (n*3 floats).  valid[i] = 0 where the reference returns std::nullopt/invalid. */
+int se8_get_data(const se8_map* map, const float* points_W, int64_t n, float* tsdf, float* weight);
+int se8_field_interp(const se8_map* map, const float* points_W, int64_t n, float* out, uint8_t* valid);
+int se8_field_grad(const se8_map* map, const float* points_W, int64_t n, float* out3, uint8_t* valid);
+
+int se8_stage_times_get(const se8_map* map, const se8_tracker* tracker, se8_stage_times* out);
+
+/* Human-readable text of the last error on this thread. */
+const char* se8_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SE8_H */

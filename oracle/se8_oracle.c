@@ -1,0 +1,1899 @@
+/*
+ * CPU oracle for the single-resolution TSDF path (allocation, update, raycast, ICP).
+ *
+ * TEST INFRASTRUCTURE ONLY.  This file is a plain-C restatement of the reference
+ * algorithm, used by tests/, by __graft_entry__.smoke() and by the cpu_baseline leg
+ * of bench.py as the checker.  The product (libse8.so) never links, loads or calls it.
+ *
+ * It is compiled twice by oracle/Makefile:
+ *   liboracle_f32.so  (SE8_REAL=float)  - same fp32 operation order as the GPU path
+ *   liboracle_f64.so  (SE8_REAL=double) - FP64 twin used to report precision errors
+ * with -O2 -ffp-contract=off -fno-fast-math and OpenMP loops where the reference has them.
+ *
+ * Floating-point evaluation order (shared with the GPU kernels):
+ *   dot3(a,b)       = a0*b0 + (a1*b1 + a2*b2)         (Eigen's 3-term redux split)
+ *   (R*v)_i         = R_i0*v0 + (R_i1*v1 + R_i2*v2)
+ *   (T*p)_i         = (R*p)_i + t_i
+ *   plane . [c;1]   = (n0*c0 + n1*c1) + (n2*c2 + n3)
+ *   project(p)      = (fx*(x*(1/z)) + cx, fy*(y*(1/z)) + cy), success iff
+ *                     |z| >= 1e-12, u,v in [-0.5, W-0.5) x [-0.5, H-0.5), z > 0
+ *   backProject(u,v)= ((u-cx)*(1/fx), (v-cy)*(1/fy), 1)
+ * srl_projection (the reference's camera library) is not vendored, so the last two
+ * lines are the convention implied by the reference's call sites; they are
+ * "parity unpinned" against the real library.
+ */
+#include <tgmath.h> /* type-generic sqrt/sin/cos/floor follow the real type */
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+#ifndef SE8_REAL
+#define SE8_REAL float
+#endif
+typedef SE8_REAL real;
+
+#define OR_API __attribute__((visibility("default")))
+
+/* ------------------------------------------------------------------------- */
+/* Key operations: include/se/map/utils/impl/key_util_impl.hpp:44-247,406-436 */
+/* masks: include/se/map/utils/key_util.hpp:19-58                            */
+/* ------------------------------------------------------------------------- */
+#define OR_SCALE_OFFSET 5
+#define OR_SCALE_MASK 0x1Full
+
+static uint64_t or_code_mask(int scale)
+{
+    /* bit pattern of CODE_MASK[scale]: 57-bit all-ones with the low 3*scale bits cleared */
+    const uint64_t full = 0x1ffffffffffffffull;
+    return (full >> (3 * scale)) << (3 * scale);
+}
+
+OR_API uint64_t or_expand(uint64_t v)
+{
+    uint64_t x = v & 0x7FFFFull;
+    x = (x | x << 32) & 0x1f00000000ffffull;
+    x = (x | x << 16) & 0x1f0000ff0000ffull;
+    x = (x | x << 8) & 0x100f00f00f00f00full;
+    x = (x | x << 4) & 0x10c30c30c30c30c3ull;
+    x = (x | x << 2) & 0x1249249249249249ull;
+    return x;
+}
+
+OR_API uint64_t or_compact(uint64_t v)
+{
+    uint64_t x = v & 0x49249249249249ull;
+    x = (x | x >> 2) & 0x10c30c30c30c30c3ull;
+    x = (x | x >> 4) & 0x100f00f00f00f00full;
+    x = (x | x >> 8) & 0x1f0000ff0000ffull;
+    x = (x | x >> 16) & 0x1f00000000ffffull;
+    x = (x | x >> 32) & 0x1fffffull;
+    return x;
+}
+
+OR_API uint64_t or_encode_code(int x, int y, int z)
+{
+    return or_expand((uint64_t) x) | (or_expand((uint64_t) y) << 1) | (or_expand((uint64_t) z) << 2);
+}
+
+OR_API void or_decode_code(uint64_t code, int out[3])
+{
+    out[0] = (int) or_compact(code);
+    out[1] = (int) or_compact(code >> 1);
+    out[2] = (int) or_compact(code >> 2);
+}
+
+/* returns 1 when no coordinate detail is lost at this scale (key_util_impl.hpp:74-86) */
+OR_API int or_encode_key(const int c[3], int scale, uint64_t* key)
+{
+    const uint64_t detailed = or_encode_code(c[0], c[1], c[2]);
+    const uint64_t code = detailed & or_code_mask(scale);
+    *key = code << OR_SCALE_OFFSET | (uint64_t) scale;
+    return detailed == code;
+}
+
+OR_API void or_decode_key(uint64_t key, int out[3], int* scale)
+{
+    *scale = (int) (key & OR_SCALE_MASK);
+    or_decode_code(key >> OR_SCALE_OFFSET, out);
+}
+
+OR_API int or_code_to_child_idx(uint64_t code, int child_scale)
+{
+    return (int) ((code >> (child_scale * 3)) & 7);
+}
+
+/* key_util_impl.hpp:212-219 */
+OR_API int or_key_at_scale(uint64_t key, int scale, uint64_t* out)
+{
+    *out = (key & (or_code_mask(scale) << OR_SCALE_OFFSET)) | (uint64_t) scale;
+    return (int) (key & OR_SCALE_MASK) <= scale;
+}
+
+OR_API uint64_t or_parent_key(uint64_t key)
+{
+    const int ps = (int) (key & OR_SCALE_MASK) + 1;
+    return (key & (or_code_mask(ps) << OR_SCALE_OFFSET)) | (uint64_t) ps;
+}
+
+OR_API uint64_t or_block_key(uint64_t key, int max_block_scale)
+{
+    return key & ((or_code_mask(max_block_scale) << OR_SCALE_OFFSET) | OR_SCALE_MASK);
+}
+
+OR_API int or_is_child(uint64_t parent, uint64_t child)
+{
+    const int ps = (int) (parent & OR_SCALE_MASK), cs = (int) (child & OR_SCALE_MASK);
+    if (cs >= ps)
+        return 0;
+    return ((parent >> OR_SCALE_OFFSET) ^ ((child >> OR_SCALE_OFFSET) & or_code_mask(ps))) == 0;
+}
+
+OR_API int or_is_siblings(uint64_t a, uint64_t b)
+{
+    const int sa = (int) (a & OR_SCALE_MASK), sb = (int) (b & OR_SCALE_MASK);
+    if (sa != sb)
+        return 0;
+    const uint64_t m = or_code_mask(sa + 1);
+    return (((a >> OR_SCALE_OFFSET) & m) ^ ((b >> OR_SCALE_OFFSET) & m)) == 0;
+}
+
+static int cmp_u64(const void* a, const void* b)
+{
+    const uint64_t x = *(const uint64_t*) a, y = *(const uint64_t*) b;
+    return x < y ? -1 : (x > y ? 1 : 0);
+}
+
+OR_API void or_sort_keys(uint64_t* keys, long n)
+{
+    qsort(keys, (size_t) n, sizeof(uint64_t), cmp_u64);
+}
+
+/* unique_keys (key_util_impl.hpp:322-340): keys sorted in place, returns count */
+OR_API long or_unique_keys(uint64_t* keys, long n, uint64_t* out)
+{
+    if (n == 0)
+        return 0;
+    or_sort_keys(keys, n);
+    long m = 0;
+    out[m++] = keys[0];
+    for (long i = 0; i < n; i++)
+        if (out[m - 1] != keys[i])
+            out[m++] = keys[i];
+    return m;
+}
+
+/* unique_codes (key_util_impl.hpp:344-366) */
+OR_API long or_unique_codes(uint64_t* keys, long n, uint64_t* out)
+{
+    if (n == 0)
+        return 0;
+    or_sort_keys(keys, n);
+    long m = 0;
+    out[m++] = keys[0];
+    for (long i = 0; i < n; i++) {
+        if ((out[m - 1] >> OR_SCALE_OFFSET) != (keys[i] >> OR_SCALE_OFFSET))
+            out[m++] = keys[i];
+        else if ((out[m - 1] & OR_SCALE_MASK) > (keys[i] & OR_SCALE_MASK))
+            out[m - 1] = keys[i];
+    }
+    return m;
+}
+
+/* unique_allocation (key_util_impl.hpp:370-396) */
+OR_API long or_unique_allocation(uint64_t* keys, long n, int max_block_scale, uint64_t* out)
+{
+    if (n == 0)
+        return 0;
+    or_sort_keys(keys, n);
+    long m = 0;
+    out[m++] = or_block_key(keys[0], max_block_scale);
+    for (long i = 0; i < n; i++) {
+        const uint64_t bk = or_block_key(keys[i], max_block_scale);
+        if (or_is_child(out[m - 1], bk))
+            out[m - 1] = bk;
+        else if (out[m - 1] != bk)
+            out[m++] = bk;
+    }
+    return m;
+}
+
+/* unique_at_scale (key_util_impl.hpp:406-436) */
+OR_API long or_unique_at_scale(uint64_t* keys, long n, int scale, uint64_t* out)
+{
+    if (n == 0)
+        return 0;
+    or_sort_keys(keys, n);
+    long m = 0, restart = 1;
+    uint64_t k;
+    for (long i = 0; i < n; i++) {
+        if (or_key_at_scale(keys[i], scale, &k)) {
+            out[m++] = k;
+            break;
+        }
+        restart++;
+    }
+    for (long i = restart; i < n; i++)
+        if (or_key_at_scale(keys[i], scale, &k) && out[m - 1] != k)
+            out[m++] = k;
+    return m;
+}
+
+/* ------------------------------------------------------------------------- */
+/* Small fixed-size linear algebra in the documented evaluation order.        */
+/* ------------------------------------------------------------------------- */
+typedef struct {
+    real R[3][3];
+    real t[3];
+} or_iso;
+
+static inline real dot3(const real a[3], const real b[3])
+{
+    return a[0] * b[0] + (a[1] * b[1] + a[2] * b[2]);
+}
+
+static inline void lin_apply(const or_iso* T, const real v[3], real out[3])
+{
+    real o[3];
+    for (int i = 0; i < 3; i++)
+        o[i] = T->R[i][0] * v[0] + (T->R[i][1] * v[1] + T->R[i][2] * v[2]);
+    memcpy(out, o, sizeof(o));
+}
+
+static inline void iso_apply(const or_iso* T, const real v[3], real out[3])
+{
+    real o[3];
+    for (int i = 0; i < 3; i++)
+        o[i] = (T->R[i][0] * v[0] + (T->R[i][1] * v[1] + T->R[i][2] * v[2])) + T->t[i];
+    memcpy(out, o, sizeof(o));
+}
+
+/* Isometry product (Eigen transform_transform_product_impl): linear = A.R*B.R,
+ * translation = A.R*B.t + A.t */
+static void iso_mul(const or_iso* A, const or_iso* B, or_iso* out)
+{
+    or_iso o;
+    for (int i = 0; i < 3; i++) {
+        for (int j = 0; j < 3; j++)
+            o.R[i][j] = A->R[i][0] * B->R[0][j] + (A->R[i][1] * B->R[1][j] + A->R[i][2] * B->R[2][j]);
+        o.t[i] = (A->R[i][0] * B->t[0] + (A->R[i][1] * B->t[1] + A->R[i][2] * B->t[2])) + A->t[i];
+    }
+    *out = o;
+}
+
+/* Isometry inverse: R' = R^T, t' = -(R' t) */
+static void iso_inv(const or_iso* A, or_iso* out)
+{
+    or_iso o;
+    for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++)
+            o.R[i][j] = A->R[j][i];
+    for (int i = 0; i < 3; i++)
+        o.t[i] = -(o.R[i][0] * A->t[0] + (o.R[i][1] * A->t[1] + o.R[i][2] * A->t[2]));
+    *out = o;
+}
+
+static void iso_from16(const real* m, or_iso* T)
+{
+    /* row-major 4x4 */
+    for (int i = 0; i < 3; i++) {
+        for (int j = 0; j < 3; j++)
+            T->R[i][j] = m[4 * i + j];
+        T->t[i] = m[4 * i + 3];
+    }
+}
+
+static void iso_to16(const or_iso* T, real* m)
+{
+    for (int i = 0; i < 3; i++) {
+        for (int j = 0; j < 3; j++)
+            m[4 * i + j] = T->R[i][j];
+        m[4 * i + 3] = T->t[i];
+    }
+    m[12] = 0;
+    m[13] = 0;
+    m[14] = 0;
+    m[15] = 1;
+}
+
+static inline real sqnorm3(const real v[3])
+{
+    return dot3(v, v);
+}
+
+static inline real norm3(const real v[3])
+{
+    return sqrt(sqnorm3(v));
+}
+
+static inline void normalized3(const real v[3], real out[3])
+{
+    const real z = sqnorm3(v);
+    if (z > 0) {
+        const real n = sqrt(z);
+        out[0] = v[0] / n;
+        out[1] = v[1] / n;
+        out[2] = v[2] / n;
+    }
+    else {
+        out[0] = v[0];
+        out[1] = v[1];
+        out[2] = v[2];
+    }
+}
+
+static inline void cross3(const real a[3], const real b[3], real out[3])
+{
+    real o[3];
+    o[0] = a[1] * b[2] - a[2] * b[1];
+    o[1] = a[2] * b[0] - a[0] * b[2];
+    o[2] = a[0] * b[1] - a[1] * b[0];
+    memcpy(out, o, sizeof(o));
+}
+
+/* ------------------------------------------------------------------------- */
+/* Pinhole camera: src/sensor/pinhole_camera.cpp:29-111,189-310,             */
+/* include/se/sensor/impl/pinhole_camera_impl.hpp:15-32                      */
+/* ------------------------------------------------------------------------- */
+typedef struct {
+    int width, height;
+    float fx, fy, cx, cy;
+    float near_plane, far_plane;
+} or_pinhole; /* same layout as se8_pinhole in include/se8.h */
+
+typedef struct {
+    int w, h;
+    real fx, fy, cx, cy, inv_fx, inv_fy, near_plane, far_plane;
+    real planes[6][4];
+    int left_hand_frame;
+} or_cam;
+
+static void cam_backproject(const or_cam* c, real u, real v, real out[3])
+{
+    out[0] = (u - c->cx) * c->inv_fx;
+    out[1] = (v - c->cy) * c->inv_fy;
+    out[2] = 1;
+}
+
+static int cam_project(const or_cam* c, const real p[3], real uv[2])
+{
+    if (fabs((double) p[2]) < 1.0e-12)
+        return 0;
+    const real rz = (real) 1 / p[2];
+    const real x = p[0] * rz, y = p[1] * rz;
+    uv[0] = c->fx * x + c->cx;
+    uv[1] = c->fy * y + c->cy;
+    if (uv[0] < (real) -0.5 || uv[1] < (real) -0.5)
+        return 0;
+    if (uv[0] >= (real) c->w - (real) 0.5 || uv[1] >= (real) c->h - (real) 0.5)
+        return 0;
+    return p[2] > 0;
+}
+
+static void plane_normal(const real p1[3], const real p2[3], const real p3[3], real out[3])
+{
+    real t1[3], t2[3], n[3];
+    for (int i = 0; i < 3; i++) {
+        t1[i] = p2[i] - p1[i];
+        t2[i] = p3[i] - p2[i];
+    }
+    cross3(t1, t2, n);
+    normalized3(n, out);
+}
+
+/* scaled = 0: the sensor itself; scaled = 1: PinholeCamera(sensor, dsf) as built for the tracker
+ * pyramid, which applies the downsampling formulas even for dsf = 1 (pinhole_camera.cpp:95-111) */
+static void cam_init_s(or_cam* c, const or_pinhole* p, int dsf, int scaled)
+{
+    const real s = (real) dsf;
+    c->w = (int) ((real) p->width / s);
+    c->h = (int) ((real) p->height / s);
+    if (!scaled) {
+        c->fx = p->fx;
+        c->fy = p->fy;
+        c->cx = p->cx;
+        c->cy = p->cy;
+    }
+    else {
+        c->fx = (real) p->fx / s;
+        c->fy = (real) p->fy / s;
+        c->cx = ((real) p->cx + (real) 0.5) / s - (real) 0.5;
+        c->cy = ((real) p->cy + (real) 0.5) / s - (real) 0.5;
+    }
+    c->inv_fx = (real) 1 / c->fx;
+    c->inv_fy = (real) 1 / c->fy;
+    c->near_plane = p->near_plane;
+    c->far_plane = p->far_plane;
+    c->left_hand_frame = (p->fx < 0.0f) ^ (p->fy < 0.0f);
+
+    /* frustum vertices: corners (0,0) (W,0) (W,H) (0,H) at near and far */
+    real tl[3], tr[3], br[3], bl[3];
+    cam_backproject(c, 0, 0, tl);
+    cam_backproject(c, (real) c->w, 0, tr);
+    cam_backproject(c, (real) c->w, (real) c->h, br);
+    cam_backproject(c, 0, (real) c->h, bl);
+    real v[8][3]; /* TLN TRN BRN BLN TLF TRF BRF BLF */
+    for (int i = 0; i < 3; i++) {
+        v[0][i] = c->near_plane * tl[i];
+        v[4][i] = c->far_plane * tl[i];
+        v[1][i] = c->near_plane * tr[i];
+        v[5][i] = c->far_plane * tr[i];
+        v[2][i] = c->near_plane * br[i];
+        v[6][i] = c->far_plane * br[i];
+        v[3][i] = c->near_plane * bl[i];
+        v[7][i] = c->far_plane * bl[i];
+    }
+    plane_normal(v[4], v[0], v[3], c->planes[0]); /* left   */
+    plane_normal(v[1], v[5], v[6], c->planes[1]); /* right  */
+    plane_normal(v[7], v[3], v[2], c->planes[2]); /* bottom */
+    plane_normal(v[5], v[1], v[0], c->planes[3]); /* top    */
+    for (int i = 0; i < 4; i++)
+        c->planes[i][3] = 0;
+    c->planes[4][0] = 0;
+    c->planes[4][1] = 0;
+    c->planes[4][2] = 1;
+    c->planes[4][3] = -c->near_plane;
+    c->planes[5][0] = 0;
+    c->planes[5][1] = 0;
+    c->planes[5][2] = -1;
+    c->planes[5][3] = c->far_plane;
+    real sign = c->fx * c->fy;
+    sign = sign / (real) fabs((double) sign);
+    for (int i = 0; i < 4; i++)
+        for (int j = 0; j < 3; j++)
+            c->planes[i][j] *= sign;
+}
+
+static void cam_init(or_cam* c, const or_pinhole* p, int dsf)
+{
+    cam_init_s(c, p, dsf, dsf != 1);
+}
+
+static int cam_sphere_in_frustum(const or_cam* c, const real ctr[3], real radius)
+{
+    for (int i = 0; i < 6; i++) {
+        const real* n = c->planes[i];
+        const real d = (n[0] * ctr[0] + n[1] * ctr[1]) + (n[2] * ctr[2] + n[3]);
+        if (d < -radius)
+            return 0;
+    }
+    return 1;
+}
+
+static inline real cam_far_dist(const or_cam* c, const real ray[3])
+{
+    real n[3];
+    normalized3(ray, n);
+    return c->far_plane / n[2];
+}
+
+static inline real cam_near_dist(const or_cam* c, const real ray[3])
+{
+    real n[3];
+    normalized3(ray, n);
+    return c->near_plane / n[2];
+}
+
+/* round_pixel: include/se/common/impl/image_utils_impl.hpp:12-15 */
+static inline void round_pixel(const real uv[2], int px[2])
+{
+    px[0] = (int) (uv[0] + (real) 0.5);
+    px[1] = (int) (uv[1] + (real) 0.5);
+}
+
+/* ------------------------------------------------------------------------- */
+/* Octree + map: include/se/map/octree/impl/octree_impl.hpp:14-19,171-199,    */
+/* include/se/map/impl/map_impl.hpp:37-64,494-590                             */
+/* ------------------------------------------------------------------------- */
+#define OR_BLOCK 8
+#define OR_BLOCK_SCALE 3
+
+typedef struct or_octant {
+    struct or_octant* parent;
+    struct or_octant* child[8];
+    int coord[3];
+    int size;
+    int is_block;
+    int timestamp;
+    unsigned char child_mask;
+    real* tsdf;   /* 512, x fastest, blocks only */
+    real* weight; /* 512 */
+} or_octant;
+
+typedef struct {
+    int size, max_scale;
+    real res, mu, max_weight, mu_factor;
+    real dim;       /* octree size * res */
+    real t_MW[3];   /* map-from-world translation (rotation is identity) */
+    real t_WM[3];
+    or_octant* root;
+    or_octant** blocks;
+    long n_blocks, cap_blocks;
+    long n_nodes;
+    int aabb_min[3], aabb_max[3], aabb_empty;
+} or_map;
+
+static unsigned or_power_two_up(float x)
+{
+    const int p = (int) ceil(log2((double) x));
+    return 1u << p;
+}
+
+static or_octant* new_octant(or_octant* parent, const int coord[3], int size, int is_block)
+{
+    or_octant* o = (or_octant*) calloc(1, sizeof(or_octant));
+    o->parent = parent;
+    memcpy(o->coord, coord, sizeof(int) * 3);
+    o->size = size;
+    o->is_block = is_block;
+    o->timestamp = -1;
+    if (is_block) {
+        o->tsdf = (real*) malloc(sizeof(real) * 512);
+        o->weight = (real*) malloc(sizeof(real) * 512);
+        for (int i = 0; i < 512; i++) {
+            o->tsdf[i] = 1; /* FieldData<TSDF>{} = {1, 0}, data_field.hpp:86-87 */
+            o->weight[i] = 0;
+        }
+    }
+    return o;
+}
+
+OR_API void* or_map_create(float dim, float res, float mu_factor, float max_weight)
+{
+    or_map* m = (or_map*) calloc(1, sizeof(or_map));
+    int size = (int) ceil((double) (dim / res));
+    if (size < 2 * OR_BLOCK)
+        size = 2 * OR_BLOCK;
+    m->size = (int) or_power_two_up((float) size);
+    int s = 0;
+    while ((1 << s) < m->size)
+        s++;
+    m->max_scale = s;
+    m->res = res;
+    m->mu = (real) res * (real) mu_factor;
+    m->mu_factor = mu_factor;
+    m->max_weight = max_weight;
+    m->dim = (real) m->size * (real) res;
+    /* default T_MW = Translation(dim/2) with the user's dim (map_impl.hpp:697) */
+    for (int i = 0; i < 3; i++) {
+        m->t_MW[i] = (real) (dim / 2.0f);
+        m->t_WM[i] = -m->t_MW[i];
+    }
+    const int zero[3] = {0, 0, 0};
+    m->root = new_octant(NULL, zero, m->size, 0);
+    m->n_nodes = 1;
+    m->cap_blocks = 1024;
+    m->blocks = (or_octant**) malloc(sizeof(or_octant*) * m->cap_blocks);
+    m->aabb_empty = 1;
+    return m;
+}
+
+static void free_octant(or_octant* o)
+{
+    if (!o)
+        return;
+    for (int i = 0; i < 8; i++)
+        free_octant(o->child[i]);
+    free(o->tsdf);
+    free(o->weight);
+    free(o);
+}
+
+OR_API void or_map_destroy(void* vm)
+{
+    or_map* m = (or_map*) vm;
+    free_octant(m->root);
+    free(m->blocks);
+    free(m);
+}
+
+OR_API int or_map_size(void* vm)
+{
+    return ((or_map*) vm)->size;
+}
+
+OR_API long or_map_num_blocks(void* vm)
+{
+    return ((or_map*) vm)->n_blocks;
+}
+
+OR_API long or_map_num_nodes(void* vm)
+{
+    return ((or_map*) vm)->n_nodes;
+}
+
+/* Map::contains (map_impl.hpp:58-64) with an identity-rotation T_MW */
+static inline int map_contains(const or_map* m, const real pW[3], real pM[3])
+{
+    for (int i = 0; i < 3; i++)
+        pM[i] = pW[i] + m->t_MW[i];
+    return pM[0] >= 0 && pM[0] < m->dim && pM[1] >= 0 && pM[1] < m->dim && pM[2] >= 0 && pM[2] < m->dim;
+}
+
+/* pointToVoxel<Safe::On> (map_impl.hpp:538-550) */
+static inline int map_point_to_voxel(const or_map* m, const real pW[3], int v[3])
+{
+    real pM[3];
+    if (!map_contains(m, pW, pM)) {
+        v[0] = v[1] = v[2] = -1;
+        return 0;
+    }
+    /* a point within an ulp of the upper bound can round to index == size; wrap it as the
+     * top-down fetcher does (it ignores bits at and above the root size) */
+    for (int i = 0; i < 3; i++)
+        v[i] = ((int) (pM[i] / m->res)) & (m->size - 1);
+    return 1;
+}
+
+static inline int map_point_to_voxel_f(const or_map* m, const real pW[3], real v[3])
+{
+    real pM[3];
+    if (!map_contains(m, pW, pM)) {
+        v[0] = v[1] = v[2] = -1;
+        return 0;
+    }
+    for (int i = 0; i < 3; i++)
+        v[i] = pM[i] / m->res;
+    return 1;
+}
+
+/* fetcher::block (fetcher_impl.hpp:107-126) */
+static or_octant* fetch_block(const or_map* m, const int c[3])
+{
+    or_octant* p = m->root;
+    or_octant* child = NULL;
+    for (int cs = p->size >> 1; cs >= OR_BLOCK; cs >>= 1) {
+        const int idx = ((c[0] & cs) > 0) + 2 * ((c[1] & cs) > 0) + 4 * ((c[2] & cs) > 0);
+        child = p->child[idx];
+        if (!child)
+            return NULL;
+        p = child;
+    }
+    return child;
+}
+
+static inline int block_voxel_idx(const or_octant* b, const int v[3])
+{
+    return (v[0] - b->coord[0]) + (v[1] - b->coord[1]) * 8 + (v[2] - b->coord[2]) * 64;
+}
+
+static void push_block(or_map* m, or_octant* b)
+{
+    if (m->n_blocks == m->cap_blocks) {
+        m->cap_blocks *= 2;
+        m->blocks = (or_octant**) realloc(m->blocks, sizeof(or_octant*) * m->cap_blocks);
+    }
+    m->blocks[m->n_blocks++] = b;
+}
+
+/* Octree::allocate (octree_impl.hpp:171-199) + aabbExtend (:267-273) */
+static int octree_allocate(or_map* m, or_octant* parent, int idx, or_octant** child)
+{
+    *child = parent->child[idx];
+    if (*child)
+        return 0;
+    const int cs = parent->size / 2;
+    const int coord[3] = {parent->coord[0] + cs * ((idx & 1) > 0),
+                          parent->coord[1] + cs * ((idx & 2) > 0),
+                          parent->coord[2] + cs * ((idx & 4) > 0)};
+    if (parent->size == 2 * OR_BLOCK) {
+        *child = new_octant(parent, coord, OR_BLOCK, 1);
+        push_block(m, *child);
+        for (int i = 0; i < 3; i++) {
+            if (m->aabb_empty || coord[i] < m->aabb_min[i])
+                m->aabb_min[i] = coord[i];
+            if (m->aabb_empty || coord[i] + OR_BLOCK - 1 > m->aabb_max[i])
+                m->aabb_max[i] = coord[i] + OR_BLOCK - 1;
+        }
+        if (m->aabb_empty) {
+            for (int i = 0; i < 3; i++) {
+                m->aabb_min[i] = coord[i];
+                m->aabb_max[i] = coord[i] + OR_BLOCK - 1;
+            }
+        }
+        m->aabb_empty = 0;
+    }
+    else {
+        *child = new_octant(parent, coord, cs, 0);
+        m->n_nodes++;
+    }
+    parent->child[idx] = *child;
+    parent->child_mask |= (unsigned char) (1u << idx);
+    return 1;
+}
+
+/* allocator::detail::allocate_key (allocator_impl.hpp:124-147) */
+static int allocate_key(or_map* m, uint64_t key, or_octant** out)
+{
+    or_octant* parent = m->root;
+    or_octant* child = NULL;
+    int cs = m->max_scale - 1;
+    const uint64_t code = key >> OR_SCALE_OFFSET;
+    int ks = (int) (key & OR_SCALE_MASK);
+    const int min_scale = ks > OR_BLOCK_SCALE ? ks : OR_BLOCK_SCALE;
+    int did = 0;
+    for (; cs >= min_scale; --cs) {
+        did = octree_allocate(m, parent, or_code_to_child_idx(code, cs), &child);
+        parent = child;
+    }
+    *out = child;
+    return did;
+}
+
+/* allocator::blocks(keys, ...) (allocator_impl.hpp:68-116).  Keys must be unique voxel keys
+ * at scale <= block scale.  Returns the allocated (or all, when only_allocated == 0) blocks in
+ * sorted key order. */
+static long allocator_blocks(or_map* m, uint64_t* keys, long n, int only_allocated, or_octant** out)
+{
+    uint64_t* tmp = (uint64_t*) malloc(sizeof(uint64_t) * (n > 0 ? n : 1));
+    uint64_t* uk = (uint64_t*) malloc(sizeof(uint64_t) * (n > 0 ? n : 1));
+    for (int scale = m->max_scale; scale > OR_BLOCK_SCALE; scale--) {
+        memcpy(tmp, keys, sizeof(uint64_t) * n);
+        const long nu = or_unique_at_scale(tmp, n, scale, uk);
+        for (long i = 0; i < nu; i++) {
+            or_octant* unused;
+            allocate_key(m, uk[i], &unused);
+        }
+    }
+    free(tmp);
+    free(uk);
+    long k = 0;
+    for (long i = 0; i < n; i++) {
+        or_octant* b;
+        const int did = allocate_key(m, keys[i], &b);
+        if (!(only_allocated && !did))
+            out[k++] = b;
+    }
+    return k;
+}
+
+/* allocator::blocks(voxel_coords, ...) (allocator_impl.hpp:40-64): returns the block coords of
+ * the returned blocks in out_coords (3 ints each), count as return value. */
+OR_API long or_allocate_blocks(void* vm, const int* voxel_coords, long n, int only_new, int* out_coords)
+{
+    or_map* m = (or_map*) vm;
+    uint64_t* keys = (uint64_t*) malloc(sizeof(uint64_t) * (n > 0 ? n : 1));
+    uint64_t* uk = (uint64_t*) malloc(sizeof(uint64_t) * (n > 0 ? n : 1));
+    for (long i = 0; i < n; i++)
+        or_encode_key(voxel_coords + 3 * i, OR_BLOCK_SCALE, &keys[i]);
+    const long nu = or_unique_keys(keys, n, uk);
+    or_octant** bl = (or_octant**) malloc(sizeof(or_octant*) * (nu > 0 ? nu : 1));
+    const long k = allocator_blocks(m, uk, nu, only_new, bl);
+    for (long i = 0; i < k; i++)
+        memcpy(out_coords + 3 * i, bl[i]->coord, sizeof(int) * 3);
+    free(keys);
+    free(uk);
+    free(bl);
+    return k;
+}
+
+OR_API int or_set_block_data(void* vm, const int block_coord[3], const real* tsdf, const real* weight)
+{
+    or_map* m = (or_map*) vm;
+    or_octant* b = fetch_block(m, block_coord);
+    if (!b)
+        return -1;
+    memcpy(b->tsdf, tsdf, sizeof(real) * 512);
+    memcpy(b->weight, weight, sizeof(real) * 512);
+    return 0;
+}
+
+OR_API int or_map_aabb(void* vm, int out[6])
+{
+    or_map* m = (or_map*) vm;
+    for (int i = 0; i < 3; i++) {
+        out[i] = m->aabb_min[i];
+        out[3 + i] = m->aabb_max[i];
+    }
+    return !m->aabb_empty;
+}
+
+static int cmp_octant_key(const void* a, const void* b)
+{
+    const or_octant* x = *(or_octant* const*) a;
+    const or_octant* y = *(or_octant* const*) b;
+    const uint64_t kx = or_encode_code(x->coord[0], x->coord[1], x->coord[2]);
+    const uint64_t ky = or_encode_code(y->coord[0], y->coord[1], y->coord[2]);
+    return kx < ky ? -1 : (kx > ky ? 1 : 0);
+}
+
+/* Blocks sorted by Morton key; tsdf/weight as 512-arrays per block (x fastest). */
+OR_API void or_map_dump_blocks(void* vm, uint64_t* keys, int* timestamps, real* tsdf, real* weight)
+{
+    or_map* m = (or_map*) vm;
+    or_octant** tmp = (or_octant**) malloc(sizeof(or_octant*) * (m->n_blocks + 1));
+    memcpy(tmp, m->blocks, sizeof(or_octant*) * m->n_blocks);
+    qsort(tmp, (size_t) m->n_blocks, sizeof(or_octant*), cmp_octant_key);
+    for (long i = 0; i < m->n_blocks; i++) {
+        or_encode_key(tmp[i]->coord, OR_BLOCK_SCALE, &keys[i]);
+        if (timestamps)
+            timestamps[i] = tmp[i]->timestamp;
+        if (tsdf)
+            memcpy(tsdf + 512 * i, tmp[i]->tsdf, sizeof(real) * 512);
+        if (weight)
+            memcpy(weight + 512 * i, tmp[i]->weight, sizeof(real) * 512);
+    }
+    free(tmp);
+}
+
+static void collect_nodes(or_octant* o, int scale, uint64_t* keys, unsigned char* masks, int* ts, long* k)
+{
+    if (!o || o->is_block)
+        return;
+    or_encode_key(o->coord, scale, &keys[*k]);
+    masks[*k] = o->child_mask;
+    if (ts)
+        ts[*k] = o->timestamp;
+    (*k)++;
+    for (int i = 0; i < 8; i++)
+        collect_nodes(o->child[i], scale - 1, keys, masks, ts, k);
+}
+
+/* All nodes (root included) as (key, child_mask, timestamp), unsorted (DFS order). */
+OR_API long or_map_dump_nodes(void* vm, uint64_t* keys, unsigned char* masks, int* timestamps)
+{
+    or_map* m = (or_map*) vm;
+    long k = 0;
+    collect_nodes(m->root, m->max_scale, keys, masks, timestamps, &k);
+    return k;
+}
+
+/* ------------------------------------------------------------------------- */
+/* Frustum fetch: raycast_carver_impl.hpp:17-32, iterator.hpp:197-238,        */
+/* iterator_impl.hpp:86-156.  Root children are always visited (the root's own */
+/* test does not prune), every other octant is pruned by its sphere test.      */
+/* ------------------------------------------------------------------------- */
+static int octant_ignored(const or_map* m, const or_cam* cam, const or_iso* T_SW, const or_octant* o)
+{
+    real cW[3], cS[3];
+    for (int i = 0; i < 3; i++) {
+        const real pM = ((real) o->coord[i] + (real) o->size * (real) 0.5) * m->res;
+        cW[i] = pM + m->t_WM[i];
+    }
+    iso_apply(T_SW, cW, cS);
+    const real r = (real) (sqrtf(3.0f) / 2.0f) * m->res * (real) o->size;
+    return !cam_sphere_in_frustum(cam, cS, r);
+}
+
+static long fetch_frustum(const or_map* m, const or_cam* cam, const or_iso* T_WS, or_octant*** out)
+{
+    or_iso T_SW;
+    iso_inv(T_WS, &T_SW);
+    long cap = 1024, n = 0;
+    or_octant** list = (or_octant**) malloc(sizeof(or_octant*) * cap);
+    long scap = 1024, sn = 0;
+    or_octant** stack = (or_octant**) malloc(sizeof(or_octant*) * scap);
+    for (int i = 0; i < 8; i++)
+        if (m->root->child[i])
+            stack[sn++] = m->root->child[i];
+    while (sn > 0) {
+        or_octant* o = stack[--sn];
+        if (octant_ignored(m, cam, &T_SW, o))
+            continue;
+        if (!o->is_block) {
+            for (int i = 0; i < 8; i++) {
+                if (o->child[i]) {
+                    if (sn == scap) {
+                        scap *= 2;
+                        stack = (or_octant**) realloc(stack, sizeof(or_octant*) * scap);
+                    }
+                    stack[sn++] = o->child[i];
+                }
+            }
+        }
+        else {
+            if (n == cap) {
+                cap *= 2;
+                list = (or_octant**) realloc(list, sizeof(or_octant*) * cap);
+            }
+            list[n++] = o;
+        }
+    }
+    free(stack);
+    *out = list;
+    return n;
+}
+
+/* ------------------------------------------------------------------------- */
+/* TSDF integration: map_integrator_impl.hpp:52-83, raycast_carver_impl.hpp:55-128, */
+/* singleres_tsdf_updater_impl.hpp:14-132, data_field_impl.hpp:39-54,        */
+/* propagator_impl.hpp:146-197                                               */
+/* ------------------------------------------------------------------------- */
+
+/* One voxel update; returns 1 if the voxel changed. */
+static inline int field_update(real* tsdf, real* weight, real sdf, real mu, real max_w)
+{
+    if (sdf < -mu)
+        return 0;
+    real u = sdf / mu;
+    if (u > (real) 1)
+        u = (real) 1;
+    if (*weight < max_w)
+        *weight = *weight + (real) 1;
+    *tsdf = (*tsdf * (*weight - (real) 1) + u) / *weight;
+    return 1;
+}
+
+static void update_block(const or_map* m, or_octant* b, const or_iso* T_CV, const or_cam* cam,
+                         const float* depth, int ts)
+{
+    b->timestamp = ts;
+    for (int x = 0; x < 8; x++) {
+        for (int y = 0; y < 8; y++) {
+            for (int z = 0; z < 8; z++) {
+                const real v[3] = {(real) (b->coord[0] + x), (real) (b->coord[1] + y), (real) (b->coord[2] + z)};
+                real pC[3];
+                iso_apply(T_CV, v, pC);
+                const real nrm = norm3(pC);
+                if (nrm > cam_far_dist(cam, pC))
+                    continue;
+                real uv[2];
+                if (!cam_project(cam, pC, uv))
+                    continue;
+                int px[2];
+                round_pixel(uv, px);
+                const real d = depth[px[0] + px[1] * cam->w];
+                if (d < cam->near_plane)
+                    continue;
+                const real mz = pC[2];
+                const real sdf = (d - mz) / mz * norm3(pC);
+                const int vi = x + 8 * y + 64 * z;
+                field_update(&b->tsdf[vi], &b->weight[vi], sdf, m->mu, m->max_weight);
+            }
+        }
+    }
+}
+
+/* The allocation band march (raycast_carver_impl.hpp:74-111) for one pixel.  Writes the
+ * keys of unallocated blocks hit by the samples into keys (at most num_steps) and returns
+ * their count. */
+static int carve_pixel(const or_map* m, const or_cam* cam, const or_iso* T_WS, const float* depth,
+                       int x, int y, real band, int num_steps, uint64_t* keys)
+{
+    const real d = depth[x + y * cam->w];
+    if (d < cam->near_plane || d > (cam->far_plane + band * (real) 0.5))
+        return 0;
+    real dir[3], pC[3], pW[3];
+    cam_backproject(cam, (real) x, (real) y, dir);
+    for (int i = 0; i < 3; i++)
+        pC[i] = d * dir[i];
+    iso_apply(T_WS, pC, pW);
+    real rev[3], diff[3];
+    for (int i = 0; i < 3; i++)
+        diff[i] = T_WS->t[i] - pW[i];
+    normalized3(diff, rev);
+    real pos[3], step[3];
+    for (int i = 0; i < 3; i++) {
+        pos[i] = pW[i] - (band * (real) 0.5) * rev[i];
+        step[i] = (rev[i] * band) / (real) num_steps;
+    }
+    int k = 0;
+    for (int s = 0; s < num_steps; s++) {
+        int vc[3];
+        if (map_point_to_voxel(m, pos, vc)) {
+            if (fetch_block(m, vc) == NULL)
+                or_encode_key(vc, OR_BLOCK_SCALE, &keys[k++]);
+        }
+        for (int i = 0; i < 3; i++)
+            pos[i] = pos[i] + step[i];
+    }
+    return k;
+}
+
+/* T_CV = T_CW * T_WM * Scaling(res) * Translation(0.5) (singleres_tsdf_updater_impl.hpp:37-42) */
+static void compute_T_CV(const or_map* m, const or_iso* T_WC, or_iso* T_CV)
+{
+    or_iso T_CW, T_WM, A;
+    iso_inv(T_WC, &T_CW);
+    memset(&T_WM, 0, sizeof(T_WM));
+    for (int i = 0; i < 3; i++) {
+        T_WM.R[i][i] = 1;
+        T_WM.t[i] = m->t_WM[i];
+    }
+    iso_mul(&T_CW, &T_WM, &A);
+    for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++)
+            A.R[i][j] = A.R[i][j] * m->res;
+    const real h[3] = {(real) 0.5, (real) 0.5, (real) 0.5};
+    for (int i = 0; i < 3; i++)
+        A.t[i] = A.t[i] + (A.R[i][0] * h[0] + (A.R[i][1] * h[1] + A.R[i][2] * h[2]));
+    *T_CV = A;
+}
+
+OR_API void or_compute_T_CV(void* vm, const real T_WC16[16], real out16[16])
+{
+    or_iso T_WC, T_CV;
+    iso_from16(T_WC16, &T_WC);
+    compute_T_CV((or_map*) vm, &T_WC, &T_CV);
+    iso_to16(&T_CV, out16);
+}
+
+/* stats: [0] frustum blocks, [1] new blocks, [2] candidate keys (with per-pixel repeats),
+ *        [3] valid pixels */
+OR_API void or_integrate(void* vm, int ts, const float* depth, const or_pinhole* pin, const real T_WC16[16],
+                         long stats[4])
+{
+    or_map* m = (or_map*) vm;
+    or_cam cam;
+    cam_init(&cam, pin, 1);
+    or_iso T_WC;
+    iso_from16(T_WC16, &T_WC);
+    /* band_ = 2 * res * truncation_boundary_factor (raycast_carver_impl.hpp:50) */
+    const real band = (real) 2 * m->res * m->mu_factor;
+
+    /* fetch-frustum (before any allocation) */
+    or_octant** frustum = NULL;
+    const long n_frustum = fetch_frustum(m, &cam, &T_WC, &frustum);
+
+    /* create-list */
+    const int num_steps = (int) ceil((double) (band / ((real) 2 * m->res)));
+    const long npx = (long) cam.w * cam.h;
+    uint64_t* cand = (uint64_t*) malloc(sizeof(uint64_t) * (size_t) npx * (size_t) num_steps);
+    int* cnt = (int*) calloc((size_t) npx, sizeof(int));
+    long valid = 0;
+#pragma omp parallel for reduction(+ : valid) schedule(static)
+    for (int x = 0; x < cam.w; x++) {
+        for (int y = 0; y < cam.h; y++) {
+            const long p = x + (long) y * cam.w;
+            const real d = depth[p];
+            if (!(d < cam.near_plane || d > (cam.far_plane + band * (real) 0.5)))
+                valid++;
+            cnt[p] = carve_pixel(m, &cam, &T_WC, depth, x, y, band, num_steps, cand + p * num_steps);
+        }
+    }
+    long nk = 0;
+    for (long p = 0; p < npx; p++)
+        for (int s = 0; s < cnt[p]; s++)
+            cand[nk++] = cand[p * num_steps + s];
+    uint64_t* uk = (uint64_t*) malloc(sizeof(uint64_t) * (nk > 0 ? nk : 1));
+    const long nu = or_unique_keys(cand, nk, uk);
+    free(cand);
+    free(cnt);
+
+    /* allocate-list */
+    or_octant** newb = (or_octant**) malloc(sizeof(or_octant*) * (nu > 0 ? nu : 1));
+    const long n_new = allocator_blocks(m, uk, nu, 1, newb);
+    free(uk);
+
+    /* update over new + frustum blocks */
+    const long nb = n_new + n_frustum;
+    or_octant** list = (or_octant**) malloc(sizeof(or_octant*) * (nb > 0 ? nb : 1));
+    memcpy(list, newb, sizeof(or_octant*) * n_new);
+    memcpy(list + n_new, frustum, sizeof(or_octant*) * n_frustum);
+    or_iso T_CV;
+    compute_T_CV(m, &T_WC, &T_CV);
+#pragma omp parallel for schedule(dynamic, 16)
+    for (long i = 0; i < nb; i++)
+        update_block(m, list[i], &T_CV, &cam, depth, ts);
+    /* propagateTimeStampToRoot: only the blocks' direct parents are reached */
+    for (long i = 0; i < nb; i++) {
+        or_octant* p = list[i]->parent;
+        if (list[i]->timestamp > p->timestamp)
+            p->timestamp = list[i]->timestamp;
+    }
+    if (stats) {
+        stats[0] = n_frustum;
+        stats[1] = n_new;
+        stats[2] = nk;
+        stats[3] = valid;
+    }
+    free(list);
+    free(newb);
+    free(frustum);
+}
+
+/* ------------------------------------------------------------------------- */
+/* Queries: visitor_impl.hpp:67-302,704-737,841-864,976-1024,1094-1251,       */
+/* map_impl.hpp:68-86,154-172,239-268                                        */
+/* ------------------------------------------------------------------------- */
+static inline int get_voxel(const or_map* m, const int v[3], real* f)
+{
+    /* getData via fetcher::leaf: data of the block if allocated, otherwise invalid */
+    if (v[0] < 0 || v[1] < 0 || v[2] < 0 || v[0] >= m->size || v[1] >= m->size || v[2] >= m->size)
+        return 0;
+    const or_octant* b = fetch_block(m, v);
+    if (!b)
+        return 0;
+    const int i = block_voxel_idx(b, v);
+    if (b->weight[i] == (real) 0)
+        return 0;
+    *f = b->tsdf[i];
+    return 1;
+}
+
+static int field_interp_vox(const or_map* m, const real vf[3], real* out)
+{
+    real bf[3];
+    int base[3];
+    for (int i = 0; i < 3; i++) {
+        bf[i] = vf[i] - (real) 0.5;
+        base[i] = (int) bf[i];
+    }
+    if (m->aabb_empty)
+        return 0;
+    for (int i = 0; i < 3; i++)
+        if (base[i] < m->aabb_min[i] || base[i] > m->aabb_max[i])
+            return 0;
+    for (int i = 0; i < 3; i++)
+        if (base[i] + 1 >= m->size || base[i] < 0)
+            return 0;
+    real d[8];
+    for (int k = 0; k < 8; k++) {
+        const int c[3] = {base[0] + (k & 1), base[1] + ((k >> 1) & 1), base[2] + ((k >> 2) & 1)};
+        const or_octant* b = fetch_block(m, c);
+        if (!b)
+            return 0;
+        const int i = block_voxel_idx(b, c);
+        if (b->weight[i] == (real) 0)
+            return 0;
+        d[k] = b->tsdf[i];
+    }
+    real t[3], tc[3];
+    for (int i = 0; i < 3; i++) {
+        t[i] = bf[i] - floor(bf[i]);
+        tc[i] = (real) 1 - t[i];
+    }
+    *out = ((d[0] * tc[0] + d[1] * t[0]) * tc[1] + (d[2] * tc[0] + d[3] * t[0]) * t[1]) * tc[2]
+        + ((d[4] * tc[0] + d[5] * t[0]) * tc[1] + (d[6] * tc[0] + d[7] * t[0]) * t[1]) * t[2];
+    return 1;
+}
+
+OR_API int or_get_field_interp(void* vm, const real pW[3], real* out)
+{
+    const or_map* m = (const or_map*) vm;
+    real vf[3];
+    if (!map_point_to_voxel_f(m, pW, vf))
+        return 0;
+    return field_interp_vox(m, vf, out);
+}
+
+/* Single-res getFieldGrad: trilinear blend of central differences at the 8 voxels around the
+ * query, all 32 samples must be valid (visitor_impl.hpp:1094-1251), then x0.5 and /res
+ * (map_impl.hpp:264-266). */
+static int field_grad_vox(const or_map* m, const real vf[3], real g[3])
+{
+    real s[3], fac[3];
+    int base[3];
+    for (int i = 0; i < 3; i++) {
+        s[i] = vf[i] - (real) 0.5;
+        fac[i] = s[i] - floor(s[i]);
+        base[i] = (int) s[i];
+    }
+    int lo[3][4]; /* per axis: b-1, b, b+1, b+2 clamped */
+    for (int i = 0; i < 3; i++) {
+        lo[i][0] = base[i] - 1 > 0 ? base[i] - 1 : 0;
+        lo[i][1] = base[i] > 0 ? base[i] : 0;
+        lo[i][2] = base[i] + 1 < m->size - 1 ? base[i] + 1 : m->size - 1;
+        lo[i][3] = base[i] + 2 < m->size - 1 ? base[i] + 2 : m->size - 1;
+    }
+    if (!fetch_block(m, base))
+        return 0;
+    /* central difference along axis a at corner (cx,cy,cz) in {0,1}^3 */
+    real cd[3][8];
+    for (int a = 0; a < 3; a++) {
+        for (int k = 0; k < 8; k++) {
+            const int c[3] = {k & 1, (k >> 1) & 1, (k >> 2) & 1};
+            int vp[3], vm_[3];
+            for (int i = 0; i < 3; i++) {
+                vp[i] = lo[i][1 + c[i]];
+                vm_[i] = lo[i][1 + c[i]];
+            }
+            vp[a] = lo[a][2 + c[a]];
+            vm_[a] = lo[a][c[a]];
+            real fp, fm;
+            if (!get_voxel(m, vp, &fp) || !get_voxel(m, vm_, &fm))
+                return 0;
+            cd[a][k] = fp - fm;
+        }
+    }
+    const real rx = (real) 1 - fac[0], ry = (real) 1 - fac[1], rz = (real) 1 - fac[2];
+    for (int a = 0; a < 3; a++) {
+        const real* c = cd[a];
+        const real v = ((c[0] * rx + c[1] * fac[0]) * ry + (c[2] * rx + c[3] * fac[0]) * fac[1]) * rz
+            + ((c[4] * rx + c[5] * fac[0]) * ry + (c[6] * rx + c[7] * fac[0]) * fac[1]) * fac[2];
+        g[a] = (real) 0.5 * v;
+    }
+    return 1;
+}
+
+OR_API int or_get_field_grad(void* vm, const real pW[3], real out[3])
+{
+    const or_map* m = (const or_map*) vm;
+    real vf[3];
+    if (!map_point_to_voxel_f(m, pW, vf))
+        return 0;
+    if (!field_grad_vox(m, vf, out))
+        return 0;
+    for (int i = 0; i < 3; i++)
+        out[i] = out[i] / m->res;
+    return 1;
+}
+
+OR_API int or_get_data(void* vm, const real pW[3], real* tsdf, real* weight)
+{
+    const or_map* m = (const or_map*) vm;
+    int v[3];
+    *tsdf = 1;
+    *weight = 0;
+    if (!map_point_to_voxel(m, pW, v))
+        return 0;
+    const or_octant* b = fetch_block(m, v);
+    if (!b)
+        return 0;
+    const int i = block_voxel_idx(b, v);
+    *tsdf = b->tsdf[i];
+    *weight = b->weight[i];
+    return 1;
+}
+
+/* ------------------------------------------------------------------------- */
+/* First allocated block along a ray, in the scaled [1,2]^3 octree space of   */
+/* voxel_block_ray_iterator_impl.hpp:19-249 (ESVO, Laine & Karras 2010).  The  */
+/* traversal runs in fp32 in both builds; only t_min/t_max leave it.          */
+/* ------------------------------------------------------------------------- */
+#define OR_STACK 23
+
+static inline int f2i(float f)
+{
+    int i;
+    memcpy(&i, &f, 4);
+    return i;
+}
+
+static inline float i2f(int i)
+{
+    float f;
+    memcpy(&f, &i, 4);
+    return f;
+}
+
+typedef struct {
+    float t_min_ray, t_max_ray; /* tcmin, tmax in metres */
+} or_ray_span;
+
+static void esvo_first_block(const or_map* m, const float o_W[3], const float d_W[3], float near_t, float far_t,
+                             or_ray_span* out)
+{
+    const int size = m->size;
+    const float scaling = (float) size * (float) m->res;
+    int lg = 0;
+    while ((1 << lg) < size / OR_BLOCK)
+        lg++;
+    const int min_scale = OR_STACK - lg;
+    int lgs = 0;
+    while ((1 << lgs) < size)
+        lgs++;
+    const float eps = exp2f((float) -lgs);
+    float d[3], o[3], tc[3], tb[3];
+    for (int i = 0; i < 3; i++) {
+        d[i] = fabsf(d_W[i]) < eps ? copysignf(eps, d_W[i]) : d_W[i];
+        /* pointToVoxel<Safe::Off> then /size + 1 */
+        const float pM = o_W[i] + (float) m->t_MW[i];
+        const float vox = pM / (float) m->res;
+        o[i] = vox / (float) size + 1.0f;
+        tc[i] = -1.f * (1.0f / fabsf(d[i]));
+        tb[i] = tc[i] * o[i];
+    }
+    int octant_mask = 7;
+    for (int i = 0; i < 3; i++) {
+        if (d[i] > 0.0f) {
+            octant_mask ^= 1 << i;
+            tb[i] = 3.0f * tc[i] - tb[i];
+        }
+    }
+    float tmin_init = fmaxf(fmaxf(2.0f * tc[0] - tb[0], 2.0f * tc[1] - tb[1]), 2.0f * tc[2] - tb[2]);
+    float tmax_init = fminf(fminf(tc[0] - tb[0], tc[1] - tb[1]), tc[2] - tb[2]);
+    float h = tmax_init;
+    tmin_init = fmaxf(tmin_init, near_t / scaling);
+    tmax_init = fminf(tmax_init, far_t / scaling);
+    float t_min = tmin_init, t_max = tmax_init;
+    float pos[3] = {1.f, 1.f, 1.f};
+    int idx = 0;
+    for (int i = 0; i < 3; i++) {
+        if (1.5f * tc[i] - tb[i] > t_min) {
+            idx ^= 1 << i;
+            pos[i] = 1.5f;
+        }
+    }
+    const or_octant* stack_parent[OR_STACK];
+    float stack_tmax[OR_STACK];
+    memset(stack_parent, 0, sizeof(stack_parent));
+    memset(stack_tmax, 0, sizeof(stack_tmax));
+    const or_octant* parent = m->root;
+    int scale = OR_STACK - 1;
+    float scale_exp2 = 0.5f;
+    float tcorner[3];
+    float tc_max = 0.f;
+    while (scale < OR_STACK) {
+        for (int i = 0; i < 3; i++)
+            tcorner[i] = pos[i] * tc[i] - tb[i];
+        tc_max = fminf(fminf(tcorner[0], tcorner[1]), tcorner[2]);
+        const or_octant* child = parent->child[idx ^ octant_mask ^ 7];
+        if (scale == min_scale && child != NULL)
+            break; /* found the first allocated block */
+        if (child != NULL && t_min <= t_max) {
+            /* descend */
+            const float tv_max = fminf(t_max, tc_max);
+            const float half = scale_exp2 * 0.5f;
+            float tcen[3];
+            for (int i = 0; i < 3; i++)
+                tcen[i] = half * tc[i] + tcorner[i];
+            if (tc_max < h) {
+                stack_parent[scale] = parent;
+                stack_tmax[scale] = t_max;
+            }
+            h = tc_max;
+            parent = child;
+            idx = 0;
+            scale--;
+            scale_exp2 = half;
+            for (int i = 0; i < 3; i++) {
+                if (tcen[i] > t_min)
+                    idx ^= 1 << i;
+            }
+            for (int i = 0; i < 3; i++)
+                pos[i] = pos[i] + scale_exp2 * (float) ((idx >> i) & 1);
+            t_max = tv_max;
+            continue;
+        }
+        /* advance */
+        const int step_mask = (tcorner[0] <= tc_max) | ((tcorner[1] <= tc_max) << 1) | ((tcorner[2] <= tc_max) << 2);
+        for (int i = 0; i < 3; i++)
+            pos[i] = pos[i] - scale_exp2 * (float) ((step_mask >> i) & 1);
+        t_min = tc_max;
+        idx ^= step_mask;
+        if ((idx & step_mask) != 0) {
+            unsigned int diff = 0;
+            for (int i = 0; i < 3; i++)
+                if (step_mask & (1 << i))
+                    diff |= (unsigned) (f2i(pos[i]) ^ f2i(pos[i] + scale_exp2));
+            scale = (f2i((float) diff) >> 23) - 127;
+            if (scale >= OR_STACK)
+                break; /* left the root: no block */
+            scale_exp2 = i2f((scale - OR_STACK + 127) << 23);
+            parent = stack_parent[scale];
+            t_max = stack_tmax[scale];
+            int sh[3];
+            for (int i = 0; i < 3; i++) {
+                sh[i] = f2i(pos[i]) >> scale;
+                pos[i] = i2f(sh[i] << scale);
+            }
+            idx = (sh[0] & 1) | ((sh[1] & 1) << 1) | ((sh[2] & 1) << 2);
+            h = 0.0f;
+        }
+    }
+    out->t_min_ray = t_min * scaling;
+    out->t_max_ray = tmax_init * scaling;
+}
+
+/* ------------------------------------------------------------------------- */
+/* TSDF raycast: raycaster_impl.hpp:456-627                                  */
+/* ------------------------------------------------------------------------- */
+static int raycast_ray(const or_map* m, const real o[3], const real d[3], real t_near, real t_far, real hit[3])
+{
+    or_ray_span span;
+    const float of[3] = {(float) o[0], (float) o[1], (float) o[2]};
+    const float df[3] = {(float) d[0], (float) d[1], (float) d[2]};
+    esvo_first_block(m, of, df, (float) t_near, (float) t_far, &span);
+    const real step = m->res;
+    const real largestep = (real) 8 * step;
+    const real t_min = span.t_min_ray;
+    if (t_min <= (real) 0)
+        return 0;
+    const real t_max = span.t_max_ray;
+    real t = t_min;
+    const real mu = m->mu;
+    if (!(t_near < t_max))
+        return 0;
+    real stepsize = largestep;
+    real p[3];
+    for (int i = 0; i < 3; i++)
+        p[i] = o[i] + d[i] * t;
+    real f_t = 1, f_tt = 0;
+    {
+        real tsdf, w;
+        or_get_data((void*) m, p, &tsdf, &w);
+        f_t = tsdf;
+    }
+    if (!(f_t >= (real) 0))
+        return 0;
+    for (; t < t_far; t += stepsize) {
+        real tsdf, w;
+        or_get_data((void*) m, p, &tsdf, &w);
+        if (w == (real) 0) {
+            stepsize = largestep;
+            for (int i = 0; i < 3; i++)
+                p[i] = p[i] + stepsize * d[i];
+            continue;
+        }
+        f_tt = tsdf;
+        if (f_tt <= (real) 0.1 && f_tt >= (real) -0.5) {
+            real fi;
+            if (or_get_field_interp((void*) m, p, &fi))
+                f_tt = fi;
+        }
+        if (f_tt < (real) 0)
+            break;
+        stepsize = f_tt * mu > step ? f_tt * mu : step;
+        for (int i = 0; i < 3; i++)
+            p[i] = p[i] + stepsize * d[i];
+        f_t = f_tt;
+    }
+    if (f_tt < (real) 0) {
+        t = t - stepsize * (f_tt - (real) 0) / (f_tt - f_t);
+        for (int i = 0; i < 3; i++)
+            hit[i] = o[i] + d[i] * t;
+        return 1;
+    }
+    return 0;
+}
+
+OR_API void or_raycast(void* vm, const or_pinhole* pin, const real T_WS16[16], real* points, real* normals,
+                       signed char* scale)
+{
+    const or_map* m = (const or_map*) vm;
+    or_cam cam;
+    cam_init(&cam, pin, 1);
+    or_iso T_WS;
+    iso_from16(T_WS16, &T_WS);
+#pragma omp parallel for schedule(dynamic, 1)
+    for (int y = 0; y < cam.h; y++) {
+        for (int x = 0; x < cam.w; x++) {
+            const long idx = x + (long) y * cam.w;
+            real dS[3], dn[3], dW[3];
+            cam_backproject(&cam, (real) x, (real) y, dS);
+            normalized3(dS, dn);
+            lin_apply(&T_WS, dn, dW);
+            real hit[3];
+            if (raycast_ray(m, T_WS.t, dW, cam_near_dist(&cam, dS), cam_far_dist(&cam, dS), hit)) {
+                if (scale)
+                    scale[idx] = 0;
+                memcpy(points + 3 * idx, hit, sizeof(real) * 3);
+                real g[3];
+                if (or_get_field_grad((void*) m, hit, g)) {
+                    normalized3(g, normals + 3 * idx);
+                }
+                else {
+                    normals[3 * idx] = normals[3 * idx + 1] = normals[3 * idx + 2] = 0;
+                }
+            }
+            else {
+                points[3 * idx] = points[3 * idx + 1] = points[3 * idx + 2] = 0;
+                normals[3 * idx] = normals[3 * idx + 1] = normals[3 * idx + 2] = 0;
+            }
+        }
+    }
+}
+
+/* ------------------------------------------------------------------------- */
+/* Preprocessing: src/map/preprocessor.cpp:109-195,                          */
+/* include/se/map/impl/preprocessor_impl.hpp:18-40                           */
+/* ------------------------------------------------------------------------- */
+OR_API void or_half_sample(const float* in, int w, int h, float* out, float e_d)
+{
+    const int ow = w / 2, oh = h / 2;
+#pragma omp parallel for
+    for (int y = 0; y < oh; y++) {
+        for (int x = 0; x < ow; x++) {
+            const int ix = 2 * x, iy = 2 * y;
+            float count = 0.0f, sum = 0.0f;
+            const float c = in[ix + iy * w];
+            for (int i = 0; i <= 1; i++) {
+                for (int j = 0; j <= 1; j++) {
+                    int px = ix + j, py = iy + i;
+                    if (px > 2 * ow - 1)
+                        px = 2 * ow - 1;
+                    if (py > 2 * oh - 1)
+                        py = 2 * oh - 1;
+                    const float v = in[px + py * w];
+                    if (fabsf(v - c) < e_d) {
+                        count += 1.0f;
+                        sum += v;
+                    }
+                }
+            }
+            out[x + y * ow] = sum / count;
+        }
+    }
+}
+
+OR_API void or_depth_to_points(const float* depth, const or_pinhole* pin, int level, real* points)
+{
+    or_cam cam;
+    cam_init_s(&cam, pin, 1 << level, 1);
+#pragma omp parallel for
+    for (int y = 0; y < cam.h; y++) {
+        for (int x = 0; x < cam.w; x++) {
+            const long i = x + (long) y * cam.w;
+            if (depth[i] > 0) {
+                real dir[3];
+                cam_backproject(&cam, (real) x, (real) y, dir);
+                for (int k = 0; k < 3; k++)
+                    points[3 * i + k] = (real) depth[i] * dir[k];
+            }
+            else {
+                points[3 * i] = points[3 * i + 1] = points[3 * i + 2] = 0;
+            }
+        }
+    }
+}
+
+OR_API void or_points_to_normals(const real* pts, int w, int h, int neg_y, real* normals)
+{
+#pragma omp parallel for
+    for (int y = 0; y < h; y++) {
+        for (int x = 0; x < w; x++) {
+            const long i = x + (long) y * w;
+            real* n = normals + 3 * i;
+            if (pts[3 * i + 2] == 0) {
+                n[0] = n[1] = n[2] = 0;
+                continue;
+            }
+            const int xl = x - 1 > 0 ? x - 1 : 0, xr = x + 1 < w - 1 ? x + 1 : w - 1;
+            const int ym = y - 1 > 0 ? y - 1 : 0, yp = y + 1 < h - 1 ? y + 1 : h - 1;
+            const int yu = neg_y ? ym : yp, yd = neg_y ? yp : ym;
+            const real* l = pts + 3 * (xl + (long) y * w);
+            const real* r = pts + 3 * (xr + (long) y * w);
+            const real* u = pts + 3 * (x + (long) yu * w);
+            const real* d = pts + 3 * (x + (long) yd * w);
+            if (l[2] == 0 || r[2] == 0 || u[2] == 0 || d[2] == 0) {
+                n[0] = n[1] = n[2] = 0;
+                continue;
+            }
+            real dx[3], dy[3], c[3];
+            for (int k = 0; k < 3; k++) {
+                dx[k] = r[k] - l[k];
+                dy[k] = u[k] - d[k];
+            }
+            cross3(dx, dy, c);
+            normalized3(c, n);
+        }
+    }
+}
+
+/* ------------------------------------------------------------------------- */
+/* ICP: include/se/tracker/impl/icp_impl.hpp:16-102, src/tracker/icp.cpp:15-246, */
+/* include/se/tracker/impl/tracker_impl.hpp:30-116,                          */
+/* include/se/common/impl/math_util_impl.hpp:117-174                         */
+/* ------------------------------------------------------------------------- */
+typedef struct {
+    int result;
+    real error;
+    real J[6];
+} or_icp_row;
+
+static void icp_track(or_icp_row* rows, int ref_w, const real* in_pts, const real* in_nrm, int w, int h,
+                      const real* ref_pts, const real* ref_nrm, const or_iso* T_WS, const or_iso* T_WS_ref,
+                      const or_cam* cam_full, real dist_th, real normal_th)
+{
+    or_iso T_refW;
+    iso_inv(T_WS_ref, &T_refW);
+#pragma omp parallel for
+    for (int y = 0; y < h; y++) {
+        for (int x = 0; x < w; x++) {
+            or_icp_row* row = rows + x + (long) y * ref_w;
+            const long i = x + (long) y * w;
+            const real* n_in = in_nrm + 3 * i;
+            if (n_in[0] == 0 && n_in[1] == 0 && n_in[2] == 0) {
+                row->result = -1;
+                continue;
+            }
+            real pW[3], pR[3], uv[2];
+            iso_apply(T_WS, in_pts + 3 * i, pW);
+            iso_apply(&T_refW, pW, pR);
+            if (!cam_project(cam_full, pR, uv)) {
+                row->result = -2;
+                continue;
+            }
+            int px[2];
+            round_pixel(uv, px);
+            const long ri = px[0] + (long) px[1] * ref_w;
+            const real rn[3] = {-ref_nrm[3 * ri], -ref_nrm[3 * ri + 1], -ref_nrm[3 * ri + 2]};
+            if (rn[0] == 0 && rn[1] == 0 && rn[2] == 0) {
+                row->result = -3;
+                continue;
+            }
+            real diff[3], nW[3];
+            for (int k = 0; k < 3; k++)
+                diff[k] = ref_pts[3 * ri + k] - pW[k];
+            lin_apply(T_WS, n_in, nW);
+            if (norm3(diff) > dist_th) {
+                row->result = -4;
+                continue;
+            }
+            if (dot3(nW, rn) < normal_th) {
+                row->result = -5;
+                continue;
+            }
+            row->result = 1;
+            row->error = dot3(rn, diff);
+            row->J[0] = rn[0];
+            row->J[1] = rn[1];
+            row->J[2] = rn[2];
+            real cp[3];
+            cross3(pW, rn, cp);
+            row->J[3] = cp[0];
+            row->J[4] = cp[1];
+            row->J[5] = cp[2];
+        }
+    }
+}
+
+/* reduceKernel: 8 row-interleaved partials (each summed in row order here), then summed into
+ * row 0 (icp.cpp:45-204).  The reference's OpenMP reduction order is unspecified. */
+static void icp_reduce(real out[32], const or_icp_row* rows, int ref_w, int w, int h)
+{
+    real part[8][32];
+    memset(part, 0, sizeof(part));
+    for (int b = 0; b < 8; b++) {
+        real* s = part[b];
+        for (int y = b; y < h; y += 8) {
+            for (int x = 0; x < w; x++) {
+                const or_icp_row* r = rows + x + (long) y * ref_w;
+                if (r->result < 1) {
+                    s[29] += r->result == -4 ? 1 : 0;
+                    s[30] += r->result == -5 ? 1 : 0;
+                    s[31] += r->result > -4 ? 1 : 0;
+                    continue;
+                }
+                s[0] += r->error * r->error;
+                for (int k = 0; k < 6; k++)
+                    s[1 + k] += r->error * r->J[k];
+                int c = 7;
+                for (int a = 0; a < 6; a++)
+                    for (int bb = a; bb < 6; bb++)
+                        s[c++] += r->J[a] * r->J[bb];
+                s[28] += 1;
+            }
+        }
+    }
+    for (int k = 0; k < 32; k++) {
+        real v = part[0][k];
+        for (int b = 1; b < 8; b++)
+            v += part[b][k];
+        out[k] = v;
+    }
+}
+
+/* solve (icp.cpp:33-41): 6x6 Cholesky of JTJ, x = 0 if it fails */
+static void icp_solve(const real vals[27], real x[6])
+{
+    real A[6][6];
+    int c = 6;
+    for (int i = 0; i < 6; i++)
+        for (int j = i; j < 6; j++) {
+            A[i][j] = vals[c++];
+            A[j][i] = A[i][j];
+        }
+    real L[6][6];
+    memset(L, 0, sizeof(L));
+    for (int k = 0; k < 6; k++) {
+        real s = A[k][k];
+        for (int j = 0; j < k; j++)
+            s -= L[k][j] * L[k][j];
+        if (!(s > 0)) {
+            for (int i = 0; i < 6; i++)
+                x[i] = 0;
+            return;
+        }
+        L[k][k] = sqrt(s);
+        for (int i = k + 1; i < 6; i++) {
+            real v = A[i][k];
+            for (int j = 0; j < k; j++)
+                v -= L[i][j] * L[k][j];
+            L[i][k] = v / L[k][k];
+        }
+    }
+    real yv[6];
+    for (int i = 0; i < 6; i++) {
+        real v = vals[i];
+        for (int j = 0; j < i; j++)
+            v -= L[i][j] * yv[j];
+        yv[i] = v / L[i][i];
+    }
+    for (int i = 5; i >= 0; i--) {
+        real v = yv[i];
+        for (int j = i + 1; j < 6; j++)
+            v -= L[j][i] * x[j];
+        x[i] = v / L[i][i];
+    }
+}
+
+/* math::exp (math_util_impl.hpp:117-174): twist a = [v; w] */
+OR_API void or_se3_exp(const real a[6], real T16[16])
+{
+    const real w[3] = {a[3], a[4], a[5]};
+    const real theta_sq = dot3(w, w);
+    const real theta = sqrt(theta_sq);
+    const real half_theta = (real) 0.5 * theta;
+    real imag, re;
+    if (theta < (real) 1e-10) {
+        const real t4 = theta_sq * theta_sq;
+        imag = (real) 0.5 - (real) (1.0 / 48.0) * theta_sq + (real) (1.0 / 3840.0) * t4;
+        re = (real) 1 - (real) 0.5 * theta_sq + (real) (1.0 / 384.0) * t4;
+    }
+    else {
+        imag = sin(half_theta) / theta;
+        re = cos(half_theta);
+    }
+    real q[4] = {re, imag * w[0], imag * w[1], imag * w[2]}; /* w x y z */
+    const real qn = sqrt(q[0] * q[0] + (q[1] * q[1] + (q[2] * q[2] + q[3] * q[3])));
+    for (int i = 0; i < 4; i++)
+        q[i] = q[i] / qn;
+    const real qw = q[0], qx = q[1], qy = q[2], qz = q[3];
+    /* Quaternion::toRotationMatrix term order */
+    const real tx = (real) 2 * qx, ty = (real) 2 * qy, tz = (real) 2 * qz;
+    const real twx = tx * qw, twy = ty * qw, twz = tz * qw;
+    const real txx = tx * qx, txy = ty * qx, txz = tz * qx;
+    const real tyy = ty * qy, tyz = tz * qy, tzz = tz * qz;
+    real R[3][3];
+    R[0][0] = (real) 1 - (tyy + tzz);
+    R[0][1] = txy - twz;
+    R[0][2] = txz + twy;
+    R[1][0] = txy + twz;
+    R[1][1] = (real) 1 - (txx + tzz);
+    R[1][2] = tyz - twx;
+    R[2][0] = txz - twy;
+    R[2][1] = tyz + twx;
+    R[2][2] = (real) 1 - (txx + tyy);
+    real V[3][3];
+    if (theta < (real) 1e-10) {
+        memcpy(V, R, sizeof(V));
+    }
+    else {
+        const real W[3][3] = {{0, -w[2], w[1]}, {w[2], 0, -w[0]}, {-w[1], w[0], 0}};
+        real W2[3][3];
+        for (int i = 0; i < 3; i++)
+            for (int j = 0; j < 3; j++)
+                W2[i][j] = W[i][0] * W[0][j] + (W[i][1] * W[1][j] + W[i][2] * W[2][j]);
+        const real c1 = ((real) 1 - cos(theta)) / theta_sq;
+        const real c2 = (theta - sin(theta)) / (theta_sq * theta);
+        for (int i = 0; i < 3; i++)
+            for (int j = 0; j < 3; j++)
+                V[i][j] = ((i == j ? (real) 1 : (real) 0) + c1 * W[i][j]) + c2 * W2[i][j];
+    }
+    for (int i = 0; i < 3; i++) {
+        for (int j = 0; j < 3; j++)
+            T16[4 * i + j] = R[i][j];
+        T16[4 * i + 3] = V[i][0] * a[0] + (V[i][1] * a[1] + V[i][2] * a[2]);
+    }
+    T16[12] = T16[13] = T16[14] = 0;
+    T16[15] = 1;
+}
+
+typedef struct {
+    int iterations[3];
+    float dist_threshold, normal_threshold, track_threshold, icp_threshold;
+} or_tracker_config;
+
+/* Tracker::track(depth, T_WS, ref points, ref normals) (tracker_impl.hpp:30-116).
+ * T_WS16 is in/out.  sums_out receives the last reduction (32 values).  Returns 1 on success. */
+OR_API int or_track(const float* depth, const or_pinhole* pin, const or_tracker_config* cfg, real T_WS16[16],
+                    const real* ref_pts, const real* ref_nrm, real sums_out[32], int* iters_out)
+{
+    or_cam cam;
+    cam_init(&cam, pin, 1);
+    const int W = cam.w, H = cam.h;
+    const int nl = 3;
+    float* dimg[3];
+    real* pts[3];
+    real* nrm[3];
+    int lw[3], lh[3];
+    for (int l = 0; l < nl; l++) {
+        lw[l] = W >> l;
+        lh[l] = H >> l;
+        dimg[l] = (float*) calloc((size_t) lw[l] * lh[l], sizeof(float));
+        pts[l] = (real*) calloc((size_t) lw[l] * lh[l] * 3, sizeof(real));
+        nrm[l] = (real*) calloc((size_t) lw[l] * lh[l] * 3, sizeof(real));
+    }
+    memcpy(dimg[0], depth, sizeof(float) * W * H);
+    const float e_d = 0.1f * 3;
+    for (int l = 1; l < nl; l++)
+        or_half_sample(dimg[l - 1], lw[l - 1], lh[l - 1], dimg[l], e_d);
+    for (int l = 0; l < nl; l++) {
+        or_depth_to_points(dimg[l], pin, l, pts[l]);
+        or_points_to_normals(pts[l], lw[l], lh[l], cam.left_hand_frame, nrm[l]);
+    }
+    or_iso T, T_ref;
+    iso_from16(T_WS16, &T);
+    T_ref = T;
+    or_icp_row* rows = (or_icp_row*) calloc((size_t) W * H, sizeof(or_icp_row));
+    real sums[32];
+    memset(sums, 0, sizeof(sums));
+    int total_iters = 0;
+    for (int level = nl - 1; level >= 0; level--) {
+        for (int it = 0; it < cfg->iterations[level]; it++) {
+            icp_track(rows, W, pts[level], nrm[level], lw[level], lh[level], ref_pts, ref_nrm, &T, &T_ref, &cam,
+                      cfg->dist_threshold, cfg->normal_threshold);
+            icp_reduce(sums, rows, W, W / (1 << level), H / (1 << level));
+            total_iters++;
+            real x[6];
+            icp_solve(sums + 1, x);
+            real D16[16];
+            or_se3_exp(x, D16);
+            or_iso D, Tn;
+            iso_from16(D16, &D);
+            iso_mul(&D, &T, &Tn);
+            T = Tn;
+            real xn = sqrt(x[0] * x[0] + (x[1] * x[1] + (x[2] * x[2] + (x[3] * x[3] + (x[4] * x[4] + x[5] * x[5])))));
+            if (xn < cfg->icp_threshold)
+                break;
+        }
+    }
+    int ok = 1;
+    if (((double) sqrt(sums[0] / sums[28]) > 2e-2) || (sums[28] / (real) (W * H) < cfg->track_threshold)) {
+        T = T_ref;
+        ok = 0;
+    }
+    iso_to16(&T, T_WS16);
+    if (sums_out)
+        memcpy(sums_out, sums, sizeof(sums));
+    if (iters_out)
+        *iters_out = total_iters;
+    for (int l = 0; l < nl; l++) {
+        free(dimg[l]);
+        free(pts[l]);
+        free(nrm[l]);
+    }
+    free(rows);
+    return ok;
+}
+
+/* One trackKernel + reduceKernel pass at one level, for unit checks of the GPU kernels. */
+OR_API void or_icp_track_reduce(const real* in_pts, const real* in_nrm, int w, int h, const real* ref_pts,
+                                const real* ref_nrm, const or_pinhole* pin, const real T_WS16[16],
+                                const real T_ref16[16], float dist_th, float normal_th, int* results,
+                                real* errors, real* J, real sums[32])
+{
+    or_cam cam;
+    cam_init(&cam, pin, 1);
+    or_iso T, Tr;
+    iso_from16(T_WS16, &T);
+    iso_from16(T_ref16, &Tr);
+    or_icp_row* rows = (or_icp_row*) calloc((size_t) cam.w * cam.h, sizeof(or_icp_row));
+    icp_track(rows, cam.w, in_pts, in_nrm, w, h, ref_pts, ref_nrm, &T, &Tr, &cam, dist_th, normal_th);
+    icp_reduce(sums, rows, cam.w, w, h);
+    for (int y = 0; y < h; y++)
+        for (int x = 0; x < w; x++) {
+            const or_icp_row* r = rows + x + (long) y * cam.w;
+            const long i = x + (long) y * w;
+            if (results)
+                results[i] = r->result;
+            if (errors)
+                errors[i] = r->error;
+            if (J)
+                memcpy(J + 6 * i, r->J, sizeof(real) * 6);
+        }
+    free(rows);
+}
+
+OR_API void or_icp_solve(const real vals[27], real x[6])
+{
+    icp_solve(vals, x);
+}
+
+OR_API int or_real_size(void)
+{
+    return (int) sizeof(real);
+}
+
+OR_API void or_set_threads(int n)
+{
+#ifdef _OPENMP
+    omp_set_num_threads(n);
+#else
+    (void) n;
+#endif
+}
+
+/* Entry t of the first allocated block along a ray and t_max (tcmin/tmax of the reference's
+ * VoxelBlockRayIterator), for checks against a brute-force ray/box search. */
+OR_API void or_first_block(void* vm, const float o_W[3], const float d_W[3], float near_t, float far_t, float out[2])
+{
+    or_ray_span s;
+    esvo_first_block((const or_map*) vm, o_W, d_W, near_t, far_t, &s);
+    out[0] = s.t_min_ray;
+    out[1] = s.t_max_ray;
+}

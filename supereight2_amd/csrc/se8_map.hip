@@ -1,0 +1,1192 @@
+// Map object, allocation pipeline and TSDF update of libse8.so.
+//
+// integrateDepth (include/se/integrator/impl/map_integrator_impl.hpp:52-83) becomes, per frame:
+//   k_frustum      FrustumIterator over the allocated blocks (raycast_carver_impl.hpp:17-32,
+//                  iterator.hpp:197-238) as a flat per-block test of the block and its ancestors
+//   k_alloc_rays   the RaycastCarver band march (raycast_carver_impl.hpp:74-111): Morton keys of
+//                  unallocated blocks appended to a candidate list
+//   radix sort     LSD sort of the candidate keys (8-bit digits over the significant key bits)
+//   k_build_level  one launch per octree scale: segment heads of the sorted keys create missing
+//                  nodes (allocator_impl.hpp:68-147, octree_impl.hpp:171-199); duplicate keys
+//                  fall in the same segment, so the unique step is the segment-head flag
+//   k_build_blocks block slots, brick index, AABB and the "new" entries of the update list
+//   k_update       Updater<Map<TSDF,Single>> (singleres_tsdf_updater_impl.hpp:14-132), one 8^3
+//                  block per 512-thread workgroup
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "se8_internal.hpp"
+
+namespace se8 {
+
+static thread_local std::string g_last_error;
+
+void set_error(const std::string& msg)
+{
+    g_last_error = msg;
+}
+
+int dev_alloc(DeviceBuffer& b, size_t bytes)
+{
+    if (b.ptr && b.bytes >= bytes)
+        return SE8_OK;
+    dev_free(b);
+    if (bytes == 0)
+        bytes = 16;
+    if (hipMalloc(&b.ptr, bytes) != hipSuccess) {
+        b.ptr = nullptr;
+        b.bytes = 0;
+        (void) hipGetLastError();
+        set_error("hipMalloc of " + std::to_string(bytes) + " bytes failed");
+        return SE8_ERR_OOM;
+    }
+    b.bytes = bytes;
+    return SE8_OK;
+}
+
+void dev_free(DeviceBuffer& b)
+{
+    if (b.ptr)
+        (void) hipFree(b.ptr);
+    b.ptr = nullptr;
+    b.bytes = 0;
+}
+
+static void plane_normal(const float p1[3], const float p2[3], const float p3[3], float out[3])
+{
+    float t1[3], t2[3], n[3];
+    for (int i = 0; i < 3; i++) {
+        t1[i] = p2[i] - p1[i];
+        t2[i] = p3[i] - p2[i];
+    }
+    cross3(t1, t2, n);
+    normalized3(n, out);
+}
+
+Cam make_cam(const se8_pinhole& p, int dsf, bool scaled)
+{
+    Cam c;
+    const float s = (float) dsf;
+    c.w = (int) ((float) p.width / s);
+    c.h = (int) ((float) p.height / s);
+    if (!scaled) {
+        c.fx = p.fx;
+        c.fy = p.fy;
+        c.cx = p.cx;
+        c.cy = p.cy;
+    }
+    else {
+        c.fx = p.fx / s;
+        c.fy = p.fy / s;
+        c.cx = (p.cx + 0.5f) / s - 0.5f;
+        c.cy = (p.cy + 0.5f) / s - 0.5f;
+    }
+    c.inv_fx = 1.f / c.fx;
+    c.inv_fy = 1.f / c.fy;
+    c.near_plane = p.near_plane;
+    c.far_plane = p.far_plane;
+    c.left_hand_frame = (p.fx < 0.0f) ^ (p.fy < 0.0f);
+    float tl[3], tr[3], br[3], bl[3];
+    cam_backproject(c, 0.f, 0.f, tl);
+    cam_backproject(c, (float) c.w, 0.f, tr);
+    cam_backproject(c, (float) c.w, (float) c.h, br);
+    cam_backproject(c, 0.f, (float) c.h, bl);
+    float v[8][3]; // near TL TR BR BL, far TL TR BR BL
+    for (int i = 0; i < 3; i++) {
+        v[0][i] = c.near_plane * tl[i];
+        v[1][i] = c.near_plane * tr[i];
+        v[2][i] = c.near_plane * br[i];
+        v[3][i] = c.near_plane * bl[i];
+        v[4][i] = c.far_plane * tl[i];
+        v[5][i] = c.far_plane * tr[i];
+        v[6][i] = c.far_plane * br[i];
+        v[7][i] = c.far_plane * bl[i];
+    }
+    plane_normal(v[4], v[0], v[3], c.planes[0]); // left
+    plane_normal(v[1], v[5], v[6], c.planes[1]); // right
+    plane_normal(v[7], v[3], v[2], c.planes[2]); // bottom
+    plane_normal(v[5], v[1], v[0], c.planes[3]); // top
+    for (int i = 0; i < 4; i++)
+        c.planes[i][3] = 0.f;
+    const float nearp[4] = {0.f, 0.f, 1.f, -c.near_plane};
+    const float farp[4] = {0.f, 0.f, -1.f, c.far_plane};
+    for (int j = 0; j < 4; j++) {
+        c.planes[4][j] = nearp[j];
+        c.planes[5][j] = farp[j];
+    }
+    float sign = c.fx * c.fy;
+    sign = sign / fabsf(sign);
+    for (int i = 0; i < 4; i++)
+        for (int j = 0; j < 3; j++)
+            c.planes[i][j] *= sign;
+    return c;
+}
+
+// ----------------------------------------------------------------------------------------------
+// small utility kernels
+// ----------------------------------------------------------------------------------------------
+__global__ void k_fill_i32(int* p, int64_t n, int v)
+{
+    for (int64_t i = blockIdx.x * (int64_t) blockDim.x + threadIdx.x; i < n; i += (int64_t) gridDim.x * blockDim.x)
+        p[i] = v;
+}
+
+__global__ void k_frame_reset(int* counters)
+{
+    if (threadIdx.x == 0) {
+        counters[kCntFrustum] = 0;
+        counters[kCntUpdate] = 0;
+        counters[kCntCandidates] = 0;
+        counters[kCntUnique] = 0;
+    }
+}
+
+static int fill_i32(DeviceBuffer& b, int64_t n, int v, hipStream_t s)
+{
+    if (n <= 0)
+        return SE8_OK;
+    const int64_t blocks = std::min<int64_t>((n + 255) / 256, 65536);
+    k_fill_i32<<<(unsigned) blocks, 256, 0, s>>>((int*) b.ptr, n, v);
+    SE8_HIP_CHECK(hipGetLastError());
+    return SE8_OK;
+}
+
+// ----------------------------------------------------------------------------------------------
+// wave-aggregated append: every lane with pred appends one value, one atomic per wave
+// ----------------------------------------------------------------------------------------------
+__device__ __forceinline__ int wave_append_slot(bool pred, int* counter)
+{
+    const uint64_t mask = __ballot(pred);
+    if (mask == 0)
+        return -1;
+    const int lane = __lane_id();
+    const int leader = __ffsll((unsigned long long) mask) - 1;
+    int base = 0;
+    if (lane == leader)
+        base = atomicAdd(counter, __popcll(mask));
+    base = __shfl(base, leader);
+    if (!pred)
+        return -1;
+    return base + __popcll(mask & ((1ull << lane) - 1ull));
+}
+
+// ----------------------------------------------------------------------------------------------
+// fetch-frustum: a block is listed iff its own sphere and those of all its ancestors below the
+// root pass sphereInFrustum (the DFS of iterator_impl.hpp:86-156 never prunes at the root).
+// ----------------------------------------------------------------------------------------------
+__global__ void k_frustum(MapView m, Iso T_SW, Cam cam, int* update_list)
+{
+    const int n = m.counters[kCntBlocks];
+    const float r_unit = sqrtf(3.0f) / 2.0f * m.res;
+    for (int base = blockIdx.x * blockDim.x; base < n; base += gridDim.x * blockDim.x) {
+        const int slot = base + threadIdx.x;
+        bool in = false;
+        if (slot < n) {
+            int c[3];
+            key_to_coord(m.block_key[slot], c);
+            in = true;
+            for (int s = kBlockScale; s < m.max_scale && in; s++) {
+                const int size = 1 << s;
+                float cW[3];
+                for (int i = 0; i < 3; i++) {
+                    const int oc = c[i] & ~(size - 1);
+                    const float pM = ((float) oc + (float) size * 0.5f) * m.res;
+                    cW[i] = pM + (-m.t_MW[i]);
+                }
+                float cS[3];
+                iso_apply(T_SW, cW, cS);
+                in = cam_sphere_in_frustum(cam, cS, r_unit * (float) size);
+            }
+        }
+        const int pos = wave_append_slot(in, &m.counters[kCntUpdate]);
+        if (in)
+            update_list[pos] = slot;
+    }
+    // the frustum part of the list is complete when this kernel ends; record its size
+}
+
+__global__ void k_record_frustum(int* counters)
+{
+    if (threadIdx.x == 0)
+        counters[kCntFrustum] = counters[kCntUpdate];
+}
+
+// ----------------------------------------------------------------------------------------------
+// create-list: band march of RaycastCarver (raycast_carver_impl.hpp:74-111)
+// ----------------------------------------------------------------------------------------------
+__global__ void k_alloc_rays(MapView m, Cam cam, Iso T_WS, const float* depth, float band, int num_steps,
+                             uint64_t* cand, int64_t cand_cap)
+{
+    const int x = blockIdx.x * blockDim.x + threadIdx.x;
+    const int y = blockIdx.y;
+    bool live = x < cam.w;
+    float pos[3] = {0.f, 0.f, 0.f}, step[3] = {0.f, 0.f, 0.f};
+    if (live) {
+        const float d = depth[x + (size_t) y * cam.w];
+        if (d < cam.near_plane || d > (cam.far_plane + band * 0.5f))
+            live = false;
+        else {
+            float dir[3], pC[3], pW[3];
+            cam_backproject(cam, (float) x, (float) y, dir);
+            for (int i = 0; i < 3; i++)
+                pC[i] = d * dir[i];
+            iso_apply(T_WS, pC, pW);
+            float diff[3], rev[3];
+            for (int i = 0; i < 3; i++)
+                diff[i] = T_WS.t[i] - pW[i];
+            normalized3(diff, rev);
+            for (int i = 0; i < 3; i++) {
+                pos[i] = pW[i] - (band * 0.5f) * rev[i];
+                step[i] = (rev[i] * band) / (float) num_steps;
+            }
+        }
+    }
+    uint64_t prev = ~0ull;
+    for (int s = 0; s < num_steps; s++) {
+        bool emit = false;
+        uint64_t key = 0;
+        if (live) {
+            float pM[3];
+            for (int i = 0; i < 3; i++)
+                pM[i] = pos[i] + m.t_MW[i];
+            if (pM[0] >= 0.f && pM[0] < m.dim && pM[1] >= 0.f && pM[1] < m.dim && pM[2] >= 0.f && pM[2] < m.dim) {
+                const int wrap = m.size - 1; // see lookup_block
+                const int vx = ((int) (pM[0] / m.res)) & wrap, vy = ((int) (pM[1] / m.res)) & wrap,
+                          vz = ((int) (pM[2] / m.res)) & wrap;
+                if (lookup_block(m, vx, vy, vz) < 0) {
+                    key = block_key_of_voxel(vx, vy, vz);
+                    emit = key != prev;
+                    prev = key;
+                }
+            }
+            for (int i = 0; i < 3; i++)
+                pos[i] = pos[i] + step[i];
+        }
+        const int slot = wave_append_slot(emit, &m.counters[kCntCandidates]);
+        if (emit && slot < cand_cap)
+            cand[slot] = key;
+    }
+}
+
+// ----------------------------------------------------------------------------------------------
+// LSD radix sort of 64-bit keys, 8-bit digits, stable (tile = 256 threads x 8 keys, striped)
+// ----------------------------------------------------------------------------------------------
+constexpr int kRsThreads = 256;
+constexpr int kRsItems = 8;
+constexpr int kRsTile = kRsThreads * kRsItems;
+
+__global__ __launch_bounds__(kRsThreads) void k_rs_hist(const uint64_t* keys, int n, int shift, uint32_t* tile_hist,
+                                                        int n_tiles)
+{
+    __shared__ uint32_t h[256];
+    h[threadIdx.x] = 0;
+    __syncthreads();
+    const int base = blockIdx.x * kRsTile;
+    for (int i = 0; i < kRsItems; i++) {
+        const int idx = base + i * kRsThreads + threadIdx.x;
+        if (idx < n)
+            atomicAdd(&h[(keys[idx] >> shift) & 255u], 1u);
+    }
+    __syncthreads();
+    tile_hist[(size_t) threadIdx.x * n_tiles + blockIdx.x] = h[threadIdx.x];
+}
+
+// In-place exclusive scan of n uint32 with one 1024-thread workgroup.
+__global__ __launch_bounds__(1024) void k_scan_single(uint32_t* data, int n)
+{
+    __shared__ uint32_t sums[1024];
+    const int t = threadIdx.x;
+    const int chunk = (n + 1023) / 1024;
+    const int start = t * chunk;
+    const int end = min(start + chunk, n);
+    uint32_t s = 0;
+    for (int i = start; i < end; i++)
+        s += data[i];
+    sums[t] = s;
+    __syncthreads();
+    for (int off = 1; off < 1024; off <<= 1) {
+        const uint32_t v = t >= off ? sums[t - off] : 0u;
+        __syncthreads();
+        sums[t] += v;
+        __syncthreads();
+    }
+    uint32_t run = sums[t] - s;
+    for (int i = start; i < end; i++) {
+        const uint32_t v = data[i];
+        data[i] = run;
+        run += v;
+    }
+}
+
+__global__ __launch_bounds__(kRsThreads) void k_rs_scatter(const uint64_t* in, uint64_t* out, int n, int shift,
+                                                           const uint32_t* tile_off, int n_tiles)
+{
+    __shared__ uint32_t base[256];
+    __shared__ uint32_t wave_cnt[kRsThreads / 64][256];
+    const int t = threadIdx.x;
+    const int lane = t & 63, wave = t >> 6;
+    base[t] = tile_off[(size_t) t * n_tiles + blockIdx.x];
+    for (int r = 0; r < kRsItems; r++) {
+        const int idx = blockIdx.x * kRsTile + r * kRsThreads + t;
+        const bool valid = idx < n;
+        const uint64_t key = valid ? in[idx] : 0ull;
+        const uint32_t d = (uint32_t) (key >> shift) & 255u;
+        uint64_t peers = __ballot(valid);
+#pragma unroll
+        for (int b = 0; b < 8; b++) {
+            const bool bit = (d >> b) & 1u;
+            const uint64_t mb = __ballot(bit);
+            peers &= bit ? mb : ~mb;
+        }
+        for (int w = 0; w < kRsThreads / 64; w++)
+            wave_cnt[w][t] = 0;
+        __syncthreads();
+        const int rank = __popcll(peers & ((1ull << lane) - 1ull));
+        if (valid && rank == 0)
+            wave_cnt[wave][d] = (uint32_t) __popcll(peers);
+        __syncthreads();
+        if (valid) {
+            uint32_t pos = base[d] + (uint32_t) rank;
+            for (int w = 0; w < wave; w++)
+                pos += wave_cnt[w][d];
+            out[pos] = key;
+        }
+        __syncthreads();
+        uint32_t add = 0;
+        for (int w = 0; w < kRsThreads / 64; w++)
+            add += wave_cnt[w][t];
+        base[t] += add;
+        __syncthreads();
+    }
+}
+
+int RadixSorter::reserve(size_t n)
+{
+    if (n <= cap)
+        return SE8_OK;
+    const size_t tiles = (n + kRsTile - 1) / kRsTile;
+    int rc = dev_alloc(keys_alt, n * sizeof(uint64_t));
+    if (rc == SE8_OK)
+        rc = dev_alloc(tile_hist, tiles * 256 * sizeof(uint32_t));
+    if (rc != SE8_OK)
+        return rc;
+    cap = n;
+    return SE8_OK;
+}
+
+void RadixSorter::release()
+{
+    dev_free(keys_alt);
+    dev_free(tile_hist);
+    dev_free(tile_flags);
+    cap = 0;
+}
+
+// Sorts keys[0..n) on bit range [lo_bit, hi_bit); returns the buffer holding the result.
+static uint64_t* radix_sort(RadixSorter& rs, uint64_t* keys, int n, int lo_bit, int hi_bit, hipStream_t s)
+{
+    const int n_tiles = (n + kRsTile - 1) / kRsTile;
+    uint64_t* a = keys;
+    uint64_t* b = (uint64_t*) rs.keys_alt.ptr;
+    for (int shift = lo_bit; shift < hi_bit; shift += 8) {
+        k_rs_hist<<<n_tiles, kRsThreads, 0, s>>>(a, n, shift, (uint32_t*) rs.tile_hist.ptr, n_tiles);
+        k_scan_single<<<1, 1024, 0, s>>>((uint32_t*) rs.tile_hist.ptr, n_tiles * 256);
+        k_rs_scatter<<<n_tiles, kRsThreads, 0, s>>>(a, b, n, shift, (const uint32_t*) rs.tile_hist.ptr, n_tiles);
+        std::swap(a, b);
+    }
+    return a;
+}
+
+// ----------------------------------------------------------------------------------------------
+// allocate-list: level-by-level octree build from the sorted candidate keys
+// ----------------------------------------------------------------------------------------------
+__global__ void k_build_level(MapView m, const uint64_t* keys, const int* n_ptr, int n_host, int* cur_node, int scale,
+                              int brick_scale)
+{
+    const int n = n_ptr ? *n_ptr : n_host;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n)
+        return;
+    const uint64_t key = keys[i];
+    const uint64_t code = key >> kScaleOffset;
+    int cur;
+    if (scale == m.max_scale - 1)
+        cur = 0;
+    else {
+        const int prev = cur_node[i];
+        cur = prev < 0 ? -1 : m.node_child[prev * 8 + (int) ((code >> (3 * (scale + 1))) & 7)];
+    }
+    cur_node[i] = cur;
+    if (cur < 0)
+        return; // only after a node-pool overflow
+    const bool head = (i == 0) || (((key ^ keys[i - 1]) >> (kScaleOffset + 3 * scale)) != 0);
+    if (!head)
+        return;
+    const int idx = (int) ((code >> (3 * scale)) & 7);
+    if (m.node_child[cur * 8 + idx] >= 0)
+        return;
+    const int nn = atomicAdd(&m.counters[kCntNodes], 1);
+    if (nn >= m.max_nodes) {
+        atomicExch(&m.counters[kCntOverflow], 1);
+        return;
+    }
+    const uint64_t mask = (0x1ffffffffffffffull >> (3 * scale)) << (3 * scale);
+    m.node_key[nn] = ((code & mask) << kScaleOffset) | (uint64_t) scale;
+    m.node_ts[nn] = -1;
+    if (scale == brick_scale) {
+        const int br = atomicAdd(&m.counters[kCntBricks], 1);
+        if (br >= m.max_bricks) {
+            atomicExch(&m.counters[kCntOverflow], 1);
+            return;
+        }
+        int c[3];
+        key_to_coord(key, c);
+        const int bs = brick_scale;
+        m.brick_top[((c[2] >> bs) * m.bricks_side + (c[1] >> bs)) * m.bricks_side + (c[0] >> bs)] = br;
+    }
+    m.node_child[cur * 8 + idx] = nn;
+}
+
+__global__ void k_build_blocks(MapView m, const uint64_t* keys, const int* n_ptr, int n_host, const int* cur_node,
+                               int* update_list, int mark_new)
+{
+    const int n = n_ptr ? *n_ptr : n_host;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n)
+        return;
+    const uint64_t key = keys[i];
+    if (i > 0 && keys[i - 1] == key)
+        return;
+    const uint64_t code = key >> kScaleOffset;
+    int parent = 0;
+    if (m.max_scale > 4) {
+        const int prev = cur_node[i];
+        parent = prev < 0 ? -1 : m.node_child[prev * 8 + (int) ((code >> 12) & 7)];
+    }
+    if (parent < 0)
+        return; // only after a node-pool overflow
+    const int idx = (int) ((code >> 9) & 7);
+    if (m.node_child[parent * 8 + idx] >= 0)
+        return; // already allocated (explicit allocation requests may repeat blocks)
+    const int slot = atomicAdd(&m.counters[kCntBlocks], 1);
+    if (slot >= m.max_blocks) {
+        atomicExch(&m.counters[kCntOverflow], 1);
+        return;
+    }
+    int c[3];
+    key_to_coord(key, c);
+    m.block_key[slot] = key;
+    m.block_parent[slot] = parent;
+    m.block_ts[slot] = -1;
+    m.node_child[parent * 8 + idx] = slot;
+    const int bs = m.brick_shift + kBlockScale;
+    const int brick = m.brick_top[((c[2] >> bs) * m.bricks_side + (c[1] >> bs)) * m.bricks_side + (c[0] >> bs)];
+    const int msk = (1 << m.brick_shift) - 1;
+    const int bx = c[0] >> 3, by = c[1] >> 3, bz = c[2] >> 3;
+    if (brick >= 0)
+        m.brick_slots[brick * 512 + (bx & msk) + ((by & msk) << 3) + ((bz & msk) << 6)] = slot;
+    atomicMin(&m.counters[kCntAabb + 0], c[0]);
+    atomicMin(&m.counters[kCntAabb + 1], c[1]);
+    atomicMin(&m.counters[kCntAabb + 2], c[2]);
+    atomicMax(&m.counters[kCntAabb + 3], c[0] + 7);
+    atomicMax(&m.counters[kCntAabb + 4], c[1] + 7);
+    atomicMax(&m.counters[kCntAabb + 5], c[2] + 7);
+    if (update_list) {
+        const int pos = atomicAdd(&m.counters[kCntUpdate], 1);
+        update_list[pos] = mark_new ? (int) ((uint32_t) slot | 0x80000000u) : slot;
+    }
+    else {
+        // explicit allocation: initialise the voxels here (integration marks new blocks instead)
+        for (int v = 0; v < 512; v++) {
+            if (m.storage == SE8_STORAGE_F32)
+                reinterpret_cast<float2*>(m.voxels)[(size_t) slot * 512 + v] = make_float2(1.f, 0.f);
+            else
+                reinterpret_cast<uint32_t*>(m.voxels)[(size_t) slot * 512 + v] = pack_f16(1.f, 0.f);
+        }
+    }
+}
+
+// ----------------------------------------------------------------------------------------------
+// update: one block (512 voxels) per workgroup of 8 wave64s
+// ----------------------------------------------------------------------------------------------
+template<int Storage>
+__global__ __launch_bounds__(512) void k_update(MapView m, const int* update_list, Iso T_CV, Cam cam,
+                                                const float* depth, int ts)
+{
+    const int n = m.counters[kCntUpdate];
+    const int t = threadIdx.x;
+    const int vx = t & 7, vy = (t >> 3) & 7, vz = t >> 6;
+    for (int e = blockIdx.x; e < n; e += gridDim.x) {
+        const int entry = update_list[e];
+        const bool fresh = entry < 0;
+        const int slot = entry & 0x7fffffff;
+        int c[3];
+        key_to_coord(m.block_key[slot], c);
+        if (t == 0) {
+            m.block_ts[slot] = ts;
+            atomicMax(&m.node_ts[m.block_parent[slot]], ts);
+        }
+        const float v[3] = {(float) (c[0] + vx), (float) (c[1] + vy), (float) (c[2] + vz)};
+        float pC[3];
+        iso_apply(T_CV, v, pC);
+        float tsdf = 1.f, weight = 0.f;
+        const size_t vi = (size_t) slot * 512 + t;
+        if (!fresh) {
+            if (Storage == SE8_STORAGE_F32) {
+                const float2 d = reinterpret_cast<const float2*>(m.voxels)[vi];
+                tsdf = d.x;
+                weight = d.y;
+            }
+            else {
+                unpack_f16(reinterpret_cast<const uint32_t*>(m.voxels)[vi], tsdf, weight);
+            }
+        }
+        bool changed = false;
+        const float nrm = sqrtf(sqnorm3(pC));
+        if (!(nrm > cam_far_dist(cam, pC))) {
+            float uv[2];
+            if (cam_project(cam, pC, uv)) {
+                const int px = (int) (uv[0] + 0.5f), py = (int) (uv[1] + 0.5f);
+                const float d = depth[px + (size_t) py * cam.w];
+                if (!(d < cam.near_plane)) {
+                    const float mz = pC[2];
+                    const float sdf = (d - mz) / mz * sqrtf(sqnorm3(pC));
+                    if (!(sdf < -m.mu)) {
+                        const float u = fminf(sdf / m.mu, 1.f);
+                        if (weight < m.max_weight)
+                            weight = weight + 1.f;
+                        tsdf = (tsdf * (weight - 1.f) + u) / weight;
+                        changed = true;
+                    }
+                }
+            }
+        }
+        if (changed || fresh) {
+            if (Storage == SE8_STORAGE_F32)
+                reinterpret_cast<float2*>(m.voxels)[vi] = make_float2(tsdf, weight);
+            else
+                reinterpret_cast<uint32_t*>(m.voxels)[vi] = pack_f16(tsdf, weight);
+        }
+    }
+}
+
+// T_CV = T_CW * T_WM * Scaling(res) * Translation(0.5) (singleres_tsdf_updater_impl.hpp:37-42)
+static Iso compute_T_CV(const se8_map* map, const Iso& T_WC)
+{
+    const Iso T_CW = iso_inv(T_WC);
+    Iso T_WM;
+    std::memset(&T_WM, 0, sizeof(T_WM));
+    for (int i = 0; i < 3; i++) {
+        T_WM.R[i][i] = 1.f;
+        T_WM.t[i] = -map->view.t_MW[i];
+    }
+    Iso A = iso_mul(T_CW, T_WM);
+    for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++)
+            A.R[i][j] = A.R[i][j] * map->view.res;
+    for (int i = 0; i < 3; i++)
+        A.t[i] = A.t[i] + (A.R[i][0] * 0.5f + (A.R[i][1] * 0.5f + A.R[i][2] * 0.5f));
+    return A;
+}
+
+static int read_counters(const se8_map* map, int* host, hipStream_t s)
+{
+    SE8_HIP_CHECK(hipMemcpyAsync(host, map->counters.ptr, sizeof(int) * kNumCounters, hipMemcpyDeviceToHost, s));
+    SE8_HIP_CHECK(hipStreamSynchronize(s));
+    return SE8_OK;
+}
+
+static int ensure_candidates(se8_map* map, size_t n)
+{
+    if (n <= map->candidates_cap)
+        return SE8_OK;
+    int rc = dev_alloc(map->candidates, n * sizeof(uint64_t));
+    if (rc == SE8_OK)
+        rc = dev_alloc(map->cur_node, n * sizeof(int));
+    if (rc == SE8_OK)
+        rc = map->sorter.reserve(n);
+    if (rc != SE8_OK)
+        return rc;
+    map->candidates_cap = n;
+    return SE8_OK;
+}
+
+// Sort the candidate keys and build nodes/blocks.  n = number of candidates (host-known).
+static int build_from_candidates(se8_map* map, uint64_t* cand, int n, int* update_list, int mark_new, hipStream_t s)
+{
+    if (n <= 0)
+        return SE8_OK;
+    const int hi_bit = kScaleOffset + 3 * map->view.max_scale;
+    const int lo_bit = kScaleOffset + 3 * kBlockScale;
+    uint64_t* sorted = radix_sort(map->sorter, cand, n, lo_bit, hi_bit, s);
+    const int brick_scale = map->view.brick_shift + kBlockScale;
+    const int grid = (n + 255) / 256;
+    for (int scale = map->view.max_scale - 1; scale > kBlockScale; scale--) {
+        k_build_level<<<grid, 256, 0, s>>>(map->view, sorted, nullptr, n, (int*) map->cur_node.ptr, scale,
+                                            brick_scale);
+    }
+    k_build_blocks<<<grid, 256, 0, s>>>(map->view, sorted, nullptr, n, (const int*) map->cur_node.ptr, update_list,
+                                        mark_new);
+    SE8_HIP_CHECK(hipGetLastError());
+    return SE8_OK;
+}
+
+static int device_count_sm()
+{
+    static int cus = 0;
+    if (cus == 0) {
+        int dev = 0;
+        (void) hipGetDevice(&dev);
+        if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0)
+            cus = 256;
+    }
+    return cus;
+}
+
+} // namespace se8
+
+using namespace se8;
+
+// ==============================================================================================
+// C-ABI
+// ==============================================================================================
+extern "C" const char* se8_last_error(void)
+{
+    return g_last_error.c_str();
+}
+
+extern "C" int se8_default_configs(se8_map_config* map_cfg, se8_tsdf_config* tsdf_cfg, se8_pinhole* pinhole,
+                                   se8_tracker_config* tracker_cfg)
+{
+    if (map_cfg) {
+        // Map::Config defaults (map.hpp:45-53)
+        for (int i = 0; i < 3; i++) {
+            map_cfg->dim[i] = 10.f;
+            map_cfg->t_MW[i] = 5.f;
+        }
+        map_cfg->res = 0.1f;
+    }
+    if (tsdf_cfg) {
+        tsdf_cfg->truncation_boundary_factor = 8.f;
+        tsdf_cfg->max_weight = 100.f;
+        tsdf_cfg->storage = SE8_STORAGE_F32;
+        tsdf_cfg->max_blocks = 0;
+    }
+    if (pinhole) {
+        // SensorBase::Config defaults (sensor.hpp:30-45); intrinsics have no defaults upstream
+        pinhole->width = 640;
+        pinhole->height = 480;
+        pinhole->fx = pinhole->fy = 525.f;
+        pinhole->cx = 319.5f;
+        pinhole->cy = 239.5f;
+        pinhole->near_plane = 0.01f;
+        pinhole->far_plane = 10.f;
+    }
+    if (tracker_cfg) {
+        // TrackerConfig defaults (tracker.hpp:22-27)
+        tracker_cfg->iterations[0] = 10;
+        tracker_cfg->iterations[1] = 5;
+        tracker_cfg->iterations[2] = 4;
+        tracker_cfg->dist_threshold = 0.1f;
+        tracker_cfg->normal_threshold = 0.8f;
+        tracker_cfg->track_threshold = 0.15f;
+        tracker_cfg->icp_threshold = 0.00001f;
+        tracker_cfg->compensated = 0;
+    }
+    return SE8_OK;
+}
+
+extern "C" int se8_map_create(const se8_map_config* cfg, const se8_tsdf_config* tcfg, se8_map** out)
+{
+    SE8_REQUIRE(cfg && tcfg && out, "se8_map_create: null argument");
+    SE8_REQUIRE(cfg->res > 0.f, "se8_map_create: res must be positive");
+    SE8_REQUIRE(tcfg->storage == SE8_STORAGE_F32 || tcfg->storage == SE8_STORAGE_F16,
+                "se8_map_create: unknown storage");
+    SE8_REQUIRE(tcfg->truncation_boundary_factor > 0.f && tcfg->max_weight >= 1.f,
+                "se8_map_create: invalid TSDF config");
+    SE8_REQUIRE(tcfg->storage == SE8_STORAGE_F32 || tcfg->max_weight <= 65535.f,
+                "se8_map_create: max_weight does not fit uint16 weights");
+    *out = nullptr;
+    se8_map* m = new se8_map();
+    m->cfg = *cfg;
+    m->tsdf = *tcfg;
+    // Octree(ceil(max dim / res)) -> power_two_up(max(size, 2*BlockSize)) (octree_impl.hpp:14-19)
+    const float maxdim = std::max(cfg->dim[0], std::max(cfg->dim[1], cfg->dim[2]));
+    int size = (int) std::ceil(maxdim / cfg->res);
+    size = std::max(size, 16);
+    const int p = (int) std::ceil(std::log2((float) size));
+    size = 1 << p;
+    SE8_REQUIRE(p <= 19, "se8_map_create: octree deeper than the 19-scale key limit");
+    MapView& v = m->view;
+    std::memset(&v, 0, sizeof(v));
+    v.size = size;
+    v.max_scale = p;
+    const int brick_scale = std::min(6, p);
+    v.brick_shift = brick_scale - kBlockScale;
+    v.bricks_side = size >> brick_scale;
+    v.res = cfg->res;
+    v.dim = (float) size * cfg->res;
+    v.mu = cfg->res * tcfg->truncation_boundary_factor;
+    v.max_weight = tcfg->max_weight;
+    for (int i = 0; i < 3; i++)
+        v.t_MW[i] = cfg->t_MW[i];
+    v.storage = tcfg->storage;
+    int64_t max_blocks = tcfg->max_blocks;
+    if (max_blocks <= 0) {
+        const int64_t blocks_side = size / 8;
+        max_blocks = std::min<int64_t>(blocks_side * blocks_side * blocks_side, (int64_t) 1 << 18);
+    }
+    v.max_blocks = max_blocks;
+    v.max_nodes = 2 * max_blocks + 64;
+    const int64_t bricks_side3 = (int64_t) v.bricks_side * v.bricks_side * v.bricks_side;
+    v.max_bricks = std::min<int64_t>(bricks_side3, max_blocks / 4 + 64);
+    const size_t vox_bytes = tcfg->storage == SE8_STORAGE_F32 ? 8 : 4;
+    int rc = SE8_OK;
+    rc = rc ? rc : dev_alloc(m->brick_top, sizeof(int) * bricks_side3);
+    rc = rc ? rc : dev_alloc(m->brick_slots, sizeof(int) * 512 * v.max_bricks);
+    rc = rc ? rc : dev_alloc(m->node_child, sizeof(int) * 8 * v.max_nodes);
+    rc = rc ? rc : dev_alloc(m->node_key, sizeof(uint64_t) * v.max_nodes);
+    rc = rc ? rc : dev_alloc(m->node_ts, sizeof(int) * v.max_nodes);
+    rc = rc ? rc : dev_alloc(m->block_key, sizeof(uint64_t) * max_blocks);
+    rc = rc ? rc : dev_alloc(m->block_parent, sizeof(int) * max_blocks);
+    rc = rc ? rc : dev_alloc(m->block_ts, sizeof(int) * max_blocks);
+    rc = rc ? rc : dev_alloc(m->voxels, vox_bytes * 512 * (size_t) max_blocks);
+    rc = rc ? rc : dev_alloc(m->counters, sizeof(int) * kNumCounters);
+    rc = rc ? rc : dev_alloc(m->update_list, sizeof(int) * max_blocks);
+    if (rc != SE8_OK) {
+        se8_map_destroy(m);
+        return rc;
+    }
+    v.brick_top = (int*) m->brick_top.ptr;
+    v.brick_slots = (int*) m->brick_slots.ptr;
+    v.node_child = (int*) m->node_child.ptr;
+    v.node_key = (uint64_t*) m->node_key.ptr;
+    v.node_ts = (int*) m->node_ts.ptr;
+    v.block_key = (uint64_t*) m->block_key.ptr;
+    v.block_parent = (int*) m->block_parent.ptr;
+    v.block_ts = (int*) m->block_ts.ptr;
+    v.voxels = m->voxels.ptr;
+    v.counters = (int*) m->counters.ptr;
+    hipStream_t s = nullptr;
+    rc = rc ? rc : fill_i32(m->brick_top, bricks_side3, -1, s);
+    rc = rc ? rc : fill_i32(m->brick_slots, 512 * v.max_bricks, -1, s);
+    rc = rc ? rc : fill_i32(m->node_child, 8 * v.max_nodes, -1, s);
+    if (rc != SE8_OK) {
+        se8_map_destroy(m);
+        return rc;
+    }
+    int host[kNumCounters];
+    std::memset(host, 0, sizeof(host));
+    host[kCntBlocks] = 0;
+    host[kCntNodes] = 1; // the root
+    host[kCntBricks] = 0;
+    host[kCntAabb + 0] = host[kCntAabb + 1] = host[kCntAabb + 2] = 0x7fffffff;
+    host[kCntAabb + 3] = host[kCntAabb + 4] = host[kCntAabb + 5] = -1;
+    const uint64_t root_key = (uint64_t) p; // code 0, scale max
+    const int root_ts = -1;
+    if (brick_scale == p) {
+        host[kCntBricks] = 1; // the root is the only brick
+        const int zero = 0;
+        if (hipMemcpy(m->brick_top.ptr, &zero, sizeof(int), hipMemcpyHostToDevice) != hipSuccess) {
+            set_error("se8_map_create: hipMemcpy failed");
+            se8_map_destroy(m);
+            return SE8_ERR_HIP;
+        }
+    }
+    if (hipMemcpy(m->counters.ptr, host, sizeof(host), hipMemcpyHostToDevice) != hipSuccess
+        || hipMemcpy(m->node_key.ptr, &root_key, sizeof(root_key), hipMemcpyHostToDevice) != hipSuccess
+        || hipMemcpy(m->node_ts.ptr, &root_ts, sizeof(root_ts), hipMemcpyHostToDevice) != hipSuccess
+        || hipDeviceSynchronize() != hipSuccess) {
+        set_error("se8_map_create: initialisation copy failed");
+        se8_map_destroy(m);
+        return SE8_ERR_HIP;
+    }
+    for (int i = 0; i < 8; i++) {
+        if (hipEventCreate(&m->ev[i]) != hipSuccess) {
+            set_error("se8_map_create: hipEventCreate failed");
+            se8_map_destroy(m);
+            return SE8_ERR_HIP;
+        }
+    }
+    *out = m;
+    return SE8_OK;
+}
+
+extern "C" int se8_map_destroy(se8_map* m)
+{
+    if (!m)
+        return SE8_OK;
+    (void) hipDeviceSynchronize();
+    DeviceBuffer* bufs[] = {&m->brick_top,    &m->brick_slots, &m->node_child, &m->node_key,   &m->node_ts,
+                            &m->block_key,    &m->block_parent, &m->block_ts,  &m->voxels,     &m->counters,
+                            &m->update_list,  &m->candidates,   &m->unique_keys, &m->cur_node};
+    for (DeviceBuffer* b : bufs)
+        dev_free(*b);
+    m->sorter.release();
+    for (int i = 0; i < 8; i++)
+        if (m->ev[i])
+            (void) hipEventDestroy(m->ev[i]);
+    delete m;
+    return SE8_OK;
+}
+
+extern "C" int se8_map_get_info(const se8_map* m, se8_map_info* out)
+{
+    SE8_REQUIRE(m && out, "se8_map_get_info: null argument");
+    int host[kNumCounters];
+    int rc = read_counters(m, host, nullptr);
+    if (rc != SE8_OK)
+        return rc;
+    out->octree_size = m->view.size;
+    out->max_scale = m->view.max_scale;
+    out->res = m->view.res;
+    out->dim = m->view.dim;
+    out->storage = m->view.storage;
+    out->num_blocks = std::min<int64_t>(host[kCntBlocks], m->view.max_blocks);
+    out->num_nodes = std::min<int64_t>(host[kCntNodes], m->view.max_nodes);
+    out->max_blocks = m->view.max_blocks;
+    out->aabb_valid = host[kCntAabb + 3] >= 0;
+    for (int i = 0; i < 3; i++) {
+        out->aabb_min[i] = host[kCntAabb + i];
+        out->aabb_max[i] = host[kCntAabb + 3 + i];
+    }
+    return host[kCntOverflow] ? SE8_ERR_CAPACITY : SE8_OK;
+}
+
+extern "C" int se8_integrate_depth(se8_map* m, int timestamp, const float* depth_dev, const se8_pinhole* sensor,
+                                   const float T_WC16[16], se8_stream stream)
+{
+    SE8_REQUIRE(m && depth_dev && sensor && T_WC16, "se8_integrate_depth: null argument");
+    SE8_REQUIRE(sensor->width > 0 && sensor->height > 0, "se8_integrate_depth: empty image");
+    hipStream_t s = (hipStream_t) stream;
+    const Cam cam = make_cam(*sensor, 1, false);
+    const Iso T_WC = iso_from16(T_WC16);
+    const float band = 2.0f * m->view.res * m->tsdf.truncation_boundary_factor;
+    const int num_steps = (int) std::ceil(band / (2.f * m->view.res));
+    const size_t cand_cap = (size_t) cam.w * cam.h * num_steps;
+    int rc = ensure_candidates(m, cand_cap);
+    if (rc != SE8_OK)
+        return rc;
+
+    SE8_HIP_CHECK(hipEventRecord(m->ev[0], s));
+    k_frame_reset<<<1, 64, 0, s>>>(m->view.counters);
+    const int cus = device_count_sm();
+    k_frustum<<<cus * 4, 256, 0, s>>>(m->view, iso_inv(T_WC), cam, (int*) m->update_list.ptr);
+    k_record_frustum<<<1, 64, 0, s>>>(m->view.counters);
+    SE8_HIP_CHECK(hipEventRecord(m->ev[1], s));
+    dim3 grid((cam.w + 63) / 64, cam.h);
+    k_alloc_rays<<<grid, 64, 0, s>>>(m->view, cam, T_WC, depth_dev, band, num_steps, (uint64_t*) m->candidates.ptr,
+                                     (int64_t) cand_cap);
+    SE8_HIP_CHECK(hipGetLastError());
+    int host[kNumCounters];
+    rc = read_counters(m, host, s);
+    if (rc != SE8_OK)
+        return rc;
+    if (host[kCntOverflow]) {
+        set_error("se8_integrate_depth: map capacity exhausted (raise se8_tsdf_config.max_blocks)");
+        return SE8_ERR_CAPACITY;
+    }
+    const int n_cand = std::min<int>(host[kCntCandidates], (int) cand_cap);
+    SE8_HIP_CHECK(hipEventRecord(m->ev[2], s));
+    rc = build_from_candidates(m, (uint64_t*) m->candidates.ptr, n_cand, (int*) m->update_list.ptr, 1, s);
+    if (rc != SE8_OK)
+        return rc;
+    SE8_HIP_CHECK(hipEventRecord(m->ev[3], s));
+    const Iso T_CV = compute_T_CV(m, T_WC);
+    if (m->view.storage == SE8_STORAGE_F32)
+        k_update<SE8_STORAGE_F32><<<cus * 4, 512, 0, s>>>(m->view, (const int*) m->update_list.ptr, T_CV, cam,
+                                                          depth_dev, timestamp);
+    else
+        k_update<SE8_STORAGE_F16><<<cus * 4, 512, 0, s>>>(m->view, (const int*) m->update_list.ptr, T_CV, cam,
+                                                          depth_dev, timestamp);
+    SE8_HIP_CHECK(hipGetLastError());
+    SE8_HIP_CHECK(hipEventRecord(m->ev[4], s));
+    m->times.last_frustum_blocks = host[kCntFrustum];
+    m->times.last_candidates = n_cand;
+    m->times.last_new_blocks = -1; // resolved lazily by se8_stage_times_get
+    return SE8_OK;
+}
+
+// ---------------------------------------------------------------------------------------------
+// explicit allocation and block access (tests, tools)
+// ---------------------------------------------------------------------------------------------
+namespace se8 {
+__global__ void k_lookup_blocks(MapView m, const int* coords, int n, int* slots)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n)
+        return;
+    const int x = coords[3 * i], y = coords[3 * i + 1], z = coords[3 * i + 2];
+    if (x < 0 || y < 0 || z < 0 || x >= m.size || y >= m.size || z >= m.size) {
+        slots[i] = -1;
+        return;
+    }
+    slots[i] = lookup_block(m, x, y, z);
+}
+
+int lookup_slots(const se8_map* m, const std::vector<int>& coords, std::vector<int>& slots)
+{
+    const int n = (int) (coords.size() / 3);
+    slots.assign(n, -1);
+    if (n == 0)
+        return SE8_OK;
+    DeviceBuffer dc, ds;
+    int rc = dev_alloc(dc, sizeof(int) * coords.size());
+    rc = rc ? rc : dev_alloc(ds, sizeof(int) * n);
+    if (rc == SE8_OK) {
+        if (hipMemcpy(dc.ptr, coords.data(), sizeof(int) * coords.size(), hipMemcpyHostToDevice) != hipSuccess)
+            rc = SE8_ERR_HIP;
+        else {
+            k_lookup_blocks<<<(n + 255) / 256, 256>>>(m->view, (const int*) dc.ptr, n, (int*) ds.ptr);
+            if (hipMemcpy(slots.data(), ds.ptr, sizeof(int) * n, hipMemcpyDeviceToHost) != hipSuccess)
+                rc = SE8_ERR_HIP;
+        }
+    }
+    if (rc == SE8_ERR_HIP)
+        set_error("lookup_slots: HIP copy failed");
+    dev_free(dc);
+    dev_free(ds);
+    return rc;
+}
+} // namespace se8
+
+extern "C" int se8_allocate_blocks(se8_map* m, const int* voxel_coords, int64_t n, int only_new,
+                                   int* out_block_coords, int64_t* out_n)
+{
+    SE8_REQUIRE(m && (n == 0 || voxel_coords) && out_n, "se8_allocate_blocks: null argument");
+    std::vector<uint64_t> keys;
+    keys.reserve((size_t) n);
+    for (int64_t i = 0; i < n; i++) {
+        const int x = voxel_coords[3 * i], y = voxel_coords[3 * i + 1], z = voxel_coords[3 * i + 2];
+        SE8_REQUIRE(x >= 0 && y >= 0 && z >= 0 && x < m->view.size && y < m->view.size && z < m->view.size,
+                    "se8_allocate_blocks: voxel outside the octree");
+        keys.push_back(block_key_of_voxel(x, y, z));
+    }
+    std::sort(keys.begin(), keys.end());
+    keys.erase(std::unique(keys.begin(), keys.end()), keys.end());
+    std::vector<int> coords;
+    for (uint64_t k : keys) {
+        int c[3];
+        key_to_coord(k, c);
+        coords.insert(coords.end(), c, c + 3);
+    }
+    std::vector<int> slots;
+    int rc = lookup_slots(m, coords, slots);
+    if (rc != SE8_OK)
+        return rc;
+    std::vector<uint64_t> missing;
+    for (size_t i = 0; i < keys.size(); i++)
+        if (slots[i] < 0)
+            missing.push_back(keys[i]);
+    if (!missing.empty()) {
+        rc = ensure_candidates(m, missing.size());
+        if (rc != SE8_OK)
+            return rc;
+        SE8_HIP_CHECK(hipMemcpy(m->candidates.ptr, missing.data(), sizeof(uint64_t) * missing.size(),
+                                hipMemcpyHostToDevice));
+        rc = build_from_candidates(m, (uint64_t*) m->candidates.ptr, (int) missing.size(), nullptr, 0, nullptr);
+        if (rc != SE8_OK)
+            return rc;
+        int host[kNumCounters];
+        rc = read_counters(m, host, nullptr);
+        if (rc != SE8_OK)
+            return rc;
+        if (host[kCntOverflow]) {
+            set_error("se8_allocate_blocks: map capacity exhausted");
+            return SE8_ERR_CAPACITY;
+        }
+    }
+    const std::vector<uint64_t>& ret = only_new ? missing : keys;
+    *out_n = (int64_t) ret.size();
+    if (out_block_coords) {
+        for (size_t i = 0; i < ret.size(); i++)
+            key_to_coord(ret[i], out_block_coords + 3 * i);
+    }
+    return SE8_OK;
+}
+
+extern "C" int se8_write_block(se8_map* m, const int bc[3], const float* tsdf512, const float* weight512)
+{
+    SE8_REQUIRE(m && bc && tsdf512 && weight512, "se8_write_block: null argument");
+    std::vector<int> coords(bc, bc + 3), slots;
+    int rc = lookup_slots(m, coords, slots);
+    if (rc != SE8_OK)
+        return rc;
+    SE8_REQUIRE(slots[0] >= 0, "se8_write_block: block not allocated");
+    const size_t off = (size_t) slots[0] * 512;
+    if (m->view.storage == SE8_STORAGE_F32) {
+        std::vector<float> buf(1024);
+        for (int i = 0; i < 512; i++) {
+            buf[2 * i] = tsdf512[i];
+            buf[2 * i + 1] = weight512[i];
+        }
+        SE8_HIP_CHECK(hipMemcpy((float2*) m->voxels.ptr + off, buf.data(), 4096, hipMemcpyHostToDevice));
+    }
+    else {
+        std::vector<uint32_t> buf(512);
+        for (int i = 0; i < 512; i++)
+            buf[i] = pack_f16(tsdf512[i], weight512[i]);
+        SE8_HIP_CHECK(hipMemcpy((uint32_t*) m->voxels.ptr + off, buf.data(), 2048, hipMemcpyHostToDevice));
+    }
+    return SE8_OK;
+}
+
+extern "C" int se8_read_block(const se8_map* m, const int bc[3], float* tsdf512, float* weight512)
+{
+    SE8_REQUIRE(m && bc && tsdf512 && weight512, "se8_read_block: null argument");
+    std::vector<int> coords(bc, bc + 3), slots;
+    int rc = lookup_slots(m, coords, slots);
+    if (rc != SE8_OK)
+        return rc;
+    SE8_REQUIRE(slots[0] >= 0, "se8_read_block: block not allocated");
+    const size_t off = (size_t) slots[0] * 512;
+    if (m->view.storage == SE8_STORAGE_F32) {
+        std::vector<float> buf(1024);
+        SE8_HIP_CHECK(hipMemcpy(buf.data(), (float2*) m->voxels.ptr + off, 4096, hipMemcpyDeviceToHost));
+        for (int i = 0; i < 512; i++) {
+            tsdf512[i] = buf[2 * i];
+            weight512[i] = buf[2 * i + 1];
+        }
+    }
+    else {
+        std::vector<uint32_t> buf(512);
+        SE8_HIP_CHECK(hipMemcpy(buf.data(), (uint32_t*) m->voxels.ptr + off, 2048, hipMemcpyDeviceToHost));
+        for (int i = 0; i < 512; i++)
+            unpack_f16(buf[i], tsdf512[i], weight512[i]);
+    }
+    return SE8_OK;
+}
+
+extern "C" int se8_dump_blocks(const se8_map* m, uint64_t* keys, int* timestamps, float* tsdf, float* weight,
+                               int64_t cap, int64_t* out_n)
+{
+    SE8_REQUIRE(m && out_n, "se8_dump_blocks: null argument");
+    int host[kNumCounters];
+    int rc = read_counters(m, host, nullptr);
+    if (rc != SE8_OK)
+        return rc;
+    const int64_t n = std::min<int64_t>(host[kCntBlocks], m->view.max_blocks);
+    *out_n = n;
+    if (!keys)
+        return SE8_OK;
+    SE8_REQUIRE(cap >= n, "se8_dump_blocks: output capacity too small");
+    std::vector<uint64_t> k(n);
+    std::vector<int> ts(n);
+    if (n > 0) {
+        SE8_HIP_CHECK(hipMemcpy(k.data(), m->block_key.ptr, sizeof(uint64_t) * n, hipMemcpyDeviceToHost));
+        SE8_HIP_CHECK(hipMemcpy(ts.data(), m->block_ts.ptr, sizeof(int) * n, hipMemcpyDeviceToHost));
+    }
+    std::vector<int64_t> order(n);
+    for (int64_t i = 0; i < n; i++)
+        order[i] = i;
+    std::sort(order.begin(), order.end(), [&](int64_t a, int64_t b) { return k[a] < k[b]; });
+    std::vector<uint32_t> vox16;
+    std::vector<float> vox32;
+    if ((tsdf || weight) && n > 0) {
+        if (m->view.storage == SE8_STORAGE_F32) {
+            vox32.resize((size_t) n * 1024);
+            SE8_HIP_CHECK(hipMemcpy(vox32.data(), m->voxels.ptr, sizeof(float) * vox32.size(), hipMemcpyDeviceToHost));
+        }
+        else {
+            vox16.resize((size_t) n * 512);
+            SE8_HIP_CHECK(
+                hipMemcpy(vox16.data(), m->voxels.ptr, sizeof(uint32_t) * vox16.size(), hipMemcpyDeviceToHost));
+        }
+    }
+    for (int64_t i = 0; i < n; i++) {
+        const int64_t s = order[i];
+        keys[i] = k[s];
+        if (timestamps)
+            timestamps[i] = ts[s];
+        for (int v = 0; v < 512 && (tsdf || weight); v++) {
+            float t, w;
+            if (m->view.storage == SE8_STORAGE_F32) {
+                t = vox32[(size_t) s * 1024 + 2 * v];
+                w = vox32[(size_t) s * 1024 + 2 * v + 1];
+            }
+            else
+                unpack_f16(vox16[(size_t) s * 512 + v], t, w);
+            if (tsdf)
+                tsdf[(size_t) i * 512 + v] = t;
+            if (weight)
+                weight[(size_t) i * 512 + v] = w;
+        }
+    }
+    return SE8_OK;
+}
+
+extern "C" int se8_dump_nodes(const se8_map* m, uint64_t* keys, uint8_t* masks, int* timestamps, int64_t cap,
+                              int64_t* out_n)
+{
+    SE8_REQUIRE(m && out_n, "se8_dump_nodes: null argument");
+    int host[kNumCounters];
+    int rc = read_counters(m, host, nullptr);
+    if (rc != SE8_OK)
+        return rc;
+    const int64_t n = std::min<int64_t>(host[kCntNodes], m->view.max_nodes);
+    *out_n = n;
+    if (!keys)
+        return SE8_OK;
+    SE8_REQUIRE(cap >= n, "se8_dump_nodes: output capacity too small");
+    std::vector<uint64_t> k(n);
+    std::vector<int> ch((size_t) n * 8), ts(n);
+    SE8_HIP_CHECK(hipMemcpy(k.data(), m->node_key.ptr, sizeof(uint64_t) * n, hipMemcpyDeviceToHost));
+    SE8_HIP_CHECK(hipMemcpy(ch.data(), m->node_child.ptr, sizeof(int) * 8 * n, hipMemcpyDeviceToHost));
+    SE8_HIP_CHECK(hipMemcpy(ts.data(), m->node_ts.ptr, sizeof(int) * n, hipMemcpyDeviceToHost));
+    std::vector<int64_t> order(n);
+    for (int64_t i = 0; i < n; i++)
+        order[i] = i;
+    std::sort(order.begin(), order.end(), [&](int64_t a, int64_t b) { return k[a] < k[b]; });
+    for (int64_t i = 0; i < n; i++) {
+        const int64_t s = order[i];
+        keys[i] = k[s];
+        uint8_t mk = 0;
+        for (int c = 0; c < 8; c++)
+            if (ch[s * 8 + c] >= 0)
+                mk |= (uint8_t) (1u << c);
+        if (masks)
+            masks[i] = mk;
+        if (timestamps)
+            timestamps[i] = ts[s];
+    }
+    return SE8_OK;
+}
+
+extern "C" int se8_stage_times_get(const se8_map* m, const se8_tracker* tr, se8_stage_times* out)
+{
+    SE8_REQUIRE(out, "se8_stage_times_get: null argument");
+    std::memset(out, 0, sizeof(*out));
+    if (m) {
+        se8_map* mm = const_cast<se8_map*>(m);
+        float ms = 0.f;
+        if (hipEventSynchronize(mm->ev[4]) == hipSuccess) {
+            if (hipEventElapsedTime(&ms, mm->ev[0], mm->ev[1]) == hipSuccess)
+                mm->times.fetch_frustum = ms;
+            if (hipEventElapsedTime(&ms, mm->ev[1], mm->ev[2]) == hipSuccess)
+                mm->times.create_list = ms;
+            if (hipEventElapsedTime(&ms, mm->ev[2], mm->ev[3]) == hipSuccess)
+                mm->times.allocate_list = ms;
+            if (hipEventElapsedTime(&ms, mm->ev[3], mm->ev[4]) == hipSuccess)
+                mm->times.update = ms;
+        }
+        if (hipEventSynchronize(mm->ev[6]) == hipSuccess && hipEventElapsedTime(&ms, mm->ev[5], mm->ev[6]) == hipSuccess)
+            mm->times.raycast = ms;
+        (void) hipGetLastError();
+        int host[kNumCounters];
+        if (read_counters(m, host, nullptr) == SE8_OK)
+            mm->times.last_new_blocks = host[kCntUpdate] - host[kCntFrustum];
+        *out = mm->times;
+    }
+    if (tr) {
+        se8_tracker* tt = const_cast<se8_tracker*>(tr);
+        float ms = 0.f;
+        if (hipEventSynchronize(tt->ev[1]) == hipSuccess && hipEventElapsedTime(&ms, tt->ev[0], tt->ev[1]) == hipSuccess)
+            out->tracking = ms;
+        (void) hipGetLastError();
+    }
+    return SE8_OK;
+}

@@ -1,0 +1,527 @@
+// Raycast and map queries of libse8.so.
+//
+// raycast_volume (include/se/map/impl/raycaster_impl.hpp:537-627): one thread per pixel in 16x8
+// tiles.  Per ray:
+//   1. first allocated block along the ray and its entry t, by a stack-based octree traversal in
+//      the scaled [1,2]^3 octree space (Laine & Karras 2010, "Efficient Sparse Voxel Octrees"),
+//      with the same float operations as VoxelBlockRayIterator
+//      (voxel_block_ray_iterator_impl.hpp:19-249) so t_min is bit-identical; the traversal stack
+//      lives in LDS;
+//   2. the TSDF march of raycast<TSDF> (raycaster_impl.hpp:456-533) with O(1) voxel lookups
+//      through the brick index instead of octree descents;
+//   3. the normal from the single-res 32-sample gradient (visitor_impl.hpp:1094-1251).
+#include <hip/hip_runtime.h>
+
+#include <cstring>
+#include <vector>
+
+#include "se8_internal.hpp"
+
+namespace se8 {
+
+constexpr int kStackDepth = 23;  // CAST_STACK_DEPTH (voxel_block_ray_iterator.hpp:15)
+constexpr int kMaxLevels = 16;   // log2(size / 8) for the largest supported octree (2^19)
+constexpr int kRayThreads = 128; // 16 x 8 pixel tile
+
+__device__ __forceinline__ int f2i(float f)
+{
+    return __float_as_int(f);
+}
+
+__device__ __forceinline__ float i2f(int i)
+{
+    return __int_as_float(i);
+}
+
+struct Span {
+    float t_min;  // tcmin(): entry t of the first allocated block (metres)
+    float t_max;  // tmax(): t_max_init (metres)
+};
+
+// st_parent / st_tmax: this thread's stack slots in LDS, stride = threads per workgroup.
+__device__ Span first_block(const MapView& m, const float oW[3], const float dW[3], float near_t, float far_t,
+                            int* st_parent, float* st_tmax, int stride)
+{
+    const float scaling = (float) m.size * m.res;
+    const int lg = m.max_scale - kBlockScale;
+    const int min_scale = kStackDepth - lg;
+    const float eps = exp2f((float) -m.max_scale);
+    float tc[3], tb[3];
+    int octant_mask = 7;
+#pragma unroll
+    for (int i = 0; i < 3; i++) {
+        const float d = fabsf(dW[i]) < eps ? copysignf(eps, dW[i]) : dW[i];
+        const float pM = oW[i] + m.t_MW[i];
+        const float o = (pM / m.res) / (float) m.size + 1.0f;
+        tc[i] = -1.f * (1.0f / fabsf(d));
+        tb[i] = tc[i] * o;
+        if (d > 0.0f) {
+            octant_mask ^= 1 << i;
+            tb[i] = 3.0f * tc[i] - tb[i];
+        }
+    }
+    float tmin_init = fmaxf(fmaxf(2.0f * tc[0] - tb[0], 2.0f * tc[1] - tb[1]), 2.0f * tc[2] - tb[2]);
+    float tmax_init = fminf(fminf(tc[0] - tb[0], tc[1] - tb[1]), tc[2] - tb[2]);
+    float h = tmax_init;
+    tmin_init = fmaxf(tmin_init, near_t / scaling);
+    tmax_init = fminf(tmax_init, far_t / scaling);
+    float t_min = tmin_init, t_max = tmax_init;
+    float pos[3] = {1.f, 1.f, 1.f};
+    int idx = 0;
+#pragma unroll
+    for (int i = 0; i < 3; i++) {
+        if (1.5f * tc[i] - tb[i] > t_min) {
+            idx ^= 1 << i;
+            pos[i] = 1.5f;
+        }
+    }
+    for (int l = 0; l < lg; l++)
+        st_parent[l * stride] = -1;
+    int parent = 0;
+    int scale = kStackDepth - 1;
+    float scale_exp2 = 0.5f;
+    // the traversal visits each octant at most once; the cap only guards against a corrupt octree
+    for (int guard = 0; scale < kStackDepth && guard < (1 << 20); guard++) {
+        float tcorner[3];
+#pragma unroll
+        for (int i = 0; i < 3; i++)
+            tcorner[i] = pos[i] * tc[i] - tb[i];
+        const float tc_max = fminf(fminf(tcorner[0], tcorner[1]), tcorner[2]);
+        const int child = m.node_child[parent * 8 + (idx ^ octant_mask ^ 7)];
+        if (scale == min_scale && child >= 0)
+            break;
+        if (child >= 0 && t_min <= t_max) {
+            const float tv_max = fminf(t_max, tc_max);
+            const float half = scale_exp2 * 0.5f;
+            float tcen[3];
+#pragma unroll
+            for (int i = 0; i < 3; i++)
+                tcen[i] = half * tc[i] + tcorner[i];
+            if (tc_max < h) {
+                const int l = kStackDepth - 1 - scale;
+                st_parent[l * stride] = parent;
+                st_tmax[l * stride] = t_max;
+            }
+            h = tc_max;
+            parent = child;
+            idx = 0;
+            scale--;
+            scale_exp2 = half;
+#pragma unroll
+            for (int i = 0; i < 3; i++)
+                if (tcen[i] > t_min)
+                    idx ^= 1 << i;
+#pragma unroll
+            for (int i = 0; i < 3; i++)
+                pos[i] = pos[i] + scale_exp2 * (float) ((idx >> i) & 1);
+            t_max = tv_max;
+            continue;
+        }
+        const int step_mask =
+            (tcorner[0] <= tc_max) | ((tcorner[1] <= tc_max) << 1) | ((tcorner[2] <= tc_max) << 2);
+#pragma unroll
+        for (int i = 0; i < 3; i++)
+            pos[i] = pos[i] - scale_exp2 * (float) ((step_mask >> i) & 1);
+        t_min = tc_max;
+        idx ^= step_mask;
+        if ((idx & step_mask) != 0) {
+            unsigned int diff = 0;
+#pragma unroll
+            for (int i = 0; i < 3; i++)
+                if (step_mask & (1 << i))
+                    diff |= (unsigned) (f2i(pos[i]) ^ f2i(pos[i] + scale_exp2));
+            scale = (f2i((float) diff) >> 23) - 127;
+            if (scale >= kStackDepth)
+                break;
+            const int l = kStackDepth - 1 - scale;
+            if (l < 0 || l >= lg || st_parent[l * stride] < 0) {
+                scale = kStackDepth; // unreachable for a consistent octree
+                break;
+            }
+            scale_exp2 = i2f((scale - kStackDepth + 127) << 23);
+            parent = st_parent[l * stride];
+            t_max = st_tmax[l * stride];
+            int sh[3];
+#pragma unroll
+            for (int i = 0; i < 3; i++) {
+                sh[i] = f2i(pos[i]) >> scale;
+                pos[i] = i2f(sh[i] << scale);
+            }
+            idx = (sh[0] & 1) | ((sh[1] & 1) << 1) | ((sh[2] & 1) << 2);
+            h = 0.0f;
+        }
+    }
+    Span s;
+    s.t_min = t_min * scaling;
+    s.t_max = tmax_init * scaling;
+    return s;
+}
+
+// pointToVoxel<Safe::On>: voxel of a world point, false outside the map
+__device__ __forceinline__ bool point_to_voxel(const MapView& m, const float p[3], int v[3])
+{
+    float pM[3];
+#pragma unroll
+    for (int i = 0; i < 3; i++)
+        pM[i] = p[i] + m.t_MW[i];
+    if (!(pM[0] >= 0.f && pM[0] < m.dim && pM[1] >= 0.f && pM[1] < m.dim && pM[2] >= 0.f && pM[2] < m.dim))
+        return false;
+#pragma unroll
+    for (int i = 0; i < 3; i++)
+        v[i] = (int) (pM[i] / m.res);
+    return true;
+}
+
+__device__ __forceinline__ bool point_to_voxel_f(const MapView& m, const float p[3], float v[3])
+{
+    float pM[3];
+#pragma unroll
+    for (int i = 0; i < 3; i++)
+        pM[i] = p[i] + m.t_MW[i];
+    if (!(pM[0] >= 0.f && pM[0] < m.dim && pM[1] >= 0.f && pM[1] < m.dim && pM[2] >= 0.f && pM[2] < m.dim))
+        return false;
+#pragma unroll
+    for (int i = 0; i < 3; i++)
+        v[i] = pM[i] / m.res;
+    return true;
+}
+
+// Map::getData<Safe::On>
+__device__ __forceinline__ void get_data(const MapView& m, const float p[3], float& tsdf, float& weight)
+{
+    int v[3];
+    if (!point_to_voxel(m, p, v)) {
+        tsdf = 1.f;
+        weight = 0.f;
+        return;
+    }
+    get_voxel(m, v[0], v[1], v[2], tsdf, weight);
+}
+
+// single-res getInterp (visitor_impl.hpp:976-1024) from voxel coordinates.  The reference's
+// octree-AABB test is implied by the requirement that the block of base be allocated.
+__device__ bool field_interp_vox(const MapView& m, const float vf[3], float& out)
+{
+    float bf[3];
+    int base[3];
+#pragma unroll
+    for (int i = 0; i < 3; i++) {
+        bf[i] = vf[i] - 0.5f;
+        base[i] = (int) bf[i];
+    }
+#pragma unroll
+    for (int i = 0; i < 3; i++)
+        if (base[i] + 1 >= m.size || base[i] < 0)
+            return false;
+    float d[8];
+#pragma unroll
+    for (int k = 0; k < 8; k++) {
+        const int x = base[0] + (k & 1), y = base[1] + ((k >> 1) & 1), z = base[2] + ((k >> 2) & 1);
+        const int slot = lookup_block(m, x, y, z);
+        if (slot < 0)
+            return false;
+        float t, w;
+        load_voxel(m, slot, (x & 7) + ((y & 7) << 3) + ((z & 7) << 6), t, w);
+        if (w == 0.f)
+            return false;
+        d[k] = t;
+    }
+    float t[3], tcm[3];
+#pragma unroll
+    for (int i = 0; i < 3; i++) {
+        t[i] = bf[i] - floorf(bf[i]);
+        tcm[i] = 1.f - t[i];
+    }
+    out = ((d[0] * tcm[0] + d[1] * t[0]) * tcm[1] + (d[2] * tcm[0] + d[3] * t[0]) * t[1]) * tcm[2]
+        + ((d[4] * tcm[0] + d[5] * t[0]) * tcm[1] + (d[6] * tcm[0] + d[7] * t[0]) * t[1]) * t[2];
+    return true;
+}
+
+__device__ __forceinline__ bool valid_field(const MapView& m, int x, int y, int z, float& f)
+{
+    float w;
+    get_voxel(m, x, y, z, f, w);
+    return w != 0.f;
+}
+
+// single-res getFieldGrad (visitor_impl.hpp:1094-1251): trilinear blend of the central differences
+// at the 8 voxels around the query; x0.5 (not yet divided by res).
+__device__ bool field_grad_vox(const MapView& m, const float vf[3], float g[3])
+{
+    float fac[3];
+    int base[3];
+    int c[3][4];
+#pragma unroll
+    for (int i = 0; i < 3; i++) {
+        const float s = vf[i] - 0.5f;
+        fac[i] = s - floorf(s);
+        base[i] = (int) s;
+        c[i][0] = max(base[i] - 1, 0);
+        c[i][1] = max(base[i], 0);
+        c[i][2] = min(base[i] + 1, m.size - 1);
+        c[i][3] = min(base[i] + 2, m.size - 1);
+    }
+    if (lookup_block(m, base[0], base[1], base[2]) < 0)
+        return false;
+    const float rx = 1.f - fac[0], ry = 1.f - fac[1], rz = 1.f - fac[2];
+#pragma unroll
+    for (int a = 0; a < 3; a++) {
+        float cd[8];
+#pragma unroll
+        for (int k = 0; k < 8; k++) {
+            const int ck[3] = {k & 1, (k >> 1) & 1, (k >> 2) & 1};
+            int vp[3], vm[3];
+#pragma unroll
+            for (int i = 0; i < 3; i++) {
+                vp[i] = c[i][1 + ck[i]];
+                vm[i] = c[i][1 + ck[i]];
+            }
+            vp[a] = c[a][2 + ck[a]];
+            vm[a] = c[a][ck[a]];
+            float fp, fm;
+            if (!valid_field(m, vp[0], vp[1], vp[2], fp) || !valid_field(m, vm[0], vm[1], vm[2], fm))
+                return false;
+            cd[k] = fp - fm;
+        }
+        const float v = ((cd[0] * rx + cd[1] * fac[0]) * ry + (cd[2] * rx + cd[3] * fac[0]) * fac[1]) * rz
+            + ((cd[4] * rx + cd[5] * fac[0]) * ry + (cd[6] * rx + cd[7] * fac[0]) * fac[1]) * fac[2];
+        g[a] = 0.5f * v;
+    }
+    return true;
+}
+
+// raycast<TSDF> (raycaster_impl.hpp:456-533)
+__device__ bool march(const MapView& m, const float o[3], const float d[3], float t_near, float t_far, int* st_parent,
+                      float* st_tmax, int stride, float hit[3])
+{
+    const Span span = first_block(m, o, d, t_near, t_far, st_parent, st_tmax, stride);
+    const float step = m.res;
+    const float largestep = 8.f * step;
+    if (span.t_min <= 0.f)
+        return false;
+    float t = span.t_min;
+    if (!(t_near < span.t_max))
+        return false;
+    float stepsize = largestep;
+    float p[3];
+#pragma unroll
+    for (int i = 0; i < 3; i++)
+        p[i] = o[i] + d[i] * t;
+    float f_t, w;
+    get_data(m, p, f_t, w);
+    float f_tt = 0.f;
+    if (!(f_t >= 0.f))
+        return false;
+    for (; t < t_far; t += stepsize) {
+        float f_new;
+        get_data(m, p, f_new, w);
+        if (w == 0.f) {
+            stepsize = largestep;
+#pragma unroll
+            for (int i = 0; i < 3; i++)
+                p[i] = p[i] + stepsize * d[i];
+            continue;
+        }
+        f_tt = f_new;
+        if (f_tt <= 0.1f && f_tt >= -0.5f) {
+            float vf[3];
+            if (point_to_voxel_f(m, p, vf)) {
+                float fi;
+                if (field_interp_vox(m, vf, fi))
+                    f_tt = fi;
+            }
+        }
+        if (f_tt < 0.f)
+            break;
+        stepsize = fmaxf(f_tt * m.mu, step);
+#pragma unroll
+        for (int i = 0; i < 3; i++)
+            p[i] = p[i] + stepsize * d[i];
+        f_t = f_tt;
+    }
+    if (f_tt < 0.f) {
+        t = t - stepsize * (f_tt - 0.f) / (f_tt - f_t);
+#pragma unroll
+        for (int i = 0; i < 3; i++)
+            hit[i] = o[i] + d[i] * t;
+        return true;
+    }
+    return false;
+}
+
+__global__ __launch_bounds__(kRayThreads) void k_raycast(MapView m, Cam cam, Iso T_WS, float* points, float* normals,
+                                                         int8_t* scale_out)
+{
+    __shared__ int st_parent[kMaxLevels * kRayThreads];
+    __shared__ float st_tmax[kMaxLevels * kRayThreads];
+    const int t = threadIdx.x;
+    const int x = blockIdx.x * 16 + (t & 15);
+    const int y = blockIdx.y * 8 + (t >> 4);
+    if (x >= cam.w || y >= cam.h)
+        return;
+    const size_t idx = x + (size_t) y * cam.w;
+    float dS[3], dn[3], dW[3];
+    cam_backproject(cam, (float) x, (float) y, dS);
+    normalized3(dS, dn);
+    lin_apply(T_WS, dn, dW);
+    float hit[3];
+    float nrm[3] = {0.f, 0.f, 0.f};
+    const bool ok = march(m, T_WS.t, dW, cam_near_dist(cam, dS), cam_far_dist(cam, dS), st_parent + t,
+                          st_tmax + t, kRayThreads, hit);
+    if (ok) {
+        if (scale_out)
+            scale_out[idx] = 0;
+        float vf[3], g[3];
+        if (point_to_voxel_f(m, hit, vf) && field_grad_vox(m, vf, g)) {
+#pragma unroll
+            for (int i = 0; i < 3; i++)
+                g[i] = g[i] / m.res;
+            normalized3(g, nrm);
+        }
+    }
+    else {
+        hit[0] = hit[1] = hit[2] = 0.f;
+    }
+    points[3 * idx + 0] = hit[0];
+    points[3 * idx + 1] = hit[1];
+    points[3 * idx + 2] = hit[2];
+    normals[3 * idx + 0] = nrm[0];
+    normals[3 * idx + 1] = nrm[1];
+    normals[3 * idx + 2] = nrm[2];
+}
+
+// ----------------------------------------------------------------------------------------------
+// point queries
+// ----------------------------------------------------------------------------------------------
+__global__ void k_query(MapView m, const float* pts, int n, int mode, float* out, uint8_t* valid)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n)
+        return;
+    const float p[3] = {pts[3 * i], pts[3 * i + 1], pts[3 * i + 2]};
+    if (mode == 0) {
+        float t, w;
+        get_data(m, p, t, w);
+        out[2 * i] = t;
+        out[2 * i + 1] = w;
+        return;
+    }
+    float vf[3];
+    if (!point_to_voxel_f(m, p, vf)) {
+        valid[i] = 0;
+        return;
+    }
+    if (mode == 1) {
+        float f;
+        const bool ok = field_interp_vox(m, vf, f);
+        valid[i] = ok;
+        out[i] = ok ? f : 0.f;
+    }
+    else {
+        float g[3] = {0.f, 0.f, 0.f};
+        const bool ok = field_grad_vox(m, vf, g);
+        valid[i] = ok;
+        for (int k = 0; k < 3; k++)
+            out[3 * i + k] = ok ? g[k] / m.res : 0.f;
+    }
+}
+
+static int run_query(const se8_map* m, const float* pts, int64_t n, int mode, std::vector<float>& out,
+                     std::vector<uint8_t>& valid)
+{
+    const int width = mode == 0 ? 2 : (mode == 1 ? 1 : 3);
+    out.assign((size_t) n * width, 0.f);
+    valid.assign((size_t) n, 0);
+    if (n == 0)
+        return SE8_OK;
+    DeviceBuffer dp, dout, dval;
+    int rc = dev_alloc(dp, sizeof(float) * 3 * n);
+    rc = rc ? rc : dev_alloc(dout, sizeof(float) * width * n);
+    rc = rc ? rc : dev_alloc(dval, (size_t) n);
+    if (rc == SE8_OK) {
+        if (hipMemcpy(dp.ptr, pts, sizeof(float) * 3 * n, hipMemcpyHostToDevice) != hipSuccess)
+            rc = SE8_ERR_HIP;
+        else {
+            k_query<<<(unsigned) ((n + 255) / 256), 256>>>(m->view, (const float*) dp.ptr, (int) n, mode,
+                                                           (float*) dout.ptr, (uint8_t*) dval.ptr);
+            if (hipMemcpy(out.data(), dout.ptr, sizeof(float) * width * n, hipMemcpyDeviceToHost) != hipSuccess
+                || hipMemcpy(valid.data(), dval.ptr, (size_t) n, hipMemcpyDeviceToHost) != hipSuccess)
+                rc = SE8_ERR_HIP;
+        }
+    }
+    if (rc == SE8_ERR_HIP)
+        set_error("map query: HIP call failed");
+    dev_free(dp);
+    dev_free(dout);
+    dev_free(dval);
+    return rc;
+}
+
+int launch_raycast(const se8_map* m, const Cam& cam, const Iso& T_WS, float* points, float* normals, int8_t* scale,
+                   hipStream_t s)
+{
+    dim3 grid((cam.w + 15) / 16, (cam.h + 7) / 8);
+    k_raycast<<<grid, kRayThreads, 0, s>>>(m->view, cam, T_WS, points, normals, scale);
+    SE8_HIP_CHECK(hipGetLastError());
+    return SE8_OK;
+}
+
+} // namespace se8
+
+using namespace se8;
+
+extern "C" int se8_raycast(const se8_map* m, const se8_pinhole* sensor, const float T_WS16[16], float* points_dev,
+                           float* normals_dev, int8_t* scale_dev, se8_stream stream)
+{
+    SE8_REQUIRE(m && sensor && T_WS16 && points_dev && normals_dev, "se8_raycast: null argument");
+    SE8_REQUIRE(sensor->width > 0 && sensor->height > 0, "se8_raycast: empty image");
+    SE8_REQUIRE(m->view.max_scale - kBlockScale <= kMaxLevels, "se8_raycast: octree too deep");
+    hipStream_t s = (hipStream_t) stream;
+    se8_map* mm = const_cast<se8_map*>(m);
+    const Cam cam = make_cam(*sensor, 1, false);
+    SE8_HIP_CHECK(hipEventRecord(mm->ev[5], s));
+    int rc = launch_raycast(m, cam, iso_from16(T_WS16), points_dev, normals_dev, scale_dev, s);
+    SE8_HIP_CHECK(hipEventRecord(mm->ev[6], s));
+    return rc;
+}
+
+extern "C" int se8_get_data(const se8_map* m, const float* pts, int64_t n, float* tsdf, float* weight)
+{
+    SE8_REQUIRE(m && (n == 0 || (pts && tsdf && weight)), "se8_get_data: null argument");
+    std::vector<float> out;
+    std::vector<uint8_t> valid;
+    int rc = run_query(m, pts, n, 0, out, valid);
+    if (rc != SE8_OK)
+        return rc;
+    for (int64_t i = 0; i < n; i++) {
+        tsdf[i] = out[2 * i];
+        weight[i] = out[2 * i + 1];
+    }
+    return SE8_OK;
+}
+
+extern "C" int se8_field_interp(const se8_map* m, const float* pts, int64_t n, float* outv, uint8_t* valid)
+{
+    SE8_REQUIRE(m && (n == 0 || (pts && outv && valid)), "se8_field_interp: null argument");
+    std::vector<float> out;
+    std::vector<uint8_t> val;
+    int rc = run_query(m, pts, n, 1, out, val);
+    if (rc != SE8_OK)
+        return rc;
+    std::memcpy(outv, out.data(), sizeof(float) * n);
+    std::memcpy(valid, val.data(), (size_t) n);
+    return SE8_OK;
+}
+
+extern "C" int se8_field_grad(const se8_map* m, const float* pts, int64_t n, float* out3, uint8_t* valid)
+{
+    SE8_REQUIRE(m && (n == 0 || (pts && out3 && valid)), "se8_field_grad: null argument");
+    std::vector<float> out;
+    std::vector<uint8_t> val;
+    int rc = run_query(m, pts, n, 2, out, val);
+    if (rc != SE8_OK)
+        return rc;
+    std::memcpy(out3, out.data(), sizeof(float) * 3 * n);
+    std::memcpy(valid, val.data(), (size_t) n);
+    return SE8_OK;
+}

@@ -1,0 +1,253 @@
+"""Python mirror of the reference's C++ API for the single-res TSDF path, over libse8.so.
+
+Names follow the reference (se::TSDFMap, se::MapIntegrator::integrateDepth,
+se::raycaster::raycast_volume, se::Tracker::track, se::PinholeCamera).  Device images are torch
+tensors on the current HIP device; results stay in HBM.  This is the check harness's interface;
+the drop-in for C++ callers is include/se8/supereight.hpp over the same C-ABI.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import capi
+
+
+def _torch():
+    import torch
+
+    return torch
+
+
+def _stream_handle(stream=None):
+    torch = _torch()
+    s = stream if stream is not None else torch.cuda.current_stream()
+    return C.c_void_p(s.cuda_stream)
+
+
+def _dev_f32(t, shape=None):
+    torch = _torch()
+    if not (isinstance(t, torch.Tensor) and t.is_cuda and t.dtype == torch.float32 and t.is_contiguous()):
+        raise ValueError("expected a contiguous float32 tensor on the GPU")
+    if shape is not None and tuple(t.shape) != tuple(shape):
+        raise ValueError(f"expected shape {shape}, got {tuple(t.shape)}")
+    return C.c_void_p(t.data_ptr())
+
+
+@dataclass
+class PinholeCamera:
+    """se::PinholeCamera::Config (pinhole_camera.hpp:24-40 + sensor.hpp:30-45)."""
+
+    width: int = 640
+    height: int = 480
+    fx: float = 525.0
+    fy: float = 525.0
+    cx: float = 319.5
+    cy: float = 239.5
+    near_plane: float = 0.4
+    far_plane: float = 6.0
+
+    def c(self) -> capi.Pinhole:
+        return capi.Pinhole(self.width, self.height, self.fx, self.fy, self.cx, self.cy, self.near_plane, self.far_plane)
+
+
+@dataclass
+class TrackerConfig:
+    """se::TrackerConfig (tracker.hpp:22-33) + the compensated-reduction switch."""
+
+    iterations: tuple = (10, 5, 4)
+    dist_threshold: float = 0.1
+    normal_threshold: float = 0.8
+    track_threshold: float = 0.15
+    icp_threshold: float = 1e-5
+    compensated: bool = False
+
+    def c(self) -> capi.TrackerConfig:
+        return capi.TrackerConfig((C.c_int * 3)(*self.iterations), self.dist_threshold, self.normal_threshold,
+                                  self.track_threshold, self.icp_threshold, int(self.compensated))
+
+
+class TSDFMap:
+    """se::TSDFMap<se::Res::Single> (map.hpp:621-622): Map(dim, res, data_config)."""
+
+    def __init__(self, dim=10.24, res=0.01, truncation_boundary_factor=8.0, max_weight=100.0, storage="f32",
+                 max_blocks=0, t_MW=None):
+        self.lib = capi.load()
+        dims = (dim, dim, dim) if np.isscalar(dim) else tuple(dim)
+        t = tuple(x / 2 for x in dims) if t_MW is None else tuple(t_MW)
+        self.map_cfg = capi.MapConfig((C.c_float * 3)(*dims), res, (C.c_float * 3)(*t))
+        st = {"f32": capi.STORAGE_F32, "f16": capi.STORAGE_F16}[storage]
+        self.tsdf_cfg = capi.TsdfConfig(truncation_boundary_factor, max_weight, st, int(max_blocks))
+        self.h = C.c_void_p()
+        capi.check(self.lib.se8_map_create(C.byref(self.map_cfg), C.byref(self.tsdf_cfg), C.byref(self.h)),
+                   "se8_map_create")
+        self.storage = storage
+
+    def close(self):
+        if getattr(self, "h", None) and self.h.value:
+            self.lib.se8_map_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def info(self) -> capi.MapInfo:
+        out = capi.MapInfo()
+        rc = self.lib.se8_map_get_info(self.h, C.byref(out))
+        if rc not in (capi.SE8_OK, capi.SE8_ERR_CAPACITY):
+            capi.check(rc, "se8_map_get_info")
+        return out
+
+    def getRes(self):
+        return self.map_cfg.res
+
+    def octree_size(self):
+        return self.info().octree_size
+
+    def aabb(self):
+        i = self.info()
+        return bool(i.aabb_valid), np.array(i.aabb_min[:]), np.array(i.aabb_max[:])
+
+    def allocate_blocks(self, voxel_coords, only_new=False):
+        c = np.ascontiguousarray(voxel_coords, dtype=np.int32).reshape(-1, 3)
+        out = np.zeros((max(len(c), 1), 3), np.int32)
+        n = C.c_int64()
+        capi.check(self.lib.se8_allocate_blocks(self.h, capi.iptr(c), len(c), int(only_new), capi.iptr(out),
+                                                C.byref(n)), "se8_allocate_blocks")
+        return out[: n.value].copy()
+
+    def write_block(self, block_coord, tsdf, weight):
+        bc = np.ascontiguousarray(block_coord, dtype=np.int32)
+        t = np.ascontiguousarray(tsdf, dtype=np.float32).reshape(512)
+        w = np.ascontiguousarray(weight, dtype=np.float32).reshape(512)
+        capi.check(self.lib.se8_write_block(self.h, capi.iptr(bc), capi.fptr(t), capi.fptr(w)), "se8_write_block")
+
+    def read_block(self, block_coord):
+        bc = np.ascontiguousarray(block_coord, dtype=np.int32)
+        t = np.zeros(512, np.float32)
+        w = np.zeros(512, np.float32)
+        capi.check(self.lib.se8_read_block(self.h, capi.iptr(bc), capi.fptr(t), capi.fptr(w)), "se8_read_block")
+        return t, w
+
+    def dump_blocks(self, with_data=True):
+        n = C.c_int64()
+        capi.check(self.lib.se8_dump_blocks(self.h, None, None, None, None, 0, C.byref(n)), "se8_dump_blocks")
+        N = n.value
+        keys = np.zeros(max(N, 1), np.uint64)
+        ts = np.zeros(max(N, 1), np.int32)
+        tsdf = np.zeros((max(N, 1), 512), np.float32) if with_data else None
+        w = np.zeros((max(N, 1), 512), np.float32) if with_data else None
+        capi.check(self.lib.se8_dump_blocks(self.h, keys.ctypes.data_as(C.POINTER(C.c_uint64)), capi.iptr(ts),
+                                            capi.fptr(tsdf) if with_data else None,
+                                            capi.fptr(w) if with_data else None, N, C.byref(n)), "se8_dump_blocks")
+        if with_data:
+            return keys[:N], ts[:N], tsdf[:N], w[:N]
+        return keys[:N], ts[:N], None, None
+
+    def dump_nodes(self):
+        n = C.c_int64()
+        capi.check(self.lib.se8_dump_nodes(self.h, None, None, None, 0, C.byref(n)), "se8_dump_nodes")
+        N = n.value
+        keys = np.zeros(max(N, 1), np.uint64)
+        masks = np.zeros(max(N, 1), np.uint8)
+        ts = np.zeros(max(N, 1), np.int32)
+        capi.check(self.lib.se8_dump_nodes(self.h, keys.ctypes.data_as(C.POINTER(C.c_uint64)),
+                                           masks.ctypes.data_as(C.POINTER(C.c_uint8)), capi.iptr(ts), N, C.byref(n)),
+                   "se8_dump_nodes")
+        return keys[:N], masks[:N], ts[:N]
+
+    def getData(self, points_W):
+        p = np.ascontiguousarray(points_W, dtype=np.float32).reshape(-1, 3)
+        t = np.zeros(len(p), np.float32)
+        w = np.zeros(len(p), np.float32)
+        capi.check(self.lib.se8_get_data(self.h, capi.fptr(p), len(p), capi.fptr(t), capi.fptr(w)), "se8_get_data")
+        return t, w
+
+    def getFieldInterp(self, points_W):
+        p = np.ascontiguousarray(points_W, dtype=np.float32).reshape(-1, 3)
+        out = np.zeros(len(p), np.float32)
+        valid = np.zeros(len(p), np.uint8)
+        capi.check(self.lib.se8_field_interp(self.h, capi.fptr(p), len(p), capi.fptr(out),
+                                             valid.ctypes.data_as(C.POINTER(C.c_uint8))), "se8_field_interp")
+        return out, valid.astype(bool)
+
+    def getFieldGrad(self, points_W):
+        p = np.ascontiguousarray(points_W, dtype=np.float32).reshape(-1, 3)
+        out = np.zeros((len(p), 3), np.float32)
+        valid = np.zeros(len(p), np.uint8)
+        capi.check(self.lib.se8_field_grad(self.h, capi.fptr(p), len(p), capi.fptr(out),
+                                           valid.ctypes.data_as(C.POINTER(C.c_uint8))), "se8_field_grad")
+        return out, valid.astype(bool)
+
+
+class MapIntegrator:
+    """se::MapIntegrator<MapT> (map_integrator.hpp:66-81)."""
+
+    def __init__(self, map_: TSDFMap):
+        self.map = map_
+
+    def integrateDepth(self, timestamp, depth, sensor: PinholeCamera, T_WC, stream=None):
+        d = _dev_f32(depth, (sensor.height, sensor.width))
+        capi.check(self.map.lib.se8_integrate_depth(self.map.h, int(timestamp), d, C.byref(sensor.c()),
+                                                    capi.pose16(T_WC), _stream_handle(stream)),
+                   "se8_integrate_depth")
+
+
+def raycast_volume(map_: TSDFMap, sensor: PinholeCamera, T_WS, points_W, normals_W, scale=None, stream=None):
+    """se::raycaster::raycast_volume (raycaster.hpp:56-64) into device tensors [H, W, 3]."""
+    shape = (sensor.height, sensor.width, 3)
+    sp = C.c_void_p(scale.data_ptr()) if scale is not None else None
+    capi.check(map_.lib.se8_raycast(map_.h, C.byref(sensor.c()), capi.pose16(T_WS), _dev_f32(points_W, shape),
+                                    _dev_f32(normals_W, shape), sp, _stream_handle(stream)), "se8_raycast")
+
+
+class Tracker:
+    """se::Tracker<MapT, SensorT> (tracker.hpp:35-84)."""
+
+    def __init__(self, map_: TSDFMap, sensor: PinholeCamera, config: TrackerConfig | None = None):
+        self.map = map_
+        self.sensor = sensor
+        self.config = config or TrackerConfig()
+        self.lib = capi.load()
+        self.h = C.c_void_p()
+        capi.check(self.lib.se8_tracker_create(C.byref(sensor.c()), C.byref(self.config.c()), C.byref(self.h)),
+                   "se8_tracker_create")
+
+    def close(self):
+        if getattr(self, "h", None) and self.h.value:
+            self.lib.se8_tracker_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def track(self, depth, T_WS, points_W=None, normals_W=None, stream=None):
+        """Returns (ok, T_WS_new, sums32).  Without reference maps the map is raycast at T_WS."""
+        T = capi.pose16(T_WS)
+        ok = C.c_int()
+        sums = np.zeros(32, np.float32)
+        shape = (self.sensor.height, self.sensor.width)
+        if points_W is None:
+            capi.check(self.lib.se8_track_raycast(self.h, self.map.h, _dev_f32(depth, shape), T, C.byref(ok),
+                                                  _stream_handle(stream)), "se8_track_raycast")
+        else:
+            capi.check(self.lib.se8_track(self.h, _dev_f32(depth, shape), T, _dev_f32(points_W, shape + (3,)),
+                                          _dev_f32(normals_W, shape + (3,)), C.byref(ok), capi.fptr(sums),
+                                          _stream_handle(stream)), "se8_track")
+        return bool(ok.value), np.array(T[:], np.float32).reshape(4, 4), sums
+
+
+def stage_times(map_: TSDFMap | None = None, tracker: Tracker | None = None) -> dict:
+    out = capi.StageTimes()
+    lib = capi.load()
+    capi.check(lib.se8_stage_times_get(map_.h if map_ else None, tracker.h if tracker else None, C.byref(out)),
+               "se8_stage_times_get")
+    return {k: getattr(out, k) for k, _ in capi.StageTimes._fields_}
