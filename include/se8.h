@@ -150,6 +150,16 @@ int se8_field_grad(const se8_map* map, const float* points_W, int64_t n, float* 
 
 int se8_stage_times_get(const se8_map* map, const se8_tracker* tracker, se8_stage_times* out);
 
+/* -------- device staging for host-image callers (the reference API takes host se::Image<T>,
+ * include/se/image/image.hpp:18-111); the C++ facade stages through these. -------- */
+#define SE8_COPY_H2D 0
+#define SE8_COPY_D2H 1
+#define SE8_COPY_D2D 2
+int se8_device_alloc(size_t bytes, void** out_dev);
+int se8_device_free(void* dev);
+int se8_copy(void* dst, const void* src, size_t bytes, int direction, se8_stream stream);
+int se8_stream_synchronize(se8_stream stream);
+
 /* Human-readable text of the last error on this thread. */
 const char* se8_last_error(void);
 

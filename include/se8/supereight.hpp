@@ -1,0 +1,372 @@
+// supereight.hpp - header-only C++ facade with the reference's API for the single-resolution
+// TSDF path, over the C-ABI of libse8.so (include/se8.h).
+//
+// Reference call sites (app/src/main.cpp:128, 137-147, 154-160) compile against these names:
+//   se::TSDFMap<se::Res::Single>          include/se/map/map.hpp:82-95, 621-622
+//   se::MapIntegrator<MapT>::integrateDepth include/se/integrator/map_integrator.hpp:66-81
+//   se::raycaster::raycast_volume          include/se/map/raycaster.hpp:56-64
+//   se::Tracker<MapT, SensorT>::track      include/se/tracker/tracker.hpp:42-73
+//   se::PinholeCamera, se::Image<T>        include/se/sensor/pinhole_camera.hpp:74-78,
+//                                          include/se/image/image.hpp:18-111
+// Images live in host memory as in the reference; each call stages them through device buffers
+// owned by the object (or a per-thread scratch for raycast_volume).  Value types below provide the
+// subset of Eigen's interface the call sites use (x(), y(), z(), translation(), matrix data).
+// Device-side failures throw std::runtime_error carrying se8_last_error().
+#pragma once
+
+#include <cmath>
+#include <cstddef>
+#include <cstdint>
+#include <cstring>
+#include <optional>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../se8.h"
+
+namespace se {
+
+namespace detail {
+inline void check(int rc, const char* what)
+{
+    if (rc != SE8_OK)
+        throw std::runtime_error(std::string(what) + " failed (" + std::to_string(rc) + "): " + se8_last_error());
+}
+
+class DeviceBuffer {
+    public:
+    DeviceBuffer() = default;
+    DeviceBuffer(const DeviceBuffer&) = delete;
+    DeviceBuffer& operator=(const DeviceBuffer&) = delete;
+    ~DeviceBuffer()
+    {
+        if (ptr_)
+            se8_device_free(ptr_);
+    }
+    void* get(size_t bytes)
+    {
+        if (bytes > bytes_) {
+            if (ptr_)
+                check(se8_device_free(ptr_), "se8_device_free");
+            ptr_ = nullptr;
+            check(se8_device_alloc(bytes, &ptr_), "se8_device_alloc");
+            bytes_ = bytes;
+        }
+        return ptr_;
+    }
+
+    private:
+    void* ptr_ = nullptr;
+    size_t bytes_ = 0;
+};
+} // namespace detail
+
+struct Vector3f {
+    float v[3] = {0.f, 0.f, 0.f};
+    Vector3f() = default;
+    Vector3f(float x, float y, float z) : v{x, y, z} {}
+    static Vector3f Zero() { return Vector3f(); }
+    static Vector3f Constant(float c) { return Vector3f(c, c, c); }
+    float& x() { return v[0]; }
+    float& y() { return v[1]; }
+    float& z() { return v[2]; }
+    float x() const { return v[0]; }
+    float y() const { return v[1]; }
+    float z() const { return v[2]; }
+    float& operator()(int i) { return v[i]; }
+    float operator()(int i) const { return v[i]; }
+    float maxCoeff() const { return std::fmax(v[0], std::fmax(v[1], v[2])); }
+    float norm() const { return std::sqrt(v[0] * v[0] + (v[1] * v[1] + v[2] * v[2])); }
+    const float* data() const { return v; }
+    bool operator==(const Vector3f& o) const { return v[0] == o.v[0] && v[1] == o.v[1] && v[2] == o.v[2]; }
+};
+static_assert(sizeof(Vector3f) == 3 * sizeof(float), "Vector3f must be 3 packed floats");
+
+struct Vector3i {
+    int v[3] = {0, 0, 0};
+    Vector3i() = default;
+    Vector3i(int x, int y, int z) : v{x, y, z} {}
+    int x() const { return v[0]; }
+    int y() const { return v[1]; }
+    int z() const { return v[2]; }
+};
+
+// Rigid transform stored as a row-major 4x4 matrix.
+struct Isometry3f {
+    float m[16] = {1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1};
+    static Isometry3f Identity() { return Isometry3f(); }
+    float& operator()(int r, int c) { return m[4 * r + c]; }
+    float operator()(int r, int c) const { return m[4 * r + c]; }
+    Vector3f translation() const { return Vector3f(m[3], m[7], m[11]); }
+    float* data() { return m; }
+    const float* data() const { return m; }
+};
+
+template<typename T>
+class Image {
+    public:
+    Image(int width, int height, const T& init = T()) : w_(width), h_(height), data_((size_t) width * height, init) {}
+    int width() const { return w_; }
+    int height() const { return h_; }
+    size_t size() const { return data_.size(); }
+    T& operator[](size_t i) { return data_[i]; }
+    const T& operator[](size_t i) const { return data_[i]; }
+    T& operator()(int x, int y) { return data_[x + (size_t) y * w_]; }
+    const T& operator()(int x, int y) const { return data_[x + (size_t) y * w_]; }
+    T* data() { return data_.data(); }
+    const T* data() const { return data_.data(); }
+
+    private:
+    int w_, h_;
+    std::vector<T> data_;
+};
+
+enum class Res { Single, Multi };
+
+// se::FieldData<Field::TSDF>::Config plus the storage precision and pool capacity
+struct TSDFDataConfig {
+    float truncation_boundary_factor = 8.f;
+    float max_weight = 100.f;
+    int storage = SE8_STORAGE_F32;
+    int64_t max_blocks = 0;
+};
+
+class PinholeCamera {
+    public:
+    struct Config {
+        int width = 0;
+        int height = 0;
+        float near_plane = 0.01f;
+        float far_plane = 10.f;
+        float fx = std::nanf("");
+        float fy = std::nanf("");
+        float cx = std::nanf("");
+        float cy = std::nanf("");
+    };
+    // PinholeCamera(Config, downsampling) (pinhole_camera.cpp:29-39, 88-92)
+    explicit PinholeCamera(const Config& c, float dsf = 1.f)
+    {
+        p_.width = (int) ((float) c.width / dsf);
+        p_.height = (int) ((float) c.height / dsf);
+        p_.fx = c.fx / dsf;
+        p_.fy = c.fy / dsf;
+        p_.cx = dsf == 1.f ? c.cx : (c.cx + 0.5f) / dsf - 0.5f;
+        p_.cy = dsf == 1.f ? c.cy : (c.cy + 0.5f) / dsf - 0.5f;
+        p_.near_plane = c.near_plane;
+        p_.far_plane = c.far_plane;
+    }
+    int imageWidth() const { return p_.width; }
+    int imageHeight() const { return p_.height; }
+    const se8_pinhole& c() const { return p_; }
+
+    private:
+    se8_pinhole p_{};
+};
+
+template<Res ResT = Res::Single>
+class TSDFMap {
+    static_assert(ResT == Res::Single, "only the single-resolution TSDF map is provided");
+
+    public:
+    struct Config {
+        Vector3f dim = Vector3f::Constant(10.f);
+        float res = 0.1f;
+        Vector3f t_MW = Vector3f::Constant(5.f); // T_MW is a translation (dim/2 by default)
+    };
+
+    TSDFMap(const Vector3f& dim, float res, const TSDFDataConfig& data_cfg = TSDFDataConfig())
+        : TSDFMap(Config{dim, res, Vector3f(dim.x() / 2, dim.y() / 2, dim.z() / 2)}, data_cfg)
+    {
+    }
+
+    TSDFMap(const Config& cfg, const TSDFDataConfig& data_cfg = TSDFDataConfig()) : res_(cfg.res)
+    {
+        se8_map_config m{{cfg.dim.x(), cfg.dim.y(), cfg.dim.z()}, cfg.res, {cfg.t_MW.x(), cfg.t_MW.y(), cfg.t_MW.z()}};
+        se8_tsdf_config t{data_cfg.truncation_boundary_factor, data_cfg.max_weight, data_cfg.storage,
+                          data_cfg.max_blocks};
+        detail::check(se8_map_create(&m, &t, &h_), "se8_map_create");
+        dim_ = cfg.dim;
+    }
+    TSDFMap(const TSDFMap&) = delete;
+    TSDFMap& operator=(const TSDFMap&) = delete;
+    ~TSDFMap() { se8_map_destroy(h_); }
+
+    float getRes() const { return res_; }
+    Vector3f getDim() const
+    {
+        const se8_map_info i = info();
+        return Vector3f::Constant(i.dim);
+    }
+    int getOctreeSize() const { return info().octree_size; }
+    se8_map_info info() const
+    {
+        se8_map_info i{};
+        detail::check(se8_map_get_info(h_, &i), "se8_map_get_info");
+        return i;
+    }
+    // getData (map_impl.hpp:68-86): {tsdf, weight}
+    std::pair<float, float> getData(const Vector3f& p) const
+    {
+        float t = 1.f, w = 0.f;
+        detail::check(se8_get_data(h_, p.data(), 1, &t, &w), "se8_get_data");
+        return {t, w};
+    }
+    // getFieldInterp (map_impl.hpp:154-172)
+    std::optional<float> getFieldInterp(const Vector3f& p) const
+    {
+        float v = 0.f;
+        uint8_t ok = 0;
+        detail::check(se8_field_interp(h_, p.data(), 1, &v, &ok), "se8_field_interp");
+        return ok ? std::optional<float>(v) : std::nullopt;
+    }
+    // getFieldGrad (map_impl.hpp:239-268)
+    std::optional<Vector3f> getFieldGrad(const Vector3f& p) const
+    {
+        Vector3f g;
+        uint8_t ok = 0;
+        detail::check(se8_field_grad(h_, p.data(), 1, g.v, &ok), "se8_field_grad");
+        return ok ? std::optional<Vector3f>(g) : std::nullopt;
+    }
+    se8_map* handle() const { return h_; }
+
+    private:
+    se8_map* h_ = nullptr;
+    float res_;
+    Vector3f dim_;
+};
+
+template<typename SensorT>
+struct Measurement {
+    const Image<float>& image;
+    const SensorT& sensor;
+    const Isometry3f T_WC;
+};
+
+// deduction guides as in measurement.hpp:26-27 and :52-58
+template<typename SensorT>
+Measurement(const Image<float>&, const SensorT&, const Isometry3f&) -> Measurement<SensorT>;
+
+template<typename SensorT>
+struct Measurements {
+    Measurement<SensorT> depth;
+};
+
+template<typename SensorT>
+Measurements(Measurement<SensorT>) -> Measurements<SensorT>;
+
+template<typename MapT>
+class MapIntegrator {
+    public:
+    explicit MapIntegrator(MapT& map) : map_(map) {}
+
+    // integrateDepth(timestamp, measurements) (map_integrator.hpp:78-81)
+    template<typename SensorT>
+    void integrateDepth(int timestamp, const Measurements<SensorT>& m, se8_stream stream = nullptr)
+    {
+        const Image<float>& d = m.depth.image;
+        const size_t bytes = sizeof(float) * d.size();
+        void* dev = depth_.get(bytes);
+        detail::check(se8_copy(dev, d.data(), bytes, SE8_COPY_H2D, stream), "se8_copy");
+        detail::check(se8_integrate_depth(map_.handle(), timestamp, (const float*) dev, &m.depth.sensor.c(),
+                                          m.depth.T_WC.data(), stream),
+                      "se8_integrate_depth");
+        detail::check(se8_stream_synchronize(stream), "se8_stream_synchronize");
+    }
+
+    private:
+    MapT& map_;
+    detail::DeviceBuffer depth_;
+};
+
+namespace raycaster {
+// raycast_volume(map, sensor, T_WS, points, normals, scale) (raycaster.hpp:56-64)
+template<typename MapT, typename SensorT>
+void raycast_volume(const MapT& map, const SensorT& sensor, const Isometry3f& T_WS, Image<Vector3f>& points_W,
+                    Image<Vector3f>& normals_W, Image<int8_t>& scale, se8_stream stream = nullptr)
+{
+    thread_local detail::DeviceBuffer pts, nrm, scl;
+    const size_t n = points_W.size();
+    void* dp = pts.get(sizeof(float) * 3 * n);
+    void* dn = nrm.get(sizeof(float) * 3 * n);
+    void* ds = scl.get(n);
+    detail::check(se8_copy(ds, scale.data(), n, SE8_COPY_H2D, stream), "se8_copy");
+    detail::check(se8_raycast(map.handle(), &sensor.c(), T_WS.data(), (float*) dp, (float*) dn, (int8_t*) ds, stream),
+                  "se8_raycast");
+    detail::check(se8_copy(points_W.data(), dp, sizeof(float) * 3 * n, SE8_COPY_D2H, stream), "se8_copy");
+    detail::check(se8_copy(normals_W.data(), dn, sizeof(float) * 3 * n, SE8_COPY_D2H, stream), "se8_copy");
+    detail::check(se8_copy(scale.data(), ds, n, SE8_COPY_D2H, stream), "se8_copy");
+    detail::check(se8_stream_synchronize(stream), "se8_stream_synchronize");
+}
+} // namespace raycaster
+
+// TrackerConfig (tracker.hpp:22-33) + the compensated-reduction switch
+struct TrackerConfig {
+    std::vector<int> iterations{10, 5, 4};
+    float dist_threshold = 0.1f;
+    float normal_threshold = 0.8f;
+    float track_threshold = 0.15f;
+    float icp_threshold = 0.00001f;
+    bool compensated = false;
+};
+
+template<typename MapT, typename SensorT>
+class Tracker {
+    public:
+    Tracker(MapT& map, const SensorT& sensor, const TrackerConfig config = TrackerConfig())
+        : map_(map), sensor_(sensor)
+    {
+        if (config.iterations.size() != 3)
+            throw std::invalid_argument("se::Tracker: exactly 3 pyramid levels are supported");
+        se8_tracker_config c{{config.iterations[0], config.iterations[1], config.iterations[2]},
+                             config.dist_threshold, config.normal_threshold, config.track_threshold,
+                             config.icp_threshold, config.compensated ? 1 : 0};
+        detail::check(se8_tracker_create(&sensor.c(), &c, &h_), "se8_tracker_create");
+    }
+    Tracker(const Tracker&) = delete;
+    Tracker& operator=(const Tracker&) = delete;
+    ~Tracker() { se8_tracker_destroy(h_); }
+
+    // track(depth, T_WS): raycasts the map at T_WS first (tracker_impl.hpp:17-26)
+    bool track(const Image<float>& depth, Isometry3f& T_WS, se8_stream stream = nullptr)
+    {
+        void* dd = stage_depth(depth, stream);
+        int ok = 0;
+        detail::check(se8_track_raycast(h_, map_.handle(), (const float*) dd, T_WS.data(), &ok, stream),
+                      "se8_track_raycast");
+        return ok != 0;
+    }
+
+    // track(depth, T_WS, points_W, normals_W) (tracker_impl.hpp:30-116)
+    bool track(const Image<float>& depth, Isometry3f& T_WS, Image<Vector3f>& points_W, Image<Vector3f>& normals_W,
+               se8_stream stream = nullptr)
+    {
+        void* dd = stage_depth(depth, stream);
+        const size_t n = points_W.size();
+        void* dp = pts_.get(sizeof(float) * 3 * n);
+        void* dn = nrm_.get(sizeof(float) * 3 * n);
+        detail::check(se8_copy(dp, points_W.data(), sizeof(float) * 3 * n, SE8_COPY_H2D, stream), "se8_copy");
+        detail::check(se8_copy(dn, normals_W.data(), sizeof(float) * 3 * n, SE8_COPY_H2D, stream), "se8_copy");
+        int ok = 0;
+        detail::check(se8_track(h_, (const float*) dd, T_WS.data(), (const float*) dp, (const float*) dn, &ok,
+                                nullptr, stream),
+                      "se8_track");
+        return ok != 0;
+    }
+
+    private:
+    void* stage_depth(const Image<float>& depth, se8_stream stream)
+    {
+        const size_t bytes = sizeof(float) * depth.size();
+        void* dd = depth_.get(bytes);
+        detail::check(se8_copy(dd, depth.data(), bytes, SE8_COPY_H2D, stream), "se8_copy");
+        return dd;
+    }
+
+    MapT& map_;
+    const SensorT& sensor_;
+    se8_tracker* h_ = nullptr;
+    detail::DeviceBuffer depth_, pts_, nrm_;
+};
+
+} // namespace se

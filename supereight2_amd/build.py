@@ -70,5 +70,21 @@ def build(verbose: bool = False) -> str:
     return LIB_PATH
 
 
+def build_facade_app() -> str:
+    """Compiles tests/cpp/facade_app.cpp (a C++ caller of include/se8/supereight.hpp) with g++."""
+    root = os.path.dirname(HERE)
+    out_dir = os.path.join(root, "tests", "cpp", "_build")
+    os.makedirs(out_dir, exist_ok=True)
+    out = os.path.join(out_dir, "facade_app")
+    cmd = ["g++", "-std=c++17", "-O2", "-Wall", "-Wextra", "-I", os.path.join(root, "include"),
+           os.path.join(root, "tests", "cpp", "facade_app.cpp"), "-L", HERE, "-lse8",
+           "-Wl,-rpath," + HERE, "-Wl,-rpath,$ORIGIN/../../../supereight2_amd", "-o", out]
+    res = subprocess.run(cmd, capture_output=True, text=True)
+    if res.returncode != 0:
+        raise RuntimeError(f"facade_app build failed:\n{res.stderr}")
+    return out
+
+
 if __name__ == "__main__":
     print(build(verbose=True))
+    print(build_facade_app())

@@ -115,6 +115,10 @@ SIGNATURES = {
     "se8_field_interp": (C.c_int, [_P, _F, C.c_int64, _F, C.POINTER(C.c_uint8)]),
     "se8_field_grad": (C.c_int, [_P, _F, C.c_int64, _F, C.POINTER(C.c_uint8)]),
     "se8_stage_times_get": (C.c_int, [_P, _P, C.POINTER(StageTimes)]),
+    "se8_device_alloc": (C.c_int, [C.c_size_t, C.POINTER(_P)]),
+    "se8_device_free": (C.c_int, [_P]),
+    "se8_copy": (C.c_int, [_P, _P, C.c_size_t, C.c_int, _P]),
+    "se8_stream_synchronize": (C.c_int, [_P]),
     "se8_last_error": (C.c_char_p, []),
 }
 

@@ -1190,3 +1190,41 @@ extern "C" int se8_stage_times_get(const se8_map* m, const se8_tracker* tr, se8_
     }
     return SE8_OK;
 }
+
+extern "C" int se8_device_alloc(size_t bytes, void** out_dev)
+{
+    SE8_REQUIRE(out_dev, "se8_device_alloc: null argument");
+    *out_dev = nullptr;
+    if (hipMalloc(out_dev, bytes ? bytes : 16) != hipSuccess) {
+        (void) hipGetLastError();
+        *out_dev = nullptr;
+        set_error("se8_device_alloc: hipMalloc of " + std::to_string(bytes) + " bytes failed");
+        return SE8_ERR_OOM;
+    }
+    return SE8_OK;
+}
+
+extern "C" int se8_device_free(void* dev)
+{
+    if (dev)
+        SE8_HIP_CHECK(hipFree(dev));
+    return SE8_OK;
+}
+
+extern "C" int se8_copy(void* dst, const void* src, size_t bytes, int direction, se8_stream stream)
+{
+    SE8_REQUIRE((dst && src) || bytes == 0, "se8_copy: null argument");
+    SE8_REQUIRE(direction >= SE8_COPY_H2D && direction <= SE8_COPY_D2D, "se8_copy: bad direction");
+    const hipMemcpyKind kind = direction == SE8_COPY_H2D ? hipMemcpyHostToDevice
+        : direction == SE8_COPY_D2H                       ? hipMemcpyDeviceToHost
+                                                          : hipMemcpyDeviceToDevice;
+    if (bytes)
+        SE8_HIP_CHECK(hipMemcpyAsync(dst, src, bytes, kind, (hipStream_t) stream));
+    return SE8_OK;
+}
+
+extern "C" int se8_stream_synchronize(se8_stream stream)
+{
+    SE8_HIP_CHECK(hipStreamSynchronize((hipStream_t) stream));
+    return SE8_OK;
+}
