@@ -50,7 +50,9 @@ def ate(est, gt):
 
 
 def run_pipeline(depth_dev, gt, n_frames, warmup, storage, compensated, torch, se, sensor):
-    """Runs the per-frame loop; returns (fps, poses, per-frame stage times of the timed frames)."""
+    """Runs the per-frame loop.  Per-stage device times come from the HIP events the library
+    records on the stream around each stage, accumulated by the library over the timed frames
+    (no host queries inside the loop).  Returns a dict of results."""
     H, W = sensor.height, sensor.width
     gmap = se.TSDFMap(10.24, 0.01, storage=storage)
     integ = se.MapIntegrator(gmap)
@@ -59,11 +61,11 @@ def run_pipeline(depth_dev, gt, n_frames, warmup, storage, compensated, torch, s
     nrm = [torch.zeros((H, W, 3), dtype=torch.float32, device="cuda") for _ in range(2)]
     T = gt[0].astype(np.float32)
     poses = []
-    stages = []
     t0 = None
     for k in range(n_frames):
         if k == warmup:
             torch.cuda.synchronize()
+            se.stage_times_reset(gmap, tracker)
             t0 = time.perf_counter()
         cur, prev = k % 2, (k + 1) % 2
         if k > 0:
@@ -71,17 +73,31 @@ def run_pipeline(depth_dev, gt, n_frames, warmup, storage, compensated, torch, s
         integ.integrateDepth(k, depth_dev[k], sensor, T)
         se.raycast_volume(gmap, sensor, T, pts[cur], nrm[cur])
         poses.append(T.copy())
-        if k >= warmup:
-            st = se.stage_times(gmap, tracker)
-            stages.append(st)
     torch.cuda.synchronize()
     t1 = time.perf_counter()
+    st = se.stage_times(gmap, tracker)
     timed = n_frames - warmup
-    fps = timed / (t1 - t0) if timed > 0 else 0.0
     info = gmap.info()
     tracker.close()
     gmap.close()
-    return fps, poses, stages, (t1 - t0), int(info.num_blocks)
+    ni, nr, nt = max(st["n_integrations"], 1), max(st["n_raycasts"], 1), max(st["n_tracks"], 1)
+    return {
+        "fps": timed / (t1 - t0) if timed > 0 else 0.0,
+        "seconds": t1 - t0,
+        "poses": poses,
+        "blocks": int(info.num_blocks),
+        "stage_ms": {
+            "fetch_frustum": st["sum_fetch_frustum"] / ni,
+            "create_list": st["sum_create_list"] / ni,
+            "allocate_list": st["sum_allocate_list"] / ni,
+            "update": st["sum_update"] / ni,
+            "raycast": st["sum_raycast"] / nr,
+            "tracking": st["sum_tracking"] / nt,
+        },
+        "counts": {"integrations": st["n_integrations"], "raycasts": st["n_raycasts"], "tracks": st["n_tracks"]},
+        "update_blocks_per_frame": st["sum_update_blocks"] / ni,
+        "candidates_per_frame": st["sum_candidates"] / ni,
+    }
 
 
 def cpu_baseline(depth_host, gt, n_frames, sensor):
@@ -142,27 +158,23 @@ def main():
     torch.cuda.synchronize()
 
     print("fp32 pipeline", file=sys.stderr, flush=True)
-    fps32, poses32, stages32, dt32, nblk32 = run_pipeline(depth_dev, gt, n_frames, args.warmup, "f32", False, torch,
-                                                          se, sensor)
-    ate32 = ate(poses32, gt)
-    res = {}
+    r32 = run_pipeline(depth_dev, gt, n_frames, args.warmup, "f32", False, torch, se, sensor)
+    ate32 = ate(r32["poses"], gt)
+    r16 = None
     if not args.no_fp16:
         print("fp16 pipeline", file=sys.stderr, flush=True)
-        fps16, poses16, stages16, dt16, nblk16 = run_pipeline(depth_dev, gt, n_frames, args.warmup, "f16", True,
-                                                              torch, se, sensor)
-        res["fp16"] = dict(fps=fps16, ate_m=ate(poses16, gt), blocks=nblk16,
-                           update_ms=float(np.mean([s["update"] for s in stages16])) if stages16 else None,
-                           raycast_ms=float(np.mean([s["raycast"] for s in stages16])) if stages16 else None)
+        r16 = run_pipeline(depth_dev, gt, n_frames, args.warmup, "f16", True, torch, se, sensor)
 
-    # roofline of the update kernel (the dominant HBM stream): algorithmic bytes per launch
-    # = 2 * s_v * 512 * N_blk + 4 * W * H with s_v = 8 B/voxel (fp32 tsdf + fp32 weight)
-    def stage_mean(stages, key):
-        return float(np.mean([s[key] for s in stages])) if stages else 0.0
+    # roofline of the update kernel (the HBM stream of the path): algorithmic bytes per launch
+    # = 2 * s_v * 512 * N_blk + 4 * W * H, s_v = 8 B/voxel (fp32 tsdf + fp32 weight), N_blk = the
+    # frame's update-list length; divided by the kernel's mean HIP-event duration
+    def update_roofline(r, s_v):
+        upd_ms = r["stage_ms"]["update"]
+        nbytes = 2 * s_v * 512 * r["update_blocks_per_frame"] + 4 * cam.width * cam.height
+        gbs = nbytes / (upd_ms * 1e-3) / 1e9 if upd_ms > 0 else 0.0
+        return nbytes, gbs
 
-    upd_ms = stage_mean(stages32, "update")
-    nblk = float(np.mean([s["last_frustum_blocks"] + s["last_new_blocks"] for s in stages32])) if stages32 else 0.0
-    upd_bytes = 2 * 8 * 512 * nblk + 4 * cam.width * cam.height
-    upd_gbs = upd_bytes / (upd_ms * 1e-3) / 1e9 if upd_ms > 0 else 0.0
+    upd_bytes, upd_gbs = update_roofline(r32, 8)
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -173,15 +185,19 @@ def main():
                "sample": f"frames 1..{args.cpu_frames - 1} of the same sequence (frame 0 untimed), "
                          f"track+integrate+raycast, OpenMP on {cores} threads, {cpu_model()}"}
 
-    ms_per_step = 1000.0 * dt32 / max(args.steps, 1)
+    fp16 = None
+    if r16 is not None:
+        b16, g16 = update_roofline(r16, 4)
+        fp16 = {"fps": r16["fps"], "ate_m": ate(r16["poses"], gt), "blocks": r16["blocks"],
+                "stage_ms": r16["stage_ms"], "update_gbs": g16}
     out = {
         "metric": "frames/s integrate+raycast+ICP @640x480,1cm; FP16-TSDF vs FP32 at stated RMSE",
-        "value": fps32,
+        "value": r32["fps"],
         "unit": "frames/s",
         "n_gpus": 1,
         "steps": n_frames - args.warmup,
         "warmup": args.warmup,
-        "ms_per_step": ms_per_step,
+        "ms_per_step": 1000.0 * r32["seconds"] / max(n_frames - args.warmup, 1),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
@@ -189,14 +205,13 @@ def main():
         "data": "synthetic: seeded room + 3 spheres, analytic depth with Kinect-like noise, 1% dropout",
         "config": {"workload": "fp32 TSDF + ICP tracking, 300-frame spline sequence, 640x480, 1 cm voxels, "
                                "10.24 m map, mu 8 voxels, max weight 100, ICP 10/5/4",
-                   "frames": n_frames, "ate_m": ate32, "blocks": nblk32,
-                   "stage_ms": {k: stage_mean(stages32, k) for k in
-                                ("fetch_frustum", "create_list", "allocate_list", "update", "raycast", "tracking")},
-                   "update_blocks_per_frame": nblk},
-        "fp16_tsdf": res.get("fp16"),
-        "fp16_over_fp32": (res["fp16"]["fps"] / fps32) if "fp16" in res and fps32 > 0 else None,
+                   "frames": n_frames, "ate_m": ate32, "blocks": r32["blocks"], "stage_ms": r32["stage_ms"],
+                   "update_blocks_per_frame": r32["update_blocks_per_frame"],
+                   "candidates_per_frame": r32["candidates_per_frame"], "event_counts": r32["counts"]},
+        "fp16_tsdf": fp16,
+        "fp16_over_fp32": (r16["fps"] / r32["fps"]) if r16 is not None and r32["fps"] > 0 else None,
         "roofline": {"bound": "hbm", "kernel": "k_update", "achieved": upd_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": upd_gbs / HBM_PEAK_GBS, "traffic": None},
+                     "frac": upd_gbs / HBM_PEAK_GBS, "traffic": None, "bytes_per_launch": upd_bytes},
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))

@@ -87,11 +87,18 @@ typedef struct {
     int aabb_min[3], aabb_max[3]; /* voxel AABB of allocated blocks (octree_impl.hpp:267-273) */
 } se8_map_info;
 
-/* Per-stage device times of the last call of each kind, in milliseconds, using the reference's
- * stage names (map_integrator_impl.hpp:63-75, raycast_carver_impl.hpp:58-126, main.cpp:101-217). */
+/* Per-stage device times (HIP events on the caller's stream), in milliseconds, using the
+ * reference's stage names (map_integrator_impl.hpp:63-75, raycast_carver_impl.hpp:58-126,
+ * main.cpp:101-217): the last completed call of each kind, and running sums over completed calls
+ * since creation or se8_stage_times_reset.  A call's events are harvested when the next call of
+ * the same kind starts (if they have completed by then) or by se8_stage_times_get. */
 typedef struct {
     float fetch_frustum, create_list, allocate_list, update, raycast, tracking;
     int64_t last_frustum_blocks, last_new_blocks, last_candidates;
+    double sum_fetch_frustum, sum_create_list, sum_allocate_list, sum_update, sum_raycast, sum_tracking;
+    int64_t n_integrations, n_raycasts, n_tracks;
+    int64_t sum_update_blocks; /* sum over integrations of (frustum + new) blocks updated */
+    int64_t sum_candidates;
 } se8_stage_times;
 
 /* -------- map lifetime: se::Map ctor/dtor (include/se/map/impl/map_impl.hpp:37-55) -------- */
@@ -125,6 +132,24 @@ int se8_track(se8_tracker* tracker, const float* depth_dev, float T_WS[16], cons
 int se8_track_raycast(se8_tracker* tracker, const se8_map* map, const float* depth_dev, float T_WS[16], int* ok,
                       se8_stream stream);
 
+/* -------- differentiation (growth pattern; no counterpart in the reference) --------
+ * Batched point-to-plane cost and its adjoint for n candidate poses (poses16: n row-major 4x4,
+ * host).  cost_k = 0.5 * sum e_i^2 over the inliers of trackKernel (icp_impl.hpp:16-102) against
+ * the reference maps raycast at T_ref (full resolution, level 0); grad6_k = d cost / d xi for the
+ * left twist T <- exp(xi) T, xi = [v; w] (math_util_impl.hpp:148-174), at xi = 0, which equals
+ * -sum e_i J_i = -b of the reference's reduction (icp.cpp:45-204); jtj21_k (may be NULL) is J^T J
+ * in the same upper-triangular layout; inliers_k (may be NULL) the inlier count. */
+int se8_icp_cost_grad_batched(se8_tracker* tracker, const float* depth_dev, const float* ref_points_dev,
+                              const float* ref_normals_dev, const float T_ref[16], int n_poses, const float* poses16,
+                              float* cost, float* grad6, float* jtj21, int* inliers, se8_stream stream);
+/* Batched raycast depth and its adjoint for n poses (poses16 host, row-major 4x4): per pixel the
+ * z-depth of the root of the trilinear TSDF along the ray, refined inside the bracket of the
+ * reference march (raycaster_impl.hpp:456-533), and d depth / d xi (6 per pixel) by the
+ * implicit-function theorem.  depth_dev n*H*W, grad_dev n*H*W*6, valid_dev n*H*W (0 where the
+ * ray misses or the field is not interpolable at the root). */
+int se8_raycast_depth_grad_batched(const se8_map* map, const se8_pinhole* sensor, int n_poses, const float* poses16,
+                                   float* depth_dev, float* grad_dev, uint8_t* valid_dev, se8_stream stream);
+
 /* -------- map queries (host arrays) -------- */
 /* se::allocator::blocks(voxel_coords, octree, root, only_allocated)
  * (include/se/map/octree/impl/allocator_impl.hpp:40-116): voxel_coords is n*3 ints; the block
@@ -149,6 +174,7 @@ int se8_field_interp(const se8_map* map, const float* points_W, int64_t n, float
 int se8_field_grad(const se8_map* map, const float* points_W, int64_t n, float* out3, uint8_t* valid);
 
 int se8_stage_times_get(const se8_map* map, const se8_tracker* tracker, se8_stage_times* out);
+int se8_stage_times_reset(se8_map* map, se8_tracker* tracker);
 
 /* -------- device staging for host-image callers (the reference API takes host se::Image<T>,
  * include/se/image/image.hpp:18-111); the C++ facade stages through these. -------- */

@@ -133,6 +133,7 @@ def lib(precision: str = "f32") -> C.CDLL:
     L.or_se3_exp.argtypes = [R, R]
     L.or_first_block.argtypes = [P, C.POINTER(C.c_float), C.POINTER(C.c_float), C.c_float, C.c_float,
                                  C.POINTER(C.c_float)]
+    L.or_raycast_depth_grad.argtypes = [P, C.POINTER(Pinhole), R, C.c_int, R, R, C.POINTER(C.c_uint8)]
     L.or_real_size.restype = C.c_int
     L.or_set_threads.argtypes = [C.c_int]
     _libs[precision] = L
@@ -335,6 +336,17 @@ class OracleMap:
         self.L.or_first_block(self.h, oo.ctypes.data_as(fp), dd.ctypes.data_as(fp), float(near_t), float(far_t),
                               out.ctypes.data_as(fp))
         return float(out[0]), float(out[1])
+
+    def raycast_depth_grad(self, pin, T_WS, iters=40):
+        """Depth of the refined field root per pixel and d depth / d xi (left twist [v; w])."""
+        T = np.ascontiguousarray(np.asarray(T_WS).reshape(16), dtype=self.dtype)
+        depth = np.zeros((pin.height, pin.width), self.dtype)
+        grad = np.zeros((pin.height, pin.width, 6), self.dtype)
+        valid = np.zeros((pin.height, pin.width), np.uint8)
+        self.L.or_raycast_depth_grad(self.h, C.byref(pin), _rp(T, self.precision), int(iters),
+                                     _rp(depth, self.precision), _rp(grad, self.precision),
+                                     valid.ctypes.data_as(C.POINTER(C.c_uint8)))
+        return depth, grad, valid.astype(bool)
 
     def T_CV(self, T_WC):
         T = np.ascontiguousarray(np.asarray(T_WC).reshape(16), dtype=self.dtype)

@@ -1371,7 +1371,18 @@ static void esvo_first_block(const or_map* m, const float o_W[3], const float d_
 /* ------------------------------------------------------------------------- */
 /* TSDF raycast: raycaster_impl.hpp:456-627                                  */
 /* ------------------------------------------------------------------------- */
+static int raycast_ray_bracket(const or_map* m, const real o[3], const real d[3], real t_near, real t_far,
+                               real hit[3], real* t_prev, real* t_hit);
+
 static int raycast_ray(const or_map* m, const real o[3], const real d[3], real t_near, real t_far, real hit[3])
+{
+    return raycast_ray_bracket(m, o, d, t_near, t_far, hit, NULL, NULL);
+}
+
+/* The march; on a hit also returns the bracket [t_prev, t_hit] of the last two samples, whose
+ * field values straddle zero. */
+static int raycast_ray_bracket(const or_map* m, const real o[3], const real d[3], real t_near, real t_far,
+                               real hit[3], real* t_prev, real* t_hit)
 {
     or_ray_span span;
     const float of[3] = {(float) o[0], (float) o[1], (float) o[2]};
@@ -1422,6 +1433,10 @@ static int raycast_ray(const or_map* m, const real o[3], const real d[3], real t
         f_t = f_tt;
     }
     if (f_tt < (real) 0) {
+        if (t_prev)
+            *t_prev = t - stepsize;
+        if (t_hit)
+            *t_hit = t;
         t = t - stepsize * (f_tt - (real) 0) / (f_tt - f_t);
         for (int i = 0; i < 3; i++)
             hit[i] = o[i] + d[i] * t;
@@ -1896,4 +1911,151 @@ OR_API void or_first_block(void* vm, const float o_W[3], const float d_W[3], flo
     esvo_first_block((const or_map*) vm, o_W, d_W, near_t, far_t, &s);
     out[0] = s.t_min_ray;
     out[1] = s.t_max_ray;
+}
+
+/* ------------------------------------------------------------------------- */
+/* Differentiable raycast depth (growth pattern, no counterpart in the         */
+/* reference).  The reference march (raycaster_impl.hpp:456-533) brackets the  */
+/* zero crossing; here the root of the trilinear field along the ray is       */
+/* refined inside that bracket (Illinois regula falsi), so depth is a smooth   */
+/* function of the pose within a voxel cell, and                             */
+/*   dt/dxi = -[g, x cross g] / (g . d)   (implicit-function theorem)          */
+/* for the left twist T <- exp(xi) T, xi = [v; w], with g the gradient of the  */
+/* trilinear field at the root x = o + t d.  depth = t * dn_z.                */
+/* ------------------------------------------------------------------------- */
+static int interp_and_grad(const or_map* m, const real pW[3], real* f, real g[3])
+{
+    real vf[3];
+    if (!map_point_to_voxel_f(m, pW, vf))
+        return 0;
+    real bf[3], t[3], tc[3];
+    int base[3];
+    for (int i = 0; i < 3; i++) {
+        bf[i] = vf[i] - (real) 0.5;
+        base[i] = (int) bf[i];
+        if (base[i] + 1 >= m->size || base[i] < 0)
+            return 0;
+    }
+    real d[8];
+    for (int k = 0; k < 8; k++) {
+        const int c[3] = {base[0] + (k & 1), base[1] + ((k >> 1) & 1), base[2] + ((k >> 2) & 1)};
+        const or_octant* b = fetch_block(m, c);
+        if (!b)
+            return 0;
+        const int i = block_voxel_idx(b, c);
+        if (b->weight[i] == (real) 0)
+            return 0;
+        d[k] = b->tsdf[i];
+    }
+    for (int i = 0; i < 3; i++) {
+        t[i] = bf[i] - floor(bf[i]);
+        tc[i] = (real) 1 - t[i];
+    }
+    *f = ((d[0] * tc[0] + d[1] * t[0]) * tc[1] + (d[2] * tc[0] + d[3] * t[0]) * t[1]) * tc[2]
+        + ((d[4] * tc[0] + d[5] * t[0]) * tc[1] + (d[6] * tc[0] + d[7] * t[0]) * t[1]) * t[2];
+    if (g) {
+        const real gx = ((d[1] - d[0]) * tc[1] + (d[3] - d[2]) * t[1]) * tc[2]
+            + ((d[5] - d[4]) * tc[1] + (d[7] - d[6]) * t[1]) * t[2];
+        const real gy = ((d[2] - d[0]) * tc[0] + (d[3] - d[1]) * t[0]) * tc[2]
+            + ((d[6] - d[4]) * tc[0] + (d[7] - d[5]) * t[0]) * t[2];
+        const real gz = ((d[4] - d[0]) * tc[0] + (d[5] - d[1]) * t[0]) * tc[1]
+            + ((d[6] - d[2]) * tc[0] + (d[7] - d[3]) * t[0]) * t[1];
+        g[0] = gx / m->res;
+        g[1] = gy / m->res;
+        g[2] = gz / m->res;
+    }
+    return 1;
+}
+
+/* Refined root in [ta, tb] of f(o + t d) (f(ta) >= 0 > f(tb)); returns 0 if the field is not
+ * interpolable somewhere on the way. */
+static int refine_root(const or_map* m, const real o[3], const real d[3], real ta, real tb, int iters, real* t_out)
+{
+    real p[3], fa, fb;
+    for (int i = 0; i < 3; i++)
+        p[i] = o[i] + d[i] * ta;
+    if (!interp_and_grad(m, p, &fa, NULL))
+        return 0;
+    for (int i = 0; i < 3; i++)
+        p[i] = o[i] + d[i] * tb;
+    if (!interp_and_grad(m, p, &fb, NULL))
+        return 0;
+    if (!(fa >= 0 && fb < 0))
+        return 0;
+    int side = 0;
+    real t = tb;
+    for (int it = 0; it < iters; it++) {
+        t = (ta * fb - tb * fa) / (fb - fa);
+        real f;
+        for (int i = 0; i < 3; i++)
+            p[i] = o[i] + d[i] * t;
+        if (!interp_and_grad(m, p, &f, NULL))
+            return 0;
+        if (f < 0) {
+            tb = t;
+            fb = f;
+            if (side == -1)
+                fa *= (real) 0.5;
+            side = -1;
+        }
+        else {
+            ta = t;
+            fa = f;
+            if (side == 1)
+                fb *= (real) 0.5;
+            side = 1;
+        }
+        if (tb - ta < (real) 1e-12)
+            break;
+    }
+    *t_out = t;
+    return 1;
+}
+
+/* Per pixel: depth (z) of the refined root and d depth / d xi (6); valid = 0 where the march
+ * misses or the field is not interpolable around the root. */
+OR_API void or_raycast_depth_grad(void* vm, const or_pinhole* pin, const real T_WS16[16], int iters, real* depth,
+                                  real* grad6, unsigned char* valid)
+{
+    const or_map* m = (const or_map*) vm;
+    or_cam cam;
+    cam_init(&cam, pin, 1);
+    or_iso T_WS;
+    iso_from16(T_WS16, &T_WS);
+#pragma omp parallel for schedule(dynamic, 1)
+    for (int y = 0; y < cam.h; y++) {
+        for (int x = 0; x < cam.w; x++) {
+            const long idx = x + (long) y * cam.w;
+            real dS[3], dn[3], dW[3], hit[3];
+            cam_backproject(&cam, (real) x, (real) y, dS);
+            normalized3(dS, dn);
+            lin_apply(&T_WS, dn, dW);
+            depth[idx] = 0;
+            valid[idx] = 0;
+            for (int k = 0; k < 6; k++)
+                grad6[6 * idx + k] = 0;
+            real ta, tb;
+            if (!raycast_ray_bracket(m, T_WS.t, dW, cam_near_dist(&cam, dS), cam_far_dist(&cam, dS), hit, &ta, &tb))
+                continue;
+            real t;
+            if (!refine_root(m, T_WS.t, dW, ta, tb, iters, &t))
+                continue;
+            real xr[3], f, g[3];
+            for (int i = 0; i < 3; i++)
+                xr[i] = T_WS.t[i] + dW[i] * t;
+            if (!interp_and_grad(m, xr, &f, g))
+                continue;
+            const real gd = dot3(g, dW);
+            if (gd == 0)
+                continue;
+            real xg[3];
+            cross3(xr, g, xg);
+            depth[idx] = t * dn[2];
+            for (int i = 0; i < 3; i++) {
+                grad6[6 * idx + i] = -g[i] / gd * dn[2];
+                grad6[6 * idx + 3 + i] = -xg[i] / gd * dn[2];
+            }
+            valid[idx] = 1;
+        }
+    }
 }

@@ -87,6 +87,17 @@ class StageTimes(C.Structure):
         ("last_frustum_blocks", C.c_int64),
         ("last_new_blocks", C.c_int64),
         ("last_candidates", C.c_int64),
+        ("sum_fetch_frustum", C.c_double),
+        ("sum_create_list", C.c_double),
+        ("sum_allocate_list", C.c_double),
+        ("sum_update", C.c_double),
+        ("sum_raycast", C.c_double),
+        ("sum_tracking", C.c_double),
+        ("n_integrations", C.c_int64),
+        ("n_raycasts", C.c_int64),
+        ("n_tracks", C.c_int64),
+        ("sum_update_blocks", C.c_int64),
+        ("sum_candidates", C.c_int64),
     ]
 
 
@@ -106,6 +117,8 @@ SIGNATURES = {
     "se8_tracker_destroy": (C.c_int, [_P]),
     "se8_track": (C.c_int, [_P, _P, _F, _P, _P, _I, _F, _P]),
     "se8_track_raycast": (C.c_int, [_P, _P, _P, _F, _I, _P]),
+    "se8_icp_cost_grad_batched": (C.c_int, [_P, _P, _P, _P, _F, C.c_int, _F, _F, _F, _F, _I, _P]),
+    "se8_raycast_depth_grad_batched": (C.c_int, [_P, C.POINTER(Pinhole), C.c_int, _F, _P, _P, _P, _P]),
     "se8_allocate_blocks": (C.c_int, [_P, _I, C.c_int64, C.c_int, _I, C.POINTER(C.c_int64)]),
     "se8_write_block": (C.c_int, [_P, _I, _F, _F]),
     "se8_read_block": (C.c_int, [_P, _I, _F, _F]),
@@ -115,6 +128,7 @@ SIGNATURES = {
     "se8_field_interp": (C.c_int, [_P, _F, C.c_int64, _F, C.POINTER(C.c_uint8)]),
     "se8_field_grad": (C.c_int, [_P, _F, C.c_int64, _F, C.POINTER(C.c_uint8)]),
     "se8_stage_times_get": (C.c_int, [_P, _P, C.POINTER(StageTimes)]),
+    "se8_stage_times_reset": (C.c_int, [_P, _P]),
     "se8_device_alloc": (C.c_int, [C.c_size_t, C.POINTER(_P)]),
     "se8_device_free": (C.c_int, [_P]),
     "se8_copy": (C.c_int, [_P, _P, C.c_size_t, C.c_int, _P]),

@@ -393,7 +393,10 @@ struct se8_map {
     se8::DeviceBuffer update_list, candidates, unique_keys, cur_node;
     se8::RadixSorter sorter;
     size_t candidates_cap = 0;
-    int64_t host_num_blocks = 0; // mirror of counters[kCntBlocks] after the last sync
+    int* host_counters = nullptr; // pinned copy of counters taken at the end of each integration
+    bool counters_pending = false; // a copy into host_counters is in flight (event ev[7])
+    bool integ_pending = false;    // events ev[0..4] of the last integration not yet harvested
+    bool ray_pending = false;      // events ev[5..6] of the last raycast not yet harvested
     se8_stage_times times{};
     hipEvent_t ev[8]{};
 };
@@ -406,7 +409,17 @@ struct se8_tracker {
     se8::DeviceBuffer partials; // per-workgroup reduction partials
     se8::DeviceBuffer state;    // counters, flags, final sums
     se8::DeviceBuffer ref_points, ref_normals; // used by se8_track_raycast
+    se8::DeviceBuffer adj_poses, adj_partials, adj_out; // batched cost/gradient evaluation
     int num_wg[3] = {0, 0, 0};
+    void* host_state = nullptr; // pinned readback of the device tracking state
+    bool track_pending = false;
     se8_stage_times times{};
     hipEvent_t ev[2]{};
 };
+
+namespace se8 {
+// stage-time bookkeeping (se8_map.hip)
+void harvest_integration(se8_map* m);
+void harvest_raycast(se8_map* m, bool block);
+void harvest_tracking(se8_tracker* t);
+} // namespace se8

@@ -279,9 +279,20 @@ constexpr int kRsThreads = 256;
 constexpr int kRsItems = 8;
 constexpr int kRsTile = kRsThreads * kRsItems;
 
-__global__ __launch_bounds__(kRsThreads) void k_rs_hist(const uint64_t* keys, int n, int shift, uint32_t* tile_hist,
-                                                        int n_tiles)
+// The key count lives in device memory (n = min(*n_dev, cap)), so the sort is enqueued without a
+// host round trip; the grid covers `cap` keys and workgroups past the live tiles return at once.
+__device__ __forceinline__ int live_count(const int* n_dev, int cap)
 {
+    return min(*n_dev, cap);
+}
+
+__global__ __launch_bounds__(kRsThreads) void k_rs_hist(const uint64_t* keys, const int* n_dev, int cap, int shift,
+                                                        uint32_t* tile_hist)
+{
+    const int n = live_count(n_dev, cap);
+    const int n_tiles = (n + kRsTile - 1) / kRsTile;
+    if ((int) blockIdx.x >= n_tiles)
+        return;
     __shared__ uint32_t h[256];
     h[threadIdx.x] = 0;
     __syncthreads();
@@ -295,10 +306,11 @@ __global__ __launch_bounds__(kRsThreads) void k_rs_hist(const uint64_t* keys, in
     tile_hist[(size_t) threadIdx.x * n_tiles + blockIdx.x] = h[threadIdx.x];
 }
 
-// In-place exclusive scan of n uint32 with one 1024-thread workgroup.
-__global__ __launch_bounds__(1024) void k_scan_single(uint32_t* data, int n)
+// In-place exclusive scan of the n_tiles * 256 tile histograms with one 1024-thread workgroup.
+__global__ __launch_bounds__(1024) void k_scan_single(uint32_t* data, const int* n_dev, int cap)
 {
     __shared__ uint32_t sums[1024];
+    const int n = ((live_count(n_dev, cap) + kRsTile - 1) / kRsTile) * 256;
     const int t = threadIdx.x;
     const int chunk = (n + 1023) / 1024;
     const int start = t * chunk;
@@ -322,9 +334,13 @@ __global__ __launch_bounds__(1024) void k_scan_single(uint32_t* data, int n)
     }
 }
 
-__global__ __launch_bounds__(kRsThreads) void k_rs_scatter(const uint64_t* in, uint64_t* out, int n, int shift,
-                                                           const uint32_t* tile_off, int n_tiles)
+__global__ __launch_bounds__(kRsThreads) void k_rs_scatter(const uint64_t* in, uint64_t* out, const int* n_dev, int cap,
+                                                           int shift, const uint32_t* tile_off)
 {
+    const int n = live_count(n_dev, cap);
+    const int n_tiles = (n + kRsTile - 1) / kRsTile;
+    if ((int) blockIdx.x >= n_tiles)
+        return;
     __shared__ uint32_t base[256];
     __shared__ uint32_t wave_cnt[kRsThreads / 64][256];
     const int t = threadIdx.x;
@@ -386,16 +402,18 @@ void RadixSorter::release()
     cap = 0;
 }
 
-// Sorts keys[0..n) on bit range [lo_bit, hi_bit); returns the buffer holding the result.
-static uint64_t* radix_sort(RadixSorter& rs, uint64_t* keys, int n, int lo_bit, int hi_bit, hipStream_t s)
+// Sorts keys[0..min(*n_dev, cap)) on bit range [lo_bit, hi_bit); returns the buffer holding the
+// result.  The number of passes is fixed by the bit range, so the result buffer is known here.
+static uint64_t* radix_sort(RadixSorter& rs, uint64_t* keys, const int* n_dev, int cap, int lo_bit, int hi_bit,
+                            hipStream_t s)
 {
-    const int n_tiles = (n + kRsTile - 1) / kRsTile;
+    const int max_tiles = (cap + kRsTile - 1) / kRsTile;
     uint64_t* a = keys;
     uint64_t* b = (uint64_t*) rs.keys_alt.ptr;
     for (int shift = lo_bit; shift < hi_bit; shift += 8) {
-        k_rs_hist<<<n_tiles, kRsThreads, 0, s>>>(a, n, shift, (uint32_t*) rs.tile_hist.ptr, n_tiles);
-        k_scan_single<<<1, 1024, 0, s>>>((uint32_t*) rs.tile_hist.ptr, n_tiles * 256);
-        k_rs_scatter<<<n_tiles, kRsThreads, 0, s>>>(a, b, n, shift, (const uint32_t*) rs.tile_hist.ptr, n_tiles);
+        k_rs_hist<<<max_tiles, kRsThreads, 0, s>>>(a, n_dev, cap, shift, (uint32_t*) rs.tile_hist.ptr);
+        k_scan_single<<<1, 1024, 0, s>>>((uint32_t*) rs.tile_hist.ptr, n_dev, cap);
+        k_rs_scatter<<<max_tiles, kRsThreads, 0, s>>>(a, b, n_dev, cap, shift, (const uint32_t*) rs.tile_hist.ptr);
         std::swap(a, b);
     }
     return a;
@@ -404,13 +422,9 @@ static uint64_t* radix_sort(RadixSorter& rs, uint64_t* keys, int n, int lo_bit, 
 // ----------------------------------------------------------------------------------------------
 // allocate-list: level-by-level octree build from the sorted candidate keys
 // ----------------------------------------------------------------------------------------------
-__global__ void k_build_level(MapView m, const uint64_t* keys, const int* n_ptr, int n_host, int* cur_node, int scale,
-                              int brick_scale)
+__device__ void build_level_one(const MapView& m, const uint64_t* keys, int i, int* cur_node, int scale,
+                                int brick_scale)
 {
-    const int n = n_ptr ? *n_ptr : n_host;
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n)
-        return;
     const uint64_t key = keys[i];
     const uint64_t code = key >> kScaleOffset;
     int cur;
@@ -451,13 +465,18 @@ __global__ void k_build_level(MapView m, const uint64_t* keys, const int* n_ptr,
     m.node_child[cur * 8 + idx] = nn;
 }
 
-__global__ void k_build_blocks(MapView m, const uint64_t* keys, const int* n_ptr, int n_host, const int* cur_node,
-                               int* update_list, int mark_new)
+// One launch per scale; keys[0..min(*n_dev, cap)) are sorted.  Grid-stride over the keys.
+__global__ void k_build_level(MapView m, const uint64_t* keys, const int* n_dev, int cap, int* cur_node, int scale,
+                              int brick_scale)
 {
-    const int n = n_ptr ? *n_ptr : n_host;
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n)
-        return;
+    const int n = min(*n_dev, cap);
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+        build_level_one(m, keys, i, cur_node, scale, brick_scale);
+}
+
+__device__ void build_block_one(const MapView& m, const uint64_t* keys, int i, const int* cur_node, int* update_list,
+                                int mark_new)
+{
     const uint64_t key = keys[i];
     if (i > 0 && keys[i - 1] == key)
         return;
@@ -508,6 +527,14 @@ __global__ void k_build_blocks(MapView m, const uint64_t* keys, const int* n_ptr
                 reinterpret_cast<uint32_t*>(m.voxels)[(size_t) slot * 512 + v] = pack_f16(1.f, 0.f);
         }
     }
+}
+
+__global__ void k_build_blocks(MapView m, const uint64_t* keys, const int* n_dev, int cap, const int* cur_node,
+                               int* update_list, int mark_new)
+{
+    const int n = min(*n_dev, cap);
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+        build_block_one(m, keys, i, cur_node, update_list, mark_new);
 }
 
 // ----------------------------------------------------------------------------------------------
@@ -616,25 +643,6 @@ static int ensure_candidates(se8_map* map, size_t n)
 }
 
 // Sort the candidate keys and build nodes/blocks.  n = number of candidates (host-known).
-static int build_from_candidates(se8_map* map, uint64_t* cand, int n, int* update_list, int mark_new, hipStream_t s)
-{
-    if (n <= 0)
-        return SE8_OK;
-    const int hi_bit = kScaleOffset + 3 * map->view.max_scale;
-    const int lo_bit = kScaleOffset + 3 * kBlockScale;
-    uint64_t* sorted = radix_sort(map->sorter, cand, n, lo_bit, hi_bit, s);
-    const int brick_scale = map->view.brick_shift + kBlockScale;
-    const int grid = (n + 255) / 256;
-    for (int scale = map->view.max_scale - 1; scale > kBlockScale; scale--) {
-        k_build_level<<<grid, 256, 0, s>>>(map->view, sorted, nullptr, n, (int*) map->cur_node.ptr, scale,
-                                            brick_scale);
-    }
-    k_build_blocks<<<grid, 256, 0, s>>>(map->view, sorted, nullptr, n, (const int*) map->cur_node.ptr, update_list,
-                                        mark_new);
-    SE8_HIP_CHECK(hipGetLastError());
-    return SE8_OK;
-}
-
 static int device_count_sm()
 {
     static int cus = 0;
@@ -645,6 +653,29 @@ static int device_count_sm()
             cus = 256;
     }
     return cus;
+}
+
+
+// Sorts the candidate keys and builds nodes and blocks.  The live count is *n_dev (device memory),
+// at most cap; nothing here waits for the device.
+static int build_from_candidates(se8_map* map, uint64_t* cand, const int* n_dev, int cap, int* update_list,
+                                 int mark_new, hipStream_t s)
+{
+    if (cap <= 0)
+        return SE8_OK;
+    const int hi_bit = kScaleOffset + 3 * map->view.max_scale;
+    const int lo_bit = kScaleOffset + 3 * kBlockScale;
+    uint64_t* sorted = radix_sort(map->sorter, cand, n_dev, cap, lo_bit, hi_bit, s);
+    const int brick_scale = map->view.brick_shift + kBlockScale;
+    const int grid = std::min((cap + 255) / 256, device_count_sm() * 4);
+    for (int scale = map->view.max_scale - 1; scale > kBlockScale; scale--) {
+        k_build_level<<<grid, 256, 0, s>>>(map->view, sorted, n_dev, cap, (int*) map->cur_node.ptr, scale,
+                                            brick_scale);
+    }
+    k_build_blocks<<<grid, 256, 0, s>>>(map->view, sorted, n_dev, cap, (const int*) map->cur_node.ptr, update_list,
+                                        mark_new);
+    SE8_HIP_CHECK(hipGetLastError());
+    return SE8_OK;
 }
 
 } // namespace se8
@@ -805,6 +836,12 @@ extern "C" int se8_map_create(const se8_map_config* cfg, const se8_tsdf_config* 
         se8_map_destroy(m);
         return SE8_ERR_HIP;
     }
+    if (hipHostMalloc((void**) &m->host_counters, sizeof(int) * kNumCounters, hipHostMallocDefault) != hipSuccess) {
+        set_error("se8_map_create: hipHostMalloc failed");
+        se8_map_destroy(m);
+        return SE8_ERR_OOM;
+    }
+    std::memcpy(m->host_counters, host, sizeof(host));
     for (int i = 0; i < 8; i++) {
         if (hipEventCreate(&m->ev[i]) != hipSuccess) {
             set_error("se8_map_create: hipEventCreate failed");
@@ -830,6 +867,8 @@ extern "C" int se8_map_destroy(se8_map* m)
     for (int i = 0; i < 8; i++)
         if (m->ev[i])
             (void) hipEventDestroy(m->ev[i]);
+    if (m->host_counters)
+        (void) hipHostFree(m->host_counters);
     delete m;
     return SE8_OK;
 }
@@ -872,6 +911,17 @@ extern "C" int se8_integrate_depth(se8_map* m, int timestamp, const float* depth
     if (rc != SE8_OK)
         return rc;
 
+    // Capacity overflow of the previous frame is reported now: its counters were copied to pinned
+    // memory at the end of that call (the copy is normally complete by this point).
+    if (m->counters_pending) {
+        SE8_HIP_CHECK(hipEventSynchronize(m->ev[7]));
+        m->counters_pending = false;
+        harvest_integration(m);
+        if (m->host_counters[kCntOverflow]) {
+            set_error("se8_integrate_depth: map capacity exhausted (raise se8_tsdf_config.max_blocks)");
+            return SE8_ERR_CAPACITY;
+        }
+    }
     SE8_HIP_CHECK(hipEventRecord(m->ev[0], s));
     k_frame_reset<<<1, 64, 0, s>>>(m->view.counters);
     const int cus = device_count_sm();
@@ -882,17 +932,9 @@ extern "C" int se8_integrate_depth(se8_map* m, int timestamp, const float* depth
     k_alloc_rays<<<grid, 64, 0, s>>>(m->view, cam, T_WC, depth_dev, band, num_steps, (uint64_t*) m->candidates.ptr,
                                      (int64_t) cand_cap);
     SE8_HIP_CHECK(hipGetLastError());
-    int host[kNumCounters];
-    rc = read_counters(m, host, s);
-    if (rc != SE8_OK)
-        return rc;
-    if (host[kCntOverflow]) {
-        set_error("se8_integrate_depth: map capacity exhausted (raise se8_tsdf_config.max_blocks)");
-        return SE8_ERR_CAPACITY;
-    }
-    const int n_cand = std::min<int>(host[kCntCandidates], (int) cand_cap);
     SE8_HIP_CHECK(hipEventRecord(m->ev[2], s));
-    rc = build_from_candidates(m, (uint64_t*) m->candidates.ptr, n_cand, (int*) m->update_list.ptr, 1, s);
+    rc = build_from_candidates(m, (uint64_t*) m->candidates.ptr, m->view.counters + kCntCandidates, (int) cand_cap,
+                               (int*) m->update_list.ptr, 1, s);
     if (rc != SE8_OK)
         return rc;
     SE8_HIP_CHECK(hipEventRecord(m->ev[3], s));
@@ -905,9 +947,11 @@ extern "C" int se8_integrate_depth(se8_map* m, int timestamp, const float* depth
                                                           depth_dev, timestamp);
     SE8_HIP_CHECK(hipGetLastError());
     SE8_HIP_CHECK(hipEventRecord(m->ev[4], s));
-    m->times.last_frustum_blocks = host[kCntFrustum];
-    m->times.last_candidates = n_cand;
-    m->times.last_new_blocks = -1; // resolved lazily by se8_stage_times_get
+    SE8_HIP_CHECK(hipMemcpyAsync(m->host_counters, m->counters.ptr, sizeof(int) * kNumCounters,
+                                 hipMemcpyDeviceToHost, s));
+    SE8_HIP_CHECK(hipEventRecord(m->ev[7], s));
+    m->counters_pending = true;
+    m->integ_pending = true;
     return SE8_OK;
 }
 
@@ -988,7 +1032,11 @@ extern "C" int se8_allocate_blocks(se8_map* m, const int* voxel_coords, int64_t 
             return rc;
         SE8_HIP_CHECK(hipMemcpy(m->candidates.ptr, missing.data(), sizeof(uint64_t) * missing.size(),
                                 hipMemcpyHostToDevice));
-        rc = build_from_candidates(m, (uint64_t*) m->candidates.ptr, (int) missing.size(), nullptr, 0, nullptr);
+        const int n_missing = (int) missing.size();
+        SE8_HIP_CHECK(hipMemcpy((int*) m->counters.ptr + kCntCandidates, &n_missing, sizeof(int),
+                                hipMemcpyHostToDevice));
+        rc = build_from_candidates(m, (uint64_t*) m->candidates.ptr, m->view.counters + kCntCandidates, n_missing,
+                                   nullptr, 0, nullptr);
         if (rc != SE8_OK)
             return rc;
         int host[kNumCounters];
@@ -1156,38 +1204,107 @@ extern "C" int se8_dump_nodes(const se8_map* m, uint64_t* keys, uint8_t* masks, 
     return SE8_OK;
 }
 
+namespace se8 {
+static float elapsed(hipEvent_t a, hipEvent_t b)
+{
+    float ms = 0.f;
+    if (hipEventElapsedTime(&ms, a, b) != hipSuccess) {
+        (void) hipGetLastError();
+        return 0.f;
+    }
+    return ms;
+}
+
+// Folds the events of the last integration into the stage times (requires ev[7] complete).
+void harvest_integration(se8_map* m)
+{
+    if (!m->integ_pending)
+        return;
+    se8_stage_times& t = m->times;
+    t.fetch_frustum = elapsed(m->ev[0], m->ev[1]);
+    t.create_list = elapsed(m->ev[1], m->ev[2]);
+    t.allocate_list = elapsed(m->ev[2], m->ev[3]);
+    t.update = elapsed(m->ev[3], m->ev[4]);
+    const int* h = m->host_counters;
+    t.last_frustum_blocks = h[kCntFrustum];
+    t.last_new_blocks = h[kCntUpdate] - h[kCntFrustum];
+    t.last_candidates = h[kCntCandidates];
+    t.sum_fetch_frustum += t.fetch_frustum;
+    t.sum_create_list += t.create_list;
+    t.sum_allocate_list += t.allocate_list;
+    t.sum_update += t.update;
+    t.sum_update_blocks += h[kCntUpdate];
+    t.sum_candidates += h[kCntCandidates];
+    t.n_integrations += 1;
+    m->integ_pending = false;
+}
+
+// Folds the events of the last raycast into the stage times; without blocking it only does so
+// if they have completed.
+void harvest_raycast(se8_map* m, bool block)
+{
+    if (!m->ray_pending)
+        return;
+    if (block)
+        (void) hipEventSynchronize(m->ev[6]);
+    if (hipEventQuery(m->ev[6]) != hipSuccess) {
+        (void) hipGetLastError();
+        if (!block)
+            m->ray_pending = false; // about to be re-recorded: the sample is dropped
+        return;
+    }
+    m->times.raycast = elapsed(m->ev[5], m->ev[6]);
+    m->times.sum_raycast += m->times.raycast;
+    m->times.n_raycasts += 1;
+    m->ray_pending = false;
+}
+
+void harvest_tracking(se8_tracker* t)
+{
+    if (!t->track_pending)
+        return;
+    (void) hipEventSynchronize(t->ev[1]);
+    t->times.tracking = elapsed(t->ev[0], t->ev[1]);
+    t->times.sum_tracking += t->times.tracking;
+    t->times.n_tracks += 1;
+    t->track_pending = false;
+}
+} // namespace se8
+
 extern "C" int se8_stage_times_get(const se8_map* m, const se8_tracker* tr, se8_stage_times* out)
 {
     SE8_REQUIRE(out, "se8_stage_times_get: null argument");
     std::memset(out, 0, sizeof(*out));
     if (m) {
         se8_map* mm = const_cast<se8_map*>(m);
-        float ms = 0.f;
-        if (hipEventSynchronize(mm->ev[4]) == hipSuccess) {
-            if (hipEventElapsedTime(&ms, mm->ev[0], mm->ev[1]) == hipSuccess)
-                mm->times.fetch_frustum = ms;
-            if (hipEventElapsedTime(&ms, mm->ev[1], mm->ev[2]) == hipSuccess)
-                mm->times.create_list = ms;
-            if (hipEventElapsedTime(&ms, mm->ev[2], mm->ev[3]) == hipSuccess)
-                mm->times.allocate_list = ms;
-            if (hipEventElapsedTime(&ms, mm->ev[3], mm->ev[4]) == hipSuccess)
-                mm->times.update = ms;
+        if (mm->counters_pending) {
+            SE8_HIP_CHECK(hipEventSynchronize(mm->ev[7]));
+            mm->counters_pending = false;
         }
-        if (hipEventSynchronize(mm->ev[6]) == hipSuccess && hipEventElapsedTime(&ms, mm->ev[5], mm->ev[6]) == hipSuccess)
-            mm->times.raycast = ms;
-        (void) hipGetLastError();
-        int host[kNumCounters];
-        if (read_counters(m, host, nullptr) == SE8_OK)
-            mm->times.last_new_blocks = host[kCntUpdate] - host[kCntFrustum];
+        harvest_integration(mm);
+        harvest_raycast(mm, true);
         *out = mm->times;
     }
     if (tr) {
         se8_tracker* tt = const_cast<se8_tracker*>(tr);
-        float ms = 0.f;
-        if (hipEventSynchronize(tt->ev[1]) == hipSuccess && hipEventElapsedTime(&ms, tt->ev[0], tt->ev[1]) == hipSuccess)
-            out->tracking = ms;
-        (void) hipGetLastError();
+        harvest_tracking(tt);
+        out->tracking = tt->times.tracking;
+        out->sum_tracking = tt->times.sum_tracking;
+        out->n_tracks = tt->times.n_tracks;
     }
+    return SE8_OK;
+}
+
+extern "C" int se8_stage_times_reset(se8_map* m, se8_tracker* tr)
+{
+    se8_stage_times out;
+    int rc = se8_stage_times_get(m, tr, &out); // harvest everything pending first
+    if (rc != SE8_OK)
+        return rc;
+    if (m)
+        std::memset(&m->times, 0, sizeof(m->times));
+    if (tr)
+        std::memset(&tr->times, 0, sizeof(tr->times));
     return SE8_OK;
 }
 

@@ -290,9 +290,10 @@ __device__ bool field_grad_vox(const MapView& m, const float vf[3], float g[3])
     return true;
 }
 
-// raycast<TSDF> (raycaster_impl.hpp:456-533)
+// raycast<TSDF> (raycaster_impl.hpp:456-533).  On a hit, bracket (if given) receives the ray
+// parameters of the last two samples, whose field values straddle zero.
 __device__ bool march(const MapView& m, const float o[3], const float d[3], float t_near, float t_far, int* st_parent,
-                      float* st_tmax, int stride, float hit[3])
+                      float* st_tmax, int stride, float hit[3], float* bracket = nullptr)
 {
     const Span span = first_block(m, o, d, t_near, t_far, st_parent, st_tmax, stride);
     const float step = m.res;
@@ -340,6 +341,10 @@ __device__ bool march(const MapView& m, const float o[3], const float d[3], floa
         f_t = f_tt;
     }
     if (f_tt < 0.f) {
+        if (bracket) {
+            bracket[0] = t - stepsize;
+            bracket[1] = t;
+        }
         t = t - stepsize * (f_tt - 0.f) / (f_tt - f_t);
 #pragma unroll
         for (int i = 0; i < 3; i++)
@@ -347,6 +352,155 @@ __device__ bool march(const MapView& m, const float o[3], const float d[3], floa
         return true;
     }
     return false;
+}
+
+// Trilinear field value and its spatial gradient (per metre) at a world point; false where the
+// reference's interpolation is undefined (visitor_impl.hpp:976-1024).
+__device__ bool interp_and_grad(const MapView& m, const float pW[3], float& f, float g[3])
+{
+    float vf[3];
+    if (!point_to_voxel_f(m, pW, vf))
+        return false;
+    float bf[3], t[3], tc[3];
+    int base[3];
+#pragma unroll
+    for (int i = 0; i < 3; i++) {
+        bf[i] = vf[i] - 0.5f;
+        base[i] = (int) bf[i];
+        if (base[i] + 1 >= m.size || base[i] < 0)
+            return false;
+    }
+    float d[8];
+#pragma unroll
+    for (int k = 0; k < 8; k++) {
+        const int x = base[0] + (k & 1), y = base[1] + ((k >> 1) & 1), z = base[2] + ((k >> 2) & 1);
+        const int slot = lookup_block(m, x, y, z);
+        if (slot < 0)
+            return false;
+        float w;
+        load_voxel(m, slot, (x & 7) + ((y & 7) << 3) + ((z & 7) << 6), d[k], w);
+        if (w == 0.f)
+            return false;
+    }
+#pragma unroll
+    for (int i = 0; i < 3; i++) {
+        t[i] = bf[i] - floorf(bf[i]);
+        tc[i] = 1.f - t[i];
+    }
+    f = ((d[0] * tc[0] + d[1] * t[0]) * tc[1] + (d[2] * tc[0] + d[3] * t[0]) * t[1]) * tc[2]
+        + ((d[4] * tc[0] + d[5] * t[0]) * tc[1] + (d[6] * tc[0] + d[7] * t[0]) * t[1]) * t[2];
+    g[0] = (((d[1] - d[0]) * tc[1] + (d[3] - d[2]) * t[1]) * tc[2] + ((d[5] - d[4]) * tc[1] + (d[7] - d[6]) * t[1]) * t[2])
+        / m.res;
+    g[1] = (((d[2] - d[0]) * tc[0] + (d[3] - d[1]) * t[0]) * tc[2] + ((d[6] - d[4]) * tc[0] + (d[7] - d[5]) * t[0]) * t[2])
+        / m.res;
+    g[2] = (((d[4] - d[0]) * tc[0] + (d[5] - d[1]) * t[0]) * tc[1] + ((d[6] - d[2]) * tc[0] + (d[7] - d[3]) * t[0]) * t[1])
+        / m.res;
+    return true;
+}
+
+// Root of the trilinear field on [ta, tb] (f(ta) >= 0 > f(tb)) by Illinois regula falsi.
+__device__ bool refine_root(const MapView& m, const float o[3], const float d[3], float ta, float tb, int iters,
+                            float& t_out)
+{
+    float p[3], fa, fb, g[3];
+#pragma unroll
+    for (int i = 0; i < 3; i++)
+        p[i] = o[i] + d[i] * ta;
+    if (!interp_and_grad(m, p, fa, g))
+        return false;
+#pragma unroll
+    for (int i = 0; i < 3; i++)
+        p[i] = o[i] + d[i] * tb;
+    if (!interp_and_grad(m, p, fb, g))
+        return false;
+    if (!(fa >= 0.f && fb < 0.f))
+        return false;
+    int side = 0;
+    float t = tb;
+    for (int it = 0; it < iters; it++) {
+        t = (ta * fb - tb * fa) / (fb - fa);
+        float f;
+#pragma unroll
+        for (int i = 0; i < 3; i++)
+            p[i] = o[i] + d[i] * t;
+        if (!interp_and_grad(m, p, f, g))
+            return false;
+        if (f < 0.f) {
+            tb = t;
+            fb = f;
+            if (side == -1)
+                fa *= 0.5f;
+            side = -1;
+        }
+        else {
+            ta = t;
+            fa = f;
+            if (side == 1)
+                fb *= 0.5f;
+            side = 1;
+        }
+        if (tb - ta <= 0.f)
+            break;
+    }
+    t_out = t;
+    return true;
+}
+
+// Batched differentiable depth: grid (tiles, poses).  d depth / d xi = dn_z * dt/dxi with
+// dt/dxi = -[g, x cross g] / (g . d) for the left twist xi = [v; w].
+__global__ __launch_bounds__(kRayThreads) void k_raycast_depth_grad(MapView m, Cam cam, const float* poses12,
+                                                                     float* depth, float* grad, uint8_t* valid)
+{
+    __shared__ int st_parent[kMaxLevels * kRayThreads];
+    __shared__ float st_tmax[kMaxLevels * kRayThreads];
+    const int k = blockIdx.z;
+    const int t = threadIdx.x;
+    const int x = blockIdx.x * 16 + (t & 15);
+    const int y = blockIdx.y * 8 + (t >> 4);
+    if (x >= cam.w || y >= cam.h)
+        return;
+    Iso T;
+    for (int i = 0; i < 3; i++) {
+        for (int j = 0; j < 3; j++)
+            T.R[i][j] = poses12[12 * k + 4 * i + j];
+        T.t[i] = poses12[12 * k + 4 * i + 3];
+    }
+    const size_t idx = (size_t) k * cam.w * cam.h + x + (size_t) y * cam.w;
+    float dS[3], dn[3], dW[3], hit[3], br[2];
+    cam_backproject(cam, (float) x, (float) y, dS);
+    normalized3(dS, dn);
+    lin_apply(T, dn, dW);
+    float out_d = 0.f, out_g[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+    uint8_t ok = 0;
+    if (march(m, T.t, dW, cam_near_dist(cam, dS), cam_far_dist(cam, dS), st_parent + t, st_tmax + t, kRayThreads, hit,
+              br)) {
+        float tr;
+        if (refine_root(m, T.t, dW, br[0], br[1], 24, tr)) {
+            float xr[3], f, g[3];
+#pragma unroll
+            for (int i = 0; i < 3; i++)
+                xr[i] = T.t[i] + dW[i] * tr;
+            if (interp_and_grad(m, xr, f, g)) {
+                const float gd = dot3(g, dW);
+                if (gd != 0.f) {
+                    float xg[3];
+                    cross3(xr, g, xg);
+                    out_d = tr * dn[2];
+#pragma unroll
+                    for (int i = 0; i < 3; i++) {
+                        out_g[i] = -g[i] / gd * dn[2];
+                        out_g[3 + i] = -xg[i] / gd * dn[2];
+                    }
+                    ok = 1;
+                }
+            }
+        }
+    }
+    depth[idx] = out_d;
+#pragma unroll
+    for (int i = 0; i < 6; i++)
+        grad[6 * idx + i] = out_g[i];
+    valid[idx] = ok;
 }
 
 __global__ __launch_bounds__(kRayThreads) void k_raycast(MapView m, Cam cam, Iso T_WS, float* points, float* normals,
@@ -479,9 +633,11 @@ extern "C" int se8_raycast(const se8_map* m, const se8_pinhole* sensor, const fl
     hipStream_t s = (hipStream_t) stream;
     se8_map* mm = const_cast<se8_map*>(m);
     const Cam cam = make_cam(*sensor, 1, false);
+    harvest_raycast(mm, false);
     SE8_HIP_CHECK(hipEventRecord(mm->ev[5], s));
     int rc = launch_raycast(m, cam, iso_from16(T_WS16), points_dev, normals_dev, scale_dev, s);
     SE8_HIP_CHECK(hipEventRecord(mm->ev[6], s));
+    mm->ray_pending = true;
     return rc;
 }
 
@@ -523,5 +679,41 @@ extern "C" int se8_field_grad(const se8_map* m, const float* pts, int64_t n, flo
         return rc;
     std::memcpy(out3, out.data(), sizeof(float) * 3 * n);
     std::memcpy(valid, val.data(), (size_t) n);
+    return SE8_OK;
+}
+
+extern "C" int se8_raycast_depth_grad_batched(const se8_map* m, const se8_pinhole* sensor, int n_poses,
+                                              const float* poses16, float* depth_dev, float* grad_dev,
+                                              uint8_t* valid_dev, se8_stream stream)
+{
+    SE8_REQUIRE(m && sensor && poses16 && depth_dev && grad_dev && valid_dev,
+                "se8_raycast_depth_grad_batched: null argument");
+    SE8_REQUIRE(n_poses > 0 && n_poses <= 65535, "se8_raycast_depth_grad_batched: n_poses out of range");
+    SE8_REQUIRE(m->view.max_scale - kBlockScale <= kMaxLevels, "se8_raycast_depth_grad_batched: octree too deep");
+    hipStream_t s = (hipStream_t) stream;
+    const Cam cam = make_cam(*sensor, 1, false);
+    std::vector<float> p12((size_t) 12 * n_poses);
+    for (int k = 0; k < n_poses; k++)
+        for (int i = 0; i < 12; i++)
+            p12[(size_t) 12 * k + i] = poses16[(size_t) 16 * k + i];
+    DeviceBuffer dp;
+    int rc = dev_alloc(dp, sizeof(float) * p12.size());
+    if (rc != SE8_OK)
+        return rc;
+    if (hipMemcpyAsync(dp.ptr, p12.data(), sizeof(float) * p12.size(), hipMemcpyHostToDevice, s) != hipSuccess) {
+        dev_free(dp);
+        set_error("se8_raycast_depth_grad_batched: pose upload failed");
+        return SE8_ERR_HIP;
+    }
+    dim3 grid((cam.w + 15) / 16, (cam.h + 7) / 8, n_poses);
+    k_raycast_depth_grad<<<grid, kRayThreads, 0, s>>>(m->view, cam, (const float*) dp.ptr, depth_dev, grad_dev,
+                                                      valid_dev);
+    const hipError_t e1 = hipGetLastError();
+    const hipError_t e2 = hipStreamSynchronize(s);
+    dev_free(dp);
+    if (e1 != hipSuccess || e2 != hipSuccess) {
+        set_error(std::string("se8_raycast_depth_grad_batched: ") + hipGetErrorString(e1 != hipSuccess ? e1 : e2));
+        return SE8_ERR_HIP;
+    }
     return SE8_OK;
 }

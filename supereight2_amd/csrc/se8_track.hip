@@ -18,6 +18,7 @@
 
 #include <cmath>
 #include <cstring>
+#include <vector>
 
 #include "se8_internal.hpp"
 
@@ -464,23 +465,43 @@ __global__ __launch_bounds__(kIcpThreads) void k_icp_iter(IcpArgs a)
     if (!is_last)
         return;
     __threadfence();
-    // last workgroup: fixed-order cross-workgroup reduction, solve, pose update
+    // last workgroup: fixed-order cross-workgroup reduction over all 256 threads (32 terms x 8
+    // strided groups of workgroup partials, then the 8 group sums in order), solve, pose update.
+    // The fence above invalidated this CU's L1, so plain loads see the other workgroups' partials.
     __shared__ float fin[32];
     __shared__ double fin_d[32];
-    if (threadIdx.x < 32) {
-        const int k = threadIdx.x;
-        const volatile float* pp = a.partials;
+    __shared__ double grp_d[8][32];
+    __shared__ float grp[8][32];
+    {
+        const int k = threadIdx.x & 31, g = threadIdx.x >> 5;
+        const float* pp = a.partials;
         if (Comp) {
             double acc = 0.0;
-            for (int b = 0; b < (int) gridDim.x; b++)
+            for (int b = g; b < (int) gridDim.x; b += 8)
                 acc += (double) pp[b * 64 + k] + (double) pp[b * 64 + 32 + k];
+            grp_d[g][k] = acc;
+        }
+        else {
+            float acc = 0.f;
+            for (int b = g; b < (int) gridDim.x; b += 8)
+                acc += pp[b * 64 + k];
+            grp[g][k] = acc;
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x < 32) {
+        const int k = threadIdx.x;
+        if (Comp) {
+            double acc = grp_d[0][k];
+            for (int g = 1; g < 8; g++)
+                acc += grp_d[g][k];
             fin_d[k] = acc;
             fin[k] = (float) acc;
         }
         else {
-            float acc = 0.f;
-            for (int b = 0; b < (int) gridDim.x; b++)
-                acc += pp[b * 64 + k];
+            float acc = grp[0][k];
+            for (int g = 1; g < 8; g++)
+                acc += grp[g][k];
             fin[k] = acc;
             fin_d[k] = (double) acc;
         }
@@ -587,9 +608,11 @@ static int run_track(se8_tracker* tr, const float* depth, float T16[16], const f
     k_icp_check<<<1, 64, 0, s>>>(st, T_ref, P.width, P.height, tr->cfg.track_threshold, tr->cfg.compensated);
     SE8_HIP_CHECK(hipGetLastError());
     SE8_HIP_CHECK(hipEventRecord(tr->ev[1], s));
-    IcpState host;
+    IcpState& host = *(IcpState*) tr->host_state;
     SE8_HIP_CHECK(hipMemcpyAsync(&host, st, sizeof(IcpState), hipMemcpyDeviceToHost, s));
     SE8_HIP_CHECK(hipStreamSynchronize(s));
+    tr->track_pending = true;
+    harvest_tracking(tr);
     Iso T;
     for (int i = 0; i < 3; i++) {
         for (int j = 0; j < 3; j++)
@@ -606,6 +629,71 @@ static int run_track(se8_tracker* tr, const float* depth, float T16[16], const f
 
 int launch_raycast(const se8_map* m, const Cam& cam, const Iso& T_WS, float* points, float* normals, int8_t* scale,
                    hipStream_t s);
+
+// ----------------------------------------------------------------------------------------------
+// Batched point-to-plane cost and its adjoint (differentiation growth pattern).
+// For pose k: cost_k = 0.5 sum_i e_i^2 with e_i the trackKernel residual against fixed reference
+// maps raycast at T_ref; grad_k = d cost / d xi (left twist T <- exp(xi) T, xi = [v; w]) at xi = 0,
+// which is -sum_i e_i J_i (de/dxi = -J); J^T J (21 terms) is returned for Gauss-Newton steps.
+// Grid (pixel workgroups, poses); per-workgroup partials, then one workgroup per pose reduces in
+// FP64 in a fixed order.
+// ----------------------------------------------------------------------------------------------
+constexpr int kAdjTerms = 29; // e^2, e J (6), J^T J (21), inliers
+
+__global__ __launch_bounds__(kIcpThreads) void k_adj_icp(IcpArgs a, const float* poses12, int n_wg, float* partials)
+{
+    const int k = blockIdx.y;
+    Iso T;
+    for (int i = 0; i < 3; i++) {
+        for (int j = 0; j < 3; j++)
+            T.R[i][j] = poses12[12 * k + 4 * i + j];
+        T.t[i] = poses12[12 * k + 4 * i + 3];
+    }
+    float s[kAdjTerms];
+    for (int q = 0; q < kAdjTerms; q++)
+        s[q] = 0.f;
+    const int n = a.w * a.h;
+    for (int p = blockIdx.x * kIcpThreads + threadIdx.x; p < n; p += n_wg * kIcpThreads) {
+        float e = 0.f, J[6];
+        if (icp_pixel(a, T, p, e, J) == 1) {
+            s[0] += e * e;
+            for (int q = 0; q < 6; q++)
+                s[1 + q] += e * J[q];
+            int c = 7;
+            for (int i = 0; i < 6; i++)
+                for (int j = i; j < 6; j++)
+                    s[c++] += J[i] * J[j];
+            s[28] += 1.f;
+        }
+    }
+    __shared__ float red[kIcpThreads / 64][kAdjTerms];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    for (int off = 32; off > 0; off >>= 1)
+        for (int q = 0; q < kAdjTerms; q++)
+            s[q] += __shfl_xor(s[q], off);
+    if (lane == 0)
+        for (int q = 0; q < kAdjTerms; q++)
+            red[wave][q] = s[q];
+    __syncthreads();
+    if (threadIdx.x < kAdjTerms) {
+        float v = 0.f;
+        for (int w = 0; w < kIcpThreads / 64; w++)
+            v += red[w][threadIdx.x];
+        partials[((size_t) k * n_wg + blockIdx.x) * kAdjTerms + threadIdx.x] = v;
+    }
+}
+
+__global__ void k_adj_icp_reduce(const float* partials, int n_wg, double* out)
+{
+    const int k = blockIdx.x;
+    const int q = threadIdx.x;
+    if (q >= kAdjTerms)
+        return;
+    double acc = 0.0;
+    for (int b = 0; b < n_wg; b++)
+        acc += (double) partials[((size_t) k * n_wg + b) * kAdjTerms + q];
+    out[(size_t) k * kAdjTerms + q] = acc;
+}
 
 } // namespace se8
 
@@ -635,6 +723,10 @@ extern "C" int se8_tracker_create(const se8_pinhole* sensor, const se8_tracker_c
     }
     rc = rc ? rc : dev_alloc(t->partials, sizeof(float) * 64 * maxwg);
     rc = rc ? rc : dev_alloc(t->state, sizeof(IcpState));
+    if (rc == SE8_OK && hipHostMalloc(&t->host_state, sizeof(IcpState), hipHostMallocDefault) != hipSuccess) {
+        t->host_state = nullptr;
+        rc = SE8_ERR_OOM;
+    }
     for (int i = 0; i < 2 && rc == SE8_OK; i++)
         if (hipEventCreate(&t->ev[i]) != hipSuccess)
             rc = SE8_ERR_HIP;
@@ -658,8 +750,13 @@ extern "C" int se8_tracker_destroy(se8_tracker* t)
     }
     dev_free(t->partials);
     dev_free(t->state);
+    if (t->host_state)
+        (void) hipHostFree(t->host_state);
     dev_free(t->ref_points);
     dev_free(t->ref_normals);
+    dev_free(t->adj_poses);
+    dev_free(t->adj_partials);
+    dev_free(t->adj_out);
     for (int i = 0; i < 2; i++)
         if (t->ev[i])
             (void) hipEventDestroy(t->ev[i]);
@@ -690,4 +787,65 @@ extern "C" int se8_track_raycast(se8_tracker* t, const se8_map* m, const float* 
         return rc;
     return run_track(t, depth_dev, T_WS, (const float*) t->ref_points.ptr, (const float*) t->ref_normals.ptr, ok,
                      nullptr, s);
+}
+
+extern "C" int se8_icp_cost_grad_batched(se8_tracker* t, const float* depth_dev, const float* ref_points_dev,
+                                         const float* ref_normals_dev, const float T_ref16[16], int n_poses,
+                                         const float* poses16, float* cost, float* grad6, float* jtj21,
+                                         int* inliers, se8_stream stream)
+{
+    SE8_REQUIRE(t && depth_dev && ref_points_dev && ref_normals_dev && T_ref16 && poses16 && cost && grad6,
+                "se8_icp_cost_grad_batched: null argument");
+    SE8_REQUIRE(n_poses > 0 && n_poses <= 4096, "se8_icp_cost_grad_batched: n_poses out of range");
+    hipStream_t s = (hipStream_t) stream;
+    const se8_pinhole& P = t->sensor;
+    const int n_wg = t->num_wg[0];
+    int rc = dev_alloc(t->adj_poses, sizeof(float) * 12 * n_poses);
+    rc = rc ? rc : dev_alloc(t->adj_partials, sizeof(float) * kAdjTerms * n_wg * n_poses);
+    rc = rc ? rc : dev_alloc(t->adj_out, sizeof(double) * kAdjTerms * n_poses);
+    if (rc != SE8_OK)
+        return rc;
+    std::vector<float> p12((size_t) 12 * n_poses);
+    for (int k = 0; k < n_poses; k++)
+        for (int i = 0; i < 12; i++)
+            p12[(size_t) 12 * k + i] = poses16[(size_t) 16 * k + i];
+    SE8_HIP_CHECK(hipMemcpyAsync(t->adj_poses.ptr, p12.data(), sizeof(float) * p12.size(), hipMemcpyHostToDevice, s));
+    const Cam c0 = make_cam(P, 1, true);
+    dim3 g((c0.w + 63) / 64, c0.h);
+    k_vertex_normal<<<g, 64, 0, s>>>(depth_dev, c0, c0.left_hand_frame, (float*) t->vert_lvl[0].ptr,
+                                     (float*) t->norm_lvl[0].ptr);
+    IcpArgs a;
+    a.vert = (const float*) t->vert_lvl[0].ptr;
+    a.norm = (const float*) t->norm_lvl[0].ptr;
+    a.ref_pts = ref_points_dev;
+    a.ref_nrm = ref_normals_dev;
+    a.partials = nullptr;
+    a.st = nullptr;
+    a.T_ref_inv = iso_inv(iso_from16(T_ref16));
+    a.cam = make_cam(P, 1, false);
+    a.w = P.width;
+    a.h = P.height;
+    a.dist_th = t->cfg.dist_threshold;
+    a.normal_th = t->cfg.normal_threshold;
+    a.icp_th = t->cfg.icp_threshold;
+    a.level = 0;
+    k_adj_icp<<<dim3(n_wg, n_poses), kIcpThreads, 0, s>>>(a, (const float*) t->adj_poses.ptr, n_wg,
+                                                          (float*) t->adj_partials.ptr);
+    k_adj_icp_reduce<<<n_poses, 64, 0, s>>>((const float*) t->adj_partials.ptr, n_wg, (double*) t->adj_out.ptr);
+    SE8_HIP_CHECK(hipGetLastError());
+    std::vector<double> out((size_t) kAdjTerms * n_poses);
+    SE8_HIP_CHECK(hipMemcpyAsync(out.data(), t->adj_out.ptr, sizeof(double) * out.size(), hipMemcpyDeviceToHost, s));
+    SE8_HIP_CHECK(hipStreamSynchronize(s));
+    for (int k = 0; k < n_poses; k++) {
+        const double* o = out.data() + (size_t) kAdjTerms * k;
+        cost[k] = (float) (0.5 * o[0]);
+        for (int q = 0; q < 6; q++)
+            grad6[6 * k + q] = (float) (-o[1 + q]);
+        if (jtj21)
+            for (int q = 0; q < 21; q++)
+                jtj21[21 * k + q] = (float) o[7 + q];
+        if (inliers)
+            inliers[k] = (int) o[28];
+    }
+    return SE8_OK;
 }

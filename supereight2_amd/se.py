@@ -251,3 +251,9 @@ def stage_times(map_: TSDFMap | None = None, tracker: Tracker | None = None) -> 
     capi.check(lib.se8_stage_times_get(map_.h if map_ else None, tracker.h if tracker else None, C.byref(out)),
                "se8_stage_times_get")
     return {k: getattr(out, k) for k, _ in capi.StageTimes._fields_}
+
+
+def stage_times_reset(map_: TSDFMap | None = None, tracker: Tracker | None = None) -> None:
+    lib = capi.load()
+    capi.check(lib.se8_stage_times_reset(map_.h if map_ else None, tracker.h if tracker else None),
+               "se8_stage_times_reset")

@@ -1,10 +1,27 @@
 // Exercises include/se8/supereight.hpp the way the reference app drives the path
-// (app/src/main.cpp:126-160): track -> integrateDepth -> raycast_volume per frame, on a synthetic
-// fronto-parallel wall at 2 m seen by a camera that slides 1 cm per frame along x.
-// Prints one line per frame: frame, tracking ok, estimated x, ground-truth x, centre hit z.
+// (app/src/main.cpp:126-160): track -> integrateDepth -> raycast_volume per frame, on synthetic
+// depth of a room corner (three planes, so point-to-plane ICP is fully constrained) seen by a
+// camera that slides 1 cm per frame along x.
+// Prints one line per frame: frame, tracking ok, estimated x, ground-truth x, centre hit depth.
+#include <cmath>
 #include <cstdio>
 
 #include "se8/supereight.hpp"
+
+namespace {
+struct V3 {
+    double x, y, z;
+};
+V3 cross(V3 a, V3 b)
+{
+    return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x};
+}
+V3 unit(V3 a)
+{
+    const double n = std::sqrt(a.x * a.x + a.y * a.y + a.z * a.z);
+    return {a.x / n, a.y / n, a.z / n};
+}
+} // namespace
 
 int main()
 {
@@ -22,32 +39,59 @@ int main()
     se::Tracker tracker(map, sensor);
     se::Image<se::Vector3f> points(640, 480), normals(640, 480);
     se::Image<int8_t> scale(640, 480, -1);
+
+    // camera frame: z forward, x right, y down; looking into the (+x, +y, floor) corner of the
+    // room [-3,3] x [-2,2] x [0,3]
+    const V3 f = unit({1.0, 0.7, -0.35});
+    const V3 xr = unit(cross(f, {0.0, 0.0, 1.0}));
+    const V3 yd = cross(f, xr);
+    const double R[3][3] = {{xr.x, yd.x, f.x}, {xr.y, yd.y, f.y}, {xr.z, yd.z, f.z}};
+    const double lo[3] = {-3.0, -2.0, 0.0}, hi[3] = {3.0, 2.0, 3.0};
+
     se::Isometry3f T_WS = se::Isometry3f::Identity();
     int failures = 0;
+    double worst = 0.0;
     for (int frame = 0; frame < 6; frame++) {
-        const float gt_x = 0.01f * frame;
-        // a wall tilted about y so that x-motion is observable: z = 2 + 0.3 (x_W)
+        const double p[3] = {0.01 * frame, 0.0, 1.4};
         se::Image<float> depth(640, 480);
         for (int v = 0; v < 480; v++)
             for (int u = 0; u < 640; u++) {
-                const float dx = (u - cfg.cx) / cfg.fx;
-                // ray x_W = gt_x + t dx, z = t ; plane z = 2 + 0.3 x_W  ->  t = (2 + 0.3 gt_x) / (1 - 0.3 dx)
-                depth(u, v) = (2.f + 0.3f * gt_x) / (1.f - 0.3f * dx);
+                const double dc[3] = {(u - 319.5) / 525.0, (v - 239.5) / 525.0, 1.0};
+                double t = 1e9;
+                for (int i = 0; i < 3; i++) {
+                    const double dw = R[i][0] * dc[0] + R[i][1] * dc[1] + R[i][2] * dc[2];
+                    if (dw > 0)
+                        t = std::fmin(t, (hi[i] - p[i]) / dw);
+                    else if (dw < 0)
+                        t = std::fmin(t, (lo[i] - p[i]) / dw);
+                }
+                depth(u, v) = (float) t; // z-depth, since dc.z = 1
             }
         bool ok = true;
         if (frame > 0)
             ok = tracker.track(depth, T_WS, points, normals);
-        else
-            T_WS(0, 3) = gt_x;
+        else {
+            for (int i = 0; i < 3; i++) {
+                for (int j = 0; j < 3; j++)
+                    T_WS(i, j) = (float) R[i][j];
+                T_WS(i, 3) = (float) p[i];
+            }
+        }
         integrator.integrateDepth(frame, se::Measurements{se::Measurement{depth, sensor, T_WS}});
         se::raycaster::raycast_volume(map, sensor, T_WS, points, normals, scale);
         const se::Vector3f c = points(320, 240);
-        std::printf("frame %d ok %d x %.5f gt %.5f hit_z %.5f scale %d\n", frame, (int) ok, T_WS(0, 3), gt_x, c.z(),
-                    (int) scale(320, 240));
+        const double dz = std::sqrt((c.x() - p[0]) * (c.x() - p[0]) + (c.y() - p[1]) * (c.y() - p[1])
+                                    + (c.z() - p[2]) * (c.z() - p[2]));
+        std::printf("frame %d ok %d x %.5f gt %.5f hit_dist %.5f depth_dist %.5f scale %d\n", frame, (int) ok,
+                    T_WS(0, 3), p[0], dz,
+                    depth(320, 240) * std::sqrt(1.0 + std::pow(0.5 / 525.0, 2) * 2), (int) scale(320, 240));
+        worst = std::fmax(worst, std::fabs(T_WS(0, 3) - p[0]));
         if (!ok)
             failures++;
     }
-    const auto f = map.getFieldInterp(se::Vector3f(0.02f, 0.f, 2.006f));
-    std::printf("interp %d %.5f blocks %lld\n", f ? 1 : 0, f ? *f : 0.f, (long long) map.info().num_blocks);
+    const auto f_interp = map.getFieldInterp(se::Vector3f(points(320, 240).x(), points(320, 240).y(),
+                                                          points(320, 240).z()));
+    std::printf("interp %d %.5f blocks %lld worst_x_err %.6f\n", f_interp ? 1 : 0, f_interp ? *f_interp : 0.f,
+                (long long) map.info().num_blocks, worst);
     return failures == 0 ? 0 : 1;
 }

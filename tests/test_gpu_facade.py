@@ -1,5 +1,5 @@
 """The header-only C++ facade (include/se8/supereight.hpp) driven like the reference app:
-track -> integrateDepth -> raycast_volume per frame on a tilted synthetic wall."""
+track -> integrateDepth -> raycast_volume per frame, on synthetic depth of a room corner."""
 import os
 import re
 import subprocess
@@ -18,12 +18,13 @@ def test_facade_app_tracks_and_maps():
     res = subprocess.run([APP], capture_output=True, text=True, timeout=120)
     print(res.stdout, res.stderr)
     assert res.returncode == 0
-    frames = re.findall(r"frame (\d+) ok (\d) x (\S+) gt (\S+) hit_z (\S+) scale (-?\d+)", res.stdout)
+    frames = re.findall(r"frame (\d+) ok (\d) x (\S+) gt (\S+) hit_dist (\S+) depth_dist (\S+) scale (-?\d+)",
+                        res.stdout)
     assert len(frames) == 6
-    for f, ok, x, gt, hz, sc in frames:
+    for f, ok, x, gt, hd, dd, sc in frames:
         assert ok == "1"
         assert abs(float(x) - float(gt)) < 2e-3
-        assert abs(float(hz) - (2.0 + 0.3 * float(gt))) < 5e-3
+        assert abs(float(hd) - float(dd)) < 1e-2  # raycast hit at the measured surface
         assert sc == "0"
     m = re.search(r"interp (\d) (\S+) blocks (\d+)", res.stdout)
-    assert m and m.group(1) == "1" and int(m.group(3)) > 1000
+    assert m and m.group(1) == "1" and abs(float(m.group(2))) < 0.2 and int(m.group(3)) > 1000
