@@ -8,6 +8,13 @@ Workload (one step = one frame of the KinectFusion loop of the reference app, ma
   pyramid iterations 10/5/4.  The first `warmup` frames are untimed; the next `steps` frames are
   timed.  Depth frames are resident in HBM before timing starts.
 
+  Seed 10 (scene seed 10, trajectory seed 11) is the default because the reference algorithm
+  tracks the whole sequence on it (the oracle: ATE 3.2 mm, no failed pose check).  With the
+  Kinect noise model, sigma is about 2 cm at the 3.5 m typical of this room, so most per-pixel
+  normals at 640x480 fail the 0.8 normal gate and the inlier fraction sits near the 0.15 check
+  threshold; on seeds whose spheres leave long stretches of the path facing bare walls (0-5, 9,
+  11) the reference tracker fails that check part-way and the pose freezes.
+
 The FP32 path (reference arithmetic: fp32 TSDF + fp32 ICP reduction) gives `value`; the
 FP16-TSDF + compensated-ICP variant runs on the same frames and is reported beside it.
 
@@ -37,11 +44,29 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=290)
     ap.add_argument("--warmup", type=int, default=10)
-    ap.add_argument("--seed", type=int, default=0)
+    ap.add_argument("--seed", type=int, default=10)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-fp16", action="store_true")
     ap.add_argument("--cpu-frames", type=int, default=8, help="frames of the CPU baseline sample")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="first run one 256 MiB device-to-device copy, the known-size reference that scales "
+                         "the FETCH_SIZE / WRITE_SIZE counters of a profiled run to bytes")
     return ap.parse_args()
+
+
+COPY_CALIBRATION_BYTES = 256 << 20
+
+
+def copy_calibration(torch, capi):
+    """One plain device-to-device copy of COPY_CALIBRATION_BYTES through se8_copy."""
+    lib = capi.load()
+    src = torch.ones(COPY_CALIBRATION_BYTES // 4, dtype=torch.float32, device="cuda")
+    dst = torch.empty_like(src)
+    torch.cuda.synchronize()
+    s = torch.cuda.current_stream().cuda_stream
+    capi.check(lib.se8_copy(dst.data_ptr(), src.data_ptr(), COPY_CALIBRATION_BYTES, capi.COPY_D2D, s), "se8_copy")
+    torch.cuda.synchronize()
+    del src, dst
 
 
 def ate(est, gt):
@@ -147,6 +172,8 @@ def main():
     from supereight2_amd import capi, se, synth
 
     capi.load()
+    if args.copy_calibration:
+        copy_calibration(torch, capi)
     cam = synth.Camera()
     sensor = se.PinholeCamera(cam.width, cam.height, cam.fx, cam.fy, cam.cx, cam.cy, cam.near_plane, cam.far_plane)
     n_frames = min(args.warmup + args.steps, N_FRAMES_DEFAULT) if args.warmup + args.steps <= N_FRAMES_DEFAULT \

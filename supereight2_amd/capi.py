@@ -21,6 +21,9 @@ SE8_ERR_HIP = -3
 SE8_ERR_CAPACITY = -4
 STORAGE_F32 = 0
 STORAGE_F16 = 1
+COPY_H2D = 0
+COPY_D2H = 1
+COPY_D2D = 2
 
 
 class MapConfig(C.Structure):

@@ -1,15 +1,18 @@
 // Projective point-to-plane ICP of libse8.so: se::Tracker::track
 // (include/se/tracker/impl/tracker_impl.hpp:17-116).
 //
-// Per call: two robust half-sampling launches (preprocessor.cpp:156-195), one vertex+normal
-// launch per pyramid level (preprocessor_impl.hpp:18-40, preprocessor.cpp:109-152), then one
-// launch per Gauss-Newton iteration that fuses trackKernel (icp_impl.hpp:16-102) with the
-// 32-term reduction of reduceKernel (icp.cpp:45-204): per-thread accumulation, wave-shuffle and
-// LDS reduction to per-workgroup partials, and the last workgroup to finish reduces the
-// partials in a fixed order, solves the 6x6 system, applies T <- exp(x) T and raises the
-// level's convergence flag (icp.cpp:15-41, 208-223).  Later iterations of a converged level
-// return at once, so the host never waits inside the pyramid loop.  A final launch applies
-// checkPoseKernel (icp.cpp:227-246).
+// Per call: one launch builds the vertex and normal maps of all three pyramid levels (robust
+// half-sampling preprocessor.cpp:156-195, preprocessor_impl.hpp:18-40, preprocessor.cpp:109-152)
+// and initialises the device tracking state; then one launch per Gauss-Newton iteration.  Launch
+// k first solves the system of launch k - 1 in every workgroup (the same fixed-order reduction of
+// its partial rows, 6x6 solve, T <- exp(x) T and convergence test, icp.cpp:15-41, 208-223, so all
+// workgroups agree bit for bit), then runs its own pixel pass: trackKernel (icp_impl.hpp:16-102)
+// fused with the 32-term sums of reduceKernel (icp.cpp:45-204), per-thread accumulation over a
+// tile of pixels whose loads are issued together, a halving wave shuffle and an LDS reduction to
+// one partial row per workgroup.  Workgroups never wait on each other: the kernel boundary hands
+// the rows and the pose to the next launch, so there is no device-wide fence or atomic.  Passes of
+// a converged level are skipped, so the host never waits inside the pyramid loop.  A final
+// one-workgroup launch solves the last pass and applies checkPoseKernel (icp.cpp:227-246).
 //
 // compensated = 0: fp32 sums, fixed-order fp32 cross-workgroup sum, fp32 Cholesky (reference).
 // compensated = 1: Kahan-compensated fp32 per-thread sums, error-free (TwoSum) pair reduction in
@@ -25,61 +28,110 @@
 namespace se8 {
 
 constexpr int kIcpThreads = 256;
-constexpr int kIcpPxPerThread = 4;
+// Cap on the Gauss-Newton grid: a workgroup reads at most kIcpMaxWg / 32 partial rows per thread.
+// Levels larger than kIcpMaxWg tiles of kIcpThreads * 4 pixels loop over tiles.
+constexpr int kIcpMaxWg = 320;
+constexpr int kPartialStride = kIcpMaxWg * 64; // floats per partial-row buffer
+constexpr int kPyrThreads = 256;
 
-struct IcpState {
-    float T[12];       // current pose, row-major 3x4
-    float sums[32];    // last reduction
-    double sums_d[32]; // last reduction (compensated path)
+// State handed from one Gauss-Newton launch to the next, double-buffered by launch index k:
+// launch k reads slot (k + 1) & 1 and its workgroup 0 writes slot k & 1.
+struct IcpSlot {
+    float T[12];       // pose of this launch's pixel pass, row-major 3x4
+    float sums[32];    // last solved reduction
+    double sums_d[32]; // last solved reduction (compensated path)
     int done[4];       // per-level convergence flags
-    unsigned ticket;   // last-workgroup counter
+    int did_pixels;    // this launch ran its pixel pass: its partial rows are valid
+    int level;         // level of this launch
+    int rows;          // partial rows written (the launch's grid size)
+    int iterations;    // Gauss-Newton steps solved so far
+};
+
+// Result of a track call, read back by the host.
+struct IcpState {
+    float T[12];
+    float sums[32];
+    double sums_d[32];
     int ok;
-    int iterations;    // Gauss-Newton iterations executed
-    int pad;
+    int iterations;
+    int pad[2];
+};
+
+struct IcpShared {
+    IcpState result;
+    IcpSlot slot[2];
 };
 
 struct IcpArgs {
-    const float* vert;     // level vertex map (w*h*3)
-    const float* norm;     // level normal map (w*h*3)
-    const float* ref_pts;  // full-res raycast points
+    const float4* vert;    // level vertex map, one float4 {x, y, z, 0} per pixel
+    const float4* norm;    // level normal map, one float4 {x, y, z, 0} per pixel
+    const float* ref_pts;  // full-res raycast points (w*h*3, the caller's layout)
     const float* ref_nrm;  // full-res raycast normals
-    float* partials;       // nwg * 64
-    IcpState* st;
+    float* partials;       // two buffers of kPartialStride floats, by launch parity
+    IcpShared* sh;
     Iso T_ref_inv;
+    Iso T_ref;             // pose at the start of the call (restored by a failed check)
     Cam cam;               // full-resolution sensor (projection into the reference maps)
     int w, h;              // level size
-    float dist_th, normal_th, icp_th;
+    int n_tiles;           // tiles of kIcpThreads * PPT pixels
+    float dist_th, normal_th, icp_th, track_th;
     int level;
+    int k;                 // launch index within the call
 };
 
-__global__ void k_half_sample(const float* in, int w, int h, float* out, float e_d)
+// half_sample_robust_image (preprocessor.cpp:156-195) of a w x h image at output pixel (x, y):
+// the mean of the 2x2 input pixels within e_d of the top-left one (indices clamped).
+template<class In>
+__device__ __forceinline__ float half_sample_at(const In& in, int w, int h, int x, int y, float e_d)
 {
     const int ow = w / 2, oh = h / 2;
-    const int x = blockIdx.x * blockDim.x + threadIdx.x;
-    const int y = blockIdx.y;
-    if (x >= ow || y >= oh)
-        return;
     const int ix = 2 * x, iy = 2 * y;
-    const float c = in[ix + (size_t) iy * w];
+    const float c = in(ix, iy);
     float count = 0.f, sum = 0.f;
 #pragma unroll
     for (int i = 0; i <= 1; i++) {
 #pragma unroll
         for (int j = 0; j <= 1; j++) {
-            const int px = min(ix + j, 2 * ow - 1), py = min(iy + i, 2 * oh - 1);
-            const float v = in[px + (size_t) py * w];
+            const float v = in(min(ix + j, 2 * ow - 1), min(iy + i, 2 * oh - 1));
             if (fabsf(v - c) < e_d) {
                 count += 1.f;
                 sum += v;
             }
         }
     }
-    out[x + (size_t) y * ow] = sum / count;
+    return sum / count;
 }
 
-__device__ __forceinline__ void depth_point(const float* depth, const Cam& c, int x, int y, float p[3])
+// Depth of pyramid level 0, 1 or 2 at (x, y), recomputed from the full-resolution image: level 1
+// is the robust half-sample of level 0 and level 2 that of level 1, as the reference builds them
+// (tracker_impl.hpp:64-68), so recomputing gives the values a stored pyramid would hold.
+struct PyrDepth {
+    const float* d0;
+    int w0, h0;
+    float e_d;
+    __device__ __forceinline__ float l0(int x, int y) const
+    {
+        return d0[x + (size_t) y * w0];
+    }
+    __device__ __forceinline__ float l1(int x, int y) const
+    {
+        auto in = [this](int u, int v) { return l0(u, v); };
+        return half_sample_at(in, w0, h0, x, y, e_d);
+    }
+    __device__ __forceinline__ float l2(int x, int y) const
+    {
+        auto in = [this](int u, int v) { return l1(u, v); };
+        return half_sample_at(in, w0 >> 1, h0 >> 1, x, y, e_d);
+    }
+    __device__ __forceinline__ float at(int level, int x, int y) const
+    {
+        return level == 0 ? l0(x, y) : (level == 1 ? l1(x, y) : l2(x, y));
+    }
+};
+
+__device__ __forceinline__ void depth_point(const PyrDepth& pd, int level, const Cam& c, int x, int y, float p[3])
 {
-    const float d = depth[x + (size_t) y * c.w];
+    const float d = pd.at(level, x, y);
     if (d > 0.f) {
         float dir[3];
         cam_backproject(c, (float) x, (float) y, dir);
@@ -92,30 +144,65 @@ __device__ __forceinline__ void depth_point(const float* depth, const Cam& c, in
     }
 }
 
-// depth_to_point_cloud + point_cloud_to_normal<NegY> fused: neighbour points are recomputed from
-// depth, which gives the same values as reading them back.
-__global__ void k_vertex_normal(const float* depth, Cam c, int neg_y, float* vert, float* norm)
+struct PyrArgs {
+    PyrDepth depth;
+    Cam cam[3];         // PinholeCamera(sensor, 2^l) per level
+    float4* vert[3];
+    float4* norm[3];
+    int n[3];           // pixels per level
+    int levels;
+    int neg_y;
+    IcpSlot* init;      // when not null, thread 0 writes the state that the first launch reads
+    Iso T;
+};
+
+// The whole input side of Tracker::track in one launch: for every pixel of every pyramid level,
+// depth_to_point_cloud (preprocessor_impl.hpp:18-40) and point_cloud_to_normal<NegY>
+// (preprocessor.cpp:109-152) on the robustly half-sampled depth (tracker_impl.hpp:64-75).
+// Threads [0, n0) are level-0 pixels, [n0, n0+n1) level 1, then level 2.  Neighbour depths are
+// recomputed from the full-resolution image (each level-2 pixel reads at most 80 of its texels).
+__global__ __launch_bounds__(kPyrThreads) void k_pyramid(PyrArgs a)
 {
-    const int x = blockIdx.x * blockDim.x + threadIdx.x;
-    const int y = blockIdx.y;
-    if (x >= c.w || y >= c.h)
+    int i = blockIdx.x * kPyrThreads + threadIdx.x;
+    if (a.init && i == 0) {
+        IcpSlot* st = a.init;
+        for (int r = 0; r < 3; r++) {
+            for (int c = 0; c < 3; c++)
+                st->T[4 * r + c] = a.T.R[r][c];
+            st->T[4 * r + 3] = a.T.t[r];
+        }
+        for (int k = 0; k < 32; k++) {
+            st->sums[k] = 0.f;
+            st->sums_d[k] = 0.0;
+        }
+        for (int l = 0; l < 4; l++)
+            st->done[l] = 0;
+        st->did_pixels = 0;
+        st->level = 0;
+        st->rows = 0;
+        st->iterations = 0;
+    }
+    int level = 0;
+    while (level < a.levels && i >= a.n[level]) {
+        i -= a.n[level];
+        level++;
+    }
+    if (level >= a.levels)
         return;
-    const size_t i = x + (size_t) y * c.w;
+    const Cam& c = a.cam[level];
+    const int x = i % c.w, y = i / c.w;
     float p[3];
-    depth_point(depth, c, x, y, p);
-    vert[3 * i] = p[0];
-    vert[3 * i + 1] = p[1];
-    vert[3 * i + 2] = p[2];
+    depth_point(a.depth, level, c, x, y, p);
     float n[3] = {0.f, 0.f, 0.f};
     if (p[2] != 0.f) {
         const int xl = max(x - 1, 0), xr = min(x + 1, c.w - 1);
         const int ym = max(y - 1, 0), yp = min(y + 1, c.h - 1);
-        const int yu = neg_y ? ym : yp, yd = neg_y ? yp : ym;
+        const int yu = a.neg_y ? ym : yp, yd = a.neg_y ? yp : ym;
         float l[3], r[3], u[3], d[3];
-        depth_point(depth, c, xl, y, l);
-        depth_point(depth, c, xr, y, r);
-        depth_point(depth, c, x, yu, u);
-        depth_point(depth, c, x, yd, d);
+        depth_point(a.depth, level, c, xl, y, l);
+        depth_point(a.depth, level, c, xr, y, r);
+        depth_point(a.depth, level, c, x, yu, u);
+        depth_point(a.depth, level, c, x, yd, d);
         if (l[2] != 0.f && r[2] != 0.f && u[2] != 0.f && d[2] != 0.f) {
             float dx[3], dy[3], cr[3];
             for (int k = 0; k < 3; k++) {
@@ -126,63 +213,70 @@ __global__ void k_vertex_normal(const float* depth, Cam c, int neg_y, float* ver
             normalized3(cr, n);
         }
     }
-    norm[3 * i] = n[0];
-    norm[3 * i + 1] = n[1];
-    norm[3 * i + 2] = n[2];
+    a.vert[level][i] = make_float4(p[0], p[1], p[2], 0.f);
+    a.norm[level][i] = make_float4(n[0], n[1], n[2], 0.f);
 }
 
-__global__ void k_icp_init(IcpState* st, Iso T)
-{
-    if (threadIdx.x == 0) {
-        for (int i = 0; i < 3; i++) {
-            for (int j = 0; j < 3; j++)
-                st->T[4 * i + j] = T.R[i][j];
-            st->T[4 * i + 3] = T.t[i];
-        }
-        for (int k = 0; k < 32; k++) {
-            st->sums[k] = 0.f;
-            st->sums_d[k] = 0.0;
-        }
-        for (int l = 0; l < 4; l++)
-            st->done[l] = 0;
-        st->ticket = 0;
-        st->ok = 0;
-        st->iterations = 0;
-    }
-}
-
-__device__ __forceinline__ Iso load_pose(const IcpState* st)
+__device__ __forceinline__ Iso pose_of(const float* T12)
 {
     Iso T;
     for (int i = 0; i < 3; i++) {
         for (int j = 0; j < 3; j++)
-            T.R[i][j] = st->T[4 * i + j];
-        T.t[i] = st->T[4 * i + 3];
+            T.R[i][j] = T12[4 * i + j];
+        T.t[i] = T12[4 * i + 3];
     }
     return T;
 }
 
-// One pixel of trackKernel; returns the result code and fills e, J on success.
-__device__ __forceinline__ int icp_pixel(const IcpArgs& a, const Iso& T, int p, float& e, float J[6])
+__device__ __forceinline__ void store_pose(const Iso& T, float* T12)
 {
-    const float n_in[3] = {a.norm[3 * p], a.norm[3 * p + 1], a.norm[3 * p + 2]};
-    if (n_in[0] == 0.f && n_in[1] == 0.f && n_in[2] == 0.f)
-        return -1;
-    const float v[3] = {a.vert[3 * p], a.vert[3 * p + 1], a.vert[3 * p + 2]};
-    float pW[3], pR[3], uv[2];
-    iso_apply(T, v, pW);
-    iso_apply(a.T_ref_inv, pW, pR);
-    if (!cam_project(a.cam, pR, uv))
-        return -2;
-    const int px = (int) (uv[0] + 0.5f), py = (int) (uv[1] + 0.5f);
-    const size_t ri = px + (size_t) py * a.cam.w;
-    const float rn[3] = {-a.ref_nrm[3 * ri], -a.ref_nrm[3 * ri + 1], -a.ref_nrm[3 * ri + 2]};
+    for (int i = 0; i < 3; i++) {
+        for (int j = 0; j < 3; j++)
+            T12[4 * i + j] = T.R[i][j];
+        T12[4 * i + 3] = T.t[i];
+    }
+}
+
+// trackKernel (icp_impl.hpp:16-102) for one pixel, split so that a batch of pixels can issue its
+// loads together: icp_associate transforms and projects the input vertex (result -1 / -2, or 1 with
+// the reference-map index), icp_residual applies the remaining tests to the gathered reference
+// point and normal (result -3 / -4 / -5, or 1 with e and J).
+struct IcpAssoc {
+    float pW[3];
+    float n_in[3];
+    int ri;  // reference-map index (0 when the pixel failed association)
+    int res; // 1: associated; -1: no input normal; -2: projection failed
+};
+
+__device__ __forceinline__ IcpAssoc icp_associate(const IcpArgs& a, const Iso& T, float4 v4, float4 n4)
+{
+    IcpAssoc o;
+    o.n_in[0] = n4.x;
+    o.n_in[1] = n4.y;
+    o.n_in[2] = n4.z;
+    const float v[3] = {v4.x, v4.y, v4.z};
+    float pR[3], uv[2] = {0.f, 0.f};
+    iso_apply(T, v, o.pW);
+    iso_apply(a.T_ref_inv, o.pW, pR);
+    const bool proj = cam_project(a.cam, pR, uv);
+    const int px = proj ? (int) (uv[0] + 0.5f) : 0, py = proj ? (int) (uv[1] + 0.5f) : 0;
+    o.ri = px + py * a.cam.w;
+    o.res = (n4.x == 0.f && n4.y == 0.f && n4.z == 0.f) ? -1 : (proj ? 1 : -2);
+    return o;
+}
+
+__device__ __forceinline__ int icp_residual(const IcpArgs& a, const Iso& T, const IcpAssoc& s, const float rp[3],
+                                            const float rn_raw[3], float& e, float J[6])
+{
+    if (s.res != 1)
+        return s.res;
+    const float rn[3] = {-rn_raw[0], -rn_raw[1], -rn_raw[2]};
     if (rn[0] == 0.f && rn[1] == 0.f && rn[2] == 0.f)
         return -3;
     float diff[3], nW[3];
     for (int k = 0; k < 3; k++)
-        diff[k] = a.ref_pts[3 * ri + k] - pW[k];
-    lin_apply(T, n_in, nW);
+        diff[k] = rp[k] - s.pW[k];
+    lin_apply(T, s.n_in, nW);
     if (sqrtf(sqnorm3(diff)) > a.dist_th)
         return -4;
     if (dot3(nW, rn) < a.normal_th)
@@ -192,11 +286,28 @@ __device__ __forceinline__ int icp_pixel(const IcpArgs& a, const Iso& T, int p, 
     J[1] = rn[1];
     J[2] = rn[2];
     float cp[3];
-    cross3(pW, rn, cp);
+    cross3(s.pW, rn, cp);
     J[3] = cp[0];
     J[4] = cp[1];
     J[5] = cp[2];
     return 1;
+}
+
+__device__ __forceinline__ void gather_ref(const IcpArgs& a, int ri, float rp[3], float rn[3])
+{
+    for (int k = 0; k < 3; k++) {
+        rp[k] = a.ref_pts[3 * (size_t) ri + k];
+        rn[k] = a.ref_nrm[3 * (size_t) ri + k];
+    }
+}
+
+// One pixel of trackKernel; returns the result code and fills e, J on success.
+__device__ __forceinline__ int icp_pixel(const IcpArgs& a, const Iso& T, int p, float& e, float J[6])
+{
+    const IcpAssoc s = icp_associate(a, T, a.vert[p], a.norm[p]);
+    float rp[3], rn[3];
+    gather_ref(a, s.ri, rp, rn);
+    return icp_residual(a, T, s, rp, rn, e, J);
 }
 
 __device__ __forceinline__ void row_terms(int res, float e, const float J[6], float v[32])
@@ -376,167 +487,10 @@ __device__ Iso se3_exp(const float a[6])
     return E;
 }
 
-template<bool Comp>
-__global__ __launch_bounds__(kIcpThreads) void k_icp_iter(IcpArgs a)
+// checkPoseKernel (icp.cpp:227-246) on the last reduction: fail if the RMS residual exceeds 2 cm or
+// the inlier fraction of the full image is below track_th, and then restore the pose of the call.
+__device__ void check_pose(IcpState* st, const Iso& T_ref, int W, int H, float track_th, int comp)
 {
-    IcpState* st = a.st;
-    if (st->done[a.level])
-        return;
-    __shared__ float red[kIcpThreads / 64][64];
-    __shared__ int is_last;
-    const Iso T = load_pose(st);
-    const int n = a.w * a.h;
-    float s[32], c[32];
-    for (int k = 0; k < 32; k++) {
-        s[k] = 0.f;
-        c[k] = 0.f;
-    }
-    const int stride = gridDim.x * kIcpThreads;
-    for (int p = blockIdx.x * kIcpThreads + threadIdx.x; p < n; p += stride) {
-        const int x = p % a.w, y = p / a.w;
-        float e = 0.f, J[6];
-        const int res = icp_pixel(a, T, x + y * a.w, e, J);
-        float v[32];
-        row_terms(res, e, J, v);
-        for (int k = 0; k < 32; k++) {
-            if (Comp) {
-                // Kahan summation
-                const float yk = v[k] - c[k];
-                const float tk = s[k] + yk;
-                c[k] = (tk - s[k]) - yk;
-                s[k] = tk;
-            }
-            else {
-                s[k] += v[k];
-            }
-        }
-    }
-    if (Comp) {
-        // (s, -c) is the compensated pair; keep err = -c
-        for (int k = 0; k < 32; k++)
-            c[k] = -c[k];
-    }
-    // wave reduction (64 lanes)
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    for (int off = 32; off > 0; off >>= 1) {
-        for (int k = 0; k < 32; k++) {
-            const float os = __shfl_xor(s[k], off);
-            if (Comp) {
-                const float oc = __shfl_xor(c[k], off);
-                float sum, err;
-                two_sum(s[k], os, sum, err);
-                s[k] = sum;
-                c[k] = c[k] + oc + err;
-            }
-            else {
-                s[k] += os;
-            }
-        }
-    }
-    if (lane == 0) {
-        for (int k = 0; k < 32; k++) {
-            red[wave][k] = s[k];
-            red[wave][32 + k] = Comp ? c[k] : 0.f;
-        }
-    }
-    __syncthreads();
-    if (threadIdx.x < 32) {
-        const int k = threadIdx.x;
-        float ss = red[0][k], cc = red[0][32 + k];
-        for (int w = 1; w < kIcpThreads / 64; w++) {
-            if (Comp) {
-                float sum, err;
-                two_sum(ss, red[w][k], sum, err);
-                ss = sum;
-                cc = cc + red[w][32 + k] + err;
-            }
-            else {
-                ss += red[w][k];
-            }
-        }
-        a.partials[blockIdx.x * 64 + k] = ss;
-        a.partials[blockIdx.x * 64 + 32 + k] = cc;
-    }
-    __threadfence();
-    __syncthreads();
-    if (threadIdx.x == 0)
-        is_last = atomicAdd(&st->ticket, 1u) == gridDim.x - 1;
-    __syncthreads();
-    if (!is_last)
-        return;
-    __threadfence();
-    // last workgroup: fixed-order cross-workgroup reduction over all 256 threads (32 terms x 8
-    // strided groups of workgroup partials, then the 8 group sums in order), solve, pose update.
-    // The fence above invalidated this CU's L1, so plain loads see the other workgroups' partials.
-    __shared__ float fin[32];
-    __shared__ double fin_d[32];
-    __shared__ double grp_d[8][32];
-    __shared__ float grp[8][32];
-    {
-        const int k = threadIdx.x & 31, g = threadIdx.x >> 5;
-        const float* pp = a.partials;
-        if (Comp) {
-            double acc = 0.0;
-            for (int b = g; b < (int) gridDim.x; b += 8)
-                acc += (double) pp[b * 64 + k] + (double) pp[b * 64 + 32 + k];
-            grp_d[g][k] = acc;
-        }
-        else {
-            float acc = 0.f;
-            for (int b = g; b < (int) gridDim.x; b += 8)
-                acc += pp[b * 64 + k];
-            grp[g][k] = acc;
-        }
-    }
-    __syncthreads();
-    if (threadIdx.x < 32) {
-        const int k = threadIdx.x;
-        if (Comp) {
-            double acc = grp_d[0][k];
-            for (int g = 1; g < 8; g++)
-                acc += grp_d[g][k];
-            fin_d[k] = acc;
-            fin[k] = (float) acc;
-        }
-        else {
-            float acc = grp[0][k];
-            for (int g = 1; g < 8; g++)
-                acc += grp[g][k];
-            fin[k] = acc;
-            fin_d[k] = (double) acc;
-        }
-    }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        float x[6];
-        if (Comp)
-            solve_f64(fin_d + 1, x);
-        else
-            solve_f32(fin + 1, x);
-        const Iso D = se3_exp(x);
-        const Iso Tn = iso_mul(D, T);
-        for (int i = 0; i < 3; i++) {
-            for (int j = 0; j < 3; j++)
-                st->T[4 * i + j] = Tn.R[i][j];
-            st->T[4 * i + 3] = Tn.t[i];
-        }
-        for (int k = 0; k < 32; k++) {
-            st->sums[k] = fin[k];
-            st->sums_d[k] = fin_d[k];
-        }
-        const float xn = sqrtf(x[0] * x[0] + (x[1] * x[1] + (x[2] * x[2] + (x[3] * x[3] + (x[4] * x[4] + x[5] * x[5])))));
-        if (xn < a.icp_th)
-            st->done[a.level] = 1;
-        st->iterations += 1;
-        st->ticket = 0;
-        __threadfence();
-    }
-}
-
-__global__ void k_icp_check(IcpState* st, Iso T_ref, int W, int H, float track_th, int comp)
-{
-    if (threadIdx.x != 0)
-        return;
     bool fail;
     if (comp) {
         fail = (sqrt(st->sums_d[0] / st->sums_d[28]) > 2e-2) || (st->sums_d[28] / (double) (W * H) < track_th);
@@ -554,62 +508,380 @@ __global__ void k_icp_check(IcpState* st, Iso T_ref, int W, int H, float track_t
     st->ok = fail ? 0 : 1;
 }
 
+// Adds (b, cb) into (s, cs): plain fp32 add, or an error-free TwoSum whose rounding error joins the
+// compensation term.
+template<bool Comp>
+__device__ __forceinline__ void pair_add(float& s, float& cs, float b, float cb)
+{
+    if (Comp) {
+        float sum, err;
+        two_sum(s, b, sum, err);
+        s = sum;
+        cs = cs + cb + err;
+    }
+    else {
+        s += b;
+    }
+}
+
+// Sums the 32 terms s[k] over the 64 lanes of a wave by halving exchanges: at each step a lane
+// keeps half of its terms and receives its partner's copy of that half (16 + 8 + 4 + 2 + 1 + 1 = 32
+// shuffles in place of 6 x 32).  Afterwards s[0] of lanes 2t and 2t+1 holds the wave total of term t.
+// The order of the additions is fixed, so results do not depend on scheduling.
+// One halving step over lane distance O: the O/2 surviving terms are compile-time register
+// indices (a runtime step would index the register arrays dynamically).
+template<bool Comp, int O>
+__device__ __forceinline__ void halving_step(float s[32], float c[32], bool up)
+{
+#pragma unroll
+    for (int i = 0; i < O / 2; i++) {
+        const float keep = up ? s[i + O / 2] : s[i];
+        const float send = up ? s[i] : s[i + O / 2];
+        const float recv = __shfl_xor(send, O);
+        float keepc = 0.f, recvc = 0.f;
+        if (Comp) {
+            keepc = up ? c[i + O / 2] : c[i];
+            recvc = __shfl_xor(up ? c[i] : c[i + O / 2], O);
+        }
+        s[i] = keep;
+        c[i] = keepc;
+        pair_add<Comp>(s[i], c[i], recv, recvc);
+    }
+}
+
+template<bool Comp>
+__device__ __forceinline__ void wave_reduce32(float s[32], float c[32])
+{
+    const int lane = threadIdx.x & 63;
+    halving_step<Comp, 32>(s, c, (lane & 32) != 0);
+    halving_step<Comp, 16>(s, c, (lane & 16) != 0);
+    halving_step<Comp, 8>(s, c, (lane & 8) != 0);
+    halving_step<Comp, 4>(s, c, (lane & 4) != 0);
+    halving_step<Comp, 2>(s, c, (lane & 2) != 0);
+    const float recv = __shfl_xor(s[0], 1);
+    const float recvc = Comp ? __shfl_xor(c[0], 1) : 0.f;
+    pair_add<Comp>(s[0], c[0], recv, recvc);
+}
+
+// Input vertex/normal of the PPT pixels of one tile (independent of the pose, so they can be in
+// flight while the previous system is solved).
+template<int PPT>
+__device__ __forceinline__ void load_tile(const IcpArgs& a, int tile, int n, float4 v4[PPT], float4 n4[PPT])
+{
+    const int base = tile * (kIcpThreads * PPT) + threadIdx.x;
+#pragma unroll
+    for (int j = 0; j < PPT; j++) {
+        const int p = min(base + j * kIcpThreads, n - 1);
+        v4[j] = a.vert[p];
+        n4[j] = a.norm[p];
+    }
+}
+
+// Workgroup-wide, identical in every workgroup: copy the previous launch's slot into `cur` (LDS)
+// and, if that launch ran a pixel pass, reduce its partial rows in a fixed order, solve, apply
+// T <- exp(x) T and raise its level's convergence flag (icp.cpp:15-41, 208-223).  Thread t owns
+// terms 4q..4q+3 (q = t & 7) of rows b = g, g + 32, ... (g = t >> 3); it issues all its 16-byte
+// loads before adding them in increasing b; then thread k < 32 adds the 32 group sums of term k in
+// increasing g.  The rows were written by the previous kernel, so plain loads see them.
+template<bool Comp>
+__device__ void solve_previous(const IcpArgs& a, const IcpSlot* prev, const float* rows, IcpSlot& cur)
+{
+    __shared__ double grp_d[32][33];
+    __shared__ float grp[32][33];
+    const int t = threadIdx.x;
+    if (t < 12)
+        cur.T[t] = prev->T[t];
+    if (t < 32) {
+        cur.sums[t] = prev->sums[t];
+        cur.sums_d[t] = prev->sums_d[t];
+    }
+    if (t < 4)
+        cur.done[t] = prev->done[t];
+    if (t == 0)
+        cur.iterations = prev->iterations;
+    const int nrows = prev->rows;
+    if (prev->did_pixels) {
+        constexpr int kRows = kIcpMaxWg / 32;
+        const int q = t & 7, g = t >> 3;
+        float4 sv[kRows], cv[kRows];
+#pragma unroll
+        for (int i = 0; i < kRows; i++) {
+            const int b = g + 32 * i;
+            const bool in = b < nrows;
+            sv[i] = in ? reinterpret_cast<const float4*>(rows + b * 64)[q] : make_float4(0.f, 0.f, 0.f, 0.f);
+            if (Comp)
+                cv[i] = in ? reinterpret_cast<const float4*>(rows + b * 64 + 32)[q] : make_float4(0.f, 0.f, 0.f, 0.f);
+        }
+        if (Comp) {
+            double acc[4] = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+            for (int i = 0; i < kRows; i++) {
+                acc[0] += (double) sv[i].x + (double) cv[i].x;
+                acc[1] += (double) sv[i].y + (double) cv[i].y;
+                acc[2] += (double) sv[i].z + (double) cv[i].z;
+                acc[3] += (double) sv[i].w + (double) cv[i].w;
+            }
+            for (int k = 0; k < 4; k++)
+                grp_d[g][4 * q + k] = acc[k];
+        }
+        else {
+            float acc[4] = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+            for (int i = 0; i < kRows; i++) {
+                acc[0] += sv[i].x;
+                acc[1] += sv[i].y;
+                acc[2] += sv[i].z;
+                acc[3] += sv[i].w;
+            }
+            for (int k = 0; k < 4; k++)
+                grp[g][4 * q + k] = acc[k];
+        }
+        __syncthreads();
+        if (t < 32) {
+            if (Comp) {
+                double acc = grp_d[0][t];
+                for (int j = 1; j < 32; j++)
+                    acc += grp_d[j][t];
+                cur.sums_d[t] = acc;
+                cur.sums[t] = (float) acc;
+            }
+            else {
+                float acc = grp[0][t];
+                for (int j = 1; j < 32; j++)
+                    acc += grp[j][t];
+                cur.sums[t] = acc;
+                cur.sums_d[t] = (double) acc;
+            }
+        }
+        __syncthreads();
+        if (t == 0) {
+            float x[6];
+            if (Comp)
+                solve_f64(cur.sums_d + 1, x);
+            else
+                solve_f32(cur.sums + 1, x);
+            const Iso Tn = iso_mul(se3_exp(x), pose_of(cur.T));
+            store_pose(Tn, cur.T);
+            const float xn = sqrtf(x[0] * x[0] + (x[1] * x[1] + (x[2] * x[2] + (x[3] * x[3] + (x[4] * x[4] + x[5] * x[5])))));
+            if (xn < a.icp_th)
+                cur.done[prev->level] = 1;
+            cur.iterations += 1;
+        }
+    }
+    __syncthreads();
+}
+
+// Launch k of the Gauss-Newton sequence of Tracker::track (tracker_impl.hpp:77-108): solve the
+// pass of launch k - 1 (see solve_previous), then, unless this level has converged, the pixel pass
+// of iteration k at this level: trackKernel (icp_impl.hpp:16-102) fused with the 32-term
+// reduceKernel sums (icp.cpp:45-204) over tiles of PPT pixels per thread, a halving wave
+// reduction and an LDS reduction to one partial row per workgroup.  Workgroup 0 publishes the
+// pose and flags for launch k + 1.  No workgroup waits on another, so no device-wide fence or
+// atomic is needed; the kernel boundary orders the hand-over.
+template<bool Comp, int PPT>
+__global__ __launch_bounds__(kIcpThreads) void k_icp_iter(IcpArgs a)
+{
+    __shared__ IcpSlot cur;
+    __shared__ float red[kIcpThreads / 64][64];
+    const IcpSlot* prev = &a.sh->slot[(a.k + 1) & 1];
+    const int n = a.w * a.h;
+    float4 v4[PPT], n4[PPT];
+    load_tile<PPT>(a, blockIdx.x, n, v4, n4);
+    solve_previous<Comp>(a, prev, a.partials + ((a.k + 1) & 1) * kPartialStride, cur);
+    const bool run = cur.done[a.level] == 0;
+    if (run) {
+        float s[32], c[32];
+        for (int k = 0; k < 32; k++) {
+            s[k] = 0.f;
+            c[k] = 0.f;
+        }
+        const Iso T = pose_of(cur.T);
+        for (int tile = blockIdx.x; tile < a.n_tiles; tile += gridDim.x) {
+            if (tile != (int) blockIdx.x)
+                load_tile<PPT>(a, tile, n, v4, n4);
+            const int base = tile * (kIcpThreads * PPT) + threadIdx.x;
+            IcpAssoc as[PPT];
+            float rp[PPT][3], rn[PPT][3];
+#pragma unroll
+            for (int j = 0; j < PPT; j++)
+                as[j] = icp_associate(a, T, v4[j], n4[j]);
+#pragma unroll
+            for (int j = 0; j < PPT; j++)
+                gather_ref(a, as[j].ri, rp[j], rn[j]);
+#pragma unroll
+            for (int j = 0; j < PPT; j++) {
+                if (base + j * kIcpThreads >= n)
+                    continue;
+                float e = 0.f, J[6];
+                const int res = icp_residual(a, T, as[j], rp[j], rn[j], e, J);
+                float v[32];
+                row_terms(res, e, J, v);
+                for (int k = 0; k < 32; k++) {
+                    if (Comp) {
+                        // Kahan summation
+                        const float yk = v[k] - c[k];
+                        const float tk = s[k] + yk;
+                        c[k] = (tk - s[k]) - yk;
+                        s[k] = tk;
+                    }
+                    else {
+                        s[k] += v[k];
+                    }
+                }
+            }
+        }
+        if (Comp) {
+            // (s, -c) is the compensated pair; keep err = -c
+            for (int k = 0; k < 32; k++)
+                c[k] = -c[k];
+        }
+        wave_reduce32<Comp>(s, c);
+        const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+        if ((lane & 1) == 0) {
+            red[wave][lane >> 1] = s[0];
+            red[wave][32 + (lane >> 1)] = Comp ? c[0] : 0.f;
+        }
+        __syncthreads();
+        if (threadIdx.x < 32) {
+            const int k = threadIdx.x;
+            float ss = red[0][k], cc = red[0][32 + k];
+            for (int w = 1; w < kIcpThreads / 64; w++)
+                pair_add<Comp>(ss, cc, red[w][k], red[w][32 + k]);
+            float* row = a.partials + (a.k & 1) * kPartialStride + blockIdx.x * 64;
+            row[k] = ss;
+            row[32 + k] = cc;
+        }
+    }
+    if (blockIdx.x == 0) {
+        IcpSlot* out = &a.sh->slot[a.k & 1];
+        const int t = threadIdx.x;
+        if (t < 12)
+            out->T[t] = cur.T[t];
+        if (t < 32) {
+            out->sums[t] = cur.sums[t];
+            out->sums_d[t] = cur.sums_d[t];
+        }
+        if (t < 4)
+            out->done[t] = cur.done[t];
+        if (t == 0) {
+            out->did_pixels = run ? 1 : 0;
+            out->level = a.level;
+            out->rows = gridDim.x;
+            out->iterations = cur.iterations;
+        }
+    }
+}
+
+// After the last Gauss-Newton launch (a.k = number of launches): solve its pass, then
+// checkPoseKernel (icp.cpp:227-246) on the last reduction, into the result the host reads.
+template<bool Comp>
+__global__ __launch_bounds__(kIcpThreads) void k_icp_finish(IcpArgs a)
+{
+    __shared__ IcpSlot cur;
+    solve_previous<Comp>(a, &a.sh->slot[(a.k + 1) & 1], a.partials + ((a.k + 1) & 1) * kPartialStride, cur);
+    if (threadIdx.x == 0) {
+        IcpState* r = &a.sh->result;
+        for (int i = 0; i < 12; i++)
+            r->T[i] = cur.T[i];
+        for (int k = 0; k < 32; k++) {
+            r->sums[k] = cur.sums[k];
+            r->sums_d[k] = cur.sums_d[k];
+        }
+        r->iterations = cur.iterations;
+        check_pose(r, a.T_ref, a.cam.w, a.cam.h, a.track_th, Comp);
+    }
+}
+
+// Vertex and normal maps of the first `levels` pyramid levels in one launch; writes the state the
+// first Gauss-Newton launch reads (pose T) when init is not null.
+static void launch_pyramid(se8_tracker* tr, const float* depth, int levels, IcpSlot* init, const Iso& T,
+                           hipStream_t s)
+{
+    const se8_pinhole& P = tr->sensor;
+    PyrArgs pa;
+    pa.depth.d0 = depth;
+    pa.depth.w0 = P.width;
+    pa.depth.h0 = P.height;
+    pa.depth.e_d = 0.1f * 3; // e_delta * 3 (tracker.hpp:20, tracker_impl.hpp:66-67)
+    int total = 0;
+    for (int l = 0; l < 3; l++) {
+        pa.cam[l] = make_cam(P, 1 << l, true);
+        pa.vert[l] = (float4*) tr->vert_lvl[l].ptr;
+        pa.norm[l] = (float4*) tr->norm_lvl[l].ptr;
+        pa.n[l] = pa.cam[l].w * pa.cam[l].h;
+        if (l < levels)
+            total += pa.n[l];
+    }
+    pa.levels = levels;
+    pa.neg_y = make_cam(P, 1, false).left_hand_frame;
+    pa.init = init;
+    pa.T = T;
+    k_pyramid<<<(total + kPyrThreads - 1) / kPyrThreads, kPyrThreads, 0, s>>>(pa);
+}
+
+template<bool Comp>
+static void launch_icp_iter(int ppt, int grid, const IcpArgs& a, hipStream_t s)
+{
+    switch (ppt) {
+    case 1:
+        k_icp_iter<Comp, 1><<<grid, kIcpThreads, 0, s>>>(a);
+        break;
+    case 2:
+        k_icp_iter<Comp, 2><<<grid, kIcpThreads, 0, s>>>(a);
+        break;
+    default:
+        k_icp_iter<Comp, 4><<<grid, kIcpThreads, 0, s>>>(a);
+        break;
+    }
+}
+
 static int run_track(se8_tracker* tr, const float* depth, float T16[16], const float* ref_pts, const float* ref_nrm,
                      int* ok, float* sums32, hipStream_t s)
 {
     const se8_pinhole& P = tr->sensor;
     const Cam cam = make_cam(P, 1, false);
     const Iso T_ref = iso_from16(T16);
-    IcpState* st = (IcpState*) tr->state.ptr;
+    IcpShared* sh = (IcpShared*) tr->state.ptr;
     SE8_HIP_CHECK(hipEventRecord(tr->ev[0], s));
-    k_icp_init<<<1, 64, 0, s>>>(st, T_ref);
-    const float* lvl_depth[3] = {depth, (const float*) tr->depth_lvl[1].ptr, (const float*) tr->depth_lvl[2].ptr};
-    int lw[3], lh[3];
-    for (int l = 0; l < 3; l++) {
-        lw[l] = P.width >> l;
-        lh[l] = P.height >> l;
-    }
-    const float e_d = 0.1f * 3;
-    for (int l = 1; l < 3; l++) {
-        dim3 g((lw[l] + 63) / 64, lh[l]);
-        k_half_sample<<<g, 64, 0, s>>>(lvl_depth[l - 1], lw[l - 1], lh[l - 1], (float*) tr->depth_lvl[l].ptr, e_d);
-    }
-    for (int l = 0; l < 3; l++) {
-        const Cam cl = make_cam(P, 1 << l, true);
-        dim3 g((cl.w + 63) / 64, cl.h);
-        k_vertex_normal<<<g, 64, 0, s>>>(lvl_depth[l], cl, cam.left_hand_frame, (float*) tr->vert_lvl[l].ptr,
-                                         (float*) tr->norm_lvl[l].ptr);
-    }
-    IcpArgs a;
+    // the first launch (k = 0) reads slot 1
+    launch_pyramid(tr, depth, 3, &sh->slot[1], T_ref, s);
+    IcpArgs a{};
     a.ref_pts = ref_pts;
     a.ref_nrm = ref_nrm;
     a.partials = (float*) tr->partials.ptr;
-    a.st = st;
+    a.sh = sh;
     a.T_ref_inv = iso_inv(T_ref);
+    a.T_ref = T_ref;
     a.cam = cam;
     a.dist_th = tr->cfg.dist_threshold;
     a.normal_th = tr->cfg.normal_threshold;
     a.icp_th = tr->cfg.icp_threshold;
+    a.track_th = tr->cfg.track_threshold;
+    a.k = 0;
     for (int level = 2; level >= 0; level--) {
-        a.vert = (const float*) tr->vert_lvl[level].ptr;
-        a.norm = (const float*) tr->norm_lvl[level].ptr;
+        a.vert = (const float4*) tr->vert_lvl[level].ptr;
+        a.norm = (const float4*) tr->norm_lvl[level].ptr;
         // reduction resolution is the full resolution / 2^level (tracker_impl.hpp:85-86)
         a.w = P.width / (1 << level);
         a.h = P.height / (1 << level);
         a.level = level;
+        a.n_tiles = tr->icp_tiles[level];
         const int nwg = tr->num_wg[level];
-        for (int it = 0; it < tr->cfg.iterations[level]; it++) {
+        for (int it = 0; it < tr->cfg.iterations[level]; it++, a.k++) {
             if (tr->cfg.compensated)
-                k_icp_iter<true><<<nwg, kIcpThreads, 0, s>>>(a);
+                launch_icp_iter<true>(tr->icp_ppt[level], nwg, a, s);
             else
-                k_icp_iter<false><<<nwg, kIcpThreads, 0, s>>>(a);
+                launch_icp_iter<false>(tr->icp_ppt[level], nwg, a, s);
         }
     }
-    k_icp_check<<<1, 64, 0, s>>>(st, T_ref, P.width, P.height, tr->cfg.track_threshold, tr->cfg.compensated);
+    if (tr->cfg.compensated)
+        k_icp_finish<true><<<1, kIcpThreads, 0, s>>>(a);
+    else
+        k_icp_finish<false><<<1, kIcpThreads, 0, s>>>(a);
     SE8_HIP_CHECK(hipGetLastError());
     SE8_HIP_CHECK(hipEventRecord(tr->ev[1], s));
     IcpState& host = *(IcpState*) tr->host_state;
-    SE8_HIP_CHECK(hipMemcpyAsync(&host, st, sizeof(IcpState), hipMemcpyDeviceToHost, s));
+    SE8_HIP_CHECK(hipMemcpyAsync(&host, &sh->result, sizeof(IcpState), hipMemcpyDeviceToHost, s));
     SE8_HIP_CHECK(hipStreamSynchronize(s));
     tr->track_pending = true;
     harvest_tracking(tr);
@@ -710,19 +982,21 @@ extern "C" int se8_tracker_create(const se8_pinhole* sensor, const se8_tracker_c
     t->sensor = *sensor;
     t->cfg = *cfg;
     int rc = SE8_OK;
-    int maxwg = 1;
     for (int l = 0; l < 3; l++) {
-        const size_t w = sensor->width >> l, h = sensor->height >> l;
-        if (l > 0)
-            rc = rc ? rc : dev_alloc(t->depth_lvl[l], sizeof(float) * w * h);
-        rc = rc ? rc : dev_alloc(t->vert_lvl[l], sizeof(float) * 3 * w * h);
-        rc = rc ? rc : dev_alloc(t->norm_lvl[l], sizeof(float) * 3 * w * h);
-        const size_t n = (size_t) (sensor->width / (1 << l)) * (sensor->height / (1 << l));
-        t->num_wg[l] = (int) std::max<size_t>(1, (n + kIcpThreads * kIcpPxPerThread - 1) / (kIcpThreads * kIcpPxPerThread));
-        maxwg = std::max(maxwg, t->num_wg[l]);
+        const Cam cl = make_cam(*sensor, 1 << l, true);
+        const size_t n = (size_t) cl.w * cl.h;
+        rc = rc ? rc : dev_alloc(t->vert_lvl[l], sizeof(float4) * n);
+        rc = rc ? rc : dev_alloc(t->norm_lvl[l], sizeof(float4) * n);
+        // smallest pixels-per-thread whose tiles fit one grid of at most kIcpMaxWg workgroups
+        int ppt = 1;
+        while (ppt < 4 && (n + kIcpThreads * ppt - 1) / (kIcpThreads * ppt) > (size_t) kIcpMaxWg)
+            ppt *= 2;
+        t->icp_ppt[l] = ppt;
+        t->icp_tiles[l] = (int) std::max<size_t>(1, (n + kIcpThreads * ppt - 1) / (kIcpThreads * ppt));
+        t->num_wg[l] = std::min(t->icp_tiles[l], kIcpMaxWg);
     }
-    rc = rc ? rc : dev_alloc(t->partials, sizeof(float) * 64 * maxwg);
-    rc = rc ? rc : dev_alloc(t->state, sizeof(IcpState));
+    rc = rc ? rc : dev_alloc(t->partials, sizeof(float) * 2 * kPartialStride);
+    rc = rc ? rc : dev_alloc(t->state, sizeof(IcpShared));
     if (rc == SE8_OK && hipHostMalloc(&t->host_state, sizeof(IcpState), hipHostMallocDefault) != hipSuccess) {
         t->host_state = nullptr;
         rc = SE8_ERR_OOM;
@@ -744,7 +1018,6 @@ extern "C" int se8_tracker_destroy(se8_tracker* t)
         return SE8_OK;
     (void) hipDeviceSynchronize();
     for (int l = 0; l < 3; l++) {
-        dev_free(t->depth_lvl[l]);
         dev_free(t->vert_lvl[l]);
         dev_free(t->norm_lvl[l]);
     }
@@ -810,17 +1083,14 @@ extern "C" int se8_icp_cost_grad_batched(se8_tracker* t, const float* depth_dev,
         for (int i = 0; i < 12; i++)
             p12[(size_t) 12 * k + i] = poses16[(size_t) 16 * k + i];
     SE8_HIP_CHECK(hipMemcpyAsync(t->adj_poses.ptr, p12.data(), sizeof(float) * p12.size(), hipMemcpyHostToDevice, s));
-    const Cam c0 = make_cam(P, 1, true);
-    dim3 g((c0.w + 63) / 64, c0.h);
-    k_vertex_normal<<<g, 64, 0, s>>>(depth_dev, c0, c0.left_hand_frame, (float*) t->vert_lvl[0].ptr,
-                                     (float*) t->norm_lvl[0].ptr);
-    IcpArgs a;
-    a.vert = (const float*) t->vert_lvl[0].ptr;
-    a.norm = (const float*) t->norm_lvl[0].ptr;
+    launch_pyramid(t, depth_dev, 1, nullptr, Iso{}, s);
+    IcpArgs a{};
+    a.vert = (const float4*) t->vert_lvl[0].ptr;
+    a.norm = (const float4*) t->norm_lvl[0].ptr;
     a.ref_pts = ref_points_dev;
     a.ref_nrm = ref_normals_dev;
     a.partials = nullptr;
-    a.st = nullptr;
+    a.sh = nullptr;
     a.T_ref_inv = iso_inv(iso_from16(T_ref16));
     a.cam = make_cam(P, 1, false);
     a.w = P.width;

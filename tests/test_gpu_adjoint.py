@@ -1,8 +1,9 @@
 """Differentiation workload on the GPU: 64 candidate poses around a query frame's ground truth
 (t ~ N(0, 3 cm), r ~ N(0, 2 deg), seeded).  The adjoint of the point-to-plane cost is checked
-against FP64 central differences (eps 1e-5) and against the reference tracker's J^T r; the adjoint
-of the raycast depth against the FP64 oracle's implicit-function gradient and FP64 central
-differences; then 20 batched Gauss-Newton steps bring the candidates to the ground truth."""
+against the reference tracker's J^T r, whose per-pixel Jacobian is checked against FP64 central
+differences; the adjoint of the raycast depth against the FP64 oracle's implicit-function gradient
+and FP64 central differences (eps 1e-5); then 20 batched Gauss-Newton steps bring the candidates
+to the minimum of the cost, which is the reference tracker's fixed point."""
 import numpy as np
 import pytest
 
@@ -69,24 +70,38 @@ def test_cost_gradient_is_reference_jtr_and_fd(scene):
         assert sums[28] == inl[k]
         np.testing.assert_allclose(grad[k], -sums[1:7], rtol=2e-3, atol=2e-3 * np.abs(sums[1:7]).max())
         np.testing.assert_allclose(cost[k], 0.5 * sums[0], rtol=2e-3)
-    # FP64 central differences of 0.5 * sum e^2 (association held fixed by the small eps)
+    # FP64 central differences, per pixel.  The cost is piecewise smooth: a pixel whose projective
+    # association or inlier test flips between T - eps and T + eps contributes a jump, and at
+    # 640x480 about eps * 500 px/rad of the pixels sit that close to a pixel border.  Such pixels
+    # are found without using J, by their second difference (a smooth residual has
+    # |e+ + e- - 2 e0| = O(eps^2)), and left out of the comparison.
+    rp64, rn64 = rp.astype(np.float64), rn.astype(np.float64)
     for k in (0, 17, 42):
-        T0 = s["cands"][k]
+        T0 = s["cands"][k].astype(np.float64)
 
-        def c64(xi):
-            T = O.se3_exp(xi, "f64") @ T0
-            _, _, _, sm = O.icp_track_reduce(in64, nr64, rp.astype(np.float64), rn.astype(np.float64), pin, T,
-                                             s["T_gt"], precision="f64")
-            return 0.5 * sm[0], sm[28]
+        def track64(xi):
+            res, err, J, _ = O.icp_track_reduce(in64, nr64, rp64, rn64, pin, O.se3_exp(xi, "f64") @ T0, s["T_gt"],
+                                                precision="f64")
+            return res == 1, err, J
 
-        eps = 1e-5
-        fd = []
-        for e in np.eye(6):
-            (cp, npl), (cm, nm) = c64(eps * e), c64(-eps * e)
-            fd.append((cp - cm) / (2 * eps))
-        fd = np.array(fd)
-        print("pose", k, "adjoint", grad[k], "fd64", fd)
-        np.testing.assert_allclose(grad[k], fd, rtol=5e-3, atol=5e-3 * np.abs(fd).max())
+        ok0, e0, J0 = track64(np.zeros(6))
+        eps = 1e-6
+        de = np.zeros(e0.shape + (6,))
+        smooth = ok0.copy()
+        for a in range(6):
+            xi = np.zeros(6)
+            xi[a] = eps
+            (okp, ep, _), (okm, em, _) = track64(xi), track64(-xi)
+            de[..., a] = (ep - em) / (2 * eps)
+            smooth &= okp & okm & (np.abs(ep + em - 2 * e0) < 1e-9)
+        assert smooth.sum() > 0.98 * ok0.sum()
+        # de/dxi = -J on every smooth pixel (left perturbation T <- exp(xi) T, xi = [v; w])
+        rel = np.linalg.norm(de[smooth] + J0[smooth], axis=-1) / np.linalg.norm(J0[smooth], axis=-1)
+        assert np.median(rel) < 1e-6 and np.max(rel) < 1e-3
+        fd = (e0[smooth][:, None] * de[smooth]).sum(0)
+        adj = -(e0[smooth][:, None] * J0[smooth]).sum(0)
+        print("pose", k, "smooth pixels", smooth.sum(), "of", ok0.sum(), "fd64", fd, "adjoint64", adj)
+        np.testing.assert_allclose(adj, fd, rtol=1e-5, atol=1e-6 * np.abs(fd).max())
 
 
 def test_autograd_function_matches_kernel(scene):
@@ -130,22 +145,41 @@ def test_depth_gradient_vs_fp64_oracle(scene):
         assert np.median(rel_fd) < 2e-2
 
 
-def test_batched_gauss_newton_refinement_converges(scene):
-    s = scene
-    T = s["cands"].copy()
-    err0 = np.linalg.norm(T[:, :3, 3] - s["T_gt"][:3, 3], axis=1)
-    for it in range(20):
-        cost, grad, jtj, inl = s["diff"].icp_cost_grad(s["tracker"], s["d"], s["ref_p"], s["ref_n"], s["T_gt"], T)
-        for k in range(N_POSES):
+def _gauss_newton(s, T, steps):
+    T = T.copy()
+    for _ in range(steps):
+        _, grad, jtj, _ = s["diff"].icp_cost_grad(s["tracker"], s["d"], s["ref_p"], s["ref_n"], s["T_gt"], T)
+        for k in range(len(T)):
             A = np.zeros((6, 6))
             A[np.triu_indices(6)] = jtj[k]
             A = A + A.T - np.diag(np.diag(A))
             step = np.linalg.solve(A.astype(np.float64), -grad[k].astype(np.float64))
             T[k] = s["diff"].se3_exp(step) @ T[k]
-    err = np.linalg.norm(T[:, :3, 3] - s["T_gt"][:3, 3], axis=1)
-    rot = [np.degrees(np.arccos(np.clip((np.trace(T[k][:3, :3].T @ s["T_gt"][:3, :3]) - 1) / 2, -1, 1)))
-           for k in range(N_POSES)]
-    print(f"initial error median {np.median(err0) * 100:.2f} cm; after 20 steps median {np.median(err) * 1000:.3f} mm,"
-          f" max {err.max() * 1000:.3f} mm, rot median {np.median(rot):.4f} deg")
-    assert np.median(err) < 2e-3
-    assert np.mean(err < 5e-3) > 0.9
+    return T
+
+
+def _rot_deg(A, B):
+    return np.degrees(np.arccos(np.clip((np.trace(A[:3, :3].T @ B[:3, :3]) - 1) / 2, -1, 1)))
+
+
+def test_batched_gauss_newton_refinement_converges(scene):
+    """The candidates descend to the minimum of the point-to-plane cost.  That minimum is the pose
+    the reference tracker converges to from the ground truth; it sits a few mm from the ground truth
+    because the model is a 5-frame fusion of noisy depth."""
+    s = scene
+    T_star = _gauss_newton(s, s["T_gt"][None], 20)[0]
+    ok, T_trk, _ = s["tracker"].track(s["d"], s["T_gt"], s["ref_p"], s["ref_n"])
+    assert ok
+    off_trk = np.linalg.norm(T_trk[:3, 3] - T_star[:3, 3])
+    off_gt = np.linalg.norm(T_star[:3, 3] - s["T_gt"][:3, 3])
+    print(f"cost minimum: {off_gt * 1000:.3f} mm from ground truth, {off_trk * 1000:.4f} mm from the tracker's pose")
+    assert off_trk < 1e-4 and _rot_deg(T_trk, T_star) < 0.01
+    assert off_gt < 5e-3
+    T = _gauss_newton(s, s["cands"], 20)
+    err0 = np.linalg.norm(s["cands"][:, :3, 3] - T_star[:3, 3], axis=1)
+    err = np.linalg.norm(T[:, :3, 3] - T_star[:3, 3], axis=1)
+    rot = np.array([_rot_deg(T[k], T_star) for k in range(N_POSES)])
+    print(f"initial distance median {np.median(err0) * 100:.2f} cm; after 20 steps median {np.median(err) * 1000:.4f} mm,"
+          f" {np.mean(err < 1e-3) * 100:.0f}% within 1 mm, rot median {np.median(rot):.5f} deg")
+    assert np.median(err) < 1e-4 and np.median(rot) < 1e-3
+    assert np.mean(err < 1e-3) > 0.7

@@ -58,5 +58,10 @@ def test_fp16_tsdf_rms_vs_fp64_oracle():
     print(f"TSDF RMS vs FP64 oracle: fp32 {rms32:.3e} m, fp16 {rms16:.3e} m over {both.sum()} voxels")
     assert rms16 <= 2e-4
     assert rms32 <= rms16
-    # fp16 rounding bound: half an fp16 ulp of |tsdf| <= 1 per update, re-rounded each frame
-    assert np.abs(e16).max() < n * MU * 2.0**-11
+    # fp16 rounding bound against the fp32 path, whose update history (allocation, band test,
+    # weights) is the same voxel for voxel: at most half an fp16 ulp of |tsdf| <= 1 (2^-12) per
+    # update, shrunk by w/(w+1) at the next one, so below n * 2^-12 after n frames.  (Against the
+    # FP64 oracle a voxel on the edge of the band can take one more or one fewer update, so the
+    # bound is stated against fp32; the FP64 comparison is the RMS above.)
+    upd = w32 > 0
+    assert np.abs(t16 - t32)[upd].max() < n * 2.0**-12
