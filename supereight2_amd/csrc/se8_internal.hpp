@@ -338,6 +338,20 @@ __device__ __forceinline__ int lookup_block(const MapView& m, int x, int y, int 
     return m.brick_slots[brick * 512 + (bx & msk) + ((by & msk) << 3) + ((bz & msk) << 6)];
 }
 
+// lookup_block without a branch on the brick: both loads are always issued (the second from brick
+// 0 when the brick is missing, whose result is then discarded), so a caller that looks up several
+// blocks gets all their loads in flight together.
+__device__ __forceinline__ int lookup_block_nb(const MapView& m, int x, int y, int z)
+{
+    const int wrap = m.size - 1;
+    const int bx = (x & wrap) >> 3, by = (y & wrap) >> 3, bz = (z & wrap) >> 3;
+    const int s = m.brick_shift;
+    const int brick = m.brick_top[((bz >> s) * m.bricks_side + (by >> s)) * m.bricks_side + (bx >> s)];
+    const int msk = (1 << s) - 1;
+    const int slot = m.brick_slots[max(brick, 0) * 512 + (bx & msk) + ((by & msk) << 3) + ((bz & msk) << 6)];
+    return brick < 0 ? -1 : slot;
+}
+
 __device__ __forceinline__ void load_voxel(const MapView& m, int slot, int vi, float& tsdf, float& weight)
 {
     if (m.storage == SE8_STORAGE_F32) {

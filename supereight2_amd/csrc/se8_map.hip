@@ -12,6 +12,7 @@
 //   k_build_blocks block slots, brick index, AABB and the "new" entries of the update list
 //   k_update       Updater<Map<TSDF,Single>> (singleres_tsdf_updater_impl.hpp:14-132), one 8^3
 //                  block per 512-thread workgroup
+#include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -655,6 +656,16 @@ static int device_count_sm()
     return cus;
 }
 
+// Workgroups of `threads` threads of kernel k that fit on the device at once.
+template<class K>
+static int resident_grid(K k, int threads)
+{
+    int per_cu = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k, threads, 0) != hipSuccess || per_cu <= 0)
+        per_cu = 1;
+    return per_cu * device_count_sm();
+}
+
 
 // Sorts the candidate keys and builds nodes and blocks.  The live count is *n_dev (device memory),
 // at most cap; nothing here waits for the device.
@@ -937,16 +948,18 @@ extern "C" int se8_integrate_depth(se8_map* m, int timestamp, const float* depth
                                (int*) m->update_list.ptr, 1, s);
     if (rc != SE8_OK)
         return rc;
-    SE8_HIP_CHECK(hipEventRecord(m->ev[3], s));
+    // ev[3] / ev[4] are stamped at the start and end of the update kernel itself
     const Iso T_CV = compute_T_CV(m, T_WC);
+    const int* ulist = (const int*) m->update_list.ptr;
+    // one resident wave of workgroups: the grid-stride loop then never leaves a second, partial
+    // round of workgroups waiting for the first to finish
     if (m->view.storage == SE8_STORAGE_F32)
-        k_update<SE8_STORAGE_F32><<<cus * 4, 512, 0, s>>>(m->view, (const int*) m->update_list.ptr, T_CV, cam,
-                                                          depth_dev, timestamp);
+        hipExtLaunchKernelGGL(k_update<SE8_STORAGE_F32>, dim3(resident_grid(k_update<SE8_STORAGE_F32>, 512)),
+                              dim3(512), 0, s, m->ev[3], m->ev[4], 0, m->view, ulist, T_CV, cam, depth_dev, timestamp);
     else
-        k_update<SE8_STORAGE_F16><<<cus * 4, 512, 0, s>>>(m->view, (const int*) m->update_list.ptr, T_CV, cam,
-                                                          depth_dev, timestamp);
+        hipExtLaunchKernelGGL(k_update<SE8_STORAGE_F16>, dim3(resident_grid(k_update<SE8_STORAGE_F16>, 512)),
+                              dim3(512), 0, s, m->ev[3], m->ev[4], 0, m->view, ulist, T_CV, cam, depth_dev, timestamp);
     SE8_HIP_CHECK(hipGetLastError());
-    SE8_HIP_CHECK(hipEventRecord(m->ev[4], s));
     SE8_HIP_CHECK(hipMemcpyAsync(m->host_counters, m->counters.ptr, sizeof(int) * kNumCounters,
                                  hipMemcpyDeviceToHost, s));
     SE8_HIP_CHECK(hipEventRecord(m->ev[7], s));

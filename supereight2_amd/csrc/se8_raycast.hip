@@ -10,6 +10,7 @@
 //   2. the TSDF march of raycast<TSDF> (raycaster_impl.hpp:456-533) with O(1) voxel lookups
 //      through the brick index instead of octree descents;
 //   3. the normal from the single-res 32-sample gradient (visitor_impl.hpp:1094-1251).
+#include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
 
 #include <cstring>
@@ -22,6 +23,12 @@ namespace se8 {
 constexpr int kStackDepth = 23;  // CAST_STACK_DEPTH (voxel_block_ray_iterator.hpp:15)
 constexpr int kMaxLevels = 16;   // log2(size / 8) for the largest supported octree (2^19)
 constexpr int kRayThreads = 128; // 16 x 8 pixel tile
+
+// Dynamic LDS of a raycast workgroup: the traversal stack of every thread.
+inline size_t ray_stack_bytes(const MapView& m)
+{
+    return (size_t) (m.max_scale - kBlockScale) * kRayThreads * (sizeof(int) + sizeof(float));
+}
 
 __device__ __forceinline__ int f2i(float f)
 {
@@ -78,6 +85,15 @@ __device__ Span first_block(const MapView& m, const float oW[3], const float dW[
     for (int l = 0; l < lg; l++)
         st_parent[l * stride] = -1;
     int parent = 0;
+    // the 8 child entries of `parent` (one 32-byte row), reloaded only when parent changes, so
+    // advancing between siblings costs no memory access
+    int4 row_lo, row_hi;
+    auto load_row = [&](int p) {
+        const int4* r = reinterpret_cast<const int4*>(m.node_child + (size_t) p * 8);
+        row_lo = r[0];
+        row_hi = r[1];
+    };
+    load_row(parent);
     int scale = kStackDepth - 1;
     float scale_exp2 = 0.5f;
     // the traversal visits each octant at most once; the cap only guards against a corrupt octree
@@ -87,7 +103,9 @@ __device__ Span first_block(const MapView& m, const float oW[3], const float dW[
         for (int i = 0; i < 3; i++)
             tcorner[i] = pos[i] * tc[i] - tb[i];
         const float tc_max = fminf(fminf(tcorner[0], tcorner[1]), tcorner[2]);
-        const int child = m.node_child[parent * 8 + (idx ^ octant_mask ^ 7)];
+        const int ci = idx ^ octant_mask ^ 7;
+        const int4 half_row = (ci & 4) ? row_hi : row_lo;
+        const int child = (ci & 2) ? ((ci & 1) ? half_row.w : half_row.z) : ((ci & 1) ? half_row.y : half_row.x);
         if (scale == min_scale && child >= 0)
             break;
         if (child >= 0 && t_min <= t_max) {
@@ -104,6 +122,7 @@ __device__ Span first_block(const MapView& m, const float oW[3], const float dW[
             }
             h = tc_max;
             parent = child;
+            load_row(parent);
             idx = 0;
             scale--;
             scale_exp2 = half;
@@ -140,6 +159,7 @@ __device__ Span first_block(const MapView& m, const float oW[3], const float dW[
             }
             scale_exp2 = i2f((scale - kStackDepth + 127) << 23);
             parent = st_parent[l * stride];
+            load_row(parent);
             t_max = st_tmax[l * stride];
             int sh[3];
 #pragma unroll
@@ -198,6 +218,57 @@ __device__ __forceinline__ void get_data(const MapView& m, const float p[3], flo
     get_voxel(m, v[0], v[1], v[2], tsdf, weight);
 }
 
+// Field values of the 8 voxels base + {0,1}^3 (x fastest) and whether all are allocated and
+// observed (weight != 0).  Every block lookup and voxel load is issued before any result is
+// tested, so the set costs the round trips of one lookup, not eight in sequence.
+__device__ __forceinline__ bool gather_corners(const MapView& m, const int base[3], float d[8])
+{
+    int slot[8];
+#pragma unroll
+    for (int k = 0; k < 8; k++)
+        slot[k] = lookup_block_nb(m, base[0] + (k & 1), base[1] + ((k >> 1) & 1), base[2] + ((k >> 2) & 1));
+    bool ok = true;
+#pragma unroll
+    for (int k = 0; k < 8; k++) {
+        const int x = base[0] + (k & 1), y = base[1] + ((k >> 1) & 1), z = base[2] + ((k >> 2) & 1);
+        float w;
+        load_voxel(m, max(slot[k], 0), (x & 7) + ((y & 7) << 3) + ((z & 7) << 6), d[k], w);
+        ok = ok && slot[k] >= 0 && w != 0.f;
+    }
+    return ok;
+}
+
+// getData<Safe::On> along a march: consecutive samples usually fall in the same block, so the
+// block slot of the previous sample is kept and the brick/slot loads are skipped when it repeats.
+struct SlotCache {
+    uint64_t key = ~0ull;
+    int slot = -1;
+};
+
+__device__ __forceinline__ void get_data_cached(const MapView& m, const float p[3], float& tsdf, float& weight,
+                                                SlotCache& cache)
+{
+    int v[3];
+    if (!point_to_voxel(m, p, v)) {
+        tsdf = 1.f;
+        weight = 0.f;
+        return;
+    }
+    const int wrap = m.size - 1;
+    const uint64_t key = (uint64_t) ((v[0] & wrap) >> 3) | ((uint64_t) ((v[1] & wrap) >> 3) << 21)
+        | ((uint64_t) ((v[2] & wrap) >> 3) << 42);
+    if (key != cache.key) {
+        cache.key = key;
+        cache.slot = lookup_block(m, v[0], v[1], v[2]);
+    }
+    if (cache.slot < 0) {
+        tsdf = 1.f;
+        weight = 0.f;
+        return;
+    }
+    load_voxel(m, cache.slot, (v[0] & 7) + ((v[1] & 7) << 3) + ((v[2] & 7) << 6), tsdf, weight);
+}
+
 // single-res getInterp (visitor_impl.hpp:976-1024) from voxel coordinates.  The reference's
 // octree-AABB test is implied by the requirement that the block of base be allocated.
 __device__ bool field_interp_vox(const MapView& m, const float vf[3], float& out)
@@ -214,18 +285,8 @@ __device__ bool field_interp_vox(const MapView& m, const float vf[3], float& out
         if (base[i] + 1 >= m.size || base[i] < 0)
             return false;
     float d[8];
-#pragma unroll
-    for (int k = 0; k < 8; k++) {
-        const int x = base[0] + (k & 1), y = base[1] + ((k >> 1) & 1), z = base[2] + ((k >> 2) & 1);
-        const int slot = lookup_block(m, x, y, z);
-        if (slot < 0)
-            return false;
-        float t, w;
-        load_voxel(m, slot, (x & 7) + ((y & 7) << 3) + ((z & 7) << 6), t, w);
-        if (w == 0.f)
-            return false;
-        d[k] = t;
-    }
+    if (!gather_corners(m, base, d))
+        return false;
     float t[3], tcm[3];
 #pragma unroll
     for (int i = 0; i < 3; i++) {
@@ -235,13 +296,6 @@ __device__ bool field_interp_vox(const MapView& m, const float vf[3], float& out
     out = ((d[0] * tcm[0] + d[1] * t[0]) * tcm[1] + (d[2] * tcm[0] + d[3] * t[0]) * t[1]) * tcm[2]
         + ((d[4] * tcm[0] + d[5] * t[0]) * tcm[1] + (d[6] * tcm[0] + d[7] * t[0]) * t[1]) * t[2];
     return true;
-}
-
-__device__ __forceinline__ bool valid_field(const MapView& m, int x, int y, int z, float& f)
-{
-    float w;
-    get_voxel(m, x, y, z, f, w);
-    return w != 0.f;
 }
 
 // single-res getFieldGrad (visitor_impl.hpp:1094-1251): trilinear blend of the central differences
@@ -261,7 +315,36 @@ __device__ bool field_grad_vox(const MapView& m, const float vf[3], float g[3])
         c[i][2] = min(base[i] + 1, m.size - 1);
         c[i][3] = min(base[i] + 2, m.size - 1);
     }
-    if (lookup_block(m, base[0], base[1], base[2]) < 0)
+    // The central differences read the 32 voxels c[0][i], c[1][j], c[2][k] of the 4x4x4
+    // neighbourhood with at most one of i, j, k on an outer layer (0 or 3).  All 32 lookups and
+    // loads are issued before any is tested; the result is undefined if any is invalid or the
+    // block of base is missing, as in the reference.
+    constexpr int kOuter = 0x9; // bit i set: index i is an outer layer
+    int slot[4][4][4];
+#pragma unroll
+    for (int i = 0; i < 4; i++)
+#pragma unroll
+        for (int j = 0; j < 4; j++)
+#pragma unroll
+            for (int k = 0; k < 4; k++)
+                if (((kOuter >> i) & 1) + ((kOuter >> j) & 1) + ((kOuter >> k) & 1) <= 1)
+                    slot[i][j][k] = lookup_block_nb(m, c[0][i], c[1][j], c[2][k]);
+    const int base_slot = lookup_block_nb(m, base[0], base[1], base[2]);
+    float F[4][4][4];
+    bool ok = base_slot >= 0;
+#pragma unroll
+    for (int i = 0; i < 4; i++)
+#pragma unroll
+        for (int j = 0; j < 4; j++)
+#pragma unroll
+            for (int k = 0; k < 4; k++)
+                if (((kOuter >> i) & 1) + ((kOuter >> j) & 1) + ((kOuter >> k) & 1) <= 1) {
+                    const int s = slot[i][j][k];
+                    float w;
+                    load_voxel(m, max(s, 0), (c[0][i] & 7) + ((c[1][j] & 7) << 3) + ((c[2][k] & 7) << 6), F[i][j][k], w);
+                    ok = ok && s >= 0 && w != 0.f;
+                }
+    if (!ok)
         return false;
     const float rx = 1.f - fac[0], ry = 1.f - fac[1], rz = 1.f - fac[2];
 #pragma unroll
@@ -270,18 +353,15 @@ __device__ bool field_grad_vox(const MapView& m, const float vf[3], float g[3])
 #pragma unroll
         for (int k = 0; k < 8; k++) {
             const int ck[3] = {k & 1, (k >> 1) & 1, (k >> 2) & 1};
-            int vp[3], vm[3];
+            int ip[3], im[3];
 #pragma unroll
             for (int i = 0; i < 3; i++) {
-                vp[i] = c[i][1 + ck[i]];
-                vm[i] = c[i][1 + ck[i]];
+                ip[i] = 1 + ck[i];
+                im[i] = 1 + ck[i];
             }
-            vp[a] = c[a][2 + ck[a]];
-            vm[a] = c[a][ck[a]];
-            float fp, fm;
-            if (!valid_field(m, vp[0], vp[1], vp[2], fp) || !valid_field(m, vm[0], vm[1], vm[2], fm))
-                return false;
-            cd[k] = fp - fm;
+            ip[a] = 2 + ck[a];
+            im[a] = ck[a];
+            cd[k] = F[ip[0]][ip[1]][ip[2]] - F[im[0]][im[1]][im[2]];
         }
         const float v = ((cd[0] * rx + cd[1] * fac[0]) * ry + (cd[2] * rx + cd[3] * fac[0]) * fac[1]) * rz
             + ((cd[4] * rx + cd[5] * fac[0]) * ry + (cd[6] * rx + cd[7] * fac[0]) * fac[1]) * fac[2];
@@ -309,13 +389,14 @@ __device__ bool march(const MapView& m, const float o[3], const float d[3], floa
     for (int i = 0; i < 3; i++)
         p[i] = o[i] + d[i] * t;
     float f_t, w;
-    get_data(m, p, f_t, w);
+    SlotCache cache;
+    get_data_cached(m, p, f_t, w, cache);
     float f_tt = 0.f;
     if (!(f_t >= 0.f))
         return false;
     for (; t < t_far; t += stepsize) {
         float f_new;
-        get_data(m, p, f_new, w);
+        get_data_cached(m, p, f_new, w, cache);
         if (w == 0.f) {
             stepsize = largestep;
 #pragma unroll
@@ -371,17 +452,8 @@ __device__ bool interp_and_grad(const MapView& m, const float pW[3], float& f, f
             return false;
     }
     float d[8];
-#pragma unroll
-    for (int k = 0; k < 8; k++) {
-        const int x = base[0] + (k & 1), y = base[1] + ((k >> 1) & 1), z = base[2] + ((k >> 2) & 1);
-        const int slot = lookup_block(m, x, y, z);
-        if (slot < 0)
-            return false;
-        float w;
-        load_voxel(m, slot, (x & 7) + ((y & 7) << 3) + ((z & 7) << 6), d[k], w);
-        if (w == 0.f)
-            return false;
-    }
+    if (!gather_corners(m, base, d))
+        return false;
 #pragma unroll
     for (int i = 0; i < 3; i++) {
         t[i] = bf[i] - floorf(bf[i]);
@@ -451,8 +523,10 @@ __device__ bool refine_root(const MapView& m, const float o[3], const float d[3]
 __global__ __launch_bounds__(kRayThreads) void k_raycast_depth_grad(MapView m, Cam cam, const float* poses12,
                                                                      float* depth, float* grad, uint8_t* valid)
 {
-    __shared__ int st_parent[kMaxLevels * kRayThreads];
-    __shared__ float st_tmax[kMaxLevels * kRayThreads];
+    // traversal stack: (max_scale - 3) levels of {parent, t_max} per thread, sized at launch
+    extern __shared__ int ray_stack[];
+    int* st_parent = ray_stack;
+    float* st_tmax = reinterpret_cast<float*>(ray_stack + (m.max_scale - kBlockScale) * kRayThreads);
     const int k = blockIdx.z;
     const int t = threadIdx.x;
     const int x = blockIdx.x * 16 + (t & 15);
@@ -506,8 +580,10 @@ __global__ __launch_bounds__(kRayThreads) void k_raycast_depth_grad(MapView m, C
 __global__ __launch_bounds__(kRayThreads) void k_raycast(MapView m, Cam cam, Iso T_WS, float* points, float* normals,
                                                          int8_t* scale_out)
 {
-    __shared__ int st_parent[kMaxLevels * kRayThreads];
-    __shared__ float st_tmax[kMaxLevels * kRayThreads];
+    // traversal stack: (max_scale - 3) levels of {parent, t_max} per thread, sized at launch
+    extern __shared__ int ray_stack[];
+    int* st_parent = ray_stack;
+    float* st_tmax = reinterpret_cast<float*>(ray_stack + (m.max_scale - kBlockScale) * kRayThreads);
     const int t = threadIdx.x;
     const int x = blockIdx.x * 16 + (t & 15);
     const int y = blockIdx.y * 8 + (t >> 4);
@@ -611,11 +687,14 @@ static int run_query(const se8_map* m, const float* pts, int64_t n, int mode, st
     return rc;
 }
 
+// ev_start / ev_stop, when given, are stamped at the start and end of the kernel itself (not at
+// enqueue time), so their difference is the kernel's execution time.
 int launch_raycast(const se8_map* m, const Cam& cam, const Iso& T_WS, float* points, float* normals, int8_t* scale,
-                   hipStream_t s)
+                   hipStream_t s, hipEvent_t ev_start, hipEvent_t ev_stop)
 {
-    dim3 grid((cam.w + 15) / 16, (cam.h + 7) / 8);
-    k_raycast<<<grid, kRayThreads, 0, s>>>(m->view, cam, T_WS, points, normals, scale);
+    const dim3 grid((cam.w + 15) / 16, (cam.h + 7) / 8);
+    hipExtLaunchKernelGGL(k_raycast, grid, dim3(kRayThreads), (uint32_t) ray_stack_bytes(m->view), s, ev_start,
+                          ev_stop, 0, m->view, cam, T_WS, points, normals, scale);
     SE8_HIP_CHECK(hipGetLastError());
     return SE8_OK;
 }
@@ -634,9 +713,7 @@ extern "C" int se8_raycast(const se8_map* m, const se8_pinhole* sensor, const fl
     se8_map* mm = const_cast<se8_map*>(m);
     const Cam cam = make_cam(*sensor, 1, false);
     harvest_raycast(mm, false);
-    SE8_HIP_CHECK(hipEventRecord(mm->ev[5], s));
-    int rc = launch_raycast(m, cam, iso_from16(T_WS16), points_dev, normals_dev, scale_dev, s);
-    SE8_HIP_CHECK(hipEventRecord(mm->ev[6], s));
+    int rc = launch_raycast(m, cam, iso_from16(T_WS16), points_dev, normals_dev, scale_dev, s, mm->ev[5], mm->ev[6]);
     mm->ray_pending = true;
     return rc;
 }
@@ -706,7 +783,7 @@ extern "C" int se8_raycast_depth_grad_batched(const se8_map* m, const se8_pinhol
         return SE8_ERR_HIP;
     }
     dim3 grid((cam.w + 15) / 16, (cam.h + 7) / 8, n_poses);
-    k_raycast_depth_grad<<<grid, kRayThreads, 0, s>>>(m->view, cam, (const float*) dp.ptr, depth_dev, grad_dev,
+    k_raycast_depth_grad<<<grid, kRayThreads, ray_stack_bytes(m->view), s>>>(m->view, cam, (const float*) dp.ptr, depth_dev, grad_dev,
                                                       valid_dev);
     const hipError_t e1 = hipGetLastError();
     const hipError_t e2 = hipStreamSynchronize(s);

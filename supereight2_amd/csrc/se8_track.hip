@@ -900,7 +900,7 @@ static int run_track(se8_tracker* tr, const float* depth, float T16[16], const f
 }
 
 int launch_raycast(const se8_map* m, const Cam& cam, const Iso& T_WS, float* points, float* normals, int8_t* scale,
-                   hipStream_t s);
+                   hipStream_t s, hipEvent_t ev_start, hipEvent_t ev_stop);
 
 // ----------------------------------------------------------------------------------------------
 // Batched point-to-plane cost and its adjoint (differentiation growth pattern).
@@ -1055,7 +1055,8 @@ extern "C" int se8_track_raycast(se8_tracker* t, const se8_map* m, const float* 
         return rc;
     hipStream_t s = (hipStream_t) stream;
     const Cam cam = make_cam(t->sensor, 1, false);
-    rc = launch_raycast(m, cam, iso_from16(T_WS), (float*) t->ref_points.ptr, (float*) t->ref_normals.ptr, nullptr, s);
+    rc = launch_raycast(m, cam, iso_from16(T_WS), (float*) t->ref_points.ptr, (float*) t->ref_normals.ptr, nullptr, s,
+                        nullptr, nullptr);
     if (rc != SE8_OK)
         return rc;
     return run_track(t, depth_dev, T_WS, (const float*) t->ref_points.ptr, (const float*) t->ref_normals.ptr, ok,
