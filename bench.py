@@ -18,7 +18,8 @@ Workload (one step = one frame of the KinectFusion loop of the reference app, ma
 The FP32 path (reference arithmetic: fp32 TSDF + fp32 ICP reduction) gives `value`; the
 FP16-TSDF + compensated-ICP variant runs on the same frames and is reported beside it.
 
-Prints one JSON line (see DESIGN.md for the field definitions).
+Prints one JSON line: the frames/s of the FP32 path as `value`, the FP16 variant beside it, the
+roofline of the TSDF update kernel and the CPU-baseline rate of the oracle on a prefix of the frames.
 """
 from __future__ import annotations
 
@@ -164,7 +165,7 @@ def cpu_model():
 def main():
     args = parse()
     if args.gpus != 1:
-        print(f"bench.py runs on one GPU only; --gpus {args.gpus} is not supported (see DESIGN.md, sharding).",
+        print(f"bench.py runs on one GPU only; --gpus {args.gpus} is not supported: the path runs on a single GPU.",
               file=sys.stderr)
         sys.exit(2)
     import torch
@@ -202,6 +203,15 @@ def main():
         return nbytes, gbs
 
     upd_bytes, upd_gbs = update_roofline(r32, 8)
+    # HBM traffic per launch of the same kernel on this workload, measured by the FETCH_SIZE and
+    # WRITE_SIZE passes recorded in profiles/k_update_tracking300_fp32_fp16.md (scaled by a 256 MiB
+    # copy); rescaled by this run's update-list length, null when that record is absent
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", "k_update_traffic.json")
+    if os.path.exists(tpath):
+        with open(tpath) as f:
+            tr = json.load(f)
+        traffic = tr["traffic_bytes_per_launch"] * r32["update_blocks_per_frame"] / tr["update_blocks_per_frame"]
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -238,7 +248,7 @@ def main():
         "fp16_tsdf": fp16,
         "fp16_over_fp32": (r16["fps"] / r32["fps"]) if r16 is not None and r32["fps"] > 0 else None,
         "roofline": {"bound": "hbm", "kernel": "k_update", "achieved": upd_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": upd_gbs / HBM_PEAK_GBS, "traffic": None, "bytes_per_launch": upd_bytes},
+                     "frac": upd_gbs / HBM_PEAK_GBS, "traffic": traffic, "bytes_per_launch": upd_bytes},
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))

@@ -1,7 +1,7 @@
 """ctypes loader of the CPU oracle (oracle/se8_oracle.c).
 
 TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and bench.py's
-cpu_baseline leg, as the checker.  The product library never uses it.
+cpu_baseline leg, to check results.  The product library never uses it.
 
 Two builds exist: precision="f32" follows the fp32 operation order of the GPU path (parity),
 precision="f64" is the FP64 twin used to report precision errors.

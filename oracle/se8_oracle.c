@@ -3,14 +3,14 @@
  *
  * TEST INFRASTRUCTURE ONLY.  This file is a plain-C restatement of the reference
  * algorithm, used by tests/, by __graft_entry__.smoke() and by the cpu_baseline leg
- * of bench.py as the checker.  The product (libse8.so) never links, loads or calls it.
+ * of bench.py, to check results.  The product (libse8.so) never links, loads or calls it.
  *
  * It is compiled twice by oracle/Makefile:
  *   liboracle_f32.so  (SE8_REAL=float)  - same fp32 operation order as the GPU path
  *   liboracle_f64.so  (SE8_REAL=double) - FP64 twin used to report precision errors
  * with -O2 -ffp-contract=off -fno-fast-math and OpenMP loops where the reference has them.
  *
- * Floating-point evaluation order (shared with the GPU kernels, see DESIGN.md):
+ * Floating-point evaluation order (shared with the GPU kernels):
  *   dot3(a,b)       = a0*b0 + (a1*b1 + a2*b2)         (Eigen's 3-term redux split)
  *   (R*v)_i         = R_i0*v0 + (R_i1*v1 + R_i2*v2)
  *   (T*p)_i         = (R*p)_i + t_i

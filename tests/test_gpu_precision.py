@@ -1,6 +1,6 @@
 """Precision variant: FP16 TSDF + uint16 weights against the FP32 path and the FP64 oracle.
 
-Bar (BASELINE north star): allocation bit-exact with the FP32 path; TSDF RMS error vs the FP64
+Bar: allocation bit-exact with the FP32 path; TSDF RMS error vs the FP64
 oracle <= 2e-4 m (error in metres = |dtsdf| * mu); weights identical."""
 import numpy as np
 import pytest
