@@ -35,6 +35,7 @@ extern "C" {
 typedef void* se8_stream; /* a hipStream_t */
 typedef struct se8_map se8_map;
 typedef struct se8_tracker se8_tracker;
+typedef struct se8_lidar se8_lidar;
 
 /* se::Map::Config (include/se/map/map.hpp:42-71).  T_MW is a pure translation (the reference's
  * raycaster assumes no rotation, include/se/map/impl/raycaster_impl.hpp:465); the default
@@ -113,6 +114,29 @@ int se8_map_get_info(const se8_map* map, se8_map_info* out);
  * RaycastCarver allocation + single-res Updater.  depth_dev is width*height floats, row-major. */
 int se8_integrate_depth(se8_map* map, int timestamp, const float* depth_dev, const se8_pinhole* sensor,
                         const float T_WC[16], se8_stream stream);
+
+/* se::OusterLidar::Config (include/se/sensor/ouster_lidar.hpp:24-44, sensor.hpp:30-45): one
+ * elevation and one azimuth offset per beam (height entries, degrees), elevations strictly
+ * descending (row 0 the highest beam).  The beam model (ouster_lidar.cpp:59-253) is evaluated by
+ * the library; its direction table is built once at creation. */
+typedef struct {
+    int width, height;
+    float near_plane, far_plane;
+    const float* beam_elevation_deg;
+    const float* beam_azimuth_deg;
+} se8_lidar_config;
+int se8_lidar_create(const se8_lidar_config* cfg, se8_lidar** out);
+int se8_lidar_destroy(se8_lidar* lidar);
+/* MapIntegrator::integrateDepth with an OusterLidar (map_integrator_impl.hpp:52-83; the sensor
+ * hooks of ouster_lidar_impl.hpp:15-32): range_dev is width*height floats of range |p| (metres,
+ * 0 = no return), row-major; T_WS is the sensor pose. */
+int se8_integrate_range(se8_map* map, int timestamp, const float* range_dev, const se8_lidar* lidar,
+                        const float T_WS[16], se8_stream stream);
+/* The beam model's project, as the updater calls it (sensor.model.project,
+ * singleres_tsdf_updater_impl.hpp:65; srl::projection::OusterLidar, not vendored), for n
+ * sensor-frame points (host arrays, n*3 floats in, n*2 floats out): in_image[i] = 1 where the
+ * projection succeeds; uv[i] holds the continuous image coordinates. */
+int se8_lidar_project(const se8_lidar* lidar, const float* points, int64_t n, float* uv, uint8_t* in_image);
 
 /* -------- raycast: se::raycaster::raycast_volume (include/se/map/raycaster.hpp:56-64,
  * impl/raycaster_impl.hpp:537-627).  points/normals are width*height*3 floats (x,y,z per pixel);

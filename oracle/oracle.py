@@ -121,6 +121,21 @@ def lib(precision: str = "f32") -> C.CDLL:
     L.or_get_data.argtypes = [P, R, R, R]
     L.or_integrate.argtypes = [P, C.c_int, C.POINTER(C.c_float), C.POINTER(Pinhole), R, C.POINTER(C.c_long)]
     L.or_raycast.argtypes = [P, C.POINTER(Pinhole), R, R, R, C.POINTER(C.c_int8)]
+    FP = C.POINTER(C.c_float)
+    L.or_lidar_create.restype = P
+    L.or_lidar_create.argtypes = [C.c_int, C.c_int, C.c_float, C.c_float, FP, FP]
+    L.or_lidar_destroy.argtypes = [P]
+    L.or_lidar_project.restype = C.c_int
+    L.or_lidar_project.argtypes = [P, R, R]
+    L.or_lidar_project_n.argtypes = [P, R, C.c_long, R, C.POINTER(C.c_uint8)]
+    L.or_lidar_backproject.argtypes = [P, C.c_int, C.c_int, R]
+    L.or_lidar_sphere_in_frustum.restype = C.c_int
+    L.or_lidar_sphere_in_frustum.argtypes = [P, R, C.c_double if precision == "f64" else C.c_float]
+    L.or_atan2_det.restype = C.c_double if precision == "f64" else C.c_float
+    L.or_atan2_det.argtypes = [C.c_double if precision == "f64" else C.c_float] * 2
+    L.or_sin_det.restype = C.c_double if precision == "f64" else C.c_float
+    L.or_sin_det.argtypes = [C.c_double if precision == "f64" else C.c_float]
+    L.or_integrate_lidar.argtypes = [P, C.c_int, FP, P, R, C.POINTER(C.c_long)]
     L.or_compute_T_CV.argtypes = [P, R, R]
     L.or_half_sample.argtypes = [C.POINTER(C.c_float), C.c_int, C.c_int, C.POINTER(C.c_float), C.c_float]
     L.or_depth_to_points.argtypes = [C.POINTER(C.c_float), C.POINTER(Pinhole), C.c_int, R]
@@ -259,6 +274,16 @@ class OracleMap:
         stats = (C.c_long * 4)()
         self.L.or_integrate(self.h, int(ts), d.ctypes.data_as(C.POINTER(C.c_float)), C.byref(pin),
                             _rp(T, self.precision), stats)
+        return dict(frustum=stats[0], new=stats[1], candidates=stats[2], valid_px=stats[3])
+
+    def integrate_lidar(self, ts, rng, lidar, T_WS):
+        """integrateDepth with an OusterLidar; rng is the H x W range image (metres)."""
+        r = np.ascontiguousarray(rng, dtype=np.float32)
+        assert r.shape == (lidar.height, lidar.width)
+        T = np.ascontiguousarray(np.asarray(T_WS).reshape(16), dtype=self.dtype)
+        stats = (C.c_long * 4)()
+        self.L.or_integrate_lidar(self.h, int(ts), r.ctypes.data_as(C.POINTER(C.c_float)), lidar.handle(self.precision),
+                                  _rp(T, self.precision), stats)
         return dict(frustum=stats[0], new=stats[1], candidates=stats[2], valid_px=stats[3])
 
     def raycast(self, pin, T_WS):
@@ -432,3 +457,65 @@ def se3_exp(a, precision="f32"):
     T = np.zeros(16, dt)
     lib(precision).or_se3_exp(_rp(v, precision), _rp(T, precision))
     return T.reshape(4, 4)
+
+
+class OracleLidar:
+    """OusterLidar (ouster_lidar.cpp:59-100) with the projection model stated in se8_oracle.c: beam
+    elevations strictly descending (row 0 highest), azimuth offsets per beam, in degrees."""
+
+    def __init__(self, width, height, near_plane, far_plane, elevation_deg, azimuth_deg=None):
+        self.width, self.height = int(width), int(height)
+        self.near_plane, self.far_plane = float(near_plane), float(far_plane)
+        self.elev = np.ascontiguousarray(elevation_deg, dtype=np.float32)
+        self.az = np.ascontiguousarray(np.zeros(height) if azimuth_deg is None else azimuth_deg, dtype=np.float32)
+        assert self.elev.shape == (height,) and self.az.shape == (height,)
+        self._h = {}
+
+    def handle(self, precision="f32"):
+        if precision not in self._h:
+            L = lib(precision)
+            FP = C.POINTER(C.c_float)
+            h = L.or_lidar_create(self.width, self.height, self.near_plane, self.far_plane, self.elev.ctypes.data_as(FP),
+                                  self.az.ctypes.data_as(FP))
+            if not h:
+                raise ValueError("lidar config rejected (need height >= 2 and strictly descending elevations)")
+            self._h[precision] = h
+        return self._h[precision]
+
+    def __del__(self):
+        for prec, h in getattr(self, "_h", {}).items():
+            lib(prec).or_lidar_destroy(h)
+
+    def project(self, pts, precision="f32"):
+        dt = _dtype(precision)
+        p = np.ascontiguousarray(pts, dtype=dt).reshape(-1, 3)
+        uv = np.zeros((len(p), 2), dt)
+        ok = np.zeros(len(p), np.uint8)
+        lib(precision).or_lidar_project_n(self.handle(precision), _rp(p, precision), len(p), _rp(uv, precision),
+                                          ok.ctypes.data_as(C.POINTER(C.c_uint8)))
+        return uv, ok.astype(bool)
+
+    def backproject(self, u, v, precision="f32"):
+        out = np.zeros(3, _dtype(precision))
+        lib(precision).or_lidar_backproject(self.handle(precision), int(u), int(v), _rp(out, precision))
+        return out
+
+    def directions(self, precision="f32"):
+        """All back-projection directions, [H, W, 3]."""
+        out = np.zeros((self.height, self.width, 3), _dtype(precision))
+        for v in range(self.height):
+            for u in range(self.width):
+                out[v, u] = self.backproject(u, v, precision)
+        return out
+
+    def sphere_in_frustum(self, c, r, precision="f32"):
+        cc = np.ascontiguousarray(c, dtype=_dtype(precision))
+        return bool(lib(precision).or_lidar_sphere_in_frustum(self.handle(precision), _rp(cc, precision), float(r)))
+
+
+def atan2_det(y, x, precision="f32"):
+    return lib(precision).or_atan2_det(float(y), float(x))
+
+
+def sin_det(x, precision="f32"):
+    return lib(precision).or_sin_det(float(x))

@@ -484,6 +484,265 @@ static inline void round_pixel(const real uv[2], int px[2])
 }
 
 /* ------------------------------------------------------------------------- */
+/* Ouster lidar: src/sensor/ouster_lidar.cpp:59-253,                          */
+/* include/se/sensor/impl/ouster_lidar_impl.hpp:15-32.  The projection model   */
+/* itself (srl::projection::OusterLidar) is not vendored with the reference;   */
+/* the model below is this restatement's own definition, shared operation for */
+/* operation with the GPU kernels (parity unpinned against srl-projection):   */
+/*   beam v has elevation e_v (strictly descending rows) and azimuth offset    */
+/*   a_v; column u looks along phi = 2 pi u / W - pi + a_v;                    */
+/*   backProject(u, v) = (cos e_v cos phi, cos e_v sin phi, sin e_v), computed */
+/*   in double and rounded to float once, into a per-pixel table;              */
+/*   project(p): e = atan2(p_z, |p_xy|) interpolated linearly between beam     */
+/*   elevations to a fractional row (extrapolated by the end gaps), accepted   */
+/*   for rows in [-0.5, H - 0.5); u = (atan2(p_y, p_x) - a_row + pi) W / 2 pi, */
+/*   wrapped into [-0.5, W - 0.5).                                             */
+/* atan and sin are fixed polynomials so that the host and the GPU round       */
+/* identically: atan(t) = t P(t^2) on [0, 1] (a least-squares fit, max error  */
+/* 8.2e-8) and the Taylor series of sin to x^17 on [0, pi].                    */
+/* ------------------------------------------------------------------------- */
+#define OR_PI 3.14159265358979323846
+
+static inline real atan_unit(real t)
+{
+    static const real c[9] = {(real) 0.9999998807907104, (real) -0.33332598209381104, (real) 0.19985906779766083,
+                              (real) -0.141612246632576,  (real) 0.10498932003974915, (real) -0.07234828174114227,
+                              (real) 0.0397808775305748,  (real) -0.014401142485439777, (real) 0.0024566699285060167};
+    const real z = t * t;
+    real r = c[8];
+    for (int k = 7; k >= 0; k--)
+        r = r * z + c[k];
+    return t * r;
+}
+
+static inline real atan2_det(real y, real x)
+{
+    const real ax = (real) fabs((double) x), ay = (real) fabs((double) y);
+    const real mx = ax > ay ? ax : ay, mn = ax > ay ? ay : ax;
+    if (mx == 0)
+        return 0;
+    real a = atan_unit(mn / mx);
+    if (ay > ax)
+        a = (real) (OR_PI / 2) - a;
+    if (x < 0)
+        a = (real) OR_PI - a;
+    if (y < 0)
+        a = -a;
+    return a;
+}
+
+static inline real sin_det(real x) /* x in [0, pi] */
+{
+    static const real c[9] = {(real) 1.0,
+                              (real) -0.1666666716337204,
+                              (real) 0.008333333767950535,
+                              (real) -0.00019841270113829523,
+                              (real) 2.7557318844628753e-06,
+                              (real) -2.5052107943679403e-08,
+                              (real) 1.6059044372074283e-10,
+                              (real) -7.647163609812713e-13,
+                              (real) 2.8114573589663704e-15};
+    const real z = x * x;
+    real r = c[8];
+    for (int k = 7; k >= 0; k--)
+        r = r * z + c[k];
+    return x * r;
+}
+
+typedef struct {
+    int w, h;
+    real near_plane, far_plane;
+    real min_elev, max_elev; /* radians, as OusterLidar's constructor computes them (float) */
+    real* elev;              /* h beam elevations, radians, strictly descending */
+    real* az;                /* h beam azimuth offsets, radians */
+    real* dir;               /* h * w * 3 back-projection directions, row-major */
+    real cols_per_rad;       /* W / (2 pi) */
+} or_lidar;
+
+/* Returns NULL unless h >= 2 and the elevations are strictly descending. */
+OR_API void* or_lidar_create(int w, int h, float near_plane, float far_plane, const float* elev_deg, const float* az_deg)
+{
+    if (w < 1 || h < 2)
+        return NULL;
+    for (int v = 1; v < h; v++)
+        if (!(elev_deg[v] < elev_deg[v - 1]))
+            return NULL;
+    or_lidar* L = (or_lidar*) calloc(1, sizeof(or_lidar));
+    L->w = w;
+    L->h = h;
+    L->near_plane = near_plane;
+    L->far_plane = far_plane;
+    L->elev = (real*) malloc(sizeof(real) * h);
+    L->az = (real*) malloc(sizeof(real) * h);
+    L->dir = (real*) malloc(sizeof(real) * 3 * (size_t) w * h);
+    const float deg_to_rad = (float) (OR_PI / 180.0); /* ouster_lidar.cpp:80 */
+    float mn = elev_deg[0], mx = elev_deg[0];
+    for (int v = 0; v < h; v++) {
+        L->elev[v] = (real) (elev_deg[v] * deg_to_rad);
+        L->az[v] = (real) (az_deg[v] * deg_to_rad);
+        mn = elev_deg[v] < mn ? elev_deg[v] : mn;
+        mx = elev_deg[v] > mx ? elev_deg[v] : mx;
+        const double e = (double) elev_deg[v] * (OR_PI / 180.0), a = (double) az_deg[v] * (OR_PI / 180.0);
+        for (int u = 0; u < w; u++) {
+            const double phi = 2.0 * OR_PI * (double) u / (double) w - OR_PI + a;
+            real* d = L->dir + 3 * ((size_t) v * w + u);
+            d[0] = (real) (float) (cos(e) * cos(phi));
+            d[1] = (real) (float) (cos(e) * sin(phi));
+            d[2] = (real) (float) sin(e);
+        }
+    }
+    L->min_elev = (real) (mn * deg_to_rad);
+    L->max_elev = (real) (mx * deg_to_rad);
+    L->cols_per_rad = (real) (float) ((double) w / (2.0 * OR_PI));
+    return L;
+}
+
+OR_API void or_lidar_destroy(void* vl)
+{
+    or_lidar* L = (or_lidar*) vl;
+    if (!L)
+        return;
+    free(L->elev);
+    free(L->az);
+    free(L->dir);
+    free(L);
+}
+
+static int lidar_project(const or_lidar* L, const real p[3], real uv[2])
+{
+    const real hxy = (real) sqrt((double) (p[0] * p[0] + p[1] * p[1]));
+    if (hxy == 0 && p[2] == 0)
+        return 0;
+    const real e = atan2_det(p[2], hxy);
+    const int h = L->h;
+    real vf;
+    if (e >= L->elev[0])
+        vf = (L->elev[0] - e) / (L->elev[0] - L->elev[1]);
+    else if (e <= L->elev[h - 1])
+        vf = (real) (h - 1) + (L->elev[h - 1] - e) / (L->elev[h - 2] - L->elev[h - 1]);
+    else {
+        int lo = 0, hi = h - 1; /* elev[lo] > e > elev[hi] */
+        while (hi - lo > 1) {
+            const int mid = (lo + hi) / 2;
+            if (L->elev[mid] >= e)
+                lo = mid;
+            else
+                hi = mid;
+        }
+        vf = (real) lo + (L->elev[lo] - e) / (L->elev[lo] - L->elev[lo + 1]);
+    }
+    if (!(vf >= (real) -0.5 && vf < (real) h - (real) 0.5))
+        return 0;
+    const int row = (int) (vf + (real) 0.5);
+    real uf = ((atan2_det(p[1], p[0]) - L->az[row]) + (real) OR_PI) * L->cols_per_rad;
+    for (int k = 0; k < 2 && uf >= (real) L->w - (real) 0.5; k++)
+        uf = uf - (real) L->w;
+    for (int k = 0; k < 2 && uf < (real) -0.5; k++)
+        uf = uf + (real) L->w;
+    if (!(uf >= (real) -0.5 && uf < (real) L->w - (real) 0.5))
+        return 0;
+    uv[0] = uf;
+    uv[1] = vf;
+    return 1;
+}
+
+/* OusterLidar::sphereInFrustumImpl (ouster_lidar.cpp:195-224); asin(c_z / |c|) is evaluated as
+ * atan2(c_z, |c_xy|), equal in exact arithmetic. */
+static int lidar_sphere_in_frustum(const or_lidar* L, const real c[3], real radius)
+{
+    const real n = norm3(c);
+    if (n - radius > L->far_plane)
+        return 0;
+    if (n + radius < L->near_plane)
+        return 0;
+    const real e = atan2_det(c[2], (real) sqrt((double) (c[0] * c[0] + c[1] * c[1])));
+    if (e < L->min_elev) {
+        const real de = (real) fabs((double) (e - L->min_elev));
+        if (sin_det(de) * n > radius)
+            return 0;
+    }
+    if (e > L->max_elev) {
+        const real de = (real) fabs((double) (e - L->max_elev));
+        if (sin_det(de) * n > radius)
+            return 0;
+    }
+    return 1;
+}
+
+OR_API int or_lidar_project(void* vl, const real p[3], real uv[2])
+{
+    return lidar_project((const or_lidar*) vl, p, uv);
+}
+
+OR_API void or_lidar_project_n(void* vl, const real* pts, long n, real* uv, unsigned char* ok)
+{
+    for (long i = 0; i < n; i++)
+        ok[i] = (unsigned char) lidar_project((const or_lidar*) vl, pts + 3 * i, uv + 2 * i);
+}
+
+OR_API void or_lidar_backproject(void* vl, int u, int v, real out[3])
+{
+    const or_lidar* L = (const or_lidar*) vl;
+    const real* d = L->dir + 3 * ((size_t) v * L->w + u);
+    out[0] = d[0];
+    out[1] = d[1];
+    out[2] = d[2];
+}
+
+OR_API int or_lidar_sphere_in_frustum(void* vl, const real c[3], real radius)
+{
+    return lidar_sphere_in_frustum((const or_lidar*) vl, c, radius);
+}
+
+OR_API real or_atan2_det(real y, real x)
+{
+    return atan2_det(y, x);
+}
+
+OR_API real or_sin_det(real x)
+{
+    return sin_det(x);
+}
+
+/* The sensor seen by integration: a pinhole camera or a lidar, through the interface the
+ * reference's templated integrator uses (backProject, project, far/near, measurementFromPoint,
+ * sphereInFrustum). */
+typedef struct {
+    const or_cam* cam;   /* pinhole when not NULL */
+    const or_lidar* lid; /* otherwise */
+    int w, h;
+    real near_plane, far_plane;
+} or_sensor;
+
+static inline void sensor_backproject(const or_sensor* s, int x, int y, real dir[3])
+{
+    if (s->cam)
+        cam_backproject(s->cam, (real) x, (real) y, dir);
+    else
+        or_lidar_backproject((void*) s->lid, x, y, dir);
+}
+
+static inline int sensor_project(const or_sensor* s, const real p[3], real uv[2])
+{
+    return s->cam ? cam_project(s->cam, p, uv) : lidar_project(s->lid, p, uv);
+}
+
+static inline real sensor_far_dist(const or_sensor* s, const real p[3])
+{
+    return s->cam ? cam_far_dist(s->cam, p) : s->far_plane;
+}
+
+static inline real sensor_measurement(const or_sensor* s, const real p[3])
+{
+    return s->cam ? p[2] : norm3(p);
+}
+
+static inline int sensor_sphere_in_frustum(const or_sensor* s, const real c[3], real radius)
+{
+    return s->cam ? cam_sphere_in_frustum(s->cam, c, radius) : lidar_sphere_in_frustum(s->lid, c, radius);
+}
+
+/* ------------------------------------------------------------------------- */
 /* Octree + map: include/se/map/octree/impl/octree_impl.hpp:14-19,171-199,    */
 /* include/se/map/impl/map_impl.hpp:37-64,494-590                             */
 /* ------------------------------------------------------------------------- */
@@ -845,7 +1104,7 @@ OR_API long or_map_dump_nodes(void* vm, uint64_t* keys, unsigned char* masks, in
 /* iterator_impl.hpp:86-156.  Root children are always visited (the root's own */
 /* test does not prune), every other octant is pruned by its sphere test.      */
 /* ------------------------------------------------------------------------- */
-static int octant_ignored(const or_map* m, const or_cam* cam, const or_iso* T_SW, const or_octant* o)
+static int octant_ignored(const or_map* m, const or_sensor* sen, const or_iso* T_SW, const or_octant* o)
 {
     real cW[3], cS[3];
     for (int i = 0; i < 3; i++) {
@@ -854,10 +1113,10 @@ static int octant_ignored(const or_map* m, const or_cam* cam, const or_iso* T_SW
     }
     iso_apply(T_SW, cW, cS);
     const real r = (real) (sqrtf(3.0f) / 2.0f) * m->res * (real) o->size;
-    return !cam_sphere_in_frustum(cam, cS, r);
+    return !sensor_sphere_in_frustum(sen, cS, r);
 }
 
-static long fetch_frustum(const or_map* m, const or_cam* cam, const or_iso* T_WS, or_octant*** out)
+static long fetch_frustum(const or_map* m, const or_sensor* sen, const or_iso* T_WS, or_octant*** out)
 {
     or_iso T_SW;
     iso_inv(T_WS, &T_SW);
@@ -870,7 +1129,7 @@ static long fetch_frustum(const or_map* m, const or_cam* cam, const or_iso* T_WS
             stack[sn++] = m->root->child[i];
     while (sn > 0) {
         or_octant* o = stack[--sn];
-        if (octant_ignored(m, cam, &T_SW, o))
+        if (octant_ignored(m, sen, &T_SW, o))
             continue;
         if (!o->is_block) {
             for (int i = 0; i < 8; i++) {
@@ -916,7 +1175,7 @@ static inline int field_update(real* tsdf, real* weight, real sdf, real mu, real
     return 1;
 }
 
-static void update_block(const or_map* m, or_octant* b, const or_iso* T_CV, const or_cam* cam,
+static void update_block(const or_map* m, or_octant* b, const or_iso* T_CV, const or_sensor* sen,
                          const float* depth, int ts)
 {
     b->timestamp = ts;
@@ -927,17 +1186,18 @@ static void update_block(const or_map* m, or_octant* b, const or_iso* T_CV, cons
                 real pC[3];
                 iso_apply(T_CV, v, pC);
                 const real nrm = norm3(pC);
-                if (nrm > cam_far_dist(cam, pC))
+                if (nrm > sensor_far_dist(sen, pC))
                     continue;
                 real uv[2];
-                if (!cam_project(cam, pC, uv))
+                if (!sensor_project(sen, pC, uv))
                     continue;
                 int px[2];
                 round_pixel(uv, px);
-                const real d = depth[px[0] + px[1] * cam->w];
-                if (d < cam->near_plane)
+                const real d = depth[px[0] + px[1] * sen->w];
+                if (d < sen->near_plane)
                     continue;
-                const real mz = pC[2];
+                /* measurementFromPoint: z for the pinhole, |p| for the lidar */
+                const real mz = sensor_measurement(sen, pC);
                 const real sdf = (d - mz) / mz * norm3(pC);
                 const int vi = x + 8 * y + 64 * z;
                 field_update(&b->tsdf[vi], &b->weight[vi], sdf, m->mu, m->max_weight);
@@ -949,14 +1209,14 @@ static void update_block(const or_map* m, or_octant* b, const or_iso* T_CV, cons
 /* The allocation band march (raycast_carver_impl.hpp:74-111) for one pixel.  Writes the
  * keys of unallocated blocks hit by the samples into keys (at most num_steps) and returns
  * their count. */
-static int carve_pixel(const or_map* m, const or_cam* cam, const or_iso* T_WS, const float* depth,
+static int carve_pixel(const or_map* m, const or_sensor* sen, const or_iso* T_WS, const float* depth,
                        int x, int y, real band, int num_steps, uint64_t* keys)
 {
-    const real d = depth[x + y * cam->w];
-    if (d < cam->near_plane || d > (cam->far_plane + band * (real) 0.5))
+    const real d = depth[x + y * sen->w];
+    if (d < sen->near_plane || d > (sen->far_plane + band * (real) 0.5))
         return 0;
     real dir[3], pC[3], pW[3];
-    cam_backproject(cam, (real) x, (real) y, dir);
+    sensor_backproject(sen, x, y, dir);
     for (int i = 0; i < 3; i++)
         pC[i] = d * dir[i];
     iso_apply(T_WS, pC, pW);
@@ -1012,12 +1272,9 @@ OR_API void or_compute_T_CV(void* vm, const real T_WC16[16], real out16[16])
 
 /* stats: [0] frustum blocks, [1] new blocks, [2] candidate keys (with per-pixel repeats),
  *        [3] valid pixels */
-OR_API void or_integrate(void* vm, int ts, const float* depth, const or_pinhole* pin, const real T_WC16[16],
-                         long stats[4])
+static void integrate_sensor(or_map* m, int ts, const float* depth, const or_sensor* sen, const real T_WC16[16],
+                             long stats[4])
 {
-    or_map* m = (or_map*) vm;
-    or_cam cam;
-    cam_init(&cam, pin, 1);
     or_iso T_WC;
     iso_from16(T_WC16, &T_WC);
     /* band_ = 2 * res * truncation_boundary_factor (raycast_carver_impl.hpp:50) */
@@ -1025,22 +1282,22 @@ OR_API void or_integrate(void* vm, int ts, const float* depth, const or_pinhole*
 
     /* fetch-frustum (before any allocation) */
     or_octant** frustum = NULL;
-    const long n_frustum = fetch_frustum(m, &cam, &T_WC, &frustum);
+    const long n_frustum = fetch_frustum(m, sen, &T_WC, &frustum);
 
     /* create-list */
     const int num_steps = (int) ceil((double) (band / ((real) 2 * m->res)));
-    const long npx = (long) cam.w * cam.h;
+    const long npx = (long) sen->w * sen->h;
     uint64_t* cand = (uint64_t*) malloc(sizeof(uint64_t) * (size_t) npx * (size_t) num_steps);
     int* cnt = (int*) calloc((size_t) npx, sizeof(int));
     long valid = 0;
 #pragma omp parallel for reduction(+ : valid) schedule(static)
-    for (int x = 0; x < cam.w; x++) {
-        for (int y = 0; y < cam.h; y++) {
-            const long p = x + (long) y * cam.w;
+    for (int x = 0; x < sen->w; x++) {
+        for (int y = 0; y < sen->h; y++) {
+            const long p = x + (long) y * sen->w;
             const real d = depth[p];
-            if (!(d < cam.near_plane || d > (cam.far_plane + band * (real) 0.5)))
+            if (!(d < sen->near_plane || d > (sen->far_plane + band * (real) 0.5)))
                 valid++;
-            cnt[p] = carve_pixel(m, &cam, &T_WC, depth, x, y, band, num_steps, cand + p * num_steps);
+            cnt[p] = carve_pixel(m, sen, &T_WC, depth, x, y, band, num_steps, cand + p * num_steps);
         }
     }
     long nk = 0;
@@ -1066,7 +1323,7 @@ OR_API void or_integrate(void* vm, int ts, const float* depth, const or_pinhole*
     compute_T_CV(m, &T_WC, &T_CV);
 #pragma omp parallel for schedule(dynamic, 16)
     for (long i = 0; i < nb; i++)
-        update_block(m, list[i], &T_CV, &cam, depth, ts);
+        update_block(m, list[i], &T_CV, sen, depth, ts);
     /* propagateTimeStampToRoot: only the blocks' direct parents are reached */
     for (long i = 0; i < nb; i++) {
         or_octant* p = list[i]->parent;
@@ -1082,6 +1339,23 @@ OR_API void or_integrate(void* vm, int ts, const float* depth, const or_pinhole*
     free(list);
     free(newb);
     free(frustum);
+}
+
+OR_API void or_integrate(void* vm, int ts, const float* depth, const or_pinhole* pin, const real T_WC16[16],
+                         long stats[4])
+{
+    or_cam cam;
+    cam_init(&cam, pin, 1);
+    const or_sensor sen = {&cam, NULL, cam.w, cam.h, cam.near_plane, cam.far_plane};
+    integrate_sensor((or_map*) vm, ts, depth, &sen, T_WC16, stats);
+}
+
+/* integrateDepth with an OusterLidar: range image (H rows of W columns, metres) */
+OR_API void or_integrate_lidar(void* vm, int ts, const float* range, void* vl, const real T_WS16[16], long stats[4])
+{
+    const or_lidar* L = (const or_lidar*) vl;
+    const or_sensor sen = {NULL, L, L->w, L->h, L->near_plane, L->far_plane};
+    integrate_sensor((or_map*) vm, ts, range, &sen, T_WS16, stats);
 }
 
 /* ------------------------------------------------------------------------- */

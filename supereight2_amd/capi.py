@@ -52,6 +52,17 @@ class Pinhole(C.Structure):
     ]
 
 
+class LidarConfig(C.Structure):
+    _fields_ = [
+        ("width", C.c_int),
+        ("height", C.c_int),
+        ("near_plane", C.c_float),
+        ("far_plane", C.c_float),
+        ("beam_elevation_deg", C.POINTER(C.c_float)),
+        ("beam_azimuth_deg", C.POINTER(C.c_float)),
+    ]
+
+
 class TrackerConfig(C.Structure):
     _fields_ = [
         ("iterations", C.c_int * 3),
@@ -115,6 +126,10 @@ SIGNATURES = {
     "se8_map_destroy": (C.c_int, [_P]),
     "se8_map_get_info": (C.c_int, [_P, C.POINTER(MapInfo)]),
     "se8_integrate_depth": (C.c_int, [_P, C.c_int, _P, C.POINTER(Pinhole), _F, _P]),
+    "se8_lidar_create": (C.c_int, [C.POINTER(LidarConfig), C.POINTER(_P)]),
+    "se8_lidar_destroy": (C.c_int, [_P]),
+    "se8_integrate_range": (C.c_int, [_P, C.c_int, _P, _P, _F, _P]),
+    "se8_lidar_project": (C.c_int, [_P, _F, C.c_int64, _F, C.POINTER(C.c_uint8)]),
     "se8_raycast": (C.c_int, [_P, C.POINTER(Pinhole), _F, _P, _P, _P, _P]),
     "se8_tracker_create": (C.c_int, [C.POINTER(Pinhole), C.POINTER(TrackerConfig), C.POINTER(_P)]),
     "se8_tracker_destroy": (C.c_int, [_P]),

@@ -216,6 +216,168 @@ SE8_HD float cam_near_dist(const Cam& c, const float ray[3])
 }
 
 // ----------------------------------------------------------------------------------------------
+// Ouster lidar (src/sensor/ouster_lidar.cpp:59-253, ouster_lidar_impl.hpp:15-32).  The beam model
+// of srl::projection::OusterLidar is not part of the reference's sources; this is the library's
+// own definition:
+//   beam v: elevation e_v (rows strictly descending), azimuth offset a_v; column u looks along
+//   phi = 2 pi u / W - pi + a_v; backProject(u, v) = (cos e_v cos phi, cos e_v sin phi, sin e_v),
+//   evaluated in double on the host and rounded to float once (a per-pixel table);
+//   project(p): e = atan2(p_z, |p_xy|) interpolated linearly between the beam elevations to a
+//   fractional row (extrapolated by the end gaps), accepted for rows in [-0.5, H - 0.5);
+//   u = (atan2(p_y, p_x) - a_row + pi) * W / (2 pi), wrapped into [-0.5, W - 0.5).
+// atan and sin are fixed polynomials so that host and device round identically: atan(t) = t P(t^2)
+// on [0, 1] (least-squares fit, max error 8.2e-8) and the Taylor series of sin to x^17 on [0, pi].
+// ----------------------------------------------------------------------------------------------
+constexpr float kPiF = (float) 3.14159265358979323846;
+constexpr float kHalfPiF = (float) (3.14159265358979323846 / 2);
+
+SE8_HD float atan_unit(float t)
+{
+    const float c[9] = {0.9999998807907104f,  -0.33332598209381104f, 0.19985906779766083f,
+                        -0.141612246632576f,  0.10498932003974915f,  -0.07234828174114227f,
+                        0.0397808775305748f,  -0.014401142485439777f, 0.0024566699285060167f};
+    const float z = t * t;
+    float r = c[8];
+#pragma unroll
+    for (int k = 7; k >= 0; k--)
+        r = r * z + c[k];
+    return t * r;
+}
+
+SE8_HD float atan2_det(float y, float x)
+{
+    const float ax = fabsf(x), ay = fabsf(y);
+    const float mx = ax > ay ? ax : ay, mn = ax > ay ? ay : ax;
+    if (mx == 0.f)
+        return 0.f;
+    float a = atan_unit(mn / mx);
+    if (ay > ax)
+        a = kHalfPiF - a;
+    if (x < 0.f)
+        a = kPiF - a;
+    if (y < 0.f)
+        a = -a;
+    return a;
+}
+
+SE8_HD float sin_det(float x) // x in [0, pi]
+{
+    const float c[9] = {1.0f,
+                        -0.1666666716337204f,
+                        0.008333333767950535f,
+                        -0.00019841270113829523f,
+                        2.7557318844628753e-06f,
+                        -2.5052107943679403e-08f,
+                        1.6059044372074283e-10f,
+                        -7.647163609812713e-13f,
+                        2.8114573589663704e-15f};
+    const float z = x * x;
+    float r = c[8];
+#pragma unroll
+    for (int k = 7; k >= 0; k--)
+        r = r * z + c[k];
+    return x * r;
+}
+
+struct LidarModel {
+    int w, h;
+    float near_plane, far_plane;
+    float min_elev, max_elev; // radians, as OusterLidar's constructor computes them
+    float cols_per_rad;       // W / (2 pi)
+    const float* elev;        // h beam elevations (radians, descending), device
+    const float* az;          // h beam azimuth offsets (radians), device
+    const float4* dir;        // h * w back-projection directions, row-major, device
+};
+
+SE8_HD bool lidar_project(const LidarModel& L, const float p[3], float uv[2])
+{
+    const float hxy = sqrtf(p[0] * p[0] + p[1] * p[1]);
+    if (hxy == 0.f && p[2] == 0.f)
+        return false;
+    const float e = atan2_det(p[2], hxy);
+    const int h = L.h;
+    float vf;
+    if (e >= L.elev[0])
+        vf = (L.elev[0] - e) / (L.elev[0] - L.elev[1]);
+    else if (e <= L.elev[h - 1])
+        vf = (float) (h - 1) + (L.elev[h - 1] - e) / (L.elev[h - 2] - L.elev[h - 1]);
+    else {
+        int lo = 0, hi = h - 1; // elev[lo] > e > elev[hi]
+        while (hi - lo > 1) {
+            const int mid = (lo + hi) / 2;
+            if (L.elev[mid] >= e)
+                lo = mid;
+            else
+                hi = mid;
+        }
+        vf = (float) lo + (L.elev[lo] - e) / (L.elev[lo] - L.elev[lo + 1]);
+    }
+    if (!(vf >= -0.5f && vf < (float) h - 0.5f))
+        return false;
+    const int row = (int) (vf + 0.5f);
+    float uf = ((atan2_det(p[1], p[0]) - L.az[row]) + kPiF) * L.cols_per_rad;
+    for (int k = 0; k < 2 && uf >= (float) L.w - 0.5f; k++)
+        uf = uf - (float) L.w;
+    for (int k = 0; k < 2 && uf < -0.5f; k++)
+        uf = uf + (float) L.w;
+    if (!(uf >= -0.5f && uf < (float) L.w - 0.5f))
+        return false;
+    uv[0] = uf;
+    uv[1] = vf;
+    return true;
+}
+
+// OusterLidar::sphereInFrustumImpl (ouster_lidar.cpp:195-224), asin(c_z / |c|) as atan2(c_z, |c_xy|)
+SE8_HD bool lidar_sphere_in_frustum(const LidarModel& L, const float c[3], float radius)
+{
+    const float n = sqrtf(sqnorm3(c));
+    if (n - radius > L.far_plane)
+        return false;
+    if (n + radius < L.near_plane)
+        return false;
+    const float e = atan2_det(c[2], sqrtf(c[0] * c[0] + c[1] * c[1]));
+    if (e < L.min_elev && sin_det(fabsf(e - L.min_elev)) * n > radius)
+        return false;
+    if (e > L.max_elev && sin_det(fabsf(e - L.max_elev)) * n > radius)
+        return false;
+    return true;
+}
+
+// The sensor interface of the reference's templated integrator (SensorBase: backProject, project,
+// farDist, near_plane, measurementFromPoint, sphereInFrustum) for the two sensor types.
+struct PinholeSensor {
+    Cam c;
+    SE8_HD int width() const { return c.w; }
+    SE8_HD int height() const { return c.h; }
+    SE8_HD float near_plane() const { return c.near_plane; }
+    SE8_HD float far_plane() const { return c.far_plane; }
+    SE8_HD void backproject(int x, int y, float dir[3]) const { cam_backproject(c, (float) x, (float) y, dir); }
+    SE8_HD bool project(const float p[3], float uv[2]) const { return cam_project(c, p, uv); }
+    SE8_HD float far_dist(const float p[3]) const { return cam_far_dist(c, p); }
+    SE8_HD float measurement(const float p[3]) const { return p[2]; }
+    SE8_HD bool sphere_in_frustum(const float ctr[3], float r) const { return cam_sphere_in_frustum(c, ctr, r); }
+};
+
+struct LidarSensor {
+    LidarModel L;
+    SE8_HD int width() const { return L.w; }
+    SE8_HD int height() const { return L.h; }
+    SE8_HD float near_plane() const { return L.near_plane; }
+    SE8_HD float far_plane() const { return L.far_plane; }
+    __device__ void backproject(int x, int y, float dir[3]) const
+    {
+        const float4 d = L.dir[(size_t) y * L.w + x];
+        dir[0] = d.x;
+        dir[1] = d.y;
+        dir[2] = d.z;
+    }
+    SE8_HD bool project(const float p[3], float uv[2]) const { return lidar_project(L, p, uv); }
+    SE8_HD float far_dist(const float*) const { return L.far_plane; }
+    SE8_HD float measurement(const float p[3]) const { return sqrtf(sqnorm3(p)); }
+    SE8_HD bool sphere_in_frustum(const float ctr[3], float r) const { return lidar_sphere_in_frustum(L, ctr, r); }
+};
+
+// ----------------------------------------------------------------------------------------------
 // Morton keys (include/se/map/utils/impl/key_util_impl.hpp:44-247)
 // ----------------------------------------------------------------------------------------------
 SE8_HD uint64_t expand_bits(uint64_t v)
@@ -430,6 +592,12 @@ struct se8_tracker {
     bool track_pending = false;
     se8_stage_times times{};
     hipEvent_t ev[2]{};
+};
+
+struct se8_lidar {
+    int width = 0, height = 0;
+    se8::LidarModel model{}; // device pointers into elev / az / dir
+    se8::DeviceBuffer elev, az, dir;
 };
 
 namespace se8 {

@@ -179,7 +179,8 @@ __device__ __forceinline__ int wave_append_slot(bool pred, int* counter)
 // fetch-frustum: a block is listed iff its own sphere and those of all its ancestors below the
 // root pass sphereInFrustum (the DFS of iterator_impl.hpp:86-156 never prunes at the root).
 // ----------------------------------------------------------------------------------------------
-__global__ void k_frustum(MapView m, Iso T_SW, Cam cam, int* update_list)
+template<class Sensor>
+__global__ void k_frustum(MapView m, Iso T_SW, Sensor sensor, int* update_list)
 {
     const int n = m.counters[kCntBlocks];
     const float r_unit = sqrtf(3.0f) / 2.0f * m.res;
@@ -200,7 +201,7 @@ __global__ void k_frustum(MapView m, Iso T_SW, Cam cam, int* update_list)
                 }
                 float cS[3];
                 iso_apply(T_SW, cW, cS);
-                in = cam_sphere_in_frustum(cam, cS, r_unit * (float) size);
+                in = sensor.sphere_in_frustum(cS, r_unit * (float) size);
             }
         }
         const int pos = wave_append_slot(in, &m.counters[kCntUpdate]);
@@ -219,20 +220,21 @@ __global__ void k_record_frustum(int* counters)
 // ----------------------------------------------------------------------------------------------
 // create-list: band march of RaycastCarver (raycast_carver_impl.hpp:74-111)
 // ----------------------------------------------------------------------------------------------
-__global__ void k_alloc_rays(MapView m, Cam cam, Iso T_WS, const float* depth, float band, int num_steps,
+template<class Sensor>
+__global__ void k_alloc_rays(MapView m, Sensor sensor, Iso T_WS, const float* depth, float band, int num_steps,
                              uint64_t* cand, int64_t cand_cap)
 {
     const int x = blockIdx.x * blockDim.x + threadIdx.x;
     const int y = blockIdx.y;
-    bool live = x < cam.w;
+    bool live = x < sensor.width();
     float pos[3] = {0.f, 0.f, 0.f}, step[3] = {0.f, 0.f, 0.f};
     if (live) {
-        const float d = depth[x + (size_t) y * cam.w];
-        if (d < cam.near_plane || d > (cam.far_plane + band * 0.5f))
+        const float d = depth[x + (size_t) y * sensor.width()];
+        if (d < sensor.near_plane() || d > (sensor.far_plane() + band * 0.5f))
             live = false;
         else {
             float dir[3], pC[3], pW[3];
-            cam_backproject(cam, (float) x, (float) y, dir);
+            sensor.backproject(x, y, dir);
             for (int i = 0; i < 3; i++)
                 pC[i] = d * dir[i];
             iso_apply(T_WS, pC, pW);
@@ -541,8 +543,8 @@ __global__ void k_build_blocks(MapView m, const uint64_t* keys, const int* n_dev
 // ----------------------------------------------------------------------------------------------
 // update: one block (512 voxels) per workgroup of 8 wave64s
 // ----------------------------------------------------------------------------------------------
-template<int Storage>
-__global__ __launch_bounds__(512) void k_update(MapView m, const int* update_list, Iso T_CV, Cam cam,
+template<int Storage, class Sensor>
+__global__ __launch_bounds__(512) void k_update(MapView m, const int* update_list, Iso T_CV, Sensor sensor,
                                                 const float* depth, int ts)
 {
     const int n = m.counters[kCntUpdate];
@@ -575,13 +577,14 @@ __global__ __launch_bounds__(512) void k_update(MapView m, const int* update_lis
         }
         bool changed = false;
         const float nrm = sqrtf(sqnorm3(pC));
-        if (!(nrm > cam_far_dist(cam, pC))) {
+        if (!(nrm > sensor.far_dist(pC))) {
             float uv[2];
-            if (cam_project(cam, pC, uv)) {
+            if (sensor.project(pC, uv)) {
                 const int px = (int) (uv[0] + 0.5f), py = (int) (uv[1] + 0.5f);
-                const float d = depth[px + (size_t) py * cam.w];
-                if (!(d < cam.near_plane)) {
-                    const float mz = pC[2];
+                const float d = depth[px + (size_t) py * sensor.width()];
+                if (!(d < sensor.near_plane())) {
+                    // measurementFromPoint: z for the pinhole, |p| for the lidar
+                    const float mz = sensor.measurement(pC);
                     const float sdf = (d - mz) / mz * sqrtf(sqnorm3(pC));
                     if (!(sdf < -m.mu)) {
                         const float u = fminf(sdf / m.mu, 1.f);
@@ -907,17 +910,18 @@ extern "C" int se8_map_get_info(const se8_map* m, se8_map_info* out)
     return host[kCntOverflow] ? SE8_ERR_CAPACITY : SE8_OK;
 }
 
-extern "C" int se8_integrate_depth(se8_map* m, int timestamp, const float* depth_dev, const se8_pinhole* sensor,
-                                   const float T_WC16[16], se8_stream stream)
+namespace se8 {
+
+// integrateDepth (map_integrator_impl.hpp:52-83) for either sensor; `what` names the entry point
+// in error messages.
+template<class Sensor>
+static int integrate_impl(se8_map* m, int timestamp, const float* depth_dev, const Sensor& sensor,
+                          const float T_WC16[16], hipStream_t s, const char* what)
 {
-    SE8_REQUIRE(m && depth_dev && sensor && T_WC16, "se8_integrate_depth: null argument");
-    SE8_REQUIRE(sensor->width > 0 && sensor->height > 0, "se8_integrate_depth: empty image");
-    hipStream_t s = (hipStream_t) stream;
-    const Cam cam = make_cam(*sensor, 1, false);
     const Iso T_WC = iso_from16(T_WC16);
     const float band = 2.0f * m->view.res * m->tsdf.truncation_boundary_factor;
     const int num_steps = (int) std::ceil(band / (2.f * m->view.res));
-    const size_t cand_cap = (size_t) cam.w * cam.h * num_steps;
+    const size_t cand_cap = (size_t) sensor.width() * sensor.height() * num_steps;
     int rc = ensure_candidates(m, cand_cap);
     if (rc != SE8_OK)
         return rc;
@@ -929,19 +933,19 @@ extern "C" int se8_integrate_depth(se8_map* m, int timestamp, const float* depth
         m->counters_pending = false;
         harvest_integration(m);
         if (m->host_counters[kCntOverflow]) {
-            set_error("se8_integrate_depth: map capacity exhausted (raise se8_tsdf_config.max_blocks)");
+            set_error(std::string(what) + ": map capacity exhausted (raise se8_tsdf_config.max_blocks)");
             return SE8_ERR_CAPACITY;
         }
     }
     SE8_HIP_CHECK(hipEventRecord(m->ev[0], s));
     k_frame_reset<<<1, 64, 0, s>>>(m->view.counters);
     const int cus = device_count_sm();
-    k_frustum<<<cus * 4, 256, 0, s>>>(m->view, iso_inv(T_WC), cam, (int*) m->update_list.ptr);
+    k_frustum<Sensor><<<cus * 4, 256, 0, s>>>(m->view, iso_inv(T_WC), sensor, (int*) m->update_list.ptr);
     k_record_frustum<<<1, 64, 0, s>>>(m->view.counters);
     SE8_HIP_CHECK(hipEventRecord(m->ev[1], s));
-    dim3 grid((cam.w + 63) / 64, cam.h);
-    k_alloc_rays<<<grid, 64, 0, s>>>(m->view, cam, T_WC, depth_dev, band, num_steps, (uint64_t*) m->candidates.ptr,
-                                     (int64_t) cand_cap);
+    dim3 grid((sensor.width() + 63) / 64, sensor.height());
+    k_alloc_rays<Sensor><<<grid, 64, 0, s>>>(m->view, sensor, T_WC, depth_dev, band, num_steps,
+                                             (uint64_t*) m->candidates.ptr, (int64_t) cand_cap);
     SE8_HIP_CHECK(hipGetLastError());
     SE8_HIP_CHECK(hipEventRecord(m->ev[2], s));
     rc = build_from_candidates(m, (uint64_t*) m->candidates.ptr, m->view.counters + kCntCandidates, (int) cand_cap,
@@ -954,11 +958,13 @@ extern "C" int se8_integrate_depth(se8_map* m, int timestamp, const float* depth
     // one resident wave of workgroups: the grid-stride loop then never leaves a second, partial
     // round of workgroups waiting for the first to finish
     if (m->view.storage == SE8_STORAGE_F32)
-        hipExtLaunchKernelGGL(k_update<SE8_STORAGE_F32>, dim3(resident_grid(k_update<SE8_STORAGE_F32>, 512)),
-                              dim3(512), 0, s, m->ev[3], m->ev[4], 0, m->view, ulist, T_CV, cam, depth_dev, timestamp);
+        hipExtLaunchKernelGGL(k_update<SE8_STORAGE_F32, Sensor>,
+                              dim3(resident_grid(k_update<SE8_STORAGE_F32, Sensor>, 512)), dim3(512), 0, s, m->ev[3],
+                              m->ev[4], 0, m->view, ulist, T_CV, sensor, depth_dev, timestamp);
     else
-        hipExtLaunchKernelGGL(k_update<SE8_STORAGE_F16>, dim3(resident_grid(k_update<SE8_STORAGE_F16>, 512)),
-                              dim3(512), 0, s, m->ev[3], m->ev[4], 0, m->view, ulist, T_CV, cam, depth_dev, timestamp);
+        hipExtLaunchKernelGGL(k_update<SE8_STORAGE_F16, Sensor>,
+                              dim3(resident_grid(k_update<SE8_STORAGE_F16, Sensor>, 512)), dim3(512), 0, s, m->ev[3],
+                              m->ev[4], 0, m->view, ulist, T_CV, sensor, depth_dev, timestamp);
     SE8_HIP_CHECK(hipGetLastError());
     SE8_HIP_CHECK(hipMemcpyAsync(m->host_counters, m->counters.ptr, sizeof(int) * kNumCounters,
                                  hipMemcpyDeviceToHost, s));
@@ -966,6 +972,148 @@ extern "C" int se8_integrate_depth(se8_map* m, int timestamp, const float* depth
     m->counters_pending = true;
     m->integ_pending = true;
     return SE8_OK;
+}
+
+// Direction table of the lidar model: evaluated in double, rounded to float once.
+static void lidar_directions(const se8_lidar_config& c, std::vector<float>& dir4)
+{
+    dir4.assign((size_t) 4 * c.width * c.height, 0.f);
+    const double pi = 3.14159265358979323846;
+    for (int v = 0; v < c.height; v++) {
+        const double e = (double) c.beam_elevation_deg[v] * (pi / 180.0), a = (double) c.beam_azimuth_deg[v] * (pi / 180.0);
+        for (int u = 0; u < c.width; u++) {
+            const double phi = 2.0 * pi * (double) u / (double) c.width - pi + a;
+            float* d = dir4.data() + 4 * ((size_t) v * c.width + u);
+            d[0] = (float) (cos(e) * cos(phi));
+            d[1] = (float) (cos(e) * sin(phi));
+            d[2] = (float) sin(e);
+        }
+    }
+}
+
+__global__ void k_lidar_project(LidarModel L, const float* pts, int64_t n, float* uv, uint8_t* ok)
+{
+    for (int64_t i = blockIdx.x * (int64_t) blockDim.x + threadIdx.x; i < n; i += (int64_t) gridDim.x * blockDim.x) {
+        const float p[3] = {pts[3 * i], pts[3 * i + 1], pts[3 * i + 2]};
+        float q[2] = {0.f, 0.f};
+        ok[i] = lidar_project(L, p, q) ? 1 : 0;
+        uv[2 * i] = q[0];
+        uv[2 * i + 1] = q[1];
+    }
+}
+
+} // namespace se8
+
+extern "C" int se8_integrate_depth(se8_map* m, int timestamp, const float* depth_dev, const se8_pinhole* sensor,
+                                   const float T_WC16[16], se8_stream stream)
+{
+    SE8_REQUIRE(m && depth_dev && sensor && T_WC16, "se8_integrate_depth: null argument");
+    SE8_REQUIRE(sensor->width > 0 && sensor->height > 0, "se8_integrate_depth: empty image");
+    const PinholeSensor ps{make_cam(*sensor, 1, false)};
+    return integrate_impl(m, timestamp, depth_dev, ps, T_WC16, (hipStream_t) stream, "se8_integrate_depth");
+}
+
+extern "C" int se8_lidar_create(const se8_lidar_config* cfg, se8_lidar** out)
+{
+    SE8_REQUIRE(cfg && out && cfg->beam_elevation_deg && cfg->beam_azimuth_deg, "se8_lidar_create: null argument");
+    SE8_REQUIRE(cfg->width > 0 && cfg->height >= 2, "se8_lidar_create: need width >= 1 and height >= 2");
+    SE8_REQUIRE(cfg->near_plane > 0.f && cfg->far_plane > cfg->near_plane, "se8_lidar_create: bad near/far planes");
+    for (int v = 1; v < cfg->height; v++)
+        SE8_REQUIRE(cfg->beam_elevation_deg[v] < cfg->beam_elevation_deg[v - 1],
+                    "se8_lidar_create: beam elevations must be strictly descending");
+    *out = nullptr;
+    se8_lidar* l = new se8_lidar();
+    l->width = cfg->width;
+    l->height = cfg->height;
+    const float deg_to_rad = (float) (3.14159265358979323846 / 180.0); // ouster_lidar.cpp:80
+    std::vector<float> elev(cfg->height), az(cfg->height), dir4;
+    float mn = cfg->beam_elevation_deg[0], mx = cfg->beam_elevation_deg[0];
+    for (int v = 0; v < cfg->height; v++) {
+        elev[v] = cfg->beam_elevation_deg[v] * deg_to_rad;
+        az[v] = cfg->beam_azimuth_deg[v] * deg_to_rad;
+        mn = std::min(mn, cfg->beam_elevation_deg[v]);
+        mx = std::max(mx, cfg->beam_elevation_deg[v]);
+    }
+    lidar_directions(*cfg, dir4);
+    int rc = dev_alloc(l->elev, sizeof(float) * elev.size());
+    rc = rc ? rc : dev_alloc(l->az, sizeof(float) * az.size());
+    rc = rc ? rc : dev_alloc(l->dir, sizeof(float) * dir4.size());
+    if (rc == SE8_OK && (hipMemcpy(l->elev.ptr, elev.data(), sizeof(float) * elev.size(), hipMemcpyHostToDevice)
+                             != hipSuccess
+                         || hipMemcpy(l->az.ptr, az.data(), sizeof(float) * az.size(), hipMemcpyHostToDevice) != hipSuccess
+                         || hipMemcpy(l->dir.ptr, dir4.data(), sizeof(float) * dir4.size(), hipMemcpyHostToDevice)
+                             != hipSuccess)) {
+        set_error("se8_lidar_create: upload failed");
+        rc = SE8_ERR_HIP;
+    }
+    if (rc != SE8_OK) {
+        se8_lidar_destroy(l);
+        return rc;
+    }
+    LidarModel& L = l->model;
+    L.w = cfg->width;
+    L.h = cfg->height;
+    L.near_plane = cfg->near_plane;
+    L.far_plane = cfg->far_plane;
+    L.min_elev = mn * deg_to_rad;
+    L.max_elev = mx * deg_to_rad;
+    L.cols_per_rad = (float) ((double) cfg->width / (2.0 * 3.14159265358979323846));
+    L.elev = (const float*) l->elev.ptr;
+    L.az = (const float*) l->az.ptr;
+    L.dir = (const float4*) l->dir.ptr;
+    *out = l;
+    return SE8_OK;
+}
+
+extern "C" int se8_lidar_destroy(se8_lidar* l)
+{
+    if (!l)
+        return SE8_OK;
+    (void) hipDeviceSynchronize();
+    dev_free(l->elev);
+    dev_free(l->az);
+    dev_free(l->dir);
+    delete l;
+    return SE8_OK;
+}
+
+extern "C" int se8_integrate_range(se8_map* m, int timestamp, const float* range_dev, const se8_lidar* lidar,
+                                   const float T_WS16[16], se8_stream stream)
+{
+    SE8_REQUIRE(m && range_dev && lidar && T_WS16, "se8_integrate_range: null argument");
+    const LidarSensor ls{lidar->model};
+    return integrate_impl(m, timestamp, range_dev, ls, T_WS16, (hipStream_t) stream, "se8_integrate_range");
+}
+
+extern "C" int se8_lidar_project(const se8_lidar* lidar, const float* points, int64_t n, float* uv,
+                                 uint8_t* in_image)
+{
+    SE8_REQUIRE(lidar && (n == 0 || (points && uv && in_image)) && n >= 0, "se8_lidar_project: bad argument");
+    if (n == 0)
+        return SE8_OK;
+    DeviceBuffer dp, du, dk;
+    int rc = dev_alloc(dp, sizeof(float) * 3 * n);
+    rc = rc ? rc : dev_alloc(du, sizeof(float) * 2 * n);
+    rc = rc ? rc : dev_alloc(dk, (size_t) n);
+    if (rc == SE8_OK) {
+        if (hipMemcpy(dp.ptr, points, sizeof(float) * 3 * n, hipMemcpyHostToDevice) != hipSuccess)
+            rc = SE8_ERR_HIP;
+        else {
+            const int64_t blocks = std::min<int64_t>((n + 255) / 256, 4096);
+            k_lidar_project<<<(unsigned) blocks, 256>>>(lidar->model, (const float*) dp.ptr, n, (float*) du.ptr,
+                                                        (uint8_t*) dk.ptr);
+            if (hipGetLastError() != hipSuccess || hipDeviceSynchronize() != hipSuccess
+                || hipMemcpy(uv, du.ptr, sizeof(float) * 2 * n, hipMemcpyDeviceToHost) != hipSuccess
+                || hipMemcpy(in_image, dk.ptr, (size_t) n, hipMemcpyDeviceToHost) != hipSuccess)
+                rc = SE8_ERR_HIP;
+        }
+        if (rc == SE8_ERR_HIP)
+            set_error("se8_lidar_project: device error");
+    }
+    dev_free(dp);
+    dev_free(du);
+    dev_free(dk);
+    return rc;
 }
 
 // ---------------------------------------------------------------------------------------------

@@ -53,6 +53,47 @@ class PinholeCamera:
         return capi.Pinhole(self.width, self.height, self.fx, self.fy, self.cx, self.cy, self.near_plane, self.far_plane)
 
 
+class OusterLidar:
+    """se::OusterLidar (ouster_lidar.hpp:12-80, ouster_lidar.cpp:59-85): a spinning lidar of
+    `height` beams, each with an elevation and an azimuth offset in degrees (row 0 the highest
+    beam), `width` columns over 360 degrees.  The beam model lives on the device (direction table,
+    elevation and azimuth tables) from construction to close()."""
+
+    def __init__(self, width, height, near_plane, far_plane, beam_elevation_angles, beam_azimuth_angles=None):
+        self.lib = capi.load()
+        self.width, self.height = int(width), int(height)
+        self.near_plane, self.far_plane = float(near_plane), float(far_plane)
+        self.elev = np.ascontiguousarray(beam_elevation_angles, dtype=np.float32)
+        az = np.zeros(self.height) if beam_azimuth_angles is None else beam_azimuth_angles
+        self.az = np.ascontiguousarray(az, dtype=np.float32)
+        if self.elev.shape != (self.height,) or self.az.shape != (self.height,):
+            raise ValueError("one elevation and one azimuth angle per beam are required")
+        cfg = capi.LidarConfig(self.width, self.height, self.near_plane, self.far_plane, capi.fptr(self.elev),
+                               capi.fptr(self.az))
+        self.h = C.c_void_p()
+        capi.check(self.lib.se8_lidar_create(C.byref(cfg), C.byref(self.h)), "se8_lidar_create")
+
+    def close(self):
+        if getattr(self, "h", None) and self.h.value:
+            self.lib.se8_lidar_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def project(self, points_S):
+        """Projects sensor-frame points (n x 3, host) with the device beam model: (uv n x 2, ok n)."""
+        p = np.ascontiguousarray(np.asarray(points_S, dtype=np.float32).reshape(-1, 3))
+        uv = np.zeros((len(p), 2), np.float32)
+        ok = np.zeros(len(p), np.uint8)
+        capi.check(self.lib.se8_lidar_project(self.h, capi.fptr(p), len(p), capi.fptr(uv),
+                                              ok.ctypes.data_as(C.POINTER(C.c_uint8))), "se8_lidar_project")
+        return uv, ok.astype(bool)
+
+
 @dataclass
 class TrackerConfig:
     """se::TrackerConfig (tracker.hpp:22-33) + the compensated-reduction switch."""
@@ -191,8 +232,13 @@ class MapIntegrator:
     def __init__(self, map_: TSDFMap):
         self.map = map_
 
-    def integrateDepth(self, timestamp, depth, sensor: PinholeCamera, T_WC, stream=None):
+    def integrateDepth(self, timestamp, depth, sensor, T_WC, stream=None):
+        """depth: [H, W] float32 device tensor: z-depth for a PinholeCamera, range for an OusterLidar."""
         d = _dev_f32(depth, (sensor.height, sensor.width))
+        if isinstance(sensor, OusterLidar):
+            capi.check(self.map.lib.se8_integrate_range(self.map.h, int(timestamp), d, sensor.h, capi.pose16(T_WC),
+                                                        _stream_handle(stream)), "se8_integrate_range")
+            return
         capi.check(self.map.lib.se8_integrate_depth(self.map.h, int(timestamp), d, C.byref(sensor.c()),
                                                     capi.pose16(T_WC), _stream_handle(stream)),
                    "se8_integrate_depth")

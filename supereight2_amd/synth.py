@@ -217,3 +217,124 @@ def render_sequence_torch(scene: Scene, cam: Camera, poses, seed: int = 0, noise
                                 torch.zeros_like(z), z)
         out[k] = z.to(torch.float32)
     return out
+
+
+# ------------------------------------------------------------------------------------------------
+# Street workload for a spinning lidar: a ground plane z = 0, axis-aligned box buildings either
+# side of a straight street along x, and vertical cylinders (poles, trunks) on the pavements.  The
+# sensor frame is x forward, y left, z up; scans are taken at STREET_SENSOR_HEIGHT along y = 0.
+# ------------------------------------------------------------------------------------------------
+STREET_SENSOR_HEIGHT = 2.0
+
+
+@dataclass
+class LidarSpec:
+    """Beam layout of the scanner: `height` beams evenly over +-fov/2 of elevation (row 0 the
+    highest), no azimuth offsets, `width` columns over 360 degrees."""
+
+    width: int = 1024
+    height: int = 128
+    fov_deg: float = 45.0
+    near_plane: float = 0.5
+    far_plane: float = 60.0
+
+    def elevations_deg(self) -> np.ndarray:
+        return np.linspace(self.fov_deg / 2, -self.fov_deg / 2, self.height).astype(np.float32)
+
+    def azimuths_deg(self) -> np.ndarray:
+        return np.zeros(self.height, np.float32)
+
+
+@dataclass
+class Street:
+    box_lo: np.ndarray    # (k, 3)
+    box_hi: np.ndarray    # (k, 3)
+    cyl_xy: np.ndarray    # (m, 2)
+    cyl_r: np.ndarray     # (m,)
+    cyl_h: np.ndarray     # (m,)
+
+
+def make_street(seed: int = 0, n_boxes: int = 40, n_cylinders: int = 200, half_length: float = 100.0) -> Street:
+    """40 buildings with sides U(5, 20) m whose street-facing wall lies U(8, 14) m from the path,
+    and 200 cylinders of radius U(0.1, 0.4) m and height U(3, 8) m at U(2.5, 7) m from the path."""
+    rng = np.random.default_rng(seed)
+    size = rng.uniform(5.0, 20.0, (n_boxes, 3))
+    cx = rng.uniform(-half_length + 10.0, half_length - 10.0, n_boxes)
+    side = rng.choice([-1.0, 1.0], n_boxes)
+    near = rng.uniform(8.0, 14.0, n_boxes)
+    cy = side * (near + size[:, 1] / 2)
+    lo = np.stack([cx - size[:, 0] / 2, cy - size[:, 1] / 2, np.zeros(n_boxes)], 1)
+    hi = np.stack([cx + size[:, 0] / 2, cy + size[:, 1] / 2, size[:, 2]], 1)
+    px = rng.uniform(-half_length, half_length, n_cylinders)
+    py = rng.choice([-1.0, 1.0], n_cylinders) * rng.uniform(2.5, 7.0, n_cylinders)
+    r = rng.uniform(0.1, 0.4, n_cylinders)
+    h = rng.uniform(3.0, 8.0, n_cylinders)
+    return Street(lo, hi, np.stack([px, py], 1), r, h)
+
+
+def street_trajectory(n: int = 200, spacing: float = 0.5) -> np.ndarray:
+    """n sensor poses T_WS (float64) along y = 0 at STREET_SENSOR_HEIGHT, centred on x = 0."""
+    poses = np.tile(np.eye(4), (n, 1, 1))
+    poses[:, 0, 3] = (np.arange(n) - n / 2) * spacing
+    poses[:, 2, 3] = STREET_SENSOR_HEIGHT
+    return poses
+
+
+def lidar_directions(spec: LidarSpec) -> np.ndarray:
+    """Unit beam directions (H, W, 3) in float64: azimuth 2 pi u / W - pi, elevation per row."""
+    e = np.radians(spec.elevations_deg().astype(np.float64))[:, None]
+    phi = 2.0 * np.pi * np.arange(spec.width, dtype=np.float64)[None, :] / spec.width - np.pi
+    phi = phi + np.radians(spec.azimuths_deg().astype(np.float64))[:, None]
+    return np.stack([np.cos(e) * np.cos(phi), np.cos(e) * np.sin(phi), np.broadcast_to(np.sin(e), phi.shape)], -1)
+
+
+def render_street_ranges(street: Street, spec: LidarSpec, poses, seed: int = 0, noise: float = 0.02,
+                         device: str = "cpu"):
+    """Analytic range images [n, H, W] (float32 torch tensor on `device`) of the street seen from
+    each pose, with N(0, noise) range noise from a seeded generator on that device; beams with no
+    return within far_plane read 0."""
+    import torch
+
+    f64 = torch.float64
+    gen = torch.Generator(device=device)
+    gen.manual_seed(seed)
+    dS = torch.tensor(lidar_directions(spec), dtype=f64, device=device).reshape(-1, 3)
+    lo = torch.tensor(street.box_lo, dtype=f64, device=device)
+    hi = torch.tensor(street.box_hi, dtype=f64, device=device)
+    cxy = torch.tensor(street.cyl_xy, dtype=f64, device=device)
+    cr = torch.tensor(street.cyl_r, dtype=f64, device=device)
+    ch = torch.tensor(street.cyl_h, dtype=f64, device=device)
+    inf = float("inf")
+    out = torch.empty((len(poses), spec.height, spec.width), dtype=torch.float32, device=device)
+    for k, T in enumerate(poses):
+        T = torch.tensor(np.asarray(T, np.float64), dtype=f64, device=device)
+        d = dS @ T[:3, :3].T
+        o = T[:3, 3]
+        # ground plane
+        t = torch.where(d[:, 2] < 0, -o[2] / d[:, 2], torch.full_like(d[:, 2], inf))
+        # boxes (slab test), in chunks of boxes
+        inv = 1.0 / d
+        for b0 in range(0, len(street.box_lo), 8):
+            t1 = (lo[None, b0:b0 + 8] - o) * inv[:, None]
+            t2 = (hi[None, b0:b0 + 8] - o) * inv[:, None]
+            tn = torch.minimum(t1, t2).amax(-1)
+            tf = torch.maximum(t1, t2).amin(-1)
+            hit = (tn <= tf) & (tn > 0)
+            t = torch.minimum(t, torch.where(hit, tn, torch.full_like(tn, inf)).amin(-1))
+        # vertical cylinders: side wall between z = 0 and the cylinder's height
+        a = d[:, 0] ** 2 + d[:, 1] ** 2
+        for c0 in range(0, len(street.cyl_r), 16):
+            oc = o[None, :2] - cxy[c0:c0 + 16]
+            b = 2.0 * (d[:, None, 0] * oc[None, :, 0] + d[:, None, 1] * oc[None, :, 1])
+            cc = (oc ** 2).sum(-1)[None] - cr[None, c0:c0 + 16] ** 2
+            disc = b * b - 4.0 * a[:, None] * cc
+            t0 = (-b - torch.sqrt(torch.clamp(disc, min=0.0))) / (2.0 * a[:, None])
+            z = o[2] + t0 * d[:, None, 2]
+            hit = (disc > 0) & (t0 > 0) & (z >= 0) & (z <= ch[None, c0:c0 + 16])
+            t = torch.minimum(t, torch.where(hit, t0, torch.full_like(t0, inf)).amin(-1))
+        r = t
+        if noise > 0:
+            r = r + torch.randn(r.shape, generator=gen, dtype=f64, device=device) * noise
+        r = torch.where(torch.isfinite(t) & (t <= spec.far_plane), r, torch.zeros_like(r))
+        out[k] = r.reshape(spec.height, spec.width).to(torch.float32)
+    return out
