@@ -20,6 +20,10 @@ FP16-TSDF + compensated-ICP variant runs on the same frames and is reported besi
 
 Prints one JSON line: the frames/s of the FP32 path as `value`, the FP16 variant beside it, the
 roofline of the TSDF update kernel and the CPU-baseline rate of the oracle on a prefix of the frames.
+
+`--workload street` runs the spinning-lidar mapping workload instead (one step = one
+integrateDepth of a 1024x128 range scan; 200 scans along a 100 m street, 204.8 m map at 2 cm,
+truncation 4 voxels) and prints its own line in the same format, in scans/s.
 """
 from __future__ import annotations
 
@@ -49,6 +53,9 @@ def parse():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-fp16", action="store_true")
     ap.add_argument("--cpu-frames", type=int, default=8, help="frames of the CPU baseline sample")
+    ap.add_argument("--workload", choices=["room", "street"], default="room",
+                    help="room: the 640x480 track+integrate+raycast loop (default); street: 200 scans of a "
+                         "1024x128 spinning lidar along a 100 m street into a 204.8 m map at 2 cm (mapping only)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="first run one 256 MiB device-to-device copy, the known-size reference that scales "
                          "the FETCH_SIZE / WRITE_SIZE counters of a profiled run to bytes")
@@ -151,6 +158,122 @@ def cpu_baseline(depth_host, gt, n_frames, sensor):
     return (n_frames - 1) / dt, cores
 
 
+STREET_SCANS = 200
+STREET_MAX_BLOCKS = 6 << 20
+
+
+def run_street(ranges_dev, poses, warmup, storage, torch, se, lidar):
+    """Integrates every scan of the street sequence at its ground-truth pose (mapping only)."""
+    gmap = se.TSDFMap(204.8, 0.02, truncation_boundary_factor=4.0, storage=storage, max_blocks=STREET_MAX_BLOCKS)
+    integ = se.MapIntegrator(gmap)
+    t0 = None
+    for k in range(len(poses)):
+        if k == warmup:
+            torch.cuda.synchronize()
+            se.stage_times_reset(gmap)
+            t0 = time.perf_counter()
+        integ.integrateDepth(k, ranges_dev[k], lidar, poses[k])
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    st = se.stage_times(gmap)
+    info = gmap.info()
+    gmap.close()
+    ni = max(st["n_integrations"], 1)
+    timed = len(poses) - warmup
+    return {
+        "scans_per_s": timed / (t1 - t0) if timed > 0 else 0.0,
+        "seconds": t1 - t0,
+        "blocks": int(info.num_blocks),
+        "nodes": int(info.num_nodes),
+        "stage_ms": {
+            "fetch_frustum": st["sum_fetch_frustum"] / ni,
+            "create_list": st["sum_create_list"] / ni,
+            "allocate_list": st["sum_allocate_list"] / ni,
+            "update": st["sum_update"] / ni,
+        },
+        "update_blocks_per_scan": st["sum_update_blocks"] / ni,
+        "candidates_per_scan": st["sum_candidates"] / ni,
+        "integrations": st["n_integrations"],
+    }
+
+
+def street_main(args, torch, se, synth):
+    """The lidar workload: one step = one integrateDepth of a 1024x128 range scan."""
+    spec = synth.LidarSpec()
+    street = synth.make_street(args.seed)
+    n = min(args.warmup + args.steps, STREET_SCANS)
+    warmup = min(args.warmup, n - 1)
+    poses = synth.street_trajectory(STREET_SCANS)[:n].astype(np.float32)
+    print(f"rendering {n} scans", file=sys.stderr, flush=True)
+    ranges_dev = synth.render_street_ranges(street, spec, poses, seed=args.seed + 1, device="cuda")
+    torch.cuda.synchronize()
+    lidar = se.OusterLidar(spec.width, spec.height, spec.near_plane, spec.far_plane, spec.elevations_deg(),
+                           spec.azimuths_deg())
+    print("fp32 street", file=sys.stderr, flush=True)
+    r32 = run_street(ranges_dev, poses, warmup, "f32", torch, se, lidar)
+    r16 = None
+    if not args.no_fp16:
+        print("fp16 street", file=sys.stderr, flush=True)
+        r16 = run_street(ranges_dev, poses, warmup, "f16", torch, se, lidar)
+    lidar.close()
+
+    def update_roofline(r, s_v):
+        upd_ms = r["stage_ms"]["update"]
+        nbytes = 2 * s_v * 512 * r["update_blocks_per_scan"] + 4 * spec.width * spec.height
+        return nbytes, (nbytes / (upd_ms * 1e-3) / 1e9 if upd_ms > 0 else 0.0)
+
+    b32, g32 = update_roofline(r32, 8)
+    cpu = None
+    if not args.no_cpu_baseline:
+        print("cpu baseline", file=sys.stderr, flush=True)
+        from oracle import oracle as O
+
+        cores = os.cpu_count() or 1
+        O.set_threads(cores)
+        olid = O.OracleLidar(spec.width, spec.height, spec.near_plane, spec.far_plane, spec.elevations_deg(),
+                             spec.azimuths_deg())
+        om = O.OracleMap(204.8, 0.02, mu_factor=4.0)
+        host = ranges_dev[: args.cpu_frames].cpu().numpy()
+        t0 = None
+        for k in range(args.cpu_frames):
+            if k == 1:
+                t0 = time.perf_counter()
+            om.integrate_lidar(k, host[k], olid, poses[k])
+        cpu = {"value": (args.cpu_frames - 1) / (time.perf_counter() - t0), "unit": "scans/s", "cores": cores,
+               "kind": "port", "sample": f"scans 1..{args.cpu_frames - 1} of the same sequence (scan 0 untimed), "
+                                         f"integrate, OpenMP on {cores} threads, {cpu_model()}"}
+    fp16 = None
+    if r16 is not None:
+        b16, g16 = update_roofline(r16, 4)
+        fp16 = {"scans_per_s": r16["scans_per_s"], "blocks": r16["blocks"], "stage_ms": r16["stage_ms"],
+                "update_gbs": g16}
+    out = {
+        "metric": "scans/s integrate @1024x128 lidar, 2cm, 204.8m map; FP16-TSDF vs FP32",
+        "value": r32["scans_per_s"],
+        "unit": "scans/s",
+        "n_gpus": 1,
+        "steps": n - warmup,
+        "warmup": warmup,
+        "ms_per_step": 1000.0 * r32["seconds"] / max(n - warmup, 1),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic: seeded street (ground, 40 box buildings, 200 cylinders), analytic range, N(0, 2 cm) noise",
+        "config": {"workload": "fp32 TSDF mapping, 1024x128 beam lidar +-22.5 deg, 60 m range, 200 scans 0.5 m "
+                               "apart along a 100 m street, 204.8 m map at 2 cm, mu 4 voxels",
+                   "scans": n, "blocks": r32["blocks"], "nodes": r32["nodes"], "stage_ms": r32["stage_ms"],
+                   "update_blocks_per_scan": r32["update_blocks_per_scan"],
+                   "candidates_per_scan": r32["candidates_per_scan"]},
+        "fp16_tsdf": fp16,
+        "fp16_over_fp32": (r16["scans_per_s"] / r32["scans_per_s"]) if r16 is not None else None,
+        "roofline": {"bound": "hbm", "kernel": "k_update", "achieved": g32, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": g32 / HBM_PEAK_GBS, "traffic": None, "bytes_per_launch": b32},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out))
+
+
 def cpu_model():
     try:
         out = subprocess.run(["lscpu"], capture_output=True, text=True).stdout
@@ -175,6 +298,9 @@ def main():
     capi.load()
     if args.copy_calibration:
         copy_calibration(torch, capi)
+    if args.workload == "street":
+        street_main(args, torch, se, synth)
+        return
     cam = synth.Camera()
     sensor = se.PinholeCamera(cam.width, cam.height, cam.fx, cam.fy, cam.cx, cam.cy, cam.near_plane, cam.far_plane)
     n_frames = min(args.warmup + args.steps, N_FRAMES_DEFAULT) if args.warmup + args.steps <= N_FRAMES_DEFAULT \

@@ -8,6 +8,7 @@
 //   se::Tracker<MapT, SensorT>::track      include/se/tracker/tracker.hpp:42-73
 //   se::PinholeCamera, se::Image<T>        include/se/sensor/pinhole_camera.hpp:74-78,
 //                                          include/se/image/image.hpp:18-111
+//   se::OusterLidar                        include/se/sensor/ouster_lidar.hpp:12-80
 // Images live in host memory as in the reference; each call stages them through device buffers
 // owned by the object (or a per-thread scratch for raycast_volume).  Value types below provide the
 // subset of Eigen's interface the call sites use (x(), y(), z(), translation(), matrix data).
@@ -164,6 +165,50 @@ class PinholeCamera {
     se8_pinhole p_{};
 };
 
+// se::OusterLidar (ouster_lidar.hpp:12-80, ouster_lidar.cpp:59-85): one elevation and one azimuth
+// offset per beam in degrees (row 0 the highest beam, elevations strictly descending).  The beam
+// tables live on the device for the object's lifetime.
+class OusterLidar {
+    public:
+    struct Config {
+        int width = 0;
+        int height = 0;
+        float near_plane = 0.01f;
+        float far_plane = 10.f;
+        std::vector<float> beam_elevation_angles;
+        std::vector<float> beam_azimuth_angles;
+    };
+    explicit OusterLidar(const Config& c) : w_(c.width), h_(c.height)
+    {
+        if ((int) c.beam_elevation_angles.size() != c.height || (int) c.beam_azimuth_angles.size() != c.height)
+            throw std::invalid_argument("OusterLidar: one elevation and one azimuth angle per beam are required");
+        const se8_lidar_config lc{c.width, c.height, c.near_plane, c.far_plane, c.beam_elevation_angles.data(),
+                                  c.beam_azimuth_angles.data()};
+        detail::check(se8_lidar_create(&lc, &l_), "se8_lidar_create");
+    }
+    OusterLidar(const OusterLidar&) = delete;
+    OusterLidar& operator=(const OusterLidar&) = delete;
+    ~OusterLidar() { se8_lidar_destroy(l_); }
+    int imageWidth() const { return w_; }
+    int imageHeight() const { return h_; }
+    const se8_lidar* handle() const { return l_; }
+
+    private:
+    int w_, h_;
+    se8_lidar* l_ = nullptr;
+};
+
+namespace detail {
+inline int integrate(se8_map* m, int ts, const float* dev, const PinholeCamera& s, const float* T, se8_stream st)
+{
+    return se8_integrate_depth(m, ts, dev, &s.c(), T, st);
+}
+inline int integrate(se8_map* m, int ts, const float* dev, const OusterLidar& s, const float* T, se8_stream st)
+{
+    return se8_integrate_range(m, ts, dev, s.handle(), T, st);
+}
+} // namespace detail
+
 template<Res ResT = Res::Single>
 class TSDFMap {
     static_assert(ResT == Res::Single, "only the single-resolution TSDF map is provided");
@@ -268,9 +313,9 @@ class MapIntegrator {
         const size_t bytes = sizeof(float) * d.size();
         void* dev = depth_.get(bytes);
         detail::check(se8_copy(dev, d.data(), bytes, SE8_COPY_H2D, stream), "se8_copy");
-        detail::check(se8_integrate_depth(map_.handle(), timestamp, (const float*) dev, &m.depth.sensor.c(),
-                                          m.depth.T_WC.data(), stream),
-                      "se8_integrate_depth");
+        detail::check(detail::integrate(map_.handle(), timestamp, (const float*) dev, m.depth.sensor,
+                                        m.depth.T_WC.data(), stream),
+                      "integrateDepth");
         detail::check(se8_stream_synchronize(stream), "se8_stream_synchronize");
     }
 

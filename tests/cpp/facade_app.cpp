@@ -2,7 +2,9 @@
 // (app/src/main.cpp:126-160): track -> integrateDepth -> raycast_volume per frame, on synthetic
 // depth of a room corner (three planes, so point-to-plane ICP is fully constrained) seen by a
 // camera that slides 1 cm per frame along x.
-// Prints one line per frame: frame, tracking ok, estimated x, ground-truth x, centre hit depth.
+// Prints one line per frame: frame, tracking ok, estimated x, ground-truth x, centre hit depth;
+// then integrates two scans of a spinning lidar through se::OusterLidar and prints the field at a
+// ground point and a wall point.
 #include <cmath>
 #include <cstdio>
 
@@ -93,5 +95,47 @@ int main()
                                                           points(320, 240).z()));
     std::printf("interp %d %.5f blocks %lld worst_x_err %.6f\n", f_interp ? 1 : 0, f_interp ? *f_interp : 0.f,
                 (long long) map.info().num_blocks, worst);
+
+    // A 512 x 32 beam lidar at 2 m over the ground plane z = 0 with a wall at x = 12, two scans
+    // 0.5 m apart, into a 51.2 m map at 4 cm voxels; the field at a ground point is then ~0.
+    se::OusterLidar::Config lc;
+    lc.width = 512;
+    lc.height = 32;
+    lc.near_plane = 0.5f;
+    lc.far_plane = 40.f;
+    for (int v = 0; v < lc.height; v++) {
+        lc.beam_elevation_angles.push_back(10.f - 30.f * (float) v / (float) (lc.height - 1));
+        lc.beam_azimuth_angles.push_back(0.f);
+    }
+    const se::OusterLidar lidar(lc);
+    se::TSDFDataConfig dc;
+    dc.truncation_boundary_factor = 4.f;
+    se::TSDFMap<se::Res::Single> lmap(se::Vector3f::Constant(51.2f), 0.04f, dc);
+    se::MapIntegrator lint(lmap);
+    const double kPi = 3.14159265358979323846;
+    for (int scan = 0; scan < 2; scan++) {
+        const double o[3] = {0.5 * scan, 0.0, 2.0};
+        se::Image<float> range(lc.width, lc.height);
+        for (int v = 0; v < lc.height; v++)
+            for (int u = 0; u < lc.width; u++) {
+                const double e = lc.beam_elevation_angles[v] * kPi / 180.0;
+                const double a = 2.0 * kPi * u / lc.width - kPi;
+                const double d[3] = {std::cos(e) * std::cos(a), std::cos(e) * std::sin(a), std::sin(e)};
+                double t = 1e9;
+                if (d[2] < 0)
+                    t = -o[2] / d[2];
+                if (d[0] > 0)
+                    t = std::fmin(t, (12.0 - o[0]) / d[0]);
+                range(u, v) = t <= lc.far_plane ? (float) t : 0.f;
+            }
+        se::Isometry3f T = se::Isometry3f::Identity();
+        T(0, 3) = (float) o[0];
+        T(2, 3) = (float) o[2];
+        lint.integrateDepth(scan, se::Measurements{se::Measurement{range, lidar, T}});
+    }
+    const auto g = lmap.getFieldInterp(se::Vector3f(-6.f, 1.f, 0.f));
+    const auto w = lmap.getFieldInterp(se::Vector3f(12.f, -2.f, 1.f));
+    std::printf("lidar ground %d %.5f wall %d %.5f blocks %lld\n", g ? 1 : 0, g ? *g : 0.f, w ? 1 : 0, w ? *w : 0.f,
+                (long long) lmap.info().num_blocks);
     return failures == 0 ? 0 : 1;
 }

@@ -1,5 +1,6 @@
 """The header-only C++ facade (include/se8/supereight.hpp) driven like the reference app:
-track -> integrateDepth -> raycast_volume per frame, on synthetic depth of a room corner."""
+track -> integrateDepth -> raycast_volume per frame, on synthetic depth of a room corner, then
+integrateDepth with an OusterLidar over a ground plane and a wall."""
 import os
 import re
 import subprocess
@@ -28,3 +29,8 @@ def test_facade_app_tracks_and_maps():
         assert sc == "0"
     m = re.search(r"interp (\d) (\S+) blocks (\d+)", res.stdout)
     assert m and m.group(1) == "1" and abs(float(m.group(2))) < 0.2 and int(m.group(3)) > 1000
+    # lidar: the field at points on the ground plane and on the wall is the zero crossing, to within
+    # the interpolation error of a 4 cm grid (mu = 16 cm, so 0.15 is 2.4 cm)
+    m = re.search(r"lidar ground (\d) (\S+) wall (\d) (\S+) blocks (\d+)", res.stdout)
+    assert m and m.group(1) == "1" and m.group(3) == "1" and int(m.group(5)) > 1000
+    assert abs(float(m.group(2))) < 0.15 and abs(float(m.group(4))) < 0.15
