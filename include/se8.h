@@ -174,6 +174,21 @@ int se8_icp_cost_grad_batched(se8_tracker* tracker, const float* depth_dev, cons
 int se8_raycast_depth_grad_batched(const se8_map* map, const se8_pinhole* sensor, int n_poses, const float* poses16,
                                    float* depth_dev, float* grad_dev, uint8_t* valid_dev, se8_stream stream);
 
+/* -------- device-resident poses (no counterpart in the reference: its poses live on the host).
+ * The same calls with the pose as 16 floats (row-major 4x4) in device memory, read by the first
+ * kernel of the call, so that a pose one call produces on the GPU feeds the next without a host
+ * round trip: se8_track_dev writes the tracked pose (and, if ok_dev is set, the check result) to
+ * device memory and returns without waiting; T_in_dev may equal T_out_dev.  Results are identical
+ * to the host-pose calls given the same pose. -------- */
+int se8_integrate_depth_dev(se8_map* map, int timestamp, const float* depth_dev, const se8_pinhole* sensor,
+                            const float* T_WC_dev, se8_stream stream);
+int se8_integrate_range_dev(se8_map* map, int timestamp, const float* range_dev, const se8_lidar* lidar,
+                            const float* T_WS_dev, se8_stream stream);
+int se8_raycast_dev(const se8_map* map, const se8_pinhole* sensor, const float* T_WS_dev, float* points_dev,
+                    float* normals_dev, int8_t* scale_dev, se8_stream stream);
+int se8_track_dev(se8_tracker* tracker, const float* depth_dev, const float* T_in_dev, float* T_out_dev,
+                  const float* ref_points_dev, const float* ref_normals_dev, int* ok_dev, se8_stream stream);
+
 /* -------- map queries (host arrays) -------- */
 /* se::allocator::blocks(voxel_coords, octree, root, only_allocated)
  * (include/se/map/octree/impl/allocator_impl.hpp:40-116): voxel_coords is n*3 ints; the block

@@ -137,6 +137,10 @@ SIGNATURES = {
     "se8_track_raycast": (C.c_int, [_P, _P, _P, _F, _I, _P]),
     "se8_icp_cost_grad_batched": (C.c_int, [_P, _P, _P, _P, _F, C.c_int, _F, _F, _F, _F, _I, _P]),
     "se8_raycast_depth_grad_batched": (C.c_int, [_P, C.POINTER(Pinhole), C.c_int, _F, _P, _P, _P, _P]),
+    "se8_integrate_depth_dev": (C.c_int, [_P, C.c_int, _P, C.POINTER(Pinhole), _P, _P]),
+    "se8_integrate_range_dev": (C.c_int, [_P, C.c_int, _P, _P, _P, _P]),
+    "se8_raycast_dev": (C.c_int, [_P, C.POINTER(Pinhole), _P, _P, _P, _P, _P]),
+    "se8_track_dev": (C.c_int, [_P, _P, _P, _P, _P, _P, _P, _P]),
     "se8_allocate_blocks": (C.c_int, [_P, _I, C.c_int64, C.c_int, _I, C.POINTER(C.c_int64)]),
     "se8_write_block": (C.c_int, [_P, _I, _F, _F]),
     "se8_read_block": (C.c_int, [_P, _I, _F, _F]),
@@ -168,6 +172,13 @@ def load(path: str = LIB_PATH) -> C.CDLL:
         return _lib
     if not os.path.exists(path):
         raise Se8Error(f"{path} is missing: build it with supereight2_amd.build.build() (no CPU fallback exists)")
+    # torch ships its own HIP runtime; load it first so that libse8 binds to the same runtime
+    # instance that owns the torch tensors handed to it (a second runtime in the process cannot
+    # allocate on the device once the first holds it)
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     lib = C.CDLL(path)
     for name, (res, args) in SIGNATURES.items():
         fn = getattr(lib, name)

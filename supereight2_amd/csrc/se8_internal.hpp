@@ -100,7 +100,7 @@ SE8_HD Iso iso_inv(const Iso& A)
     return o;
 }
 
-inline Iso iso_from16(const float* m)
+SE8_HD Iso iso_from16(const float* m)
 {
     Iso T;
     for (int i = 0; i < 3; i++) {
@@ -111,7 +111,7 @@ inline Iso iso_from16(const float* m)
     return T;
 }
 
-inline void iso_to16(const Iso& T, float* m)
+SE8_HD void iso_to16(const Iso& T, float* m)
 {
     for (int i = 0; i < 3; i++) {
         for (int j = 0; j < 3; j++)
@@ -471,6 +471,33 @@ struct MapView {
     int64_t max_blocks, max_nodes, max_bricks;
 };
 
+// Per-integration poses derived from T_WS (in device memory, so that the pose itself may come
+// from device memory): the sensor pose, its inverse for the frustum test and T_CV of the updater.
+struct FramePoses {
+    Iso T_WS;
+    Iso T_SW;
+    Iso T_CV;
+};
+
+// T_CV = T_CW * T_WM * Scaling(res) * Translation(0.5) (singleres_tsdf_updater_impl.hpp:37-42)
+SE8_HD Iso compute_T_CV(const MapView& v, const Iso& T_WC)
+{
+    const Iso T_CW = iso_inv(T_WC);
+    Iso T_WM;
+    for (int i = 0; i < 3; i++) {
+        for (int j = 0; j < 3; j++)
+            T_WM.R[i][j] = i == j ? 1.f : 0.f;
+        T_WM.t[i] = -v.t_MW[i];
+    }
+    Iso A = iso_mul(T_CW, T_WM);
+    for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++)
+            A.R[i][j] = A.R[i][j] * v.res;
+    for (int i = 0; i < 3; i++)
+        A.t[i] = A.t[i] + (A.R[i][0] * 0.5f + (A.R[i][1] * 0.5f + A.R[i][2] * 0.5f));
+    return A;
+}
+
 enum Counter {
     kCntBlocks = 0,
     kCntNodes = 1,
@@ -567,6 +594,7 @@ struct se8_map {
         voxels, counters;
     // per-frame scratch
     se8::DeviceBuffer update_list, candidates, unique_keys, cur_node;
+    se8::DeviceBuffer frame_poses; // FramePoses of the integration in flight
     se8::RadixSorter sorter;
     size_t candidates_cap = 0;
     int* host_counters = nullptr; // pinned copy of counters taken at the end of each integration

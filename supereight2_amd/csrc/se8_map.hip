@@ -136,13 +136,19 @@ __global__ void k_fill_i32(int* p, int64_t n, int v)
         p[i] = v;
 }
 
-__global__ void k_frame_reset(int* counters)
+// Start of an integration: clears the per-frame counters and derives the frame's poses from T_WS,
+// read from device memory when T16_dev is set, else taken from the argument.
+__global__ void k_frame_begin(MapView m, Iso T_arg, const float* T16_dev, FramePoses* fp)
 {
     if (threadIdx.x == 0) {
-        counters[kCntFrustum] = 0;
-        counters[kCntUpdate] = 0;
-        counters[kCntCandidates] = 0;
-        counters[kCntUnique] = 0;
+        m.counters[kCntFrustum] = 0;
+        m.counters[kCntUpdate] = 0;
+        m.counters[kCntCandidates] = 0;
+        m.counters[kCntUnique] = 0;
+        const Iso T = T16_dev ? iso_from16(T16_dev) : T_arg;
+        fp->T_WS = T;
+        fp->T_SW = iso_inv(T);
+        fp->T_CV = compute_T_CV(m, T);
     }
 }
 
@@ -180,9 +186,10 @@ __device__ __forceinline__ int wave_append_slot(bool pred, int* counter)
 // root pass sphereInFrustum (the DFS of iterator_impl.hpp:86-156 never prunes at the root).
 // ----------------------------------------------------------------------------------------------
 template<class Sensor>
-__global__ void k_frustum(MapView m, Iso T_SW, Sensor sensor, int* update_list)
+__global__ void k_frustum(MapView m, const FramePoses* fp, Sensor sensor, int* update_list)
 {
     const int n = m.counters[kCntBlocks];
+    const Iso T_SW = fp->T_SW;
     const float r_unit = sqrtf(3.0f) / 2.0f * m.res;
     for (int base = blockIdx.x * blockDim.x; base < n; base += gridDim.x * blockDim.x) {
         const int slot = base + threadIdx.x;
@@ -221,9 +228,10 @@ __global__ void k_record_frustum(int* counters)
 // create-list: band march of RaycastCarver (raycast_carver_impl.hpp:74-111)
 // ----------------------------------------------------------------------------------------------
 template<class Sensor>
-__global__ void k_alloc_rays(MapView m, Sensor sensor, Iso T_WS, const float* depth, float band, int num_steps,
-                             uint64_t* cand, int64_t cand_cap)
+__global__ void k_alloc_rays(MapView m, Sensor sensor, const FramePoses* fp, const float* depth, float band,
+                             int num_steps, uint64_t* cand, int64_t cand_cap)
 {
+    const Iso T_WS = fp->T_WS;
     const int x = blockIdx.x * blockDim.x + threadIdx.x;
     const int y = blockIdx.y;
     bool live = x < sensor.width();
@@ -544,9 +552,10 @@ __global__ void k_build_blocks(MapView m, const uint64_t* keys, const int* n_dev
 // update: one block (512 voxels) per workgroup of 8 wave64s
 // ----------------------------------------------------------------------------------------------
 template<int Storage, class Sensor>
-__global__ __launch_bounds__(512) void k_update(MapView m, const int* update_list, Iso T_CV, Sensor sensor,
-                                                const float* depth, int ts)
+__global__ __launch_bounds__(512) void k_update(MapView m, const int* update_list, const FramePoses* fp,
+                                                Sensor sensor, const float* depth, int ts)
 {
+    const Iso T_CV = fp->T_CV;
     const int n = m.counters[kCntUpdate];
     const int t = threadIdx.x;
     const int vx = t & 7, vy = (t >> 3) & 7, vz = t >> 6;
@@ -603,25 +612,6 @@ __global__ __launch_bounds__(512) void k_update(MapView m, const int* update_lis
                 reinterpret_cast<uint32_t*>(m.voxels)[vi] = pack_f16(tsdf, weight);
         }
     }
-}
-
-// T_CV = T_CW * T_WM * Scaling(res) * Translation(0.5) (singleres_tsdf_updater_impl.hpp:37-42)
-static Iso compute_T_CV(const se8_map* map, const Iso& T_WC)
-{
-    const Iso T_CW = iso_inv(T_WC);
-    Iso T_WM;
-    std::memset(&T_WM, 0, sizeof(T_WM));
-    for (int i = 0; i < 3; i++) {
-        T_WM.R[i][i] = 1.f;
-        T_WM.t[i] = -map->view.t_MW[i];
-    }
-    Iso A = iso_mul(T_CW, T_WM);
-    for (int i = 0; i < 3; i++)
-        for (int j = 0; j < 3; j++)
-            A.R[i][j] = A.R[i][j] * map->view.res;
-    for (int i = 0; i < 3; i++)
-        A.t[i] = A.t[i] + (A.R[i][0] * 0.5f + (A.R[i][1] * 0.5f + A.R[i][2] * 0.5f));
-    return A;
 }
 
 static int read_counters(const se8_map* map, int* host, hipStream_t s)
@@ -802,6 +792,7 @@ extern "C" int se8_map_create(const se8_map_config* cfg, const se8_tsdf_config* 
     rc = rc ? rc : dev_alloc(m->voxels, vox_bytes * 512 * (size_t) max_blocks);
     rc = rc ? rc : dev_alloc(m->counters, sizeof(int) * kNumCounters);
     rc = rc ? rc : dev_alloc(m->update_list, sizeof(int) * max_blocks);
+    rc = rc ? rc : dev_alloc(m->frame_poses, sizeof(FramePoses));
     if (rc != SE8_OK) {
         se8_map_destroy(m);
         return rc;
@@ -874,7 +865,8 @@ extern "C" int se8_map_destroy(se8_map* m)
     (void) hipDeviceSynchronize();
     DeviceBuffer* bufs[] = {&m->brick_top,    &m->brick_slots, &m->node_child, &m->node_key,   &m->node_ts,
                             &m->block_key,    &m->block_parent, &m->block_ts,  &m->voxels,     &m->counters,
-                            &m->update_list,  &m->candidates,   &m->unique_keys, &m->cur_node};
+                            &m->update_list,  &m->candidates,   &m->unique_keys, &m->cur_node,
+                            &m->frame_poses};
     for (DeviceBuffer* b : bufs)
         dev_free(*b);
     m->sorter.release();
@@ -912,13 +904,14 @@ extern "C" int se8_map_get_info(const se8_map* m, se8_map_info* out)
 
 namespace se8 {
 
-// integrateDepth (map_integrator_impl.hpp:52-83) for either sensor; `what` names the entry point
-// in error messages.
+// integrateDepth (map_integrator_impl.hpp:52-83) for either sensor; the pose is T_WC16 (host) or,
+// when T16_dev is set, 16 floats in device memory read by the first kernel.  `what` names the
+// entry point in error messages.
 template<class Sensor>
 static int integrate_impl(se8_map* m, int timestamp, const float* depth_dev, const Sensor& sensor,
-                          const float T_WC16[16], hipStream_t s, const char* what)
+                          const float* T_WC16, const float* T16_dev, hipStream_t s, const char* what)
 {
-    const Iso T_WC = iso_from16(T_WC16);
+    const Iso T_arg = T_WC16 ? iso_from16(T_WC16) : Iso{};
     const float band = 2.0f * m->view.res * m->tsdf.truncation_boundary_factor;
     const int num_steps = (int) std::ceil(band / (2.f * m->view.res));
     const size_t cand_cap = (size_t) sensor.width() * sensor.height() * num_steps;
@@ -937,14 +930,15 @@ static int integrate_impl(se8_map* m, int timestamp, const float* depth_dev, con
             return SE8_ERR_CAPACITY;
         }
     }
+    FramePoses* fp = (FramePoses*) m->frame_poses.ptr;
     SE8_HIP_CHECK(hipEventRecord(m->ev[0], s));
-    k_frame_reset<<<1, 64, 0, s>>>(m->view.counters);
+    k_frame_begin<<<1, 64, 0, s>>>(m->view, T_arg, T16_dev, fp);
     const int cus = device_count_sm();
-    k_frustum<Sensor><<<cus * 4, 256, 0, s>>>(m->view, iso_inv(T_WC), sensor, (int*) m->update_list.ptr);
+    k_frustum<Sensor><<<cus * 4, 256, 0, s>>>(m->view, fp, sensor, (int*) m->update_list.ptr);
     k_record_frustum<<<1, 64, 0, s>>>(m->view.counters);
     SE8_HIP_CHECK(hipEventRecord(m->ev[1], s));
     dim3 grid((sensor.width() + 63) / 64, sensor.height());
-    k_alloc_rays<Sensor><<<grid, 64, 0, s>>>(m->view, sensor, T_WC, depth_dev, band, num_steps,
+    k_alloc_rays<Sensor><<<grid, 64, 0, s>>>(m->view, sensor, fp, depth_dev, band, num_steps,
                                              (uint64_t*) m->candidates.ptr, (int64_t) cand_cap);
     SE8_HIP_CHECK(hipGetLastError());
     SE8_HIP_CHECK(hipEventRecord(m->ev[2], s));
@@ -953,18 +947,17 @@ static int integrate_impl(se8_map* m, int timestamp, const float* depth_dev, con
     if (rc != SE8_OK)
         return rc;
     // ev[3] / ev[4] are stamped at the start and end of the update kernel itself
-    const Iso T_CV = compute_T_CV(m, T_WC);
     const int* ulist = (const int*) m->update_list.ptr;
     // one resident wave of workgroups: the grid-stride loop then never leaves a second, partial
     // round of workgroups waiting for the first to finish
     if (m->view.storage == SE8_STORAGE_F32)
         hipExtLaunchKernelGGL(k_update<SE8_STORAGE_F32, Sensor>,
                               dim3(resident_grid(k_update<SE8_STORAGE_F32, Sensor>, 512)), dim3(512), 0, s, m->ev[3],
-                              m->ev[4], 0, m->view, ulist, T_CV, sensor, depth_dev, timestamp);
+                              m->ev[4], 0, m->view, ulist, (const FramePoses*) fp, sensor, depth_dev, timestamp);
     else
         hipExtLaunchKernelGGL(k_update<SE8_STORAGE_F16, Sensor>,
                               dim3(resident_grid(k_update<SE8_STORAGE_F16, Sensor>, 512)), dim3(512), 0, s, m->ev[3],
-                              m->ev[4], 0, m->view, ulist, T_CV, sensor, depth_dev, timestamp);
+                              m->ev[4], 0, m->view, ulist, (const FramePoses*) fp, sensor, depth_dev, timestamp);
     SE8_HIP_CHECK(hipGetLastError());
     SE8_HIP_CHECK(hipMemcpyAsync(m->host_counters, m->counters.ptr, sizeof(int) * kNumCounters,
                                  hipMemcpyDeviceToHost, s));
@@ -1010,7 +1003,17 @@ extern "C" int se8_integrate_depth(se8_map* m, int timestamp, const float* depth
     SE8_REQUIRE(m && depth_dev && sensor && T_WC16, "se8_integrate_depth: null argument");
     SE8_REQUIRE(sensor->width > 0 && sensor->height > 0, "se8_integrate_depth: empty image");
     const PinholeSensor ps{make_cam(*sensor, 1, false)};
-    return integrate_impl(m, timestamp, depth_dev, ps, T_WC16, (hipStream_t) stream, "se8_integrate_depth");
+    return integrate_impl(m, timestamp, depth_dev, ps, T_WC16, nullptr, (hipStream_t) stream, "se8_integrate_depth");
+}
+
+extern "C" int se8_integrate_depth_dev(se8_map* m, int timestamp, const float* depth_dev, const se8_pinhole* sensor,
+                                       const float* T_WC_dev, se8_stream stream)
+{
+    SE8_REQUIRE(m && depth_dev && sensor && T_WC_dev, "se8_integrate_depth_dev: null argument");
+    SE8_REQUIRE(sensor->width > 0 && sensor->height > 0, "se8_integrate_depth_dev: empty image");
+    const PinholeSensor ps{make_cam(*sensor, 1, false)};
+    return integrate_impl(m, timestamp, depth_dev, ps, nullptr, T_WC_dev, (hipStream_t) stream,
+                          "se8_integrate_depth_dev");
 }
 
 extern "C" int se8_lidar_create(const se8_lidar_config* cfg, se8_lidar** out)
@@ -1082,7 +1085,16 @@ extern "C" int se8_integrate_range(se8_map* m, int timestamp, const float* range
 {
     SE8_REQUIRE(m && range_dev && lidar && T_WS16, "se8_integrate_range: null argument");
     const LidarSensor ls{lidar->model};
-    return integrate_impl(m, timestamp, range_dev, ls, T_WS16, (hipStream_t) stream, "se8_integrate_range");
+    return integrate_impl(m, timestamp, range_dev, ls, T_WS16, nullptr, (hipStream_t) stream, "se8_integrate_range");
+}
+
+extern "C" int se8_integrate_range_dev(se8_map* m, int timestamp, const float* range_dev, const se8_lidar* lidar,
+                                       const float* T_WS_dev, se8_stream stream)
+{
+    SE8_REQUIRE(m && range_dev && lidar && T_WS_dev, "se8_integrate_range_dev: null argument");
+    const LidarSensor ls{lidar->model};
+    return integrate_impl(m, timestamp, range_dev, ls, nullptr, T_WS_dev, (hipStream_t) stream,
+                          "se8_integrate_range_dev");
 }
 
 extern "C" int se8_lidar_project(const se8_lidar* lidar, const float* points, int64_t n, float* uv,

@@ -577,9 +577,11 @@ __global__ __launch_bounds__(kRayThreads) void k_raycast_depth_grad(MapView m, C
     valid[idx] = ok;
 }
 
-__global__ __launch_bounds__(kRayThreads) void k_raycast(MapView m, Cam cam, Iso T_WS, float* points, float* normals,
-                                                         int8_t* scale_out)
+__global__ __launch_bounds__(kRayThreads) void k_raycast(MapView m, Cam cam, Iso T_arg, const float* T16_dev,
+                                                         float* points, float* normals, int8_t* scale_out)
 {
+    // the pose comes from device memory when T16_dev is set (a pose a previous kernel produced)
+    const Iso T_WS = T16_dev ? iso_from16(T16_dev) : T_arg;
     // traversal stack: (max_scale - 3) levels of {parent, t_max} per thread, sized at launch
     extern __shared__ int ray_stack[];
     int* st_parent = ray_stack;
@@ -689,12 +691,12 @@ static int run_query(const se8_map* m, const float* pts, int64_t n, int mode, st
 
 // ev_start / ev_stop, when given, are stamped at the start and end of the kernel itself (not at
 // enqueue time), so their difference is the kernel's execution time.
-int launch_raycast(const se8_map* m, const Cam& cam, const Iso& T_WS, float* points, float* normals, int8_t* scale,
-                   hipStream_t s, hipEvent_t ev_start, hipEvent_t ev_stop)
+int launch_raycast(const se8_map* m, const Cam& cam, const Iso& T_WS, const float* T16_dev, float* points,
+                   float* normals, int8_t* scale, hipStream_t s, hipEvent_t ev_start, hipEvent_t ev_stop)
 {
     const dim3 grid((cam.w + 15) / 16, (cam.h + 7) / 8);
     hipExtLaunchKernelGGL(k_raycast, grid, dim3(kRayThreads), (uint32_t) ray_stack_bytes(m->view), s, ev_start,
-                          ev_stop, 0, m->view, cam, T_WS, points, normals, scale);
+                          ev_stop, 0, m->view, cam, T_WS, T16_dev, points, normals, scale);
     SE8_HIP_CHECK(hipGetLastError());
     return SE8_OK;
 }
@@ -712,8 +714,24 @@ extern "C" int se8_raycast(const se8_map* m, const se8_pinhole* sensor, const fl
     hipStream_t s = (hipStream_t) stream;
     se8_map* mm = const_cast<se8_map*>(m);
     const Cam cam = make_cam(*sensor, 1, false);
-    harvest_raycast(mm, false);
-    int rc = launch_raycast(m, cam, iso_from16(T_WS16), points_dev, normals_dev, scale_dev, s, mm->ev[5], mm->ev[6]);
+    harvest_raycast(mm, true);
+    int rc = launch_raycast(m, cam, iso_from16(T_WS16), nullptr, points_dev, normals_dev, scale_dev, s, mm->ev[5],
+                            mm->ev[6]);
+    mm->ray_pending = true;
+    return rc;
+}
+
+extern "C" int se8_raycast_dev(const se8_map* m, const se8_pinhole* sensor, const float* T_WS_dev, float* points_dev,
+                               float* normals_dev, int8_t* scale_dev, se8_stream stream)
+{
+    SE8_REQUIRE(m && sensor && T_WS_dev && points_dev && normals_dev, "se8_raycast_dev: null argument");
+    SE8_REQUIRE(sensor->width > 0 && sensor->height > 0, "se8_raycast_dev: empty image");
+    SE8_REQUIRE(m->view.max_scale - kBlockScale <= kMaxLevels, "se8_raycast_dev: octree too deep");
+    hipStream_t s = (hipStream_t) stream;
+    se8_map* mm = const_cast<se8_map*>(m);
+    const Cam cam = make_cam(*sensor, 1, false);
+    harvest_raycast(mm, true);
+    int rc = launch_raycast(m, cam, Iso{}, T_WS_dev, points_dev, normals_dev, scale_dev, s, mm->ev[5], mm->ev[6]);
     mm->ray_pending = true;
     return rc;
 }

@@ -60,6 +60,8 @@ struct IcpState {
 struct IcpShared {
     IcpState result;
     IcpSlot slot[2];
+    Iso T_ref;     // pose at the start of the call (restored by a failed check)
+    Iso T_ref_inv; // its inverse: projection of world points into the reference maps
 };
 
 struct IcpArgs {
@@ -69,14 +71,16 @@ struct IcpArgs {
     const float* ref_nrm;  // full-res raycast normals
     float* partials;       // two buffers of kPartialStride floats, by launch parity
     IcpShared* sh;
-    Iso T_ref_inv;
-    Iso T_ref;             // pose at the start of the call (restored by a failed check)
+    Iso T_ref_inv;         // batched adjoint only; the tracker reads sh->T_ref_inv
     Cam cam;               // full-resolution sensor (projection into the reference maps)
     int w, h;              // level size
     int n_tiles;           // tiles of kIcpThreads * PPT pixels
     float dist_th, normal_th, icp_th, track_th;
     int level;
     int k;                 // launch index within the call
+    int prev_rows;         // partial rows launch k - 1 wrote if it ran its pixel pass (its grid size)
+    float* T_out_dev;      // k_icp_finish: the tracked pose also goes here (16 floats) when set
+    int* ok_dev;           // k_icp_finish: the check result also goes here when set
 };
 
 // half_sample_robust_image (preprocessor.cpp:156-195) of a w x h image at output pixel (x, y):
@@ -152,24 +156,30 @@ struct PyrArgs {
     int n[3];           // pixels per level
     int levels;
     int neg_y;
-    IcpSlot* init;      // when not null, thread 0 writes the state that the first launch reads
-    Iso T;
+    IcpShared* init;    // when not null, thread 0 writes the state that the first launch reads
+    Iso T;              // start pose, unless T16_dev is set
+    const float* T16_dev;
 };
 
 // The whole input side of Tracker::track in one launch: for every pixel of every pyramid level,
 // depth_to_point_cloud (preprocessor_impl.hpp:18-40) and point_cloud_to_normal<NegY>
 // (preprocessor.cpp:109-152) on the robustly half-sampled depth (tracker_impl.hpp:64-75).
-// Threads [0, n0) are level-0 pixels, [n0, n0+n1) level 1, then level 2.  Neighbour depths are
-// recomputed from the full-resolution image (each level-2 pixel reads at most 80 of its texels).
+// Threads [0, n2) are level-2 pixels, then level 1, then level 0 (with fewer levels, the coarsest
+// first).  Neighbour depths are recomputed from the full-resolution image (each level-2 pixel
+// reads at most 80 of its texels).
 __global__ __launch_bounds__(kPyrThreads) void k_pyramid(PyrArgs a)
 {
     int i = blockIdx.x * kPyrThreads + threadIdx.x;
     if (a.init && i == 0) {
-        IcpSlot* st = a.init;
+        // the first Gauss-Newton launch (k = 0) reads slot 1
+        IcpSlot* st = &a.init->slot[1];
+        const Iso T = a.T16_dev ? iso_from16(a.T16_dev) : a.T;
+        a.init->T_ref = T;
+        a.init->T_ref_inv = iso_inv(T);
         for (int r = 0; r < 3; r++) {
             for (int c = 0; c < 3; c++)
-                st->T[4 * r + c] = a.T.R[r][c];
-            st->T[4 * r + 3] = a.T.t[r];
+                st->T[4 * r + c] = T.R[r][c];
+            st->T[4 * r + 3] = T.t[r];
         }
         for (int k = 0; k < 32; k++) {
             st->sums[k] = 0.f;
@@ -182,12 +192,14 @@ __global__ __launch_bounds__(kPyrThreads) void k_pyramid(PyrArgs a)
         st->rows = 0;
         st->iterations = 0;
     }
-    int level = 0;
-    while (level < a.levels && i >= a.n[level]) {
+    // coarsest level first: its pixels cost the most (each recomputes up to 80 texels), and
+    // dispatching them first keeps them from forming the tail of the launch
+    int level = a.levels - 1;
+    while (level >= 0 && i >= a.n[level]) {
         i -= a.n[level];
-        level++;
+        level--;
     }
-    if (level >= a.levels)
+    if (level < 0)
         return;
     const Cam& c = a.cam[level];
     const int x = i % c.w, y = i / c.w;
@@ -248,7 +260,8 @@ struct IcpAssoc {
     int res; // 1: associated; -1: no input normal; -2: projection failed
 };
 
-__device__ __forceinline__ IcpAssoc icp_associate(const IcpArgs& a, const Iso& T, float4 v4, float4 n4)
+__device__ __forceinline__ IcpAssoc icp_associate(const IcpArgs& a, const Iso& T_ref_inv, const Iso& T, float4 v4,
+                                                  float4 n4)
 {
     IcpAssoc o;
     o.n_in[0] = n4.x;
@@ -257,7 +270,7 @@ __device__ __forceinline__ IcpAssoc icp_associate(const IcpArgs& a, const Iso& T
     const float v[3] = {v4.x, v4.y, v4.z};
     float pR[3], uv[2] = {0.f, 0.f};
     iso_apply(T, v, o.pW);
-    iso_apply(a.T_ref_inv, o.pW, pR);
+    iso_apply(T_ref_inv, o.pW, pR);
     const bool proj = cam_project(a.cam, pR, uv);
     const int px = proj ? (int) (uv[0] + 0.5f) : 0, py = proj ? (int) (uv[1] + 0.5f) : 0;
     o.ri = px + py * a.cam.w;
@@ -304,7 +317,7 @@ __device__ __forceinline__ void gather_ref(const IcpArgs& a, int ri, float rp[3]
 // One pixel of trackKernel; returns the result code and fills e, J on success.
 __device__ __forceinline__ int icp_pixel(const IcpArgs& a, const Iso& T, int p, float& e, float J[6])
 {
-    const IcpAssoc s = icp_associate(a, T, a.vert[p], a.norm[p]);
+    const IcpAssoc s = icp_associate(a, a.T_ref_inv, T, a.vert[p], a.norm[p]);
     float rp[3], rn[3];
     gather_ref(a, s.ri, rp, rn);
     return icp_residual(a, T, s, rp, rn, e, J);
@@ -577,78 +590,116 @@ __device__ __forceinline__ void load_tile(const IcpArgs& a, int tile, int n, flo
     }
 }
 
-// Workgroup-wide, identical in every workgroup: copy the previous launch's slot into `cur` (LDS)
-// and, if that launch ran a pixel pass, reduce its partial rows in a fixed order, solve, apply
-// T <- exp(x) T and raise its level's convergence flag (icp.cpp:15-41, 208-223).  Thread t owns
-// terms 4q..4q+3 (q = t & 7) of rows b = g, g + 32, ... (g = t >> 3); it issues all its 16-byte
-// loads before adding them in increasing b; then thread k < 32 adds the 32 group sums of term k in
-// increasing g.  The rows were written by the previous kernel, so plain loads see them.
+// Solving the previous launch's pass is split in three so that its memory round trips overlap:
+// load_rows issues the loads of the partial rows (their count is known on the host, so they do not
+// wait for the slot), slot_word / put_slot_word bring the previous slot into LDS, rows_to_lds adds
+// each thread's rows and solve_lds adds the groups and solves.  Identical in every workgroup.  Thread t owns terms 4q..4q+3 (q = t & 7) of rows
+// b = g, g + 32, ... (g = t >> 3); it adds its rows in increasing b; then thread k < 32 adds the
+// 32 group sums of term k in increasing g.  The rows were written by the previous kernel, so plain
+// loads see them.
+constexpr int kRowsPerThread = kIcpMaxWg / 32;
+
 template<bool Comp>
-__device__ void solve_previous(const IcpArgs& a, const IcpSlot* prev, const float* rows, IcpSlot& cur)
+struct RowFrag {
+    float4 sv[kRowsPerThread];
+    float4 cv[Comp ? kRowsPerThread : 1];
+};
+
+// Loads every row slot a thread may own (the buffers hold kIcpMaxWg rows, so all are in bounds)
+// without waiting; rows at or beyond nrows are masked when they are added.
+template<bool Comp>
+__device__ __forceinline__ void load_rows(const float* rows, RowFrag<Comp>& f)
 {
-    __shared__ double grp_d[32][33];
-    __shared__ float grp[32][33];
-    const int t = threadIdx.x;
-    if (t < 12)
-        cur.T[t] = prev->T[t];
-    if (t < 32) {
-        cur.sums[t] = prev->sums[t];
-        cur.sums_d[t] = prev->sums_d[t];
+    const int t = threadIdx.x, q = t & 7, g = t >> 3;
+#pragma unroll
+    for (int i = 0; i < kRowsPerThread; i++) {
+        const int b = g + 32 * i;
+        f.sv[i] = reinterpret_cast<const float4*>(rows + b * 64)[q];
+        if (Comp)
+            f.cv[i] = reinterpret_cast<const float4*>(rows + b * 64 + 32)[q];
     }
-    if (t < 4)
-        cur.done[t] = prev->done[t];
-    if (t == 0)
-        cur.iterations = prev->iterations;
-    const int nrows = prev->rows;
-    if (prev->did_pixels) {
-        constexpr int kRows = kIcpMaxWg / 32;
-        const int q = t & 7, g = t >> 3;
-        float4 sv[kRows], cv[kRows];
+}
+
+// The previous launch's slot is copied to LDS as whole 32-bit words, one per thread, so the copy
+// costs a single load per thread: slot_word issues it, put_slot_word stores it (the caller
+// synchronises before reading `cur`).
+constexpr int kSlotWords = (int) (sizeof(IcpSlot) / 4);
+static_assert(sizeof(IcpSlot) % 8 == 0 && kSlotWords <= kIcpThreads, "slot copy: one word per thread");
+
+__device__ __forceinline__ uint32_t slot_word(const IcpSlot* prev)
+{
+    const int t = threadIdx.x;
+    return t < kSlotWords ? reinterpret_cast<const uint32_t*>(prev)[t] : 0u;
+}
+
+__device__ __forceinline__ void put_slot_word(IcpSlot& cur, uint32_t w)
+{
+    const int t = threadIdx.x;
+    if (t < kSlotWords)
+        reinterpret_cast<uint32_t*>(&cur)[t] = w;
+}
+
+// Per-group partial sums of the previous launch's rows (LDS), between rows_to_lds and solve_lds.
+struct RowLds {
+    double d[32][33];
+    float f[32][33];
+};
+
+// Adds this thread's rows (those below a.prev_rows) and stores the group sums in LDS.  Waits only
+// for the row loads, so it runs before any later loads are issued.  The caller synchronises.
+template<bool Comp>
+__device__ __forceinline__ void rows_to_lds(const IcpArgs& a, const RowFrag<Comp>& f, RowLds& r)
+{
+    const int t = threadIdx.x, q = t & 7, g = t >> 3;
+    const int nrows = a.prev_rows;
+    if (Comp) {
+        double acc[4] = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
-        for (int i = 0; i < kRows; i++) {
-            const int b = g + 32 * i;
-            const bool in = b < nrows;
-            sv[i] = in ? reinterpret_cast<const float4*>(rows + b * 64)[q] : make_float4(0.f, 0.f, 0.f, 0.f);
-            if (Comp)
-                cv[i] = in ? reinterpret_cast<const float4*>(rows + b * 64 + 32)[q] : make_float4(0.f, 0.f, 0.f, 0.f);
+        for (int i = 0; i < kRowsPerThread; i++) {
+            const bool in = g + 32 * i < nrows;
+            acc[0] += in ? (double) f.sv[i].x + (double) f.cv[i].x : 0.0;
+            acc[1] += in ? (double) f.sv[i].y + (double) f.cv[i].y : 0.0;
+            acc[2] += in ? (double) f.sv[i].z + (double) f.cv[i].z : 0.0;
+            acc[3] += in ? (double) f.sv[i].w + (double) f.cv[i].w : 0.0;
         }
-        if (Comp) {
-            double acc[4] = {0.0, 0.0, 0.0, 0.0};
+        for (int k = 0; k < 4; k++)
+            r.d[g][4 * q + k] = acc[k];
+    }
+    else {
+        float acc[4] = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-            for (int i = 0; i < kRows; i++) {
-                acc[0] += (double) sv[i].x + (double) cv[i].x;
-                acc[1] += (double) sv[i].y + (double) cv[i].y;
-                acc[2] += (double) sv[i].z + (double) cv[i].z;
-                acc[3] += (double) sv[i].w + (double) cv[i].w;
-            }
-            for (int k = 0; k < 4; k++)
-                grp_d[g][4 * q + k] = acc[k];
+        for (int i = 0; i < kRowsPerThread; i++) {
+            const bool in = g + 32 * i < nrows;
+            acc[0] += in ? f.sv[i].x : 0.f;
+            acc[1] += in ? f.sv[i].y : 0.f;
+            acc[2] += in ? f.sv[i].z : 0.f;
+            acc[3] += in ? f.sv[i].w : 0.f;
         }
-        else {
-            float acc[4] = {0.f, 0.f, 0.f, 0.f};
-#pragma unroll
-            for (int i = 0; i < kRows; i++) {
-                acc[0] += sv[i].x;
-                acc[1] += sv[i].y;
-                acc[2] += sv[i].z;
-                acc[3] += sv[i].w;
-            }
-            for (int k = 0; k < 4; k++)
-                grp[g][4 * q + k] = acc[k];
-        }
-        __syncthreads();
+        for (int k = 0; k < 4; k++)
+            r.f[g][4 * q + k] = acc[k];
+    }
+}
+
+// If the previous launch ran its pixel pass (cur.did_pixels), add the group sums in a fixed order,
+// solve, apply T <- exp(x) T and raise that launch's level's convergence flag (icp.cpp:15-41,
+// 208-223).  LDS only; ends with a barrier.
+template<bool Comp>
+__device__ void solve_lds(const IcpArgs& a, const RowLds& r, IcpSlot& cur)
+{
+    const int t = threadIdx.x;
+    if (cur.did_pixels) {
         if (t < 32) {
             if (Comp) {
-                double acc = grp_d[0][t];
+                double acc = r.d[0][t];
                 for (int j = 1; j < 32; j++)
-                    acc += grp_d[j][t];
+                    acc += r.d[j][t];
                 cur.sums_d[t] = acc;
                 cur.sums[t] = (float) acc;
             }
             else {
-                float acc = grp[0][t];
+                float acc = r.f[0][t];
                 for (int j = 1; j < 32; j++)
-                    acc += grp[j][t];
+                    acc += r.f[j][t];
                 cur.sums[t] = acc;
                 cur.sums_d[t] = (double) acc;
             }
@@ -664,7 +715,7 @@ __device__ void solve_previous(const IcpArgs& a, const IcpSlot* prev, const floa
             store_pose(Tn, cur.T);
             const float xn = sqrtf(x[0] * x[0] + (x[1] * x[1] + (x[2] * x[2] + (x[3] * x[3] + (x[4] * x[4] + x[5] * x[5])))));
             if (xn < a.icp_th)
-                cur.done[prev->level] = 1;
+                cur.done[cur.level] = 1;
             cur.iterations += 1;
         }
     }
@@ -682,12 +733,23 @@ template<bool Comp, int PPT>
 __global__ __launch_bounds__(kIcpThreads) void k_icp_iter(IcpArgs a)
 {
     __shared__ IcpSlot cur;
+    __shared__ RowLds rl;
     __shared__ float red[kIcpThreads / 64][64];
     const IcpSlot* prev = &a.sh->slot[(a.k + 1) & 1];
     const int n = a.w * a.h;
+    // issue order = wait order: the slot word first, then the tile and the partial rows; the rows
+    // are added before any later load is issued, so that waiting for them waits for nothing else
+    const uint32_t sw = slot_word(prev);
     float4 v4[PPT], n4[PPT];
     load_tile<PPT>(a, blockIdx.x, n, v4, n4);
-    solve_previous<Comp>(a, prev, a.partials + ((a.k + 1) & 1) * kPartialStride, cur);
+    {
+        RowFrag<Comp> frag;
+        load_rows<Comp>(a.partials + ((a.k + 1) & 1) * kPartialStride, frag);
+        put_slot_word(cur, sw);
+        rows_to_lds<Comp>(a, frag, rl);
+    }
+    __syncthreads();
+    solve_lds<Comp>(a, rl, cur);
     const bool run = cur.done[a.level] == 0;
     if (run) {
         float s[32], c[32];
@@ -696,6 +758,7 @@ __global__ __launch_bounds__(kIcpThreads) void k_icp_iter(IcpArgs a)
             c[k] = 0.f;
         }
         const Iso T = pose_of(cur.T);
+        const Iso T_ref_inv = a.sh->T_ref_inv;
         for (int tile = blockIdx.x; tile < a.n_tiles; tile += gridDim.x) {
             if (tile != (int) blockIdx.x)
                 load_tile<PPT>(a, tile, n, v4, n4);
@@ -704,7 +767,7 @@ __global__ __launch_bounds__(kIcpThreads) void k_icp_iter(IcpArgs a)
             float rp[PPT][3], rn[PPT][3];
 #pragma unroll
             for (int j = 0; j < PPT; j++)
-                as[j] = icp_associate(a, T, v4[j], n4[j]);
+                as[j] = icp_associate(a, T_ref_inv, T, v4[j], n4[j]);
 #pragma unroll
             for (int j = 0; j < PPT; j++)
                 gather_ref(a, as[j].ri, rp[j], rn[j]);
@@ -778,7 +841,16 @@ template<bool Comp>
 __global__ __launch_bounds__(kIcpThreads) void k_icp_finish(IcpArgs a)
 {
     __shared__ IcpSlot cur;
-    solve_previous<Comp>(a, &a.sh->slot[(a.k + 1) & 1], a.partials + ((a.k + 1) & 1) * kPartialStride, cur);
+    __shared__ RowLds rl;
+    const uint32_t sw = slot_word(&a.sh->slot[(a.k + 1) & 1]);
+    {
+        RowFrag<Comp> frag;
+        load_rows<Comp>(a.partials + ((a.k + 1) & 1) * kPartialStride, frag);
+        put_slot_word(cur, sw);
+        rows_to_lds<Comp>(a, frag, rl);
+    }
+    __syncthreads();
+    solve_lds<Comp>(a, rl, cur);
     if (threadIdx.x == 0) {
         IcpState* r = &a.sh->result;
         for (int i = 0; i < 12; i++)
@@ -788,14 +860,22 @@ __global__ __launch_bounds__(kIcpThreads) void k_icp_finish(IcpArgs a)
             r->sums_d[k] = cur.sums_d[k];
         }
         r->iterations = cur.iterations;
-        check_pose(r, a.T_ref, a.cam.w, a.cam.h, a.track_th, Comp);
+        check_pose(r, a.sh->T_ref, a.cam.w, a.cam.h, a.track_th, Comp);
+        if (a.T_out_dev) {
+            for (int q = 0; q < 12; q++)
+                a.T_out_dev[q] = r->T[q];
+            a.T_out_dev[12] = a.T_out_dev[13] = a.T_out_dev[14] = 0.f;
+            a.T_out_dev[15] = 1.f;
+        }
+        if (a.ok_dev)
+            *a.ok_dev = r->ok;
     }
 }
 
 // Vertex and normal maps of the first `levels` pyramid levels in one launch; writes the state the
 // first Gauss-Newton launch reads (pose T) when init is not null.
-static void launch_pyramid(se8_tracker* tr, const float* depth, int levels, IcpSlot* init, const Iso& T,
-                           hipStream_t s)
+static void launch_pyramid(se8_tracker* tr, const float* depth, int levels, IcpShared* init, const Iso& T,
+                           const float* T16_dev, hipStream_t s)
 {
     const se8_pinhole& P = tr->sensor;
     PyrArgs pa;
@@ -816,6 +896,7 @@ static void launch_pyramid(se8_tracker* tr, const float* depth, int levels, IcpS
     pa.neg_y = make_cam(P, 1, false).left_hand_frame;
     pa.init = init;
     pa.T = T;
+    pa.T16_dev = T16_dev;
     k_pyramid<<<(total + kPyrThreads - 1) / kPyrThreads, kPyrThreads, 0, s>>>(pa);
 }
 
@@ -835,24 +916,24 @@ static void launch_icp_iter(int ppt, int grid, const IcpArgs& a, hipStream_t s)
     }
 }
 
-static int run_track(se8_tracker* tr, const float* depth, float T16[16], const float* ref_pts, const float* ref_nrm,
-                     int* ok, float* sums32, hipStream_t s)
+// Enqueues Tracker::track (tracker_impl.hpp:17-116): the pyramid, every Gauss-Newton launch and the
+// final check.  The start pose is T16_host, or 16 floats at T_in_dev; the result lands in
+// IcpShared::result, and also at T_out_dev / ok_dev when they are set.  Nothing waits on the host.
+static int enqueue_track(se8_tracker* tr, const float* depth, const float* T16_host, const float* T_in_dev,
+                         float* T_out_dev, int* ok_dev, const float* ref_pts, const float* ref_nrm, hipStream_t s)
 {
     const se8_pinhole& P = tr->sensor;
-    const Cam cam = make_cam(P, 1, false);
-    const Iso T_ref = iso_from16(T16);
     IcpShared* sh = (IcpShared*) tr->state.ptr;
+    // the previous call's stage events, before ev[0] / ev[1] are recorded again
+    harvest_tracking(tr);
     SE8_HIP_CHECK(hipEventRecord(tr->ev[0], s));
-    // the first launch (k = 0) reads slot 1
-    launch_pyramid(tr, depth, 3, &sh->slot[1], T_ref, s);
+    launch_pyramid(tr, depth, 3, sh, T16_host ? iso_from16(T16_host) : Iso{}, T_in_dev, s);
     IcpArgs a{};
     a.ref_pts = ref_pts;
     a.ref_nrm = ref_nrm;
     a.partials = (float*) tr->partials.ptr;
     a.sh = sh;
-    a.T_ref_inv = iso_inv(T_ref);
-    a.T_ref = T_ref;
-    a.cam = cam;
+    a.cam = make_cam(P, 1, false);
     a.dist_th = tr->cfg.dist_threshold;
     a.normal_th = tr->cfg.normal_threshold;
     a.icp_th = tr->cfg.icp_threshold;
@@ -872,26 +953,34 @@ static int run_track(se8_tracker* tr, const float* depth, float T16[16], const f
                 launch_icp_iter<true>(tr->icp_ppt[level], nwg, a, s);
             else
                 launch_icp_iter<false>(tr->icp_ppt[level], nwg, a, s);
+            a.prev_rows = nwg;
         }
     }
+    a.T_out_dev = T_out_dev;
+    a.ok_dev = ok_dev;
     if (tr->cfg.compensated)
         k_icp_finish<true><<<1, kIcpThreads, 0, s>>>(a);
     else
         k_icp_finish<false><<<1, kIcpThreads, 0, s>>>(a);
     SE8_HIP_CHECK(hipGetLastError());
     SE8_HIP_CHECK(hipEventRecord(tr->ev[1], s));
+    tr->track_pending = true;
+    return SE8_OK;
+}
+
+// Tracker::track with a host pose: enqueue, then wait for the result and write it back.
+static int run_track(se8_tracker* tr, const float* depth, float T16[16], const float* ref_pts, const float* ref_nrm,
+                     int* ok, float* sums32, hipStream_t s)
+{
+    int rc = enqueue_track(tr, depth, T16, nullptr, nullptr, nullptr, ref_pts, ref_nrm, s);
+    if (rc != SE8_OK)
+        return rc;
+    IcpShared* sh = (IcpShared*) tr->state.ptr;
     IcpState& host = *(IcpState*) tr->host_state;
     SE8_HIP_CHECK(hipMemcpyAsync(&host, &sh->result, sizeof(IcpState), hipMemcpyDeviceToHost, s));
     SE8_HIP_CHECK(hipStreamSynchronize(s));
-    tr->track_pending = true;
     harvest_tracking(tr);
-    Iso T;
-    for (int i = 0; i < 3; i++) {
-        for (int j = 0; j < 3; j++)
-            T.R[i][j] = host.T[4 * i + j];
-        T.t[i] = host.T[4 * i + 3];
-    }
-    iso_to16(T, T16);
+    iso_to16(iso_from16(host.T), T16);
     if (ok)
         *ok = host.ok;
     if (sums32)
@@ -899,8 +988,8 @@ static int run_track(se8_tracker* tr, const float* depth, float T16[16], const f
     return SE8_OK;
 }
 
-int launch_raycast(const se8_map* m, const Cam& cam, const Iso& T_WS, float* points, float* normals, int8_t* scale,
-                   hipStream_t s, hipEvent_t ev_start, hipEvent_t ev_stop);
+int launch_raycast(const se8_map* m, const Cam& cam, const Iso& T_WS, const float* T16_dev, float* points,
+                   float* normals, int8_t* scale, hipStream_t s, hipEvent_t ev_start, hipEvent_t ev_stop);
 
 // ----------------------------------------------------------------------------------------------
 // Batched point-to-plane cost and its adjoint (differentiation growth pattern).
@@ -1044,6 +1133,15 @@ extern "C" int se8_track(se8_tracker* t, const float* depth_dev, float T_WS[16],
     return run_track(t, depth_dev, T_WS, ref_points_dev, ref_normals_dev, ok, sums32, (hipStream_t) stream);
 }
 
+extern "C" int se8_track_dev(se8_tracker* t, const float* depth_dev, const float* T_in_dev, float* T_out_dev,
+                             const float* ref_points_dev, const float* ref_normals_dev, int* ok_dev, se8_stream stream)
+{
+    SE8_REQUIRE(t && depth_dev && T_in_dev && T_out_dev && ref_points_dev && ref_normals_dev,
+                "se8_track_dev: null argument");
+    return enqueue_track(t, depth_dev, nullptr, T_in_dev, T_out_dev, ok_dev, ref_points_dev, ref_normals_dev,
+                         (hipStream_t) stream);
+}
+
 extern "C" int se8_track_raycast(se8_tracker* t, const se8_map* m, const float* depth_dev, float T_WS[16], int* ok,
                                  se8_stream stream)
 {
@@ -1055,8 +1153,8 @@ extern "C" int se8_track_raycast(se8_tracker* t, const se8_map* m, const float* 
         return rc;
     hipStream_t s = (hipStream_t) stream;
     const Cam cam = make_cam(t->sensor, 1, false);
-    rc = launch_raycast(m, cam, iso_from16(T_WS), (float*) t->ref_points.ptr, (float*) t->ref_normals.ptr, nullptr, s,
-                        nullptr, nullptr);
+    rc = launch_raycast(m, cam, iso_from16(T_WS), nullptr, (float*) t->ref_points.ptr, (float*) t->ref_normals.ptr,
+                        nullptr, s, nullptr, nullptr);
     if (rc != SE8_OK)
         return rc;
     return run_track(t, depth_dev, T_WS, (const float*) t->ref_points.ptr, (const float*) t->ref_normals.ptr, ok,
@@ -1084,7 +1182,7 @@ extern "C" int se8_icp_cost_grad_batched(se8_tracker* t, const float* depth_dev,
         for (int i = 0; i < 12; i++)
             p12[(size_t) 12 * k + i] = poses16[(size_t) 16 * k + i];
     SE8_HIP_CHECK(hipMemcpyAsync(t->adj_poses.ptr, p12.data(), sizeof(float) * p12.size(), hipMemcpyHostToDevice, s));
-    launch_pyramid(t, depth_dev, 1, nullptr, Iso{}, s);
+    launch_pyramid(t, depth_dev, 1, nullptr, Iso{}, nullptr, s);
     IcpArgs a{};
     a.vert = (const float4*) t->vert_lvl[0].ptr;
     a.norm = (const float4*) t->norm_lvl[0].ptr;

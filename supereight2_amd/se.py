@@ -36,6 +36,17 @@ def _dev_f32(t, shape=None):
     return C.c_void_p(t.data_ptr())
 
 
+def _dev_pose(T):
+    """Device pointer of a pose held in device memory (a float32 CUDA tensor of 16 values, row-major
+    4x4), or None for a host pose."""
+    torch = _torch()
+    if isinstance(T, torch.Tensor) and T.is_cuda:
+        if T.dtype != torch.float32 or T.numel() != 16 or not T.is_contiguous():
+            raise ValueError("a device pose is a contiguous float32 tensor of 16 values")
+        return C.c_void_p(T.data_ptr())
+    return None
+
+
 @dataclass
 class PinholeCamera:
     """se::PinholeCamera::Config (pinhole_camera.hpp:24-40 + sensor.hpp:30-45)."""
@@ -233,21 +244,37 @@ class MapIntegrator:
         self.map = map_
 
     def integrateDepth(self, timestamp, depth, sensor, T_WC, stream=None):
-        """depth: [H, W] float32 device tensor: z-depth for a PinholeCamera, range for an OusterLidar."""
+        """depth: [H, W] float32 device tensor: z-depth for a PinholeCamera, range for an OusterLidar.
+        T_WC: a host 4x4 pose, or a device pose (float32 CUDA tensor of 16 values) that the
+        integration reads on the GPU."""
         d = _dev_f32(depth, (sensor.height, sensor.width))
+        Td = _dev_pose(T_WC)
+        lib, st = self.map.lib, _stream_handle(stream)
         if isinstance(sensor, OusterLidar):
-            capi.check(self.map.lib.se8_integrate_range(self.map.h, int(timestamp), d, sensor.h, capi.pose16(T_WC),
-                                                        _stream_handle(stream)), "se8_integrate_range")
+            if Td is not None:
+                capi.check(lib.se8_integrate_range_dev(self.map.h, int(timestamp), d, sensor.h, Td, st),
+                           "se8_integrate_range_dev")
+            else:
+                capi.check(lib.se8_integrate_range(self.map.h, int(timestamp), d, sensor.h, capi.pose16(T_WC), st),
+                           "se8_integrate_range")
             return
-        capi.check(self.map.lib.se8_integrate_depth(self.map.h, int(timestamp), d, C.byref(sensor.c()),
-                                                    capi.pose16(T_WC), _stream_handle(stream)),
-                   "se8_integrate_depth")
+        if Td is not None:
+            capi.check(lib.se8_integrate_depth_dev(self.map.h, int(timestamp), d, C.byref(sensor.c()), Td, st),
+                       "se8_integrate_depth_dev")
+        else:
+            capi.check(lib.se8_integrate_depth(self.map.h, int(timestamp), d, C.byref(sensor.c()), capi.pose16(T_WC),
+                                               st), "se8_integrate_depth")
 
 
 def raycast_volume(map_: TSDFMap, sensor: PinholeCamera, T_WS, points_W, normals_W, scale=None, stream=None):
     """se::raycaster::raycast_volume (raycaster.hpp:56-64) into device tensors [H, W, 3]."""
     shape = (sensor.height, sensor.width, 3)
     sp = C.c_void_p(scale.data_ptr()) if scale is not None else None
+    Td = _dev_pose(T_WS)
+    if Td is not None:
+        capi.check(map_.lib.se8_raycast_dev(map_.h, C.byref(sensor.c()), Td, _dev_f32(points_W, shape),
+                                            _dev_f32(normals_W, shape), sp, _stream_handle(stream)), "se8_raycast_dev")
+        return
     capi.check(map_.lib.se8_raycast(map_.h, C.byref(sensor.c()), capi.pose16(T_WS), _dev_f32(points_W, shape),
                                     _dev_f32(normals_W, shape), sp, _stream_handle(stream)), "se8_raycast")
 
@@ -289,6 +316,18 @@ class Tracker:
                                           _dev_f32(normals_W, shape + (3,)), C.byref(ok), capi.fptr(sums),
                                           _stream_handle(stream)), "se8_track")
         return bool(ok.value), np.array(T[:], np.float32).reshape(4, 4), sums
+
+    def track_device(self, depth, T_in, T_out, points_W, normals_W, ok=None, stream=None):
+        """Tracker::track with device poses (float32 CUDA tensors of 16 values; T_in may be T_out):
+        enqueued on the stream, nothing is read back.  ok: optional int32 CUDA tensor of one value."""
+        shape = (self.sensor.height, self.sensor.width)
+        Ti, To = _dev_pose(T_in), _dev_pose(T_out)
+        if Ti is None or To is None:
+            raise ValueError("track_device takes device poses")
+        okp = C.c_void_p(ok.data_ptr()) if ok is not None else None
+        capi.check(self.lib.se8_track_dev(self.h, _dev_f32(depth, shape), Ti, To, _dev_f32(points_W, shape + (3,)),
+                                          _dev_f32(normals_W, shape + (3,)), okp, _stream_handle(stream)),
+                   "se8_track_dev")
 
 
 def stage_times(map_: TSDFMap | None = None, tracker: Tracker | None = None) -> dict:
