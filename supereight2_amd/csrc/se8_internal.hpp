@@ -578,7 +578,7 @@ int dev_alloc(DeviceBuffer& b, size_t bytes);
 void dev_free(DeviceBuffer& b);
 
 struct RadixSorter {
-    DeviceBuffer keys_alt, tile_hist, tile_flags;
+    DeviceBuffer keys_alt, tile_hist;
     size_t cap = 0;
     int reserve(size_t n);
     void release();

@@ -1,11 +1,12 @@
 // Map object, allocation pipeline and TSDF update of libse8.so.
 //
 // integrateDepth (include/se/integrator/impl/map_integrator_impl.hpp:52-83) becomes, per frame:
+//   k_frame_begin  per-frame counters cleared; T_WS, its inverse and T_CV derived from the pose
 //   k_frustum      FrustumIterator over the allocated blocks (raycast_carver_impl.hpp:17-32,
 //                  iterator.hpp:197-238) as a flat per-block test of the block and its ancestors
 //   k_alloc_rays   the RaycastCarver band march (raycast_carver_impl.hpp:74-111): Morton keys of
 //                  unallocated blocks appended to a candidate list
-//   radix sort     LSD sort of the candidate keys (8-bit digits over the significant key bits)
+//   radix sort     LSD sort of the candidate keys (11-bit digits over the significant key bits)
 //   k_build_level  one launch per octree scale: segment heads of the sorted keys create missing
 //                  nodes (allocator_impl.hpp:68-147, octree_impl.hpp:171-199); duplicate keys
 //                  fall in the same segment, so the unique step is the segment-head flag
@@ -215,13 +216,8 @@ __global__ void k_frustum(MapView m, const FramePoses* fp, Sensor sensor, int* u
         if (in)
             update_list[pos] = slot;
     }
-    // the frustum part of the list is complete when this kernel ends; record its size
-}
-
-__global__ void k_record_frustum(int* counters)
-{
-    if (threadIdx.x == 0)
-        counters[kCntFrustum] = counters[kCntUpdate];
+    // the frustum part of the list is complete when this kernel ends; the carver kernel records
+    // its size
 }
 
 // ----------------------------------------------------------------------------------------------
@@ -234,6 +230,10 @@ __global__ void k_alloc_rays(MapView m, Sensor sensor, const FramePoses* fp, con
     const Iso T_WS = fp->T_WS;
     const int x = blockIdx.x * blockDim.x + threadIdx.x;
     const int y = blockIdx.y;
+    // the frustum part of the update list is complete (the frustum kernel has ended): record its
+    // size; nothing in this kernel touches the update counter
+    if (x == 0 && y == 0)
+        m.counters[kCntFrustum] = m.counters[kCntUpdate];
     bool live = x < sensor.width();
     float pos[3] = {0.f, 0.f, 0.f}, step[3] = {0.f, 0.f, 0.f};
     if (live) {
@@ -284,11 +284,15 @@ __global__ void k_alloc_rays(MapView m, Sensor sensor, const FramePoses* fp, con
 }
 
 // ----------------------------------------------------------------------------------------------
-// LSD radix sort of 64-bit keys, 8-bit digits, stable (tile = 256 threads x 8 keys, striped)
+// LSD radix sort of 64-bit keys, 11-bit digits, stable (tile = 256 threads x 8 keys, striped).
+// 11-bit digits cover the 21 significant key bits of a 1024^3 map in two passes (three at 16384^3).
 // ----------------------------------------------------------------------------------------------
 constexpr int kRsThreads = 256;
 constexpr int kRsItems = 8;
 constexpr int kRsTile = kRsThreads * kRsItems;
+constexpr int kRsBits = 11;
+constexpr int kRsBins = 1 << kRsBits;
+constexpr int kRsBinsPerThread = kRsBins / kRsThreads;
 
 // The key count lives in device memory (n = min(*n_dev, cap)), so the sort is enqueued without a
 // host round trip; the grid covers `cap` keys and workgroups past the live tiles return at once.
@@ -304,24 +308,29 @@ __global__ __launch_bounds__(kRsThreads) void k_rs_hist(const uint64_t* keys, co
     const int n_tiles = (n + kRsTile - 1) / kRsTile;
     if ((int) blockIdx.x >= n_tiles)
         return;
-    __shared__ uint32_t h[256];
-    h[threadIdx.x] = 0;
+    __shared__ uint32_t h[kRsBins];
+    for (int j = 0; j < kRsBinsPerThread; j++)
+        h[j * kRsThreads + threadIdx.x] = 0;
     __syncthreads();
     const int base = blockIdx.x * kRsTile;
     for (int i = 0; i < kRsItems; i++) {
         const int idx = base + i * kRsThreads + threadIdx.x;
         if (idx < n)
-            atomicAdd(&h[(keys[idx] >> shift) & 255u], 1u);
+            atomicAdd(&h[(keys[idx] >> shift) & (kRsBins - 1)], 1u);
     }
     __syncthreads();
-    tile_hist[(size_t) threadIdx.x * n_tiles + blockIdx.x] = h[threadIdx.x];
+    for (int j = 0; j < kRsBinsPerThread; j++) {
+        const int d = j * kRsThreads + threadIdx.x;
+        tile_hist[(size_t) d * n_tiles + blockIdx.x] = h[d];
+    }
 }
 
-// In-place exclusive scan of the n_tiles * 256 tile histograms with one 1024-thread workgroup.
+// In-place exclusive scan of the n_tiles * kRsBins tile histograms (digit-major) with one
+// 1024-thread workgroup.
 __global__ __launch_bounds__(1024) void k_scan_single(uint32_t* data, const int* n_dev, int cap)
 {
     __shared__ uint32_t sums[1024];
-    const int n = ((live_count(n_dev, cap) + kRsTile - 1) / kRsTile) * 256;
+    const int n = ((live_count(n_dev, cap) + kRsTile - 1) / kRsTile) * kRsBins;
     const int t = threadIdx.x;
     const int chunk = (n + 1023) / 1024;
     const int start = t * chunk;
@@ -345,6 +354,8 @@ __global__ __launch_bounds__(1024) void k_scan_single(uint32_t* data, const int*
     }
 }
 
+// Scatter of one tile: per round of 256 keys, lanes with equal digits are ranked by ballots, the
+// per-wave digit counts go through LDS, and only the entries a round wrote are cleared again.
 __global__ __launch_bounds__(kRsThreads) void k_rs_scatter(const uint64_t* in, uint64_t* out, const int* n_dev, int cap,
                                                            int shift, const uint32_t* tile_off)
 {
@@ -352,29 +363,34 @@ __global__ __launch_bounds__(kRsThreads) void k_rs_scatter(const uint64_t* in, u
     const int n_tiles = (n + kRsTile - 1) / kRsTile;
     if ((int) blockIdx.x >= n_tiles)
         return;
-    __shared__ uint32_t base[256];
-    __shared__ uint32_t wave_cnt[kRsThreads / 64][256];
+    __shared__ uint32_t base[kRsBins];
+    __shared__ uint32_t wave_cnt[kRsThreads / 64][kRsBins];
     const int t = threadIdx.x;
     const int lane = t & 63, wave = t >> 6;
-    base[t] = tile_off[(size_t) t * n_tiles + blockIdx.x];
+    for (int j = 0; j < kRsBinsPerThread; j++) {
+        const int d = j * kRsThreads + t;
+        base[d] = tile_off[(size_t) d * n_tiles + blockIdx.x];
+        for (int w = 0; w < kRsThreads / 64; w++)
+            wave_cnt[w][d] = 0;
+    }
+    __syncthreads();
     for (int r = 0; r < kRsItems; r++) {
         const int idx = blockIdx.x * kRsTile + r * kRsThreads + t;
         const bool valid = idx < n;
         const uint64_t key = valid ? in[idx] : 0ull;
-        const uint32_t d = (uint32_t) (key >> shift) & 255u;
+        const uint32_t d = (uint32_t) (key >> shift) & (kRsBins - 1);
         uint64_t peers = __ballot(valid);
 #pragma unroll
-        for (int b = 0; b < 8; b++) {
+        for (int b = 0; b < kRsBits; b++) {
             const bool bit = (d >> b) & 1u;
             const uint64_t mb = __ballot(bit);
             peers &= bit ? mb : ~mb;
         }
-        for (int w = 0; w < kRsThreads / 64; w++)
-            wave_cnt[w][t] = 0;
-        __syncthreads();
         const int rank = __popcll(peers & ((1ull << lane) - 1ull));
-        if (valid && rank == 0)
-            wave_cnt[wave][d] = (uint32_t) __popcll(peers);
+        const uint32_t cnt = (uint32_t) __popcll(peers);
+        const bool leader = valid && rank == 0;
+        if (leader)
+            wave_cnt[wave][d] = cnt;
         __syncthreads();
         if (valid) {
             uint32_t pos = base[d] + (uint32_t) rank;
@@ -383,10 +399,10 @@ __global__ __launch_bounds__(kRsThreads) void k_rs_scatter(const uint64_t* in, u
             out[pos] = key;
         }
         __syncthreads();
-        uint32_t add = 0;
-        for (int w = 0; w < kRsThreads / 64; w++)
-            add += wave_cnt[w][t];
-        base[t] += add;
+        if (leader) {
+            atomicAdd(&base[d], cnt);
+            wave_cnt[wave][d] = 0;
+        }
         __syncthreads();
     }
 }
@@ -398,7 +414,7 @@ int RadixSorter::reserve(size_t n)
     const size_t tiles = (n + kRsTile - 1) / kRsTile;
     int rc = dev_alloc(keys_alt, n * sizeof(uint64_t));
     if (rc == SE8_OK)
-        rc = dev_alloc(tile_hist, tiles * 256 * sizeof(uint32_t));
+        rc = dev_alloc(tile_hist, tiles * kRsBins * sizeof(uint32_t));
     if (rc != SE8_OK)
         return rc;
     cap = n;
@@ -409,7 +425,6 @@ void RadixSorter::release()
 {
     dev_free(keys_alt);
     dev_free(tile_hist);
-    dev_free(tile_flags);
     cap = 0;
 }
 
@@ -421,7 +436,7 @@ static uint64_t* radix_sort(RadixSorter& rs, uint64_t* keys, const int* n_dev, i
     const int max_tiles = (cap + kRsTile - 1) / kRsTile;
     uint64_t* a = keys;
     uint64_t* b = (uint64_t*) rs.keys_alt.ptr;
-    for (int shift = lo_bit; shift < hi_bit; shift += 8) {
+    for (int shift = lo_bit; shift < hi_bit; shift += kRsBits) {
         k_rs_hist<<<max_tiles, kRsThreads, 0, s>>>(a, n_dev, cap, shift, (uint32_t*) rs.tile_hist.ptr);
         k_scan_single<<<1, 1024, 0, s>>>((uint32_t*) rs.tile_hist.ptr, n_dev, cap);
         k_rs_scatter<<<max_tiles, kRsThreads, 0, s>>>(a, b, n_dev, cap, shift, (const uint32_t*) rs.tile_hist.ptr);
@@ -935,7 +950,6 @@ static int integrate_impl(se8_map* m, int timestamp, const float* depth_dev, con
     k_frame_begin<<<1, 64, 0, s>>>(m->view, T_arg, T16_dev, fp);
     const int cus = device_count_sm();
     k_frustum<Sensor><<<cus * 4, 256, 0, s>>>(m->view, fp, sensor, (int*) m->update_list.ptr);
-    k_record_frustum<<<1, 64, 0, s>>>(m->view.counters);
     SE8_HIP_CHECK(hipEventRecord(m->ev[1], s));
     dim3 grid((sensor.width() + 63) / 64, sensor.height());
     k_alloc_rays<Sensor><<<grid, 64, 0, s>>>(m->view, sensor, fp, depth_dev, band, num_steps,

@@ -539,22 +539,61 @@ __device__ __forceinline__ void pair_add(float& s, float& cs, float b, float cb)
 
 // Sums the 32 terms s[k] over the 64 lanes of a wave by halving exchanges: at each step a lane
 // keeps half of its terms and receives its partner's copy of that half (16 + 8 + 4 + 2 + 1 + 1 = 32
-// shuffles in place of 6 x 32).  Afterwards s[0] of lanes 2t and 2t+1 holds the wave total of term t.
-// The order of the additions is fixed, so results do not depend on scheduling.
-// One halving step over lane distance O: the O/2 surviving terms are compile-time register
-// indices (a runtime step would index the register arrays dynamically).
+// exchanges in place of 6 x 32).  Afterwards s[0] of lanes 2t and 2t+1 holds the wave total of term
+// t.  Every exchange is a VALU lane permutation, no LDS traffic: the two cross-row steps use the
+// permlane swaps (partner = lane ^ 32, then lane ^ 16), the in-row steps DPP (row mirror: lane ^ 15,
+// half-row mirror: lane ^ 7, quad permutations: lane ^ 2, lane ^ 1).  Each step's partner differs
+// from the lane in exactly the bit that decides which half it keeps, and the masks 32, 16, 15, 7,
+// 2, 1 are independent, so every lane's total covers all 64 lanes once.  The order of the
+// additions is fixed, so results do not depend on scheduling.
+
+// Cross-row step (O = 32 or 16): for each pair (s[i], s[i + O/2]) one swap leaves, in every lane,
+// its own copy of the term it keeps and its partner's copy (in either order: the fp32 sum and the
+// TwoSum error term are symmetric).
 template<bool Comp, int O>
-__device__ __forceinline__ void halving_step(float s[32], float c[32], bool up)
+__device__ __forceinline__ void swap_step(float s[32], float c[32])
+{
+#pragma unroll
+    for (int i = 0; i < O / 2; i++) {
+        const auto r = O == 32 ? __builtin_amdgcn_permlane32_swap(__float_as_uint(s[i]), __float_as_uint(s[i + O / 2]),
+                                                                 false, false)
+                               : __builtin_amdgcn_permlane16_swap(__float_as_uint(s[i]), __float_as_uint(s[i + O / 2]),
+                                                                  false, false);
+        float a = __uint_as_float(r[0]), ca = 0.f, cb = 0.f;
+        const float b = __uint_as_float(r[1]);
+        if (Comp) {
+            const auto rc = O == 32 ? __builtin_amdgcn_permlane32_swap(__float_as_uint(c[i]),
+                                                                      __float_as_uint(c[i + O / 2]), false, false)
+                                    : __builtin_amdgcn_permlane16_swap(__float_as_uint(c[i]),
+                                                                       __float_as_uint(c[i + O / 2]), false, false);
+            ca = __uint_as_float(rc[0]);
+            cb = __uint_as_float(rc[1]);
+        }
+        pair_add<Comp>(a, ca, b, cb);
+        s[i] = a;
+        c[i] = ca;
+    }
+}
+
+template<int CTRL>
+__device__ __forceinline__ float dpp_f(float v)
+{
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, 0xf, 0xf, false));
+}
+
+// In-row step over the DPP permutation CTRL (an involution pairing the lanes with `up` clear with
+// those with `up` set): the O/2 surviving terms are compile-time register indices.
+template<bool Comp, int O, int CTRL>
+__device__ __forceinline__ void dpp_step(float s[32], float c[32], bool up)
 {
 #pragma unroll
     for (int i = 0; i < O / 2; i++) {
         const float keep = up ? s[i + O / 2] : s[i];
-        const float send = up ? s[i] : s[i + O / 2];
-        const float recv = __shfl_xor(send, O);
+        const float recv = dpp_f<CTRL>(up ? s[i] : s[i + O / 2]);
         float keepc = 0.f, recvc = 0.f;
         if (Comp) {
             keepc = up ? c[i + O / 2] : c[i];
-            recvc = __shfl_xor(up ? c[i] : c[i + O / 2], O);
+            recvc = dpp_f<CTRL>(up ? c[i] : c[i + O / 2]);
         }
         s[i] = keep;
         c[i] = keepc;
@@ -562,17 +601,22 @@ __device__ __forceinline__ void halving_step(float s[32], float c[32], bool up)
     }
 }
 
+constexpr int kDppRowMirror = 0x140;     // lane ^ 15 within a row of 16
+constexpr int kDppRowHalfMirror = 0x141; // lane ^ 7 within a half-row of 8
+constexpr int kDppQuad2301 = 0x4e;       // lane ^ 2
+constexpr int kDppQuad1032 = 0xb1;       // lane ^ 1
+
 template<bool Comp>
 __device__ __forceinline__ void wave_reduce32(float s[32], float c[32])
 {
     const int lane = threadIdx.x & 63;
-    halving_step<Comp, 32>(s, c, (lane & 32) != 0);
-    halving_step<Comp, 16>(s, c, (lane & 16) != 0);
-    halving_step<Comp, 8>(s, c, (lane & 8) != 0);
-    halving_step<Comp, 4>(s, c, (lane & 4) != 0);
-    halving_step<Comp, 2>(s, c, (lane & 2) != 0);
-    const float recv = __shfl_xor(s[0], 1);
-    const float recvc = Comp ? __shfl_xor(c[0], 1) : 0.f;
+    swap_step<Comp, 32>(s, c);
+    swap_step<Comp, 16>(s, c);
+    dpp_step<Comp, 8, kDppRowMirror>(s, c, (lane & 8) != 0);
+    dpp_step<Comp, 4, kDppRowHalfMirror>(s, c, (lane & 4) != 0);
+    dpp_step<Comp, 2, kDppQuad2301>(s, c, (lane & 2) != 0);
+    const float recv = dpp_f<kDppQuad1032>(s[0]);
+    const float recvc = Comp ? dpp_f<kDppQuad1032>(c[0]) : 0.f;
     pair_add<Comp>(s[0], c[0], recv, recvc);
 }
 
