@@ -256,30 +256,75 @@ __global__ void k_alloc_rays(MapView m, Sensor sensor, const FramePoses* fp, con
             }
         }
     }
+    // The band samples are taken in batches: every sample's voxel and both lookup loads of the
+    // batch are in flight together, then the keys of unallocated blocks are emitted in sample
+    // order (consecutive repeats along the ray dropped), with one atomic per wave for the batch.
+    constexpr int kBatch = 8;
     uint64_t prev = ~0ull;
-    for (int s = 0; s < num_steps; s++) {
-        bool emit = false;
-        uint64_t key = 0;
-        if (live) {
-            float pM[3];
-            for (int i = 0; i < 3; i++)
-                pM[i] = pos[i] + m.t_MW[i];
-            if (pM[0] >= 0.f && pM[0] < m.dim && pM[1] >= 0.f && pM[1] < m.dim && pM[2] >= 0.f && pM[2] < m.dim) {
-                const int wrap = m.size - 1; // see lookup_block
-                const int vx = ((int) (pM[0] / m.res)) & wrap, vy = ((int) (pM[1] / m.res)) & wrap,
-                          vz = ((int) (pM[2] / m.res)) & wrap;
-                if (lookup_block(m, vx, vy, vz) < 0) {
-                    key = block_key_of_voxel(vx, vy, vz);
-                    emit = key != prev;
-                    prev = key;
+    for (int s0 = 0; s0 < num_steps; s0 += kBatch) {
+        int vox[kBatch][3];
+        bool inb[kBatch];
+#pragma unroll
+        for (int j = 0; j < kBatch; j++) {
+            inb[j] = false;
+            vox[j][0] = vox[j][1] = vox[j][2] = 0;
+            if (live && s0 + j < num_steps) {
+                float pM[3];
+                for (int i = 0; i < 3; i++)
+                    pM[i] = pos[i] + m.t_MW[i];
+                if (pM[0] >= 0.f && pM[0] < m.dim && pM[1] >= 0.f && pM[1] < m.dim && pM[2] >= 0.f && pM[2] < m.dim) {
+                    const int wrap = m.size - 1; // see lookup_block
+                    vox[j][0] = ((int) (pM[0] / m.res)) & wrap;
+                    vox[j][1] = ((int) (pM[1] / m.res)) & wrap;
+                    vox[j][2] = ((int) (pM[2] / m.res)) & wrap;
+                    inb[j] = true;
                 }
+                for (int i = 0; i < 3; i++)
+                    pos[i] = pos[i] + step[i];
             }
-            for (int i = 0; i < 3; i++)
-                pos[i] = pos[i] + step[i];
         }
-        const int slot = wave_append_slot(emit, &m.counters[kCntCandidates]);
-        if (emit && slot < cand_cap)
-            cand[slot] = key;
+        int slot[kBatch];
+#pragma unroll
+        for (int j = 0; j < kBatch; j++)
+            slot[j] = lookup_block_nb(m, vox[j][0], vox[j][1], vox[j][2]);
+        uint64_t keys[kBatch];
+        bool emit[kBatch];
+        int cnt = 0;
+#pragma unroll
+        for (int j = 0; j < kBatch; j++) {
+            keys[j] = block_key_of_voxel(vox[j][0], vox[j][1], vox[j][2]);
+            emit[j] = false;
+            if (inb[j] && slot[j] < 0) {
+                emit[j] = keys[j] != prev;
+                prev = keys[j];
+            }
+            cnt += emit[j] ? 1 : 0;
+        }
+        // exclusive prefix of the per-lane counts (at most kBatch, 4 bits) over the wave
+        const uint64_t lt = (1ull << __lane_id()) - 1ull;
+        int before = 0, total = 0;
+#pragma unroll
+        for (int b = 0; b < 4; b++) {
+            const uint64_t mb = __ballot((cnt >> b) & 1);
+            before += __popcll(mb & lt) << b;
+            total += __popcll(mb) << b;
+        }
+        if (total == 0)
+            continue;
+        const int leader = __ffsll((unsigned long long) __ballot(cnt > 0)) - 1;
+        int base = 0;
+        if (__lane_id() == leader)
+            base = atomicAdd(&m.counters[kCntCandidates], total);
+        base = __shfl(base, leader);
+        int64_t at = (int64_t) base + before;
+#pragma unroll
+        for (int j = 0; j < kBatch; j++) {
+            if (emit[j]) {
+                if (at < cand_cap)
+                    cand[at] = keys[j];
+                at++;
+            }
+        }
     }
 }
 
@@ -335,8 +380,20 @@ __global__ __launch_bounds__(1024) void k_scan_single(uint32_t* data, const int*
     const int chunk = (n + 1023) / 1024;
     const int start = t * chunk;
     const int end = min(start + chunk, n);
+    // each thread walks its contiguous chunk 16 elements at a time, all 16 loads in flight
+    constexpr int kBatch = 16;
     uint32_t s = 0;
-    for (int i = start; i < end; i++)
+    int i = start;
+    for (; i + kBatch <= end; i += kBatch) {
+        uint32_t v[kBatch];
+#pragma unroll
+        for (int j = 0; j < kBatch; j++)
+            v[j] = data[i + j];
+#pragma unroll
+        for (int j = 0; j < kBatch; j++)
+            s += v[j];
+    }
+    for (; i < end; i++)
         s += data[i];
     sums[t] = s;
     __syncthreads();
@@ -347,7 +404,19 @@ __global__ __launch_bounds__(1024) void k_scan_single(uint32_t* data, const int*
         __syncthreads();
     }
     uint32_t run = sums[t] - s;
-    for (int i = start; i < end; i++) {
+    i = start;
+    for (; i + kBatch <= end; i += kBatch) {
+        uint32_t v[kBatch];
+#pragma unroll
+        for (int j = 0; j < kBatch; j++)
+            v[j] = data[i + j];
+#pragma unroll
+        for (int j = 0; j < kBatch; j++) {
+            data[i + j] = run;
+            run += v[j];
+        }
+    }
+    for (; i < end; i++) {
         const uint32_t v = data[i];
         data[i] = run;
         run += v;
