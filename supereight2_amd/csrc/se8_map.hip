@@ -635,65 +635,120 @@ __global__ void k_build_blocks(MapView m, const uint64_t* keys, const int* n_dev
 // ----------------------------------------------------------------------------------------------
 // update: one block (512 voxels) per workgroup of 8 wave64s
 // ----------------------------------------------------------------------------------------------
+// Raw voxel word of a storage type: float2 {tsdf, weight} (FP32) or packed half + uint16 (FP16).
+template<int Storage>
+struct VoxelRaw {
+    float2 v;
+    __device__ __forceinline__ void load(const MapView& m, size_t vi) { v = reinterpret_cast<const float2*>(m.voxels)[vi]; }
+    __device__ __forceinline__ void clear() { v = make_float2(1.f, 0.f); }
+    __device__ __forceinline__ void get(float& tsdf, float& weight) const
+    {
+        tsdf = v.x;
+        weight = v.y;
+    }
+    __device__ __forceinline__ static void store(const MapView& m, size_t vi, float tsdf, float weight)
+    {
+        reinterpret_cast<float2*>(m.voxels)[vi] = make_float2(tsdf, weight);
+    }
+};
+
+template<>
+struct VoxelRaw<SE8_STORAGE_F16> {
+    uint32_t v;
+    __device__ __forceinline__ void load(const MapView& m, size_t vi) { v = reinterpret_cast<const uint32_t*>(m.voxels)[vi]; }
+    __device__ __forceinline__ void clear() { v = pack_f16(1.f, 0.f); }
+    __device__ __forceinline__ void get(float& tsdf, float& weight) const { unpack_f16(v, tsdf, weight); }
+    __device__ __forceinline__ static void store(const MapView& m, size_t vi, float tsdf, float weight)
+    {
+        reinterpret_cast<uint32_t*>(m.voxels)[vi] = pack_f16(tsdf, weight);
+    }
+};
+
+constexpr int kUpdChunk = 512; // blocks of a workgroup whose list entries are staged in LDS at once
+
+// Per voxel as singleres_tsdf_updater_impl.hpp:44-128.  A workgroup owns the list entries
+// blockIdx.x, blockIdx.x + gridDim.x, ...: it stages up to kUpdChunk of them (entry, key, parent)
+// in LDS with one round of loads (stamping the block and parent timestamps at the same time), then
+// walks them with the voxels of the next block already in flight, so that a block costs one memory
+// round trip (the depth sample) instead of a chain of four.
 template<int Storage, class Sensor>
 __global__ __launch_bounds__(512) void k_update(MapView m, const int* update_list, const FramePoses* fp,
                                                 Sensor sensor, const float* depth, int ts)
 {
+    __shared__ int s_entry[kUpdChunk];
+    __shared__ uint64_t s_key[kUpdChunk];
     const Iso T_CV = fp->T_CV;
     const int n = m.counters[kCntUpdate];
     const int t = threadIdx.x;
     const int vx = t & 7, vy = (t >> 3) & 7, vz = t >> 6;
-    for (int e = blockIdx.x; e < n; e += gridDim.x) {
-        const int entry = update_list[e];
-        const bool fresh = entry < 0;
-        const int slot = entry & 0x7fffffff;
-        int c[3];
-        key_to_coord(m.block_key[slot], c);
-        if (t == 0) {
+    const int G = gridDim.x;
+    const int iters = n > (int) blockIdx.x ? (n - 1 - (int) blockIdx.x) / G + 1 : 0;
+    for (int c0 = 0; c0 < iters; c0 += kUpdChunk) {
+        const int cnt = min(kUpdChunk, iters - c0);
+        if (c0 > 0)
+            __syncthreads(); // the previous chunk's entries are no longer read
+        if (t < cnt) {
+            const int entry = update_list[blockIdx.x + (size_t) (c0 + t) * G];
+            const int slot = entry & 0x7fffffff;
+            s_entry[t] = entry;
+            s_key[t] = m.block_key[slot];
+            // block timestamp and one-level parent propagation (propagator_impl.hpp:146-197)
             m.block_ts[slot] = ts;
             atomicMax(&m.node_ts[m.block_parent[slot]], ts);
         }
-        const float v[3] = {(float) (c[0] + vx), (float) (c[1] + vy), (float) (c[2] + vz)};
-        float pC[3];
-        iso_apply(T_CV, v, pC);
-        float tsdf = 1.f, weight = 0.f;
-        const size_t vi = (size_t) slot * 512 + t;
-        if (!fresh) {
-            if (Storage == SE8_STORAGE_F32) {
-                const float2 d = reinterpret_cast<const float2*>(m.voxels)[vi];
-                tsdf = d.x;
-                weight = d.y;
-            }
-            else {
-                unpack_f16(reinterpret_cast<const uint32_t*>(m.voxels)[vi], tsdf, weight);
-            }
+        __syncthreads();
+        VoxelRaw<Storage> cur;
+        {
+            const int e0 = s_entry[0];
+            if (e0 < 0)
+                cur.clear();
+            else
+                cur.load(m, (size_t) e0 * 512 + t);
         }
-        bool changed = false;
-        const float nrm = sqrtf(sqnorm3(pC));
-        if (!(nrm > sensor.far_dist(pC))) {
-            float uv[2];
-            if (sensor.project(pC, uv)) {
-                const int px = (int) (uv[0] + 0.5f), py = (int) (uv[1] + 0.5f);
-                const float d = depth[px + (size_t) py * sensor.width()];
-                if (!(d < sensor.near_plane())) {
-                    // measurementFromPoint: z for the pinhole, |p| for the lidar
-                    const float mz = sensor.measurement(pC);
-                    const float sdf = (d - mz) / mz * sqrtf(sqnorm3(pC));
-                    if (!(sdf < -m.mu)) {
-                        const float u = fminf(sdf / m.mu, 1.f);
-                        if (weight < m.max_weight)
-                            weight = weight + 1.f;
-                        tsdf = (tsdf * (weight - 1.f) + u) / weight;
-                        changed = true;
+        for (int i = 0; i < cnt; i++) {
+            const int entry = s_entry[i];
+            const bool fresh = entry < 0;
+            const int slot = entry & 0x7fffffff;
+            // the next block's voxels, issued before this block's depth sample
+            VoxelRaw<Storage> nxt;
+            nxt.clear();
+            if (i + 1 < cnt) {
+                const int en = s_entry[i + 1];
+                if (en >= 0)
+                    nxt.load(m, (size_t) en * 512 + t);
+            }
+            int c[3];
+            key_to_coord(s_key[i], c);
+            const float v[3] = {(float) (c[0] + vx), (float) (c[1] + vy), (float) (c[2] + vz)};
+            float pC[3];
+            iso_apply(T_CV, v, pC);
+            float tsdf, weight;
+            cur.get(tsdf, weight);
+            const size_t vi = (size_t) slot * 512 + t;
+            bool changed = false;
+            const float nrm = sqrtf(sqnorm3(pC));
+            if (!(nrm > sensor.far_dist(pC))) {
+                float uv[2];
+                if (sensor.project(pC, uv)) {
+                    const int px = (int) (uv[0] + 0.5f), py = (int) (uv[1] + 0.5f);
+                    const float d = depth[px + (size_t) py * sensor.width()];
+                    if (!(d < sensor.near_plane())) {
+                        // measurementFromPoint: z for the pinhole, |p| for the lidar
+                        const float mz = sensor.measurement(pC);
+                        const float sdf = (d - mz) / mz * sqrtf(sqnorm3(pC));
+                        if (!(sdf < -m.mu)) {
+                            const float u = fminf(sdf / m.mu, 1.f);
+                            if (weight < m.max_weight)
+                                weight = weight + 1.f;
+                            tsdf = (tsdf * (weight - 1.f) + u) / weight;
+                            changed = true;
+                        }
                     }
                 }
             }
-        }
-        if (changed || fresh) {
-            if (Storage == SE8_STORAGE_F32)
-                reinterpret_cast<float2*>(m.voxels)[vi] = make_float2(tsdf, weight);
-            else
-                reinterpret_cast<uint32_t*>(m.voxels)[vi] = pack_f16(tsdf, weight);
+            if (changed || fresh)
+                VoxelRaw<Storage>::store(m, vi, tsdf, weight);
+            cur = nxt;
         }
     }
 }
