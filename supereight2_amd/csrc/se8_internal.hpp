@@ -354,6 +354,24 @@ struct PinholeSensor {
     SE8_HD void backproject(int x, int y, float dir[3]) const { cam_backproject(c, (float) x, (float) y, dir); }
     SE8_HD bool project(const float p[3], float uv[2]) const { return cam_project(c, p, uv); }
     SE8_HD float far_dist(const float p[3]) const { return cam_far_dist(c, p); }
+    // nrm > far_dist(p) with nrm = |p| (the updater's far test), without the two divisions in the
+    // common case: far_dist = far / (z / nrm) is bracketed by an estimate with a relative error
+    // below 1e-6 (a hardware reciprocal and three products), and only a point inside that bracket
+    // takes the exact path, so the result is that of the exact expression.
+    __device__ bool beyond_far(const float p[3], float nrm) const
+    {
+        if (p[2] < 0.f)
+            return true; // z / nrm < 0, so far_dist < 0 <= nrm
+        if (p[2] > 0.f && nrm > 0.f) {
+            const float est = c.far_plane * nrm * __builtin_amdgcn_rcpf(p[2]);
+            const float tol = est * 1e-5f;
+            if (nrm > est + tol)
+                return true;
+            if (nrm < est - tol)
+                return false;
+        }
+        return nrm > cam_far_dist(c, p);
+    }
     SE8_HD float measurement(const float p[3]) const { return p[2]; }
     SE8_HD bool sphere_in_frustum(const float ctr[3], float r) const { return cam_sphere_in_frustum(c, ctr, r); }
 };
@@ -373,6 +391,7 @@ struct LidarSensor {
     }
     SE8_HD bool project(const float p[3], float uv[2]) const { return lidar_project(L, p, uv); }
     SE8_HD float far_dist(const float*) const { return L.far_plane; }
+    __device__ bool beyond_far(const float*, float nrm) const { return nrm > L.far_plane; }
     SE8_HD float measurement(const float p[3]) const { return sqrtf(sqnorm3(p)); }
     SE8_HD bool sphere_in_frustum(const float ctr[3], float r) const { return lidar_sphere_in_frustum(L, ctr, r); }
 };

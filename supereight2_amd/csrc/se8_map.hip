@@ -665,12 +665,32 @@ struct VoxelRaw<SE8_STORAGE_F16> {
 };
 
 constexpr int kUpdChunk = 512; // blocks of a workgroup whose list entries are staged in LDS at once
+constexpr int kUpdK = 2;       // blocks a workgroup updates together (their loads in flight together)
+
+// The part of the per-voxel update that follows the depth sample (singleres_tsdf_updater_impl.hpp:
+// 80-128); `d_ok` is false when the voxel failed the far or projection test.
+__device__ __forceinline__ bool tsdf_update_voxel(const MapView& m, float d, bool d_ok, float near_plane, float mz,
+                                                  float nrm, float& tsdf, float& weight)
+{
+    if (!d_ok || d < near_plane)
+        return false;
+    const float sdf = (d - mz) / mz * nrm;
+    if (sdf < -m.mu)
+        return false;
+    // sdf >= mu > 0 implies sdf / mu >= 1 (rounded division is monotone), so the division is only
+    // needed below mu
+    const float u = sdf >= m.mu ? 1.f : fminf(sdf / m.mu, 1.f);
+    if (weight < m.max_weight)
+        weight = weight + 1.f;
+    tsdf = (tsdf * (weight - 1.f) + u) / weight;
+    return true;
+}
 
 // Per voxel as singleres_tsdf_updater_impl.hpp:44-128.  A workgroup owns the list entries
-// blockIdx.x, blockIdx.x + gridDim.x, ...: it stages up to kUpdChunk of them (entry, key, parent)
-// in LDS with one round of loads (stamping the block and parent timestamps at the same time), then
-// walks them with the voxels of the next block already in flight, so that a block costs one memory
-// round trip (the depth sample) instead of a chain of four.
+// blockIdx.x, blockIdx.x + gridDim.x, ...: it stages up to kUpdChunk of them (entry, key) in LDS
+// with one round of loads (stamping the block and parent timestamps at the same time), then walks
+// them kUpdK blocks at a time: the voxels of the next group are loaded while the current group
+// projects, and the group's depth samples are all in flight together.
 template<int Storage, class Sensor>
 __global__ __launch_bounds__(512) void k_update(MapView m, const int* update_list, const FramePoses* fp,
                                                 Sensor sensor, const float* depth, int ts)
@@ -697,58 +717,61 @@ __global__ __launch_bounds__(512) void k_update(MapView m, const int* update_lis
             atomicMax(&m.node_ts[m.block_parent[slot]], ts);
         }
         __syncthreads();
-        VoxelRaw<Storage> cur;
-        {
-            const int e0 = s_entry[0];
-            if (e0 < 0)
-                cur.clear();
-            else
-                cur.load(m, (size_t) e0 * 512 + t);
+        VoxelRaw<Storage> cur[kUpdK];
+#pragma unroll
+        for (int k = 0; k < kUpdK; k++) {
+            cur[k].clear();
+            if (k < cnt && s_entry[k] >= 0)
+                cur[k].load(m, (size_t) s_entry[k] * 512 + t);
         }
-        for (int i = 0; i < cnt; i++) {
-            const int entry = s_entry[i];
-            const bool fresh = entry < 0;
-            const int slot = entry & 0x7fffffff;
-            // the next block's voxels, issued before this block's depth sample
-            VoxelRaw<Storage> nxt;
-            nxt.clear();
-            if (i + 1 < cnt) {
-                const int en = s_entry[i + 1];
-                if (en >= 0)
-                    nxt.load(m, (size_t) en * 512 + t);
+        for (int i = 0; i < cnt; i += kUpdK) {
+            // the next group's voxels, issued before this group's depth samples
+            VoxelRaw<Storage> nxt[kUpdK];
+#pragma unroll
+            for (int k = 0; k < kUpdK; k++) {
+                nxt[k].clear();
+                const int j = i + kUpdK + k;
+                if (j < cnt && s_entry[j] >= 0)
+                    nxt[k].load(m, (size_t) s_entry[j] * 512 + t);
             }
-            int c[3];
-            key_to_coord(s_key[i], c);
-            const float v[3] = {(float) (c[0] + vx), (float) (c[1] + vy), (float) (c[2] + vz)};
-            float pC[3];
-            iso_apply(T_CV, v, pC);
-            float tsdf, weight;
-            cur.get(tsdf, weight);
-            const size_t vi = (size_t) slot * 512 + t;
-            bool changed = false;
-            const float nrm = sqrtf(sqnorm3(pC));
-            if (!(nrm > sensor.far_dist(pC))) {
-                float uv[2];
-                if (sensor.project(pC, uv)) {
-                    const int px = (int) (uv[0] + 0.5f), py = (int) (uv[1] + 0.5f);
-                    const float d = depth[px + (size_t) py * sensor.width()];
-                    if (!(d < sensor.near_plane())) {
-                        // measurementFromPoint: z for the pinhole, |p| for the lidar
-                        const float mz = sensor.measurement(pC);
-                        const float sdf = (d - mz) / mz * sqrtf(sqnorm3(pC));
-                        if (!(sdf < -m.mu)) {
-                            const float u = fminf(sdf / m.mu, 1.f);
-                            if (weight < m.max_weight)
-                                weight = weight + 1.f;
-                            tsdf = (tsdf * (weight - 1.f) + u) / weight;
-                            changed = true;
-                        }
+            float pC[kUpdK][3], nrm[kUpdK], dval[kUpdK];
+            bool d_ok[kUpdK];
+#pragma unroll
+            for (int k = 0; k < kUpdK; k++) {
+                d_ok[k] = false;
+                dval[k] = 0.f;
+                nrm[k] = 0.f;
+                pC[k][0] = pC[k][1] = pC[k][2] = 0.f;
+                if (i + k < cnt) {
+                    int c[3];
+                    key_to_coord(s_key[i + k], c);
+                    const float v[3] = {(float) (c[0] + vx), (float) (c[1] + vy), (float) (c[2] + vz)};
+                    iso_apply(T_CV, v, pC[k]);
+                    nrm[k] = sqrtf(sqnorm3(pC[k]));
+                    float uv[2];
+                    if (!sensor.beyond_far(pC[k], nrm[k]) && sensor.project(pC[k], uv)) {
+                        const int px = (int) (uv[0] + 0.5f), py = (int) (uv[1] + 0.5f);
+                        dval[k] = depth[px + (size_t) py * sensor.width()];
+                        d_ok[k] = true;
                     }
                 }
             }
-            if (changed || fresh)
-                VoxelRaw<Storage>::store(m, vi, tsdf, weight);
-            cur = nxt;
+#pragma unroll
+            for (int k = 0; k < kUpdK; k++) {
+                if (i + k >= cnt)
+                    break;
+                const int entry = s_entry[i + k];
+                float tsdf, weight;
+                cur[k].get(tsdf, weight);
+                // measurementFromPoint: z for the pinhole, |p| for the lidar
+                const bool changed = tsdf_update_voxel(m, dval[k], d_ok[k], sensor.near_plane(),
+                                                       sensor.measurement(pC[k]), nrm[k], tsdf, weight);
+                if (changed || entry < 0)
+                    VoxelRaw<Storage>::store(m, (size_t) (entry & 0x7fffffff) * 512 + t, tsdf, weight);
+            }
+#pragma unroll
+            for (int k = 0; k < kUpdK; k++)
+                cur[k] = nxt[k];
         }
     }
 }
