@@ -691,9 +691,11 @@ __device__ __forceinline__ bool tsdf_update_voxel(const MapView& m, float d, boo
 // with one round of loads (stamping the block and parent timestamps at the same time), then walks
 // them kUpdK blocks at a time: the voxels of the next group are loaded while the current group
 // projects, and the group's depth samples are all in flight together.
+// amdgpu_waves_per_eu(8): the scalar registers must stay within the budget of 8 waves per SIMD
+// (96 allocated), or the one resident wave of workgroups shrinks from 4 to 3 per CU.
 template<int Storage, class Sensor>
-__global__ __launch_bounds__(512) void k_update(MapView m, const int* update_list, const FramePoses* fp,
-                                                Sensor sensor, const float* depth, int ts)
+__global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(8, 8))) void k_update(
+    MapView m, const int* update_list, const FramePoses* fp, Sensor sensor, const float* depth, int ts)
 {
     __shared__ int s_entry[kUpdChunk];
     __shared__ uint64_t s_key[kUpdChunk];
