@@ -767,8 +767,8 @@ __device__ void solve_lds(const IcpArgs& a, const RowLds& r, IcpSlot& cur)
 }
 
 // Launch k of the Gauss-Newton sequence of Tracker::track (tracker_impl.hpp:77-108): solve the
-// pass of launch k - 1 (see solve_previous), then, unless this level has converged, the pixel pass
-// of iteration k at this level: trackKernel (icp_impl.hpp:16-102) fused with the 32-term
+// pass of launch k - 1 (load_rows, slot_word, rows_to_lds, solve_lds), then, unless this level has
+// converged, the pixel pass of iteration k at this level: trackKernel (icp_impl.hpp:16-102) fused with the 32-term
 // reduceKernel sums (icp.cpp:45-204) over tiles of PPT pixels per thread, a halving wave
 // reduction and an LDS reduction to one partial row per workgroup.  Workgroup 0 publishes the
 // pose and flags for launch k + 1.  No workgroup waits on another, so no device-wide fence or
