@@ -27,7 +27,9 @@ extern "C" {
 #define SE8_ERR_INVALID -1  /* bad argument (null pointer, size mismatch, unsupported config) */
 #define SE8_ERR_OOM -2      /* device allocation failed */
 #define SE8_ERR_HIP -3      /* a HIP runtime call failed */
-#define SE8_ERR_CAPACITY -4 /* the block/node/brick pool capacity fixed at map creation is exhausted */
+#define SE8_ERR_CAPACITY -4 /* the block/node/brick pool capacity fixed at map creation is exhausted:
+                             * reported by the next integration, after which the map keeps the blocks
+                             * that fit, stays readable and refuses further integration */
 
 #define SE8_STORAGE_F32 0 /* float tsdf + float weight, 8 B/voxel (reference layout minus padding) */
 #define SE8_STORAGE_F16 1 /* half tsdf + uint16 weight, 4 B/voxel */

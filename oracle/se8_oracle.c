@@ -1701,7 +1701,8 @@ static int raycast_ray_bracket(const or_map* m, const real o[3], const real d[3]
         }
         if (f_tt < (real) 0)
             break;
-        stepsize = f_tt * mu > step ? f_tt * mu : step;
+        /* std::max(f_tt * mu, step): a NaN first argument is returned */
+        stepsize = f_tt * mu < step ? step : f_tt * mu;
         for (int i = 0; i < 3; i++)
             p[i] = p[i] + stepsize * d[i];
         f_t = f_tt;

@@ -618,6 +618,7 @@ struct se8_map {
     size_t candidates_cap = 0;
     int* host_counters = nullptr; // pinned copy of counters taken at the end of each integration
     bool counters_pending = false; // a copy into host_counters is in flight (event ev[7])
+    bool overflowed = false;       // a capacity overflow was reported: integration is refused from then on
     bool integ_pending = false;    // events ev[0..4] of the last integration not yet harvested
     bool ray_pending = false;      // events ev[5..6] of the last raycast not yet harvested
     se8_stage_times times{};

@@ -189,7 +189,9 @@ __device__ __forceinline__ int wave_append_slot(bool pred, int* counter)
 template<class Sensor>
 __global__ void k_frustum(MapView m, const FramePoses* fp, Sensor sensor, int* update_list)
 {
-    const int n = m.counters[kCntBlocks];
+    // the block counter runs past the pool when an allocation overflows; only slots below the
+    // capacity hold blocks
+    const int n = (int) min((int64_t) m.counters[kCntBlocks], m.max_blocks);
     const Iso T_SW = fp->T_SW;
     const float r_unit = sqrtf(3.0f) / 2.0f * m.res;
     for (int base = blockIdx.x * blockDim.x; base < n; base += gridDim.x * blockDim.x) {
@@ -677,9 +679,13 @@ __device__ __forceinline__ bool tsdf_update_voxel(const MapView& m, float d, boo
     const float sdf = (d - mz) / mz * nrm;
     if (sdf < -m.mu)
         return false;
-    // sdf >= mu > 0 implies sdf / mu >= 1 (rounded division is monotone), so the division is only
-    // needed below mu
-    const float u = sdf >= m.mu ? 1.f : fminf(sdf / m.mu, 1.f);
+    // std::min(sdf / mu, 1) (data_field_impl.hpp:45), which keeps a NaN quotient; sdf >= mu > 0
+    // implies sdf / mu >= 1 (rounded division is monotone), so the division is only needed below mu
+    float u = 1.f;
+    if (!(sdf >= m.mu)) {
+        const float q = sdf / m.mu;
+        u = 1.f < q ? 1.f : q;
+    }
     if (weight < m.max_weight)
         weight = weight + 1.f;
     tsdf = (tsdf * (weight - 1.f) + u) / weight;
@@ -1084,15 +1090,18 @@ static int integrate_impl(se8_map* m, int timestamp, const float* depth_dev, con
         return rc;
 
     // Capacity overflow of the previous frame is reported now: its counters were copied to pinned
-    // memory at the end of that call (the copy is normally complete by this point).
+    // memory at the end of that call (the copy is normally complete by this point).  The map keeps
+    // the blocks that fit, stays readable, and refuses further integration.
     if (m->counters_pending) {
         SE8_HIP_CHECK(hipEventSynchronize(m->ev[7]));
         m->counters_pending = false;
         harvest_integration(m);
-        if (m->host_counters[kCntOverflow]) {
-            set_error(std::string(what) + ": map capacity exhausted (raise se8_tsdf_config.max_blocks)");
-            return SE8_ERR_CAPACITY;
-        }
+        if (m->host_counters[kCntOverflow])
+            m->overflowed = true;
+    }
+    if (m->overflowed) {
+        set_error(std::string(what) + ": map capacity exhausted (raise se8_tsdf_config.max_blocks)");
+        return SE8_ERR_CAPACITY;
     }
     FramePoses* fp = (FramePoses*) m->frame_poses.ptr;
     SE8_HIP_CHECK(hipEventRecord(m->ev[0], s));
@@ -1380,6 +1389,7 @@ extern "C" int se8_allocate_blocks(se8_map* m, const int* voxel_coords, int64_t 
         if (rc != SE8_OK)
             return rc;
         if (host[kCntOverflow]) {
+            m->overflowed = true;
             set_error("se8_allocate_blocks: map capacity exhausted");
             return SE8_ERR_CAPACITY;
         }

@@ -415,7 +415,9 @@ __device__ bool march(const MapView& m, const float o[3], const float d[3], floa
         }
         if (f_tt < 0.f)
             break;
-        stepsize = fmaxf(f_tt * m.mu, step);
+        // std::max(f_tt * mu, step) (raycaster_impl.hpp:515): a NaN first argument is returned
+        const float fs = f_tt * m.mu;
+        stepsize = fs < step ? step : fs;
 #pragma unroll
         for (int i = 0; i < 3; i++)
             p[i] = p[i] + stepsize * d[i];

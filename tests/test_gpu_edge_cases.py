@@ -1,0 +1,123 @@
+"""Edge cases of the integration path on the GPU against the oracle.
+
+- invalid depth values (NaN, +-inf, negative, 0, below the near plane, beyond the far plane) mixed
+  into two tracking frames: allocation, timestamps, weights and octree bit-exact, TSDF values equal
+  (NaN where the reference's std::min(sdf / mu, 1) keeps a NaN), raycast equal;
+- an all-invalid frame (no block allocated, every ray a miss), then a valid frame;
+- a map whose block pool is too small: the overflow is reported as SE8_ERR_CAPACITY at the next
+  integration, the map keeps the blocks that fit, stays readable, and refuses further integration.
+"""
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from tests.helpers import CAM, spline_frames
+
+pytestmark = pytest.mark.gpu
+
+torch = pytest.importorskip("torch")
+
+
+def _sensor(se):
+    return se.PinholeCamera(CAM.width, CAM.height, CAM.fx, CAM.fy, CAM.cx, CAM.cy, CAM.near_plane, CAM.far_plane)
+
+
+def _pin():
+    return O.pinhole(CAM.width, CAM.height, CAM.fx, CAM.fy, CAM.cx, CAM.cy, CAM.near_plane, CAM.far_plane)
+
+
+def _corrupt(depth, seed):
+    d = depth.copy()
+    r = np.random.default_rng(seed).random(d.shape)
+    for i, v in enumerate([np.nan, np.inf, -np.inf, -1.0, 0.0, 0.05, 9.0]):
+        d[(r >= 0.01 * i) & (r < 0.01 * (i + 1))] = v
+    return d.astype(np.float32)
+
+
+def _compare_maps(gmap, omap):
+    gk, gts, gt, gw = gmap.dump_blocks()
+    ok, ots, ot, ow = omap.dump_blocks()
+    np.testing.assert_array_equal(gk, ok)
+    np.testing.assert_array_equal(gts, ots)
+    np.testing.assert_array_equal(gw, ow)
+    np.testing.assert_array_equal(np.isnan(gt), np.isnan(ot))
+    fin = ~np.isnan(gt)
+    np.testing.assert_array_equal(gt[fin].view(np.uint32), ot[fin].view(np.uint32))
+    for a, b in zip(gmap.dump_nodes(), omap.dump_nodes()):
+        np.testing.assert_array_equal(a, b)
+    return int(np.isnan(gt).sum())
+
+
+def test_invalid_depth_values_match_oracle():
+    from supereight2_amd import se
+
+    depth, poses = spline_frames(2)
+    sensor, pin = _sensor(se), _pin()
+    gmap, omap = se.TSDFMap(10.24, 0.01), O.OracleMap(10.24, 0.01)
+    integ = se.MapIntegrator(gmap)
+    for k in range(2):
+        d = _corrupt(depth[k], 100 + k)
+        print(f"frame {k}: {np.isnan(d).sum()} NaN, {np.isinf(d).sum()} inf pixels", flush=True)
+        integ.integrateDepth(k, torch.from_numpy(d).cuda(), sensor, poses[k])
+        omap.integrate(k, d, pin, poses[k])
+    torch.cuda.synchronize()
+    n_nan = _compare_maps(gmap, omap)
+    print("NaN voxels", n_nan)
+    assert n_nan > 0  # NaN depth samples reach the TSDF as in the reference
+    pts = torch.zeros((CAM.height, CAM.width, 3), dtype=torch.float32, device="cuda")
+    nrm = torch.zeros_like(pts)
+    se.raycast_volume(gmap, sensor, poses[1], pts, nrm)
+    op, on, _ = omap.raycast(pin, poses[1])
+    gp, gn = pts.cpu().numpy(), nrm.cpu().numpy()
+    np.testing.assert_array_equal(gp.reshape(op.shape), op)
+    np.testing.assert_array_equal(gn.reshape(on.shape), on)
+    gmap.close()
+
+
+def test_all_invalid_frame_then_valid_frame():
+    from supereight2_amd import se
+
+    depth, poses = spline_frames(2)
+    sensor, pin = _sensor(se), _pin()
+    gmap, omap = se.TSDFMap(10.24, 0.01), O.OracleMap(10.24, 0.01)
+    integ = se.MapIntegrator(gmap)
+    empty = np.zeros_like(depth[0])
+    integ.integrateDepth(0, torch.from_numpy(empty).cuda(), sensor, poses[0])
+    omap.integrate(0, empty, pin, poses[0])
+    torch.cuda.synchronize()
+    assert gmap.info().num_blocks == 0 and omap.num_blocks == 0
+    pts = torch.ones((CAM.height, CAM.width, 3), dtype=torch.float32, device="cuda")
+    nrm = torch.ones_like(pts)
+    se.raycast_volume(gmap, sensor, poses[0], pts, nrm)
+    assert float(pts.abs().sum()) == 0.0 and float(nrm.abs().sum()) == 0.0
+    integ.integrateDepth(1, torch.from_numpy(depth[1]).cuda(), sensor, poses[1])
+    omap.integrate(1, depth[1], pin, poses[1])
+    torch.cuda.synchronize()
+    assert _compare_maps(gmap, omap) == 0
+    gmap.close()
+
+
+def test_capacity_overflow_is_reported_and_map_stays_readable():
+    from supereight2_amd import capi, se
+
+    depth, poses = spline_frames(2)
+    sensor = _sensor(se)
+    cap = 512
+    gmap = se.TSDFMap(10.24, 0.01, max_blocks=cap)
+    integ = se.MapIntegrator(gmap)
+    integ.integrateDepth(0, torch.from_numpy(depth[0]).cuda(), sensor, poses[0])  # needs ~25k blocks
+    torch.cuda.synchronize()
+    for k in (1, 2):
+        with pytest.raises(capi.Se8Error, match="capacity"):
+            integ.integrateDepth(k, torch.from_numpy(depth[1]).cuda(), sensor, poses[1])
+    torch.cuda.synchronize()
+    info = gmap.info()
+    assert info.num_blocks == cap
+    keys, ts, tsdf, w = gmap.dump_blocks()
+    assert len(keys) == cap and (ts == 0).all()
+    pts = torch.zeros((CAM.height, CAM.width, 3), dtype=torch.float32, device="cuda")
+    nrm = torch.zeros_like(pts)
+    se.raycast_volume(gmap, sensor, poses[0], pts, nrm)
+    torch.cuda.synchronize()
+    assert np.isfinite(pts.cpu().numpy()).all()
+    gmap.close()

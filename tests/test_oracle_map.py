@@ -262,3 +262,33 @@ def test_first_block_matches_brute_force():
         assert t0 == pytest.approx(entry, abs=2e-5), (o, d)
         n_checked += 1
     assert n_checked > 100
+
+
+def test_nan_depth_sample_propagates_as_in_reference():
+    """A NaN depth sample passes the near-plane test (NaN < near is false, singleres_tsdf_updater_impl
+    .hpp:71) and std::min(sdf / mu, 1) keeps the NaN quotient (data_field_impl.hpp:45), so every
+    voxel that projects onto that pixel gets a NaN TSDF with its weight counted; an fmin-style
+    minimum (which drops the NaN) would leave those voxels at tsdf 1 instead.  The carver skips the
+    NaN ray (its points fall outside the map), so only blocks allocated by valid neighbours hold
+    such voxels."""
+    pin = O.pinhole()
+    depth = np.full((pin.height, pin.width), 2.0, np.float32)
+    depth[200:280, 280:360] = np.nan
+    m = O.OracleMap(10.24, 0.01)
+    m.integrate(0, depth, pin, np.eye(4))
+    keys, ts, tsdf, w = m.dump_blocks()
+    nan = np.isnan(tsdf)
+    assert nan.sum() > 1000
+    np.testing.assert_array_equal(w[nan], 1.0)
+    # the NaN voxels all lie in front of the wall, inside the NaN window's frustum
+    idx = np.arange(512)
+    off = np.stack([idx & 7, (idx >> 3) & 7, idx >> 6], -1)
+    for k, bn in zip(keys, nan):
+        if not bn.any():
+            continue
+        c, _ = O.decode_key(int(k))
+        p = (c[None, :] + off + 0.5) * 0.01 - 5.12
+        u = 525.0 * p[:, 0] / p[:, 2] + 319.5
+        v = 525.0 * p[:, 1] / p[:, 2] + 239.5
+        ui, vi = (u[bn] + 0.5).astype(int), (v[bn] + 0.5).astype(int)
+        assert ((ui >= 280) & (ui < 360) & (vi >= 200) & (vi < 280)).all()
