@@ -5,7 +5,9 @@
   (NaN where the reference's std::min(sdf / mu, 1) keeps a NaN), raycast equal;
 - an all-invalid frame (no block allocated, every ray a miss), then a valid frame;
 - a map whose block pool is too small: the overflow is reported as SE8_ERR_CAPACITY at the next
-  integration, the map keeps the blocks that fit, stays readable, and refuses further integration.
+  integration, the map keeps the blocks that fit, stays readable, and refuses further integration;
+- tracking against empty reference maps (the check fails and the pose is restored bit for bit);
+- an odd 322 x 243 camera through integration, raycast and tracking.
 """
 import numpy as np
 import pytest
@@ -120,4 +122,60 @@ def test_capacity_overflow_is_reported_and_map_stays_readable():
     se.raycast_volume(gmap, sensor, poses[0], pts, nrm)
     torch.cuda.synchronize()
     assert np.isfinite(pts.cpu().numpy()).all()
+    gmap.close()
+
+
+def test_track_failure_restores_pose():
+    """Against empty reference maps no pixel associates: the check fails (icp.cpp:227-246) and the
+    pose is restored bit for bit, as in the oracle."""
+    from supereight2_amd import se
+
+    depth, poses = spline_frames(2)
+    sensor, pin = _sensor(se), _pin()
+    gmap = se.TSDFMap(10.24, 0.01)
+    tracker = se.Tracker(gmap, sensor)
+    zeros = torch.zeros((CAM.height, CAM.width, 3), dtype=torch.float32, device="cuda")
+    T0 = poses[0].copy()
+    ok_g, T_g, sums_g = tracker.track(torch.from_numpy(depth[1]).cuda(), T0, zeros, zeros)
+    z = np.zeros((CAM.height, CAM.width, 3), np.float32)
+    ok_o, T_o, sums_o, _ = O.track(depth[1], pin, T0, z, z)
+    assert not ok_g and not ok_o
+    np.testing.assert_array_equal(T_g.view(np.uint32), T0.view(np.uint32))
+    assert sums_g[28] == sums_o[28] == 0
+    tracker.close()
+    gmap.close()
+
+
+def test_odd_image_size_pipeline_matches_oracle():
+    """A 322 x 243 camera (odd sizes at every pyramid level, partial raycast tiles): integration and
+    raycast bit-exact, tracking within the tolerance of the 640 x 480 test."""
+    from supereight2_amd import se, synth
+
+    cam = synth.Camera(width=322, height=243, fx=264.0, fy=264.0, cx=160.5, cy=121.0, near_plane=0.4, far_plane=6.0)
+    scene = synth.make_scene(0)
+    poses = np.stack(synth.spline_trajectory(300, seed=1)[:5]).astype(np.float32)
+    rng = np.random.default_rng(7)
+    depth = np.stack([synth.render_depth(scene, cam, T, rng) for T in poses])
+    sensor = se.PinholeCamera(cam.width, cam.height, cam.fx, cam.fy, cam.cx, cam.cy, cam.near_plane, cam.far_plane)
+    pin = O.pinhole(cam.width, cam.height, cam.fx, cam.fy, cam.cx, cam.cy, cam.near_plane, cam.far_plane)
+    gmap, omap = se.TSDFMap(10.24, 0.01), O.OracleMap(10.24, 0.01)
+    integ = se.MapIntegrator(gmap)
+    for k in range(4):
+        integ.integrateDepth(k, torch.from_numpy(depth[k]).cuda(), sensor, poses[k])
+        omap.integrate(k, depth[k], pin, poses[k])
+    torch.cuda.synchronize()
+    assert _compare_maps(gmap, omap) == 0
+    pts = torch.zeros((cam.height, cam.width, 3), dtype=torch.float32, device="cuda")
+    nrm = torch.zeros_like(pts)
+    se.raycast_volume(gmap, sensor, poses[3], pts, nrm)
+    op, on, _ = omap.raycast(pin, poses[3])
+    np.testing.assert_array_equal(pts.cpu().numpy(), op)
+    np.testing.assert_array_equal(nrm.cpu().numpy(), on)
+    tracker = se.Tracker(gmap, sensor)
+    ok_g, T_g, _ = tracker.track(torch.from_numpy(depth[4]).cuda(), poses[3], pts, nrm)
+    ok_o, T_o, _, _ = O.track(depth[4], pin, poses[3], op, on)
+    print("odd size track ok", ok_g, ok_o)
+    assert ok_g == ok_o
+    np.testing.assert_allclose(T_g, T_o, atol=1e-4)
+    tracker.close()
     gmap.close()
