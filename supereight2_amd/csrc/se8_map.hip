@@ -571,67 +571,104 @@ __global__ void k_build_level(MapView m, const uint64_t* keys, const int* n_dev,
         build_level_one(m, keys, i, cur_node, scale, brick_scale);
 }
 
-__device__ void build_block_one(const MapView& m, const uint64_t* keys, int i, const int* cur_node, int* update_list,
-                                int mark_new)
+// Minimum / maximum over the lanes of a wave (every lane active).
+__device__ __forceinline__ int wave_min_i32(int v)
 {
-    const uint64_t key = keys[i];
-    if (i > 0 && keys[i - 1] == key)
-        return;
-    const uint64_t code = key >> kScaleOffset;
-    int parent = 0;
-    if (m.max_scale > 4) {
-        const int prev = cur_node[i];
-        parent = prev < 0 ? -1 : m.node_child[prev * 8 + (int) ((code >> 12) & 7)];
-    }
-    if (parent < 0)
-        return; // only after a node-pool overflow
-    const int idx = (int) ((code >> 9) & 7);
-    if (m.node_child[parent * 8 + idx] >= 0)
-        return; // already allocated (explicit allocation requests may repeat blocks)
-    const int slot = atomicAdd(&m.counters[kCntBlocks], 1);
-    if (slot >= m.max_blocks) {
-        atomicExch(&m.counters[kCntOverflow], 1);
-        return;
-    }
-    int c[3];
-    key_to_coord(key, c);
-    m.block_key[slot] = key;
-    m.block_parent[slot] = parent;
-    m.block_ts[slot] = -1;
-    m.node_child[parent * 8 + idx] = slot;
-    const int bs = m.brick_shift + kBlockScale;
-    const int brick = m.brick_top[((c[2] >> bs) * m.bricks_side + (c[1] >> bs)) * m.bricks_side + (c[0] >> bs)];
-    const int msk = (1 << m.brick_shift) - 1;
-    const int bx = c[0] >> 3, by = c[1] >> 3, bz = c[2] >> 3;
-    if (brick >= 0)
-        m.brick_slots[brick * 512 + (bx & msk) + ((by & msk) << 3) + ((bz & msk) << 6)] = slot;
-    atomicMin(&m.counters[kCntAabb + 0], c[0]);
-    atomicMin(&m.counters[kCntAabb + 1], c[1]);
-    atomicMin(&m.counters[kCntAabb + 2], c[2]);
-    atomicMax(&m.counters[kCntAabb + 3], c[0] + 7);
-    atomicMax(&m.counters[kCntAabb + 4], c[1] + 7);
-    atomicMax(&m.counters[kCntAabb + 5], c[2] + 7);
-    if (update_list) {
-        const int pos = atomicAdd(&m.counters[kCntUpdate], 1);
-        update_list[pos] = mark_new ? (int) ((uint32_t) slot | 0x80000000u) : slot;
-    }
-    else {
-        // explicit allocation: initialise the voxels here (integration marks new blocks instead)
-        for (int v = 0; v < 512; v++) {
-            if (m.storage == SE8_STORAGE_F32)
-                reinterpret_cast<float2*>(m.voxels)[(size_t) slot * 512 + v] = make_float2(1.f, 0.f);
-            else
-                reinterpret_cast<uint32_t*>(m.voxels)[(size_t) slot * 512 + v] = pack_f16(1.f, 0.f);
-        }
-    }
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1)
+        v = min(v, __shfl_xor(v, o));
+    return v;
 }
 
+__device__ __forceinline__ int wave_max_i32(int v)
+{
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1)
+        v = max(v, __shfl_xor(v, o));
+    return v;
+}
+
+// Blocks of the sorted, deduplicated candidate keys that are not allocated yet: each gets a slot,
+// its parent's child entry, its brick entry, the AABB and (integration) an update-list entry marked
+// new, or (explicit allocation) its voxels initialised.  The slot, AABB and update-list atomics are
+// aggregated per wave, so the loop runs over whole waves (every lane reaches each ballot).
 __global__ void k_build_blocks(MapView m, const uint64_t* keys, const int* n_dev, int cap, const int* cur_node,
                                int* update_list, int mark_new)
 {
     const int n = min(*n_dev, cap);
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
-        build_block_one(m, keys, i, cur_node, update_list, mark_new);
+    const int stride = gridDim.x * blockDim.x;
+    for (int base = blockIdx.x * blockDim.x; base < n; base += stride) {
+        const int i = base + threadIdx.x;
+        bool want = false;
+        uint64_t key = 0;
+        int parent = -1, idx = 0;
+        if (i < n) {
+            key = keys[i];
+            if (i == 0 || keys[i - 1] != key) {
+                const uint64_t code = key >> kScaleOffset;
+                parent = 0;
+                if (m.max_scale > 4) {
+                    const int prev = cur_node[i];
+                    parent = prev < 0 ? -1 : m.node_child[prev * 8 + (int) ((code >> 12) & 7)];
+                }
+                idx = (int) ((code >> 9) & 7);
+                // a missing parent only follows a node-pool overflow; an existing child is an
+                // allocated block (explicit allocation requests may repeat blocks)
+                want = parent >= 0 && m.node_child[parent * 8 + idx] < 0;
+            }
+        }
+        const int slot = wave_append_slot(want, &m.counters[kCntBlocks]);
+        const bool made = want && slot < m.max_blocks;
+        if (want && !made)
+            atomicExch(&m.counters[kCntOverflow], 1);
+        int c[3] = {0, 0, 0};
+        if (made) {
+            key_to_coord(key, c);
+            m.block_key[slot] = key;
+            m.block_parent[slot] = parent;
+            m.block_ts[slot] = -1;
+            m.node_child[parent * 8 + idx] = slot;
+            const int bs = m.brick_shift + kBlockScale;
+            const int brick =
+                m.brick_top[((c[2] >> bs) * m.bricks_side + (c[1] >> bs)) * m.bricks_side + (c[0] >> bs)];
+            const int msk = (1 << m.brick_shift) - 1;
+            const int bx = c[0] >> 3, by = c[1] >> 3, bz = c[2] >> 3;
+            if (brick >= 0)
+                m.brick_slots[brick * 512 + (bx & msk) + ((by & msk) << 3) + ((bz & msk) << 6)] = slot;
+        }
+        // AABB of the allocated blocks (octree_impl.hpp:267-273): one atomic per wave and bound
+        if (__ballot(made)) {
+            const int lo[3] = {made ? c[0] : 0x7fffffff, made ? c[1] : 0x7fffffff, made ? c[2] : 0x7fffffff};
+            const int hi[3] = {made ? c[0] + 7 : -1, made ? c[1] + 7 : -1, made ? c[2] + 7 : -1};
+            int wlo[3], whi[3];
+#pragma unroll
+            for (int a = 0; a < 3; a++) {
+                wlo[a] = wave_min_i32(lo[a]);
+                whi[a] = wave_max_i32(hi[a]);
+            }
+            if (__lane_id() == 0) {
+#pragma unroll
+                for (int a = 0; a < 3; a++) {
+                    atomicMin(&m.counters[kCntAabb + a], wlo[a]);
+                    atomicMax(&m.counters[kCntAabb + 3 + a], whi[a]);
+                }
+            }
+        }
+        if (update_list) {
+            const int pos = wave_append_slot(made, &m.counters[kCntUpdate]);
+            if (made)
+                update_list[pos] = mark_new ? (int) ((uint32_t) slot | 0x80000000u) : slot;
+        }
+        else if (made) {
+            // explicit allocation: initialise the voxels here (integration marks new blocks instead)
+            for (int v = 0; v < 512; v++) {
+                if (m.storage == SE8_STORAGE_F32)
+                    reinterpret_cast<float2*>(m.voxels)[(size_t) slot * 512 + v] = make_float2(1.f, 0.f);
+                else
+                    reinterpret_cast<uint32_t*>(m.voxels)[(size_t) slot * 512 + v] = pack_f16(1.f, 0.f);
+            }
+        }
+    }
 }
 
 // ----------------------------------------------------------------------------------------------
