@@ -287,6 +287,12 @@ struct LidarModel {
     const float* elev;        // h beam elevations (radians, descending), device
     const float* az;          // h beam azimuth offsets (radians), device
     const float4* dir;        // h * w back-projection directions, row-major, device
+    // Search hint for the elevation table: bucket b of nbucket equal slices of [elev[h-1], elev[0]]
+    // (counted from the top) starts its search at row bucket[b].  Only the speed of the search
+    // depends on it: the row found is the unique one with elev[row] >= e > elev[row + 1].
+    const int* bucket;
+    int nbucket;
+    float bucket_scale; // nbucket / (elev[0] - elev[h - 1])
 };
 
 SE8_HD bool lidar_project(const LidarModel& L, const float p[3], float uv[2])
@@ -301,6 +307,18 @@ SE8_HD bool lidar_project(const LidarModel& L, const float p[3], float uv[2])
         vf = (L.elev[0] - e) / (L.elev[0] - L.elev[1]);
     else if (e <= L.elev[h - 1])
         vf = (float) (h - 1) + (L.elev[h - 1] - e) / (L.elev[h - 2] - L.elev[h - 1]);
+    else if (L.bucket) {
+        // elev[0] > e > elev[h - 1] (a NaN e fails both tests above and the range test below):
+        // start at the bucket's row and walk to the unique lo with elev[lo] >= e > elev[lo + 1]
+        const float u = (L.elev[0] - e) * L.bucket_scale;
+        const int b = u >= 0.f ? min((int) u, L.nbucket - 1) : 0;
+        int lo = L.bucket[b];
+        while (lo > 0 && L.elev[lo] < e)
+            lo--;
+        while (lo < h - 2 && L.elev[lo + 1] >= e)
+            lo++;
+        vf = (float) lo + (L.elev[lo] - e) / (L.elev[lo] - L.elev[lo + 1]);
+    }
     else {
         int lo = 0, hi = h - 1; // elev[lo] > e > elev[hi]
         while (hi - lo > 1) {
@@ -644,8 +662,8 @@ struct se8_tracker {
 
 struct se8_lidar {
     int width = 0, height = 0;
-    se8::LidarModel model{}; // device pointers into elev / az / dir
-    se8::DeviceBuffer elev, az, dir;
+    se8::LidarModel model{}; // device pointers into elev / az / dir / bucket
+    se8::DeviceBuffer elev, az, dir, bucket;
 };
 
 namespace se8 {

@@ -1247,13 +1247,26 @@ extern "C" int se8_lidar_create(const se8_lidar_config* cfg, se8_lidar** out)
         mx = std::max(mx, cfg->beam_elevation_deg[v]);
     }
     lidar_directions(*cfg, dir4);
+    // search hint: for bucket b (elevations below elev[0] - b * span / nb), the row of its top edge
+    const int nb = 4 * cfg->height;
+    const float span = elev[0] - elev[cfg->height - 1];
+    std::vector<int> bucket(nb);
+    for (int b = 0, lo = 0; b < nb; b++) {
+        const float top = elev[0] - span * ((float) b / (float) nb);
+        while (lo < cfg->height - 2 && elev[lo + 1] >= top)
+            lo++;
+        bucket[b] = lo;
+    }
     int rc = dev_alloc(l->elev, sizeof(float) * elev.size());
     rc = rc ? rc : dev_alloc(l->az, sizeof(float) * az.size());
     rc = rc ? rc : dev_alloc(l->dir, sizeof(float) * dir4.size());
+    rc = rc ? rc : dev_alloc(l->bucket, sizeof(int) * bucket.size());
     if (rc == SE8_OK && (hipMemcpy(l->elev.ptr, elev.data(), sizeof(float) * elev.size(), hipMemcpyHostToDevice)
                              != hipSuccess
                          || hipMemcpy(l->az.ptr, az.data(), sizeof(float) * az.size(), hipMemcpyHostToDevice) != hipSuccess
                          || hipMemcpy(l->dir.ptr, dir4.data(), sizeof(float) * dir4.size(), hipMemcpyHostToDevice)
+                             != hipSuccess
+                         || hipMemcpy(l->bucket.ptr, bucket.data(), sizeof(int) * bucket.size(), hipMemcpyHostToDevice)
                              != hipSuccess)) {
         set_error("se8_lidar_create: upload failed");
         rc = SE8_ERR_HIP;
@@ -1273,6 +1286,9 @@ extern "C" int se8_lidar_create(const se8_lidar_config* cfg, se8_lidar** out)
     L.elev = (const float*) l->elev.ptr;
     L.az = (const float*) l->az.ptr;
     L.dir = (const float4*) l->dir.ptr;
+    L.bucket = (const int*) l->bucket.ptr;
+    L.nbucket = nb;
+    L.bucket_scale = (float) nb / span;
     *out = l;
     return SE8_OK;
 }
@@ -1285,6 +1301,7 @@ extern "C" int se8_lidar_destroy(se8_lidar* l)
     dev_free(l->elev);
     dev_free(l->az);
     dev_free(l->dir);
+    dev_free(l->bucket);
     delete l;
     return SE8_OK;
 }
