@@ -66,8 +66,9 @@ typedef struct {
 } se8_pinhole;
 
 /* se::TrackerConfig (include/se/tracker/tracker.hpp:22-33).  compensated = 1 selects
- * Kahan-compensated fp32 per-workgroup partials, an FP64 final reduction and an FP64 LDLT
- * solve; 0 is the plain fp32 reduction + fp32 Cholesky like the reference. */
+ * compensated fp32 per-workgroup partials (TwoSum sum-and-error pairs across lanes and waves), an
+ * FP64 final reduction and an FP64 LDLT solve; 0 is the plain fp32 reduction + fp32 Cholesky like
+ * the reference. */
 typedef struct {
     int iterations[3];
     float dist_threshold;

@@ -15,8 +15,9 @@
 // one-workgroup launch solves the last pass and applies checkPoseKernel (icp.cpp:227-246).
 //
 // compensated = 0: fp32 sums, fixed-order fp32 cross-workgroup sum, fp32 Cholesky (reference).
-// compensated = 1: Kahan-compensated fp32 per-thread sums, error-free (TwoSum) pair reduction in
-// the workgroup, FP64 cross-workgroup sum and FP64 LDL^T solve.
+// compensated = 1: fp32 per-thread sums over a thread's few pixels, compensated (TwoSum sum and
+// error pairs) combination across lanes and waves into each workgroup's partial, FP64
+// cross-workgroup sum and FP64 LDL^T solve.
 #include <hip/hip_runtime.h>
 
 #include <cmath>
@@ -823,24 +824,12 @@ __global__ __launch_bounds__(kIcpThreads) void k_icp_iter(IcpArgs a)
                 const int res = icp_residual(a, T, as[j], rp[j], rn[j], e, J);
                 float v[32];
                 row_terms(res, e, J, v);
-                for (int k = 0; k < 32; k++) {
-                    if (Comp) {
-                        // Kahan summation
-                        const float yk = v[k] - c[k];
-                        const float tk = s[k] + yk;
-                        c[k] = (tk - s[k]) - yk;
-                        s[k] = tk;
-                    }
-                    else {
-                        s[k] += v[k];
-                    }
-                }
+                // a thread sums only the pixels of its tiles (PPT per tile; one tile per workgroup
+                // at 640x480), a few fp32 roundings; compensation (Comp) starts at the lane and wave
+                // combination below, where the sums grow
+                for (int k = 0; k < 32; k++)
+                    s[k] += v[k];
             }
-        }
-        if (Comp) {
-            // (s, -c) is the compensated pair; keep err = -c
-            for (int k = 0; k < 32; k++)
-                c[k] = -c[k];
         }
         wave_reduce32<Comp>(s, c);
         const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
