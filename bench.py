@@ -85,7 +85,40 @@ def ate(est, gt):
     return float(np.sqrt(np.mean(e**2))) if len(e) else 0.0
 
 
-def run_pipeline(depth_dev, gt, n_frames, warmup, storage, compensated, torch, se, sensor, device_pose=True):
+def mesh_bench(gmap, torch, se, cpu):
+    """Map::mesh of the final map: wall time per call into a preallocated face buffer (count pass,
+    scan, read-back of the total, emit pass), averaged over repeated calls; with cpu, the oracle's
+    mesher (C restatement, OpenMP) on the same voxels."""
+    ref = gmap.mesh()
+    buf = torch.empty((ref.shape[0] + 1024, 3, 3), dtype=torch.float32, device="cuda")
+    gmap.mesh(out=buf)
+    torch.cuda.synchronize()
+    reps = 20
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        gmap.mesh(out=buf)
+    torch.cuda.synchronize()
+    ms = 1000.0 * (time.perf_counter() - t0) / reps
+    out = {"faces": int(ref.shape[0]), "ms_per_mesh": ms}
+    if cpu:
+        from oracle import oracle as O
+
+        keys, _, tsdf, w = gmap.dump_blocks()
+        coords = np.array([O.decode_key(int(k))[0] for k in keys], np.int32).reshape(-1, 3)
+        omap = O.OracleMap(10.24, 0.01)
+        omap.load_blocks(coords, tsdf, w)
+        cores = os.cpu_count() or 1
+        O.set_threads(cores)
+        t0 = time.perf_counter()
+        f = omap.mesh()
+        out["cpu_ms_per_mesh"] = 1000.0 * (time.perf_counter() - t0)
+        out["cpu_threads"] = cores
+        out["cpu_faces_equal"] = int(len(f)) == out["faces"]
+    return out
+
+
+def run_pipeline(depth_dev, gt, n_frames, warmup, storage, compensated, torch, se, sensor, device_pose=True,
+                 mesh=None):
     """Runs the per-frame loop.  With device_pose (the default), each frame's pose stays in device
     memory: track writes pose k from pose k - 1 and integrate / raycast read it there, so the host
     only enqueues work and never waits inside the loop; the poses are read back after it.  Without
@@ -128,6 +161,7 @@ def run_pipeline(depth_dev, gt, n_frames, warmup, storage, compensated, torch, s
     st = se.stage_times(gmap, tracker)
     timed = n_frames - warmup
     info = gmap.info()
+    mesh_out = mesh_bench(gmap, torch, se, mesh == "cpu") if mesh else None
     tracker.close()
     gmap.close()
     ni, nr, nt = max(st["n_integrations"], 1), max(st["n_raycasts"], 1), max(st["n_tracks"], 1)
@@ -147,6 +181,7 @@ def run_pipeline(depth_dev, gt, n_frames, warmup, storage, compensated, torch, s
         "counts": {"integrations": st["n_integrations"], "raycasts": st["n_raycasts"], "tracks": st["n_tracks"]},
         "update_blocks_per_frame": st["sum_update_blocks"] / ni,
         "candidates_per_frame": st["sum_candidates"] / ni,
+        "mesh": mesh_out,
     }
 
 
@@ -329,7 +364,8 @@ def main():
     torch.cuda.synchronize()
 
     print("fp32 pipeline", file=sys.stderr, flush=True)
-    r32 = run_pipeline(depth_dev, gt, n_frames, args.warmup, "f32", False, torch, se, sensor)
+    r32 = run_pipeline(depth_dev, gt, n_frames, args.warmup, "f32", False, torch, se, sensor,
+                       mesh="gpu" if args.no_cpu_baseline else "cpu")
     ate32 = ate(r32["poses"], gt)
     print("fp32 pipeline, host poses", file=sys.stderr, flush=True)
     r32h = run_pipeline(depth_dev, gt, n_frames, args.warmup, "f32", False, torch, se, sensor, device_pose=False)
@@ -401,6 +437,8 @@ def main():
         "roofline": {"bound": "hbm", "kernel": "k_update", "achieved": upd_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": upd_gbs / HBM_PEAK_GBS, "traffic": traffic, "bytes_per_launch": upd_bytes},
         "cpu_baseline": cpu,
+        "mesh": dict(r32["mesh"], blocks=r32["blocks"],
+                     note="Map::mesh (single-res marching cubes) of the final FP32 map, after the timed loop"),
     }
     print(json.dumps(out))
 

@@ -27,9 +27,11 @@ extern "C" {
 #define SE8_ERR_INVALID -1  /* bad argument (null pointer, size mismatch, unsupported config) */
 #define SE8_ERR_OOM -2      /* device allocation failed */
 #define SE8_ERR_HIP -3      /* a HIP runtime call failed */
-#define SE8_ERR_CAPACITY -4 /* the block/node/brick pool capacity fixed at map creation is exhausted:
-                             * reported by the next integration, after which the map keeps the blocks
-                             * that fit, stays readable and refuses further integration */
+#define SE8_ERR_CAPACITY -4 /* a fixed capacity is exhausted.  The block/node/brick pool (fixed at map
+                             * creation): reported by the next integration, after which the map keeps
+                             * the blocks that fit, stays readable and refuses further integration.
+                             * se8_mesh: the caller's face buffer is too small (nothing written). */
+#define SE8_ERR_IO -5       /* a file could not be written */
 
 #define SE8_STORAGE_F32 0 /* float tsdf + float weight, 8 B/voxel (reference layout minus padding) */
 #define SE8_STORAGE_F16 1 /* half tsdf + uint16 weight, 4 B/voxel */
@@ -214,6 +216,26 @@ int se8_dump_nodes(const se8_map* map, uint64_t* keys, uint8_t* child_masks, int
 int se8_get_data(const se8_map* map, const float* points_W, int64_t n, float* tsdf, float* weight);
 int se8_field_interp(const se8_map* map, const float* points_W, int64_t n, float* out, uint8_t* valid);
 int se8_field_grad(const se8_map* map, const float* points_W, int64_t n, float* out3, uint8_t* valid);
+
+/* -------- surface mesh: se::Map::mesh / saveMesh (include/se/map/map.hpp:396-413,
+ * impl/map_impl.hpp:452-469) through the single-resolution marching cubes
+ * (include/se/map/algorithms/impl/marching_cube_impl.hpp:832-889, 1175-1203).
+ * se8_mesh writes each face as 9 floats (its three vertices x,y,z in frame O, metres; T_OW NULL =
+ * identity) into faces_dev (device memory for max_faces faces) and the number of faces to
+ * *num_faces.  max_faces = 0 only counts.  A mesh larger than max_faces writes nothing and returns
+ * SE8_ERR_CAPACITY with *num_faces set.  The call waits for the stream (the count is read back);
+ * the faces are then written asynchronously on it.  Face order: block slot, then cell, then the
+ * case's triangle order (fixed for a given map; the reference's order follows its OpenMP
+ * schedule).  The reference's min_desired_scale is unused for single-resolution maps. */
+int se8_mesh(const se8_map* map, const float T_OW[16], float* faces_dev, int64_t max_faces, int64_t* num_faces,
+             se8_stream stream);
+/* se::io::save_mesh (include/se/map/io/mesh_io.hpp:44-46, impl/mesh_io_impl.hpp:14-230): writes n
+ * faces (host memory, 9 floats each) as a .ply, .vtk or .obj file chosen by the extension, every
+ * vertex mapped by T_OM (NULL = identity).  SE8_ERR_INVALID for another extension (the reference
+ * throws std::invalid_argument), SE8_ERR_IO if the file cannot be written. */
+int se8_save_mesh(const float* faces, int64_t n, const char* filename, const float T_OM[16]);
+/* se::io::has_supported_mesh_extension (include/se/map/io/mesh_io.hpp:24-29): 1 for .ply/.vtk/.obj */
+int se8_has_supported_mesh_extension(const char* filename);
 
 int se8_stage_times_get(const se8_map* map, const se8_tracker* tracker, se8_stage_times* out);
 int se8_stage_times_reset(se8_map* map, se8_tracker* tracker);

@@ -113,6 +113,8 @@ def lib(precision: str = "f32") -> C.CDLL:
     L.or_allocate_blocks.argtypes = [P, I, C.c_long, C.c_int, I]
     L.or_set_block_data.restype = C.c_int
     L.or_set_block_data.argtypes = [P, I, R, R]
+    L.or_mesh.restype = C.c_long
+    L.or_mesh.argtypes = [P, R, R, C.c_long]
     L.or_get_field_interp.restype = C.c_int
     L.or_get_field_interp.argtypes = [P, R, R]
     L.or_get_field_grad.restype = C.c_int
@@ -333,6 +335,25 @@ class OracleMap:
         t = np.ascontiguousarray(tsdf, dtype=self.dtype)
         w = np.ascontiguousarray(weight, dtype=self.dtype)
         return self.L.or_set_block_data(self.h, _ip(bc), _rp(t, self.precision), _rp(w, self.precision))
+
+    def load_blocks(self, block_coords, tsdf, weight):
+        """Allocates the given blocks and sets their 512 voxels each (e.g. a dump of another map)."""
+        bc = np.ascontiguousarray(block_coords, dtype=np.int32).reshape(-1, 3)
+        if len(bc):
+            self.allocate_blocks(bc)
+        for i in range(len(bc)):
+            assert self.set_block_data(bc[i], tsdf[i], weight[i]) == 0
+
+    def mesh(self, T_OW=None):
+        """Map::mesh (map_impl.hpp:452-458): (F, 3, 3) triangle vertices in frame O, in the
+        reference's loop order over this map's blocks."""
+        T = np.eye(4, dtype=self.dtype) if T_OW is None else np.ascontiguousarray(T_OW, dtype=self.dtype)
+        R = C.POINTER(C.c_double if self.precision == "f64" else C.c_float)
+        n = self.L.or_mesh(self.h, _rp(T, self.precision), R(), 0)
+        faces = np.zeros((max(n, 1), 3, 3), self.dtype)
+        n2 = self.L.or_mesh(self.h, _rp(T, self.precision), _rp(faces, self.precision), n)
+        assert n2 == n
+        return faces[:n]
 
     def field_interp(self, p):
         pp = np.ascontiguousarray(p, dtype=self.dtype)

@@ -2334,3 +2334,140 @@ OR_API void or_raycast_depth_grad(void* vm, const or_pinhole* pin, const real T_
         }
     }
 }
+
+/* ------------------------------------------------------------------------- */
+/* Single-res marching cubes: marching_cube_impl.hpp:83-95 (compute_intersection), */
+/* 97-127 (edge_to_corners), 143-158 (gather_data), 20-38 (isosurface::occupied),  */
+/* 814-818 (checkVertex), 832-889 (marching_cube_kernel); the vertices mapped by   */
+/* T_OV = T_OW * T_WM * Scaling(res) (map_impl.hpp:452-458, octree_impl.hpp:121-133). */
+/* The case table is the one the library meshes with; it is pinned by its own     */
+/* properties in the tests.                                                      */
+/* ------------------------------------------------------------------------- */
+#include "../supereight2_amd/csrc/se8_mc_table.h"
+
+/* getData: the voxel of an allocated block, else the default {1, 0} (visitor_impl.hpp:67-81) */
+static inline void mc_get_data(const or_map* m, const int v[3], real* tsdf, real* weight)
+{
+    const or_octant* b = fetch_block(m, v);
+    if (!b) {
+        *tsdf = 1;
+        *weight = 0;
+        return;
+    }
+    const int i = block_voxel_idx(b, v);
+    *tsdf = b->tsdf[i];
+    *weight = b->weight[i];
+}
+
+static inline void mc_corner(int x, int y, int z, int k, int c[3])
+{
+    c[0] = x + (int) ((SE8_MC_CORNER_DX >> k) & 1);
+    c[1] = y + (int) ((SE8_MC_CORNER_DY >> k) & 1);
+    c[2] = z + (int) ((SE8_MC_CORNER_DZ >> k) & 1);
+}
+
+/* Triangles of the cell (x,y,z) into tri[] (3 vertices each, map voxel units); returns the count. */
+static int mc_cell(const or_map* m, int x, int y, int z, real tri[5][3][3])
+{
+    unsigned idx = 0;
+    for (int k = 0; k < 8; k++) {
+        int c[3];
+        real f, w;
+        mc_corner(x, y, z, k, c);
+        mc_get_data(m, c, &f, &w);
+        if (w == 0)
+            return 0; /* an unknown corner: no face (isosurface::occupied) */
+        if (f < 0)
+            idx |= 1u << k;
+    }
+    const uint64_t word = se8_mc_tri[idx];
+    const int ntri = (int) (word >> 60);
+    int n = 0;
+    for (int j = 0; j < ntri; j++) {
+        int skip = 0;
+        for (int v = 0; v < 3 && !skip; v++) {
+            const int e = (int) ((word >> (4 * (3 * j + v))) & 0xF);
+            const int a = (int) ((SE8_MC_EDGE_A >> (4 * e)) & 0xF);
+            const int b = (int) ((SE8_MC_EDGE_B >> (4 * e)) & 0xF);
+            int c0[3], c1[3];
+            mc_corner(x, y, z, a, c0);
+            mc_corner(x, y, z, b, c1);
+            real v0, v1, w;
+            mc_get_data(m, c0, &v0, &w);
+            mc_get_data(m, c1, &v1, &w);
+            const real t = ((real) 0 - v0) / (v1 - v0);
+            for (int i = 0; i < 3; i++) {
+                const real p0 = (real) c0[i] + (real) 0.5;
+                const real p1 = (real) c1[i] + (real) 0.5;
+                tri[n][v][i] = p0 + t * (p1 - p0);
+            }
+            const real* p = tri[n][v];
+            const real dim = (real) m->size;
+            if (p[0] <= 0 || p[1] <= 0 || p[2] <= 0 || p[0] > dim || p[1] > dim || p[2] > dim)
+                skip = 1; /* checkVertex */
+        }
+        if (!skip)
+            n++;
+    }
+    return n;
+}
+
+/* Blocks in m->blocks order, cells x, y, z nested as the reference loops.  faces: cap * 9 reals
+ * (may be NULL with cap 0 to count).  Returns the number of faces. */
+OR_API long or_mesh(void* vm, const real T_OW16[16], real* faces, long cap)
+{
+    const or_map* m = (const or_map*) vm;
+    or_iso T_OW, T_WM, T_OV;
+    iso_from16(T_OW16, &T_OW);
+    for (int i = 0; i < 3; i++) {
+        for (int j = 0; j < 3; j++)
+            T_WM.R[i][j] = i == j ? 1 : 0;
+        T_WM.t[i] = -m->t_MW[i];
+    }
+    iso_mul(&T_OW, &T_WM, &T_OV);
+    for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++)
+            T_OV.R[i][j] = T_OV.R[i][j] * m->res;
+    const long nb = m->n_blocks;
+    long* count = (long*) calloc((size_t) nb + 1, sizeof(long));
+    for (int pass = 0; pass < 2; pass++) {
+#pragma omp parallel for schedule(dynamic, 16)
+        for (long bi = 0; bi < nb; bi++) {
+            const or_octant* b = m->blocks[bi];
+            int last[3];
+            for (int i = 0; i < 3; i++)
+                last[i] = b->coord[i] + OR_BLOCK < m->size - 1 ? b->coord[i] + OR_BLOCK : m->size - 1;
+            long out = pass ? count[bi] : 0;
+            for (int x = b->coord[0]; x < last[0]; x++)
+                for (int y = b->coord[1]; y < last[1]; y++)
+                    for (int z = b->coord[2]; z < last[2]; z++) {
+                        real tri[5][3][3];
+                        const int n = mc_cell(m, x, y, z, tri);
+                        if (pass && out + n <= cap)
+                            for (int j = 0; j < n; j++)
+                                for (int v = 0; v < 3; v++) {
+                                    real o[3];
+                                    iso_apply(&T_OV, tri[j][v], o);
+                                    memcpy(faces + 9 * (out + j) + 3 * v, o, sizeof(o));
+                                }
+                        out += n;
+                    }
+            if (!pass)
+                count[bi] = out;
+        }
+        if (!pass) {
+            long run = 0;
+            for (long bi = 0; bi < nb; bi++) {
+                const long c = count[bi];
+                count[bi] = run;
+                run += c;
+            }
+            count[nb] = run;
+            if (!faces || cap < run)
+                break;
+        }
+    }
+    const long total = count[nb];
+    free(count);
+    return total;
+}

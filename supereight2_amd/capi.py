@@ -19,6 +19,7 @@ SE8_ERR_INVALID = -1
 SE8_ERR_OOM = -2
 SE8_ERR_HIP = -3
 SE8_ERR_CAPACITY = -4
+SE8_ERR_IO = -5
 STORAGE_F32 = 0
 STORAGE_F16 = 1
 COPY_H2D = 0
@@ -149,6 +150,9 @@ SIGNATURES = {
     "se8_get_data": (C.c_int, [_P, _F, C.c_int64, _F, _F]),
     "se8_field_interp": (C.c_int, [_P, _F, C.c_int64, _F, C.POINTER(C.c_uint8)]),
     "se8_field_grad": (C.c_int, [_P, _F, C.c_int64, _F, C.POINTER(C.c_uint8)]),
+    "se8_mesh": (C.c_int, [_P, _F, _P, C.c_int64, C.POINTER(C.c_int64), _P]),
+    "se8_save_mesh": (C.c_int, [_F, C.c_int64, C.c_char_p, _F]),
+    "se8_has_supported_mesh_extension": (C.c_int, [C.c_char_p]),
     "se8_stage_times_get": (C.c_int, [_P, _P, C.POINTER(StageTimes)]),
     "se8_stage_times_reset": (C.c_int, [_P, _P]),
     "se8_device_alloc": (C.c_int, [C.c_size_t, C.POINTER(_P)]),

@@ -641,6 +641,9 @@ struct se8_map {
     bool ray_pending = false;      // events ev[5..6] of the last raycast not yet harvested
     se8_stage_times times{};
     hipEvent_t ev[8]{};
+    // marching cubes: per-block face counts, scanned in place into offsets (+ the total); grown on
+    // demand by the (logically const) se8_mesh
+    mutable se8::DeviceBuffer mesh_offsets;
 };
 
 struct se8_tracker {

@@ -1073,7 +1073,7 @@ extern "C" int se8_map_destroy(se8_map* m)
     DeviceBuffer* bufs[] = {&m->brick_top,    &m->brick_slots, &m->node_child, &m->node_key,   &m->node_ts,
                             &m->block_key,    &m->block_parent, &m->block_ts,  &m->voxels,     &m->counters,
                             &m->update_list,  &m->candidates,   &m->unique_keys, &m->cur_node,
-                            &m->frame_poses};
+                            &m->frame_poses,  &m->mesh_offsets};
     for (DeviceBuffer* b : bufs)
         dev_free(*b);
     m->sorter.release();

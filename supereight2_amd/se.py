@@ -213,6 +213,42 @@ class TSDFMap:
                    "se8_dump_nodes")
         return keys[:N], masks[:N], ts[:N]
 
+    def mesh(self, T_OW=None, min_desired_scale=0, stream=None, out=None):
+        """Map::mesh (map.hpp:396-404, map_impl.hpp:452-458): the surface as an (F, 3, 3) float32
+        CUDA tensor of triangle vertices in frame O, metres (T_OW: host 4x4, default identity).
+        out: an optional float32 CUDA tensor of shape (C, 3, 3) to write into when the mesh fits (no
+        second counting pass).  min_desired_scale is unused by a single-resolution map, as in the
+        reference."""
+        torch = _torch()
+        T = None if T_OW is None else capi.pose16(T_OW)
+        st = _stream_handle(stream)
+        n = C.c_int64()
+        if out is not None:
+            if not (out.is_cuda and out.dtype == torch.float32 and out.is_contiguous() and out.dim() == 3
+                    and tuple(out.shape[1:]) == (3, 3)):
+                raise ValueError("out must be a contiguous float32 CUDA tensor of shape (C, 3, 3)")
+            rc = self.lib.se8_mesh(self.h, T, C.c_void_p(out.data_ptr()), out.shape[0], C.byref(n), st)
+            if rc == capi.SE8_OK:
+                return out[: n.value]
+            if rc != capi.SE8_ERR_CAPACITY:
+                capi.check(rc, "se8_mesh")
+        else:
+            capi.check(self.lib.se8_mesh(self.h, T, None, 0, C.byref(n), st), "se8_mesh")
+        F = n.value
+        faces = torch.empty((max(F, 1), 3, 3), dtype=torch.float32, device="cuda")
+        if F:
+            capi.check(self.lib.se8_mesh(self.h, T, C.c_void_p(faces.data_ptr()), F, C.byref(n), st), "se8_mesh")
+        return faces[:F]
+
+    def saveMesh(self, filename, T_OW=None, min_desired_scale=0):
+        """Map::saveMesh (map.hpp:406-413, map_impl.hpp:462-469): writes mesh(T_OW) as .ply, .vtk or
+        .obj by the extension; returns 0 on success and 1 if the file cannot be written, and raises
+        ValueError for another extension (the reference throws std::invalid_argument)."""
+        torch = _torch()
+        faces = self.mesh(T_OW, min_desired_scale)
+        torch.cuda.current_stream().synchronize()
+        return save_mesh(faces.cpu().numpy(), filename)
+
     def getData(self, points_W):
         p = np.ascontiguousarray(points_W, dtype=np.float32).reshape(-1, 3)
         t = np.zeros(len(p), np.float32)
@@ -235,6 +271,26 @@ class TSDFMap:
         capi.check(self.lib.se8_field_grad(self.h, capi.fptr(p), len(p), capi.fptr(out),
                                            valid.ctypes.data_as(C.POINTER(C.c_uint8))), "se8_field_grad")
         return out, valid.astype(bool)
+
+
+def has_supported_mesh_extension(filename: str) -> bool:
+    """se::io::has_supported_mesh_extension (mesh_io.hpp:24-29)."""
+    return bool(capi.load().se8_has_supported_mesh_extension(str(filename).encode()))
+
+
+def save_mesh(faces, filename, T_OM=None):
+    """se::io::save_mesh (mesh_io.hpp:44-46, impl/mesh_io_impl.hpp:14-230) of host faces (F, 3, 3):
+    0 on success, 1 if the file cannot be written; ValueError for an unsupported extension."""
+    lib = capi.load()
+    f = np.ascontiguousarray(faces, dtype=np.float32).reshape(-1, 9)
+    T = None if T_OM is None else capi.pose16(T_OM)
+    rc = lib.se8_save_mesh(capi.fptr(f) if len(f) else None, len(f), str(filename).encode(), T)
+    if rc == capi.SE8_ERR_IO:
+        return 1
+    if rc == capi.SE8_ERR_INVALID:
+        raise ValueError(lib.se8_last_error().decode())
+    capi.check(rc, "se8_save_mesh")
+    return 0
 
 
 class MapIntegrator:

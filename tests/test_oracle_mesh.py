@@ -1,0 +1,249 @@
+"""Marching cubes: the case table, the oracle's mesher and the mesh file writers, on the CPU.
+
+The case table (supereight2_amd/csrc/se8_mc_table.h) is pinned by what any correct marching-cubes
+triangulation must satisfy, checked for all 256 cases: every triangle vertex lies on an edge whose
+corners differ in inside-ness and every such edge carries a vertex; every triangle faces away from
+the inside corners (its normal has a positive dot product with the gradient of the trilinear
+interpolant, the direction the reference's `occupied` isosurface orients faces in).  The oracle's
+mesher (oracle/se8_oracle.c, or_mesh) is pinned by the reference's meshing unit test setup
+(test/map/meshing/meshing_unittest.cpp:14-77: a 16-voxel octree, the four blocks at x = 8, tsdf +1
+for local x < 4 and -1 otherwise -> 450 faces on the plane x = 12), by the exact root of a linear
+field, by a closed, consistently oriented sphere of Euler characteristic 2, and by the unknown-data
+and map-border rules (isosurface::occupied, the loop bound size - 1).  The writers in libse8.so
+(host code) are checked against the layouts of mesh_io_impl.hpp.
+"""
+import re
+
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+
+TABLE = open(__file__.rsplit("/tests/", 1)[0] + "/supereight2_amd/csrc/se8_mc_table.h").read()
+WORDS = [int(w, 16) for w in re.findall(r"0x([0-9a-f]{16})ull", TABLE)]
+DX, DY, DZ = 0x66, 0xF0, 0xCC
+EA = int(re.search(r"SE8_MC_EDGE_A 0x([0-9a-f]+)ull", TABLE).group(1), 16)
+EB = int(re.search(r"SE8_MC_EDGE_B 0x([0-9a-f]+)ull", TABLE).group(1), 16)
+CORNER = np.array([[(DX >> k) & 1, (DY >> k) & 1, (DZ >> k) & 1] for k in range(8)], float)
+EDGES = [((EA >> 4 * e) & 15, (EB >> 4 * e) & 15) for e in range(12)]
+
+
+def _tris(c):
+    w = WORDS[c]
+    return [[(w >> (4 * (3 * j + v))) & 15 for v in range(3)] for j in range(w >> 60)]
+
+
+def test_corner_and_edge_numbering():
+    """gather_data's corner order (marching_cube_impl.hpp:143-158) and edge_to_corners
+    (marching_cube_impl.hpp:97-127): every edge joins two corners one unit apart."""
+    assert len(WORDS) == 256
+    assert CORNER.tolist() == [[0, 0, 0], [1, 0, 0], [1, 0, 1], [0, 0, 1], [0, 1, 0], [1, 1, 0], [1, 1, 1], [0, 1, 1]]
+    for a, b in EDGES:
+        assert np.abs(CORNER[a] - CORNER[b]).sum() == 1
+    assert len({frozenset(e) for e in EDGES}) == 12
+    for c in range(256):
+        w = WORDS[c]
+        n = w >> 60
+        assert n <= 5
+        for k in range(15):
+            nib = (w >> (4 * k)) & 15
+            assert (nib < 12) if k < 3 * n else (nib == 15)
+
+
+def test_case_table_uses_exactly_the_crossed_edges():
+    for c in range(256):
+        inside = [(c >> k) & 1 for k in range(8)]
+        crossed = {e for e, (a, b) in enumerate(EDGES) if inside[a] != inside[b]}
+        used = {e for t in _tris(c) for e in t}
+        assert used == crossed, c
+    assert _tris(0) == [] and _tris(255) == []
+
+
+def _trilinear_grad(f, p):
+    g = np.zeros(3)
+    for k in range(8):
+        c = CORNER[k]
+        w = [p[i] if c[i] else 1 - p[i] for i in range(3)]
+        s = [1 if c[i] else -1 for i in range(3)]
+        g += f[k] * np.array([s[0] * w[1] * w[2], s[1] * w[0] * w[2], s[2] * w[0] * w[1]])
+    return g
+
+
+def test_case_table_faces_point_away_from_inside():
+    for c in range(256):
+        f = np.array([-1.0 if (c >> k) & 1 else 1.0 for k in range(8)])
+        for t in _tris(c):
+            V = [(CORNER[EDGES[e][0]] + CORNER[EDGES[e][1]]) / 2 for e in t]
+            n = np.cross(V[1] - V[0], V[2] - V[0])
+            assert n @ _trilinear_grad(f, sum(V) / 3) > 0, (c, t)
+
+
+def _map_with(size_vox, res, blocks):
+    """An oracle map of size_vox voxels with the given {block_coord: (tsdf512, weight512)}."""
+    m = O.OracleMap(size_vox * res, res)
+    assert m.size == size_vox
+    bc = np.array(list(blocks.keys()), np.int32).reshape(-1, 3)
+    m.load_blocks(bc, np.array([v[0] for v in blocks.values()]), np.array([v[1] for v in blocks.values()]))
+    return m
+
+
+def _local(i):
+    return i & 7, (i >> 3) & 7, i >> 6
+
+
+def test_reference_meshing_unittest_setup():
+    """meshing_unittest.cpp:14-77 (EqualScaleNeighbour): 15 x 15 straddling cells, 2 faces each."""
+    t = np.array([1.0 if _local(i)[0] < 4 else -1.0 for i in range(512)], np.float32)
+    w = np.ones(512, np.float32)
+    blocks = {(8, y, z): (t, w) for y in (0, 8) for z in (0, 8)}
+    res = 0.01
+    m = _map_with(16, res, blocks)
+    f = m.mesh()
+    assert f.shape == (450, 3, 3)
+    # x = 12 in voxel units (surface between voxel centres 11.5 and 12.5), world = 12 res - dim / 2
+    np.testing.assert_array_equal(f[:, :, 0], np.float32(12 * np.float32(res)) - np.float32(0.08))
+    n = np.cross(f[:, 1] - f[:, 0], f[:, 2] - f[:, 0])
+    assert (n[:, 0] < 0).all() and np.allclose(n[:, 1:], 0)  # away from the occupied side x >= 12
+    # T_OW is applied to every vertex
+    T = np.eye(4, dtype=np.float32)
+    T[:3, 3] = [1.0, -2.0, 0.5]
+    np.testing.assert_allclose(m.mesh(T), f + T[:3, 3], atol=1e-6)
+
+
+def _field_map(fn, size_vox=32, res=0.01, weight_fn=None):
+    blocks = {}
+    nb = size_vox // 8
+    for bz in range(nb):
+        for by in range(nb):
+            for bx in range(nb):
+                t = np.zeros(512, np.float32)
+                w = np.ones(512, np.float32)
+                for i in range(512):
+                    x, y, z = _local(i)
+                    p = np.array([bx * 8 + x + 0.5, by * 8 + y + 0.5, bz * 8 + z + 0.5])
+                    t[i] = fn(p)
+                    if weight_fn is not None:
+                        w[i] = weight_fn(p)
+                blocks[(bx * 8, by * 8, bz * 8)] = (t, w)
+    return _map_with(size_vox, res, blocks)
+
+
+def _to_voxels(f, m_size, res=0.01):
+    dim = m_size * res
+    return (f.astype(np.float64) + dim / 2) / res
+
+
+def test_linear_field_vertices_are_exact_roots():
+    n = np.array([0.3, -0.5, 0.81])
+    n /= np.linalg.norm(n)
+    p0 = np.array([15.3, 16.1, 14.7])
+    m = _field_map(lambda p: np.clip((p - p0) @ n / 8.0, -1, 1))
+    f = m.mesh(np.diag([100.0, 100.0, 100.0, 1.0]).astype(np.float32))  # metres -> voxel units x 100
+    v = f.reshape(-1, 3).astype(np.float64) + 16.0  # T_WM = -dim/2 = -0.16 m = -16 voxels
+    assert len(v) > 300
+    np.testing.assert_allclose((v - p0) @ n, 0, atol=2e-5)
+    tri = f.astype(np.float64)
+    nrm = np.cross(tri[:, 1] - tri[:, 0], tri[:, 2] - tri[:, 0])
+    assert (nrm @ n > 0).all()  # away from the inside (tsdf < 0) half-space
+
+
+def test_sphere_mesh_is_closed_and_outward():
+    c, r = np.array([16.2, 15.7, 16.4]), 9.3
+    m = _field_map(lambda p: np.clip((np.linalg.norm(p - c) - r) / 8.0, -1, 1))
+    f = _to_voxels(m.mesh(), 32)
+    q = np.round(f.reshape(-1, 3) * 1e4).astype(np.int64)  # neighbouring cells interpolate a shared
+    _, vid = np.unique(q, axis=0, return_inverse=True)  # edge from opposite ends: merge to 1e-4 voxel
+    tri = vid.reshape(-1, 3)
+    assert (tri[:, 0] != tri[:, 1]).all() and (tri[:, 1] != tri[:, 2]).all() and (tri[:, 0] != tri[:, 2]).all()
+    directed = np.concatenate([tri[:, [0, 1]], tri[:, [1, 2]], tri[:, [2, 0]]])
+    d = {tuple(e) for e in directed.tolist()}
+    assert len(d) == len(directed)  # every directed edge once: consistent orientation
+    assert all((b, a) in d for a, b in d)  # and its twin: closed
+    V, E, F = int(vid.max()) + 1, len(directed) // 2, len(tri)
+    assert V - E + F == 2
+    cen = f.mean(axis=1)
+    nrm = np.cross(f[:, 1] - f[:, 0], f[:, 2] - f[:, 0])
+    assert (np.einsum("ij,ij->i", nrm, cen - c) > 0).all()
+    np.testing.assert_allclose(np.linalg.norm(f.reshape(-1, 3) - c, axis=1), r, atol=0.05)
+
+
+def test_unknown_corners_and_map_border():
+    """No face from a cell with a weight-0 corner (isosurface::occupied); the cells of the first and
+    the last column inside the map are meshed (loop bound size - 1)."""
+    c, r = np.array([16.0, 16.0, 16.0]), 6.0
+    sphere = lambda p: np.clip((np.linalg.norm(p - c) - r) / 8.0, -1, 1)
+    full = _field_map(sphere).mesh()
+    holes = _field_map(sphere, weight_fn=lambda p: 0.0 if p[0] > 16 else 1.0).mesh()
+    vx = _to_voxels(holes, 32)[:, :, 0]
+    assert len(holes) < len(full) and (vx.max(axis=1) <= 16.0 + 1e-5).all()
+    # the first and the last cell columns (x = 0: centres 0.5..1.5; x = size - 2 = 30: centres
+    # 30.5..31.5) are meshed; x = size - 1 has no +1 corner inside the map and is not
+    for x0 in (0.6, 31.4):
+        f = _to_voxels(_field_map(lambda p: np.clip((p[0] - x0) / 8.0, -1, 1)).mesh(), 32)
+        assert len(f) == 2 * 31 * 31
+        np.testing.assert_allclose(f[:, :, 0], x0, atol=1e-4)
+
+
+def test_has_supported_mesh_extension():
+    """mesh_io_unittest.cpp:12-25."""
+    from supereight2_amd import se
+
+    cases = [("foo.ply", True), ("/foo/bar.vtk", True), ("foo/bar.obj", True), (".ply", True), ("foo.png", False),
+             ("", False)]
+    for name, ok in cases:
+        assert se.has_supported_mesh_extension(name) == ok
+
+
+def _faces():
+    return np.array([[[0.5, 1.0, 2.25], [1.5, 1.0, 2.25], [0.5, 2.0, 2.25]],
+                     [[-1.0, 0.125, 3.0], [1e-7, 123456.7, 3.0], [0.0, 0.0, -0.0]]], np.float32)
+
+
+def test_save_mesh_ply(tmp_path):
+    from supereight2_amd import se
+
+    p = tmp_path / "m.ply"
+    assert se.save_mesh(_faces(), str(p)) == 0
+    lines = p.read_text().split("\n")
+    assert lines[:12] == ["ply", "format ascii 1.0", "comment Mesh generated by supereight2_amd", "element vertex 6",
+                          "property float x", "property float y", "property float z", "element face 2",
+                          "property list uchar int vertex_index", "property uchar red", "property uchar green",
+                          "property uchar blue"]
+    assert lines[12] == "end_header"
+    verts = np.array([[float(x) for x in l.split()] for l in lines[13:19]])
+    np.testing.assert_allclose(verts, _faces().reshape(-1, 3), rtol=5e-6)  # 6 significant digits
+    assert lines[13] == "0.5 1 2.25" and lines[17] == "1e-07 123457 3"  # C++ stream (%g) formatting
+    assert lines[19:21] == ["3 0 1 2 102 194 165", "3 3 4 5 102 194 165"] and lines[21] == ""
+
+
+def test_save_mesh_vtk_and_obj(tmp_path):
+    from supereight2_amd import se
+
+    T = np.eye(4, dtype=np.float32)
+    T[:3, 3] = [1, 2, 3]
+    v = tmp_path / "m.vtk"
+    assert se.save_mesh(_faces(), str(v), T) == 0
+    text = v.read_text()
+    head = "# vtk DataFile Version 3.0\nMesh generated by supereight2_amd\nASCII\nDATASET POLYDATA\n\nPOINTS 6 float\n"
+    assert text.startswith(head)
+    body = text[len(head):].split("\n")
+    np.testing.assert_allclose(np.array([[float(x) for x in l.split()] for l in body[:6]]),
+                               _faces().reshape(-1, 3) + [1, 2, 3], rtol=5e-6)
+    assert body[6:10] == ["", "POLYGONS 2 8", "3 0 1 2", "3 3 4 5"]
+    assert body[10:14] == ["", "CELL_DATA 2", "", "SCALARS scale unsigned_char"]
+    assert body[14:17] == ["LOOKUP_TABLE default", "0", "0"]
+    assert body[17] == "COLOR_SCALARS scale_colour 3" and body[18] == body[19] == "0.4 0.760784 0.647059"
+    o = tmp_path / "m.obj"
+    assert se.save_mesh(_faces(), str(o)) == 0
+    lines = o.read_text().split("\n")
+    assert lines[:3] == ["# Mesh generated by supereight2_amd", "# 6 vertices", "# 2 faces"]
+    assert lines[3] == "v 0.5 1 2.25" and lines[9:11] == ["f 1 2 3", "f 4 5 6"]
+
+
+def test_save_mesh_errors(tmp_path):
+    from supereight2_amd import se
+
+    with pytest.raises(ValueError, match="unknown file extension"):
+        se.save_mesh(_faces(), str(tmp_path / "m.png"))
+    assert se.save_mesh(_faces(), str(tmp_path / "missing_dir" / "m.ply")) == 1
+    assert se.save_mesh(np.zeros((0, 3, 3), np.float32), str(tmp_path / "e.ply")) == 0
