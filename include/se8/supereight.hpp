@@ -9,6 +9,8 @@
 //   se::PinholeCamera, se::Image<T>        include/se/sensor/pinhole_camera.hpp:74-78,
 //                                          include/se/image/image.hpp:18-111
 //   se::OusterLidar                        include/se/sensor/ouster_lidar.hpp:12-80
+//   se::Map::mesh / saveMesh, se::Triangle include/se/map/map.hpp:396-413,
+//                                          include/se/map/algorithms/mesh.hpp:48-103
 // Images live in host memory as in the reference; each call stages them through device buffers
 // owned by the object (or a per-thread scratch for raycast_volume).  Value types below provide the
 // subset of Eigen's interface the call sites use (x(), y(), z(), translation(), matrix data).
@@ -21,6 +23,7 @@
 #include <cstring>
 #include <optional>
 #include <stdexcept>
+#include <utility>
 #include <string>
 #include <vector>
 
@@ -122,6 +125,15 @@ class Image {
     int w_, h_;
     std::vector<T> data_;
 };
+
+// se::MeshFace<3> / se::Triangle (include/se/map/algorithms/mesh.hpp:48-66, 88-103): three
+// vertices and the octree scale the face came from (always 0 for a single-resolution map).
+struct Triangle {
+    Vector3f vertexes[3];
+    std::int8_t scale = 0;
+    static constexpr size_t num_vertexes = 3;
+};
+typedef std::vector<Triangle> TriangleMesh;
 
 enum class Res { Single, Multi };
 
@@ -273,12 +285,57 @@ class TSDFMap {
         detail::check(se8_field_grad(h_, p.data(), 1, g.v, &ok), "se8_field_grad");
         return ok ? std::optional<Vector3f>(g) : std::nullopt;
     }
+    typedef TriangleMesh SurfaceMesh;
+
+    // mesh (map.hpp:396-404, map_impl.hpp:452-458): the surface in frame O (metres), vertices
+    // mapped by T_OW; min_desired_scale is unused by a single-resolution map.  The faces are
+    // meshed on the device and copied back.
+    SurfaceMesh mesh(const Isometry3f& T_OW = Isometry3f::Identity(), const int min_desired_scale = 0) const
+    {
+        (void) min_desired_scale;
+        int64_t n = 0;
+        detail::check(se8_mesh(h_, T_OW.data(), nullptr, 0, &n, nullptr), "se8_mesh");
+        SurfaceMesh out((size_t) n);
+        if (n == 0)
+            return out;
+        void* dev = mesh_buf_.get(sizeof(float) * 9 * (size_t) n);
+        detail::check(se8_mesh(h_, T_OW.data(), (float*) dev, n, &n, nullptr), "se8_mesh");
+        std::vector<float> host(9 * (size_t) n);
+        detail::check(se8_copy(host.data(), dev, sizeof(float) * host.size(), SE8_COPY_D2H, nullptr), "se8_copy");
+        detail::check(se8_stream_synchronize(nullptr), "se8_stream_synchronize");
+        for (size_t f = 0; f < out.size(); f++)
+            for (int v = 0; v < 3; v++)
+                out[f].vertexes[v] = Vector3f(host[9 * f + 3 * v], host[9 * f + 3 * v + 1], host[9 * f + 3 * v + 2]);
+        return out;
+    }
+
+    // saveMesh (map.hpp:406-413, map_impl.hpp:462-469): .ply, .vtk or .obj by the extension;
+    // 0 on success, non-zero if the file cannot be written; std::invalid_argument for another
+    // extension, as io::save_mesh throws (mesh_io_impl.hpp:14-29).
+    int saveMesh(const std::string& filename, const Isometry3f& T_OW = Isometry3f::Identity(),
+                 const int min_desired_scale = 0) const
+    {
+        if (!se8_has_supported_mesh_extension(filename.c_str()))
+            throw std::invalid_argument("unknown file extension: " + filename);
+        const SurfaceMesh m = mesh(T_OW, min_desired_scale);
+        std::vector<float> flat(9 * m.size());
+        for (size_t f = 0; f < m.size(); f++)
+            for (int v = 0; v < 3; v++)
+                std::memcpy(&flat[9 * f + 3 * v], m[f].vertexes[v].data(), sizeof(float) * 3);
+        const int rc = se8_save_mesh(flat.data(), (int64_t) m.size(), filename.c_str(), nullptr);
+        if (rc == SE8_ERR_IO)
+            return 1;
+        detail::check(rc, "se8_save_mesh");
+        return 0;
+    }
+
     se8_map* handle() const { return h_; }
 
     private:
     se8_map* h_ = nullptr;
     float res_;
     Vector3f dim_;
+    mutable detail::DeviceBuffer mesh_buf_;
 };
 
 template<typename SensorT>

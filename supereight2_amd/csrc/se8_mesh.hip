@@ -12,12 +12,15 @@
 //
 // Here one 256-thread workgroup meshes one block: it stages the block's 8^3 voxels and the 217
 // voxels of the 7 neighbour blocks that the +1 corners reach (a 9^3 corner grid) into LDS, then
-// every thread meshes two cells from LDS.  Two launches: the count pass writes each block's face
-// count, one workgroup scans the counts into offsets, and the emit pass recomputes the cells and
-// writes every face at its block's offset plus its rank inside the block.  The output order is
-// therefore fixed for a given pool (block slot, then cell index x + 8y + 64z, then the case's
-// triangle order); the reference's order depends on its OpenMP schedule, so faces are compared as
-// a set.
+// every thread meshes two cells from LDS.  The launches: a neighbour table (the 7 neighbour slots of
+// every block, so that a workgroup's halo gather is two memory round trips: table, then voxels);
+// the count pass (faces per block); a two-level scan of the counts, packed with a non-empty flag,
+// giving each block its face offset and its rank among the non-empty blocks; a compaction of the
+// non-empty blocks; and the emit pass over those only, which recomputes the cells and writes the
+// faces through an LDS stage so that the stores are contiguous across the workgroup.  The host reads
+// the total back between the scan and the emit (the capacity check).  The output order is fixed
+// for a given pool (block slot, then cell index x + 8y + 64z, then the case's triangle order); the
+// reference's order depends on its OpenMP schedule, so faces are compared as a set.
 #include "se8_internal.hpp"
 #include "se8_mc_table.h"
 
@@ -74,64 +77,80 @@ __device__ __forceinline__ bool vertex_outside(const float p[3], float dim)
     return p[0] <= 0.f || p[1] <= 0.f || p[2] <= 0.f || p[0] > dim || p[1] > dim || p[2] > dim;
 }
 
-// Exclusive prefix of v over the workgroup and the workgroup total (64-wide waves).
-__device__ __forceinline__ uint32_t block_exclusive_scan(uint32_t v, uint32_t* s_wave, uint32_t& total)
+// Exclusive prefix of v over the workgroup and the workgroup total (64-wide waves, 32-bit halves
+// shuffled separately).
+__device__ __forceinline__ uint64_t block_exclusive_scan(uint64_t v, uint64_t* s_wave, uint64_t& total)
 {
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    uint32_t inc = v;
+    uint64_t inc = v;
 #pragma unroll
     for (int d = 1; d < 64; d <<= 1) {
-        const uint32_t o = __shfl_up(inc, d, 64);
+        const uint32_t lo = __shfl_up((uint32_t) inc, d, 64);
+        const uint32_t hi = __shfl_up((uint32_t) (inc >> 32), d, 64);
         if (lane >= d)
-            inc += o;
+            inc += ((uint64_t) hi << 32) | lo;
     }
     if (lane == 63)
         s_wave[wave] = inc;
     __syncthreads();
-    uint32_t before = 0;
+    uint64_t before = 0;
     total = 0;
 #pragma unroll
     for (int w = 0; w < kMeshThreads / 64; w++) {
-        const uint32_t s = s_wave[w];
-        before += w < wave ? s : 0u;
+        const uint64_t s = s_wave[w];
+        before += w < wave ? s : 0ull;
         total += s;
     }
+    __syncthreads(); // s_wave may be reused by the caller
     return before + inc - v;
 }
 
-// Count pass (Emit = false): block_faces[slot] = number of faces of the block.
-// Emit pass (Emit = true): block_faces[slot] = offset of the block's first face; its faces are
-// written to faces (9 floats each: three vertices in the output frame) in the fixed order.
-template <bool Emit>
-__global__ __launch_bounds__(kMeshThreads) void k_mesh(MapView m, Iso T_OV, uint64_t* block_faces, float* faces)
-{
-    __shared__ float s_tsdf[kGrid];
-    __shared__ uint8_t s_valid[kGrid];
-    __shared__ uint64_t s_tri[256];
-    __shared__ int s_nb[8];
-    __shared__ uint32_t s_wave[kMeshThreads / 64];
+// Per-block scan value: face count in the high bits, 1 for a block with faces in the low bits, so
+// that one exclusive scan yields both the face offset and the rank among non-empty blocks.
+constexpr int kPackShift = 28;
+constexpr uint64_t kPackMask = (1ull << kPackShift) - 1;
 
-    const int slot = blockIdx.x;
-    const int t = threadIdx.x;
-    s_tri[t] = se8_mc_tri[t];
+// Neighbour table: nb[8 slot + t] = slot of the block at (bc + 8 (t & 1, t >> 1 & 1, t >> 2 & 1)),
+// -1 if unallocated or beyond the map's upper faces (no wrap-around: the cells that would read it
+// are outside the meshed range), nb[8 slot] = slot.
+__global__ __launch_bounds__(256) void k_mesh_neighbours(MapView m, int n, int* nb)
+{
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= 8 * n)
+        return;
+    const int slot = i >> 3, t = i & 7;
     int bc[3];
     key_to_coord(m.block_key[slot], bc);
-    if (t < 8) {
-        // neighbour t = (dx, dy, dz) bits; none beyond the map's upper face (no wrap-around): the
-        // cells that would read it are outside the meshed range anyway
-        const int nx = bc[0] + 8 * (t & 1), ny = bc[1] + 8 * ((t >> 1) & 1), nz = bc[2] + 8 * ((t >> 2) & 1);
-        int nb = slot;
-        if (t != 0)
-            nb = (nx < m.size && ny < m.size && nz < m.size) ? lookup_block_nb(m, nx, ny, nz) : -1;
-        s_nb[t] = nb;
-    }
+    const int nx = bc[0] + 8 * (t & 1), ny = bc[1] + 8 * ((t >> 1) & 1), nz = bc[2] + 8 * ((t >> 2) & 1);
+    int r = slot;
+    if (t != 0)
+        r = (nx < m.size && ny < m.size && nz < m.size) ? lookup_block_nb(m, nx, ny, nz) : -1;
+    nb[i] = r;
+}
+
+struct MeshLds {
+    float tsdf[kGrid];
+    uint8_t valid[kGrid];
+    uint64_t tri[256];
+    uint64_t wave[kMeshThreads / 64];
+    int nb[8];
+};
+
+// Stages the 9^3 corner grid of block `slot` (its 512 voxels and the +1 faces of its neighbours)
+// and the case table into LDS.
+__device__ __forceinline__ void stage_block(const MapView& m, int slot, const int* nb_tab, MeshLds& L)
+{
+    const int t = threadIdx.x;
+    L.tri[t] = se8_mc_tri[t];
+    if (t < 8)
+        L.nb[t] = nb_tab[8 * slot + t];
 #pragma unroll
     for (int i = t; i < 512; i += kMeshThreads) {
         float ts, w;
         load_voxel(m, slot, i, ts, w);
         const int g = (i & 7) + 9 * ((i >> 3) & 7) + 81 * (i >> 6);
-        s_tsdf[g] = ts;
-        s_valid[g] = w != 0.f;
+        L.tsdf[g] = ts;
+        L.valid[g] = w != 0.f;
     }
     __syncthreads();
     if (t < kHalo) {
@@ -159,97 +178,120 @@ __global__ __launch_bounds__(kMeshThreads) void k_mesh(MapView m, Iso T_OV, uint
         else {
             x = 8, y = 8, z = 8;
         }
-        const int nb = s_nb[(x >> 3) | ((y >> 3) << 1) | ((z >> 3) << 2)];
+        const int s = L.nb[(x >> 3) | ((y >> 3) << 1) | ((z >> 3) << 2)];
         float ts = 1.f, w = 0.f; // getData of an unallocated block: the default {1, 0}
-        if (nb >= 0)
-            load_voxel(m, nb, (x & 7) + 8 * (y & 7) + 64 * (z & 7), ts, w);
+        if (s >= 0)
+            load_voxel(m, s, (x & 7) + 8 * (y & 7) + 64 * (z & 7), ts, w);
         const int g = x + 9 * y + 81 * z;
-        s_tsdf[g] = ts;
-        s_valid[g] = w != 0.f;
+        L.tsdf[g] = ts;
+        L.valid[g] = w != 0.f;
     }
     __syncthreads();
+}
 
+// The two cells of this thread (cell index t and t + 256 of the block): case words and the mask
+// of faces kept by checkVertex (bit j = triangle j).
+struct ThreadCells {
+    uint64_t word[2];
+    uint32_t keep[2];
+};
+
+__device__ __forceinline__ void cell_geometry(int ci, const int bc[3], int lc[3], int c[3], int& g)
+{
+    lc[0] = ci & 7, lc[1] = (ci >> 3) & 7, lc[2] = ci >> 6;
+    c[0] = bc[0] + lc[0], c[1] = bc[1] + lc[1], c[2] = bc[2] + lc[2];
+    g = lc[0] + 9 * lc[1] + 81 * lc[2];
+}
+
+__device__ __forceinline__ uint32_t classify_cells(const MapView& m, const MeshLds& L, const int bc[3], ThreadCells& tc)
+{
     const float dim = (float) m.size;
     uint32_t n_faces = 0;
-    uint32_t keep[2];
-    uint64_t word[2];
 #pragma unroll
     for (int h = 0; h < 2; h++) {
-        const int ci = t + h * kMeshThreads;
-        const int lc[3] = {ci & 7, (ci >> 3) & 7, ci >> 6};
-        const int c[3] = {bc[0] + lc[0], bc[1] + lc[1], bc[2] + lc[2]};
-        const int g = lc[0] + 9 * lc[1] + 81 * lc[2];
+        int lc[3], c[3], g;
+        cell_geometry(threadIdx.x + h * kMeshThreads, bc, lc, c, g);
         uint32_t idx = 0;
         bool all_valid = true;
 #pragma unroll
         for (int k = 0; k < 8; k++) {
             const int gk = corner_at(g, k);
-            all_valid = all_valid && s_valid[gk];
-            idx |= (s_tsdf[gk] < 0.f ? 1u : 0u) << k;
+            all_valid = all_valid && L.valid[gk];
+            idx |= (L.tsdf[gk] < 0.f ? 1u : 0u) << k;
         }
         const bool in_range = c[0] < m.size - 1 && c[1] < m.size - 1 && c[2] < m.size - 1;
-        word[h] = s_tri[(all_valid && in_range) ? idx : 0u];
-        keep[h] = 0;
-        const int ntri = (int) (word[h] >> 60);
+        const uint64_t word = L.tri[(all_valid && in_range) ? idx : 0u];
+        uint32_t keep = 0;
+        const int ntri = (int) (word >> 60);
         for (int j = 0; j < ntri; j++) {
             bool out = false;
 #pragma unroll
             for (int v = 0; v < 3; v++) {
                 float p[3];
-                edge_vertex(s_tsdf, g, c, (int) ((word[h] >> (4 * (3 * j + v))) & 0xF), p);
+                edge_vertex(L.tsdf, g, c, (int) ((word >> (4 * (3 * j + v))) & 0xF), p);
                 out = out || vertex_outside(p, dim);
             }
-            keep[h] |= (out ? 0u : 1u) << j;
+            keep |= (out ? 0u : 1u) << j;
         }
-        n_faces += __builtin_popcount(keep[h]);
+        tc.word[h] = word;
+        tc.keep[h] = keep;
+        n_faces += __builtin_popcount(keep);
     }
-
-    uint32_t total;
-    const uint32_t rank = block_exclusive_scan(n_faces, s_wave, total);
-    if (!Emit) {
-        if (t == 0)
-            block_faces[slot] = total;
-        return;
-    }
-    uint64_t out = block_faces[slot] + rank;
-#pragma unroll
-    for (int h = 0; h < 2; h++) {
-        if (!keep[h])
-            continue;
-        const int ci = t + h * kMeshThreads;
-        const int lc[3] = {ci & 7, (ci >> 3) & 7, ci >> 6};
-        const int c[3] = {bc[0] + lc[0], bc[1] + lc[1], bc[2] + lc[2]};
-        const int g = lc[0] + 9 * lc[1] + 81 * lc[2];
-        for (int j = 0; j < 5; j++) {
-            if (!((keep[h] >> j) & 1))
-                continue;
-            float* f = faces + out * 9;
-#pragma unroll
-            for (int v = 0; v < 3; v++) {
-                float p[3], o[3];
-                edge_vertex(s_tsdf, g, c, (int) ((word[h] >> (4 * (3 * j + v))) & 0xF), p);
-                iso_apply(T_OV, p, o);
-                f[3 * v] = o[0];
-                f[3 * v + 1] = o[1];
-                f[3 * v + 2] = o[2];
-            }
-            out++;
-        }
-    }
+    return n_faces;
 }
 
-// In-place exclusive scan of the n per-block face counts with one 1024-thread workgroup;
-// data[n] receives the total.
-__global__ __launch_bounds__(1024) void k_mesh_scan(uint64_t* data, int n)
+// Count pass: scan[slot] = (faces << kPackShift) | (faces > 0) for every block of the pool.
+__global__ __launch_bounds__(kMeshThreads) void k_mesh_count(MapView m, const int* nb_tab, uint64_t* scan)
+{
+    __shared__ MeshLds L;
+    const int slot = blockIdx.x;
+    int bc[3];
+    key_to_coord(m.block_key[slot], bc);
+    stage_block(m, slot, nb_tab, L);
+    ThreadCells tc;
+    const uint32_t n_faces = classify_cells(m, L, bc, tc);
+    uint64_t total;
+    (void) block_exclusive_scan(n_faces, L.wave, total);
+    if (threadIdx.x == 0)
+        scan[slot] = (total << kPackShift) | (total > 0 ? 1ull : 0ull);
+}
+
+constexpr int kScanTile = 2048; // scan values per workgroup of the tile scan (8 per thread)
+
+// Exclusive scan of the packed values within tiles of kScanTile; tile_sums[tile] = the tile total.
+__global__ __launch_bounds__(256) void k_mesh_scan_tiles(uint64_t* data, int n, uint64_t* tile_sums)
+{
+    __shared__ uint64_t s_wave[4];
+    const int base = blockIdx.x * kScanTile + threadIdx.x * 8;
+    uint64_t v[8], s = 0;
+#pragma unroll
+    for (int j = 0; j < 8; j++) {
+        v[j] = base + j < n ? data[base + j] : 0ull;
+        s += v[j];
+    }
+    uint64_t total;
+    uint64_t run = block_exclusive_scan(s, s_wave, total);
+#pragma unroll
+    for (int j = 0; j < 8; j++) {
+        if (base + j < n)
+            data[base + j] = run;
+        run += v[j];
+    }
+    if (threadIdx.x == 0)
+        tile_sums[blockIdx.x] = total;
+}
+
+// Exclusive scan of the n_tiles tile totals with one workgroup; tile_sums[n_tiles] = grand total.
+__global__ __launch_bounds__(1024) void k_mesh_scan_top(uint64_t* tile_sums, int n_tiles)
 {
     __shared__ uint64_t sums[1024];
     const int t = threadIdx.x;
-    const int chunk = (n + 1023) / 1024;
-    const int start = min(t * chunk, n);
-    const int end = min(start + chunk, n);
+    const int chunk = (n_tiles + 1023) / 1024;
+    const int start = min(t * chunk, n_tiles);
+    const int end = min(start + chunk, n_tiles);
     uint64_t s = 0;
     for (int i = start; i < end; i++)
-        s += data[i];
+        s += tile_sums[i];
     sums[t] = s;
     __syncthreads();
     for (int off = 1; off < 1024; off <<= 1) {
@@ -260,12 +302,82 @@ __global__ __launch_bounds__(1024) void k_mesh_scan(uint64_t* data, int n)
     }
     uint64_t run = sums[t] - s;
     for (int i = start; i < end; i++) {
-        const uint64_t v = data[i];
-        data[i] = run;
+        const uint64_t v = tile_sums[i];
+        tile_sums[i] = run;
         run += v;
     }
     if (t == 1023)
-        data[n] = sums[1023];
+        tile_sums[n_tiles] = sums[1023];
+}
+
+__device__ __forceinline__ uint64_t scanned(const uint64_t* scan, const uint64_t* tile_sums, int slot)
+{
+    return scan[slot] + tile_sums[slot / kScanTile];
+}
+
+// list[rank among non-empty blocks] = slot, for every block with faces.
+__global__ __launch_bounds__(256) void k_mesh_compact(const uint64_t* scan, const uint64_t* tile_sums, int n,
+                                                      int* list)
+{
+    const int slot = blockIdx.x * 256 + threadIdx.x;
+    if (slot >= n)
+        return;
+    const uint64_t here = scanned(scan, tile_sums, slot);
+    const uint64_t next = slot + 1 < n ? scanned(scan, tile_sums, slot + 1) : tile_sums[(n + kScanTile - 1) / kScanTile];
+    if ((next & kPackMask) != (here & kPackMask))
+        list[here & kPackMask] = slot;
+}
+
+constexpr int kStageFaces = 256; // faces staged in LDS per write round
+
+// Emit pass over the non-empty blocks: faces written at the block's offset, in the fixed order,
+// through an LDS stage so that the global stores are contiguous across the workgroup.
+__global__ __launch_bounds__(kMeshThreads) void k_mesh_emit(MapView m, Iso T_OV, const int* nb_tab,
+                                                             const uint64_t* scan, const uint64_t* tile_sums,
+                                                             const int* list, float* faces)
+{
+    __shared__ MeshLds L;
+    __shared__ float s_out[kStageFaces * 9];
+    const int slot = list[blockIdx.x];
+    int bc[3];
+    key_to_coord(m.block_key[slot], bc);
+    const uint64_t first = scanned(scan, tile_sums, slot) >> kPackShift;
+    stage_block(m, slot, nb_tab, L);
+    ThreadCells tc;
+    const uint32_t n_faces = classify_cells(m, L, bc, tc);
+    uint64_t total;
+    const uint32_t rank = (uint32_t) block_exclusive_scan(n_faces, L.wave, total);
+    float* dst = faces + first * 9;
+    for (uint32_t round = 0; round < (uint32_t) total; round += kStageFaces) {
+        uint32_t r = rank;
+#pragma unroll
+        for (int h = 0; h < 2; h++) {
+            int lc[3], c[3], g;
+            cell_geometry(threadIdx.x + h * kMeshThreads, bc, lc, c, g);
+            for (int j = 0; j < 5; j++) {
+                if (!((tc.keep[h] >> j) & 1))
+                    continue;
+                if (r >= round && r < round + kStageFaces) {
+                    float* f = s_out + (r - round) * 9;
+#pragma unroll
+                    for (int v = 0; v < 3; v++) {
+                        float p[3], o[3];
+                        edge_vertex(L.tsdf, g, c, (int) ((tc.word[h] >> (4 * (3 * j + v))) & 0xF), p);
+                        iso_apply(T_OV, p, o);
+                        f[3 * v] = o[0];
+                        f[3 * v + 1] = o[1];
+                        f[3 * v + 2] = o[2];
+                    }
+                }
+                r++;
+            }
+        }
+        __syncthreads();
+        const int nf = (int) min((uint64_t) kStageFaces, total - round);
+        for (int i = threadIdx.x; i < nf * 9; i += kMeshThreads)
+            dst[(size_t) round * 9 + i] = s_out[i];
+        __syncthreads();
+    }
 }
 
 } // namespace
@@ -302,26 +414,40 @@ extern "C" int se8_mesh(const se8_map* m, const float T_OW[16], float* faces_dev
     *num_faces = 0;
     if (n == 0)
         return SE8_OK;
-    int rc = dev_alloc(m->mesh_offsets, sizeof(uint64_t) * (size_t) (n + 1));
+    // scratch: scan values (n), tile sums (n_tiles + 1), neighbour table (8 n ints), list (n ints)
+    const int64_t n_tiles = (n + kScanTile - 1) / kScanTile;
+    const size_t off_tiles = sizeof(uint64_t) * (size_t) n;
+    const size_t off_nb = off_tiles + sizeof(uint64_t) * (size_t) (n_tiles + 1);
+    const size_t off_list = off_nb + sizeof(int) * 8 * (size_t) n;
+    int rc = dev_alloc(m->mesh_offsets, off_list + sizeof(int) * (size_t) n);
     if (rc != SE8_OK)
         return rc;
-    uint64_t* offs = (uint64_t*) m->mesh_offsets.ptr;
+    char* base = (char*) m->mesh_offsets.ptr;
+    uint64_t* scan = (uint64_t*) base;
+    uint64_t* tile_sums = (uint64_t*) (base + off_tiles);
+    int* nb = (int*) (base + off_nb);
+    int* list = (int*) (base + off_list);
     const Iso T_OV = mesh_T_OV(m, T_OW);
-    k_mesh<false><<<(unsigned) n, kMeshThreads, 0, s>>>(m->view, T_OV, offs, nullptr);
-    k_mesh_scan<<<1, 1024, 0, s>>>(offs, (int) n);
+    k_mesh_neighbours<<<(unsigned) ((8 * n + 255) / 256), 256, 0, s>>>(m->view, (int) n, nb);
+    k_mesh_count<<<(unsigned) n, kMeshThreads, 0, s>>>(m->view, nb, scan);
+    k_mesh_scan_tiles<<<(unsigned) n_tiles, 256, 0, s>>>(scan, (int) n, tile_sums);
+    k_mesh_scan_top<<<1, 1024, 0, s>>>(tile_sums, (int) n_tiles);
     uint64_t total = 0;
-    SE8_HIP_CHECK(hipMemcpyAsync(&total, offs + n, sizeof(total), hipMemcpyDeviceToHost, s));
+    SE8_HIP_CHECK(hipMemcpyAsync(&total, tile_sums + n_tiles, sizeof(total), hipMemcpyDeviceToHost, s));
     SE8_HIP_CHECK(hipStreamSynchronize(s));
     SE8_HIP_CHECK(hipGetLastError());
-    *num_faces = (int64_t) total;
-    if (max_faces == 0)
-        return SE8_OK; // count only
-    if ((int64_t) total > max_faces) {
-        set_error("se8_mesh: the mesh has " + std::to_string(total) + " faces, the buffer holds " +
+    const int64_t n_faces = (int64_t) (total >> kPackShift);
+    const int64_t n_nonempty = (int64_t) (total & kPackMask);
+    *num_faces = n_faces;
+    if (max_faces == 0 || n_faces == 0)
+        return SE8_OK; // count only, or nothing to write
+    if (n_faces > max_faces) {
+        set_error("se8_mesh: the mesh has " + std::to_string(n_faces) + " faces, the buffer holds " +
                   std::to_string(max_faces));
         return SE8_ERR_CAPACITY;
     }
-    k_mesh<true><<<(unsigned) n, kMeshThreads, 0, s>>>(m->view, T_OV, offs, faces_dev);
+    k_mesh_compact<<<(unsigned) ((n + 255) / 256), 256, 0, s>>>(scan, tile_sums, (int) n, list);
+    k_mesh_emit<<<(unsigned) n_nonempty, kMeshThreads, 0, s>>>(m->view, T_OV, nb, scan, tile_sums, list, faces_dev);
     SE8_HIP_CHECK(hipGetLastError());
     return SE8_OK;
 }

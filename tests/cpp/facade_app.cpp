@@ -96,6 +96,27 @@ int main()
     std::printf("interp %d %.5f blocks %lld worst_x_err %.6f\n", f_interp ? 1 : 0, f_interp ? *f_interp : 0.f,
                 (long long) map.info().num_blocks, worst);
 
+    // surface mesh of the room: every vertex within 2 voxels of a wall, floor or ceiling
+    const se::TSDFMap<se::Res::Single>::SurfaceMesh mesh = map.mesh();
+    double mesh_worst = 0.0;
+    for (const se::Triangle& t : mesh)
+        for (const se::Vector3f& v : t.vertexes) {
+            double d = 1e9;
+            for (int i = 0; i < 3; i++)
+                d = std::fmin(d, std::fmin(std::fabs(v(i) - lo[i]), std::fabs(v(i) - hi[i])));
+            mesh_worst = std::fmax(mesh_worst, d);
+        }
+    const int saved = map.saveMesh("facade_app_mesh.ply");
+    bool threw = false;
+    try {
+        map.saveMesh("facade_app_mesh.png");
+    }
+    catch (const std::invalid_argument&) {
+        threw = true;
+    }
+    std::printf("mesh faces %zu worst_plane_dist %.5f saved %d bad_ext_threw %d\n", mesh.size(), mesh_worst, saved,
+                (int) threw);
+
     // A 512 x 32 beam lidar at 2 m over the ground plane z = 0 with a wall at x = 12, two scans
     // 0.5 m apart, into a 51.2 m map at 4 cm voxels; the field at a ground point is then ~0.
     se::OusterLidar::Config lc;

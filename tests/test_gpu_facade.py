@@ -13,10 +13,10 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 APP = os.path.join(ROOT, "tests", "cpp", "_build", "facade_app")
 
 
-def test_facade_app_tracks_and_maps():
+def test_facade_app_tracks_and_maps(tmp_path):
     assert os.path.exists(APP), "build with __graft_entry__.build() first"
     print("running facade_app", flush=True)
-    res = subprocess.run([APP], capture_output=True, text=True, timeout=120)
+    res = subprocess.run([APP], capture_output=True, text=True, timeout=120, cwd=str(tmp_path))
     print(res.stdout, res.stderr)
     assert res.returncode == 0
     frames = re.findall(r"frame (\d+) ok (\d) x (\S+) gt (\S+) hit_dist (\S+) depth_dist (\S+) scale (-?\d+)",
@@ -29,6 +29,11 @@ def test_facade_app_tracks_and_maps():
         assert sc == "0"
     m = re.search(r"interp (\d) (\S+) blocks (\d+)", res.stdout)
     assert m and m.group(1) == "1" and abs(float(m.group(2))) < 0.2 and int(m.group(3)) > 1000
+    # Map::mesh / saveMesh: the room's surfaces, a .ply written in the working directory
+    m = re.search(r"mesh faces (\d+) worst_plane_dist (\S+) saved (\d) bad_ext_threw (\d)", res.stdout)
+    assert m and int(m.group(1)) > 10000 and float(m.group(2)) < 0.02 and m.group(3) == "0" and m.group(4) == "1"
+    ply = (tmp_path / "facade_app_mesh.ply").read_text().split("\n")
+    assert ply[0] == "ply" and ply[3] == f"element vertex {3 * int(m.group(1))}"
     # lidar: the field at points on the ground plane and on the wall is the zero crossing, to within
     # the interpolation error of a 4 cm grid (mu = 16 cm, so 0.15 is 2.4 cm)
     m = re.search(r"lidar ground (\d) (\S+) wall (\d) (\S+) blocks (\d+)", res.stdout)
