@@ -135,6 +135,33 @@ struct Triangle {
 };
 typedef std::vector<Triangle> TriangleMesh;
 
+namespace io {
+// se::io::has_supported_mesh_extension (include/se/map/io/mesh_io.hpp:24-29)
+inline bool has_supported_mesh_extension(const std::string& filename)
+{
+    return se8_has_supported_mesh_extension(filename.c_str()) != 0;
+}
+
+// se::io::save_mesh (include/se/map/io/mesh_io.hpp:44-46, impl/mesh_io_impl.hpp:14-230): .ply, .vtk
+// or .obj by the extension, each vertex mapped by T_OM; 0 on success, 1 if the file cannot be
+// written, std::invalid_argument for another extension.
+inline int save_mesh(const TriangleMesh& mesh, const std::string& filename,
+                     const Isometry3f& T_OM = Isometry3f::Identity())
+{
+    if (!has_supported_mesh_extension(filename))
+        throw std::invalid_argument("unknown file extension: " + filename);
+    std::vector<float> flat(9 * mesh.size());
+    for (size_t f = 0; f < mesh.size(); f++)
+        for (int v = 0; v < 3; v++)
+            std::memcpy(&flat[9 * f + 3 * v], mesh[f].vertexes[v].data(), sizeof(float) * 3);
+    const int rc = se8_save_mesh(flat.data(), (int64_t) mesh.size(), filename.c_str(), T_OM.data());
+    if (rc == SE8_ERR_IO)
+        return 1;
+    detail::check(rc, "se8_save_mesh");
+    return 0;
+}
+} // namespace io
+
 enum class Res { Single, Multi };
 
 // se::FieldData<Field::TSDF>::Config plus the storage precision and pool capacity
@@ -315,18 +342,9 @@ class TSDFMap {
     int saveMesh(const std::string& filename, const Isometry3f& T_OW = Isometry3f::Identity(),
                  const int min_desired_scale = 0) const
     {
-        if (!se8_has_supported_mesh_extension(filename.c_str()))
+        if (!io::has_supported_mesh_extension(filename))
             throw std::invalid_argument("unknown file extension: " + filename);
-        const SurfaceMesh m = mesh(T_OW, min_desired_scale);
-        std::vector<float> flat(9 * m.size());
-        for (size_t f = 0; f < m.size(); f++)
-            for (int v = 0; v < 3; v++)
-                std::memcpy(&flat[9 * f + 3 * v], m[f].vertexes[v].data(), sizeof(float) * 3);
-        const int rc = se8_save_mesh(flat.data(), (int64_t) m.size(), filename.c_str(), nullptr);
-        if (rc == SE8_ERR_IO)
-            return 1;
-        detail::check(rc, "se8_save_mesh");
-        return 0;
+        return io::save_mesh(mesh(T_OW, min_desired_scale), filename);
     }
 
     se8_map* handle() const { return h_; }

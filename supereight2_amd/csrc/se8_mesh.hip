@@ -70,12 +70,14 @@ __device__ __forceinline__ void edge_vertex(const float* s_tsdf, int g, const in
     }
 }
 
-// checkVertex (marching_cube_impl.hpp:814-818): true if the vertex lies outside (0, dim]^3.  A NaN
-// coordinate compares false everywhere and keeps the face, as in the reference.
-__device__ __forceinline__ bool vertex_outside(const float p[3], float dim)
-{
-    return p[0] <= 0.f || p[1] <= 0.f || p[2] <= 0.f || p[0] > dim || p[1] > dim || p[2] > dim;
-}
+// checkVertex (marching_cube_impl.hpp:814-818) drops a face with a vertex outside (0, size]^3.  In
+// the single-resolution mesher it never fires, so every triangle of a cell's case is kept: the
+// two corners of a vertex's edge differ in inside-ness (tsdf < 0), so t = (0 - v0) / (v1 - v0) is
+// in [0, 1] (|v1 - v0| >= |v0| holds after rounding, and a division or sum that rounds monotonically
+// stays in that range; an overflow of v1 - v0 gives t = 0) or is NaN (a NaN or infinite corner);
+// with p1 - p0 = +-1 exactly, each coordinate lies between the two voxel centres c + 0.5 and
+// c + 1.5 (c <= size - 2), or is NaN, which checkVertex keeps.  The oracle applies checkVertex
+// literally; the parity tests cover NaN and infinite voxels.
 
 // Exclusive prefix of v over the workgroup and the workgroup total (64-wide waves, 32-bit halves
 // shuffled separately).
@@ -131,6 +133,7 @@ __global__ __launch_bounds__(256) void k_mesh_neighbours(MapView m, int n, int* 
 struct MeshLds {
     float tsdf[kGrid];
     uint8_t valid[kGrid];
+    uint8_t cell_case[512]; // case index of each cell (0 for no face)
     uint64_t tri[256];
     uint64_t wave[kMeshThreads / 64];
     int nb[8];
@@ -189,11 +192,9 @@ __device__ __forceinline__ void stage_block(const MapView& m, int slot, const in
     __syncthreads();
 }
 
-// The two cells of this thread (cell index t and t + 256 of the block): case words and the mask
-// of faces kept by checkVertex (bit j = triangle j).
+// The two cells of this thread (cell index t and t + 256 of the block): their case words.
 struct ThreadCells {
     uint64_t word[2];
-    uint32_t keep[2];
 };
 
 __device__ __forceinline__ void cell_geometry(int ci, const int bc[3], int lc[3], int c[3], int& g)
@@ -203,9 +204,11 @@ __device__ __forceinline__ void cell_geometry(int ci, const int bc[3], int lc[3]
     g = lc[0] + 9 * lc[1] + 81 * lc[2];
 }
 
-__device__ __forceinline__ uint32_t classify_cells(const MapView& m, const MeshLds& L, const int bc[3], ThreadCells& tc)
+// Case words of the thread's cells (isosurface::occupied: 0 if any corner is unknown; cells at
+// x, y or z = size - 1 are outside the reference's loop range) and their number of faces; the case
+// indices are also kept in L.cell_case.
+__device__ __forceinline__ uint32_t classify_cells(const MapView& m, MeshLds& L, const int bc[3], ThreadCells& tc)
 {
-    const float dim = (float) m.size;
     uint32_t n_faces = 0;
 #pragma unroll
     for (int h = 0; h < 2; h++) {
@@ -220,22 +223,10 @@ __device__ __forceinline__ uint32_t classify_cells(const MapView& m, const MeshL
             idx |= (L.tsdf[gk] < 0.f ? 1u : 0u) << k;
         }
         const bool in_range = c[0] < m.size - 1 && c[1] < m.size - 1 && c[2] < m.size - 1;
-        const uint64_t word = L.tri[(all_valid && in_range) ? idx : 0u];
-        uint32_t keep = 0;
-        const int ntri = (int) (word >> 60);
-        for (int j = 0; j < ntri; j++) {
-            bool out = false;
-#pragma unroll
-            for (int v = 0; v < 3; v++) {
-                float p[3];
-                edge_vertex(L.tsdf, g, c, (int) ((word >> (4 * (3 * j + v))) & 0xF), p);
-                out = out || vertex_outside(p, dim);
-            }
-            keep |= (out ? 0u : 1u) << j;
-        }
-        tc.word[h] = word;
-        tc.keep[h] = keep;
-        n_faces += __builtin_popcount(keep);
+        const uint32_t cs = (all_valid && in_range) ? idx : 0u;
+        L.cell_case[threadIdx.x + h * kMeshThreads] = (uint8_t) cs;
+        tc.word[h] = L.tri[cs];
+        n_faces += (uint32_t) (tc.word[h] >> 60);
     }
     return n_faces;
 }
@@ -328,15 +319,18 @@ __global__ __launch_bounds__(256) void k_mesh_compact(const uint64_t* scan, cons
         list[here & kPackMask] = slot;
 }
 
-constexpr int kStageFaces = 256; // faces staged in LDS per write round
+constexpr int kStageFaces = kMeshThreads; // faces per write round: one per thread
+constexpr int kMaxBlockFaces = 512 * 5;
 
-// Emit pass over the non-empty blocks: faces written at the block's offset, in the fixed order,
-// through an LDS stage so that the global stores are contiguous across the workgroup.
+// Emit pass over the non-empty blocks.  The block's faces are listed in LDS in the fixed order
+// (cell index << 3 | triangle), then computed one face per thread in rounds of kStageFaces, each
+// round staged in LDS and stored contiguously at the block's offset.
 __global__ __launch_bounds__(kMeshThreads) void k_mesh_emit(MapView m, Iso T_OV, const int* nb_tab,
                                                              const uint64_t* scan, const uint64_t* tile_sums,
                                                              const int* list, float* faces)
 {
     __shared__ MeshLds L;
+    __shared__ uint16_t s_face[kMaxBlockFaces];
     __shared__ float s_out[kStageFaces * 9];
     const int slot = list[blockIdx.x];
     int bc[3];
@@ -345,35 +339,38 @@ __global__ __launch_bounds__(kMeshThreads) void k_mesh_emit(MapView m, Iso T_OV,
     stage_block(m, slot, nb_tab, L);
     ThreadCells tc;
     const uint32_t n_faces = classify_cells(m, L, bc, tc);
-    uint64_t total;
-    const uint32_t rank = (uint32_t) block_exclusive_scan(n_faces, L.wave, total);
+    uint64_t total64;
+    uint32_t r = (uint32_t) block_exclusive_scan(n_faces, L.wave, total64);
+    const uint32_t total = (uint32_t) total64;
+#pragma unroll
+    for (int h = 0; h < 2; h++) {
+        const int ci = threadIdx.x + h * kMeshThreads;
+        const int ntri = (int) (tc.word[h] >> 60);
+        for (int j = 0; j < ntri; j++)
+            s_face[r++] = (uint16_t) ((ci << 3) | j);
+    }
+    __syncthreads();
     float* dst = faces + first * 9;
-    for (uint32_t round = 0; round < (uint32_t) total; round += kStageFaces) {
-        uint32_t r = rank;
-#pragma unroll
-        for (int h = 0; h < 2; h++) {
+    for (uint32_t round = 0; round < total; round += kStageFaces) {
+        const uint32_t f = round + threadIdx.x;
+        if (f < total) {
+            const int ci = s_face[f] >> 3, j = s_face[f] & 7;
             int lc[3], c[3], g;
-            cell_geometry(threadIdx.x + h * kMeshThreads, bc, lc, c, g);
-            for (int j = 0; j < 5; j++) {
-                if (!((tc.keep[h] >> j) & 1))
-                    continue;
-                if (r >= round && r < round + kStageFaces) {
-                    float* f = s_out + (r - round) * 9;
+            cell_geometry(ci, bc, lc, c, g);
+            const uint64_t word = L.tri[L.cell_case[ci]];
+            float* o9 = s_out + threadIdx.x * 9;
 #pragma unroll
-                    for (int v = 0; v < 3; v++) {
-                        float p[3], o[3];
-                        edge_vertex(L.tsdf, g, c, (int) ((tc.word[h] >> (4 * (3 * j + v))) & 0xF), p);
-                        iso_apply(T_OV, p, o);
-                        f[3 * v] = o[0];
-                        f[3 * v + 1] = o[1];
-                        f[3 * v + 2] = o[2];
-                    }
-                }
-                r++;
+            for (int v = 0; v < 3; v++) {
+                float p[3], o[3];
+                edge_vertex(L.tsdf, g, c, (int) ((word >> (4 * (3 * j + v))) & 0xF), p);
+                iso_apply(T_OV, p, o);
+                o9[3 * v] = o[0];
+                o9[3 * v + 1] = o[1];
+                o9[3 * v + 2] = o[2];
             }
         }
         __syncthreads();
-        const int nf = (int) min((uint64_t) kStageFaces, total - round);
+        const int nf = (int) min((uint32_t) kStageFaces, total - round);
         for (int i = threadIdx.x; i < nf * 9; i += kMeshThreads)
             dst[(size_t) round * 9 + i] = s_out[i];
         __syncthreads();

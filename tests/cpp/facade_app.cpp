@@ -106,7 +106,7 @@ int main()
                 d = std::fmin(d, std::fmin(std::fabs(v(i) - lo[i]), std::fabs(v(i) - hi[i])));
             mesh_worst = std::fmax(mesh_worst, d);
         }
-    const int saved = map.saveMesh("facade_app_mesh.ply");
+    const int saved = map.saveMesh("facade_app_mesh.ply") + se::io::save_mesh(mesh, "facade_app_mesh.obj");
     bool threw = false;
     try {
         map.saveMesh("facade_app_mesh.png");

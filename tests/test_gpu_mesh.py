@@ -3,6 +3,7 @@
 Faces are compared as a set (the reference's face order follows its OpenMP schedule, the GPU's
 the block pool): every face's nine floats bit-exact, NaN vertices (from NaN tsdf samples, kept by
 checkVertex as in the reference) compared as NaN.  Cases: the reference meshing unit test setup;
+blocks of special values (+-inf, NaN, +-3e38, signed zeros, denormals) and unknown voxels;
 maps integrated from the seeded tracking frames with FP32 storage (oracle integrated alongside) and
 FP16 storage (oracle loaded with the GPU map's blocks); a non-identity T_OW; corrupted depth (NaN
 voxels); an empty map; a face buffer too small (SE8_ERR_CAPACITY with the count); saveMesh of the
@@ -61,6 +62,34 @@ def test_mesh_reference_unittest_setup():
     g = gmap.mesh().cpu().numpy()
     assert g.shape == (450, 3, 3)
     _assert_same_faces(g, omap.mesh())
+    gmap.close()
+
+
+def test_mesh_special_values_match_oracle():
+    """Blocks filled with special tsdf values (+-inf, NaN, +-3e38, +-0, +-1e-45, ordinary values)
+    and zero weights, at the map's upper corner too: the GPU keeps every triangle of a cell's case,
+    the oracle applies checkVertex literally, and the two meshes are equal."""
+    from supereight2_amd import se
+
+    rng = np.random.default_rng(11)
+    special = np.array([np.inf, -np.inf, np.nan, 3e38, -3e38, 0.0, -0.0, 1e-45, -1e-45, 0.5, -0.5, 1.0, -1.0],
+                       np.float32)
+    gmap, omap = se.TSDFMap(0.32, 0.01), O.OracleMap(0.32, 0.01)  # 32 voxels: blocks 0, 8, 16, 24
+    bcs = np.array([[x, y, z] for x in (8, 16, 24) for y in (16, 24) for z in (0, 24)], np.int32)
+    gmap.allocate_blocks(bcs)
+    omap.allocate_blocks(bcs)
+    for bc in bcs:
+        t = rng.choice(special, 512).astype(np.float32)
+        plain = rng.random(512) < 0.5
+        t[plain] = rng.uniform(-1, 1, int(plain.sum())).astype(np.float32)
+        w = (rng.random(512) < 0.9).astype(np.float32)
+        gmap.write_block(bc, t, w)
+        assert omap.set_block_data(bc, t, w) == 0
+    g = gmap.mesh().cpu().numpy()
+    o = omap.mesh()
+    print("special-value faces", len(g), "with NaN", int(np.isnan(g).any(axis=(1, 2)).sum()), flush=True)
+    assert len(g) > 1000 and np.isnan(g).any()
+    _assert_same_faces(g, o)
     gmap.close()
 
 
