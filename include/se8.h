@@ -40,6 +40,7 @@ typedef void* se8_stream; /* a hipStream_t */
 typedef struct se8_map se8_map;
 typedef struct se8_tracker se8_tracker;
 typedef struct se8_lidar se8_lidar;
+typedef struct se8_reader se8_reader;
 
 /* se::Map::Config (include/se/map/map.hpp:42-71).  T_MW is a pure translation (the reference's
  * raycaster assumes no rotation, include/se/map/impl/raycaster_impl.hpp:465); the default
@@ -236,6 +237,51 @@ int se8_mesh(const se8_map* map, const float T_OW[16], float* faces_dev, int64_t
 int se8_save_mesh(const float* faces, int64_t n, const char* filename, const float T_OM[16]);
 /* se::io::has_supported_mesh_extension (include/se/map/io/mesh_io.hpp:24-29): 1 for .ply/.vtk/.obj */
 int se8_has_supported_mesh_extension(const char* filename);
+
+/* -------- input side of the frame loop (app/src/main.cpp:104-120) -------- */
+/* se::preprocessor::downsample_depth (src/map/preprocessor.cpp:19-84): each output pixel is the
+ * median of its ratio x ratio input region's depths >= 1e-5 (region[n / 2] of the ascending order),
+ * 0 where there are none; the input must be the same integer multiple of the output in both
+ * directions.  map_dev (NULL, or w_out*h_out uint64) receives the input pixel index x + y*w_in the
+ * median came from (the region's first pixel where none is valid; among equal depths the first in
+ * the region's row-major order).  Device buffers, asynchronous on the stream. */
+int se8_downsample_depth(const float* in_dev, int w_in, int h_in, float* out_dev, int w_out, int h_out,
+                         uint64_t* map_dev, se8_stream stream);
+/* The same over depth as a reader stores it: raw uint16 converted to metres first as the readers
+ * do, raw / scale (divide = 1: RAW millimetres, scale 1000, reader_raw.cpp:116-121) or raw * scale
+ * (divide = 0: TUM inverse scale, reader_tum.cpp nextDepth).  w_out = w_in only converts. */
+int se8_downsample_depth_u16(const uint16_t* in_dev, int w_in, int h_in, float scale, int divide, float* out_dev,
+                             int w_out, int h_out, uint64_t* map_dev, se8_stream stream);
+
+/* Dataset readers (app/include/reader_base.hpp:24-62, reader_raw.hpp, reader_tum.hpp; host code):
+ * the RAW format (one file: per frame [u32 w][u32 h][w*h u16 depth, mm][u32 w][u32 h][w*h RGB8]) and
+ * the TUM RGB-D directory (depth.txt, rgb.txt, depth/ 16-bit PNGs; colour is not read).  Ground
+ * truth: TUM groundtruth.txt interpolated at the depth timestamps (TUM), or one line per frame
+ * whose last 7 columns are tx ty tz qx qy qz qw (RAW). */
+#define SE8_READER_RAW 0
+#define SE8_READER_TUM 1
+#define SE8_READER_OK 0    /* se::ReaderStatus::ok */
+#define SE8_READER_SKIP 1  /* ::skip (invalid pose of this frame) */
+#define SE8_READER_EOF 2   /* ::eof */
+#define SE8_READER_ERROR 3 /* ::error */
+typedef struct {
+    int type;                      /* SE8_READER_RAW or SE8_READER_TUM */
+    const char* sequence_path;     /* the .raw file, or the TUM directory */
+    const char* ground_truth_file; /* NULL or "" for none */
+    float inverse_scale;           /* TUM metres per stored unit; 0 = 1/5000 (reader_tum.hpp:46) */
+} se8_reader_config;
+/* SE8_ERR_IO (with the reason in se8_last_error) where the reference's reader would start in the
+ * error status. */
+int se8_reader_create(const se8_reader_config* cfg, se8_reader** out);
+int se8_reader_destroy(se8_reader* reader);
+int se8_reader_info(const se8_reader* reader, int* width, int* height, int64_t* num_frames, int* has_ground_truth);
+/* Reader::nextData (reader_base.cpp:536-572): the next frame's depth as stored (width*height uint16,
+ * host) with the conversion to metres (*scale, *divide as in se8_downsample_depth_u16) and, if T_WB
+ * is not NULL, its ground-truth pose; *status is the merged reader status. */
+int se8_reader_next_raw(se8_reader* reader, uint16_t* depth, float* scale, int* divide, float T_WB[16], int* status);
+/* The same with the depth converted to metres on the host, as the reference returns it. */
+int se8_reader_next(se8_reader* reader, float* depth, float T_WB[16], int* status);
+int se8_reader_restart(se8_reader* reader);
 
 int se8_stage_times_get(const se8_map* map, const se8_tracker* tracker, se8_stage_times* out);
 int se8_stage_times_reset(se8_map* map, se8_tracker* tracker);

@@ -115,6 +115,10 @@ def lib(precision: str = "f32") -> C.CDLL:
     L.or_set_block_data.argtypes = [P, I, R, R]
     L.or_mesh.restype = C.c_long
     L.or_mesh.argtypes = [P, R, R, C.c_long]
+    L.or_depth_from_u16.argtypes = [C.POINTER(C.c_uint16), C.c_long, C.c_float, C.c_int, C.POINTER(C.c_float)]
+    L.or_downsample_depth.restype = C.c_int
+    L.or_downsample_depth.argtypes = [C.POINTER(C.c_float), C.c_int, C.c_int, C.POINTER(C.c_float), C.c_int,
+                                      C.c_int, C.POINTER(U64)]
     L.or_get_field_interp.restype = C.c_int
     L.or_get_field_interp.argtypes = [P, R, R]
     L.or_get_field_grad.restype = C.c_int
@@ -414,6 +418,29 @@ def track(depth, pin, T_WS, ref_pts, ref_nrm, cfg=None, precision="f32"):
     ok = L.or_track(d.ctypes.data_as(C.POINTER(C.c_float)), C.byref(pin), C.byref(cfg), _rp(T, precision),
                     _rp(rp, precision), _rp(rn, precision), _rp(sums, precision), C.byref(iters))
     return bool(ok), T.reshape(4, 4), sums, int(iters.value)
+
+
+def depth_from_u16(raw, scale, divide):
+    """Readers' conversion of stored uint16 depth: raw / scale (RAW, millimetres, scale 1000) or
+    raw * scale (TUM inverse scale), in float32."""
+    r = np.ascontiguousarray(raw, dtype=np.uint16)
+    out = np.zeros(r.shape, np.float32)
+    lib().or_depth_from_u16(r.ctypes.data_as(C.POINTER(C.c_uint16)), r.size, float(scale), int(divide),
+                            out.ctypes.data_as(C.POINTER(C.c_float)))
+    return out
+
+
+def downsample_depth(depth, w_out, h_out):
+    """preprocessor::downsample_depth: (output depth [h_out, w_out], index map [h_out, w_out] uint64)."""
+    d = np.ascontiguousarray(depth, dtype=np.float32)
+    h_in, w_in = d.shape
+    out = np.zeros((h_out, w_out), np.float32)
+    mp = np.zeros((h_out, w_out), np.uint64)
+    rc = lib().or_downsample_depth(d.ctypes.data_as(C.POINTER(C.c_float)), w_in, h_in,
+                                   out.ctypes.data_as(C.POINTER(C.c_float)), w_out, h_out, _u64p(mp))
+    if rc != 0:
+        raise ValueError("downsample_depth: the input must be the same integer multiple of the output")
+    return out, mp
 
 
 def half_sample(depth, e_d=0.1 * 3):

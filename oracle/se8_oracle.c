@@ -2471,3 +2471,58 @@ OR_API long or_mesh(void* vm, const real T_OW16[16], real* faces, long cap)
     free(count);
     return total;
 }
+
+/* ------------------------------------------------------------------------- */
+/* Input side: preprocessor::downsample_depth (src/map/preprocessor.cpp:19-84) */
+/* and the readers' uint16 -> metres conversion (app/src/reader_raw.cpp:116-121, */
+/* app/src/reader_tum.cpp nextDepth).  The region's valid depths are sorted   */
+/* stably (equal depths keep the region's row-major order; std::sort leaves it */
+/* unspecified) and region[n / 2] is taken.                                   */
+/* ------------------------------------------------------------------------- */
+OR_API void or_depth_from_u16(const uint16_t* raw, long n, float scale, int divide, float* out)
+{
+    for (long i = 0; i < n; i++)
+        out[i] = divide ? (float) raw[i] / scale : (float) raw[i] * scale;
+}
+
+OR_API int or_downsample_depth(const float* in, int w_in, int h_in, float* out, int w_out, int h_out, uint64_t* map)
+{
+    if (w_out <= 0 || h_out <= 0 || w_in % w_out || h_in % h_out || w_in / w_out != h_in / h_out)
+        return -1;
+    const int ratio = w_in / w_out;
+    float* dv = (float*) malloc(sizeof(float) * ratio * ratio);
+    int* ix = (int*) malloc(sizeof(int) * ratio * ratio);
+    for (int y_out = 0; y_out < h_out; y_out++)
+        for (int x_out = 0; x_out < w_out; x_out++) {
+            const int xb = x_out * ratio, yb = y_out * ratio;
+            int n = 0;
+            for (int yr = 0; yr < ratio; yr++)
+                for (int xr = 0; xr < ratio; xr++) {
+                    const float d = in[(size_t) (yb + yr) * w_in + xb + xr];
+                    if (d >= 1e-5f) {
+                        /* insertion keeps equal depths in arrival order */
+                        int k = n++;
+                        while (k > 0 && dv[k - 1] > d) {
+                            dv[k] = dv[k - 1];
+                            ix[k] = ix[k - 1];
+                            k--;
+                        }
+                        dv[k] = d;
+                        ix[k] = (xb + xr) + (yb + yr) * w_in;
+                    }
+                }
+            if (n == 0) {
+                out[(size_t) y_out * w_out + x_out] = 0.0f;
+                if (map)
+                    map[(size_t) y_out * w_out + x_out] = (uint64_t) xb + (uint64_t) yb * (uint64_t) w_in;
+            }
+            else {
+                out[(size_t) y_out * w_out + x_out] = dv[n / 2];
+                if (map)
+                    map[(size_t) y_out * w_out + x_out] = (uint64_t) ix[n / 2];
+            }
+        }
+    free(dv);
+    free(ix);
+    return 0;
+}

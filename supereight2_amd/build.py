@@ -16,7 +16,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 BUILD_DIR = os.path.join(HERE, "_build")
 LIB_PATH = os.path.join(HERE, "libse8.so")
-SOURCES = ["se8_map.hip", "se8_raycast.hip", "se8_track.hip", "se8_mesh.hip"]
+SOURCES = ["se8_map.hip", "se8_raycast.hip", "se8_track.hip", "se8_mesh.hip", "se8_preprocess.hip", "se8_readers.cpp"]
 ARCH = os.environ.get("SE8_OFFLOAD_ARCH", "gfx950")
 
 HIP_FLAGS = [
@@ -60,7 +60,7 @@ def build(verbose: bool = False) -> str:
     with ThreadPoolExecutor(max_workers=len(SOURCES)) as ex:
         objs = list(ex.map(lambda s: _compile(s, verbose), SOURCES))
     tmp = LIB_PATH + ".tmp"
-    cmd = [hipcc(), f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, *objs]
+    cmd = [hipcc(), f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, *objs, "-lz"]
     if verbose:
         print(" ".join(cmd), flush=True)
     res = subprocess.run(cmd, capture_output=True, text=True)

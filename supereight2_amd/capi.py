@@ -116,6 +116,18 @@ class StageTimes(C.Structure):
     ]
 
 
+class ReaderConfig(C.Structure):
+    _fields_ = [
+        ("type", C.c_int),
+        ("sequence_path", C.c_char_p),
+        ("ground_truth_file", C.c_char_p),
+        ("inverse_scale", C.c_float),
+    ]
+
+
+READER_RAW = 0
+READER_TUM = 1
+
 _P = C.c_void_p
 _F = C.POINTER(C.c_float)
 _I = C.POINTER(C.c_int)
@@ -153,6 +165,14 @@ SIGNATURES = {
     "se8_mesh": (C.c_int, [_P, _F, _P, C.c_int64, C.POINTER(C.c_int64), _P]),
     "se8_save_mesh": (C.c_int, [_F, C.c_int64, C.c_char_p, _F]),
     "se8_has_supported_mesh_extension": (C.c_int, [C.c_char_p]),
+    "se8_downsample_depth": (C.c_int, [_P, C.c_int, C.c_int, _P, C.c_int, C.c_int, _P, _P]),
+    "se8_downsample_depth_u16": (C.c_int, [_P, C.c_int, C.c_int, C.c_float, C.c_int, _P, C.c_int, C.c_int, _P, _P]),
+    "se8_reader_create": (C.c_int, [C.POINTER(ReaderConfig), C.POINTER(_P)]),
+    "se8_reader_destroy": (C.c_int, [_P]),
+    "se8_reader_info": (C.c_int, [_P, _I, _I, C.POINTER(C.c_int64), _I]),
+    "se8_reader_next_raw": (C.c_int, [_P, C.POINTER(C.c_uint16), _F, _I, _F, _I]),
+    "se8_reader_next": (C.c_int, [_P, _F, _F, _I]),
+    "se8_reader_restart": (C.c_int, [_P]),
     "se8_stage_times_get": (C.c_int, [_P, _P, C.POINTER(StageTimes)]),
     "se8_stage_times_reset": (C.c_int, [_P, _P]),
     "se8_device_alloc": (C.c_int, [C.c_size_t, C.POINTER(_P)]),

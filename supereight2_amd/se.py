@@ -293,6 +293,91 @@ def save_mesh(faces, filename, T_OM=None):
     return 0
 
 
+def downsample_depth(input_depth, output_depth, downsample_map=None, stream=None):
+    """se::preprocessor::downsample_depth (preprocessor.cpp:19-84) on device tensors: input [H, W]
+    float32, output [H/r, W/r] float32; downsample_map (optional, [H/r, W/r] int64/uint64 CUDA
+    tensor) receives the index map the reference returns."""
+    h_in, w_in = input_depth.shape
+    h_out, w_out = output_depth.shape
+    mp = None if downsample_map is None else C.c_void_p(downsample_map.data_ptr())
+    capi.check(capi.load().se8_downsample_depth(_dev_f32(input_depth), w_in, h_in, _dev_f32(output_depth), w_out,
+                                                h_out, mp, _stream_handle(stream)), "se8_downsample_depth")
+    return downsample_map
+
+
+def downsample_depth_u16(raw_depth, scale, divide, output_depth, downsample_map=None, stream=None):
+    """downsample_depth over depth as a reader stores it (uint16 [H, W] CUDA tensor), converted to
+    metres as the reader would (raw / scale if divide, else raw * scale) inside the same kernel."""
+    torch = _torch()
+    if not (raw_depth.is_cuda and raw_depth.dtype in (torch.uint16, torch.int16) and raw_depth.is_contiguous()):
+        raise ValueError("raw_depth must be a contiguous 16-bit CUDA tensor")
+    h_in, w_in = raw_depth.shape
+    h_out, w_out = output_depth.shape
+    mp = None if downsample_map is None else C.c_void_p(downsample_map.data_ptr())
+    capi.check(capi.load().se8_downsample_depth_u16(C.c_void_p(raw_depth.data_ptr()), w_in, h_in, float(scale),
+                                                    int(divide), _dev_f32(output_depth), w_out, h_out, mp,
+                                                    _stream_handle(stream)), "se8_downsample_depth_u16")
+    return downsample_map
+
+
+class ReaderStatus:
+    """se::ReaderStatus (app/include/reader_base.hpp:45-62)."""
+
+    ok, skip, eof, error = 0, 1, 2, 3
+
+
+class Reader:
+    """se::RAWReader / se::TUMReader (app/include/reader_raw.hpp, reader_tum.hpp) over se8_reader_*:
+    reader_type "raw" or "tum".  Construction raises capi.Se8Error where the reference's reader
+    would start in the error status."""
+
+    def __init__(self, reader_type, sequence_path, ground_truth_file="", inverse_scale=0.0):
+        self.lib = capi.load()
+        t = {"raw": capi.READER_RAW, "tum": capi.READER_TUM}[reader_type]
+        self._cfg = capi.ReaderConfig(t, str(sequence_path).encode(), str(ground_truth_file or "").encode(),
+                                      float(inverse_scale))
+        self.h = C.c_void_p()
+        capi.check(self.lib.se8_reader_create(C.byref(self._cfg), C.byref(self.h)), "se8_reader_create")
+        w, hh, gt = C.c_int(), C.c_int(), C.c_int()
+        n = C.c_int64()
+        capi.check(self.lib.se8_reader_info(self.h, C.byref(w), C.byref(hh), C.byref(n), C.byref(gt)),
+                   "se8_reader_info")
+        self.width, self.height, self.num_frames, self.has_ground_truth = w.value, hh.value, n.value, bool(gt.value)
+
+    def close(self):
+        if getattr(self, "h", None) and self.h.value:
+            self.lib.se8_reader_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def nextData(self, with_pose=True):
+        """Reader::nextData: (status, depth [H, W] float32 metres, T_WB 4x4 or None)."""
+        depth = np.zeros((self.height, self.width), np.float32)
+        T = np.zeros(16, np.float32)
+        st = C.c_int()
+        capi.check(self.lib.se8_reader_next(self.h, capi.fptr(depth), capi.fptr(T) if with_pose else None,
+                                            C.byref(st)), "se8_reader_next")
+        return st.value, depth, (T.reshape(4, 4) if with_pose and st.value in (0, 1) else None)
+
+    def nextDataRaw(self, with_pose=True):
+        """The next frame as stored: (status, depth [H, W] uint16, scale, divide, T_WB or None)."""
+        depth = np.zeros((self.height, self.width), np.uint16)
+        T = np.zeros(16, np.float32)
+        scale, divide, st = C.c_float(), C.c_int(), C.c_int()
+        capi.check(self.lib.se8_reader_next_raw(self.h, depth.ctypes.data_as(C.POINTER(C.c_uint16)), C.byref(scale),
+                                                C.byref(divide), capi.fptr(T) if with_pose else None, C.byref(st)),
+                   "se8_reader_next_raw")
+        return st.value, depth, scale.value, divide.value, (T.reshape(4, 4) if with_pose and st.value in (0, 1) else None)
+
+    def restart(self):
+        capi.check(self.lib.se8_reader_restart(self.h), "se8_reader_restart")
+
+
 class MapIntegrator:
     """se::MapIntegrator<MapT> (map_integrator.hpp:66-81)."""
 
