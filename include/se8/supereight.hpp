@@ -11,6 +11,8 @@
 //   se::OusterLidar                        include/se/sensor/ouster_lidar.hpp:12-80
 //   se::Map::mesh / saveMesh, se::Triangle include/se/map/map.hpp:396-413,
 //                                          include/se/map/algorithms/mesh.hpp:48-103
+//   se::preprocessor::downsample_depth     src/map/preprocessor.cpp:19-84
+//   se::Reader / create_reader (app)       app/include/reader_base.hpp:45-120, reader.hpp:22
 // Images live in host memory as in the reference; each call stages them through device buffers
 // owned by the object (or a per-thread scratch for raycast_volume).  Value types below provide the
 // subset of Eigen's interface the call sites use (x(), y(), z(), translation(), matrix data).
@@ -355,6 +357,103 @@ class TSDFMap {
     Vector3f dim_;
     mutable detail::DeviceBuffer mesh_buf_;
 };
+
+namespace preprocessor {
+// downsample_depth (preprocessor.cpp:19-84): median of each ratio x ratio region of valid depths
+// (>= 1e-5) on the device; returns the index map (input pixel of each output pixel).
+inline Image<size_t> downsample_depth(const Image<float>& input_depth_img, Image<float>& output_depth_img)
+{
+    const size_t n_in = input_depth_img.size(), n_out = output_depth_img.size();
+    detail::DeviceBuffer din, dout, dmap;
+    void* pin = din.get(sizeof(float) * n_in);
+    void* pout = dout.get(sizeof(float) * n_out);
+    void* pmap = dmap.get(sizeof(uint64_t) * n_out);
+    detail::check(se8_copy(pin, input_depth_img.data(), sizeof(float) * n_in, SE8_COPY_H2D, nullptr), "se8_copy");
+    detail::check(se8_downsample_depth((const float*) pin, input_depth_img.width(), input_depth_img.height(),
+                                       (float*) pout, output_depth_img.width(), output_depth_img.height(),
+                                       (uint64_t*) pmap, nullptr),
+                  "se8_downsample_depth");
+    Image<size_t> map(output_depth_img.width(), output_depth_img.height());
+    static_assert(sizeof(size_t) == sizeof(uint64_t), "size_t must be 64-bit");
+    detail::check(se8_copy(output_depth_img.data(), pout, sizeof(float) * n_out, SE8_COPY_D2H, nullptr), "se8_copy");
+    detail::check(se8_copy(map.data(), pmap, sizeof(uint64_t) * n_out, SE8_COPY_D2H, nullptr), "se8_copy");
+    detail::check(se8_stream_synchronize(nullptr), "se8_stream_synchronize");
+    return map;
+}
+} // namespace preprocessor
+
+// Dataset readers (app/include/reader_base.hpp, reader_raw.hpp, reader_tum.hpp) over the library's
+// host readers: depth and ground-truth poses (colour is not read).
+enum class ReaderType { RAW = SE8_READER_RAW, TUM = SE8_READER_TUM };
+enum class ReaderStatus : int { ok = SE8_READER_OK, skip = SE8_READER_SKIP, eof = SE8_READER_EOF, error = SE8_READER_ERROR };
+
+class Reader {
+    public:
+    struct Config {
+        ReaderType reader_type = ReaderType::RAW;
+        std::string sequence_path;
+        std::string ground_truth_file;
+        float inverse_scale = 0.0f;
+    };
+
+    explicit Reader(const Config& c) : cfg_(c)
+    {
+        se8_reader_config rc{(int) c.reader_type, cfg_.sequence_path.c_str(), cfg_.ground_truth_file.c_str(),
+                             c.inverse_scale};
+        if (se8_reader_create(&rc, &h_) != SE8_OK) {
+            h_ = nullptr;
+            status_ = ReaderStatus::error; // as the reference's reader starts on an unreadable dataset
+            return;
+        }
+        int64_t n = 0;
+        detail::check(se8_reader_info(h_, &w_, &hgt_, &n, nullptr), "se8_reader_info");
+        num_frames_ = (size_t) n;
+    }
+    Reader(const Reader&) = delete;
+    Reader& operator=(const Reader&) = delete;
+    ~Reader() { se8_reader_destroy(h_); }
+
+    bool good() const { return status_ == ReaderStatus::ok || status_ == ReaderStatus::skip; }
+    ReaderStatus status() const { return status_; }
+    size_t numFrames() const { return num_frames_; }
+    int depthWidth() const { return w_; }
+    int depthHeight() const { return hgt_; }
+
+    ReaderStatus nextData(Image<float>& depth_image) { return next(depth_image, nullptr); }
+    ReaderStatus nextData(Image<float>& depth_image, Isometry3f& T_WB) { return next(depth_image, &T_WB); }
+    void restart()
+    {
+        if (h_) {
+            detail::check(se8_reader_restart(h_), "se8_reader_restart");
+            status_ = ReaderStatus::ok;
+        }
+    }
+
+    private:
+    ReaderStatus next(Image<float>& depth_image, Isometry3f* T)
+    {
+        if (!h_)
+            return status_;
+        if (depth_image.width() != w_ || depth_image.height() != hgt_)
+            depth_image = Image<float>(w_, hgt_);
+        int st = SE8_READER_ERROR;
+        detail::check(se8_reader_next(h_, depth_image.data(), T ? T->data() : nullptr, &st), "se8_reader_next");
+        status_ = (ReaderStatus) st;
+        return status_;
+    }
+
+    Config cfg_;
+    se8_reader* h_ = nullptr;
+    int w_ = 0, hgt_ = 0;
+    size_t num_frames_ = 0;
+    ReaderStatus status_ = ReaderStatus::ok;
+};
+
+// create_reader (app/include/reader.hpp:22)
+inline Reader* create_reader(const Reader::Config& config)
+{
+    return new Reader(config);
+}
 
 template<typename SensorT>
 struct Measurement {

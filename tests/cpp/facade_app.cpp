@@ -5,6 +5,7 @@
 // Prints one line per frame: frame, tracking ok, estimated x, ground-truth x, centre hit depth;
 // then integrates two scans of a spinning lidar through se::OusterLidar and prints the field at a
 // ground point and a wall point.
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 
@@ -25,8 +26,52 @@ V3 unit(V3 a)
 }
 } // namespace
 
-int main()
+// The reference app's input steps (main.cpp:104-120) on a TUM sequence: read, downsample by 2,
+// integrate at the ground-truth pose; prints the frames read and the valid fraction.
+static int run_tum(const std::string& seq)
 {
+    se::Reader::Config rc;
+    rc.reader_type = se::ReaderType::TUM;
+    rc.sequence_path = seq;
+    rc.ground_truth_file = seq + "/groundtruth.txt";
+    se::Reader* reader = se::create_reader(rc);
+    se::PinholeCamera::Config cfg;
+    cfg.width = 640;
+    cfg.height = 480;
+    cfg.fx = 481.2f;
+    cfg.fy = -480.f;
+    cfg.cx = 319.5f;
+    cfg.cy = 239.5f;
+    cfg.near_plane = 0.4f;
+    cfg.far_plane = 6.f;
+    const se::PinholeCamera sensor(cfg, 2.f);
+    se::TSDFMap<se::Res::Single> map(se::Vector3f::Constant(10.24f), 0.01f);
+    se::MapIntegrator integrator(map);
+    se::Image<float> input(640, 480), processed(320, 240);
+    se::Isometry3f T_WB;
+    int frames = 0;
+    double valid = 0.0;
+    while (reader->nextData(input, T_WB) == se::ReaderStatus::ok) {
+        const se::Image<size_t> ds_map = se::preprocessor::downsample_depth(input, processed);
+        integrator.integrateDepth(frames, se::Measurements{se::Measurement{processed, sensor, T_WB}});
+        size_t n = 0;
+        for (size_t i = 0; i < processed.size(); i++)
+            n += processed[i] > 0.f;
+        valid += (double) n / processed.size();
+        (void) ds_map;
+        frames++;
+    }
+    const auto mesh = map.mesh();
+    std::printf("tum frames %d status %d valid %.4f blocks %lld faces %zu\n", frames, (int) reader->status(),
+                valid / std::max(frames, 1), (long long) map.info().num_blocks, mesh.size());
+    delete reader;
+    return frames == 3 ? 0 : 1;
+}
+
+int main(int argc, char** argv)
+{
+    if (argc > 1)
+        return run_tum(argv[1]);
     se::PinholeCamera::Config cfg;
     cfg.width = 640;
     cfg.height = 480;

@@ -39,3 +39,16 @@ def test_facade_app_tracks_and_maps(tmp_path):
     m = re.search(r"lidar ground (\d) (\S+) wall (\d) (\S+) blocks (\d+)", res.stdout)
     assert m and m.group(1) == "1" and m.group(3) == "1" and int(m.group(5)) > 1000
     assert abs(float(m.group(2))) < 0.15 and abs(float(m.group(4))) < 0.15
+
+
+def test_facade_app_reads_tum_sequence():
+    """se::create_reader + se::preprocessor::downsample_depth + integrateDepth + mesh in C++ on the
+    reference's 3-frame ICL-NUIM sequence (tests/golden/tum_icl3)."""
+    seq = os.path.join(ROOT, "tests", "golden", "tum_icl3")
+    print("running facade_app on the TUM sequence", flush=True)
+    res = subprocess.run([APP, seq], capture_output=True, text=True, timeout=120)
+    print(res.stdout, res.stderr)
+    assert res.returncode == 0
+    m = re.search(r"tum frames (\d+) status (\d) valid (\S+) blocks (\d+) faces (\d+)", res.stdout)
+    assert m and m.group(1) == "3" and m.group(2) == "3"  # the status after the last frame: error
+    assert float(m.group(3)) > 0.95 and int(m.group(4)) > 5000 and int(m.group(5)) > 10000
