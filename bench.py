@@ -31,6 +31,7 @@ truncation 4 voxels) and prints its own line in the same format, in scans/s.
 from __future__ import annotations
 
 import argparse
+import ctypes as C
 import json
 import os
 import platform
@@ -141,6 +142,37 @@ def mesh_bench(gmap, torch, se, cpu):
     torch.cuda.synchronize()
     ms = 1000.0 * (time.perf_counter() - t0) / reps
     out = {"faces": int(ref.shape[0]), "ms_per_mesh": ms}
+    # debug views of the same map: one field-slice plane (octree size squared lookups), kernel time
+    # from a replayed HIP graph of 20 launches; Map::structure (leaves) wall time per call
+    size = gmap.octree_size()
+    sl = torch.empty((size, size), dtype=torch.float32, device="cuda")
+    s_cap = torch.cuda.Stream()
+    s_cap.wait_stream(torch.cuda.current_stream())
+    g = torch.cuda.CUDAGraph()
+    with torch.cuda.stream(s_cap):
+        gmap.lib.se8_field_slice(gmap.h, 2, size // 2, C.c_void_p(sl.data_ptr()), C.c_void_p(s_cap.cuda_stream))
+        torch.cuda.synchronize()
+        with torch.cuda.graph(g, stream=s_cap):
+            for _ in range(20):
+                gmap.lib.se8_field_slice(gmap.h, 2, size // 2, C.c_void_p(sl.data_ptr()),
+                                         C.c_void_p(s_cap.cuda_stream))
+    g.replay()
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(5):
+        g.replay()
+    e1.record()
+    torch.cuda.synchronize()
+    out["field_slice_us"] = 1000.0 * e0.elapsed_time(e1) / 100
+    gmap.structure()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(10):
+        q, _ = gmap.structure()
+    torch.cuda.synchronize()
+    out["structure_ms"] = 1000.0 * (time.perf_counter() - t0) / 10
+    out["structure_quads"] = int(q.shape[0])
     if cpu:
         from oracle import oracle as O
 

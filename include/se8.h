@@ -231,12 +231,37 @@ int se8_field_grad(const se8_map* map, const float* points_W, int64_t n, float* 
 int se8_mesh(const se8_map* map, const float T_OW[16], float* faces_dev, int64_t max_faces, int64_t* num_faces,
              se8_stream stream);
 /* se::io::save_mesh (include/se/map/io/mesh_io.hpp:44-46, impl/mesh_io_impl.hpp:14-230): writes n
- * faces (host memory, 9 floats each) as a .ply, .vtk or .obj file chosen by the extension, every
- * vertex mapped by T_OM (NULL = identity).  SE8_ERR_INVALID for another extension (the reference
- * throws std::invalid_argument), SE8_ERR_IO if the file cannot be written. */
-int se8_save_mesh(const float* faces, int64_t n, const char* filename, const float T_OM[16]);
+ * faces of nv = 3 (surface) or 4 (octree structure) vertices (host memory, 3*nv floats per face)
+ * as a .ply, .vtk or .obj file chosen by the extension, every vertex mapped by T_OM (NULL =
+ * identity), each face coloured by its scale (scales: n int8, NULL = all 0; colour_utils.hpp:23-30).
+ * SE8_ERR_INVALID for another extension (the reference throws std::invalid_argument), SE8_ERR_IO if
+ * the file cannot be written. */
+int se8_save_mesh(const float* vertices, int64_t n, int nv, const int8_t* scales, const char* filename,
+                  const float T_OM[16]);
 /* se::io::has_supported_mesh_extension (include/se/map/io/mesh_io.hpp:24-29): 1 for .ply/.vtk/.obj */
 int se8_has_supported_mesh_extension(const char* filename);
+
+/* -------- debug views -------- */
+/* Map::saveFieldSlices (include/se/map/impl/map_impl.hpp:272-308) with io::save_3d_slice_vtk
+ * (include/se/map/io/impl/octree_io_impl.hpp:19-80): the tsdf (get_field of getData; 1 where no
+ * block is allocated) over the octree's three axis-aligned planes through the voxel of point_W, each
+ * as a VTK rectilinear grid; a NULL or empty filename skips that plane.  SE8_ERR_IO if a file cannot
+ * be written (the reference prints an error and goes on). */
+int se8_save_field_slices(const se8_map* map, const char* filename_x, const char* filename_y, const char* filename_z,
+                          const float point_W[3]);
+/* One plane of the above on the device: axis 0/1/2 at voxel coordinate index (outside the octree:
+ * all 1); out_dev holds size*size floats in save_3d_slice_vtk's order (the slower free coordinate
+ * outer).  Asynchronous on the stream. */
+int se8_field_slice(const se8_map* map, int axis, int index, float* out_dev, se8_stream stream);
+/* Map::structure (map_impl.hpp:473-479) -> octree_structure_mesh
+ * (include/se/map/algorithms/impl/structure_meshing_impl.hpp:15-60): six quads per octant (every node
+ * and block, or with only_leaves the octants without children), 12 floats per quad (4 vertices in
+ * frame O, metres; T_OW NULL = identity) into quads_dev and the face scale (blocks 0, nodes 7) into
+ * scales_dev (may be NULL), both device memory for max_quads quads; *num_quads = the count, and
+ * max_quads = 0 only counts.  Too small a buffer: SE8_ERR_CAPACITY, nothing written.  Order: nodes,
+ * then blocks (the reference's follows its octree iterator). */
+int se8_structure(const se8_map* map, const float T_OW[16], int only_leaves, float* quads_dev, int8_t* scales_dev,
+                  int64_t max_quads, int64_t* num_quads, se8_stream stream);
 
 /* -------- input side of the frame loop (app/src/main.cpp:104-120) -------- */
 /* se::preprocessor::downsample_depth (src/map/preprocessor.cpp:19-84): each output pixel is the

@@ -137,6 +137,14 @@ struct Triangle {
 };
 typedef std::vector<Triangle> TriangleMesh;
 
+// se::Quad (mesh.hpp:79-82): the octree structure's faces (scale 0 for blocks, 7 for nodes).
+struct Quad {
+    Vector3f vertexes[4];
+    std::int8_t scale = 0;
+    static constexpr size_t num_vertexes = 4;
+};
+typedef std::vector<Quad> QuadMesh;
+
 namespace io {
 // se::io::has_supported_mesh_extension (include/se/map/io/mesh_io.hpp:24-29)
 inline bool has_supported_mesh_extension(const std::string& filename)
@@ -147,16 +155,21 @@ inline bool has_supported_mesh_extension(const std::string& filename)
 // se::io::save_mesh (include/se/map/io/mesh_io.hpp:44-46, impl/mesh_io_impl.hpp:14-230): .ply, .vtk
 // or .obj by the extension, each vertex mapped by T_OM; 0 on success, 1 if the file cannot be
 // written, std::invalid_argument for another extension.
-inline int save_mesh(const TriangleMesh& mesh, const std::string& filename,
+template<typename FaceT>
+inline int save_mesh(const std::vector<FaceT>& mesh, const std::string& filename,
                      const Isometry3f& T_OM = Isometry3f::Identity())
 {
     if (!has_supported_mesh_extension(filename))
         throw std::invalid_argument("unknown file extension: " + filename);
-    std::vector<float> flat(9 * mesh.size());
+    constexpr int nv = (int) FaceT::num_vertexes;
+    std::vector<float> flat(3 * nv * mesh.size());
     for (size_t f = 0; f < mesh.size(); f++)
-        for (int v = 0; v < 3; v++)
-            std::memcpy(&flat[9 * f + 3 * v], mesh[f].vertexes[v].data(), sizeof(float) * 3);
-    const int rc = se8_save_mesh(flat.data(), (int64_t) mesh.size(), filename.c_str(), T_OM.data());
+        for (int v = 0; v < nv; v++)
+            std::memcpy(&flat[3 * nv * f + 3 * v], mesh[f].vertexes[v].data(), sizeof(float) * 3);
+    std::vector<int8_t> scales(mesh.size());
+    for (size_t f = 0; f < mesh.size(); f++)
+        scales[f] = mesh[f].scale;
+    const int rc = se8_save_mesh(flat.data(), (int64_t) mesh.size(), nv, scales.data(), filename.c_str(), T_OM.data());
     if (rc == SE8_ERR_IO)
         return 1;
     detail::check(rc, "se8_save_mesh");
@@ -347,6 +360,52 @@ class TSDFMap {
         if (!io::has_supported_mesh_extension(filename))
             throw std::invalid_argument("unknown file extension: " + filename);
         return io::save_mesh(mesh(T_OW, min_desired_scale), filename);
+    }
+
+    typedef QuadMesh StructureMesh;
+
+    // structure (map.hpp:415-421, map_impl.hpp:473-479): six quads per octant (only the leaves by
+    // default), vertices mapped by T_OW, metres.
+    StructureMesh structure(const Isometry3f& T_OW = Isometry3f::Identity(), const bool only_leaves = true) const
+    {
+        int64_t n = 0;
+        detail::check(se8_structure(h_, T_OW.data(), only_leaves, nullptr, nullptr, 0, &n, nullptr), "se8_structure");
+        StructureMesh out((size_t) n);
+        if (n == 0)
+            return out;
+        void* dq = mesh_buf_.get(sizeof(float) * 12 * (size_t) n + (size_t) n);
+        int8_t* ds = (int8_t*) ((char*) dq + sizeof(float) * 12 * (size_t) n);
+        detail::check(se8_structure(h_, T_OW.data(), only_leaves, (float*) dq, ds, n, &n, nullptr), "se8_structure");
+        std::vector<float> q(12 * (size_t) n);
+        std::vector<int8_t> sc((size_t) n);
+        detail::check(se8_copy(q.data(), dq, sizeof(float) * q.size(), SE8_COPY_D2H, nullptr), "se8_copy");
+        detail::check(se8_copy(sc.data(), ds, sc.size(), SE8_COPY_D2H, nullptr), "se8_copy");
+        detail::check(se8_stream_synchronize(nullptr), "se8_stream_synchronize");
+        for (size_t f = 0; f < out.size(); f++) {
+            for (int v = 0; v < 4; v++)
+                out[f].vertexes[v] = Vector3f(q[12 * f + 3 * v], q[12 * f + 3 * v + 1], q[12 * f + 3 * v + 2]);
+            out[f].scale = sc[f];
+        }
+        return out;
+    }
+
+    // saveStructure (map.hpp:422-430, map_impl.hpp:484-490)
+    int saveStructure(const std::string& filename, const Isometry3f& T_OW = Isometry3f::Identity(),
+                      const bool only_leaves = true) const
+    {
+        if (!io::has_supported_mesh_extension(filename))
+            throw std::invalid_argument("unknown file extension: " + filename);
+        return io::save_mesh(structure(T_OW, only_leaves), filename);
+    }
+
+    // saveFieldSlices (map.hpp:309-325, map_impl.hpp:272-308): empty names skip a plane.
+    int saveFieldSlices(const std::string& filename_x, const std::string& filename_y, const std::string& filename_z,
+                        const Vector3f& point_W) const
+    {
+        detail::check(se8_save_field_slices(h_, filename_x.c_str(), filename_y.c_str(), filename_z.c_str(),
+                                            point_W.data()),
+                      "se8_save_field_slices");
+        return 0;
     }
 
     se8_map* handle() const { return h_; }

@@ -115,6 +115,9 @@ def lib(precision: str = "f32") -> C.CDLL:
     L.or_set_block_data.argtypes = [P, I, R, R]
     L.or_mesh.restype = C.c_long
     L.or_mesh.argtypes = [P, R, R, C.c_long]
+    L.or_field_slice.argtypes = [P, C.c_int, C.c_int, R]
+    L.or_structure.restype = C.c_long
+    L.or_structure.argtypes = [P, R, C.c_int, R, C.POINTER(C.c_int8), C.c_long]
     L.or_depth_from_u16.argtypes = [C.POINTER(C.c_uint16), C.c_long, C.c_float, C.c_int, C.POINTER(C.c_float)]
     L.or_downsample_depth.restype = C.c_int
     L.or_downsample_depth.argtypes = [C.POINTER(C.c_float), C.c_int, C.c_int, C.POINTER(C.c_float), C.c_int,
@@ -358,6 +361,24 @@ class OracleMap:
         n2 = self.L.or_mesh(self.h, _rp(T, self.precision), _rp(faces, self.precision), n)
         assert n2 == n
         return faces[:n]
+
+    def field_slice(self, axis, index):
+        """One plane of Map::saveFieldSlices: [size, size] tsdf values (1 where unallocated)."""
+        out = np.zeros((self.size, self.size), self.dtype)
+        self.L.or_field_slice(self.h, int(axis), int(index), _rp(out, self.precision))
+        return out
+
+    def structure(self, T_OW=None, only_leaves=True):
+        """Map::structure: (quads [Q, 4, 3], scales [Q]) in the octree's depth-first order."""
+        T = np.eye(4, dtype=self.dtype) if T_OW is None else np.ascontiguousarray(T_OW, dtype=self.dtype)
+        R = C.POINTER(C.c_double if self.precision == "f64" else C.c_float)
+        n = self.L.or_structure(self.h, _rp(T, self.precision), int(only_leaves), R(), C.POINTER(C.c_int8)(), 0)
+        q = np.zeros((max(n, 1), 4, 3), self.dtype)
+        sc = np.zeros(max(n, 1), np.int8)
+        n2 = self.L.or_structure(self.h, _rp(T, self.precision), int(only_leaves), _rp(q, self.precision),
+                                 sc.ctypes.data_as(C.POINTER(C.c_int8)), n)
+        assert n2 == n
+        return q[:n], sc[:n]
 
     def field_interp(self, p):
         pp = np.ascontiguousarray(p, dtype=self.dtype)

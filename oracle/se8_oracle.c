@@ -2526,3 +2526,77 @@ OR_API int or_downsample_depth(const float* in, int w_in, int h_in, float* out, 
     free(ix);
     return 0;
 }
+
+/* ------------------------------------------------------------------------- */
+/* Debug views: Map::saveFieldSlices' planes (map_impl.hpp:272-308,           */
+/* octree_io_impl.hpp:19-80: z outer, y, x inner) and octree_structure_mesh   */
+/* (structure_meshing_impl.hpp:15-60) in the octree iterator's depth-first    */
+/* order, vertices mapped by T_OV = T_OW * T_WM * Scaling(res).               */
+/* ------------------------------------------------------------------------- */
+OR_API void or_field_slice(void* vm, int axis, int index, real* out)
+{
+    const or_map* m = (const or_map*) vm;
+    const int n = m->size;
+    for (int outer = 0; outer < n; outer++)
+        for (int inner = 0; inner < n; inner++) {
+            int v[3];
+            if (axis == 0)
+                v[0] = index, v[1] = inner, v[2] = outer;
+            else if (axis == 1)
+                v[0] = inner, v[1] = index, v[2] = outer;
+            else
+                v[0] = inner, v[1] = outer, v[2] = index;
+            real t = 1, w = 0;
+            if (index >= 0 && index < n)
+                mc_get_data(m, v, &t, &w);
+            out[(size_t) outer * n + inner] = t;
+        }
+}
+
+static const int or_quad_corners[6][4] = {{0, 2, 3, 1}, {1, 3, 7, 5}, {5, 7, 6, 4}, {0, 4, 6, 2}, {0, 1, 5, 4}, {2, 6, 7, 3}};
+
+static long structure_rec(const or_octant* o, const or_iso* T, int only_leaves, real* quads, signed char* scales,
+                          long n, long cap)
+{
+    int leaf = 1;
+    for (int k = 0; k < 8 && !o->is_block; k++)
+        leaf = leaf && !o->child[k];
+    if (!only_leaves || leaf) {
+        if (quads && n + 6 <= cap) {
+            real corner[8][3];
+            for (int k = 0; k < 8; k++) {
+                const real p[3] = {(real) (o->coord[0] + ((k & 1) ? o->size : 0)),
+                                   (real) (o->coord[1] + ((k & 2) ? o->size : 0)),
+                                   (real) (o->coord[2] + ((k & 4) ? o->size : 0))};
+                iso_apply(T, p, corner[k]);
+            }
+            for (int f = 0; f < 6; f++) {
+                for (int v = 0; v < 4; v++)
+                    memcpy(quads + 12 * (n + f) + 3 * v, corner[or_quad_corners[f][v]], sizeof(real) * 3);
+                scales[n + f] = o->is_block ? 0 : 7;
+            }
+        }
+        n += 6;
+    }
+    for (int k = 0; k < 8 && !o->is_block; k++)
+        if (o->child[k])
+            n = structure_rec(o->child[k], T, only_leaves, quads, scales, n, cap);
+    return n;
+}
+
+OR_API long or_structure(void* vm, const real T_OW16[16], int only_leaves, real* quads, signed char* scales, long cap)
+{
+    const or_map* m = (const or_map*) vm;
+    or_iso T_OW, T_WM, T_OV;
+    iso_from16(T_OW16, &T_OW);
+    for (int i = 0; i < 3; i++) {
+        for (int j = 0; j < 3; j++)
+            T_WM.R[i][j] = i == j ? 1 : 0;
+        T_WM.t[i] = -m->t_MW[i];
+    }
+    iso_mul(&T_OW, &T_WM, &T_OV);
+    for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++)
+            T_OV.R[i][j] = T_OV.R[i][j] * m->res;
+    return structure_rec(m->root, &T_OV, only_leaves, quads, scales, 0, cap);
+}

@@ -163,8 +163,11 @@ SIGNATURES = {
     "se8_field_interp": (C.c_int, [_P, _F, C.c_int64, _F, C.POINTER(C.c_uint8)]),
     "se8_field_grad": (C.c_int, [_P, _F, C.c_int64, _F, C.POINTER(C.c_uint8)]),
     "se8_mesh": (C.c_int, [_P, _F, _P, C.c_int64, C.POINTER(C.c_int64), _P]),
-    "se8_save_mesh": (C.c_int, [_F, C.c_int64, C.c_char_p, _F]),
+    "se8_save_mesh": (C.c_int, [_F, C.c_int64, C.c_int, C.POINTER(C.c_int8), C.c_char_p, _F]),
     "se8_has_supported_mesh_extension": (C.c_int, [C.c_char_p]),
+    "se8_save_field_slices": (C.c_int, [_P, C.c_char_p, C.c_char_p, C.c_char_p, _F]),
+    "se8_field_slice": (C.c_int, [_P, C.c_int, C.c_int, _P, _P]),
+    "se8_structure": (C.c_int, [_P, _F, C.c_int, _P, _P, C.c_int64, C.POINTER(C.c_int64), _P]),
     "se8_downsample_depth": (C.c_int, [_P, C.c_int, C.c_int, _P, C.c_int, C.c_int, _P, _P]),
     "se8_downsample_depth_u16": (C.c_int, [_P, C.c_int, C.c_int, C.c_float, C.c_int, _P, C.c_int, C.c_int, _P, _P]),
     "se8_reader_create": (C.c_int, [C.POINTER(ReaderConfig), C.POINTER(_P)]),

@@ -451,8 +451,9 @@ extern "C" int se8_mesh(const se8_map* m, const float T_OW[16], float* faces_dev
 
 // ----------------------------------------------------------------------------------------------
 // mesh files: include/se/map/io/impl/mesh_io_impl.hpp:14-29 (dispatch on the extension), 31-113
-// (VTK), 116-190 (PLY), 194-230 (OBJ); the same layouts, with the vertices written as triangle soup
-// (3 per face, face f uses vertices 3f..3f+2) and C++ stream formatting of floats.
+// (VTK), 116-190 (PLY), 194-230 (OBJ); the same layouts for faces of 3 (surface) or 4 (octree
+// structure) vertices, written as a vertex soup (face f uses vertices nv*f .. nv*f + nv - 1) with C++
+// stream formatting of floats, and each face coloured by its scale.
 // ----------------------------------------------------------------------------------------------
 namespace {
 
@@ -462,9 +463,16 @@ bool ends_with(const std::string& s, const char* suffix)
     return s.size() >= n && s.compare(s.size() - n, n, suffix) == 0;
 }
 
-// se::colours::scale[0] (include/se/common/colour_utils.hpp:23): every single-resolution face has
-// scale 0.
-constexpr int kScale0Rgb[3] = {102, 194, 165};
+// se::colours::scale (include/se/common/colour_utils.hpp:23-30); scale_colour clamps to the last
+// entry (colour_utils_impl.hpp:40-47).
+constexpr int kScaleRgb[8][3] = {{102, 194, 165}, {252, 141, 98}, {141, 160, 203}, {231, 138, 195},
+                                 {166, 216, 84},  {255, 217, 47}, {229, 196, 148}, {179, 179, 179}};
+
+const int* scale_rgb(const int8_t* scales, int64_t f)
+{
+    const int s = scales ? scales[f] : 0;
+    return kScaleRgb[s < 0 ? 0 : (s > 7 ? 7 : s)];
+}
 
 void apply_T(const float* T, const float* v, float* o)
 {
@@ -485,9 +493,11 @@ extern "C" int se8_has_supported_mesh_extension(const char* filename)
     return ends_with(f, ".ply") || ends_with(f, ".vtk") || ends_with(f, ".obj");
 }
 
-extern "C" int se8_save_mesh(const float* faces, int64_t n, const char* filename, const float T_OM[16])
+extern "C" int se8_save_mesh(const float* vertices, int64_t n, int nv, const int8_t* scales, const char* filename,
+                             const float T_OM[16])
 {
-    SE8_REQUIRE(filename && n >= 0 && (n == 0 || faces), "se8_save_mesh: null argument");
+    SE8_REQUIRE(filename && n >= 0 && (n == 0 || vertices), "se8_save_mesh: null argument");
+    SE8_REQUIRE(nv == 3 || nv == 4, "se8_save_mesh: faces have 3 or 4 vertices");
     const std::string fname(filename);
     const int kind = ends_with(fname, ".ply") ? 0 : ends_with(fname, ".vtk") ? 1 : ends_with(fname, ".obj") ? 2 : -1;
     SE8_REQUIRE(kind >= 0, "unknown file extension: " + fname);
@@ -496,51 +506,66 @@ extern "C" int se8_save_mesh(const float* faces, int64_t n, const char* filename
         set_error("Error writing mesh file " + fname);
         return SE8_ERR_IO;
     }
-    const int64_t nv = 3 * n;
-    auto vertex = [&](int64_t i, float o[3]) { apply_T(T_OM, faces + 3 * i, o); };
+    const int64_t n_vert = nv * n;
+    auto vertex = [&](int64_t i, float o[3]) { apply_T(T_OM, vertices + 3 * i, o); };
+    auto indices = [&](int64_t f, int64_t base) {
+        for (int v = 0; v < nv; v++)
+            file << " " << nv * f + v + base;
+    };
     if (kind == 0) {
         file << "ply\nformat ascii 1.0\ncomment Mesh generated by supereight2_amd\n";
-        file << "element vertex " << nv << "\nproperty float x\nproperty float y\nproperty float z\n";
+        file << "element vertex " << n_vert << "\nproperty float x\nproperty float y\nproperty float z\n";
         file << "element face " << n << "\nproperty list uchar int vertex_index\n";
         file << "property uchar red\nproperty uchar green\nproperty uchar blue\nend_header\n";
-        for (int64_t i = 0; i < nv; i++) {
+        for (int64_t i = 0; i < n_vert; i++) {
             float o[3];
             vertex(i, o);
             file << o[0] << " " << o[1] << " " << o[2] << "\n";
         }
-        for (int64_t f = 0; f < n; f++)
-            file << 3 << " " << 3 * f << " " << 3 * f + 1 << " " << 3 * f + 2 << " " << kScale0Rgb[0] << " "
-                 << kScale0Rgb[1] << " " << kScale0Rgb[2] << "\n";
+        for (int64_t f = 0; f < n; f++) {
+            const int* rgb = scale_rgb(scales, f);
+            file << nv;
+            indices(f, 0);
+            file << " " << rgb[0] << " " << rgb[1] << " " << rgb[2] << "\n";
+        }
     }
     else if (kind == 1) {
         file << "# vtk DataFile Version 3.0\nMesh generated by supereight2_amd\nASCII\nDATASET POLYDATA\n\n";
-        file << "POINTS " << nv << " float\n";
-        for (int64_t i = 0; i < nv; i++) {
+        file << "POINTS " << n_vert << " float\n";
+        for (int64_t i = 0; i < n_vert; i++) {
             float o[3];
             vertex(i, o);
             file << o[0] << " " << o[1] << " " << o[2] << "\n";
         }
-        file << "\nPOLYGONS " << n << " " << n * 4 << "\n";
-        for (int64_t f = 0; f < n; f++)
-            file << 3 << " " << 3 * f << " " << 3 * f + 1 << " " << 3 * f + 2 << "\n";
+        file << "\nPOLYGONS " << n << " " << n * (1 + nv) << "\n";
+        for (int64_t f = 0; f < n; f++) {
+            file << nv;
+            indices(f, 0);
+            file << "\n";
+        }
         file << "\nCELL_DATA " << n << "\n\n";
         file << "SCALARS scale unsigned_char\nLOOKUP_TABLE default\n";
         for (int64_t f = 0; f < n; f++)
-            file << 0 << "\n";
+            file << (scales ? (int) scales[f] : 0) << "\n";
         file << "COLOR_SCALARS scale_colour 3\n";
-        for (int64_t f = 0; f < n; f++)
-            file << kScale0Rgb[0] / 255.0f << " " << kScale0Rgb[1] / 255.0f << " " << kScale0Rgb[2] / 255.0f << "\n";
+        for (int64_t f = 0; f < n; f++) {
+            const int* rgb = scale_rgb(scales, f);
+            file << rgb[0] / 255.0f << " " << rgb[1] / 255.0f << " " << rgb[2] / 255.0f << "\n";
+        }
         file << "\n";
     }
     else {
-        file << "# Mesh generated by supereight2_amd\n# " << nv << " vertices\n# " << n << " faces\n";
-        for (int64_t i = 0; i < nv; i++) {
+        file << "# Mesh generated by supereight2_amd\n# " << n_vert << " vertices\n# " << n << " faces\n";
+        for (int64_t i = 0; i < n_vert; i++) {
             float o[3];
             vertex(i, o);
             file << "v " << o[0] << " " << o[1] << " " << o[2] << "\n";
         }
-        for (int64_t f = 0; f < n; f++)
-            file << "f " << 3 * f + 1 << " " << 3 * f + 2 << " " << 3 * f + 3 << "\n";
+        for (int64_t f = 0; f < n; f++) {
+            file << "f";
+            indices(f, 1);
+            file << "\n";
+        }
     }
     file.close();
     if (!file) {

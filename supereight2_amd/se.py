@@ -249,6 +249,47 @@ class TSDFMap:
         torch.cuda.current_stream().synchronize()
         return save_mesh(faces.cpu().numpy(), filename)
 
+    def saveFieldSlices(self, filename_x, filename_y, filename_z, point_W):
+        """Map::saveFieldSlices (map.hpp:309-325, map_impl.hpp:272-308): VTK planes of the tsdf through
+        the voxel of point_W; an empty filename skips that plane.  Returns 0 (SE8_ERR_IO raises)."""
+        p = np.ascontiguousarray(point_W, dtype=np.float32).reshape(3)
+        capi.check(self.lib.se8_save_field_slices(self.h, str(filename_x or "").encode(), str(filename_y or "").encode(),
+                                                  str(filename_z or "").encode(), capi.fptr(p)), "se8_save_field_slices")
+        return 0
+
+    def field_slice(self, axis, index, stream=None):
+        """One plane of saveFieldSlices on the device: [size, size] float32 CUDA tensor, rows = the slower
+        free coordinate."""
+        torch = _torch()
+        n = self.octree_size()
+        out = torch.empty((n, n), dtype=torch.float32, device="cuda")
+        capi.check(self.lib.se8_field_slice(self.h, int(axis), int(index), C.c_void_p(out.data_ptr()),
+                                            _stream_handle(stream)), "se8_field_slice")
+        return out
+
+    def structure(self, T_OW=None, only_leaves=True, stream=None):
+        """Map::structure (map.hpp:415-421, map_impl.hpp:473-479): (quads [Q, 4, 3] float32 CUDA tensor,
+        scales [Q] int8 CUDA tensor), six quads per octant (only the leaves by default, as the reference)."""
+        torch = _torch()
+        T = None if T_OW is None else capi.pose16(T_OW)
+        st = _stream_handle(stream)
+        n = C.c_int64()
+        capi.check(self.lib.se8_structure(self.h, T, int(only_leaves), None, None, 0, C.byref(n), st), "se8_structure")
+        Q = n.value
+        quads = torch.empty((max(Q, 1), 4, 3), dtype=torch.float32, device="cuda")
+        scales = torch.empty(max(Q, 1), dtype=torch.int8, device="cuda")
+        if Q:
+            capi.check(self.lib.se8_structure(self.h, T, int(only_leaves), C.c_void_p(quads.data_ptr()),
+                                              C.c_void_p(scales.data_ptr()), Q, C.byref(n), st), "se8_structure")
+        return quads[:Q], scales[:Q]
+
+    def saveStructure(self, filename, T_OW=None, only_leaves=True):
+        """Map::saveStructure (map.hpp:422-430, map_impl.hpp:484-490)."""
+        torch = _torch()
+        q, sc = self.structure(T_OW, only_leaves)
+        torch.cuda.current_stream().synchronize()
+        return save_mesh(q.cpu().numpy(), filename, scales=sc.cpu().numpy())
+
     def getData(self, points_W):
         p = np.ascontiguousarray(points_W, dtype=np.float32).reshape(-1, 3)
         t = np.zeros(len(p), np.float32)
@@ -278,13 +319,19 @@ def has_supported_mesh_extension(filename: str) -> bool:
     return bool(capi.load().se8_has_supported_mesh_extension(str(filename).encode()))
 
 
-def save_mesh(faces, filename, T_OM=None):
-    """se::io::save_mesh (mesh_io.hpp:44-46, impl/mesh_io_impl.hpp:14-230) of host faces (F, 3, 3):
-    0 on success, 1 if the file cannot be written; ValueError for an unsupported extension."""
+def save_mesh(faces, filename, T_OM=None, scales=None):
+    """se::io::save_mesh (mesh_io.hpp:44-46, impl/mesh_io_impl.hpp:14-230) of host faces (F, 3, 3)
+    triangles or (F, 4, 3) quads, with optional per-face scales (F int8): 0 on success, 1 if the
+    file cannot be written; ValueError for an unsupported extension."""
     lib = capi.load()
-    f = np.ascontiguousarray(faces, dtype=np.float32).reshape(-1, 9)
+    fa = np.ascontiguousarray(faces, dtype=np.float32)
+    nv = fa.shape[1] if fa.ndim == 3 else 3
+    f = fa.reshape(-1, 3 * nv)
+    sc = None if scales is None else np.ascontiguousarray(scales, dtype=np.int8)
     T = None if T_OM is None else capi.pose16(T_OM)
-    rc = lib.se8_save_mesh(capi.fptr(f) if len(f) else None, len(f), str(filename).encode(), T)
+    rc = lib.se8_save_mesh(capi.fptr(f) if len(f) else None, len(f), nv,
+                           sc.ctypes.data_as(C.POINTER(C.c_int8)) if sc is not None else None,
+                           str(filename).encode(), T)
     if rc == capi.SE8_ERR_IO:
         return 1
     if rc == capi.SE8_ERR_INVALID:

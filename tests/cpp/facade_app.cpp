@@ -161,6 +161,12 @@ int main(int argc, char** argv)
     }
     std::printf("mesh faces %zu worst_plane_dist %.5f saved %d bad_ext_threw %d\n", mesh.size(), mesh_worst, saved,
                 (int) threw);
+    const auto structure = map.structure();
+    const int saved_structure = map.saveStructure("facade_app_structure.vtk", se::Isometry3f::Identity(), false);
+    const int saved_slices =
+        map.saveFieldSlices("", "", "facade_app_slice_z.vtk", se::Vector3f(0.f, 0.f, 1.4f));
+    std::printf("structure quads %zu leaves %lld saved %d slices %d\n", structure.size(),
+                (long long) map.info().num_blocks, saved_structure, saved_slices);
 
     // A 512 x 32 beam lidar at 2 m over the ground plane z = 0 with a wall at x = 12, two scans
     // 0.5 m apart, into a 51.2 m map at 4 cm voxels; the field at a ground point is then ~0.

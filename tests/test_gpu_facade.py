@@ -34,6 +34,10 @@ def test_facade_app_tracks_and_maps(tmp_path):
     assert m and int(m.group(1)) > 10000 and float(m.group(2)) < 0.02 and m.group(3) == "0" and m.group(4) == "1"
     ply = (tmp_path / "facade_app_mesh.ply").read_text().split("\n")
     assert ply[0] == "ply" and ply[3] == f"element vertex {3 * int(m.group(1))}"
+    # Map::structure (leaves: 6 quads per block), saveStructure and saveFieldSlices
+    m = re.search(r"structure quads (\d+) leaves (\d+) saved (\d) slices (\d)", res.stdout)
+    assert m and int(m.group(1)) == 6 * int(m.group(2)) and m.group(3) == "0" and m.group(4) == "0"
+    assert (tmp_path / "facade_app_structure.vtk").exists() and (tmp_path / "facade_app_slice_z.vtk").exists()
     # lidar: the field at points on the ground plane and on the wall is the zero crossing, to within
     # the interpolation error of a 4 cm grid (mu = 16 cm, so 0.15 is 2.4 cm)
     m = re.search(r"lidar ground (\d) (\S+) wall (\d) (\S+) blocks (\d+)", res.stdout)

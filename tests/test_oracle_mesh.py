@@ -247,3 +247,72 @@ def test_save_mesh_errors(tmp_path):
         se.save_mesh(_faces(), str(tmp_path / "m.png"))
     assert se.save_mesh(_faces(), str(tmp_path / "missing_dir" / "m.ply")) == 1
     assert se.save_mesh(np.zeros((0, 3, 3), np.float32), str(tmp_path / "e.ply")) == 0
+
+
+def test_oracle_structure_cubes():
+    """octree_structure_mesh (structure_meshing_impl.hpp:15-60): six outward quads per octant, the
+    blocks (scale 0) as the leaves, nodes (scale 7) added without only_leaves; an empty map's root
+    is its only leaf."""
+    m = O.OracleMap(0.64, 0.01)  # 64 voxels: root (64), nodes of 32 and 16, blocks of 8
+    q, sc = m.structure()
+    assert q.shape == (6, 4, 3) and (sc == 7).all()
+    blocks = np.array([[0, 0, 0], [8, 0, 0], [40, 16, 56]], np.int32)
+    m.allocate_blocks(blocks)
+    ql, sl = m.structure()
+    qa, sa = m.structure(only_leaves=False)
+    n_nodes = m.num_nodes
+    assert ql.shape == (6 * 3, 4, 3) and (sl == 0).all()
+    assert qa.shape == (6 * (3 + n_nodes), 4, 3) and (sa == 7).sum() == 6 * n_nodes
+    for q in (ql, qa):
+        v = (q.astype(np.float64) + 0.32) / 0.01  # voxel units
+        for k in range(0, len(v), 6):
+            cube = v[k:k + 6].reshape(-1, 3)
+            lo, hi = cube.min(0), cube.max(0)
+            size = np.round(hi - lo)
+            assert np.allclose(hi - lo, size, atol=1e-4) and (size == size[0]).all() and size[0] in (8, 16, 32, 64)
+            assert np.allclose(np.round(lo) % size[0], 0) and np.allclose(lo, np.round(lo), atol=1e-4)
+            for f in v[k:k + 6]:
+                n = np.cross(f[1] - f[0], f[2] - f[1])
+                assert np.dot(n, f.mean(0) - (lo + hi) / 2) > 0  # outward
+                assert np.isclose(np.abs(n).max(), size[0] ** 2, rtol=1e-5)
+    leaf_lo = sorted(tuple(np.round((ql[k].reshape(-1, 3).min(0) + 0.32) / 0.01).astype(int)) for k in range(0, 18, 6))
+    assert leaf_lo == sorted(tuple(b) for b in blocks.tolist())
+
+
+def test_oracle_field_slice():
+    m = O.OracleMap(0.32, 0.01)
+    m.allocate_blocks(np.array([[8, 16, 24]], np.int32))
+    t = np.arange(512, dtype=np.float32) / 512
+    w = np.ones(512, np.float32)
+    m.set_block_data(np.array([8, 16, 24], np.int32), t, w)
+    s = m.field_slice(2, 27)  # z = 27 crosses the block at local z = 3
+    expect = np.ones((32, 32), np.float32)
+    for y in range(16, 24):
+        for x in range(8, 16):
+            expect[y, x] = t[(x - 8) + 8 * (y - 16) + 64 * 3]
+    np.testing.assert_array_equal(s, expect)
+    s = m.field_slice(0, 9)  # x = 9: rows z, columns y
+    assert s[24 + 2, 16 + 5] == t[1 + 8 * 5 + 64 * 2]
+    assert (m.field_slice(1, -1) == 1).all() and (m.field_slice(1, 32) == 1).all()
+
+
+def test_save_quads_with_scales(tmp_path):
+    from supereight2_amd import se
+
+    q = np.arange(2 * 4 * 3, dtype=np.float32).reshape(2, 4, 3)
+    p = tmp_path / "s.ply"
+    assert se.save_mesh(q, str(p), scales=np.array([0, 7], np.int8)) == 0
+    lines = p.read_text().split("\n")
+    assert lines[3] == "element vertex 8" and lines[7] == "element face 2"
+    assert lines[21:23] == ["4 0 1 2 3 102 194 165", "4 4 5 6 7 179 179 179"]
+    v = tmp_path / "s.vtk"
+    assert se.save_mesh(q, str(v), scales=np.array([3, 9], np.int8)) == 0
+    body = v.read_text().split("\n")
+    i = body.index("POLYGONS 2 10")
+    assert body[i + 1:i + 3] == ["4 0 1 2 3", "4 4 5 6 7"]
+    j = body.index("LOOKUP_TABLE default")
+    assert body[j + 1:j + 3] == ["3", "9"]
+    assert body[j + 4] == "0.905882 0.541176 0.764706" and body[j + 5] == "0.701961 0.701961 0.701961"
+    o = tmp_path / "s.obj"
+    assert se.save_mesh(q, str(o)) == 0
+    assert o.read_text().split("\n")[-3:-1] == ["f 1 2 3 4", "f 5 6 7 8"]
