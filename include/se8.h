@@ -35,6 +35,10 @@ extern "C" {
 
 #define SE8_STORAGE_F32 0 /* float tsdf + float weight, 8 B/voxel (reference layout minus padding) */
 #define SE8_STORAGE_F16 1 /* half tsdf + uint16 weight, 4 B/voxel */
+#define SE8_STORAGE_MULTIRES 2 /* multi-resolution TSDF (se::TSDFMap<se::Res::Multi>, block.hpp:66-166):
+                                * per block the data and the past data of scales 0..3 (512 + 64 + 8 + 1
+                                * entries of float tsdf + float weight each) and its current and minimum
+                                * scale; integrated by the multi-resolution updater */
 
 typedef void* se8_stream; /* a hipStream_t */
 typedef struct se8_map se8_map;
@@ -208,6 +212,11 @@ int se8_read_block(const se8_map* map, const int block_coord[3], float* tsdf512,
  * their timestamps; tsdf/weight (512 per block) may be NULL.  cap = capacity of the arrays. */
 int se8_dump_blocks(const se8_map* map, uint64_t* keys, int* timestamps, float* tsdf, float* weight, int64_t cap,
                     int64_t* out_n);
+/* The blocks of a multi-resolution map (SE8_STORAGE_MULTIRES) sorted by key: their current and
+ * minimum scale (2 ints per block, -1 = none), data and past data (585 entries of {tsdf, weight} per
+ * block: scale 0 (512, x fastest), 1 (64), 2 (8), 3 (1); block.hpp:66-166); any output may be NULL. */
+int se8_dump_blocks_multires(const se8_map* map, uint64_t* keys, int* scales, float* data, float* past_data, int64_t cap,
+                             int64_t* out_n);
 /* All octree nodes (root included) sorted by key, with child masks (octant.hpp:19-72) and
  * timestamps (propagator_impl.hpp:146-197). */
 int se8_dump_nodes(const se8_map* map, uint64_t* keys, uint8_t* child_masks, int* timestamps, int64_t cap,

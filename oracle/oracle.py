@@ -116,6 +116,8 @@ def lib(precision: str = "f32") -> C.CDLL:
     L.or_mesh.restype = C.c_long
     L.or_mesh.argtypes = [P, R, R, C.c_long]
     L.or_field_slice.argtypes = [P, C.c_int, C.c_int, R]
+    L.or_map_set_multires.argtypes = [P]
+    L.or_map_dump_multires.argtypes = [P, C.POINTER(U64), I, R, R]
     L.or_structure.restype = C.c_long
     L.or_structure.argtypes = [P, R, C.c_int, R, C.POINTER(C.c_int8), C.c_long]
     L.or_depth_from_u16.argtypes = [C.POINTER(C.c_uint16), C.c_long, C.c_float, C.c_int, C.POINTER(C.c_float)]
@@ -253,12 +255,15 @@ def unique_allocation(keys, max_block_scale):
 # map
 # ----------------------------------------------------------------------------------------------
 class OracleMap:
-    def __init__(self, dim=10.24, res=0.01, mu_factor=8.0, max_weight=100.0, precision="f32"):
+    def __init__(self, dim=10.24, res=0.01, mu_factor=8.0, max_weight=100.0, precision="f32", multires=False):
         self.precision = precision
         self.L = lib(precision)
         self.dtype = _dtype(precision)
         self.h = self.L.or_map_create(float(dim), float(res), float(mu_factor), float(max_weight))
         self.res = res
+        self.multires = multires
+        if multires:
+            self.L.or_map_set_multires(self.h)
 
     def __del__(self):
         if getattr(self, "h", None):
@@ -316,6 +321,16 @@ class OracleMap:
         if with_data:
             return keys[:n], ts[:n], tsdf[:n], w[:n]
         return keys[:n], ts[:n], None, None
+
+    def dump_multires(self):
+        """(keys, scales [N, 2] (current, minimum), data [N, 585, 2], past data [N, 585, 2]) sorted by key."""
+        n = self.num_blocks
+        keys = np.zeros(max(n, 1), np.uint64)
+        sc = np.zeros((max(n, 1), 2), np.int32)
+        data = np.zeros((max(n, 1), 585, 2), self.dtype)
+        past = np.zeros((max(n, 1), 585, 2), self.dtype)
+        self.L.or_map_dump_multires(self.h, _u64p(keys), _ip(sc), _rp(data, self.precision), _rp(past, self.precision))
+        return keys[:n], sc[:n], data[:n], past[:n]
 
     def dump_nodes(self):
         n = self.num_nodes

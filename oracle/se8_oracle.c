@@ -759,6 +759,10 @@ typedef struct or_octant {
     unsigned char child_mask;
     real* tsdf;   /* 512, x fastest, blocks only */
     real* weight; /* 512 */
+    /* multi-resolution blocks: 585 entries (scales 0..3: 512, 64, 8, 1, x fastest) of {tsdf, weight},
+     * then their past data; current / minimum scale (-1 = none) and the scale chosen this frame */
+    real* mr;
+    int cur_scale, min_scale, new_scale;
 } or_octant;
 
 typedef struct {
@@ -772,6 +776,7 @@ typedef struct {
     long n_blocks, cap_blocks;
     long n_nodes;
     int aabb_min[3], aabb_max[3], aabb_empty;
+    int multires; /* TSDFMap<Res::Multi>: blocks carry the multi-resolution data */
 } or_map;
 
 static unsigned or_power_two_up(float x)
@@ -837,6 +842,7 @@ static void free_octant(or_octant* o)
         free_octant(o->child[i]);
     free(o->tsdf);
     free(o->weight);
+    free(o->mr);
     free(o);
 }
 
@@ -1270,6 +1276,256 @@ OR_API void or_compute_T_CV(void* vm, const real T_WC16[16], real out16[16])
     iso_to16(&T_CV, out16);
 }
 
+/* ------------------------------------------------------------------------- */
+/* Multi-resolution TSDF updater: multires_tsdf_updater_impl.hpp:14-254,     */
+/* PinholeCamera::computeIntegrationScaleImpl (pinhole_camera.cpp:115-152),   */
+/* propagateBlockDown / propagateBlockUp (propagator_impl.hpp:18-117),        */
+/* up_prop_mean (data_impl.hpp:14-53), multi-resolution getFieldInterp        */
+/* (visitor_impl.hpp:399-460, 905-971).  The reference's OpenMP loop lets a   */
+/* block's interpolation read a neighbour updated in the same frame, so its   */
+/* result depends on the schedule; this restatement takes the schedule in     */
+/* which every scale choice and down-propagation reads the map as it was      */
+/* before the frame's updates, then every block updates and propagates up.    */
+/* ------------------------------------------------------------------------- */
+static const int or_mr_offsets[4] = {0, 512, 576, 584};
+
+static inline int mr_idx(int x, int y, int z, int s)
+{
+    const int n = 8 >> s;
+    return or_mr_offsets[s] + (x >> s) + (y >> s) * n + (z >> s) * n * n;
+}
+
+/* entry i: data {mr[2i], mr[2i+1]}, past data {mr[1170 + 2i], mr[1171 + 2i]} */
+static void mr_init(or_octant* b)
+{
+    if (b->mr)
+        return;
+    b->mr = (real*) malloc(sizeof(real) * 4 * 585);
+    for (int i = 0; i < 2 * 585; i++) {
+        b->mr[2 * i] = 1;
+        b->mr[2 * i + 1] = 0;
+    }
+    b->cur_scale = -1;
+    b->min_scale = -1;
+}
+
+/* pixel_voxel_ratio_per_scale = {1.5, 3, 6} (sensor.hpp:56) */
+static int mr_scale_from_z(real z, real scaled_pixel, real res)
+{
+    const real pv = z * scaled_pixel / res;
+    int scale = 0;
+    const real ratios[3] = {(real) 1.5, (real) 3, (real) 6};
+    for (int i = 0; i < 3; i++) {
+        if (pv < ratios[i])
+            break;
+        scale++;
+    }
+    return scale < 3 ? scale : 3;
+}
+
+static int mr_integration_scale(real z, real scaled_pixel, real res, int last, int min_scale)
+{
+    const int scale = mr_scale_from_z(z, scaled_pixel, res);
+    if (scale > last && min_scale != -1)
+        return mr_scale_from_z(z - (real) 0.25, scaled_pixel, res);
+    if (scale < last && min_scale != -1)
+        return mr_scale_from_z(z + (real) 0.25, scaled_pixel, res);
+    return scale;
+}
+
+static int mr_interp(const or_map* m, const real cf[3], int desired, real* out)
+{
+    const int ic[3] = {(int) cf[0], (int) cf[1], (int) cf[2]};
+    const or_octant* leaf = fetch_block(m, ic);
+    const int leaf_scale = leaf ? (leaf->mr ? leaf->cur_scale : -1) : 0;
+    const int init = leaf ? (leaf_scale > desired ? leaf_scale : desired) : 0;
+    for (int scale = init; scale <= 3; scale++) {
+        const int stride = 1 << scale;
+        const real inv = (real) 1 / (real) stride;
+        real bf[3];
+        int bi[3];
+        int in_box = !m->aabb_empty, in_map = 1;
+        for (int i = 0; i < 3; i++) {
+            bf[i] = inv * cf[i] - (real) 0.5;
+            bi[i] = stride * (int) bf[i];
+            in_box = in_box && m->aabb_min[i] <= bi[i] && bi[i] <= m->aabb_max[i];
+            in_map = in_map && bi[i] >= 0 && bi[i] + stride < m->size;
+        }
+        if (!in_box)
+            return 0;
+        if (!in_map)
+            continue;
+        real d[8];
+        int coarser = 0, valid = 1;
+        for (int k = 0; k < 8; k++) {
+            const int c[3] = {bi[0] + stride * (k & 1), bi[1] + stride * ((k >> 1) & 1), bi[2] + stride * ((k >> 2) & 1)};
+            const or_octant* nb = fetch_block(m, c);
+            d[k] = 1;
+            if (!nb) {
+                valid = 0;
+                continue;
+            }
+            if (nb->mr && nb->cur_scale > scale) {
+                coarser = 1;
+                continue;
+            }
+            if (!nb->mr) {
+                valid = 0;
+                continue;
+            }
+            const int e = mr_idx(c[0] - nb->coord[0], c[1] - nb->coord[1], c[2] - nb->coord[2], scale);
+            d[k] = nb->mr[2 * e];
+            valid = valid && nb->mr[2 * e + 1] != 0;
+        }
+        if (coarser)
+            continue;
+        if (!valid)
+            return 0;
+        real t[3], tc[3];
+        for (int i = 0; i < 3; i++) {
+            t[i] = bf[i] - floor(bf[i]);
+            tc[i] = (real) 1 - t[i];
+        }
+        *out = ((d[0] * tc[0] + d[1] * t[0]) * tc[1] + (d[2] * tc[0] + d[3] * t[0]) * t[1]) * tc[2] +
+               ((d[4] * tc[0] + d[5] * t[0]) * tc[1] + (d[6] * tc[0] + d[7] * t[0]) * t[1]) * t[2];
+        return 1;
+    }
+    return 0;
+}
+
+/* scale choice and down-propagation (reads only pre-frame data of other blocks) */
+static void mr_scale_down(const or_map* m, or_octant* b, const or_iso* T_CW, real scaled_pixel)
+{
+    real cW[3], cC[3];
+    for (int i = 0; i < 3; i++)
+        cW[i] = ((real) b->coord[i] + (real) 4) * m->res - m->t_MW[i];
+    iso_apply(T_CW, cW, cC);
+    const int last = b->cur_scale;
+    int s = mr_integration_scale(cC[2], scaled_pixel, m->res, last, b->min_scale);
+    if (s < last - 1)
+        s = last - 1;
+    b->new_scale = s;
+    real* data = b->mr;
+    real* past = b->mr + 2 * 585;
+    for (int ps = last; ps > s; ps--) {
+        const int cs = ps - 1, cn = 8 >> cs, cstride = 1 << cs;
+        for (int i = 0; i < cn * cn * cn; i++) {
+            const int lx = (i % cn) * cstride, ly = ((i / cn) % cn) * cstride, lz = (i / (cn * cn)) * cstride;
+            const int ci = mr_idx(lx, ly, lz, cs), pi = mr_idx(lx, ly, lz, ps);
+            if (data[2 * ci + 1] != 0) {
+                const real v = data[2 * ci] + (data[2 * pi] - past[2 * pi]);
+                data[2 * ci] = v < (real) -1 ? (real) -1 : v; /* std::max(v, -1) keeps a NaN */
+                data[2 * ci + 1] = fmin(data[2 * ci + 1] + past[2 * pi + 1], m->max_weight);
+                past[2 * ci + 1] = past[2 * pi + 1];
+            }
+            else {
+                const real cf[3] = {(real) (b->coord[0] + lx) + (real) 0.5 * (real) cstride,
+                                    (real) (b->coord[1] + ly) + (real) 0.5 * (real) cstride,
+                                    (real) (b->coord[2] + lz) + (real) 0.5 * (real) cstride};
+                real f;
+                if (mr_interp(m, cf, cs, &f)) {
+                    data[2 * ci] = f;
+                    data[2 * ci + 1] = data[2 * pi + 1];
+                    past[2 * ci] = f;
+                    past[2 * ci + 1] = 0;
+                }
+            }
+        }
+        const int pn = 8 >> ps;
+        for (int i = 0; i < pn * pn * pn; i++) {
+            const int pi = or_mr_offsets[ps] + i;
+            past[2 * pi] = data[2 * pi];
+            past[2 * pi + 1] = 0;
+        }
+    }
+}
+
+/* commit the scale, update the voxels of that scale, up-propagate */
+static void mr_update_up(const or_map* m, or_octant* b, const or_iso* T_CV, const or_sensor* sen, const float* depth,
+                         int ts)
+{
+    b->timestamp = ts;
+    const int curr = b->new_scale;
+    b->cur_scale = curr;
+    b->min_scale = b->min_scale < 0 ? curr : (b->min_scale < curr ? b->min_scale : curr);
+    real* data = b->mr;
+    real* past = b->mr + 2 * 585;
+    const int stride = 1 << curr;
+    for (int x = 0; x < 8; x += stride)
+        for (int y = 0; y < 8; y += stride)
+            for (int z = 0; z < 8; z += stride) {
+                const real v[3] = {(real) (b->coord[0] + x), (real) (b->coord[1] + y), (real) (b->coord[2] + z)};
+                real pC[3];
+                iso_apply(T_CV, v, pC);
+                if (norm3(pC) > sensor_far_dist(sen, pC))
+                    continue;
+                real uv[2];
+                if (!sensor_project(sen, pC, uv))
+                    continue;
+                int px[2];
+                round_pixel(uv, px);
+                const real d = depth[px[0] + px[1] * sen->w];
+                if (d < sen->near_plane)
+                    continue;
+                const real mz = sensor_measurement(sen, pC);
+                const real sdf = (d - mz) / mz * norm3(pC);
+                const int vi = mr_idx(x, y, z, curr);
+                field_update(&data[2 * vi], &data[2 * vi + 1], sdf, m->mu, m->max_weight);
+                past[2 * vi + 1] = past[2 * vi + 1] + (real) 1;
+            }
+    for (int cs = curr; cs < 3; cs++) {
+        const int ps = cs + 1, pn = 8 >> ps, cstride = 1 << cs;
+        for (int i = 0; i < pn * pn * pn; i++) {
+            const int x0 = (i % pn) << ps, y0 = ((i / pn) % pn) << ps, z0 = (i / (pn * pn)) << ps;
+            real tsum = 0, wsum = 0;
+            int nv = 0;
+            for (int k = 0; k < 8; k++) {
+                const int ci = mr_idx(x0 + cstride * (k & 1), y0 + cstride * ((k >> 1) & 1), z0 + cstride * ((k >> 2) & 1), cs);
+                if (data[2 * ci + 1] != 0) {
+                    tsum += data[2 * ci];
+                    wsum += data[2 * ci + 1];
+                    nv++;
+                }
+            }
+            const int pi = mr_idx(x0, y0, z0, ps);
+            if (nv > 0) {
+                data[2 * pi] = tsum / (real) nv;
+                data[2 * pi + 1] = ceil(wsum / (real) nv);
+                past[2 * pi] = data[2 * pi];
+                past[2 * pi + 1] = 0;
+            }
+            else {
+                past[2 * pi] = 1;
+                past[2 * pi + 1] = 0;
+            }
+        }
+    }
+}
+
+OR_API void or_map_set_multires(void* vm)
+{
+    ((or_map*) vm)->multires = 1;
+}
+
+/* blocks sorted by key: scales (current, minimum), 585 x {tsdf, weight} data and past data */
+OR_API void or_map_dump_multires(void* vm, uint64_t* keys, int* scales, real* data, real* past)
+{
+    or_map* m = (or_map*) vm;
+    or_octant** tmp = (or_octant**) malloc(sizeof(or_octant*) * (m->n_blocks + 1));
+    memcpy(tmp, m->blocks, sizeof(or_octant*) * m->n_blocks);
+    qsort(tmp, (size_t) m->n_blocks, sizeof(or_octant*), cmp_octant_key);
+    for (long i = 0; i < m->n_blocks; i++) {
+        or_octant* b = tmp[i];
+        mr_init(b);
+        or_encode_key(b->coord, OR_BLOCK_SCALE, &keys[i]);
+        scales[2 * i] = b->cur_scale;
+        scales[2 * i + 1] = b->min_scale;
+        memcpy(data + 2 * 585 * i, b->mr, sizeof(real) * 2 * 585);
+        memcpy(past + 2 * 585 * i, b->mr + 2 * 585, sizeof(real) * 2 * 585);
+    }
+    free(tmp);
+}
+
 /* stats: [0] frustum blocks, [1] new blocks, [2] candidate keys (with per-pixel repeats),
  *        [3] valid pixels */
 static void integrate_sensor(or_map* m, int ts, const float* depth, const or_sensor* sen, const real T_WC16[16],
@@ -1321,9 +1577,23 @@ static void integrate_sensor(or_map* m, int ts, const float* depth, const or_sen
     memcpy(list + n_new, frustum, sizeof(or_octant*) * n_frustum);
     or_iso T_CV;
     compute_T_CV(m, &T_WC, &T_CV);
+    if (m->multires && sen->cam) {
+        or_iso T_CW;
+        iso_inv(&T_WC, &T_CW);
+        for (long i = 0; i < nb; i++)
+            mr_init(list[i]);
 #pragma omp parallel for schedule(dynamic, 16)
-    for (long i = 0; i < nb; i++)
-        update_block(m, list[i], &T_CV, sen, depth, ts);
+        for (long i = 0; i < nb; i++)
+            mr_scale_down(m, list[i], &T_CW, sen->cam->inv_fx);
+#pragma omp parallel for schedule(dynamic, 16)
+        for (long i = 0; i < nb; i++)
+            mr_update_up(m, list[i], &T_CV, sen, depth, ts);
+    }
+    else {
+#pragma omp parallel for schedule(dynamic, 16)
+        for (long i = 0; i < nb; i++)
+            update_block(m, list[i], &T_CV, sen, depth, ts);
+    }
     /* propagateTimeStampToRoot: only the blocks' direct parents are reached */
     for (long i = 0; i < nb; i++) {
         or_octant* p = list[i]->parent;

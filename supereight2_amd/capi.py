@@ -22,6 +22,7 @@ SE8_ERR_CAPACITY = -4
 SE8_ERR_IO = -5
 STORAGE_F32 = 0
 STORAGE_F16 = 1
+STORAGE_MULTIRES = 2
 COPY_H2D = 0
 COPY_D2H = 1
 COPY_D2D = 2
@@ -158,6 +159,7 @@ SIGNATURES = {
     "se8_write_block": (C.c_int, [_P, _I, _F, _F]),
     "se8_read_block": (C.c_int, [_P, _I, _F, _F]),
     "se8_dump_blocks": (C.c_int, [_P, C.POINTER(C.c_uint64), _I, _F, _F, C.c_int64, C.POINTER(C.c_int64)]),
+    "se8_dump_blocks_multires": (C.c_int, [_P, C.POINTER(C.c_uint64), _I, _F, _F, C.c_int64, C.POINTER(C.c_int64)]),
     "se8_dump_nodes": (C.c_int, [_P, C.POINTER(C.c_uint64), C.POINTER(C.c_uint8), _I, C.c_int64, C.POINTER(C.c_int64)]),
     "se8_get_data": (C.c_int, [_P, _F, C.c_int64, _F, _F]),
     "se8_field_interp": (C.c_int, [_P, _F, C.c_int64, _F, C.POINTER(C.c_uint8)]),

@@ -505,6 +505,7 @@ struct MapView {
     int* block_ts;
     void* voxels;       // max_blocks * 512 voxels
     int* counters;      // [0] blocks, [1] nodes, [2] bricks, [3] overflow flag, [4..9] aabb
+    int* block_scale;   // multi-resolution maps: per block {current scale, minimum scale}, -1 = none
     int64_t max_blocks, max_nodes, max_bricks;
 };
 
@@ -603,6 +604,45 @@ __device__ __forceinline__ void get_voxel(const MapView& m, int x, int y, int z,
     load_voxel(m, slot, (x & 7) + ((y & 7) << 3) + ((z & 7) << 6), tsdf, weight);
 }
 
+// Multi-resolution block layout (block.hpp:66-166): entries of scale s start at kMrOffset[s], x
+// fastest, (8 >> s) per side; the data of the 585 entries, then their past data, float2 {tsdf,
+// weight} each.
+constexpr int kMrEntries = 585;
+constexpr int kMrBlockFloat2 = 2 * kMrEntries;
+SE8_HD int mr_offset(int s)
+{
+    return s == 0 ? 0 : (s == 1 ? 512 : (s == 2 ? 576 : 584));
+}
+// entry of the voxel with block-local coordinates (x, y, z) at scale s
+SE8_HD int mr_index(int x, int y, int z, int s)
+{
+    const int n = 8 >> s;
+    return mr_offset(s) + (x >> s) + (y >> s) * n + (z >> s) * n * n;
+}
+
+// FieldData<TSDF>::update (data_field_impl.hpp:39-54) of one voxel from a depth sample d (skipped
+// unless d_ok and d >= near_plane); returns whether the voxel changed.
+__device__ __forceinline__ bool tsdf_update_voxel(const MapView& m, float d, bool d_ok, float near_plane, float mz,
+                                                  float nrm, float& tsdf, float& weight)
+{
+    if (!d_ok || d < near_plane)
+        return false;
+    const float sdf = (d - mz) / mz * nrm;
+    if (sdf < -m.mu)
+        return false;
+    // std::min(sdf / mu, 1) (data_field_impl.hpp:45), which keeps a NaN quotient; sdf >= mu > 0
+    // implies sdf / mu >= 1 (rounded division is monotone), so the division is only needed below mu
+    float u = 1.f;
+    if (!(sdf >= m.mu)) {
+        const float q = sdf / m.mu;
+        u = 1.f < q ? 1.f : q;
+    }
+    if (weight < m.max_weight)
+        weight = weight + 1.f;
+    tsdf = (tsdf * (weight - 1.f) + u) / weight;
+    return true;
+}
+
 // ----------------------------------------------------------------------------------------------
 // host-side map object
 // ----------------------------------------------------------------------------------------------
@@ -628,7 +668,7 @@ struct se8_map {
     se8_tsdf_config tsdf;
     se8::MapView view;
     se8::DeviceBuffer brick_top, brick_slots, node_child, node_key, node_ts, block_key, block_parent, block_ts,
-        voxels, counters;
+        voxels, counters, block_scale;
     // per-frame scratch
     se8::DeviceBuffer update_list, candidates, unique_keys, cur_node;
     se8::DeviceBuffer frame_poses; // FramePoses of the integration in flight
@@ -644,7 +684,17 @@ struct se8_map {
     // marching cubes: per-block face counts, scanned in place into offsets (+ the total); grown on
     // demand by the (logically const) se8_mesh
     mutable se8::DeviceBuffer mesh_offsets;
+    se8::DeviceBuffer mr_new_scale; // multi-resolution update: each list entry's new scale
 };
+
+namespace se8 {
+// Multi-resolution TSDF update of the frame's update list (pinhole sensor).
+int launch_multires_update(se8_map* m, const FramePoses* fp, const PinholeSensor& sensor, const float* depth, int ts,
+                           hipStream_t s);
+
+// Entry points that read or write single-resolution voxels refuse a multi-resolution map.
+inline bool is_multires(const se8_map* m);
+} // namespace se8
 
 struct se8_tracker {
     se8_pinhole sensor;
@@ -675,3 +725,8 @@ void harvest_integration(se8_map* m);
 void harvest_raycast(se8_map* m, bool block);
 void harvest_tracking(se8_tracker* t);
 } // namespace se8
+
+inline bool se8::is_multires(const se8_map* m)
+{
+    return m->view.storage == SE8_STORAGE_MULTIRES;
+}

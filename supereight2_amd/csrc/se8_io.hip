@@ -67,7 +67,8 @@ __global__ __launch_bounds__(256) void k_structure(MapView m, const int* node_li
     }
     else {
         key = m.block_key[o - n_listed];
-        scale = 0; // single resolution: every block at scale 0
+        // the block's current scale (always 0 in a single-resolution map)
+        scale = m.storage == SE8_STORAGE_MULTIRES ? (int8_t) m.block_scale[2 * (o - n_listed)] : (int8_t) 0;
     }
     int c[3];
     key_to_coord(key, c);
@@ -161,6 +162,7 @@ bool write_slice_vtk(const std::string& filename, const int lower[3], const int 
 
 extern "C" int se8_field_slice(const se8_map* m, int axis, int index, float* out_dev, se8_stream stream)
 {
+    SE8_REQUIRE(!m || !is_multires(m), "se8_field_slice: not available for a multi-resolution map");
     SE8_REQUIRE(m && out_dev && axis >= 0 && axis < 3, "se8_field_slice: bad argument");
     const int64_t n = (int64_t) m->view.size * m->view.size;
     k_field_slice<<<(unsigned) ((n + 255) / 256), 256, 0, (hipStream_t) stream>>>(m->view, axis, index, out_dev);
@@ -171,6 +173,7 @@ extern "C" int se8_field_slice(const se8_map* m, int axis, int index, float* out
 extern "C" int se8_save_field_slices(const se8_map* m, const char* filename_x, const char* filename_y,
                                      const char* filename_z, const float point_W[3])
 {
+    SE8_REQUIRE(!m || !is_multires(m), "se8_save_field_slices: not available for a multi-resolution map");
     SE8_REQUIRE(m && point_W, "se8_save_field_slices: null argument");
     // pointToVoxel (map_impl.hpp:541-550): (-1, -1, -1) for a point outside the map
     int v[3] = {-1, -1, -1};

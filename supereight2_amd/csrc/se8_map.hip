@@ -19,6 +19,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <type_traits>
 #include <vector>
 
 #include "se8_internal.hpp"
@@ -661,12 +662,19 @@ __global__ void k_build_blocks(MapView m, const uint64_t* keys, const int* n_dev
         }
         else if (made) {
             // explicit allocation: initialise the voxels here (integration marks new blocks instead)
-            for (int v = 0; v < 512; v++) {
-                if (m.storage == SE8_STORAGE_F32)
-                    reinterpret_cast<float2*>(m.voxels)[(size_t) slot * 512 + v] = make_float2(1.f, 0.f);
-                else
-                    reinterpret_cast<uint32_t*>(m.voxels)[(size_t) slot * 512 + v] = pack_f16(1.f, 0.f);
+            if (m.storage == SE8_STORAGE_MULTIRES) {
+                for (int v = 0; v < kMrBlockFloat2; v++)
+                    reinterpret_cast<float2*>(m.voxels)[(size_t) slot * kMrBlockFloat2 + v] = make_float2(1.f, 0.f);
+                m.block_scale[2 * slot] = -1;
+                m.block_scale[2 * slot + 1] = -1;
             }
+            else
+                for (int v = 0; v < 512; v++) {
+                    if (m.storage == SE8_STORAGE_F32)
+                        reinterpret_cast<float2*>(m.voxels)[(size_t) slot * 512 + v] = make_float2(1.f, 0.f);
+                    else
+                        reinterpret_cast<uint32_t*>(m.voxels)[(size_t) slot * 512 + v] = pack_f16(1.f, 0.f);
+                }
         }
     }
 }
@@ -705,29 +713,6 @@ struct VoxelRaw<SE8_STORAGE_F16> {
 
 constexpr int kUpdChunk = 512; // blocks of a workgroup whose list entries are staged in LDS at once
 constexpr int kUpdK = 2;       // blocks a workgroup updates together (their loads in flight together)
-
-// The part of the per-voxel update that follows the depth sample (singleres_tsdf_updater_impl.hpp:
-// 80-128); `d_ok` is false when the voxel failed the far or projection test.
-__device__ __forceinline__ bool tsdf_update_voxel(const MapView& m, float d, bool d_ok, float near_plane, float mz,
-                                                  float nrm, float& tsdf, float& weight)
-{
-    if (!d_ok || d < near_plane)
-        return false;
-    const float sdf = (d - mz) / mz * nrm;
-    if (sdf < -m.mu)
-        return false;
-    // std::min(sdf / mu, 1) (data_field_impl.hpp:45), which keeps a NaN quotient; sdf >= mu > 0
-    // implies sdf / mu >= 1 (rounded division is monotone), so the division is only needed below mu
-    float u = 1.f;
-    if (!(sdf >= m.mu)) {
-        const float q = sdf / m.mu;
-        u = 1.f < q ? 1.f : q;
-    }
-    if (weight < m.max_weight)
-        weight = weight + 1.f;
-    tsdf = (tsdf * (weight - 1.f) + u) / weight;
-    return true;
-}
 
 // Per voxel as singleres_tsdf_updater_impl.hpp:44-128.  A workgroup owns the list entries
 // blockIdx.x, blockIdx.x + gridDim.x, ...: it stages up to kUpdChunk of them (entry, key) in LDS
@@ -946,11 +931,12 @@ extern "C" int se8_map_create(const se8_map_config* cfg, const se8_tsdf_config* 
 {
     SE8_REQUIRE(cfg && tcfg && out, "se8_map_create: null argument");
     SE8_REQUIRE(cfg->res > 0.f, "se8_map_create: res must be positive");
-    SE8_REQUIRE(tcfg->storage == SE8_STORAGE_F32 || tcfg->storage == SE8_STORAGE_F16,
+    SE8_REQUIRE(tcfg->storage == SE8_STORAGE_F32 || tcfg->storage == SE8_STORAGE_F16 ||
+                    tcfg->storage == SE8_STORAGE_MULTIRES,
                 "se8_map_create: unknown storage");
     SE8_REQUIRE(tcfg->truncation_boundary_factor > 0.f && tcfg->max_weight >= 1.f,
                 "se8_map_create: invalid TSDF config");
-    SE8_REQUIRE(tcfg->storage == SE8_STORAGE_F32 || tcfg->max_weight <= 65535.f,
+    SE8_REQUIRE(tcfg->storage != SE8_STORAGE_F16 || tcfg->max_weight <= 65535.f,
                 "se8_map_create: max_weight does not fit uint16 weights");
     *out = nullptr;
     se8_map* m = new se8_map();
@@ -986,7 +972,8 @@ extern "C" int se8_map_create(const se8_map_config* cfg, const se8_tsdf_config* 
     v.max_nodes = 2 * max_blocks + 64;
     const int64_t bricks_side3 = (int64_t) v.bricks_side * v.bricks_side * v.bricks_side;
     v.max_bricks = std::min<int64_t>(bricks_side3, max_blocks / 4 + 64);
-    const size_t vox_bytes = tcfg->storage == SE8_STORAGE_F32 ? 8 : 4;
+    const size_t block_bytes = tcfg->storage == SE8_STORAGE_F32 ? 8 * 512
+                               : tcfg->storage == SE8_STORAGE_F16 ? 4 * 512 : 8 * (size_t) kMrBlockFloat2;
     int rc = SE8_OK;
     rc = rc ? rc : dev_alloc(m->brick_top, sizeof(int) * bricks_side3);
     rc = rc ? rc : dev_alloc(m->brick_slots, sizeof(int) * 512 * v.max_bricks);
@@ -996,7 +983,9 @@ extern "C" int se8_map_create(const se8_map_config* cfg, const se8_tsdf_config* 
     rc = rc ? rc : dev_alloc(m->block_key, sizeof(uint64_t) * max_blocks);
     rc = rc ? rc : dev_alloc(m->block_parent, sizeof(int) * max_blocks);
     rc = rc ? rc : dev_alloc(m->block_ts, sizeof(int) * max_blocks);
-    rc = rc ? rc : dev_alloc(m->voxels, vox_bytes * 512 * (size_t) max_blocks);
+    rc = rc ? rc : dev_alloc(m->voxels, block_bytes * (size_t) max_blocks);
+    if (tcfg->storage == SE8_STORAGE_MULTIRES)
+        rc = rc ? rc : dev_alloc(m->block_scale, sizeof(int) * 2 * (size_t) max_blocks);
     rc = rc ? rc : dev_alloc(m->counters, sizeof(int) * kNumCounters);
     rc = rc ? rc : dev_alloc(m->update_list, sizeof(int) * max_blocks);
     rc = rc ? rc : dev_alloc(m->frame_poses, sizeof(FramePoses));
@@ -1014,6 +1003,7 @@ extern "C" int se8_map_create(const se8_map_config* cfg, const se8_tsdf_config* 
     v.block_ts = (int*) m->block_ts.ptr;
     v.voxels = m->voxels.ptr;
     v.counters = (int*) m->counters.ptr;
+    v.block_scale = (int*) m->block_scale.ptr;
     hipStream_t s = nullptr;
     rc = rc ? rc : fill_i32(m->brick_top, bricks_side3, -1, s);
     rc = rc ? rc : fill_i32(m->brick_slots, 512 * v.max_bricks, -1, s);
@@ -1073,7 +1063,7 @@ extern "C" int se8_map_destroy(se8_map* m)
     DeviceBuffer* bufs[] = {&m->brick_top,    &m->brick_slots, &m->node_child, &m->node_key,   &m->node_ts,
                             &m->block_key,    &m->block_parent, &m->block_ts,  &m->voxels,     &m->counters,
                             &m->update_list,  &m->candidates,   &m->unique_keys, &m->cur_node,
-                            &m->frame_poses,  &m->mesh_offsets};
+                            &m->frame_poses,  &m->mesh_offsets, &m->block_scale, &m->mr_new_scale};
     for (DeviceBuffer* b : bufs)
         dev_free(*b);
     m->sorter.release();
@@ -1157,16 +1147,31 @@ static int integrate_impl(se8_map* m, int timestamp, const float* depth_dev, con
         return rc;
     // ev[3] / ev[4] are stamped at the start and end of the update kernel itself
     const int* ulist = (const int*) m->update_list.ptr;
-    // one resident wave of workgroups: the grid-stride loop then never leaves a second, partial
-    // round of workgroups waiting for the first to finish
-    if (m->view.storage == SE8_STORAGE_F32)
+    if (is_multires(m)) {
+        if constexpr (std::is_same<Sensor, PinholeSensor>::value) {
+            SE8_HIP_CHECK(hipEventRecord(m->ev[3], s));
+            rc = launch_multires_update(m, fp, sensor, depth_dev, timestamp, s);
+            if (rc != SE8_OK)
+                return rc;
+            SE8_HIP_CHECK(hipEventRecord(m->ev[4], s));
+        }
+        else {
+            set_error(std::string(what) + ": the multi-resolution updater takes a pinhole camera");
+            return SE8_ERR_INVALID;
+        }
+    }
+    else if (m->view.storage == SE8_STORAGE_F32) {
+        // one resident wave of workgroups: the grid-stride loop then never leaves a second, partial
+        // round of workgroups waiting for the first to finish
         hipExtLaunchKernelGGL(k_update<SE8_STORAGE_F32, Sensor>,
                               dim3(resident_grid(k_update<SE8_STORAGE_F32, Sensor>, 512)), dim3(512), 0, s, m->ev[3],
                               m->ev[4], 0, m->view, ulist, (const FramePoses*) fp, sensor, depth_dev, timestamp);
-    else
+    }
+    else {
         hipExtLaunchKernelGGL(k_update<SE8_STORAGE_F16, Sensor>,
                               dim3(resident_grid(k_update<SE8_STORAGE_F16, Sensor>, 512)), dim3(512), 0, s, m->ev[3],
                               m->ev[4], 0, m->view, ulist, (const FramePoses*) fp, sensor, depth_dev, timestamp);
+    }
     SE8_HIP_CHECK(hipGetLastError());
     SE8_HIP_CHECK(hipMemcpyAsync(m->host_counters, m->counters.ptr, sizeof(int) * kNumCounters,
                                  hipMemcpyDeviceToHost, s));
@@ -1309,6 +1314,7 @@ extern "C" int se8_lidar_destroy(se8_lidar* l)
 extern "C" int se8_integrate_range(se8_map* m, int timestamp, const float* range_dev, const se8_lidar* lidar,
                                    const float T_WS16[16], se8_stream stream)
 {
+    SE8_REQUIRE(!m || !is_multires(m), "se8_integrate_range: not available for a multi-resolution map");
     SE8_REQUIRE(m && range_dev && lidar && T_WS16, "se8_integrate_range: null argument");
     const LidarSensor ls{lidar->model};
     return integrate_impl(m, timestamp, range_dev, ls, T_WS16, nullptr, (hipStream_t) stream, "se8_integrate_range");
@@ -1317,6 +1323,7 @@ extern "C" int se8_integrate_range(se8_map* m, int timestamp, const float* range
 extern "C" int se8_integrate_range_dev(se8_map* m, int timestamp, const float* range_dev, const se8_lidar* lidar,
                                        const float* T_WS_dev, se8_stream stream)
 {
+    SE8_REQUIRE(!m || !is_multires(m), "se8_integrate_range_dev: not available for a multi-resolution map");
     SE8_REQUIRE(m && range_dev && lidar && T_WS_dev, "se8_integrate_range_dev: null argument");
     const LidarSensor ls{lidar->model};
     return integrate_impl(m, timestamp, range_dev, ls, nullptr, T_WS_dev, (hipStream_t) stream,
@@ -1459,6 +1466,7 @@ extern "C" int se8_allocate_blocks(se8_map* m, const int* voxel_coords, int64_t 
 
 extern "C" int se8_write_block(se8_map* m, const int bc[3], const float* tsdf512, const float* weight512)
 {
+    SE8_REQUIRE(!m || !is_multires(m), "se8_write_block: not available for a multi-resolution map");
     SE8_REQUIRE(m && bc && tsdf512 && weight512, "se8_write_block: null argument");
     std::vector<int> coords(bc, bc + 3), slots;
     int rc = lookup_slots(m, coords, slots);
@@ -1485,6 +1493,7 @@ extern "C" int se8_write_block(se8_map* m, const int bc[3], const float* tsdf512
 
 extern "C" int se8_read_block(const se8_map* m, const int bc[3], float* tsdf512, float* weight512)
 {
+    SE8_REQUIRE(!m || !is_multires(m), "se8_read_block: not available for a multi-resolution map");
     SE8_REQUIRE(m && bc && tsdf512 && weight512, "se8_read_block: null argument");
     std::vector<int> coords(bc, bc + 3), slots;
     int rc = lookup_slots(m, coords, slots);
@@ -1512,6 +1521,7 @@ extern "C" int se8_read_block(const se8_map* m, const int bc[3], float* tsdf512,
 extern "C" int se8_dump_blocks(const se8_map* m, uint64_t* keys, int* timestamps, float* tsdf, float* weight,
                                int64_t cap, int64_t* out_n)
 {
+    SE8_REQUIRE(!m || !is_multires(m), "se8_dump_blocks: not available for a multi-resolution map");
     SE8_REQUIRE(m && out_n, "se8_dump_blocks: null argument");
     int host[kNumCounters];
     int rc = read_counters(m, host, nullptr);

@@ -401,6 +401,7 @@ static Iso mesh_T_OV(const se8_map* m, const float* T_OW16)
 extern "C" int se8_mesh(const se8_map* m, const float T_OW[16], float* faces_dev, int64_t max_faces,
                         int64_t* num_faces, se8_stream stream)
 {
+    SE8_REQUIRE(!m || !is_multires(m), "se8_mesh: not available for a multi-resolution map");
     SE8_REQUIRE(m && num_faces, "se8_mesh: null argument");
     SE8_REQUIRE(max_faces >= 0 && (max_faces == 0 || faces_dev), "se8_mesh: bad output buffer");
     hipStream_t s = (hipStream_t) stream;

@@ -710,6 +710,7 @@ using namespace se8;
 extern "C" int se8_raycast(const se8_map* m, const se8_pinhole* sensor, const float T_WS16[16], float* points_dev,
                            float* normals_dev, int8_t* scale_dev, se8_stream stream)
 {
+    SE8_REQUIRE(!m || !is_multires(m), "se8_raycast: not available for a multi-resolution map");
     SE8_REQUIRE(m && sensor && T_WS16 && points_dev && normals_dev, "se8_raycast: null argument");
     SE8_REQUIRE(sensor->width > 0 && sensor->height > 0, "se8_raycast: empty image");
     SE8_REQUIRE(m->view.max_scale - kBlockScale <= kMaxLevels, "se8_raycast: octree too deep");
@@ -726,6 +727,7 @@ extern "C" int se8_raycast(const se8_map* m, const se8_pinhole* sensor, const fl
 extern "C" int se8_raycast_dev(const se8_map* m, const se8_pinhole* sensor, const float* T_WS_dev, float* points_dev,
                                float* normals_dev, int8_t* scale_dev, se8_stream stream)
 {
+    SE8_REQUIRE(!m || !is_multires(m), "se8_raycast_dev: not available for a multi-resolution map");
     SE8_REQUIRE(m && sensor && T_WS_dev && points_dev && normals_dev, "se8_raycast_dev: null argument");
     SE8_REQUIRE(sensor->width > 0 && sensor->height > 0, "se8_raycast_dev: empty image");
     SE8_REQUIRE(m->view.max_scale - kBlockScale <= kMaxLevels, "se8_raycast_dev: octree too deep");
@@ -740,6 +742,7 @@ extern "C" int se8_raycast_dev(const se8_map* m, const se8_pinhole* sensor, cons
 
 extern "C" int se8_get_data(const se8_map* m, const float* pts, int64_t n, float* tsdf, float* weight)
 {
+    SE8_REQUIRE(!m || !is_multires(m), "se8_get_data: not available for a multi-resolution map");
     SE8_REQUIRE(m && (n == 0 || (pts && tsdf && weight)), "se8_get_data: null argument");
     std::vector<float> out;
     std::vector<uint8_t> valid;
@@ -755,6 +758,7 @@ extern "C" int se8_get_data(const se8_map* m, const float* pts, int64_t n, float
 
 extern "C" int se8_field_interp(const se8_map* m, const float* pts, int64_t n, float* outv, uint8_t* valid)
 {
+    SE8_REQUIRE(!m || !is_multires(m), "se8_field_interp: not available for a multi-resolution map");
     SE8_REQUIRE(m && (n == 0 || (pts && outv && valid)), "se8_field_interp: null argument");
     std::vector<float> out;
     std::vector<uint8_t> val;
@@ -768,6 +772,7 @@ extern "C" int se8_field_interp(const se8_map* m, const float* pts, int64_t n, f
 
 extern "C" int se8_field_grad(const se8_map* m, const float* pts, int64_t n, float* out3, uint8_t* valid)
 {
+    SE8_REQUIRE(!m || !is_multires(m), "se8_field_grad: not available for a multi-resolution map");
     SE8_REQUIRE(m && (n == 0 || (pts && out3 && valid)), "se8_field_grad: null argument");
     std::vector<float> out;
     std::vector<uint8_t> val;
@@ -783,6 +788,7 @@ extern "C" int se8_raycast_depth_grad_batched(const se8_map* m, const se8_pinhol
                                               const float* poses16, float* depth_dev, float* grad_dev,
                                               uint8_t* valid_dev, se8_stream stream)
 {
+    SE8_REQUIRE(!m || !is_multires(m), "se8_raycast_depth_grad_batched: not available for a multi-resolution map");
     SE8_REQUIRE(m && sensor && poses16 && depth_dev && grad_dev && valid_dev,
                 "se8_raycast_depth_grad_batched: null argument");
     SE8_REQUIRE(n_poses > 0 && n_poses <= 65535, "se8_raycast_depth_grad_batched: n_poses out of range");

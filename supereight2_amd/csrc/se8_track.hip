@@ -1178,6 +1178,7 @@ extern "C" int se8_track_dev(se8_tracker* t, const float* depth_dev, const float
 extern "C" int se8_track_raycast(se8_tracker* t, const se8_map* m, const float* depth_dev, float T_WS[16], int* ok,
                                  se8_stream stream)
 {
+    SE8_REQUIRE(!m || !is_multires(m), "se8_track_raycast: not available for a multi-resolution map");
     SE8_REQUIRE(t && m && depth_dev && T_WS, "se8_track_raycast: null argument");
     const size_t n = (size_t) t->sensor.width * t->sensor.height;
     int rc = dev_alloc(t->ref_points, sizeof(float) * 3 * n);

@@ -130,7 +130,7 @@ class TSDFMap:
         dims = (dim, dim, dim) if np.isscalar(dim) else tuple(dim)
         t = tuple(x / 2 for x in dims) if t_MW is None else tuple(t_MW)
         self.map_cfg = capi.MapConfig((C.c_float * 3)(*dims), res, (C.c_float * 3)(*t))
-        st = {"f32": capi.STORAGE_F32, "f16": capi.STORAGE_F16}[storage]
+        st = {"f32": capi.STORAGE_F32, "f16": capi.STORAGE_F16, "multires": capi.STORAGE_MULTIRES}[storage]
         self.tsdf_cfg = capi.TsdfConfig(truncation_boundary_factor, max_weight, st, int(max_blocks))
         self.h = C.c_void_p()
         capi.check(self.lib.se8_map_create(C.byref(self.map_cfg), C.byref(self.tsdf_cfg), C.byref(self.h)),
@@ -200,6 +200,22 @@ class TSDFMap:
         if with_data:
             return keys[:N], ts[:N], tsdf[:N], w[:N]
         return keys[:N], ts[:N], None, None
+
+    def dump_blocks_multires(self):
+        """A multi-resolution map's blocks sorted by key: (keys, scales [N, 2] (current, minimum),
+        data [N, 585, 2], past data [N, 585, 2]) with entries of scales 0..3 (block.hpp:66-166)."""
+        n = C.c_int64()
+        capi.check(self.lib.se8_dump_blocks_multires(self.h, None, None, None, None, 0, C.byref(n)),
+                   "se8_dump_blocks_multires")
+        N = n.value
+        keys = np.zeros(max(N, 1), np.uint64)
+        sc = np.zeros((max(N, 1), 2), np.int32)
+        data = np.zeros((max(N, 1), 585, 2), np.float32)
+        past = np.zeros((max(N, 1), 585, 2), np.float32)
+        capi.check(self.lib.se8_dump_blocks_multires(self.h, keys.ctypes.data_as(C.POINTER(C.c_uint64)), capi.iptr(sc),
+                                                     capi.fptr(data), capi.fptr(past), N, C.byref(n)),
+                   "se8_dump_blocks_multires")
+        return keys[:N], sc[:N], data[:N], past[:N]
 
     def dump_nodes(self):
         n = C.c_int64()
