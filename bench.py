@@ -258,6 +258,58 @@ def run_pipeline(depth_dev, gt, n_frames, warmup, storage, compensated, torch, s
     }
 
 
+MR_RES = 0.004
+MR_STEP = 3
+
+
+def run_multires(depth_dev, gt, n_frames, warmup, torch, se, sensor):
+    """Multi-resolution TSDF integration (SE8_STORAGE_MULTIRES, 4 mm voxels, where the 1.5
+    pixel/voxel threshold falls at 3.15 m inside the room) of every third frame of the workload at
+    its ground-truth pose: frames/s and the mean per-frame stage times from the library's events."""
+    gmap = se.TSDFMap(10.24, MR_RES, storage="multires", max_blocks=1 << 20)
+    integ = se.MapIntegrator(gmap)
+    idx = list(range(0, n_frames, MR_STEP))
+    torch.cuda.synchronize()
+    t0 = None
+    for j, k in enumerate(idx):
+        if j == warmup:
+            torch.cuda.synchronize()
+            se.stage_times_reset(gmap, None)
+            t0 = time.perf_counter()
+        integ.integrateDepth(j, depth_dev[k], sensor, gt[k].astype(np.float32))
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    st = se.stage_times(gmap, None)
+    ni = max(st["n_integrations"], 1)
+    _, sc, _, _ = gmap.dump_blocks_multires()
+    out = {"frames": len(idx) - warmup, "fps": (len(idx) - warmup) / dt, "blocks": int(len(sc)),
+           "blocks_per_scale": np.bincount(sc[:, 0] + 1, minlength=5)[1:].tolist(),
+           "stage_ms": {"fetch_frustum": st["sum_fetch_frustum"] / ni, "create_list": st["sum_create_list"] / ni,
+                        "allocate_list": st["sum_allocate_list"] / ni, "update": st["sum_update"] / ni},
+           "update_blocks_per_frame": st["sum_update_blocks"] / ni,
+           "workload": f"every {MR_STEP}rd frame of the tracking sequence at its ground-truth pose, 10.24 m map at "
+                       f"{MR_RES * 1000:.0f} mm, multi-resolution TSDF (scales 0-3)"}
+    gmap.close()
+    return out
+
+
+def cpu_baseline_multires(depth_host, gt, sensor, n=4):
+    """The oracle's multi-resolution updater (OpenMP) on frames 0, 3, .. of the same sequence."""
+    from oracle import oracle as O
+
+    O.set_threads(os.cpu_count() or 1)
+    pin = O.pinhole(sensor.width, sensor.height, sensor.fx, sensor.fy, sensor.cx, sensor.cy, sensor.near_plane,
+                    sensor.far_plane)
+    m = O.OracleMap(10.24, MR_RES, multires=True)
+    t0 = None
+    for j in range(n):
+        if j == 1:
+            t0 = time.perf_counter()
+        k = j * MR_STEP
+        m.integrate(j, depth_host[k], pin, gt[k].astype(np.float32))
+    return (n - 1) / (time.perf_counter() - t0)
+
+
 def cpu_baseline(depth_host, gt, n_frames, sensor):
     """The oracle (C restatement of the reference, OpenMP) on a bounded prefix of the workload."""
     from oracle import oracle as O
@@ -469,6 +521,12 @@ def main():
             tr = json.load(f)
         traffic = tr["traffic_bytes_per_launch"] * r32["update_blocks_per_frame"] / tr["update_blocks_per_frame"]
 
+    print("multi-resolution TSDF", file=sys.stderr, flush=True)
+    rmr = run_multires(depth_dev, gt, n_frames, 5, torch, se, sensor)
+    if not args.no_cpu_baseline:
+        rmr["cpu_fps"] = cpu_baseline_multires(depth_dev[: 3 * MR_STEP + 1].cpu().numpy(), gt, sensor)
+        rmr["cpu_threads"] = os.cpu_count() or 1
+
     cpu = None
     if not args.no_cpu_baseline:
         print("cpu baseline", file=sys.stderr, flush=True)
@@ -512,6 +570,7 @@ def main():
         "cpu_baseline": cpu,
         "mesh": dict(r32["mesh"], blocks=r32["blocks"],
                      note="Map::mesh (single-res marching cubes) of the final FP32 map, after the timed loop"),
+        "multires": rmr,
         "downsample": dict(input_bench(depth_dev, torch, se),
                            note="downsample_depth fused with the uint16 mm -> m conversion, index map written; "
                                 "kernel time from a replayed HIP graph of 100 launches"),
