@@ -546,8 +546,29 @@ enum Counter {
     kCntUpdate = 11,
     kCntCandidates = 12,
     kCntUnique = 13,
-    kNumCounters = 16
+    kCntMrNew = 14,  // multi-resolution update: blocks allocated this frame
+    kCntMrDown = 15, // multi-resolution update: blocks moving to a finer scale
+    kCntMrScale = 16, // 4 counters: multi-resolution update list entries at scale 0..3
+    kNumCounters = 20
 };
+
+// Wave-aggregated append: every lane with pred gets the next free position of *counter (one
+// atomic per wave); -1 for the other lanes.
+__device__ __forceinline__ int wave_append_slot(bool pred, int* counter)
+{
+    const uint64_t mask = __ballot(pred);
+    if (mask == 0)
+        return -1;
+    const int lane = __lane_id();
+    const int leader = __ffsll((unsigned long long) mask) - 1;
+    int base = 0;
+    if (lane == leader)
+        base = atomicAdd(counter, __popcll(mask));
+    base = __shfl(base, leader);
+    if (!pred)
+        return -1;
+    return base + __popcll(mask & ((1ull << lane) - 1ull));
+}
 
 // Block slot of the block containing voxel (x,y,z); -1 if unallocated.  Coordinates are taken
 // modulo the octree size, as the reference's top-down fetcher ignores bits at and above the

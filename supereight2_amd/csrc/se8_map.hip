@@ -147,6 +147,10 @@ __global__ void k_frame_begin(MapView m, Iso T_arg, const float* T16_dev, FrameP
         m.counters[kCntUpdate] = 0;
         m.counters[kCntCandidates] = 0;
         m.counters[kCntUnique] = 0;
+        m.counters[kCntMrNew] = 0;
+        m.counters[kCntMrDown] = 0;
+        for (int i = 0; i < 4; i++)
+            m.counters[kCntMrScale + i] = 0;
         const Iso T = T16_dev ? iso_from16(T16_dev) : T_arg;
         fp->T_WS = T;
         fp->T_SW = iso_inv(T);
@@ -162,25 +166,6 @@ static int fill_i32(DeviceBuffer& b, int64_t n, int v, hipStream_t s)
     k_fill_i32<<<(unsigned) blocks, 256, 0, s>>>((int*) b.ptr, n, v);
     SE8_HIP_CHECK(hipGetLastError());
     return SE8_OK;
-}
-
-// ----------------------------------------------------------------------------------------------
-// wave-aggregated append: every lane with pred appends one value, one atomic per wave
-// ----------------------------------------------------------------------------------------------
-__device__ __forceinline__ int wave_append_slot(bool pred, int* counter)
-{
-    const uint64_t mask = __ballot(pred);
-    if (mask == 0)
-        return -1;
-    const int lane = __lane_id();
-    const int leader = __ffsll((unsigned long long) mask) - 1;
-    int base = 0;
-    if (lane == leader)
-        base = atomicAdd(counter, __popcll(mask));
-    base = __shfl(base, leader);
-    if (!pred)
-        return -1;
-    return base + __popcll(mask & ((1ull << lane) - 1ull));
 }
 
 // ----------------------------------------------------------------------------------------------

@@ -16,12 +16,16 @@
 // neighbour block that another thread is updating in the same frame, so its result depends on
 // the schedule.  This implementation fixes one legal schedule: every block's scale choice and
 // down-propagation read the map as it stood before the frame's updates (the new scales are
-// committed afterwards), then every block updates and propagates up.  Three launches, one
-// workgroup per list entry at a time (grid-stride):
-//   k_mr_init_new      default data, past data and scales of the blocks allocated this frame
-//   k_mr_scale_down    integration scale (kept aside), down-propagation level by level
-//   k_mr_update_up     commit the scale, update the voxels of that scale, up-propagation,
-//                      block and parent timestamps
+// committed afterwards), then every block updates and propagates up.  Four launches:
+//   k_mr_scales        one thread per list entry: the integration scale (kept aside), and the
+//                      entries of new blocks and of blocks moving to a finer scale appended to two
+//                      short lists
+//   k_mr_init_new      one workgroup per new block: default data, past data and scales
+//   k_mr_down          one workgroup per block moving finer: down-propagation level by level
+//   k_mr_update_up     one workgroup per list entry: the entries of the block's current and coarser
+//                      scales (a contiguous suffix of its data and of its past data) staged in LDS,
+//                      the scale committed, the voxels of that scale updated, the coarser scales
+//                      recomputed up, the suffix written back; block and parent timestamps
 #include "se8_internal.hpp"
 
 #include <algorithm>
@@ -131,17 +135,77 @@ __device__ bool field_interp_mr(const MapView& m, const float cf[3], int desired
     return false;
 }
 
-__global__ __launch_bounds__(kMrThreads) void k_mr_init_new(MapView m, const int* update_list)
+// The six lists of k_mr_scales: new blocks, blocks moving finer, entries of scale 0..3; their
+// counters are consecutive from kCntMrNew.
+constexpr int kMrLists = 6;
+
+__global__ __launch_bounds__(256) void k_mr_scales(MapView m, const int* update_list, const FramePoses* fp,
+                                                   float scaled_pixel, int* new_scale, int* lists, int list_cap)
 {
+    __shared__ int s_cnt[kMrLists], s_base[kMrLists];
     const int n = m.counters[kCntUpdate];
-    for (int e = blockIdx.x; e < n; e += gridDim.x) {
-        const int entry = update_list[e];
-        if (entry >= 0)
-            continue; // blocks from earlier frames keep their data
-        const int slot = entry & 0x7fffffff;
+    const int lane = __lane_id();
+    const int iters = (n + (int) (gridDim.x * blockDim.x) - 1) / (int) (gridDim.x * blockDim.x);
+    for (int it = 0; it < iters; it++) {
+        const int e = (it * (int) gridDim.x + (int) blockIdx.x) * (int) blockDim.x + (int) threadIdx.x;
+        bool is_new = false, finer = false;
+        int s = -1;
+        if (e < n) {
+            const int entry = update_list[e];
+            const int slot = entry & 0x7fffffff;
+            is_new = entry < 0;
+            int bc[3];
+            key_to_coord(m.block_key[slot], bc);
+            // block centre: T_WM * ((coord + 8 * 0.5) * res), in the camera frame
+            float cW[3], cC[3];
+            for (int i = 0; i < 3; i++)
+                cW[i] = ((float) bc[i] + 4.0f) * m.res - m.t_MW[i];
+            iso_apply(fp->T_SW, cW, cC);
+            const int last = is_new ? -1 : m.block_scale[2 * slot];
+            const int mins = is_new ? -1 : m.block_scale[2 * slot + 1];
+            s = max(integration_scale(cC[2], scaled_pixel, m.res, last, mins), last - 1);
+            new_scale[e] = s;
+            finer = s < last;
+        }
+        const bool in_list[kMrLists] = {is_new, finer, s == 0, s == 1, s == 2, s == 3};
+        // positions: lane rank within the wave, wave offset within the workgroup (LDS atomics),
+        // workgroup base (one global atomic per list and workgroup)
+        if (threadIdx.x < kMrLists)
+            s_cnt[threadIdx.x] = 0;
+        __syncthreads();
+        int pos[kMrLists];
+#pragma unroll
+        for (int j = 0; j < kMrLists; j++) {
+            const uint64_t mask = __ballot(in_list[j]);
+            int off = 0;
+            if (mask != 0) {
+                const int leader = __ffsll((unsigned long long) mask) - 1;
+                if (lane == leader)
+                    off = atomicAdd(&s_cnt[j], __popcll(mask));
+                off = __shfl(off, leader);
+            }
+            pos[j] = off + __popcll(mask & ((1ull << lane) - 1ull));
+        }
+        __syncthreads();
+        if (threadIdx.x < kMrLists)
+            s_base[threadIdx.x] = s_cnt[threadIdx.x] > 0 ? atomicAdd(&m.counters[kCntMrNew + threadIdx.x], s_cnt[threadIdx.x]) : 0;
+        __syncthreads();
+#pragma unroll
+        for (int j = 0; j < kMrLists; j++)
+            if (in_list[j])
+                lists[(size_t) j * list_cap + s_base[j] + pos[j]] = e;
+        __syncthreads(); // s_cnt and s_base are reused by the next iteration
+    }
+}
+
+__global__ __launch_bounds__(kMrThreads) void k_mr_init_new(MapView m, const int* update_list, const int* new_list)
+{
+    const int n = m.counters[kCntMrNew];
+    for (int i = blockIdx.x; i < n; i += gridDim.x) {
+        const int slot = update_list[new_list[i]] & 0x7fffffff;
         float2* blk = mr_block(m, slot);
-        for (int i = threadIdx.x; i < kMrBlockFloat2; i += kMrThreads)
-            blk[i] = make_float2(1.f, 0.f);
+        for (int j = threadIdx.x; j < kMrBlockFloat2; j += kMrThreads)
+            blk[j] = make_float2(1.f, 0.f);
         if (threadIdx.x == 0) {
             m.block_scale[2 * slot] = -1;
             m.block_scale[2 * slot + 1] = -1;
@@ -149,39 +213,23 @@ __global__ __launch_bounds__(kMrThreads) void k_mr_init_new(MapView m, const int
     }
 }
 
-__global__ __launch_bounds__(kMrThreads) void k_mr_scale_down(MapView m, const int* update_list, const FramePoses* fp,
-                                                               float scaled_pixel, int* new_scale)
+__global__ __launch_bounds__(kMrThreads) void k_mr_down(MapView m, const int* update_list, const int* down_list,
+                                                         const int* new_scale)
 {
-    __shared__ int s_scale[2];
-    const int n = m.counters[kCntUpdate];
-    for (int e = blockIdx.x; e < n; e += gridDim.x) {
+    const int n = m.counters[kCntMrDown];
+    for (int i = blockIdx.x; i < n; i += gridDim.x) {
+        const int e = down_list[i];
         const int slot = update_list[e] & 0x7fffffff;
+        const int last = m.block_scale[2 * slot], curr = new_scale[e];
         int bc[3];
         key_to_coord(m.block_key[slot], bc);
-        __syncthreads(); // s_scale of the previous entry is no longer read
-        if (threadIdx.x == 0) {
-            // block centre: T_WM * ((coord + 8 * 0.5) * res), in the camera frame
-            float cW[3], cC[3];
-            for (int i = 0; i < 3; i++)
-                cW[i] = ((float) bc[i] + 4.0f) * m.res - m.t_MW[i];
-            iso_apply(fp->T_SW, cW, cC);
-            const int last = m.block_scale[2 * slot], mins = m.block_scale[2 * slot + 1];
-            const int s = max(integration_scale(cC[2], scaled_pixel, m.res, last, mins), last - 1);
-            new_scale[e] = s;
-            s_scale[0] = last;
-            s_scale[1] = s;
-        }
-        __syncthreads();
-        const int last = s_scale[0], curr = s_scale[1];
-        if (curr >= last)
-            continue;
         float2* blk = mr_block(m, slot);
         float2* past = blk + kMrEntries;
         for (int ps = last; ps > curr; ps--) {
             const int cs = ps - 1, cn = 8 >> cs, cstride = 1 << cs;
             // children of every parent of this level
-            for (int i = threadIdx.x; i < cn * cn * cn; i += kMrThreads) {
-                const int lx = (i % cn) * cstride, ly = ((i / cn) % cn) * cstride, lz = (i / (cn * cn)) * cstride;
+            for (int j = threadIdx.x; j < cn * cn * cn; j += kMrThreads) {
+                const int lx = (j % cn) * cstride, ly = ((j / cn) % cn) * cstride, lz = (j / (cn * cn)) * cstride;
                 const int ci = mr_index(lx, ly, lz, cs), pi = mr_index(lx, ly, lz, ps);
                 const float2 p = blk[pi], pp = past[pi];
                 float2 c = blk[ci], cp = past[ci];
@@ -209,8 +257,8 @@ __global__ __launch_bounds__(kMrThreads) void k_mr_scale_down(MapView m, const i
             __syncthreads();
             // parent_down_funct: the parent's past data restarts from its current value
             const int pn = 8 >> ps;
-            for (int i = threadIdx.x; i < pn * pn * pn; i += kMrThreads) {
-                const int pi = mr_offset(ps) + i;
+            for (int j = threadIdx.x; j < pn * pn * pn; j += kMrThreads) {
+                const int pi = mr_offset(ps) + j;
                 past[pi] = make_float2(blk[pi].x, 0.f);
             }
             __syncthreads();
@@ -218,74 +266,106 @@ __global__ __launch_bounds__(kMrThreads) void k_mr_scale_down(MapView m, const i
     }
 }
 
-__global__ __launch_bounds__(kMrThreads) void k_mr_update_up(MapView m, const int* update_list, const FramePoses* fp,
-                                                              PinholeSensor sensor, const float* depth, int ts,
-                                                              const int* new_scale)
+// One list entry of scale S per workgroup of T threads: the suffix of entries of scales S..3 (data
+// and past data) in LDS, the (8 >> S)^3 voxels of scale S updated (all depth samples of a thread in
+// flight together), the coarser scales recomputed up, the suffix written back.
+template <int S, int T>
+__global__ __launch_bounds__(T) void k_mr_update_up(MapView m, const int* update_list, const int* list,
+                                                    const FramePoses* fp, PinholeSensor sensor, const float* depth,
+                                                    int ts)
 {
+    constexpr int kOff = S == 0 ? 0 : (S == 1 ? 512 : (S == 2 ? 576 : 584));
+    constexpr int kLen = kMrEntries - kOff;
+    constexpr int kSn = 8 >> S, kVox = kSn * kSn * kSn, kStride = 1 << S;
+    constexpr int kPer = (kVox + T - 1) / T;
+    __shared__ float2 s_data[kLen], s_past[kLen];
     const Iso T_CV = fp->T_CV;
-    const int n = m.counters[kCntUpdate];
-    for (int e = blockIdx.x; e < n; e += gridDim.x) {
+    const int n = m.counters[kCntMrScale + S];
+    for (int li = blockIdx.x; li < n; li += gridDim.x) {
+        const int e = list[li];
         const int slot = update_list[e] & 0x7fffffff;
-        const int curr = new_scale[e];
         int bc[3];
         key_to_coord(m.block_key[slot], bc);
-        __syncthreads(); // the previous entry's block is complete
+        float2* blk = mr_block(m, slot);
+        float2* past = blk + kMrEntries;
+        __syncthreads(); // the previous entry's suffix has been written back
+        for (int i = threadIdx.x; i < kLen; i += T) {
+            s_data[i] = blk[kOff + i];
+            s_past[i] = past[kOff + i];
+        }
         if (threadIdx.x == 0) {
             m.block_ts[slot] = ts;
             atomicMax(&m.node_ts[m.block_parent[slot]], ts);
             const int mins = m.block_scale[2 * slot + 1];
-            m.block_scale[2 * slot] = curr;
-            m.block_scale[2 * slot + 1] = mins < 0 ? curr : min(mins, curr);
+            m.block_scale[2 * slot] = S;
+            m.block_scale[2 * slot + 1] = mins < 0 ? S : min(mins, S);
         }
-        float2* blk = mr_block(m, slot);
-        float2* past = blk + kMrEntries;
-        const int sn = 8 >> curr, stride = 1 << curr;
-        for (int i = threadIdx.x; i < sn * sn * sn; i += kMrThreads) {
-            const int lx = (i % sn) * stride, ly = ((i / sn) % sn) * stride, lz = (i / (sn * sn)) * stride;
-            const float v[3] = {(float) (bc[0] + lx), (float) (bc[1] + ly), (float) (bc[2] + lz)};
-            float pC[3];
-            iso_apply(T_CV, v, pC);
-            const float nrm = sqrtf(sqnorm3(pC));
-            float uv[2];
-            if (sensor.beyond_far(pC, nrm) || !sensor.project(pC, uv))
+        float d[kPer], mz[kPer], nrm[kPer];
+        bool ok[kPer];
+#pragma unroll
+        for (int h = 0; h < kPer; h++) {
+            const int i = threadIdx.x + h * T;
+            ok[h] = false;
+            d[h] = mz[h] = nrm[h] = 0.f;
+            if (i < kVox) {
+                const int lx = (i % kSn) * kStride, ly = ((i / kSn) % kSn) * kStride, lz = (i / (kSn * kSn)) * kStride;
+                const float v[3] = {(float) (bc[0] + lx), (float) (bc[1] + ly), (float) (bc[2] + lz)};
+                float pC[3];
+                iso_apply(T_CV, v, pC);
+                nrm[h] = sqrtf(sqnorm3(pC));
+                float uv[2];
+                if (!sensor.beyond_far(pC, nrm[h]) && sensor.project(pC, uv)) {
+                    const int px = (int) (uv[0] + 0.5f), py = (int) (uv[1] + 0.5f);
+                    d[h] = depth[px + (size_t) py * sensor.width()];
+                    mz[h] = sensor.measurement(pC);
+                    ok[h] = true;
+                }
+            }
+        }
+        __syncthreads(); // the suffix is in LDS
+#pragma unroll
+        for (int h = 0; h < kPer; h++) {
+            const int i = threadIdx.x + h * T;
+            if (!ok[h] || d[h] < sensor.near_plane())
                 continue;
-            const int px = (int) (uv[0] + 0.5f), py = (int) (uv[1] + 0.5f);
-            const float d = depth[px + (size_t) py * sensor.width()];
-            if (d < sensor.near_plane())
-                continue;
-            const int vi = mr_index(lx, ly, lz, curr);
-            float2 c = blk[vi];
-            tsdf_update_voxel(m, d, true, sensor.near_plane(), sensor.measurement(pC), nrm, c.x, c.y);
-            blk[vi] = c;
-            past[vi].y = past[vi].y + 1.f;
+            // the entries of scale S are the suffix's first kVox, in the same x-fastest order
+            float2 c = s_data[i];
+            tsdf_update_voxel(m, d[h], true, sensor.near_plane(), mz[h], nrm[h], c.x, c.y);
+            s_data[i] = c;
+            s_past[i].y = s_past[i].y + 1.f;
         }
         __syncthreads();
         // up-propagation: the mean of the valid children, level by level
-        for (int cs = curr; cs < 3; cs++) {
+#pragma unroll
+        for (int cs = S; cs < 3; cs++) {
             const int ps = cs + 1, pn = 8 >> ps, cstride = 1 << cs;
-            for (int i = threadIdx.x; i < pn * pn * pn; i += kMrThreads) {
+            for (int i = threadIdx.x; i < pn * pn * pn; i += T) {
                 const int px0 = (i % pn) << ps, py0 = ((i / pn) % pn) << ps, pz0 = (i / (pn * pn)) << ps;
                 float ts_sum = 0.f, w_sum = 0.f;
                 int nv = 0;
                 for (int k = 0; k < 8; k++) {
-                    const float2 c = blk[mr_index(px0 + cstride * (k & 1), py0 + cstride * ((k >> 1) & 1),
-                                                  pz0 + cstride * ((k >> 2) & 1), cs)];
+                    const float2 c = s_data[mr_index(px0 + cstride * (k & 1), py0 + cstride * ((k >> 1) & 1),
+                                                     pz0 + cstride * ((k >> 2) & 1), cs) - kOff];
                     if (c.y != 0.f) {
                         ts_sum += c.x;
                         w_sum += c.y;
                         nv++;
                     }
                 }
-                const int pi = mr_index(px0, py0, pz0, ps);
+                const int pi = mr_index(px0, py0, pz0, ps) - kOff;
                 if (nv > 0) {
                     const float t = ts_sum / (float) nv;
-                    blk[pi] = make_float2(t, ceilf(w_sum / (float) nv));
-                    past[pi] = make_float2(t, 0.f);
+                    s_data[pi] = make_float2(t, ceilf(w_sum / (float) nv));
+                    s_past[pi] = make_float2(t, 0.f);
                 }
                 else
-                    past[pi] = make_float2(1.f, 0.f);
+                    s_past[pi] = make_float2(1.f, 0.f);
             }
             __syncthreads();
+        }
+        for (int i = threadIdx.x; i < kLen; i += T) {
+            blk[kOff + i] = s_data[i];
+            past[kOff + i] = s_past[i];
         }
     }
 }
@@ -295,7 +375,9 @@ __global__ __launch_bounds__(kMrThreads) void k_mr_update_up(MapView m, const in
 int se8::launch_multires_update(se8_map* m, const FramePoses* fp, const PinholeSensor& sensor, const float* depth,
                                 int ts, hipStream_t s)
 {
-    int rc = dev_alloc(m->mr_new_scale, sizeof(int) * (size_t) m->view.max_blocks);
+    // per list entry: the new scale; the lists of new blocks, of blocks moving finer, and of the
+    // entries of each scale
+    int rc = dev_alloc(m->mr_new_scale, sizeof(int) * (1 + kMrLists) * (size_t) m->view.max_blocks);
     if (rc != SE8_OK)
         return rc;
     static int cus = 0;
@@ -306,10 +388,20 @@ int se8::launch_multires_update(se8_map* m, const FramePoses* fp, const PinholeS
     }
     const int* ulist = (const int*) m->update_list.ptr;
     int* ns = (int*) m->mr_new_scale.ptr;
+    int* new_list = ns + m->view.max_blocks;
+    int* down_list = new_list + m->view.max_blocks;
+    int* scale_lists = down_list + m->view.max_blocks;
+    const int cap = (int) m->view.max_blocks;
     const unsigned grid = (unsigned) (cus * 8);
-    k_mr_init_new<<<grid, kMrThreads, 0, s>>>(m->view, ulist);
-    k_mr_scale_down<<<grid, kMrThreads, 0, s>>>(m->view, ulist, fp, sensor.c.inv_fx, ns);
-    k_mr_update_up<<<grid, kMrThreads, 0, s>>>(m->view, ulist, fp, sensor, depth, ts, ns);
+    k_mr_scales<<<grid, 256, 0, s>>>(m->view, ulist, fp, sensor.c.inv_fx, ns, new_list, cap);
+    k_mr_init_new<<<grid, kMrThreads, 0, s>>>(m->view, ulist, new_list);
+    k_mr_down<<<grid, kMrThreads, 0, s>>>(m->view, ulist, down_list, ns);
+    // scale 0: 512 voxels per block, 256 threads; coarser scales: one wave per block, 4x as many
+    // blocks in flight per CU
+    k_mr_update_up<0, 256><<<grid, 256, 0, s>>>(m->view, ulist, scale_lists, fp, sensor, depth, ts);
+    k_mr_update_up<1, 64><<<grid * 4, 64, 0, s>>>(m->view, ulist, scale_lists + cap, fp, sensor, depth, ts);
+    k_mr_update_up<2, 64><<<grid * 4, 64, 0, s>>>(m->view, ulist, scale_lists + 2 * (size_t) cap, fp, sensor, depth, ts);
+    k_mr_update_up<3, 64><<<grid * 4, 64, 0, s>>>(m->view, ulist, scale_lists + 3 * (size_t) cap, fp, sensor, depth, ts);
     SE8_HIP_CHECK(hipGetLastError());
     return SE8_OK;
 }
