@@ -696,6 +696,7 @@ struct se8_map {
     se8::RadixSorter sorter;
     size_t candidates_cap = 0;
     int* host_counters = nullptr; // pinned copy of counters taken at the end of each integration
+    int* host_counters_dev = nullptr; // the same memory as the device addresses it
     bool counters_pending = false; // a copy into host_counters is in flight (event ev[7])
     bool overflowed = false;       // a capacity overflow was reported: integration is refused from then on
     bool integ_pending = false;    // events ev[0..4] of the last integration not yet harvested
