@@ -613,6 +613,23 @@ __device__ __forceinline__ void load_voxel(const MapView& m, int slot, int vi, f
     }
 }
 
+// load_voxel for a storage fixed at compile time: kernels that load many voxels per thread are
+// instantiated per storage so that no load sits behind a branch on the storage type.
+template <int ST>
+__device__ __forceinline__ void load_voxel_t(const MapView& m, int slot, int vi, float& tsdf, float& weight)
+{
+    static_assert(ST == SE8_STORAGE_F32 || ST == SE8_STORAGE_F16, "single-resolution storage");
+    if constexpr (ST == SE8_STORAGE_F32) {
+        const float2 v = reinterpret_cast<const float2*>(m.voxels)[(size_t) slot * 512 + vi];
+        tsdf = v.x;
+        weight = v.y;
+    }
+    else {
+        const uint32_t v = reinterpret_cast<const uint32_t*>(m.voxels)[(size_t) slot * 512 + vi];
+        unpack_f16(v, tsdf, weight);
+    }
+}
+
 // getData semantics: data of the voxel if its block is allocated, otherwise {1, 0}.
 __device__ __forceinline__ void get_voxel(const MapView& m, int x, int y, int z, float& tsdf, float& weight)
 {

@@ -207,6 +207,7 @@ __device__ __forceinline__ bool point_to_voxel_f(const MapView& m, const float p
 }
 
 // Map::getData<Safe::On>
+template <int ST>
 __device__ __forceinline__ void get_data(const MapView& m, const float p[3], float& tsdf, float& weight)
 {
     int v[3];
@@ -215,12 +216,19 @@ __device__ __forceinline__ void get_data(const MapView& m, const float p[3], flo
         weight = 0.f;
         return;
     }
-    get_voxel(m, v[0], v[1], v[2], tsdf, weight);
+    const int slot = lookup_block(m, v[0], v[1], v[2]);
+    if (slot < 0) {
+        tsdf = 1.f;
+        weight = 0.f;
+        return;
+    }
+    load_voxel_t<ST>(m, slot, (v[0] & 7) + ((v[1] & 7) << 3) + ((v[2] & 7) << 6), tsdf, weight);
 }
 
 // Field values of the 8 voxels base + {0,1}^3 (x fastest) and whether all are allocated and
 // observed (weight != 0).  Every block lookup and voxel load is issued before any result is
 // tested, so the set costs the round trips of one lookup, not eight in sequence.
+template <int ST>
 __device__ __forceinline__ bool gather_corners(const MapView& m, const int base[3], float d[8])
 {
     int slot[8];
@@ -232,7 +240,7 @@ __device__ __forceinline__ bool gather_corners(const MapView& m, const int base[
     for (int k = 0; k < 8; k++) {
         const int x = base[0] + (k & 1), y = base[1] + ((k >> 1) & 1), z = base[2] + ((k >> 2) & 1);
         float w;
-        load_voxel(m, max(slot[k], 0), (x & 7) + ((y & 7) << 3) + ((z & 7) << 6), d[k], w);
+        load_voxel_t<ST>(m, max(slot[k], 0), (x & 7) + ((y & 7) << 3) + ((z & 7) << 6), d[k], w);
         ok = ok && slot[k] >= 0 && w != 0.f;
     }
     return ok;
@@ -245,6 +253,7 @@ struct SlotCache {
     int slot = -1;
 };
 
+template <int ST>
 __device__ __forceinline__ void get_data_cached(const MapView& m, const float p[3], float& tsdf, float& weight,
                                                 SlotCache& cache)
 {
@@ -266,11 +275,12 @@ __device__ __forceinline__ void get_data_cached(const MapView& m, const float p[
         weight = 0.f;
         return;
     }
-    load_voxel(m, cache.slot, (v[0] & 7) + ((v[1] & 7) << 3) + ((v[2] & 7) << 6), tsdf, weight);
+    load_voxel_t<ST>(m, cache.slot, (v[0] & 7) + ((v[1] & 7) << 3) + ((v[2] & 7) << 6), tsdf, weight);
 }
 
 // single-res getInterp (visitor_impl.hpp:976-1024) from voxel coordinates.  The reference's
 // octree-AABB test is implied by the requirement that the block of base be allocated.
+template <int ST>
 __device__ bool field_interp_vox(const MapView& m, const float vf[3], float& out)
 {
     float bf[3];
@@ -285,7 +295,7 @@ __device__ bool field_interp_vox(const MapView& m, const float vf[3], float& out
         if (base[i] + 1 >= m.size || base[i] < 0)
             return false;
     float d[8];
-    if (!gather_corners(m, base, d))
+    if (!gather_corners<ST>(m, base, d))
         return false;
     float t[3], tcm[3];
 #pragma unroll
@@ -300,6 +310,7 @@ __device__ bool field_interp_vox(const MapView& m, const float vf[3], float& out
 
 // single-res getFieldGrad (visitor_impl.hpp:1094-1251): trilinear blend of the central differences
 // at the 8 voxels around the query; x0.5 (not yet divided by res).
+template <int ST>
 __device__ bool field_grad_vox(const MapView& m, const float vf[3], float g[3])
 {
     float fac[3];
@@ -316,34 +327,57 @@ __device__ bool field_grad_vox(const MapView& m, const float vf[3], float g[3])
         c[i][3] = min(base[i] + 2, m.size - 1);
     }
     // The central differences read the 32 voxels c[0][i], c[1][j], c[2][k] of the 4x4x4
-    // neighbourhood with at most one of i, j, k on an outer layer (0 or 3).  All 32 lookups and
-    // loads are issued before any is tested; the result is undefined if any is invalid or the
-    // block of base is missing, as in the reference.
-    constexpr int kOuter = 0x9; // bit i set: index i is an outer layer
-    int slot[4][4][4];
+    // neighbourhood with at most one of i, j, k on an outer layer (0 or 3).  The 4 coordinates of
+    // an axis are consecutive (or clamped) voxels, so they lie in at most two blocks: hi[a][n]
+    // says whether c[a][n] lies in the block of c[a][3] rather than that of c[a][0].  The slots of
+    // those 2x2x2 blocks are looked up once and each voxel selects its own.  The result is
+    // undefined if any of the 32 voxels is invalid or the block of base is missing, as in the
+    // reference.
+    bool hi[3][4];
 #pragma unroll
-    for (int i = 0; i < 4; i++)
+    for (int a = 0; a < 3; a++)
 #pragma unroll
-        for (int j = 0; j < 4; j++)
+        for (int n = 0; n < 4; n++)
+            hi[a][n] = (c[a][n] >> 3) != (c[a][0] >> 3);
+    int bs[8];
 #pragma unroll
-            for (int k = 0; k < 4; k++)
-                if (((kOuter >> i) & 1) + ((kOuter >> j) & 1) + ((kOuter >> k) & 1) <= 1)
-                    slot[i][j][k] = lookup_block_nb(m, c[0][i], c[1][j], c[2][k]);
+    for (int k = 0; k < 8; k++)
+        bs[k] = lookup_block_nb(m, c[0][(k & 1) ? 3 : 0], c[1][(k & 2) ? 3 : 0], c[2][(k & 4) ? 3 : 0]);
     const int base_slot = lookup_block_nb(m, base[0], base[1], base[2]);
+    constexpr int kOuter = 0x9; // bit i set: index i is an outer layer
     float F[4][4][4];
     bool ok = base_slot >= 0;
+    // the voxels of one (y, z) column: all four x for an inner (y, z), the inner two otherwise
+    auto load_column = [&](int j, int k) {
+        const int lo = hi[2][k] ? (hi[1][j] ? bs[6] : bs[4]) : (hi[1][j] ? bs[2] : bs[0]);
+        const int up = hi[2][k] ? (hi[1][j] ? bs[7] : bs[5]) : (hi[1][j] ? bs[3] : bs[1]);
+        const bool inner = ((kOuter >> j) & 1) + ((kOuter >> k) & 1) == 0;
 #pragma unroll
-    for (int i = 0; i < 4; i++)
-#pragma unroll
-        for (int j = 0; j < 4; j++)
-#pragma unroll
-            for (int k = 0; k < 4; k++)
-                if (((kOuter >> i) & 1) + ((kOuter >> j) & 1) + ((kOuter >> k) & 1) <= 1) {
-                    const int s = slot[i][j][k];
-                    float w;
-                    load_voxel(m, max(s, 0), (c[0][i] & 7) + ((c[1][j] & 7) << 3) + ((c[2][k] & 7) << 6), F[i][j][k], w);
-                    ok = ok && s >= 0 && w != 0.f;
-                }
+        for (int i = 0; i < 4; i++)
+            if (inner || !((kOuter >> i) & 1)) {
+                const int sl = hi[0][i] ? up : lo;
+                float w;
+                load_voxel_t<ST>(m, max(sl, 0), (c[0][i] & 7) + ((c[1][j] & 7) << 3) + ((c[2][k] & 7) << 6),
+                                 F[i][j][k], w);
+                ok = ok && sl >= 0 && w != 0.f;
+            }
+    };
+    // three batches of loads (inner y and z, outer y, outer z) keep at most 16 voxel loads in
+    // flight, which holds the kernel under 96 VGPRs
+    load_column(1, 1);
+    load_column(2, 1);
+    load_column(1, 2);
+    load_column(2, 2);
+    __builtin_amdgcn_sched_barrier(0);
+    load_column(0, 1);
+    load_column(3, 1);
+    load_column(0, 2);
+    load_column(3, 2);
+    __builtin_amdgcn_sched_barrier(0);
+    load_column(1, 0);
+    load_column(2, 0);
+    load_column(1, 3);
+    load_column(2, 3);
     if (!ok)
         return false;
     const float rx = 1.f - fac[0], ry = 1.f - fac[1], rz = 1.f - fac[2];
@@ -372,6 +406,7 @@ __device__ bool field_grad_vox(const MapView& m, const float vf[3], float g[3])
 
 // raycast<TSDF> (raycaster_impl.hpp:456-533).  On a hit, bracket (if given) receives the ray
 // parameters of the last two samples, whose field values straddle zero.
+template <int ST>
 __device__ bool march(const MapView& m, const float o[3], const float d[3], float t_near, float t_far, int* st_parent,
                       float* st_tmax, int stride, float hit[3], float* bracket = nullptr)
 {
@@ -390,13 +425,13 @@ __device__ bool march(const MapView& m, const float o[3], const float d[3], floa
         p[i] = o[i] + d[i] * t;
     float f_t, w;
     SlotCache cache;
-    get_data_cached(m, p, f_t, w, cache);
+    get_data_cached<ST>(m, p, f_t, w, cache);
     float f_tt = 0.f;
     if (!(f_t >= 0.f))
         return false;
     for (; t < t_far; t += stepsize) {
         float f_new;
-        get_data_cached(m, p, f_new, w, cache);
+        get_data_cached<ST>(m, p, f_new, w, cache);
         if (w == 0.f) {
             stepsize = largestep;
 #pragma unroll
@@ -409,7 +444,7 @@ __device__ bool march(const MapView& m, const float o[3], const float d[3], floa
             float vf[3];
             if (point_to_voxel_f(m, p, vf)) {
                 float fi;
-                if (field_interp_vox(m, vf, fi))
+                if (field_interp_vox<ST>(m, vf, fi))
                     f_tt = fi;
             }
         }
@@ -439,6 +474,7 @@ __device__ bool march(const MapView& m, const float o[3], const float d[3], floa
 
 // Trilinear field value and its spatial gradient (per metre) at a world point; false where the
 // reference's interpolation is undefined (visitor_impl.hpp:976-1024).
+template <int ST>
 __device__ bool interp_and_grad(const MapView& m, const float pW[3], float& f, float g[3])
 {
     float vf[3];
@@ -454,7 +490,7 @@ __device__ bool interp_and_grad(const MapView& m, const float pW[3], float& f, f
             return false;
     }
     float d[8];
-    if (!gather_corners(m, base, d))
+    if (!gather_corners<ST>(m, base, d))
         return false;
 #pragma unroll
     for (int i = 0; i < 3; i++) {
@@ -473,6 +509,7 @@ __device__ bool interp_and_grad(const MapView& m, const float pW[3], float& f, f
 }
 
 // Root of the trilinear field on [ta, tb] (f(ta) >= 0 > f(tb)) by Illinois regula falsi.
+template <int ST>
 __device__ bool refine_root(const MapView& m, const float o[3], const float d[3], float ta, float tb, int iters,
                             float& t_out)
 {
@@ -480,12 +517,12 @@ __device__ bool refine_root(const MapView& m, const float o[3], const float d[3]
 #pragma unroll
     for (int i = 0; i < 3; i++)
         p[i] = o[i] + d[i] * ta;
-    if (!interp_and_grad(m, p, fa, g))
+    if (!interp_and_grad<ST>(m, p, fa, g))
         return false;
 #pragma unroll
     for (int i = 0; i < 3; i++)
         p[i] = o[i] + d[i] * tb;
-    if (!interp_and_grad(m, p, fb, g))
+    if (!interp_and_grad<ST>(m, p, fb, g))
         return false;
     if (!(fa >= 0.f && fb < 0.f))
         return false;
@@ -497,7 +534,7 @@ __device__ bool refine_root(const MapView& m, const float o[3], const float d[3]
 #pragma unroll
         for (int i = 0; i < 3; i++)
             p[i] = o[i] + d[i] * t;
-        if (!interp_and_grad(m, p, f, g))
+        if (!interp_and_grad<ST>(m, p, f, g))
             return false;
         if (f < 0.f) {
             tb = t;
@@ -522,6 +559,7 @@ __device__ bool refine_root(const MapView& m, const float o[3], const float d[3]
 
 // Batched differentiable depth: grid (tiles, poses).  d depth / d xi = dn_z * dt/dxi with
 // dt/dxi = -[g, x cross g] / (g . d) for the left twist xi = [v; w].
+template <int ST>
 __global__ __launch_bounds__(kRayThreads) void k_raycast_depth_grad(MapView m, Cam cam, const float* poses12,
                                                                      float* depth, float* grad, uint8_t* valid)
 {
@@ -548,15 +586,15 @@ __global__ __launch_bounds__(kRayThreads) void k_raycast_depth_grad(MapView m, C
     lin_apply(T, dn, dW);
     float out_d = 0.f, out_g[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
     uint8_t ok = 0;
-    if (march(m, T.t, dW, cam_near_dist(cam, dS), cam_far_dist(cam, dS), st_parent + t, st_tmax + t, kRayThreads, hit,
+    if (march<ST>(m, T.t, dW, cam_near_dist(cam, dS), cam_far_dist(cam, dS), st_parent + t, st_tmax + t, kRayThreads, hit,
               br)) {
         float tr;
-        if (refine_root(m, T.t, dW, br[0], br[1], 24, tr)) {
+        if (refine_root<ST>(m, T.t, dW, br[0], br[1], 24, tr)) {
             float xr[3], f, g[3];
 #pragma unroll
             for (int i = 0; i < 3; i++)
                 xr[i] = T.t[i] + dW[i] * tr;
-            if (interp_and_grad(m, xr, f, g)) {
+            if (interp_and_grad<ST>(m, xr, f, g)) {
                 const float gd = dot3(g, dW);
                 if (gd != 0.f) {
                     float xg[3];
@@ -579,6 +617,7 @@ __global__ __launch_bounds__(kRayThreads) void k_raycast_depth_grad(MapView m, C
     valid[idx] = ok;
 }
 
+template <int ST>
 __global__ __launch_bounds__(kRayThreads) void k_raycast(MapView m, Cam cam, Iso T_arg, const float* T16_dev,
                                                          float* points, float* normals, int8_t* scale_out)
 {
@@ -600,13 +639,13 @@ __global__ __launch_bounds__(kRayThreads) void k_raycast(MapView m, Cam cam, Iso
     lin_apply(T_WS, dn, dW);
     float hit[3];
     float nrm[3] = {0.f, 0.f, 0.f};
-    const bool ok = march(m, T_WS.t, dW, cam_near_dist(cam, dS), cam_far_dist(cam, dS), st_parent + t,
+    const bool ok = march<ST>(m, T_WS.t, dW, cam_near_dist(cam, dS), cam_far_dist(cam, dS), st_parent + t,
                           st_tmax + t, kRayThreads, hit);
     if (ok) {
         if (scale_out)
             scale_out[idx] = 0;
         float vf[3], g[3];
-        if (point_to_voxel_f(m, hit, vf) && field_grad_vox(m, vf, g)) {
+        if (point_to_voxel_f(m, hit, vf) && field_grad_vox<ST>(m, vf, g)) {
 #pragma unroll
             for (int i = 0; i < 3; i++)
                 g[i] = g[i] / m.res;
@@ -627,6 +666,7 @@ __global__ __launch_bounds__(kRayThreads) void k_raycast(MapView m, Cam cam, Iso
 // ----------------------------------------------------------------------------------------------
 // point queries
 // ----------------------------------------------------------------------------------------------
+template <int ST>
 __global__ void k_query(MapView m, const float* pts, int n, int mode, float* out, uint8_t* valid)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -635,7 +675,7 @@ __global__ void k_query(MapView m, const float* pts, int n, int mode, float* out
     const float p[3] = {pts[3 * i], pts[3 * i + 1], pts[3 * i + 2]};
     if (mode == 0) {
         float t, w;
-        get_data(m, p, t, w);
+        get_data<ST>(m, p, t, w);
         out[2 * i] = t;
         out[2 * i + 1] = w;
         return;
@@ -647,13 +687,13 @@ __global__ void k_query(MapView m, const float* pts, int n, int mode, float* out
     }
     if (mode == 1) {
         float f;
-        const bool ok = field_interp_vox(m, vf, f);
+        const bool ok = field_interp_vox<ST>(m, vf, f);
         valid[i] = ok;
         out[i] = ok ? f : 0.f;
     }
     else {
         float g[3] = {0.f, 0.f, 0.f};
-        const bool ok = field_grad_vox(m, vf, g);
+        const bool ok = field_grad_vox<ST>(m, vf, g);
         valid[i] = ok;
         for (int k = 0; k < 3; k++)
             out[3 * i + k] = ok ? g[k] / m.res : 0.f;
@@ -676,7 +716,8 @@ static int run_query(const se8_map* m, const float* pts, int64_t n, int mode, st
         if (hipMemcpy(dp.ptr, pts, sizeof(float) * 3 * n, hipMemcpyHostToDevice) != hipSuccess)
             rc = SE8_ERR_HIP;
         else {
-            k_query<<<(unsigned) ((n + 255) / 256), 256>>>(m->view, (const float*) dp.ptr, (int) n, mode,
+            auto* kq = m->view.storage == SE8_STORAGE_F32 ? k_query<SE8_STORAGE_F32> : k_query<SE8_STORAGE_F16>;
+            kq<<<(unsigned) ((n + 255) / 256), 256>>>(m->view, (const float*) dp.ptr, (int) n, mode,
                                                            (float*) dout.ptr, (uint8_t*) dval.ptr);
             if (hipMemcpy(out.data(), dout.ptr, sizeof(float) * width * n, hipMemcpyDeviceToHost) != hipSuccess
                 || hipMemcpy(valid.data(), dval.ptr, (size_t) n, hipMemcpyDeviceToHost) != hipSuccess)
@@ -697,7 +738,8 @@ int launch_raycast(const se8_map* m, const Cam& cam, const Iso& T_WS, const floa
                    float* normals, int8_t* scale, hipStream_t s, hipEvent_t ev_start, hipEvent_t ev_stop)
 {
     const dim3 grid((cam.w + 15) / 16, (cam.h + 7) / 8);
-    hipExtLaunchKernelGGL(k_raycast, grid, dim3(kRayThreads), (uint32_t) ray_stack_bytes(m->view), s, ev_start,
+    auto* kr = m->view.storage == SE8_STORAGE_F32 ? k_raycast<SE8_STORAGE_F32> : k_raycast<SE8_STORAGE_F16>;
+    hipExtLaunchKernelGGL(kr, grid, dim3(kRayThreads), (uint32_t) ray_stack_bytes(m->view), s, ev_start,
                           ev_stop, 0, m->view, cam, T_WS, T16_dev, points, normals, scale);
     SE8_HIP_CHECK(hipGetLastError());
     return SE8_OK;
@@ -809,7 +851,8 @@ extern "C" int se8_raycast_depth_grad_batched(const se8_map* m, const se8_pinhol
         return SE8_ERR_HIP;
     }
     dim3 grid((cam.w + 15) / 16, (cam.h + 7) / 8, n_poses);
-    k_raycast_depth_grad<<<grid, kRayThreads, ray_stack_bytes(m->view), s>>>(m->view, cam, (const float*) dp.ptr, depth_dev, grad_dev,
+    auto* kd = m->view.storage == SE8_STORAGE_F32 ? k_raycast_depth_grad<SE8_STORAGE_F32> : k_raycast_depth_grad<SE8_STORAGE_F16>;
+    kd<<<grid, kRayThreads, ray_stack_bytes(m->view), s>>>(m->view, cam, (const float*) dp.ptr, depth_dev, grad_dev,
                                                       valid_dev);
     const hipError_t e1 = hipGetLastError();
     const hipError_t e2 = hipStreamSynchronize(s);
