@@ -96,8 +96,11 @@ __device__ Span first_block(const MapView& m, const float oW[3], const float dW[
     load_row(parent);
     int scale = kStackDepth - 1;
     float scale_exp2 = 0.5f;
-    // the traversal visits each octant at most once; the cap only guards against a corrupt octree
+    // the traversal visits each octant at most once; the cap only guards against a corrupt octree.
+    // A descent and a pop both end with the row of the new parent; that load is issued from one
+    // place at the end of the iteration, so a wave whose lanes take both paths issues one gather.
     for (int guard = 0; scale < kStackDepth && guard < (1 << 20); guard++) {
+        bool reload = false;
         float tcorner[3];
 #pragma unroll
         for (int i = 0; i < 3; i++)
@@ -122,7 +125,7 @@ __device__ Span first_block(const MapView& m, const float oW[3], const float dW[
             }
             h = tc_max;
             parent = child;
-            load_row(parent);
+            reload = true;
             idx = 0;
             scale--;
             scale_exp2 = half;
@@ -134,42 +137,45 @@ __device__ Span first_block(const MapView& m, const float oW[3], const float dW[
             for (int i = 0; i < 3; i++)
                 pos[i] = pos[i] + scale_exp2 * (float) ((idx >> i) & 1);
             t_max = tv_max;
-            continue;
         }
-        const int step_mask =
-            (tcorner[0] <= tc_max) | ((tcorner[1] <= tc_max) << 1) | ((tcorner[2] <= tc_max) << 2);
-#pragma unroll
-        for (int i = 0; i < 3; i++)
-            pos[i] = pos[i] - scale_exp2 * (float) ((step_mask >> i) & 1);
-        t_min = tc_max;
-        idx ^= step_mask;
-        if ((idx & step_mask) != 0) {
-            unsigned int diff = 0;
+        else {
+            const int step_mask =
+                (tcorner[0] <= tc_max) | ((tcorner[1] <= tc_max) << 1) | ((tcorner[2] <= tc_max) << 2);
 #pragma unroll
             for (int i = 0; i < 3; i++)
-                if (step_mask & (1 << i))
-                    diff |= (unsigned) (f2i(pos[i]) ^ f2i(pos[i] + scale_exp2));
-            scale = (f2i((float) diff) >> 23) - 127;
-            if (scale >= kStackDepth)
-                break;
-            const int l = kStackDepth - 1 - scale;
-            if (l < 0 || l >= lg || st_parent[l * stride] < 0) {
-                scale = kStackDepth; // unreachable for a consistent octree
-                break;
-            }
-            scale_exp2 = i2f((scale - kStackDepth + 127) << 23);
-            parent = st_parent[l * stride];
-            load_row(parent);
-            t_max = st_tmax[l * stride];
-            int sh[3];
+                pos[i] = pos[i] - scale_exp2 * (float) ((step_mask >> i) & 1);
+            t_min = tc_max;
+            idx ^= step_mask;
+            if ((idx & step_mask) != 0) {
+                unsigned int diff = 0;
 #pragma unroll
-            for (int i = 0; i < 3; i++) {
-                sh[i] = f2i(pos[i]) >> scale;
-                pos[i] = i2f(sh[i] << scale);
+                for (int i = 0; i < 3; i++)
+                    if (step_mask & (1 << i))
+                        diff |= (unsigned) (f2i(pos[i]) ^ f2i(pos[i] + scale_exp2));
+                scale = (f2i((float) diff) >> 23) - 127;
+                if (scale >= kStackDepth)
+                    break;
+                const int l = kStackDepth - 1 - scale;
+                if (l < 0 || l >= lg || st_parent[l * stride] < 0) {
+                    scale = kStackDepth; // unreachable for a consistent octree
+                    break;
+                }
+                scale_exp2 = i2f((scale - kStackDepth + 127) << 23);
+                parent = st_parent[l * stride];
+                reload = true;
+                t_max = st_tmax[l * stride];
+                int sh[3];
+#pragma unroll
+                for (int i = 0; i < 3; i++) {
+                    sh[i] = f2i(pos[i]) >> scale;
+                    pos[i] = i2f(sh[i] << scale);
+                }
+                idx = (sh[0] & 1) | ((sh[1] & 1) << 1) | ((sh[2] & 1) << 2);
+                h = 0.0f;
             }
-            idx = (sh[0] & 1) | ((sh[1] & 1) << 1) | ((sh[2] & 1) << 2);
-            h = 0.0f;
         }
+        if (reload)
+            load_row(parent);
     }
     Span s;
     s.t_min = t_min * scaling;
