@@ -482,6 +482,10 @@ SE8_HD void unpack_f16(uint32_t v, float& tsdf, float& weight)
     weight = (float) (v >> 16);
 }
 
+// Maps of at most this many voxels a side also keep a dense block table (at most 512 MiB): a
+// block lookup is then one load instead of the brick id and the brick's slot.
+constexpr int kDenseMaxSize = 4096;
+
 // Everything a kernel needs to address the map, passed by value.
 struct MapView {
     int size;         // octree size in voxels
@@ -497,6 +501,8 @@ struct MapView {
     int storage;
     int* brick_top;     // bricks_side^3 brick ids (-1 = none)
     int* brick_slots;   // per brick 512 block slots (-1 = none), index (bx&m) + 8(by&m) + 64(bz&m)
+    int* block_dense;   // maps of at most kDenseMaxSize voxels a side: the slot of every block
+                        // (-1 = none), index (bz * S + by) * S + bx, S = size / 8; null otherwise
     int* node_child;    // per node 8 children: node index, or block slot below scale-4 nodes
     uint64_t* node_key; // per node key
     int* node_ts;       // per node timestamp
@@ -578,6 +584,10 @@ __device__ __forceinline__ int lookup_block(const MapView& m, int x, int y, int 
 {
     const int wrap = m.size - 1;
     const int bx = (x & wrap) >> 3, by = (y & wrap) >> 3, bz = (z & wrap) >> 3;
+    if (m.block_dense) {
+        const int side = m.size >> 3;
+        return m.block_dense[(bz * side + by) * side + bx];
+    }
     const int s = m.brick_shift;
     const int brick = m.brick_top[((bz >> s) * m.bricks_side + (by >> s)) * m.bricks_side + (bx >> s)];
     if (brick < 0)
@@ -593,6 +603,10 @@ __device__ __forceinline__ int lookup_block_nb(const MapView& m, int x, int y, i
 {
     const int wrap = m.size - 1;
     const int bx = (x & wrap) >> 3, by = (y & wrap) >> 3, bz = (z & wrap) >> 3;
+    if (m.block_dense) {
+        const int side = m.size >> 3;
+        return m.block_dense[(bz * side + by) * side + bx];
+    }
     const int s = m.brick_shift;
     const int brick = m.brick_top[((bz >> s) * m.bricks_side + (by >> s)) * m.bricks_side + (bx >> s)];
     const int msk = (1 << s) - 1;
@@ -705,6 +719,7 @@ struct se8_map {
     se8_map_config cfg;
     se8_tsdf_config tsdf;
     se8::MapView view;
+    se8::DeviceBuffer block_dense;
     se8::DeviceBuffer brick_top, brick_slots, node_child, node_key, node_ts, block_key, block_parent, block_ts,
         voxels, counters, block_scale;
     // per-frame scratch

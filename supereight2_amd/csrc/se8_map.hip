@@ -627,6 +627,10 @@ __global__ void k_build_blocks(MapView m, const uint64_t* keys, const int* n_dev
             const int bx = c[0] >> 3, by = c[1] >> 3, bz = c[2] >> 3;
             if (brick >= 0)
                 m.brick_slots[brick * 512 + (bx & msk) + ((by & msk) << 3) + ((bz & msk) << 6)] = slot;
+            if (m.block_dense) {
+                const int side = m.size >> 3;
+                m.block_dense[(bz * side + by) * side + bx] = slot;
+            }
         }
         // AABB of the allocated blocks (octree_impl.hpp:267-273): one atomic per wave and bound
         if (__ballot(made)) {
@@ -968,6 +972,9 @@ extern "C" int se8_map_create(const se8_map_config* cfg, const se8_tsdf_config* 
     int rc = SE8_OK;
     rc = rc ? rc : dev_alloc(m->brick_top, sizeof(int) * bricks_side3);
     rc = rc ? rc : dev_alloc(m->brick_slots, sizeof(int) * 512 * v.max_bricks);
+    const int64_t dense_side = size / 8;
+    if (size <= kDenseMaxSize)
+        rc = rc ? rc : dev_alloc(m->block_dense, sizeof(int) * dense_side * dense_side * dense_side);
     rc = rc ? rc : dev_alloc(m->node_child, sizeof(int) * 8 * v.max_nodes);
     rc = rc ? rc : dev_alloc(m->node_key, sizeof(uint64_t) * v.max_nodes);
     rc = rc ? rc : dev_alloc(m->node_ts, sizeof(int) * v.max_nodes);
@@ -986,6 +993,7 @@ extern "C" int se8_map_create(const se8_map_config* cfg, const se8_tsdf_config* 
     }
     v.brick_top = (int*) m->brick_top.ptr;
     v.brick_slots = (int*) m->brick_slots.ptr;
+    v.block_dense = (int*) m->block_dense.ptr;
     v.node_child = (int*) m->node_child.ptr;
     v.node_key = (uint64_t*) m->node_key.ptr;
     v.node_ts = (int*) m->node_ts.ptr;
@@ -998,6 +1006,8 @@ extern "C" int se8_map_create(const se8_map_config* cfg, const se8_tsdf_config* 
     hipStream_t s = nullptr;
     rc = rc ? rc : fill_i32(m->brick_top, bricks_side3, -1, s);
     rc = rc ? rc : fill_i32(m->brick_slots, 512 * v.max_bricks, -1, s);
+    if (m->block_dense.ptr)
+        rc = rc ? rc : fill_i32(m->block_dense, dense_side * dense_side * dense_side, -1, s);
     rc = rc ? rc : fill_i32(m->node_child, 8 * v.max_nodes, -1, s);
     if (rc != SE8_OK) {
         se8_map_destroy(m);
@@ -1057,7 +1067,8 @@ extern "C" int se8_map_destroy(se8_map* m)
     DeviceBuffer* bufs[] = {&m->brick_top,    &m->brick_slots, &m->node_child, &m->node_key,   &m->node_ts,
                             &m->block_key,    &m->block_parent, &m->block_ts,  &m->voxels,     &m->counters,
                             &m->update_list,  &m->candidates,   &m->unique_keys, &m->cur_node,
-                            &m->frame_poses,  &m->mesh_offsets, &m->block_scale, &m->mr_new_scale};
+                            &m->frame_poses,  &m->mesh_offsets, &m->block_scale, &m->mr_new_scale,
+                            &m->block_dense};
     for (DeviceBuffer* b : bufs)
         dev_free(*b);
     m->sorter.release();
