@@ -627,6 +627,43 @@ __device__ __forceinline__ void load_voxel(const MapView& m, int slot, int vi, f
     }
 }
 
+// lookup_block / lookup_block_nb with the choice of block table fixed at compile time (Dense: the
+// map has block_dense), for kernels instantiated per table.
+template <bool Dense>
+__device__ __forceinline__ int lookup_block_t(const MapView& m, int x, int y, int z)
+{
+    if constexpr (Dense) {
+        const int wrap = m.size - 1, side = m.size >> 3;
+        return m.block_dense[((((z & wrap) >> 3) * side) + ((y & wrap) >> 3)) * side + ((x & wrap) >> 3)];
+    }
+    else {
+        const int wrap = m.size - 1;
+        const int bx = (x & wrap) >> 3, by = (y & wrap) >> 3, bz = (z & wrap) >> 3;
+        const int s = m.brick_shift;
+        const int brick = m.brick_top[((bz >> s) * m.bricks_side + (by >> s)) * m.bricks_side + (bx >> s)];
+        if (brick < 0)
+            return -1;
+        const int msk = (1 << s) - 1;
+        return m.brick_slots[brick * 512 + (bx & msk) + ((by & msk) << 3) + ((bz & msk) << 6)];
+    }
+}
+template <bool Dense>
+__device__ __forceinline__ int lookup_block_nb_t(const MapView& m, int x, int y, int z)
+{
+    if constexpr (Dense) {
+        return lookup_block_t<true>(m, x, y, z);
+    }
+    else {
+        const int wrap = m.size - 1;
+        const int bx = (x & wrap) >> 3, by = (y & wrap) >> 3, bz = (z & wrap) >> 3;
+        const int s = m.brick_shift;
+        const int brick = m.brick_top[((bz >> s) * m.bricks_side + (by >> s)) * m.bricks_side + (bx >> s)];
+        const int msk = (1 << s) - 1;
+        const int slot = m.brick_slots[max(brick, 0) * 512 + (bx & msk) + ((by & msk) << 3) + ((bz & msk) << 6)];
+        return brick < 0 ? -1 : slot;
+    }
+}
+
 // load_voxel for a storage fixed at compile time: kernels that load many voxels per thread are
 // instantiated per storage so that no load sits behind a branch on the storage type.
 template <int ST>

@@ -20,6 +20,15 @@
 
 namespace se8 {
 
+// The raycast kernels and their helpers are instantiated per map layout: ST = storage
+// (SE8_STORAGE_F32 = 0 or SE8_STORAGE_F16 = 1) + 2 if the map has the dense block table.
+template <typename K>
+K pick_kernel(const MapView& m, K f32_brick, K f16_brick, K f32_dense, K f16_dense)
+{
+    const bool f32 = m.storage == SE8_STORAGE_F32;
+    return m.block_dense ? (f32 ? f32_dense : f16_dense) : (f32 ? f32_brick : f16_brick);
+}
+
 constexpr int kStackDepth = 23;  // CAST_STACK_DEPTH (voxel_block_ray_iterator.hpp:15)
 constexpr int kMaxLevels = 16;   // log2(size / 8) for the largest supported octree (2^19)
 constexpr int kRayThreads = 128; // 16 x 8 pixel tile
@@ -222,13 +231,13 @@ __device__ __forceinline__ void get_data(const MapView& m, const float p[3], flo
         weight = 0.f;
         return;
     }
-    const int slot = lookup_block(m, v[0], v[1], v[2]);
+    const int slot = lookup_block_t<(ST >> 1) != 0>(m, v[0], v[1], v[2]);
     if (slot < 0) {
         tsdf = 1.f;
         weight = 0.f;
         return;
     }
-    load_voxel_t<ST>(m, slot, (v[0] & 7) + ((v[1] & 7) << 3) + ((v[2] & 7) << 6), tsdf, weight);
+    load_voxel_t<(ST & 1)>(m, slot, (v[0] & 7) + ((v[1] & 7) << 3) + ((v[2] & 7) << 6), tsdf, weight);
 }
 
 // Field values of the 8 voxels base + {0,1}^3 (x fastest) and whether all are allocated and
@@ -240,13 +249,13 @@ __device__ __forceinline__ bool gather_corners(const MapView& m, const int base[
     int slot[8];
 #pragma unroll
     for (int k = 0; k < 8; k++)
-        slot[k] = lookup_block_nb(m, base[0] + (k & 1), base[1] + ((k >> 1) & 1), base[2] + ((k >> 2) & 1));
+        slot[k] = lookup_block_nb_t<(ST >> 1) != 0>(m, base[0] + (k & 1), base[1] + ((k >> 1) & 1), base[2] + ((k >> 2) & 1));
     bool ok = true;
 #pragma unroll
     for (int k = 0; k < 8; k++) {
         const int x = base[0] + (k & 1), y = base[1] + ((k >> 1) & 1), z = base[2] + ((k >> 2) & 1);
         float w;
-        load_voxel_t<ST>(m, max(slot[k], 0), (x & 7) + ((y & 7) << 3) + ((z & 7) << 6), d[k], w);
+        load_voxel_t<(ST & 1)>(m, max(slot[k], 0), (x & 7) + ((y & 7) << 3) + ((z & 7) << 6), d[k], w);
         ok = ok && slot[k] >= 0 && w != 0.f;
     }
     return ok;
@@ -274,14 +283,14 @@ __device__ __forceinline__ void get_data_cached(const MapView& m, const float p[
         | ((uint64_t) ((v[2] & wrap) >> 3) << 42);
     if (key != cache.key) {
         cache.key = key;
-        cache.slot = lookup_block(m, v[0], v[1], v[2]);
+        cache.slot = lookup_block_t<(ST >> 1) != 0>(m, v[0], v[1], v[2]);
     }
     if (cache.slot < 0) {
         tsdf = 1.f;
         weight = 0.f;
         return;
     }
-    load_voxel_t<ST>(m, cache.slot, (v[0] & 7) + ((v[1] & 7) << 3) + ((v[2] & 7) << 6), tsdf, weight);
+    load_voxel_t<(ST & 1)>(m, cache.slot, (v[0] & 7) + ((v[1] & 7) << 3) + ((v[2] & 7) << 6), tsdf, weight);
 }
 
 // single-res getInterp (visitor_impl.hpp:976-1024) from voxel coordinates.  The reference's
@@ -348,8 +357,8 @@ __device__ bool field_grad_vox(const MapView& m, const float vf[3], float g[3])
     int bs[8];
 #pragma unroll
     for (int k = 0; k < 8; k++)
-        bs[k] = lookup_block_nb(m, c[0][(k & 1) ? 3 : 0], c[1][(k & 2) ? 3 : 0], c[2][(k & 4) ? 3 : 0]);
-    const int base_slot = lookup_block_nb(m, base[0], base[1], base[2]);
+        bs[k] = lookup_block_nb_t<(ST >> 1) != 0>(m, c[0][(k & 1) ? 3 : 0], c[1][(k & 2) ? 3 : 0], c[2][(k & 4) ? 3 : 0]);
+    const int base_slot = lookup_block_nb_t<(ST >> 1) != 0>(m, base[0], base[1], base[2]);
     constexpr int kOuter = 0x9; // bit i set: index i is an outer layer
     float F[4][4][4];
     bool ok = base_slot >= 0;
@@ -363,7 +372,7 @@ __device__ bool field_grad_vox(const MapView& m, const float vf[3], float g[3])
             if (inner || !((kOuter >> i) & 1)) {
                 const int sl = hi[0][i] ? up : lo;
                 float w;
-                load_voxel_t<ST>(m, max(sl, 0), (c[0][i] & 7) + ((c[1][j] & 7) << 3) + ((c[2][k] & 7) << 6),
+                load_voxel_t<(ST & 1)>(m, max(sl, 0), (c[0][i] & 7) + ((c[1][j] & 7) << 3) + ((c[2][k] & 7) << 6),
                                  F[i][j][k], w);
                 ok = ok && sl >= 0 && w != 0.f;
             }
@@ -722,7 +731,7 @@ static int run_query(const se8_map* m, const float* pts, int64_t n, int mode, st
         if (hipMemcpy(dp.ptr, pts, sizeof(float) * 3 * n, hipMemcpyHostToDevice) != hipSuccess)
             rc = SE8_ERR_HIP;
         else {
-            auto* kq = m->view.storage == SE8_STORAGE_F32 ? k_query<SE8_STORAGE_F32> : k_query<SE8_STORAGE_F16>;
+            auto* kq = pick_kernel(m->view, k_query<0>, k_query<1>, k_query<2>, k_query<3>);
             kq<<<(unsigned) ((n + 255) / 256), 256>>>(m->view, (const float*) dp.ptr, (int) n, mode,
                                                            (float*) dout.ptr, (uint8_t*) dval.ptr);
             if (hipMemcpy(out.data(), dout.ptr, sizeof(float) * width * n, hipMemcpyDeviceToHost) != hipSuccess
@@ -744,7 +753,7 @@ int launch_raycast(const se8_map* m, const Cam& cam, const Iso& T_WS, const floa
                    float* normals, int8_t* scale, hipStream_t s, hipEvent_t ev_start, hipEvent_t ev_stop)
 {
     const dim3 grid((cam.w + 15) / 16, (cam.h + 7) / 8);
-    auto* kr = m->view.storage == SE8_STORAGE_F32 ? k_raycast<SE8_STORAGE_F32> : k_raycast<SE8_STORAGE_F16>;
+    auto* kr = pick_kernel(m->view, k_raycast<0>, k_raycast<1>, k_raycast<2>, k_raycast<3>);
     hipExtLaunchKernelGGL(kr, grid, dim3(kRayThreads), (uint32_t) ray_stack_bytes(m->view), s, ev_start,
                           ev_stop, 0, m->view, cam, T_WS, T16_dev, points, normals, scale);
     SE8_HIP_CHECK(hipGetLastError());
@@ -857,7 +866,8 @@ extern "C" int se8_raycast_depth_grad_batched(const se8_map* m, const se8_pinhol
         return SE8_ERR_HIP;
     }
     dim3 grid((cam.w + 15) / 16, (cam.h + 7) / 8, n_poses);
-    auto* kd = m->view.storage == SE8_STORAGE_F32 ? k_raycast_depth_grad<SE8_STORAGE_F32> : k_raycast_depth_grad<SE8_STORAGE_F16>;
+    auto* kd = pick_kernel(m->view, k_raycast_depth_grad<0>, k_raycast_depth_grad<1>, k_raycast_depth_grad<2>,
+                           k_raycast_depth_grad<3>);
     kd<<<grid, kRayThreads, ray_stack_bytes(m->view), s>>>(m->view, cam, (const float*) dp.ptr, depth_dev, grad_dev,
                                                       valid_dev);
     const hipError_t e1 = hipGetLastError();
