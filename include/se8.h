@@ -176,6 +176,19 @@ int se8_track_raycast(se8_tracker* tracker, const se8_map* map, const float* dep
 int se8_icp_cost_grad_batched(se8_tracker* tracker, const float* depth_dev, const float* ref_points_dev,
                               const float* ref_normals_dev, const float T_ref[16], int n_poses, const float* poses16,
                               float* cost, float* grad6, float* jtj21, int* inliers, se8_stream stream);
+/* No counterpart in the reference.  se8_icp_cost_grad_batched for callers that cannot allocate
+ * device memory (code under HIP graph capture, serving loops that forbid hipMalloc): the same
+ * kernels and results (bit for bit), with the candidate poses (poses16_dev: n row-major 4x4) and
+ * the results in device memory (cost_dev n, grad6_dev n*6, jtj21_dev n*21 or NULL, inliers_dev n
+ * or NULL); T_ref is read on the host at call time.  Allocates nothing and never waits: the
+ * per-workgroup partials live in a fixed array of the library (at most 64 poses per call;
+ * n_poses above that returns SE8_ERR_INVALID), one per device, held until the process ends.
+ * Every call on a device shares that array and the tracker's level-0 maps, so calls that overlap
+ * on one device (other streams, other threads) race. */
+int se8_icp_cost_grad_batched_noalloc(se8_tracker* tracker, const float* depth_dev, const float* ref_points_dev,
+                                      const float* ref_normals_dev, const float T_ref[16], int n_poses,
+                                      const float* poses16_dev, float* cost_dev, float* grad6_dev, float* jtj21_dev,
+                                      int* inliers_dev, se8_stream stream);
 /* Batched raycast depth and its adjoint for n poses (poses16 host, row-major 4x4): per pixel the
  * z-depth of the root of the trilinear TSDF along the ray, refined inside the bracket of the
  * reference march (raycaster_impl.hpp:456-533), and d depth / d xi (6 per pixel) by the

@@ -150,6 +150,7 @@ SIGNATURES = {
     "se8_track": (C.c_int, [_P, _P, _F, _P, _P, _I, _F, _P]),
     "se8_track_raycast": (C.c_int, [_P, _P, _P, _F, _I, _P]),
     "se8_icp_cost_grad_batched": (C.c_int, [_P, _P, _P, _P, _F, C.c_int, _F, _F, _F, _F, _I, _P]),
+    "se8_icp_cost_grad_batched_noalloc": (C.c_int, [_P, _P, _P, _P, _F, C.c_int, _P, _P, _P, _P, _P, _P]),
     "se8_raycast_depth_grad_batched": (C.c_int, [_P, C.POINTER(Pinhole), C.c_int, _F, _P, _P, _P, _P]),
     "se8_integrate_depth_dev": (C.c_int, [_P, C.c_int, _P, C.POINTER(Pinhole), _P, _P]),
     "se8_integrate_range_dev": (C.c_int, [_P, C.c_int, _P, _P, _P, _P]),

@@ -22,6 +22,7 @@
 
 #include <cmath>
 #include <cstring>
+#include <mutex>
 #include <vector>
 
 #include "se8_internal.hpp"
@@ -1034,14 +1035,16 @@ int launch_raycast(const se8_map* m, const Cam& cam, const Iso& T_WS, const floa
 // ----------------------------------------------------------------------------------------------
 constexpr int kAdjTerms = 29; // e^2, e J (6), J^T J (21), inliers
 
-__global__ __launch_bounds__(kIcpThreads) void k_adj_icp(IcpArgs a, const float* poses12, int n_wg, float* partials)
+// poses: pose k's rows R|t start at poses[pose_stride * k] (12: packed 3x4, 16: row-major 4x4)
+__global__ __launch_bounds__(kIcpThreads) void k_adj_icp(IcpArgs a, const float* poses, int pose_stride, int n_wg,
+                                                        float* partials)
 {
     const int k = blockIdx.y;
     Iso T;
     for (int i = 0; i < 3; i++) {
         for (int j = 0; j < 3; j++)
-            T.R[i][j] = poses12[12 * k + 4 * i + j];
-        T.t[i] = poses12[12 * k + 4 * i + 3];
+            T.R[i][j] = poses[(size_t) pose_stride * k + 4 * i + j];
+        T.t[i] = poses[(size_t) pose_stride * k + 4 * i + 3];
     }
     float s[kAdjTerms];
     for (int q = 0; q < kAdjTerms; q++)
@@ -1077,7 +1080,11 @@ __global__ __launch_bounds__(kIcpThreads) void k_adj_icp(IcpArgs a, const float*
     }
 }
 
-__global__ void k_adj_icp_reduce(const float* partials, int n_wg, double* out)
+// Per pose, the FP64 sum of the workgroup partials in workgroup order.  The sums go to out (29
+// doubles per pose) when it is set; otherwise the results are written in their final form, as the
+// host converts them: cost = 0.5 e^2 sum, grad = -(e J sum), J^T J, inlier count.
+__global__ void k_adj_icp_reduce(const float* partials, int n_wg, double* out, float* cost, float* grad6, float* jtj21,
+                                 int* inliers)
 {
     const int k = blockIdx.x;
     const int q = threadIdx.x;
@@ -1086,7 +1093,46 @@ __global__ void k_adj_icp_reduce(const float* partials, int n_wg, double* out)
     double acc = 0.0;
     for (int b = 0; b < n_wg; b++)
         acc += (double) partials[((size_t) k * n_wg + b) * kAdjTerms + q];
-    out[(size_t) k * kAdjTerms + q] = acc;
+    if (out) {
+        out[(size_t) k * kAdjTerms + q] = acc;
+        return;
+    }
+    if (q == 0)
+        cost[k] = (float) (0.5 * acc);
+    else if (q < 7)
+        grad6[6 * k + q - 1] = (float) (-acc);
+    else if (q < 28) {
+        if (jtj21)
+            jtj21[21 * k + q - 7] = (float) acc;
+    }
+    else if (inliers)
+        inliers[k] = (int) acc;
+}
+
+// Scratch of se8_icp_cost_grad_batched_noalloc: the per-workgroup partials of at most
+// kNoAllocMaxPoses poses (kIcpMaxWg workgroups each), 2.4 MB held by every device the library
+// runs on until the process ends.  Host code reaches it only through the address that
+// hipGetSymbolAddress returns for the current device (noalloc_scratch_ptr).
+constexpr int kNoAllocMaxPoses = 64;
+static __device__ __align__(256) float scratch[(size_t) kNoAllocMaxPoses * kIcpMaxWg * kAdjTerms];
+
+// Device address of `scratch` on the current device, looked up once per device.
+constexpr int kNoAllocMaxDevices = 64;
+static int noalloc_scratch_ptr(float** out)
+{
+    static std::mutex mu;
+    static float* ptrs[kNoAllocMaxDevices] = {};
+    int dev = 0;
+    SE8_HIP_CHECK(hipGetDevice(&dev));
+    SE8_REQUIRE(dev >= 0 && dev < kNoAllocMaxDevices, "se8_icp_cost_grad_batched_noalloc: device index out of range");
+    std::lock_guard<std::mutex> lock(mu);
+    if (!ptrs[dev]) {
+        void* p = nullptr;
+        SE8_HIP_CHECK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        ptrs[dev] = (float*) p;
+    }
+    *out = ptrs[dev];
+    return SE8_OK;
 }
 
 } // namespace se8
@@ -1232,9 +1278,10 @@ extern "C" int se8_icp_cost_grad_batched(se8_tracker* t, const float* depth_dev,
     a.normal_th = t->cfg.normal_threshold;
     a.icp_th = t->cfg.icp_threshold;
     a.level = 0;
-    k_adj_icp<<<dim3(n_wg, n_poses), kIcpThreads, 0, s>>>(a, (const float*) t->adj_poses.ptr, n_wg,
+    k_adj_icp<<<dim3(n_wg, n_poses), kIcpThreads, 0, s>>>(a, (const float*) t->adj_poses.ptr, 12, n_wg,
                                                           (float*) t->adj_partials.ptr);
-    k_adj_icp_reduce<<<n_poses, 64, 0, s>>>((const float*) t->adj_partials.ptr, n_wg, (double*) t->adj_out.ptr);
+    k_adj_icp_reduce<<<n_poses, 64, 0, s>>>((const float*) t->adj_partials.ptr, n_wg, (double*) t->adj_out.ptr,
+                                            nullptr, nullptr, nullptr, nullptr);
     SE8_HIP_CHECK(hipGetLastError());
     std::vector<double> out((size_t) kAdjTerms * n_poses);
     SE8_HIP_CHECK(hipMemcpyAsync(out.data(), t->adj_out.ptr, sizeof(double) * out.size(), hipMemcpyDeviceToHost, s));
@@ -1250,5 +1297,44 @@ extern "C" int se8_icp_cost_grad_batched(se8_tracker* t, const float* depth_dev,
         if (inliers)
             inliers[k] = (int) o[28];
     }
+    return SE8_OK;
+}
+
+extern "C" int se8_icp_cost_grad_batched_noalloc(se8_tracker* t, const float* depth_dev, const float* ref_points_dev,
+                                                 const float* ref_normals_dev, const float T_ref16[16], int n_poses,
+                                                 const float* poses16_dev, float* cost_dev, float* grad6_dev,
+                                                 float* jtj21_dev, int* inliers_dev, se8_stream stream)
+{
+    SE8_REQUIRE(t && depth_dev && ref_points_dev && ref_normals_dev && T_ref16 && poses16_dev && cost_dev && grad6_dev,
+                "se8_icp_cost_grad_batched_noalloc: null argument");
+    SE8_REQUIRE(n_poses > 0 && n_poses <= kNoAllocMaxPoses,
+                "se8_icp_cost_grad_batched_noalloc: n_poses must be in [1, 64] (the size of the library's scratch)");
+    const int n_wg = t->num_wg[0];
+    SE8_REQUIRE(n_wg > 0 && n_wg <= kIcpMaxWg, "se8_icp_cost_grad_batched_noalloc: tracker grid out of range");
+    float* partials = nullptr;
+    int rc = noalloc_scratch_ptr(&partials);
+    if (rc != SE8_OK)
+        return rc;
+    hipStream_t s = (hipStream_t) stream;
+    const se8_pinhole& P = t->sensor;
+    // the level-0 vertex and normal maps go to the tracker's own pyramid buffers (allocated with
+    // the tracker)
+    launch_pyramid(t, depth_dev, 1, nullptr, Iso{}, nullptr, s);
+    IcpArgs a{};
+    a.vert = (const float4*) t->vert_lvl[0].ptr;
+    a.norm = (const float4*) t->norm_lvl[0].ptr;
+    a.ref_pts = ref_points_dev;
+    a.ref_nrm = ref_normals_dev;
+    a.T_ref_inv = iso_inv(iso_from16(T_ref16));
+    a.cam = make_cam(P, 1, false);
+    a.w = P.width;
+    a.h = P.height;
+    a.dist_th = t->cfg.dist_threshold;
+    a.normal_th = t->cfg.normal_threshold;
+    a.icp_th = t->cfg.icp_threshold;
+    a.level = 0;
+    k_adj_icp<<<dim3(n_wg, n_poses), kIcpThreads, 0, s>>>(a, poses16_dev, 16, n_wg, partials);
+    k_adj_icp_reduce<<<n_poses, 64, 0, s>>>(partials, n_wg, nullptr, cost_dev, grad6_dev, jtj21_dev, inliers_dev);
+    SE8_HIP_CHECK(hipGetLastError());
     return SE8_OK;
 }
