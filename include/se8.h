@@ -72,12 +72,18 @@ typedef struct {
     float near_plane, far_plane;
 } se8_pinhole;
 
-/* se::TrackerConfig (include/se/tracker/tracker.hpp:22-33).  compensated = 1 selects
- * compensated fp32 per-workgroup partials (TwoSum sum-and-error pairs across lanes and waves), an
- * FP64 final reduction and an FP64 LDLT solve; 0 is the plain fp32 reduction + fp32 Cholesky like
- * the reference. */
+/* se::TrackerConfig (include/se/tracker/tracker.hpp:22-33).  iterations[0..num_levels) are the
+ * Gauss-Newton iterations per pyramid level, finest first (the reference's std::vector
+ * iterations, whose size is the number of levels, tracker_impl.hpp:50-84); 1 <= num_levels <=
+ * SE8_TRACKER_MAX_LEVELS, and the coarsest level must keep at least one pixel in each direction
+ * (se8_tracker_create returns SE8_ERR_INVALID otherwise).  compensated = 1 selects compensated
+ * per-workgroup partials (fp32 sums within each wave, TwoSum sum-and-error pairs across a
+ * workgroup's waves), an FP64 cross-workgroup reduction and an FP64 LDLT solve; 0 is the plain
+ * fp32 reduction + fp32 Cholesky like the reference. */
+#define SE8_TRACKER_MAX_LEVELS 6
 typedef struct {
-    int iterations[3];
+    int iterations[SE8_TRACKER_MAX_LEVELS];
+    int num_levels;
     float dist_threshold;
     float normal_threshold;
     float track_threshold;
@@ -162,6 +168,11 @@ int se8_tracker_destroy(se8_tracker* tracker);
  * value; sums32 (may be NULL) receives the last reduction (icp.cpp:45-204 layout). */
 int se8_track(se8_tracker* tracker, const float* depth_dev, float T_WS[16], const float* ref_points_dev,
               const float* ref_normals_dev, int* ok, float* sums32, se8_stream stream);
+/* The tracker's input pyramid of the last track call at one level (0 = full resolution):
+ * input_point_cloud_S and input_normals_S of tracker_impl.hpp:59-75 (depth_to_point_cloud of the
+ * robustly half-sampled depth, point_cloud_to_normal), (W >> level) * (H >> level) * 3 floats each
+ * into host arrays (either may be NULL).  Waits for the device. */
+int se8_tracker_read_pyramid(const se8_tracker* tracker, int level, float* points, float* normals);
 /* Tracker::track(depth, T_WS): raycasts map at T_WS first (tracker_impl.hpp:17-26). */
 int se8_track_raycast(se8_tracker* tracker, const se8_map* map, const float* depth_dev, float T_WS[16], int* ok,
                       se8_stream stream);

@@ -594,11 +594,19 @@ class Tracker {
     Tracker(MapT& map, const SensorT& sensor, const TrackerConfig config = TrackerConfig())
         : map_(map), sensor_(sensor)
     {
-        if (config.iterations.size() != 3)
-            throw std::invalid_argument("se::Tracker: exactly 3 pyramid levels are supported");
-        se8_tracker_config c{{config.iterations[0], config.iterations[1], config.iterations[2]},
-                             config.dist_threshold, config.normal_threshold, config.track_threshold,
-                             config.icp_threshold, config.compensated ? 1 : 0};
+        // one pyramid level per entry of iterations (tracker_impl.hpp:50-84)
+        if (config.iterations.empty() || config.iterations.size() > SE8_TRACKER_MAX_LEVELS)
+            throw std::invalid_argument("se::Tracker: 1 to " + std::to_string(SE8_TRACKER_MAX_LEVELS)
+                                        + " pyramid levels are supported");
+        se8_tracker_config c{};
+        for (size_t l = 0; l < config.iterations.size(); l++)
+            c.iterations[l] = config.iterations[l];
+        c.num_levels = (int) config.iterations.size();
+        c.dist_threshold = config.dist_threshold;
+        c.normal_threshold = config.normal_threshold;
+        c.track_threshold = config.track_threshold;
+        c.icp_threshold = config.icp_threshold;
+        c.compensated = config.compensated ? 1 : 0;
         detail::check(se8_tracker_create(&sensor.c(), &c, &h_), "se8_tracker_create");
     }
     Tracker(const Tracker&) = delete;

@@ -35,9 +35,13 @@ class Pinhole(C.Structure):
     ]
 
 
+MAX_LEVELS = 6  # OR_MAX_LEVELS
+
+
 class TrackerConfig(C.Structure):
     _fields_ = [
-        ("iterations", C.c_int * 3),
+        ("iterations", C.c_int * MAX_LEVELS),
+        ("num_levels", C.c_int),
         ("dist_threshold", C.c_float),
         ("normal_threshold", C.c_float),
         ("track_threshold", C.c_float),
@@ -50,7 +54,10 @@ def pinhole(width=640, height=480, fx=525.0, fy=525.0, cx=319.5, cy=239.5, near_
 
 
 def tracker_config(iterations=(10, 5, 4), dist=0.1, normal=0.8, track=0.15, icp=1e-5) -> TrackerConfig:
-    return TrackerConfig((C.c_int * 3)(*iterations), dist, normal, track, icp)
+    it = list(iterations)
+    assert 1 <= len(it) <= MAX_LEVELS
+    return TrackerConfig((C.c_int * MAX_LEVELS)(*(it + [0] * (MAX_LEVELS - len(it)))), len(it), dist, normal, track,
+                         icp)
 
 
 _libs: dict = {}
@@ -117,6 +124,9 @@ def lib(precision: str = "f32") -> C.CDLL:
     L.or_mesh.argtypes = [P, R, R, C.c_long]
     L.or_field_slice.argtypes = [P, C.c_int, C.c_int, R]
     L.or_map_set_multires.argtypes = [P]
+    L.or_map_set_storage_f16.argtypes = [P]
+    L.or_half_round.argtypes = [C.c_float]
+    L.or_half_round.restype = C.c_float
     L.or_map_dump_multires.argtypes = [P, C.POINTER(U64), I, R, R]
     L.or_structure.restype = C.c_long
     L.or_structure.argtypes = [P, R, C.c_int, R, C.POINTER(C.c_int8), C.c_long]
@@ -255,7 +265,10 @@ def unique_allocation(keys, max_block_scale):
 # map
 # ----------------------------------------------------------------------------------------------
 class OracleMap:
-    def __init__(self, dim=10.24, res=0.01, mu_factor=8.0, max_weight=100.0, precision="f32", multires=False):
+    def __init__(self, dim=10.24, res=0.01, mu_factor=8.0, max_weight=100.0, precision="f32", multires=False,
+                 storage="f32"):
+        """storage="f16": tsdf values are rounded to the nearest binary16 after every update, as an
+        FP16-TSDF map stores them (weights are integers, exact either way)."""
         self.precision = precision
         self.L = lib(precision)
         self.dtype = _dtype(precision)
@@ -264,6 +277,10 @@ class OracleMap:
         self.multires = multires
         if multires:
             self.L.or_map_set_multires(self.h)
+        if storage == "f16":
+            self.L.or_map_set_storage_f16(self.h)
+        elif storage != "f32":
+            raise ValueError(f"unknown storage {storage!r}")
 
     def __del__(self):
         if getattr(self, "h", None):
@@ -454,6 +471,13 @@ def track(depth, pin, T_WS, ref_pts, ref_nrm, cfg=None, precision="f32"):
     ok = L.or_track(d.ctypes.data_as(C.POINTER(C.c_float)), C.byref(pin), C.byref(cfg), _rp(T, precision),
                     _rp(rp, precision), _rp(rn, precision), _rp(sums, precision), C.byref(iters))
     return bool(ok), T.reshape(4, 4), sums, int(iters.value)
+
+
+def half_round(x):
+    """The nearest binary16 value of each float32 in x (ties to even), as float32."""
+    a = np.ascontiguousarray(x, dtype=np.float32)
+    out = np.array([lib().or_half_round(float(v)) for v in a.reshape(-1)], np.float32)
+    return out.reshape(a.shape)
 
 
 def depth_from_u16(raw, scale, divide):

@@ -777,6 +777,7 @@ typedef struct {
     long n_nodes;
     int aabb_min[3], aabb_max[3], aabb_empty;
     int multires; /* TSDFMap<Res::Multi>: blocks carry the multi-resolution data */
+    int storage_f16; /* tsdf stored as IEEE binary16: rounded to the nearest half after each update */
 } or_map;
 
 static unsigned or_power_two_up(float x)
@@ -1167,6 +1168,31 @@ static long fetch_frustum(const or_map* m, const or_sensor* sen, const or_iso* T
 /* propagator_impl.hpp:146-197                                               */
 /* ------------------------------------------------------------------------- */
 
+/* The nearest IEEE binary16 value of x (ties to even; overflow to infinity; infinities and NaN
+ * kept), as a float: the storage rounding of an FP16 TSDF map.  x = m 2^e with m in [0.5, 1); a
+ * normal half keeps 11 significant bits, so the quantum is 2^(e - 11), and 2^-24 below the
+ * smallest normal half (2^-14).  Dividing by a power of two is exact and rintf rounds to the
+ * nearest integer, ties to even. */
+static float half_round(float x)
+{
+    if (isnan(x) || isinf(x))
+        return x;
+    const float ax = fabsf(x);
+    if (ax >= 65520.f) /* halfway between the largest half (65504) and 2^16 rounds up */
+        return x < 0.f ? -INFINITY : INFINITY;
+    int e = 0;
+    (void) frexpf(ax, &e);
+    const int qe = e - 11 < -24 ? -24 : e - 11;
+    const float q = ldexpf(1.0f, qe);
+    const float r = rintf(ax / q) * q;
+    return signbit(x) ? -r : r;
+}
+
+OR_API float or_half_round(float x)
+{
+    return half_round(x);
+}
+
 /* One voxel update; returns 1 if the voxel changed. */
 static inline int field_update(real* tsdf, real* weight, real sdf, real mu, real max_w)
 {
@@ -1206,7 +1232,8 @@ static void update_block(const or_map* m, or_octant* b, const or_iso* T_CV, cons
                 const real mz = sensor_measurement(sen, pC);
                 const real sdf = (d - mz) / mz * norm3(pC);
                 const int vi = x + 8 * y + 64 * z;
-                field_update(&b->tsdf[vi], &b->weight[vi], sdf, m->mu, m->max_weight);
+                if (field_update(&b->tsdf[vi], &b->weight[vi], sdf, m->mu, m->max_weight) && m->storage_f16)
+                    b->tsdf[vi] = (real) half_round((float) b->tsdf[vi]);
             }
         }
     }
@@ -1500,6 +1527,11 @@ static void mr_update_up(const or_map* m, or_octant* b, const or_iso* T_CV, cons
             }
         }
     }
+}
+
+OR_API void or_map_set_storage_f16(void* vm)
+{
+    ((or_map*) vm)->storage_f16 = 1;
 }
 
 OR_API void or_map_set_multires(void* vm)
@@ -2323,8 +2355,10 @@ OR_API void or_se3_exp(const real a[6], real T16[16])
     T16[15] = 1;
 }
 
+#define OR_MAX_LEVELS 6
 typedef struct {
-    int iterations[3];
+    int iterations[OR_MAX_LEVELS]; /* per level, finest first; num_levels = iterations.size() */
+    int num_levels;
     float dist_threshold, normal_threshold, track_threshold, icp_threshold;
 } or_tracker_config;
 
@@ -2336,11 +2370,12 @@ OR_API int or_track(const float* depth, const or_pinhole* pin, const or_tracker_
     or_cam cam;
     cam_init(&cam, pin, 1);
     const int W = cam.w, H = cam.h;
-    const int nl = 3;
-    float* dimg[3];
-    real* pts[3];
-    real* nrm[3];
-    int lw[3], lh[3];
+    const int nl = cfg->num_levels < 1 ? 1 : (cfg->num_levels > OR_MAX_LEVELS ? OR_MAX_LEVELS : cfg->num_levels);
+    /* tracker_impl.hpp:50-84: one level per iterations entry */
+    float* dimg[OR_MAX_LEVELS] = {0};
+    real* pts[OR_MAX_LEVELS] = {0};
+    real* nrm[OR_MAX_LEVELS] = {0};
+    int lw[OR_MAX_LEVELS], lh[OR_MAX_LEVELS];
     for (int l = 0; l < nl; l++) {
         lw[l] = W >> l;
         lh[l] = H >> l;

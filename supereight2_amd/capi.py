@@ -65,9 +65,13 @@ class LidarConfig(C.Structure):
     ]
 
 
+TRACKER_MAX_LEVELS = 6  # SE8_TRACKER_MAX_LEVELS
+
+
 class TrackerConfig(C.Structure):
     _fields_ = [
-        ("iterations", C.c_int * 3),
+        ("iterations", C.c_int * TRACKER_MAX_LEVELS),
+        ("num_levels", C.c_int),
         ("dist_threshold", C.c_float),
         ("normal_threshold", C.c_float),
         ("track_threshold", C.c_float),
@@ -148,6 +152,7 @@ SIGNATURES = {
     "se8_tracker_create": (C.c_int, [C.POINTER(Pinhole), C.POINTER(TrackerConfig), C.POINTER(_P)]),
     "se8_tracker_destroy": (C.c_int, [_P]),
     "se8_track": (C.c_int, [_P, _P, _F, _P, _P, _I, _F, _P]),
+    "se8_tracker_read_pyramid": (C.c_int, [_P, C.c_int, _F, _F]),
     "se8_track_raycast": (C.c_int, [_P, _P, _P, _F, _I, _P]),
     "se8_icp_cost_grad_batched": (C.c_int, [_P, _P, _P, _P, _F, C.c_int, _F, _F, _F, _F, _I, _P]),
     "se8_icp_cost_grad_batched_noalloc": (C.c_int, [_P, _P, _P, _P, _F, C.c_int, _P, _P, _P, _P, _P, _P]),

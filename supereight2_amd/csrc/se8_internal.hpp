@@ -790,14 +790,14 @@ inline bool is_multires(const se8_map* m);
 struct se8_tracker {
     se8_pinhole sensor;
     se8_tracker_config cfg;
-    se8::DeviceBuffer vert_lvl[3], norm_lvl[3]; // float4 per pixel of each pyramid level
+    se8::DeviceBuffer vert_lvl[SE8_TRACKER_MAX_LEVELS], norm_lvl[SE8_TRACKER_MAX_LEVELS]; // float4 per pixel, per level
     se8::DeviceBuffer partials; // per-workgroup reduction partials
     se8::DeviceBuffer state;    // counters, flags, final sums
     se8::DeviceBuffer ref_points, ref_normals; // used by se8_track_raycast
     se8::DeviceBuffer adj_poses, adj_partials, adj_out; // batched cost/gradient evaluation
-    int num_wg[3] = {0, 0, 0};    // Gauss-Newton grid per level
-    int icp_ppt[3] = {1, 1, 1};   // pixels per thread per tile
-    int icp_tiles[3] = {0, 0, 0}; // tiles of 256 * ppt pixels
+    int num_wg[SE8_TRACKER_MAX_LEVELS] = {};    // Gauss-Newton grid per level
+    int icp_ppt[SE8_TRACKER_MAX_LEVELS] = {};   // pixels per thread per tile
+    int icp_tiles[SE8_TRACKER_MAX_LEVELS] = {}; // tiles of 256 * ppt pixels
     void* host_state = nullptr; // pinned readback of the device tracking state
     bool track_pending = false;
     se8_stage_times times{};

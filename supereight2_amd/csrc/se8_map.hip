@@ -910,9 +910,12 @@ extern "C" int se8_default_configs(se8_map_config* map_cfg, se8_tsdf_config* tsd
     }
     if (tracker_cfg) {
         // TrackerConfig defaults (tracker.hpp:22-27)
+        for (int l = 0; l < SE8_TRACKER_MAX_LEVELS; l++)
+            tracker_cfg->iterations[l] = 0;
         tracker_cfg->iterations[0] = 10;
         tracker_cfg->iterations[1] = 5;
         tracker_cfg->iterations[2] = 4;
+        tracker_cfg->num_levels = 3;
         tracker_cfg->dist_threshold = 0.1f;
         tracker_cfg->normal_threshold = 0.8f;
         tracker_cfg->track_threshold = 0.15f;

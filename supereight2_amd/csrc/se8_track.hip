@@ -35,6 +35,10 @@ constexpr int kIcpThreads = 256;
 constexpr int kIcpMaxWg = 320;
 constexpr int kPartialStride = kIcpMaxWg * 64; // floats per partial-row buffer
 constexpr int kPyrThreads = 256;
+constexpr int kMaxLevels = SE8_TRACKER_MAX_LEVELS;
+// Level-0 tile side of a k_pyramid workgroup: a multiple of 2^(L-1) for every supported level count
+constexpr int kPyrTile = 32;
+static_assert((kPyrTile >> (kMaxLevels - 1)) >= 1, "every level needs a tile pixel");
 
 // State handed from one Gauss-Newton launch to the next, double-buffered by launch index k:
 // launch k reads slot (k + 1) & 1 and its workgroup 0 writes slot k & 1.
@@ -42,7 +46,7 @@ struct IcpSlot {
     float T[12];       // pose of this launch's pixel pass, row-major 3x4
     float sums[32];    // last solved reduction
     double sums_d[32]; // last solved reduction (compensated path)
-    int done[4];       // per-level convergence flags
+    int done[kMaxLevels]; // per-level convergence flags
     int did_pixels;    // this launch ran its pixel pass: its partial rows are valid
     int level;         // level of this launch
     int rows;          // partial rows written (the launch's grid size)
@@ -85,94 +89,49 @@ struct IcpArgs {
     int* ok_dev;           // k_icp_finish: the check result also goes here when set
 };
 
-// half_sample_robust_image (preprocessor.cpp:156-195) of a w x h image at output pixel (x, y):
-// the mean of the 2x2 input pixels within e_d of the top-left one (indices clamped).
-template<class In>
-__device__ __forceinline__ float half_sample_at(const In& in, int w, int h, int x, int y, float e_d)
-{
-    const int ow = w / 2, oh = h / 2;
-    const int ix = 2 * x, iy = 2 * y;
-    const float c = in(ix, iy);
-    float count = 0.f, sum = 0.f;
-#pragma unroll
-    for (int i = 0; i <= 1; i++) {
-#pragma unroll
-        for (int j = 0; j <= 1; j++) {
-            const float v = in(min(ix + j, 2 * ow - 1), min(iy + i, 2 * oh - 1));
-            if (fabsf(v - c) < e_d) {
-                count += 1.f;
-                sum += v;
-            }
-        }
-    }
-    return sum / count;
-}
-
-// Depth of pyramid level 0, 1 or 2 at (x, y), recomputed from the full-resolution image: level 1
-// is the robust half-sample of level 0 and level 2 that of level 1, as the reference builds them
-// (tracker_impl.hpp:64-68), so recomputing gives the values a stored pyramid would hold.
-struct PyrDepth {
-    const float* d0;
-    int w0, h0;
-    float e_d;
-    __device__ __forceinline__ float l0(int x, int y) const
-    {
-        return d0[x + (size_t) y * w0];
-    }
-    __device__ __forceinline__ float l1(int x, int y) const
-    {
-        auto in = [this](int u, int v) { return l0(u, v); };
-        return half_sample_at(in, w0, h0, x, y, e_d);
-    }
-    __device__ __forceinline__ float l2(int x, int y) const
-    {
-        auto in = [this](int u, int v) { return l1(u, v); };
-        return half_sample_at(in, w0 >> 1, h0 >> 1, x, y, e_d);
-    }
-    __device__ __forceinline__ float at(int level, int x, int y) const
-    {
-        return level == 0 ? l0(x, y) : (level == 1 ? l1(x, y) : l2(x, y));
-    }
+// Backprojection of pyramid level l: PinholeCamera(sensor, 2^l) (pinhole_camera.cpp:29-60).
+struct PyrLevel {
+    int w, h;
+    float cx, cy, inv_fx, inv_fy;
+    float4* vert; // one float4 {x, y, z, 0} per pixel
+    float4* norm;
 };
 
-__device__ __forceinline__ void depth_point(const PyrDepth& pd, int level, const Cam& c, int x, int y, float p[3])
-{
-    const float d = pd.at(level, x, y);
-    if (d > 0.f) {
-        float dir[3];
-        cam_backproject(c, (float) x, (float) y, dir);
-        p[0] = d * dir[0];
-        p[1] = d * dir[1];
-        p[2] = d * dir[2];
-    }
-    else {
-        p[0] = p[1] = p[2] = 0.f;
-    }
-}
-
 struct PyrArgs {
-    PyrDepth depth;
-    Cam cam[3];         // PinholeCamera(sensor, 2^l) per level
-    float4* vert[3];
-    float4* norm[3];
-    int n[3];           // pixels per level
+    const float* d0;    // full-resolution depth, w0 x h0
+    float e_d;          // e_delta * 3 (tracker.hpp:20, tracker_impl.hpp:66-67)
+    PyrLevel lv[kMaxLevels];
     int levels;
-    int neg_y;
+    int neg_y;          // point_cloud_to_normal<true> (left-handed sensor frame)
+    int tiles_x;        // level-0 tiles per row
     IcpShared* init;    // when not null, thread 0 writes the state that the first launch reads
     Iso T;              // start pose, unless T16_dev is set
     const float* T16_dev;
 };
 
-// The whole input side of Tracker::track in one launch: for every pixel of every pyramid level,
-// depth_to_point_cloud (preprocessor_impl.hpp:18-40) and point_cloud_to_normal<NegY>
-// (preprocessor.cpp:109-152) on the robustly half-sampled depth (tracker_impl.hpp:64-75).
-// Threads [0, n2) are level-2 pixels, then level 1, then level 0 (with fewer levels, the coarsest
-// first).  Neighbour depths are recomputed from the full-resolution image (each level-2 pixel
-// reads at most 80 of its texels).
+// LDS floats of one k_pyramid workgroup: the level-0 region of side kPyrTile + 2 * 2^(L-1) (the
+// tile and a ring of 2^(L-1) pixels) and each coarser level's region of half the side.
+inline size_t pyramid_lds_bytes(int levels)
+{
+    size_t n = 0;
+    for (int l = 0, side = kPyrTile + (2 << (levels - 1)); l < levels; l++, side >>= 1)
+        n += (size_t) side * side;
+    return n * sizeof(float);
+}
+
+// The whole input side of Tracker::track (tracker_impl.hpp:50-75) in one launch.  A workgroup owns
+// an aligned kPyrTile x kPyrTile tile of the full-resolution image.  A pixel of level l is the
+// robust half-sample of an aligned 2^l x 2^l square of level 0 (half_sample_robust_image,
+// preprocessor.cpp:156-195: its index clamp never moves a pixel of the image), so the workgroup
+// stages its tile plus a ring of 2^(L-1) pixels in LDS, half-samples level by level in LDS (each
+// level keeps a ring of at least one pixel, which the normals of the tile need), then writes the
+// tile's vertices (depth_to_point_cloud, preprocessor_impl.hpp:18-40) and normals
+// (point_cloud_to_normal, preprocessor.cpp:109-152) at every level.  Pixels of a region outside the
+// image are never read by pixels inside it (the normals clamp their neighbours to the image).
 __global__ __launch_bounds__(kPyrThreads) void k_pyramid(PyrArgs a)
 {
-    int i = blockIdx.x * kPyrThreads + threadIdx.x;
-    if (a.init && i == 0) {
+    extern __shared__ float pyr[];
+    if (a.init && blockIdx.x == 0 && threadIdx.x == 0) {
         // the first Gauss-Newton launch (k = 0) reads slot 1
         IcpSlot* st = &a.init->slot[1];
         const Iso T = a.T16_dev ? iso_from16(a.T16_dev) : a.T;
@@ -187,48 +146,113 @@ __global__ __launch_bounds__(kPyrThreads) void k_pyramid(PyrArgs a)
             st->sums[k] = 0.f;
             st->sums_d[k] = 0.0;
         }
-        for (int l = 0; l < 4; l++)
+        for (int l = 0; l < kMaxLevels; l++)
             st->done[l] = 0;
         st->did_pixels = 0;
         st->level = 0;
         st->rows = 0;
         st->iterations = 0;
     }
-    // coarsest level first: its pixels cost the most (each recomputes up to 80 texels), and
-    // dispatching them first keeps them from forming the tail of the launch
-    int level = a.levels - 1;
-    while (level >= 0 && i >= a.n[level]) {
-        i -= a.n[level];
-        level--;
-    }
-    if (level < 0)
-        return;
-    const Cam& c = a.cam[level];
-    const int x = i % c.w, y = i / c.w;
-    float p[3];
-    depth_point(a.depth, level, c, x, y, p);
-    float n[3] = {0.f, 0.f, 0.f};
-    if (p[2] != 0.f) {
-        const int xl = max(x - 1, 0), xr = min(x + 1, c.w - 1);
-        const int ym = max(y - 1, 0), yp = min(y + 1, c.h - 1);
-        const int yu = a.neg_y ? ym : yp, yd = a.neg_y ? yp : ym;
-        float l[3], r[3], u[3], d[3];
-        depth_point(a.depth, level, c, xl, y, l);
-        depth_point(a.depth, level, c, xr, y, r);
-        depth_point(a.depth, level, c, x, yu, u);
-        depth_point(a.depth, level, c, x, yd, d);
-        if (l[2] != 0.f && r[2] != 0.f && u[2] != 0.f && d[2] != 0.f) {
-            float dx[3], dy[3], cr[3];
-            for (int k = 0; k < 3; k++) {
-                dx[k] = r[k] - l[k];
-                dy[k] = u[k] - d[k];
-            }
-            cross3(dx, dy, cr);
-            normalized3(cr, n);
+    const int L = a.levels;
+    const int S = 1 << (L - 1);                 // ring at level 0
+    const int tx = blockIdx.x % a.tiles_x, ty = blockIdx.x / a.tiles_x;
+    const int ox = tx * kPyrTile - S, oy = ty * kPyrTile - S; // level-0 region origin, a multiple of 2^(L-1)
+    int side = kPyrTile + 2 * S;
+    {
+        const int w0 = a.lv[0].w, h0 = a.lv[0].h;
+        for (int i = threadIdx.x; i < side * side; i += kPyrThreads) {
+            const int gx = ox + i % side, gy = oy + i / side;
+            pyr[i] = (gx >= 0 && gx < w0 && gy >= 0 && gy < h0) ? a.d0[gx + (size_t) gy * w0] : 0.f;
         }
     }
-    a.vert[level][i] = make_float4(p[0], p[1], p[2], 0.f);
-    a.norm[level][i] = make_float4(n[0], n[1], n[2], 0.f);
+    int off = 0;
+    for (int l = 1; l < L; l++) {
+        __syncthreads();
+        const int poff = off, pside = side;
+        off += side * side;
+        side >>= 1;
+        const int lox = ox >> l, loy = oy >> l; // exact: ox, oy are multiples of 2^l
+        for (int i = threadIdx.x; i < side * side; i += kPyrThreads) {
+            const int rx = i % side, ry = i / side;
+            const int gx = lox + rx, gy = loy + ry;
+            float out = 0.f;
+            if (gx >= 0 && gx < a.lv[l].w && gy >= 0 && gy < a.lv[l].h) {
+                const float* in = pyr + poff + 2 * ry * pside + 2 * rx;
+                const float c = in[0];
+                float count = 0.f, sum = 0.f;
+#pragma unroll
+                for (int ii = 0; ii <= 1; ii++) {
+#pragma unroll
+                    for (int jj = 0; jj <= 1; jj++) {
+                        const float v = in[ii * pside + jj];
+                        if (fabsf(v - c) < a.e_d) {
+                            count += 1.f;
+                            sum += v;
+                        }
+                    }
+                }
+                out = sum / count;
+            }
+            pyr[off + i] = out;
+        }
+    }
+    __syncthreads();
+    // vertices and normals of the tile's pixels, all levels: level l has (kPyrTile >> l)^2 of them
+    int total = 0;
+    for (int l = 0; l < L; l++)
+        total += (kPyrTile >> l) * (kPyrTile >> l);
+    for (int i = threadIdx.x; i < total; i += kPyrThreads) {
+        int l = 0, j = i, loff = 0, lside = kPyrTile + 2 * S;
+        while (j >= (kPyrTile >> l) * (kPyrTile >> l)) {
+            j -= (kPyrTile >> l) * (kPyrTile >> l);
+            loff += lside * lside;
+            lside >>= 1;
+            l++;
+        }
+        const PyrLevel& P = a.lv[l];
+        const int tside = kPyrTile >> l, ring = S >> l;
+        const int gx = (ox >> l) + ring + j % tside, gy = (oy >> l) + ring + j / tside;
+        if (gx >= P.w || gy >= P.h)
+            continue;
+        const float* D = pyr + loff;
+        const int rox = ox >> l, roy = oy >> l; // region origin at level l
+        auto point = [&](int x, int y, float p[3]) {
+            const float d = D[(x - rox) + (y - roy) * lside];
+            if (d > 0.f) {
+                const float dir[3] = {((float) x - P.cx) * P.inv_fx, ((float) y - P.cy) * P.inv_fy, 1.f};
+                p[0] = d * dir[0];
+                p[1] = d * dir[1];
+                p[2] = d * dir[2];
+            }
+            else
+                p[0] = p[1] = p[2] = 0.f;
+        };
+        float p[3];
+        point(gx, gy, p);
+        float n[3] = {0.f, 0.f, 0.f};
+        if (p[2] != 0.f) {
+            const int xl = max(gx - 1, 0), xr = min(gx + 1, P.w - 1);
+            const int ym = max(gy - 1, 0), yp = min(gy + 1, P.h - 1);
+            const int yu = a.neg_y ? ym : yp, yd = a.neg_y ? yp : ym;
+            float pl[3], pr[3], pu[3], pd[3];
+            point(xl, gy, pl);
+            point(xr, gy, pr);
+            point(gx, yu, pu);
+            point(gx, yd, pd);
+            if (pl[2] != 0.f && pr[2] != 0.f && pu[2] != 0.f && pd[2] != 0.f) {
+                float dx[3], dy[3], cr[3];
+                for (int k = 0; k < 3; k++) {
+                    dx[k] = pr[k] - pl[k];
+                    dy[k] = pu[k] - pd[k];
+                }
+                cross3(dx, dy, cr);
+                normalized3(cr, n);
+            }
+        }
+        const size_t o = gx + (size_t) gy * P.w;
+        P.vert[o] = make_float4(p[0], p[1], p[2], 0.f);
+        P.norm[o] = make_float4(n[0], n[1], n[2], 0.f);
+    }
 }
 
 __device__ __forceinline__ Iso pose_of(const float* T12)
@@ -858,7 +882,7 @@ __global__ __launch_bounds__(kIcpThreads) void k_icp_iter(IcpArgs a)
             out->sums[t] = cur.sums[t];
             out->sums_d[t] = cur.sums_d[t];
         }
-        if (t < 4)
+        if (t < kMaxLevels)
             out->done[t] = cur.done[t];
         if (t == 0) {
             out->did_pixels = run ? 1 : 0;
@@ -912,26 +936,28 @@ static void launch_pyramid(se8_tracker* tr, const float* depth, int levels, IcpS
                            const float* T16_dev, hipStream_t s)
 {
     const se8_pinhole& P = tr->sensor;
-    PyrArgs pa;
-    pa.depth.d0 = depth;
-    pa.depth.w0 = P.width;
-    pa.depth.h0 = P.height;
-    pa.depth.e_d = 0.1f * 3; // e_delta * 3 (tracker.hpp:20, tracker_impl.hpp:66-67)
-    int total = 0;
-    for (int l = 0; l < 3; l++) {
-        pa.cam[l] = make_cam(P, 1 << l, true);
-        pa.vert[l] = (float4*) tr->vert_lvl[l].ptr;
-        pa.norm[l] = (float4*) tr->norm_lvl[l].ptr;
-        pa.n[l] = pa.cam[l].w * pa.cam[l].h;
-        if (l < levels)
-            total += pa.n[l];
+    PyrArgs pa{};
+    pa.d0 = depth;
+    pa.e_d = 0.1f * 3; // e_delta * 3 (tracker.hpp:20, tracker_impl.hpp:66-67)
+    for (int l = 0; l < levels; l++) {
+        const Cam c = make_cam(P, 1 << l, true);
+        pa.lv[l].w = c.w;
+        pa.lv[l].h = c.h;
+        pa.lv[l].cx = c.cx;
+        pa.lv[l].cy = c.cy;
+        pa.lv[l].inv_fx = c.inv_fx;
+        pa.lv[l].inv_fy = c.inv_fy;
+        pa.lv[l].vert = (float4*) tr->vert_lvl[l].ptr;
+        pa.lv[l].norm = (float4*) tr->norm_lvl[l].ptr;
     }
     pa.levels = levels;
     pa.neg_y = make_cam(P, 1, false).left_hand_frame;
+    pa.tiles_x = (P.width + kPyrTile - 1) / kPyrTile;
+    const int tiles_y = (P.height + kPyrTile - 1) / kPyrTile;
     pa.init = init;
     pa.T = T;
     pa.T16_dev = T16_dev;
-    k_pyramid<<<(total + kPyrThreads - 1) / kPyrThreads, kPyrThreads, 0, s>>>(pa);
+    k_pyramid<<<pa.tiles_x * tiles_y, kPyrThreads, pyramid_lds_bytes(levels), s>>>(pa);
 }
 
 template<bool Comp>
@@ -961,7 +987,7 @@ static int enqueue_track(se8_tracker* tr, const float* depth, const float* T16_h
     // the previous call's stage events, before ev[0] / ev[1] are recorded again
     harvest_tracking(tr);
     SE8_HIP_CHECK(hipEventRecord(tr->ev[0], s));
-    launch_pyramid(tr, depth, 3, sh, T16_host ? iso_from16(T16_host) : Iso{}, T_in_dev, s);
+    launch_pyramid(tr, depth, tr->cfg.num_levels, sh, T16_host ? iso_from16(T16_host) : Iso{}, T_in_dev, s);
     IcpArgs a{};
     a.ref_pts = ref_pts;
     a.ref_nrm = ref_nrm;
@@ -973,7 +999,7 @@ static int enqueue_track(se8_tracker* tr, const float* depth, const float* T16_h
     a.icp_th = tr->cfg.icp_threshold;
     a.track_th = tr->cfg.track_threshold;
     a.k = 0;
-    for (int level = 2; level >= 0; level--) {
+    for (int level = tr->cfg.num_levels - 1; level >= 0; level--) {
         a.vert = (const float4*) tr->vert_lvl[level].ptr;
         a.norm = (const float4*) tr->norm_lvl[level].ptr;
         // reduction resolution is the full resolution / 2^level (tracker_impl.hpp:85-86)
@@ -1142,15 +1168,21 @@ using namespace se8;
 extern "C" int se8_tracker_create(const se8_pinhole* sensor, const se8_tracker_config* cfg, se8_tracker** out)
 {
     SE8_REQUIRE(sensor && cfg && out, "se8_tracker_create: null argument");
-    SE8_REQUIRE(sensor->width >= 4 && sensor->height >= 4, "se8_tracker_create: image too small");
-    for (int l = 0; l < 3; l++)
+    SE8_REQUIRE(sensor->width >= 1 && sensor->height >= 1, "se8_tracker_create: empty image");
+    SE8_REQUIRE(cfg->num_levels >= 1 && cfg->num_levels <= kMaxLevels,
+                "se8_tracker_create: num_levels must be in [1, SE8_TRACKER_MAX_LEVELS]");
+    SE8_REQUIRE((sensor->width >> (cfg->num_levels - 1)) >= 1 && (sensor->height >> (cfg->num_levels - 1)) >= 1,
+                "se8_tracker_create: the coarsest pyramid level has no pixels");
+    for (int l = 0; l < cfg->num_levels; l++)
         SE8_REQUIRE(cfg->iterations[l] >= 0 && cfg->iterations[l] < 1000, "se8_tracker_create: bad iterations");
     *out = nullptr;
     se8_tracker* t = new se8_tracker();
     t->sensor = *sensor;
     t->cfg = *cfg;
+    for (int l = cfg->num_levels; l < kMaxLevels; l++)
+        t->cfg.iterations[l] = 0;
     int rc = SE8_OK;
-    for (int l = 0; l < 3; l++) {
+    for (int l = 0; l < cfg->num_levels; l++) {
         const Cam cl = make_cam(*sensor, 1 << l, true);
         const size_t n = (size_t) cl.w * cl.h;
         rc = rc ? rc : dev_alloc(t->vert_lvl[l], sizeof(float4) * n);
@@ -1185,7 +1217,7 @@ extern "C" int se8_tracker_destroy(se8_tracker* t)
     if (!t)
         return SE8_OK;
     (void) hipDeviceSynchronize();
-    for (int l = 0; l < 3; l++) {
+    for (int l = 0; l < kMaxLevels; l++) {
         dev_free(t->vert_lvl[l]);
         dev_free(t->norm_lvl[l]);
     }
@@ -1219,6 +1251,29 @@ extern "C" int se8_track_dev(se8_tracker* t, const float* depth_dev, const float
                 "se8_track_dev: null argument");
     return enqueue_track(t, depth_dev, nullptr, T_in_dev, T_out_dev, ok_dev, ref_points_dev, ref_normals_dev,
                          (hipStream_t) stream);
+}
+
+extern "C" int se8_tracker_read_pyramid(const se8_tracker* t, int level, float* points, float* normals)
+{
+    SE8_REQUIRE(t, "se8_tracker_read_pyramid: null tracker");
+    SE8_REQUIRE(level >= 0 && level < t->cfg.num_levels, "se8_tracker_read_pyramid: level out of range");
+    const Cam c = make_cam(t->sensor, 1 << level, true);
+    const size_t n = (size_t) c.w * c.h;
+    std::vector<float4> buf(n);
+    const DeviceBuffer* src[2] = {&t->vert_lvl[level], &t->norm_lvl[level]};
+    float* dst[2] = {points, normals};
+    for (int k = 0; k < 2; k++) {
+        if (!dst[k])
+            continue;
+        SE8_HIP_CHECK(hipDeviceSynchronize());
+        SE8_HIP_CHECK(hipMemcpy(buf.data(), src[k]->ptr, sizeof(float4) * n, hipMemcpyDeviceToHost));
+        for (size_t i = 0; i < n; i++) {
+            dst[k][3 * i] = buf[i].x;
+            dst[k][3 * i + 1] = buf[i].y;
+            dst[k][3 * i + 2] = buf[i].z;
+        }
+    }
+    return SE8_OK;
 }
 
 extern "C" int se8_track_raycast(se8_tracker* t, const se8_map* m, const float* depth_dev, float T_WS[16], int* ok,

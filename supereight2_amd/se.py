@@ -117,8 +117,13 @@ class TrackerConfig:
     compensated: bool = False
 
     def c(self) -> capi.TrackerConfig:
-        return capi.TrackerConfig((C.c_int * 3)(*self.iterations), self.dist_threshold, self.normal_threshold,
-                                  self.track_threshold, self.icp_threshold, int(self.compensated))
+        it = list(self.iterations)
+        if not 1 <= len(it) <= capi.TRACKER_MAX_LEVELS:
+            raise ValueError(f"1 to {capi.TRACKER_MAX_LEVELS} pyramid levels are supported, got {len(it)}")
+        padded = it + [0] * (capi.TRACKER_MAX_LEVELS - len(it))
+        return capi.TrackerConfig((C.c_int * capi.TRACKER_MAX_LEVELS)(*padded), len(it), self.dist_threshold,
+                                  self.normal_threshold, self.track_threshold, self.icp_threshold,
+                                  int(self.compensated))
 
 
 class TSDFMap:
@@ -520,6 +525,16 @@ class Tracker:
                                           _dev_f32(normals_W, shape + (3,)), C.byref(ok), capi.fptr(sums),
                                           _stream_handle(stream)), "se8_track")
         return bool(ok.value), np.array(T[:], np.float32).reshape(4, 4), sums
+
+    def read_pyramid(self, level):
+        """The input vertex and normal maps of pyramid level `level` from the last track call
+        ((H >> level, W >> level, 3) float32 each, host)."""
+        h, w = self.sensor.height >> level, self.sensor.width >> level
+        pts = np.zeros((h, w, 3), np.float32)
+        nrm = np.zeros((h, w, 3), np.float32)
+        capi.check(self.lib.se8_tracker_read_pyramid(self.h, int(level), capi.fptr(pts), capi.fptr(nrm)),
+                   "se8_tracker_read_pyramid")
+        return pts, nrm
 
     def track_device(self, depth, T_in, T_out, points_W, normals_W, ok=None, stream=None):
         """Tracker::track with device poses (float32 CUDA tensors of 16 values; T_in may be T_out):

@@ -32,7 +32,8 @@ def test_default_configs_match_reference_defaults():
     m, t, p, tr = capi.MapConfig(), capi.TsdfConfig(), capi.Pinhole(), capi.TrackerConfig()
     assert lib.se8_default_configs(ctypes.byref(m), ctypes.byref(t), ctypes.byref(p), ctypes.byref(tr)) == 0
     assert t.truncation_boundary_factor == 8 and t.max_weight == 100
-    assert list(tr.iterations) == [10, 5, 4]
+    assert tr.num_levels == 3 and list(tr.iterations)[:3] == [10, 5, 4]
+    assert not any(list(tr.iterations)[3:])
     assert abs(tr.dist_threshold - 0.1) < 1e-7 and abs(tr.normal_threshold - 0.8) < 1e-7
     assert abs(tr.track_threshold - 0.15) < 1e-7 and abs(tr.icp_threshold - 1e-5) < 1e-12
 
