@@ -56,6 +56,7 @@ def parse():
     ap.add_argument("--seed", type=int, default=10)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-fp16", action="store_true")
+    ap.add_argument("--no-fp64", action="store_true", help="skip the FP64-oracle error leg (about a minute of host time)")
     ap.add_argument("--cpu-frames", type=int, default=8, help="frames of the CPU baseline sample")
     ap.add_argument("--workload", choices=["room", "street"], default="room",
                     help="room: the 640x480 track+integrate+raycast loop (default); street: 200 scans of a "
@@ -190,58 +191,78 @@ def mesh_bench(gmap, torch, se, cpu):
     return out
 
 
-def run_pipeline(depth_dev, gt, n_frames, warmup, storage, compensated, torch, se, sensor, device_pose=True,
-                 mesh=None):
-    """Runs the per-frame loop.  With device_pose (the default), each frame's pose stays in device
-    memory: track writes pose k from pose k - 1 and integrate / raycast read it there, so the host
-    only enqueues work and never waits inside the loop; the poses are read back after it.  Without
-    it, the loop is the reference API's (track returns the pose to the host, which passes it on).
-    Per-stage device times come from the HIP events the library records on the stream around each
-    stage, accumulated by the library over the timed frames.  Returns a dict of results."""
+def run_pipeline(depth_dev, gt, n_frames, warmup, steps, storage, compensated, torch, se, sensor, timing="off",
+                 device_pose=True, keep_points=False, dump=False, mesh=None):
+    """Runs the per-frame loop over all n_frames frames: frames [0, warmup) untimed, exactly `steps`
+    frames [warmup, warmup + steps) timed, the rest of the sequence untimed after them (its time is
+    kept separately for the whole-sequence rate).  With device_pose (the default), each frame's pose
+    stays in device memory: track writes pose k from pose k - 1 and integrate / raycast read it there,
+    so the host only enqueues work.  Without it, the loop is the reference API's (track returns the
+    pose to the host, which passes it on).  `timing` is the library's stage-timing mode: "off" (no
+    events, no host waits: the throughput figure), "stages" or "levels" (per-stage and per-pyramid-
+    level device times from HIP events, accumulated over the timed frames).  keep_points keeps every
+    frame's raycast vertex map on the device; dump returns the final map's blocks."""
     H, W = sensor.height, sensor.width
     gmap = se.TSDFMap(10.24, 0.01, storage=storage)
     integ = se.MapIntegrator(gmap)
     tracker = se.Tracker(gmap, sensor, se.TrackerConfig(compensated=compensated))
-    pts = [torch.zeros((H, W, 3), dtype=torch.float32, device="cuda") for _ in range(2)]
-    nrm = [torch.zeros((H, W, 3), dtype=torch.float32, device="cuda") for _ in range(2)]
+    se.set_stage_timing(gmap, tracker, timing)
+    if keep_points:
+        pts = torch.zeros((n_frames, H, W, 3), dtype=torch.float32, device="cuda")
+    else:
+        pts = torch.zeros((2, H, W, 3), dtype=torch.float32, device="cuda")
+    nrm = torch.zeros((2, H, W, 3), dtype=torch.float32, device="cuda")
     T = gt[0].astype(np.float32)
     T_dev = torch.zeros((n_frames, 16), dtype=torch.float32, device="cuda")
     T_dev[0] = torch.from_numpy(T.reshape(16))
     poses = []
     torch.cuda.synchronize()
-    t0 = None
+    t0 = t1 = t2 = None
     for k in range(n_frames):
         if k == warmup:
             torch.cuda.synchronize()
             se.stage_times_reset(gmap, tracker)
             t0 = time.perf_counter()
+        if k == warmup + steps:
+            torch.cuda.synchronize()
+            t1 = time.perf_counter()
+            st = se.stage_times(gmap, tracker)
+        pk = k if keep_points else k % 2
+        pp = (k - 1) if keep_points else (k + 1) % 2
         cur, prev = k % 2, (k + 1) % 2
         if device_pose:
             if k > 0:
-                tracker.track_device(depth_dev[k], T_dev[k - 1], T_dev[k], pts[prev], nrm[prev])
+                tracker.track_device(depth_dev[k], T_dev[k - 1], T_dev[k], pts[pp], nrm[prev])
             integ.integrateDepth(k, depth_dev[k], sensor, T_dev[k])
-            se.raycast_volume(gmap, sensor, T_dev[k], pts[cur], nrm[cur])
+            se.raycast_volume(gmap, sensor, T_dev[k], pts[pk], nrm[cur])
         else:
             if k > 0:
-                _, T, _ = tracker.track(depth_dev[k], T, pts[prev], nrm[prev])
+                _, T, _ = tracker.track(depth_dev[k], T, pts[pp], nrm[prev])
             integ.integrateDepth(k, depth_dev[k], sensor, T)
-            se.raycast_volume(gmap, sensor, T, pts[cur], nrm[cur])
+            se.raycast_volume(gmap, sensor, T, pts[pk], nrm[cur])
             poses.append(T.copy())
     torch.cuda.synchronize()
-    t1 = time.perf_counter()
+    t2 = time.perf_counter()
+    if t1 is None:
+        t1 = t2
+        st = se.stage_times(gmap, tracker)
     if device_pose:
         poses = list(T_dev.cpu().numpy().reshape(n_frames, 4, 4))
-    st = se.stage_times(gmap, tracker)
-    timed = n_frames - warmup
     info = gmap.info()
     mesh_out = mesh_bench(gmap, torch, se, mesh == "cpu") if mesh else None
+    blocks = gmap.dump_blocks() if dump else None
     tracker.close()
     gmap.close()
     ni, nr, nt = max(st["n_integrations"], 1), max(st["n_raycasts"], 1), max(st["n_tracks"], 1)
+    rest = n_frames - warmup - steps
     return {
-        "fps": timed / (t1 - t0) if timed > 0 else 0.0,
+        "fps": steps / (t1 - t0) if steps > 0 else 0.0,
         "seconds": t1 - t0,
+        "sequence_fps": (n_frames - warmup) / (t2 - t0),
+        "rest_frames": rest,
         "poses": poses,
+        "points": pts if keep_points else None,
+        "blocks_dump": blocks,
         "blocks": int(info.num_blocks),
         "stage_ms": {
             "fetch_frustum": st["sum_fetch_frustum"] / ni,
@@ -251,6 +272,7 @@ def run_pipeline(depth_dev, gt, n_frames, warmup, storage, compensated, torch, s
             "raycast": st["sum_raycast"] / nr,
             "tracking": st["sum_tracking"] / nt,
         },
+        "track_level_ms": [st["sum_track_level"][l] / max(st["n_track_level_launches"][l], 1) for l in range(3)],
         "counts": {"integrations": st["n_integrations"], "raycasts": st["n_raycasts"], "tracks": st["n_tracks"]},
         "update_blocks_per_frame": st["sum_update_blocks"] / ni,
         "candidates_per_frame": st["sum_candidates"] / ni,
@@ -462,6 +484,116 @@ def cpu_model():
     return platform.processor() or "unknown"
 
 
+FIXTURE = os.path.join(ROOT, "tests", "golden", "raycast_touched_spline300_seed10.json")
+
+
+def fp64_errors(depth_host, gt, runs, sensor, mu, log_every=50):
+    """The FP64 oracle (the C restatement in double precision) runs the reference loop on the same
+    frames (track from the previous estimate, integrate, raycast); each GPU run is compared with it:
+    tracked poses, every frame's raycast vertex map, and the final TSDF (per voxel observed in both
+    maps, split into voxels with the same weight, i.e. the same number of updates, and the count
+    whose weights differ)."""
+    from oracle import oracle as O
+
+    O.set_threads(os.cpu_count() or 1)
+    pin = O.pinhole(sensor.width, sensor.height, sensor.fx, sensor.fy, sensor.cx, sensor.cy, sensor.near_plane,
+                    sensor.far_plane)
+    m = O.OracleMap(10.24, 0.01, precision="f64")
+    n = len(depth_host)
+    T = gt[0].astype(np.float64)
+    p64 = n64 = None
+    poses64 = []
+    acc = {name: {"vmax": 0.0, "sq": 0.0, "cnt": 0, "p999": [], "hit_mismatch": 0} for name in runs}
+    t0 = time.perf_counter()
+    for k in range(n):
+        if k > 0:
+            _, T, _, _ = O.track(depth_host[k], pin, T, p64, n64, precision="f64")
+        m.integrate(k, depth_host[k], pin, T)
+        p64, n64, _ = m.raycast(pin, T)
+        poses64.append(T.copy())
+        h64 = np.any(p64 != 0, axis=-1)
+        for name, r in runs.items():
+            g = r["points"][k].cpu().numpy()
+            hg = np.any(g != 0, axis=-1)
+            both = hg & h64
+            e = np.linalg.norm(g[both].astype(np.float64) - p64[both], axis=-1)
+            a = acc[name]
+            if e.size:
+                a["vmax"] = max(a["vmax"], float(e.max()))
+                a["p999"].append(float(np.quantile(e, 0.999)))
+                a["sq"] += float(np.sum(e * e))
+                a["cnt"] += int(e.size)
+            a["hit_mismatch"] += int(np.sum(hg != h64))
+        if log_every and k % log_every == 0:
+            print(f"fp64 oracle frame {k} ({time.perf_counter() - t0:.0f} s)", file=sys.stderr, flush=True)
+    keys64, _, t64, w64 = m.dump_blocks()
+    out = {"frames": n, "seconds": time.perf_counter() - t0, "threads": os.cpu_count() or 1}
+    ate64 = ate(poses64, gt)
+    out["ate64_m"] = ate64
+    for name, r in runs.items():
+        pg = r["poses"]
+        dt = [float(np.linalg.norm(np.asarray(a)[:3, 3] - b[:3, 3])) for a, b in zip(pg, poses64)]
+        dr = [float(np.degrees(np.arccos(np.clip((np.trace(np.asarray(a)[:3, :3].T @ b[:3, :3]) - 1) / 2, -1, 1))))
+              for a, b in zip(pg, poses64)]
+        keys, _, tg, wg = r["blocks_dump"]
+        common, ig, io = np.intersect1d(keys, keys64, return_indices=True)
+        tg_c, wg_c, to_c, wo_c = tg[ig], wg[ig], t64[io], w64[io]
+        obs = (wg_c > 0) & (wo_c > 0)
+        same = obs & (wg_c == wo_c)
+        d_same = (tg_c[same].astype(np.float64) - to_c[same]) * mu
+        d_all = (tg_c[obs].astype(np.float64) - to_c[obs]) * mu
+        a = acc[name]
+        out[name] = {
+            "tsdf_rms_m": float(np.sqrt(np.mean(d_same ** 2))) if d_same.size else None,
+            "tsdf_max_m": float(np.abs(d_same).max()) if d_same.size else None,
+            "tsdf_voxels": int(same.sum()),
+            "tsdf_voxels_weight_differs": int((obs & (wg_c != wo_c)).sum()),
+            "tsdf_rms_all_observed_m": float(np.sqrt(np.mean(d_all ** 2))) if d_all.size else None,
+            "blocks_only_gpu": int(len(keys) - len(common)), "blocks_only_fp64": int(len(keys64) - len(common)),
+            "vertex_max_m": a["vmax"],
+            "vertex_rms_m": float(np.sqrt(a["sq"] / max(a["cnt"], 1))),
+            "vertex_p999_max_m": float(max(a["p999"])) if a["p999"] else None,
+            "hit_mismatch_px_per_frame": a["hit_mismatch"] / n,
+            "pose_max_m": float(max(dt)), "pose_max_deg": float(max(dr)), "pose_final_m": dt[-1],
+        }
+    return out
+
+
+def rooflines(r, fixture, cam, warmup, steps, s_v):
+    """Algorithmic bytes (definitions in the workload fixture's header and DESIGN) over measured
+    device time for the raycast, each ICP level, the update and the whole frame."""
+    W, H = cam.width, cam.height
+    fr = fixture["frames"][warmup: warmup + steps]
+    ray_bytes = float(np.mean([24 * W * H + s_v * f["raycast_voxels"] + 8 * f["raycast_blocks"] for f in fr]))
+    out = {}
+    st = r["stage_ms"]
+
+    def entry(kernel, nbytes, ms):
+        gbs = nbytes / (ms * 1e-3) / 1e9 if ms > 0 else 0.0
+        return {"bound": "hbm", "kernel": kernel, "achieved": gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": gbs / HBM_PEAK_GBS, "bytes_per_launch": nbytes, "us_per_launch": ms * 1000.0}
+
+    out["k_raycast"] = entry("k_raycast", ray_bytes, st["raycast"])
+    upd_bytes = 2 * s_v * 512 * r["update_blocks_per_frame"] + 4 * W * H
+    out["k_update"] = entry("k_update", upd_bytes, st["update"])
+    icp_iters = np.mean([f["icp_iterations"] for f in fr if "icp_iterations" in f], axis=0)
+    icp_frame_bytes = 0.0
+    for l in range(3):
+        n_l = (W >> l) * (H >> l)
+        out[f"k_icp_iter_level{l}"] = entry(f"k_icp_iter (level {l})", 48.0 * n_l, r["track_level_ms"][l])
+        icp_frame_bytes += icp_iters[l] * 48.0 * n_l
+    cand = r["candidates_per_frame"]
+    new_blocks = float(np.mean([f["new_blocks"] for f in fr]))
+    alloc_bytes = 4 * W * H + 16 * cand + 16 * new_blocks
+    frame_bytes = alloc_bytes + upd_bytes + ray_bytes + icp_frame_bytes + 4 * W * H
+    out["frame"] = {"bound": "hbm", "achieved": frame_bytes * r["fps"] / 1e9, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                    "frac": frame_bytes * r["fps"] / 1e9 / HBM_PEAK_GBS, "bytes_per_frame": frame_bytes,
+                    "parts": {"allocation": alloc_bytes, "update": upd_bytes, "raycast": ray_bytes,
+                              "icp": icp_frame_bytes, "pyramid": 4 * W * H},
+                    "icp_iterations_per_level": icp_iters.tolist()}
+    return out
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -480,52 +612,67 @@ def main():
         return
     cam = synth.Camera()
     sensor = se.PinholeCamera(cam.width, cam.height, cam.fx, cam.fy, cam.cx, cam.cy, cam.near_plane, cam.far_plane)
-    n_frames = min(args.warmup + args.steps, N_FRAMES_DEFAULT) if args.warmup + args.steps <= N_FRAMES_DEFAULT \
-        else args.warmup + args.steps
+    # the whole 300-frame sequence always runs; exactly `steps` frames after `warmup` are timed
+    n_frames = max(N_FRAMES_DEFAULT, args.warmup + args.steps)
+    warmup, steps = args.warmup, args.steps
     scene = synth.make_scene(args.seed)
     gt = np.stack(synth.spline_trajectory(max(n_frames, 2), seed=args.seed + 1))[:n_frames]
     print(f"rendering {n_frames} frames", file=sys.stderr, flush=True)
     depth_dev = synth.render_sequence_torch(scene, cam, gt, seed=args.seed + 1000)
     torch.cuda.synchronize()
+    with open(FIXTURE) as f:
+        fixture = json.load(f)
+    fp64 = not args.no_fp64 and args.seed == 10
 
     print("fp32 pipeline", file=sys.stderr, flush=True)
-    r32 = run_pipeline(depth_dev, gt, n_frames, args.warmup, "f32", False, torch, se, sensor,
-                       mesh="gpu" if args.no_cpu_baseline else "cpu")
+    r32 = run_pipeline(depth_dev, gt, n_frames, warmup, steps, "f32", False, torch, se, sensor, timing="off",
+                       keep_points=fp64, dump=fp64, mesh="gpu" if args.no_cpu_baseline else "cpu")
     ate32 = ate(r32["poses"], gt)
+    print("fp32 pipeline, stage and level timing", file=sys.stderr, flush=True)
+    r32t = run_pipeline(depth_dev, gt, n_frames, warmup, steps, "f32", False, torch, se, sensor, timing="levels")
     print("fp32 pipeline, host poses", file=sys.stderr, flush=True)
-    r32h = run_pipeline(depth_dev, gt, n_frames, args.warmup, "f32", False, torch, se, sensor, device_pose=False)
-    r16 = r16p = None
+    r32h = run_pipeline(depth_dev, gt, n_frames, warmup, steps, "f32", False, torch, se, sensor, timing="off",
+                        device_pose=False)
+    r16 = r16t = r16p = None
     if not args.no_fp16:
         print("fp16 pipeline", file=sys.stderr, flush=True)
-        r16 = run_pipeline(depth_dev, gt, n_frames, args.warmup, "f16", True, torch, se, sensor)
+        r16 = run_pipeline(depth_dev, gt, n_frames, warmup, steps, "f16", True, torch, se, sensor, timing="off",
+                           keep_points=fp64, dump=fp64)
+        print("fp16 pipeline, stage and level timing", file=sys.stderr, flush=True)
+        r16t = run_pipeline(depth_dev, gt, n_frames, warmup, steps, "f16", True, torch, se, sensor, timing="levels")
         print("fp16 storage, plain ICP", file=sys.stderr, flush=True)
-        r16p = run_pipeline(depth_dev, gt, n_frames, args.warmup, "f16", False, torch, se, sensor)
+        r16p = run_pipeline(depth_dev, gt, n_frames, warmup, steps, "f16", False, torch, se, sensor, timing="off")
 
-    # roofline of the update kernel (the HBM stream of the path): algorithmic bytes per launch
-    # = 2 * s_v * 512 * N_blk + 4 * W * H, s_v = 8 B/voxel (fp32 tsdf + fp32 weight), N_blk = the
-    # frame's update-list length; divided by the kernel's mean HIP-event duration
-    def update_roofline(r, s_v):
-        upd_ms = r["stage_ms"]["update"]
-        nbytes = 2 * s_v * 512 * r["update_blocks_per_frame"] + 4 * cam.width * cam.height
-        gbs = nbytes / (upd_ms * 1e-3) / 1e9 if upd_ms > 0 else 0.0
-        return nbytes, gbs
-
-    upd_bytes, upd_gbs = update_roofline(r32, 8)
-    # HBM traffic per launch of the same kernel on this workload, measured by the FETCH_SIZE and
-    # WRITE_SIZE passes recorded in profiles/k_update_tracking300_fp32_fp16.md (scaled by a 256 MiB
-    # copy); rescaled by this run's update-list length, null when that record is absent
-    traffic = None
-    tpath = os.path.join(ROOT, "profiles", "k_update_traffic.json")
-    if os.path.exists(tpath):
-        with open(tpath) as f:
-            tr = json.load(f)
-        traffic = tr["traffic_bytes_per_launch"] * r32["update_blocks_per_frame"] / tr["update_blocks_per_frame"]
+    # stage times and level times come from the timed-event runs on the same frames
+    r32["stage_ms"], r32["track_level_ms"] = r32t["stage_ms"], r32t["track_level_ms"]
+    r32["update_blocks_per_frame"], r32["candidates_per_frame"] = r32t["update_blocks_per_frame"], \
+        r32t["candidates_per_frame"]
+    roof32 = rooflines(r32, fixture, cam, warmup, steps, 8)
+    # HBM traffic per launch of the raycast and update kernels on this workload, measured by the
+    # FETCH_SIZE and WRITE_SIZE passes recorded under profiles/ (scaled by a 256 MiB copy); null
+    # when that record is absent
+    for kern, fname in (("k_raycast", "k_raycast_traffic.json"), ("k_update", "k_update_traffic.json")):
+        tpath = os.path.join(ROOT, "profiles", fname)
+        roof32[kern]["traffic"] = None
+        if os.path.exists(tpath):
+            with open(tpath) as f:
+                tr = json.load(f)
+            roof32[kern]["traffic"] = tr["traffic_bytes_per_launch"]
+            roof32[kern]["traffic_source"] = f"profiles/{fname}"
 
     print("multi-resolution TSDF", file=sys.stderr, flush=True)
     rmr = run_multires(depth_dev, gt, n_frames, 5, torch, se, sensor)
     if not args.no_cpu_baseline:
         rmr["cpu_fps"] = cpu_baseline_multires(depth_dev[: 3 * MR_STEP + 1].cpu().numpy(), gt, sensor)
         rmr["cpu_threads"] = os.cpu_count() or 1
+
+    errors = None
+    if fp64:
+        print("fp64 oracle over the sequence", file=sys.stderr, flush=True)
+        runs = {"fp32": r32}
+        if r16 is not None:
+            runs["fp16_tsdf"] = r16
+        errors = fp64_errors(depth_dev.cpu().numpy(), gt, runs, sensor, mu=0.08)
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -538,35 +685,55 @@ def main():
 
     fp16 = None
     if r16 is not None:
-        b16, g16 = update_roofline(r16, 4)
-        fp16 = {"fps": r16["fps"], "ate_m": ate(r16["poses"], gt), "blocks": r16["blocks"],
-                "stage_ms": r16["stage_ms"], "update_gbs": g16,
+        r16["stage_ms"], r16["track_level_ms"] = r16t["stage_ms"], r16t["track_level_ms"]
+        r16["update_blocks_per_frame"], r16["candidates_per_frame"] = r16t["update_blocks_per_frame"], \
+            r16t["candidates_per_frame"]
+        roof16 = rooflines(r16, fixture, cam, warmup, steps, 4)
+        fp16 = {"fps": r16["fps"], "sequence_fps": r16["sequence_fps"], "ate_m": ate(r16["poses"], gt),
+                "ate_minus_fp32_m": ate(r16["poses"], gt) - ate32, "blocks": r16["blocks"],
+                "stage_ms": r16["stage_ms"], "track_level_us": [1000 * x for x in r16["track_level_ms"]],
+                "stage_delta_ms_vs_fp32": {k: r16["stage_ms"][k] - r32["stage_ms"][k] for k in r32["stage_ms"]},
+                "rooflines": {k: {"achieved": v["achieved"], "frac": v["frac"]} for k, v in roof16.items()},
                 "storage_only": {"fps": r16p["fps"], "ate_m": ate(r16p["poses"], gt),
                                  "note": "FP16 TSDF storage with the plain fp32 ICP reduction"}}
+        if errors is not None:
+            fp16["fp64"] = errors["fp16_tsdf"]
+    main_roof = dict(roof32["k_raycast"])
     out = {
         "metric": "frames/s integrate+raycast+ICP @640x480,1cm; FP16-TSDF vs FP32 at stated RMSE",
         "value": r32["fps"],
         "unit": "frames/s",
         "n_gpus": 1,
-        "steps": n_frames - args.warmup,
-        "warmup": args.warmup,
-        "ms_per_step": 1000.0 * r32["seconds"] / max(n_frames - args.warmup, 1),
+        "steps": steps,
+        "warmup": warmup,
+        "ms_per_step": 1000.0 * r32["seconds"] / max(steps, 1),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic: seeded room + 3 spheres, analytic depth with Kinect-like noise, 1% dropout",
-        "config": {"workload": "fp32 TSDF + ICP tracking, 300-frame spline sequence, 640x480, 1 cm voxels, "
-                               "10.24 m map, mu 8 voxels, max weight 100, ICP 10/5/4",
-                   "frames": n_frames, "ate_m": ate32, "blocks": r32["blocks"], "stage_ms": r32["stage_ms"],
+        "config": {"workload": f"fp32 TSDF + ICP tracking on frames {warmup}..{warmup + steps - 1} of the "
+                               f"{n_frames}-frame spline sequence (the rest runs untimed), 640x480, 1 cm voxels, "
+                               f"10.24 m map, mu 8 voxels, max weight 100, ICP 10/5/4",
+                   "frames": n_frames, "timed_frames": [warmup, warmup + steps - 1],
+                   "sequence_fps": r32["sequence_fps"], "ate_m": ate32, "blocks": r32["blocks"],
+                   "stage_ms": r32["stage_ms"], "track_level_us": [1000 * x for x in r32["track_level_ms"]],
+                   "stage_note": "stage and level times from HIP events in a second run of the same frames; the "
+                                 "timed run records no events",
                    "poses": "device-resident (track -> integrate -> raycast without host round trips)",
                    "host_pose_fps": r32h["fps"], "host_pose_ate_m": ate(r32h["poses"], gt),
                    "update_blocks_per_frame": r32["update_blocks_per_frame"],
-                   "candidates_per_frame": r32["candidates_per_frame"], "event_counts": r32["counts"]},
+                   "candidates_per_frame": r32["candidates_per_frame"]},
         "fp16_tsdf": fp16,
         "fp16_over_fp32": (r16["fps"] / r32["fps"]) if r16 is not None and r32["fps"] > 0 else None,
-        "roofline": {"bound": "hbm", "kernel": "k_update", "achieved": upd_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": upd_gbs / HBM_PEAK_GBS, "traffic": traffic, "bytes_per_launch": upd_bytes},
+        "fp64": ({"fp32": errors["fp32"], "ate64_m": errors["ate64_m"], "frames": errors["frames"],
+                  "seconds": errors["seconds"], "threads": errors["threads"],
+                  "note": "errors against the FP64 oracle's own run of the same frames (track, integrate, "
+                          "raycast): poses, every frame's vertex map (pixels both hit), final TSDF (voxels with the "
+                          "same weight in both maps, in metres); the FP16 path's errors are in fp16_tsdf.fp64"}
+                 if errors is not None else None),
+        "roofline": main_roof,
+        "rooflines": roof32,
         "cpu_baseline": cpu,
         "mesh": dict(r32["mesh"], blocks=r32["blocks"],
                      note="Map::mesh (single-res marching cubes) of the final FP32 map, after the timed loop"),

@@ -116,6 +116,11 @@ typedef struct {
     int64_t n_integrations, n_raycasts, n_tracks;
     int64_t sum_update_blocks; /* sum over integrations of (frustum + new) blocks updated */
     int64_t sum_candidates;
+    /* SE8_TIMING_LEVELS only: per pyramid level (finest first), the summed device time from the
+     * first Gauss-Newton launch of the level to the first launch of the next (the last level: to the
+     * end of its last launch), and the number of launches it covers */
+    double sum_track_level[SE8_TRACKER_MAX_LEVELS];
+    int64_t n_track_level_launches[SE8_TRACKER_MAX_LEVELS];
 } se8_stage_times;
 
 /* -------- map lifetime: se::Map ctor/dtor (include/se/map/impl/map_impl.hpp:37-55) -------- */
@@ -341,6 +346,22 @@ int se8_reader_next_raw(se8_reader* reader, uint16_t* depth, float* scale, int* 
 int se8_reader_next(se8_reader* reader, float* depth, float T_WB[16], int* status);
 int se8_reader_restart(se8_reader* reader);
 
+/* Stage timing of a map and/or a tracker (either may be NULL).  The reference's TICK/TOCK
+ * (perfstats_impl.hpp:352-388) are always on; here they are a mode:
+ *   SE8_TIMING_STAGES (the default): HIP events around each stage; a call waits on the host for the
+ *     previous call of its kind to finish (its events are read then), so the host runs at most about
+ *     one frame ahead.
+ *   SE8_TIMING_LEVELS: also events between the pyramid levels of each track call
+ *     (se8_stage_times.sum_track_level).
+ *   SE8_TIMING_OFF: no events and no host waits inside integrate / raycast / track: the *_dev calls
+ *     of a frame only enqueue kernels (they may be captured in a HIP graph once the map's and
+ *     tracker's buffers exist), and the host runs ahead freely.  A capacity overflow is then reported
+ *     by se8_map_get_info, and by the first integration after timing is switched back on.
+ * Switching waits for the stage events still pending. */
+#define SE8_TIMING_OFF 0
+#define SE8_TIMING_STAGES 1
+#define SE8_TIMING_LEVELS 2
+int se8_set_stage_timing(se8_map* map, se8_tracker* tracker, int mode);
 int se8_stage_times_get(const se8_map* map, const se8_tracker* tracker, se8_stage_times* out);
 int se8_stage_times_reset(se8_map* map, se8_tracker* tracker);
 

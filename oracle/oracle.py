@@ -125,6 +125,8 @@ def lib(precision: str = "f32") -> C.CDLL:
     L.or_field_slice.argtypes = [P, C.c_int, C.c_int, R]
     L.or_map_set_multires.argtypes = [P]
     L.or_map_set_storage_f16.argtypes = [P]
+    L.or_map_touch_begin.argtypes = [P]
+    L.or_map_touch_end.argtypes = [P, C.POINTER(C.c_long), C.POINTER(U64)]
     L.or_half_round.argtypes = [C.c_float]
     L.or_half_round.restype = C.c_float
     L.or_map_dump_multires.argtypes = [P, C.POINTER(U64), I, R, R]
@@ -162,7 +164,7 @@ def lib(precision: str = "f32") -> C.CDLL:
     L.or_depth_to_points.argtypes = [C.POINTER(C.c_float), C.POINTER(Pinhole), C.c_int, R]
     L.or_points_to_normals.argtypes = [R, C.c_int, C.c_int, C.c_int, R]
     L.or_track.restype = C.c_int
-    L.or_track.argtypes = [C.POINTER(C.c_float), C.POINTER(Pinhole), C.POINTER(TrackerConfig), R, R, R, R, I]
+    L.or_track.argtypes = [C.POINTER(C.c_float), C.POINTER(Pinhole), C.POINTER(TrackerConfig), R, R, R, R, I, I]
     L.or_icp_track_reduce.argtypes = [R, R, C.c_int, C.c_int, R, R, C.POINTER(Pinhole), R, R, C.c_float,
                                       C.c_float, I, R, R, R]
     L.or_icp_solve.argtypes = [R, R]
@@ -317,6 +319,20 @@ class OracleMap:
                                   _rp(T, self.precision), stats)
         return dict(frustum=stats[0], new=stats[1], candidates=stats[2], valid_px=stats[3])
 
+    def raycast_touched(self, pin, T_WS, bits=False):
+        """raycast() that also counts the distinct voxels whose data it reads and the distinct blocks
+        holding them: (points, normals, scale, voxels, blocks), and with bits the read mask (8 uint64
+        words per block in allocation order, bit i of a block = its voxel i)."""
+        n_blocks = self.num_blocks
+        self.L.or_map_touch_begin(self.h)
+        p, n, s = self.raycast(pin, T_WS)
+        out = (C.c_long * 2)()
+        mask = np.zeros(max(n_blocks, 1) * 8, np.uint64)
+        self.L.or_map_touch_end(self.h, out, _u64p(mask) if bits else None)
+        if bits:
+            return p, n, s, int(out[0]), int(out[1]), mask[: n_blocks * 8].reshape(n_blocks, 8)
+        return p, n, s, int(out[0]), int(out[1])
+
     def raycast(self, pin, T_WS):
         T = np.ascontiguousarray(np.asarray(T_WS).reshape(16), dtype=self.dtype)
         pts = np.zeros((pin.height, pin.width, 3), self.dtype)
@@ -458,7 +474,7 @@ class OracleMap:
         return out.reshape(4, 4)
 
 
-def track(depth, pin, T_WS, ref_pts, ref_nrm, cfg=None, precision="f32"):
+def track(depth, pin, T_WS, ref_pts, ref_nrm, cfg=None, precision="f32", with_level_iterations=False):
     L = lib(precision)
     dt = _dtype(precision)
     cfg = cfg or tracker_config()
@@ -468,8 +484,11 @@ def track(depth, pin, T_WS, ref_pts, ref_nrm, cfg=None, precision="f32"):
     rn = np.ascontiguousarray(ref_nrm, dtype=dt)
     sums = np.zeros(32, dt)
     iters = C.c_int()
+    per_level = (C.c_int * MAX_LEVELS)()
     ok = L.or_track(d.ctypes.data_as(C.POINTER(C.c_float)), C.byref(pin), C.byref(cfg), _rp(T, precision),
-                    _rp(rp, precision), _rp(rn, precision), _rp(sums, precision), C.byref(iters))
+                    _rp(rp, precision), _rp(rn, precision), _rp(sums, precision), C.byref(iters), per_level)
+    if with_level_iterations:
+        return bool(ok), T.reshape(4, 4), sums, int(iters.value), list(per_level)[: cfg.num_levels]
     return bool(ok), T.reshape(4, 4), sums, int(iters.value)
 
 

@@ -763,6 +763,7 @@ typedef struct or_octant {
      * then their past data; current / minimum scale (-1 = none) and the scale chosen this frame */
     real* mr;
     int cur_scale, min_scale, new_scale;
+    long slot; /* blocks: index in or_map.blocks */
 } or_octant;
 
 typedef struct {
@@ -778,7 +779,18 @@ typedef struct {
     int aabb_min[3], aabb_max[3], aabb_empty;
     int multires; /* TSDFMap<Res::Multi>: blocks carry the multi-resolution data */
     int storage_f16; /* tsdf stored as IEEE binary16: rounded to the nearest half after each update */
+    /* while set (or_map_touch_begin .. _end): one bit per voxel of every block, raised by each read
+     * of the voxel's data through getData, the interpolation corners and the gradient samples */
+    uint64_t* touch;
+    long touch_blocks;
 } or_map;
+
+/* records a read of voxel i of block b (no-op unless recording) */
+static inline void touch_voxel(const or_map* m, const or_octant* b, int i)
+{
+    if (m->touch && b->slot < m->touch_blocks)
+        __atomic_fetch_or(&m->touch[b->slot * 8 + (i >> 6)], 1ull << (i & 63), __ATOMIC_RELAXED);
+}
 
 static unsigned or_power_two_up(float x)
 {
@@ -852,6 +864,7 @@ OR_API void or_map_destroy(void* vm)
     or_map* m = (or_map*) vm;
     free_octant(m->root);
     free(m->blocks);
+    free(m->touch);
     free(m);
 }
 
@@ -931,6 +944,7 @@ static void push_block(or_map* m, or_octant* b)
         m->cap_blocks *= 2;
         m->blocks = (or_octant**) realloc(m->blocks, sizeof(or_octant*) * m->cap_blocks);
     }
+    b->slot = m->n_blocks;
     m->blocks[m->n_blocks++] = b;
 }
 
@@ -1673,6 +1687,7 @@ static inline int get_voxel(const or_map* m, const int v[3], real* f)
     if (!b)
         return 0;
     const int i = block_voxel_idx(b, v);
+    touch_voxel(m, b, i);
     if (b->weight[i] == (real) 0)
         return 0;
     *f = b->tsdf[i];
@@ -1702,6 +1717,7 @@ static int field_interp_vox(const or_map* m, const real vf[3], real* out)
         if (!b)
             return 0;
         const int i = block_voxel_idx(b, c);
+        touch_voxel(m, b, i);
         if (b->weight[i] == (real) 0)
             return 0;
         d[k] = b->tsdf[i];
@@ -1799,9 +1815,40 @@ OR_API int or_get_data(void* vm, const real pW[3], real* tsdf, real* weight)
     if (!b)
         return 0;
     const int i = block_voxel_idx(b, v);
+    touch_voxel(m, b, i);
     *tsdf = b->tsdf[i];
     *weight = b->weight[i];
     return 1;
+}
+
+/* Distinct voxel reads of the raycast (the algorithmic bytes of raycast_volume): between begin and
+ * end, every voxel whose data a query reads is marked; end returns the number of distinct voxels
+ * (out[0]) and of distinct blocks with at least one of them (out[1]). */
+OR_API void or_map_touch_begin(void* vm)
+{
+    or_map* m = (or_map*) vm;
+    free(m->touch);
+    m->touch_blocks = m->n_blocks;
+    m->touch = (uint64_t*) calloc((size_t) (m->n_blocks > 0 ? m->n_blocks : 1) * 8, sizeof(uint64_t));
+}
+
+OR_API void or_map_touch_end(void* vm, long out[2], uint64_t* bits)
+{
+    or_map* m = (or_map*) vm;
+    long nv = 0, nb = 0;
+    if (bits && m->touch)
+        memcpy(bits, m->touch, sizeof(uint64_t) * 8 * (size_t) m->touch_blocks);
+    for (long b = 0; m->touch && b < m->touch_blocks; b++) {
+        long c = 0;
+        for (int w = 0; w < 8; w++)
+            c += __builtin_popcountll(m->touch[b * 8 + w]);
+        nv += c;
+        nb += c > 0;
+    }
+    free(m->touch);
+    m->touch = NULL;
+    out[0] = nv;
+    out[1] = nb;
 }
 
 /* ------------------------------------------------------------------------- */
@@ -2365,7 +2412,8 @@ typedef struct {
 /* Tracker::track(depth, T_WS, ref points, ref normals) (tracker_impl.hpp:30-116).
  * T_WS16 is in/out.  sums_out receives the last reduction (32 values).  Returns 1 on success. */
 OR_API int or_track(const float* depth, const or_pinhole* pin, const or_tracker_config* cfg, real T_WS16[16],
-                    const real* ref_pts, const real* ref_nrm, real sums_out[32], int* iters_out)
+                    const real* ref_pts, const real* ref_nrm, real sums_out[32], int* iters_out,
+                    int* level_iters_out)
 {
     or_cam cam;
     cam_init(&cam, pin, 1);
@@ -2404,6 +2452,8 @@ OR_API int or_track(const float* depth, const or_pinhole* pin, const or_tracker_
                       cfg->dist_threshold, cfg->normal_threshold);
             icp_reduce(sums, rows, W, W / (1 << level), H / (1 << level));
             total_iters++;
+            if (level_iters_out)
+                level_iters_out[level]++;
             real x[6];
             icp_solve(sums + 1, x);
             real D16[16];

@@ -28,6 +28,9 @@ COPY_D2H = 1
 COPY_D2D = 2
 
 
+TRACKER_MAX_LEVELS = 6  # SE8_TRACKER_MAX_LEVELS
+
+
 class MapConfig(C.Structure):
     _fields_ = [("dim", C.c_float * 3), ("res", C.c_float), ("t_MW", C.c_float * 3)]
 
@@ -63,9 +66,6 @@ class LidarConfig(C.Structure):
         ("beam_elevation_deg", C.POINTER(C.c_float)),
         ("beam_azimuth_deg", C.POINTER(C.c_float)),
     ]
-
-
-TRACKER_MAX_LEVELS = 6  # SE8_TRACKER_MAX_LEVELS
 
 
 class TrackerConfig(C.Structure):
@@ -118,7 +118,12 @@ class StageTimes(C.Structure):
         ("n_tracks", C.c_int64),
         ("sum_update_blocks", C.c_int64),
         ("sum_candidates", C.c_int64),
+        ("sum_track_level", C.c_double * TRACKER_MAX_LEVELS),
+        ("n_track_level_launches", C.c_int64 * TRACKER_MAX_LEVELS),
     ]
+
+
+TIMING_OFF, TIMING_STAGES, TIMING_LEVELS = 0, 1, 2
 
 
 class ReaderConfig(C.Structure):
@@ -184,6 +189,7 @@ SIGNATURES = {
     "se8_reader_next_raw": (C.c_int, [_P, C.POINTER(C.c_uint16), _F, _I, _F, _I]),
     "se8_reader_next": (C.c_int, [_P, _F, _F, _I]),
     "se8_reader_restart": (C.c_int, [_P]),
+    "se8_set_stage_timing": (C.c_int, [_P, _P, C.c_int]),
     "se8_stage_times_get": (C.c_int, [_P, _P, C.POINTER(StageTimes)]),
     "se8_stage_times_reset": (C.c_int, [_P, _P]),
     "se8_device_alloc": (C.c_int, [C.c_size_t, C.POINTER(_P)]),

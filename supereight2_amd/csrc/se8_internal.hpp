@@ -772,6 +772,7 @@ struct se8_map {
     bool ray_pending = false;      // events ev[5..6] of the last raycast not yet harvested
     se8_stage_times times{};
     hipEvent_t ev[8]{};
+    int timing = SE8_TIMING_STAGES; // se8_set_stage_timing
     // marching cubes: per-block face counts, scanned in place into offsets (+ the total); grown on
     // demand by the (logically const) se8_mesh
     mutable se8::DeviceBuffer mesh_offsets;
@@ -802,6 +803,11 @@ struct se8_tracker {
     bool track_pending = false;
     se8_stage_times times{};
     hipEvent_t ev[2]{};
+    int timing = SE8_TIMING_STAGES; // se8_set_stage_timing
+    // SE8_TIMING_LEVELS: an event before each level's first launch (index = level) and one after the
+    // last launch of the call (index SE8_TRACKER_MAX_LEVELS); created on first use
+    hipEvent_t ev_lvl[SE8_TRACKER_MAX_LEVELS + 1]{};
+    bool lvl_pending = false;
 };
 
 struct se8_lidar {

@@ -1138,17 +1138,23 @@ static int integrate_impl(se8_map* m, int timestamp, const float* depth_dev, con
         set_error(std::string(what) + ": map capacity exhausted (raise se8_tsdf_config.max_blocks)");
         return SE8_ERR_CAPACITY;
     }
+    // with timing off no event is recorded and nothing here waits for the device
+    const bool timed = m->timing != SE8_TIMING_OFF;
+    hipEvent_t ev_upd0 = timed ? m->ev[3] : nullptr, ev_upd1 = timed ? m->ev[4] : nullptr;
     FramePoses* fp = (FramePoses*) m->frame_poses.ptr;
-    SE8_HIP_CHECK(hipEventRecord(m->ev[0], s));
+    if (timed)
+        SE8_HIP_CHECK(hipEventRecord(m->ev[0], s));
     k_frame_begin<<<1, 64, 0, s>>>(m->view, T_arg, T16_dev, fp);
     const int cus = device_count_sm();
     k_frustum<Sensor><<<cus * 4, 256, 0, s>>>(m->view, fp, sensor, (int*) m->update_list.ptr);
-    SE8_HIP_CHECK(hipEventRecord(m->ev[1], s));
+    if (timed)
+        SE8_HIP_CHECK(hipEventRecord(m->ev[1], s));
     dim3 grid((sensor.width() + 63) / 64, sensor.height());
     k_alloc_rays<Sensor><<<grid, 64, 0, s>>>(m->view, sensor, fp, depth_dev, band, num_steps,
                                              (uint64_t*) m->candidates.ptr, (int64_t) cand_cap);
     SE8_HIP_CHECK(hipGetLastError());
-    SE8_HIP_CHECK(hipEventRecord(m->ev[2], s));
+    if (timed)
+        SE8_HIP_CHECK(hipEventRecord(m->ev[2], s));
     rc = build_from_candidates(m, (uint64_t*) m->candidates.ptr, m->view.counters + kCntCandidates, (int) cand_cap,
                                (int*) m->update_list.ptr, 1, s);
     if (rc != SE8_OK)
@@ -1157,11 +1163,13 @@ static int integrate_impl(se8_map* m, int timestamp, const float* depth_dev, con
     const int* ulist = (const int*) m->update_list.ptr;
     if (is_multires(m)) {
         if constexpr (std::is_same<Sensor, PinholeSensor>::value) {
-            SE8_HIP_CHECK(hipEventRecord(m->ev[3], s));
+            if (timed)
+                SE8_HIP_CHECK(hipEventRecord(m->ev[3], s));
             rc = launch_multires_update(m, fp, sensor, depth_dev, timestamp, s);
             if (rc != SE8_OK)
                 return rc;
-            SE8_HIP_CHECK(hipEventRecord(m->ev[4], s));
+            if (timed)
+                SE8_HIP_CHECK(hipEventRecord(m->ev[4], s));
         }
         else {
             set_error(std::string(what) + ": the multi-resolution updater takes a pinhole camera");
@@ -1172,15 +1180,17 @@ static int integrate_impl(se8_map* m, int timestamp, const float* depth_dev, con
         // one resident wave of workgroups: the grid-stride loop then never leaves a second, partial
         // round of workgroups waiting for the first to finish
         hipExtLaunchKernelGGL(k_update<SE8_STORAGE_F32, Sensor>,
-                              dim3(resident_grid(k_update<SE8_STORAGE_F32, Sensor>, 512)), dim3(512), 0, s, m->ev[3],
-                              m->ev[4], 0, m->view, ulist, (const FramePoses*) fp, sensor, depth_dev, timestamp);
+                              dim3(resident_grid(k_update<SE8_STORAGE_F32, Sensor>, 512)), dim3(512), 0, s, ev_upd0,
+                              ev_upd1, 0, m->view, ulist, (const FramePoses*) fp, sensor, depth_dev, timestamp);
     }
     else {
         hipExtLaunchKernelGGL(k_update<SE8_STORAGE_F16, Sensor>,
-                              dim3(resident_grid(k_update<SE8_STORAGE_F16, Sensor>, 512)), dim3(512), 0, s, m->ev[3],
-                              m->ev[4], 0, m->view, ulist, (const FramePoses*) fp, sensor, depth_dev, timestamp);
+                              dim3(resident_grid(k_update<SE8_STORAGE_F16, Sensor>, 512)), dim3(512), 0, s, ev_upd0,
+                              ev_upd1, 0, m->view, ulist, (const FramePoses*) fp, sensor, depth_dev, timestamp);
     }
     SE8_HIP_CHECK(hipGetLastError());
+    if (!timed)
+        return SE8_OK;
     // the counters reach the pinned host copy from a one-wave kernel (a copy engine transfer costs
     // more than the kernel at 80 bytes); the host reads them after event 7
     k_publish_counters<<<1, 64, 0, s>>>(m->view.counters, m->host_counters_dev);
@@ -1688,8 +1698,49 @@ void harvest_tracking(se8_tracker* t)
     t->times.sum_tracking += t->times.tracking;
     t->times.n_tracks += 1;
     t->track_pending = false;
+    if (t->lvl_pending) {
+        // levels run coarsest first: level l lasts until the first launch of level l - 1
+        for (int l = t->cfg.num_levels - 1; l >= 0; l--) {
+            hipEvent_t end = l > 0 ? t->ev_lvl[l - 1] : t->ev_lvl[SE8_TRACKER_MAX_LEVELS];
+            t->times.sum_track_level[l] += elapsed(t->ev_lvl[l], end);
+            t->times.n_track_level_launches[l] += t->cfg.iterations[l];
+        }
+        t->lvl_pending = false;
+    }
 }
 } // namespace se8
+
+extern "C" int se8_stage_times_get(const se8_map* m, const se8_tracker* tr, se8_stage_times* out);
+
+extern "C" int se8_set_stage_timing(se8_map* m, se8_tracker* tr, int mode)
+{
+    SE8_REQUIRE(mode == SE8_TIMING_OFF || mode == SE8_TIMING_STAGES || mode == SE8_TIMING_LEVELS,
+                "se8_set_stage_timing: unknown mode");
+    se8_stage_times tmp;
+    int rc = se8_stage_times_get(m, tr, &tmp); // harvest what is pending under the old mode
+    if (rc != SE8_OK)
+        return rc;
+    if (m) {
+        if (m->timing == SE8_TIMING_OFF && mode != SE8_TIMING_OFF) {
+            // an overflow during untimed integrations is reported from now on
+            int host[kNumCounters];
+            rc = read_counters(m, host, nullptr);
+            if (rc != SE8_OK)
+                return rc;
+            if (host[kCntOverflow])
+                m->overflowed = true;
+        }
+        m->timing = mode;
+    }
+    if (tr) {
+        if (mode == SE8_TIMING_LEVELS)
+            for (int i = 0; i <= SE8_TRACKER_MAX_LEVELS; i++)
+                if (!tr->ev_lvl[i])
+                    SE8_HIP_CHECK(hipEventCreate(&tr->ev_lvl[i]));
+        tr->timing = mode;
+    }
+    return SE8_OK;
+}
 
 extern "C" int se8_stage_times_get(const se8_map* m, const se8_tracker* tr, se8_stage_times* out)
 {
@@ -1711,6 +1762,10 @@ extern "C" int se8_stage_times_get(const se8_map* m, const se8_tracker* tr, se8_
         out->tracking = tt->times.tracking;
         out->sum_tracking = tt->times.sum_tracking;
         out->n_tracks = tt->times.n_tracks;
+        for (int l = 0; l < SE8_TRACKER_MAX_LEVELS; l++) {
+            out->sum_track_level[l] = tt->times.sum_track_level[l];
+            out->n_track_level_launches[l] = tt->times.n_track_level_launches[l];
+        }
     }
     return SE8_OK;
 }

@@ -774,10 +774,11 @@ extern "C" int se8_raycast(const se8_map* m, const se8_pinhole* sensor, const fl
     hipStream_t s = (hipStream_t) stream;
     se8_map* mm = const_cast<se8_map*>(m);
     const Cam cam = make_cam(*sensor, 1, false);
+    const bool timed = mm->timing != SE8_TIMING_OFF;
     harvest_raycast(mm, true);
-    int rc = launch_raycast(m, cam, iso_from16(T_WS16), nullptr, points_dev, normals_dev, scale_dev, s, mm->ev[5],
-                            mm->ev[6]);
-    mm->ray_pending = true;
+    int rc = launch_raycast(m, cam, iso_from16(T_WS16), nullptr, points_dev, normals_dev, scale_dev, s,
+                            timed ? mm->ev[5] : nullptr, timed ? mm->ev[6] : nullptr);
+    mm->ray_pending = timed;
     return rc;
 }
 
@@ -791,9 +792,11 @@ extern "C" int se8_raycast_dev(const se8_map* m, const se8_pinhole* sensor, cons
     hipStream_t s = (hipStream_t) stream;
     se8_map* mm = const_cast<se8_map*>(m);
     const Cam cam = make_cam(*sensor, 1, false);
+    const bool timed = mm->timing != SE8_TIMING_OFF;
     harvest_raycast(mm, true);
-    int rc = launch_raycast(m, cam, Iso{}, T_WS_dev, points_dev, normals_dev, scale_dev, s, mm->ev[5], mm->ev[6]);
-    mm->ray_pending = true;
+    int rc = launch_raycast(m, cam, Iso{}, T_WS_dev, points_dev, normals_dev, scale_dev, s,
+                            timed ? mm->ev[5] : nullptr, timed ? mm->ev[6] : nullptr);
+    mm->ray_pending = timed;
     return rc;
 }
 

@@ -984,9 +984,12 @@ static int enqueue_track(se8_tracker* tr, const float* depth, const float* T16_h
 {
     const se8_pinhole& P = tr->sensor;
     IcpShared* sh = (IcpShared*) tr->state.ptr;
-    // the previous call's stage events, before ev[0] / ev[1] are recorded again
+    // the previous call's stage events, before ev[0] / ev[1] are recorded again (with timing off,
+    // nothing is recorded and nothing waits)
+    const bool timed = tr->timing != SE8_TIMING_OFF, levels = tr->timing == SE8_TIMING_LEVELS;
     harvest_tracking(tr);
-    SE8_HIP_CHECK(hipEventRecord(tr->ev[0], s));
+    if (timed)
+        SE8_HIP_CHECK(hipEventRecord(tr->ev[0], s));
     launch_pyramid(tr, depth, tr->cfg.num_levels, sh, T16_host ? iso_from16(T16_host) : Iso{}, T_in_dev, s);
     IcpArgs a{};
     a.ref_pts = ref_pts;
@@ -1008,6 +1011,8 @@ static int enqueue_track(se8_tracker* tr, const float* depth, const float* T16_h
         a.level = level;
         a.n_tiles = tr->icp_tiles[level];
         const int nwg = tr->num_wg[level];
+        if (levels)
+            SE8_HIP_CHECK(hipEventRecord(tr->ev_lvl[level], s));
         for (int it = 0; it < tr->cfg.iterations[level]; it++, a.k++) {
             if (tr->cfg.compensated)
                 launch_icp_iter<true>(tr->icp_ppt[level], nwg, a, s);
@@ -1016,6 +1021,8 @@ static int enqueue_track(se8_tracker* tr, const float* depth, const float* T16_h
             a.prev_rows = nwg;
         }
     }
+    if (levels)
+        SE8_HIP_CHECK(hipEventRecord(tr->ev_lvl[kMaxLevels], s));
     a.T_out_dev = T_out_dev;
     a.ok_dev = ok_dev;
     if (tr->cfg.compensated)
@@ -1023,8 +1030,11 @@ static int enqueue_track(se8_tracker* tr, const float* depth, const float* T16_h
     else
         k_icp_finish<false><<<1, kIcpThreads, 0, s>>>(a);
     SE8_HIP_CHECK(hipGetLastError());
-    SE8_HIP_CHECK(hipEventRecord(tr->ev[1], s));
-    tr->track_pending = true;
+    if (timed) {
+        SE8_HIP_CHECK(hipEventRecord(tr->ev[1], s));
+        tr->track_pending = true;
+        tr->lvl_pending = levels;
+    }
     return SE8_OK;
 }
 
@@ -1233,6 +1243,9 @@ extern "C" int se8_tracker_destroy(se8_tracker* t)
     for (int i = 0; i < 2; i++)
         if (t->ev[i])
             (void) hipEventDestroy(t->ev[i]);
+    for (int i = 0; i <= kMaxLevels; i++)
+        if (t->ev_lvl[i])
+            (void) hipEventDestroy(t->ev_lvl[i]);
     delete t;
     return SE8_OK;
 }

@@ -554,7 +554,19 @@ def stage_times(map_: TSDFMap | None = None, tracker: Tracker | None = None) -> 
     lib = capi.load()
     capi.check(lib.se8_stage_times_get(map_.h if map_ else None, tracker.h if tracker else None, C.byref(out)),
                "se8_stage_times_get")
-    return {k: getattr(out, k) for k, _ in capi.StageTimes._fields_}
+    res = {}
+    for k, _ in capi.StageTimes._fields_:
+        v = getattr(out, k)
+        res[k] = list(v) if isinstance(v, C.Array) else v
+    return res
+
+
+def set_stage_timing(map_: TSDFMap | None = None, tracker: Tracker | None = None, mode: str = "stages") -> None:
+    """mode "off" (no events, no host waits: the host runs ahead), "stages" (default) or "levels"
+    (stages plus per-pyramid-level ICP time)."""
+    m = {"off": capi.TIMING_OFF, "stages": capi.TIMING_STAGES, "levels": capi.TIMING_LEVELS}[mode]
+    capi.check(capi.load().se8_set_stage_timing(map_.h if map_ else None, tracker.h if tracker else None, m),
+               "se8_set_stage_timing")
 
 
 def stage_times_reset(map_: TSDFMap | None = None, tracker: Tracker | None = None) -> None:

@@ -292,3 +292,69 @@ def test_nan_depth_sample_propagates_as_in_reference():
         v = 525.0 * p[:, 1] / p[:, 2] + 239.5
         ui, vi = (u[bn] + 0.5).astype(int), (v[bn] + 0.5).astype(int)
         assert ((ui >= 280) & (ui < 360) & (vi >= 200) & (vi < 280)).all()
+
+
+def _popcount_rows(bits):
+    return np.array([sum(bin(int(w)).count("1") for w in row) for row in bits])
+
+
+def test_raycast_touched_set_is_union_of_its_rays():
+    """The raycast's read set (the voxels behind the raycast roofline's algorithmic bytes) over a
+    16 x 12 window of the image equals the union of the read sets of its 192 rays cast one by one
+    (a 1 x 1 camera whose principal point puts its pixel on the same ray), and the counts are the
+    popcount of that set and the number of blocks it touches."""
+    from supereight2_amd import synth
+
+    cam = synth.Camera()
+    scene = synth.make_scene(0)
+    poses = synth.circle_trajectory(50)[:2]
+    rng = np.random.default_rng(1000)
+    m = O.OracleMap(10.24, 0.01)
+    pin = O.pinhole()
+    for k, T in enumerate(poses):
+        m.integrate(k, synth.render_depth(scene, cam, T, rng), pin, T.astype(np.float32))
+    T = poses[1].astype(np.float32)
+    x0, y0, w, h = 300, 200, 16, 12
+    win = O.pinhole(w, h, cam.fx, cam.fy, cam.cx - x0, cam.cy - y0, cam.near_plane, cam.far_plane)
+    _, _, _, V, B, bits = m.raycast_touched(win, T, bits=True)
+    union = np.zeros_like(bits)
+    for y in range(h):
+        for x in range(w):
+            one = O.pinhole(1, 1, cam.fx, cam.fy, cam.cx - (x0 + x), cam.cy - (y0 + y), cam.near_plane, cam.far_plane)
+            _, _, _, v1, b1, bits1 = m.raycast_touched(one, T, bits=True)
+            assert 0 < v1 <= 512 * b1
+            union |= bits1
+    np.testing.assert_array_equal(bits, union)
+    per_block = _popcount_rows(bits)
+    assert V == per_block.sum() and B == int((per_block > 0).sum())
+    assert V > 192  # every ray reads more than its hit voxel
+
+
+@pytest.mark.parametrize("tsdf", [1.0, 0.25])
+def test_raycast_touched_one_block_kat(tsdf):
+    """One observed block of constant tsdf on an axis-aligned ray: the march samples it every
+    max(tsdf * mu, res) = 8 voxels (tsdf 1) or 2 voxels (tsdf 0.25, above the interpolation band),
+    so it reads 1 or 4 voxels of the block, starting at the voxel of the block entry sample (when
+    that sample rounds into the block; otherwise it takes the unobserved large step of 8 voxels and
+    lands on the block's last voxel column)."""
+    m = O.OracleMap(32.0, 1.0)  # 32^3 voxels of 1 m, T_MW = 16 m
+    assert m.size == 32
+    m.allocate_blocks([[8, 8, 8]])
+    assert m.set_block_data([8, 8, 8], np.full(512, tsdf), np.ones(512)) == 0
+    # camera at world (-20, -4.3, -4.7) looking along +x: camera z -> world x
+    T = np.array([[0, 0, 1, -20.0], [1, 0, 0, -4.3], [0, 1, 0, -4.7], [0, 0, 0, 1]], np.float32)
+    one = O.pinhole(1, 1, 1.0, 1.0, 0.0, 0.0, 0.4, 60.0)
+    _, _, _, V, B, bits = m.raycast_touched(one, T, bits=True)
+    t_min, _ = m.first_block(T[:3, 3], np.array([1, 0, 0], np.float32), 0.4, 60.0)
+    assert abs(t_min - 12.0) < 1e-4  # the block's x face at world -8
+    x_entry = np.float32(np.float32(-20.0) + np.float32(t_min)) + np.float32(16.0)
+    in_block = x_entry >= 8.0
+    step = 8 if tsdf == 1.0 else 2
+    xs = list(range(8, 16, step)) if in_block else [15]
+    want = np.zeros(512, bool)
+    for x in xs:
+        want[(x - 8) + 8 * (11 - 8) + 64 * (11 - 8)] = True
+    got = np.array([(int(bits[0][i >> 6]) >> (i & 63)) & 1 for i in range(512)], bool)
+    print("entry x", x_entry, "voxels read", np.nonzero(got)[0])
+    assert B == 1 and V == len(xs)
+    np.testing.assert_array_equal(got, want)
