@@ -487,75 +487,107 @@ def cpu_model():
 FIXTURE = os.path.join(ROOT, "tests", "golden", "raycast_touched_spline300_seed10.json")
 
 
-def fp64_errors(depth_host, gt, runs, sensor, mu, log_every=50):
-    """The FP64 oracle (the C restatement in double precision) runs the reference loop on the same
-    frames (track from the previous estimate, integrate, raycast); each GPU run is compared with it:
-    tracked poses, every frame's raycast vertex map, and the final TSDF (per voxel observed in both
-    maps, split into voxels with the same weight, i.e. the same number of updates, and the count
-    whose weights differ)."""
+def _tsdf_compare(dump, dump64, mu):
+    """Final maps: voxels observed in both, split into those with the same weight (the same number of
+    updates, where the difference is arithmetic and storage precision) and those whose weights
+    differ (a band-edge decision went the other way)."""
+    keys, _, tg, wg = dump
+    keys64, _, t64, w64 = dump64
+    common, ig, io = np.intersect1d(keys, keys64, return_indices=True)
+    tg_c, wg_c, to_c, wo_c = tg[ig], wg[ig], t64[io], w64[io]
+    obs = (wg_c > 0) & (wo_c > 0)
+    same = obs & (wg_c == wo_c)
+    d_same = (tg_c[same].astype(np.float64) - to_c[same]) * mu
+    d_all = (tg_c[obs].astype(np.float64) - to_c[obs]) * mu
+    return {"tsdf_rms_m": float(np.sqrt(np.mean(d_same ** 2))) if d_same.size else None,
+            "tsdf_max_m": float(np.abs(d_same).max()) if d_same.size else None,
+            "tsdf_voxels": int(same.sum()),
+            "tsdf_voxels_weight_differs": int((obs & (wg_c != wo_c)).sum()),
+            "tsdf_rms_all_observed_m": float(np.sqrt(np.mean(d_all ** 2))) if d_all.size else None,
+            "blocks_only_gpu": int(len(keys) - len(common)), "blocks_only_fp64": int(len(keys64) - len(common))}
+
+
+class _VertexErr:
+    def __init__(self):
+        self.vmax, self.sq, self.cnt, self.p999, self.mismatch, self.frames = 0.0, 0.0, 0, [], 0, 0
+
+    def add(self, g, p64):
+        hg, h64 = np.any(g != 0, axis=-1), np.any(p64 != 0, axis=-1)
+        both = hg & h64
+        e = np.linalg.norm(g[both].astype(np.float64) - p64[both], axis=-1)
+        if e.size:
+            self.vmax = max(self.vmax, float(e.max()))
+            self.p999.append(float(np.quantile(e, 0.999)))
+            self.sq += float(np.sum(e * e))
+            self.cnt += int(e.size)
+        self.mismatch += int(np.sum(hg != h64))
+        self.frames += 1
+
+    def out(self):
+        return {"vertex_max_m": self.vmax, "vertex_rms_m": float(np.sqrt(self.sq / max(self.cnt, 1))),
+                "vertex_p999_max_m": float(max(self.p999)) if self.p999 else None,
+                "hit_mismatch_px_per_frame": self.mismatch / max(self.frames, 1), "frames_compared": self.frames}
+
+
+def fp64_errors(depth_host, gt, runs, sensor, mu, log_every=50, raycast_every=10):
+    """Errors of each GPU run against the FP64 oracle (the C restatement in double precision) on the
+    same frames, two ways:
+    * same_poses: the FP64 oracle integrates every frame at the pose the GPU run used (its own
+      tracked pose) and raycasts at it every `raycast_every` frames: the TSDF of the final maps and
+      the vertex maps differ only by arithmetic and storage precision;
+    * own_tracking: the FP64 oracle runs the whole reference loop itself (track from the previous
+      estimate, integrate, raycast): the poses of the two runs, every frame's vertex map and the
+      final TSDF, which then also carry the two trajectories' difference."""
     from oracle import oracle as O
 
     O.set_threads(os.cpu_count() or 1)
     pin = O.pinhole(sensor.width, sensor.height, sensor.fx, sensor.fy, sensor.cx, sensor.cy, sensor.near_plane,
                     sensor.far_plane)
-    m = O.OracleMap(10.24, 0.01, precision="f64")
     n = len(depth_host)
+    t0 = time.perf_counter()
+    out = {"frames": n, "threads": os.cpu_count() or 1}
+    res = {name: {} for name in runs}
+    for name, r in runs.items():
+        m = O.OracleMap(10.24, 0.01, precision="f64")
+        ve = _VertexErr()
+        for k in range(n):
+            T = np.asarray(r["poses"][k], np.float32).astype(np.float64)
+            m.integrate(k, depth_host[k], pin, T)
+            if k % raycast_every == raycast_every - 1 or k == n - 1:
+                p64, _, _ = m.raycast(pin, T)
+                ve.add(r["points"][k].cpu().numpy(), p64)
+            if log_every and k % log_every == 0:
+                print(f"fp64 oracle at the {name} poses, frame {k} ({time.perf_counter() - t0:.0f} s)",
+                      file=sys.stderr, flush=True)
+        res[name]["same_poses"] = dict(_tsdf_compare(r["blocks_dump"], m.dump_blocks(), mu), **ve.out())
+        del m
+    m = O.OracleMap(10.24, 0.01, precision="f64")
     T = gt[0].astype(np.float64)
     p64 = n64 = None
     poses64 = []
-    acc = {name: {"vmax": 0.0, "sq": 0.0, "cnt": 0, "p999": [], "hit_mismatch": 0} for name in runs}
-    t0 = time.perf_counter()
+    ves = {name: _VertexErr() for name in runs}
     for k in range(n):
         if k > 0:
             _, T, _, _ = O.track(depth_host[k], pin, T, p64, n64, precision="f64")
         m.integrate(k, depth_host[k], pin, T)
         p64, n64, _ = m.raycast(pin, T)
         poses64.append(T.copy())
-        h64 = np.any(p64 != 0, axis=-1)
         for name, r in runs.items():
-            g = r["points"][k].cpu().numpy()
-            hg = np.any(g != 0, axis=-1)
-            both = hg & h64
-            e = np.linalg.norm(g[both].astype(np.float64) - p64[both], axis=-1)
-            a = acc[name]
-            if e.size:
-                a["vmax"] = max(a["vmax"], float(e.max()))
-                a["p999"].append(float(np.quantile(e, 0.999)))
-                a["sq"] += float(np.sum(e * e))
-                a["cnt"] += int(e.size)
-            a["hit_mismatch"] += int(np.sum(hg != h64))
+            ves[name].add(r["points"][k].cpu().numpy(), p64)
         if log_every and k % log_every == 0:
-            print(f"fp64 oracle frame {k} ({time.perf_counter() - t0:.0f} s)", file=sys.stderr, flush=True)
-    keys64, _, t64, w64 = m.dump_blocks()
-    out = {"frames": n, "seconds": time.perf_counter() - t0, "threads": os.cpu_count() or 1}
-    ate64 = ate(poses64, gt)
-    out["ate64_m"] = ate64
+            print(f"fp64 oracle tracking, frame {k} ({time.perf_counter() - t0:.0f} s)", file=sys.stderr, flush=True)
+    dump64 = m.dump_blocks()
+    out["ate64_m"] = ate(poses64, gt)
     for name, r in runs.items():
         pg = r["poses"]
         dt = [float(np.linalg.norm(np.asarray(a)[:3, 3] - b[:3, 3])) for a, b in zip(pg, poses64)]
         dr = [float(np.degrees(np.arccos(np.clip((np.trace(np.asarray(a)[:3, :3].T @ b[:3, :3]) - 1) / 2, -1, 1))))
               for a, b in zip(pg, poses64)]
-        keys, _, tg, wg = r["blocks_dump"]
-        common, ig, io = np.intersect1d(keys, keys64, return_indices=True)
-        tg_c, wg_c, to_c, wo_c = tg[ig], wg[ig], t64[io], w64[io]
-        obs = (wg_c > 0) & (wo_c > 0)
-        same = obs & (wg_c == wo_c)
-        d_same = (tg_c[same].astype(np.float64) - to_c[same]) * mu
-        d_all = (tg_c[obs].astype(np.float64) - to_c[obs]) * mu
-        a = acc[name]
-        out[name] = {
-            "tsdf_rms_m": float(np.sqrt(np.mean(d_same ** 2))) if d_same.size else None,
-            "tsdf_max_m": float(np.abs(d_same).max()) if d_same.size else None,
-            "tsdf_voxels": int(same.sum()),
-            "tsdf_voxels_weight_differs": int((obs & (wg_c != wo_c)).sum()),
-            "tsdf_rms_all_observed_m": float(np.sqrt(np.mean(d_all ** 2))) if d_all.size else None,
-            "blocks_only_gpu": int(len(keys) - len(common)), "blocks_only_fp64": int(len(keys64) - len(common)),
-            "vertex_max_m": a["vmax"],
-            "vertex_rms_m": float(np.sqrt(a["sq"] / max(a["cnt"], 1))),
-            "vertex_p999_max_m": float(max(a["p999"])) if a["p999"] else None,
-            "hit_mismatch_px_per_frame": a["hit_mismatch"] / n,
-            "pose_max_m": float(max(dt)), "pose_max_deg": float(max(dr)), "pose_final_m": dt[-1],
-        }
+        res[name]["own_tracking"] = dict(_tsdf_compare(r["blocks_dump"], dump64, mu), **ves[name].out(),
+                                         pose_max_m=float(max(dt)), pose_max_deg=float(max(dr)),
+                                         pose_final_m=dt[-1], ate_minus_fp64_m=ate(pg, gt) - out["ate64_m"])
+        out[name] = res[name]
+    out["seconds"] = time.perf_counter() - t0
     return out
 
 
@@ -728,9 +760,12 @@ def main():
         "fp16_over_fp32": (r16["fps"] / r32["fps"]) if r16 is not None and r32["fps"] > 0 else None,
         "fp64": ({"fp32": errors["fp32"], "ate64_m": errors["ate64_m"], "frames": errors["frames"],
                   "seconds": errors["seconds"], "threads": errors["threads"],
-                  "note": "errors against the FP64 oracle's own run of the same frames (track, integrate, "
-                          "raycast): poses, every frame's vertex map (pixels both hit), final TSDF (voxels with the "
-                          "same weight in both maps, in metres); the FP16 path's errors are in fp16_tsdf.fp64"}
+                  "note": "errors against the FP64 oracle on the same frames. same_poses: FP64 integration at "
+                          "the GPU run's own poses (final TSDF over voxels with the same weight in both maps, in "
+                          "metres; vertex maps of every 10th frame at those poses, pixels both hit): arithmetic "
+                          "and storage precision only. own_tracking: the FP64 oracle's own tracked run (poses, "
+                          "every frame's vertex map, final TSDF), which adds the two trajectories' difference. "
+                          "The FP16 path's errors are in fp16_tsdf.fp64"}
                  if errors is not None else None),
         "roofline": main_roof,
         "rooflines": roof32,

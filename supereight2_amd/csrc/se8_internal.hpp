@@ -503,6 +503,9 @@ struct MapView {
     int* brick_slots;   // per brick 512 block slots (-1 = none), index (bx&m) + 8(by&m) + 64(bz&m)
     int* block_dense;   // maps of at most kDenseMaxSize voxels a side: the slot of every block
                         // (-1 = none), index (bz * S + by) * S + bx, S = size / 8; null otherwise
+    int* node_dense;    // with block_dense: per scale s in [4, max_scale), the node index of every
+                        // octant of that scale (-1 = none), at node_dense_off[s] + dense_index(s)
+    int64_t node_dense_off[20];
     int* node_child;    // per node 8 children: node index, or block slot below scale-4 nodes
     uint64_t* node_key; // per node key
     int* node_ts;       // per node timestamp
@@ -555,7 +558,8 @@ enum Counter {
     kCntMrNew = 14,  // multi-resolution update: blocks allocated this frame
     kCntMrDown = 15, // multi-resolution update: blocks moving to a finer scale
     kCntMrScale = 16, // 4 counters: multi-resolution update list entries at scale 0..3
-    kNumCounters = 20
+    kCntNodesStart = 20, // node count at the start of the integration in flight (its nodes are new)
+    kNumCounters = 21
 };
 
 // Wave-aggregated append: every lane with pred gets the next free position of *counter (one
@@ -574,6 +578,13 @@ __device__ __forceinline__ int wave_append_slot(bool pred, int* counter)
     if (!pred)
         return -1;
     return base + __popcll(mask & ((1ull << lane) - 1ull));
+}
+
+// Index of the octant of scale s containing voxel c in a dense table of (size >> s)^3 entries.
+__device__ __forceinline__ int64_t dense_index(const MapView& m, const int c[3], int s)
+{
+    const int64_t side = m.size >> s;
+    return ((int64_t) (c[2] >> s) * side + (c[1] >> s)) * side + (c[0] >> s);
 }
 
 // Block slot of the block containing voxel (x,y,z); -1 if unallocated.  Coordinates are taken
@@ -756,7 +767,7 @@ struct se8_map {
     se8_map_config cfg;
     se8_tsdf_config tsdf;
     se8::MapView view;
-    se8::DeviceBuffer block_dense;
+    se8::DeviceBuffer block_dense, node_dense;
     se8::DeviceBuffer brick_top, brick_slots, node_child, node_key, node_ts, block_key, block_parent, block_ts,
         voxels, counters, block_scale;
     // per-frame scratch
@@ -792,7 +803,8 @@ struct se8_tracker {
     se8_pinhole sensor;
     se8_tracker_config cfg;
     se8::DeviceBuffer vert_lvl[SE8_TRACKER_MAX_LEVELS], norm_lvl[SE8_TRACKER_MAX_LEVELS]; // float4 per pixel, per level
-    se8::DeviceBuffer partials; // per-workgroup reduction partials
+    se8::DeviceBuffer partials; // per-workgroup reduction partials (plain path)
+    se8::DeviceBuffer bins;     // fixed-point partial sums (compensated path), cleared by each track call
     se8::DeviceBuffer state;    // counters, flags, final sums
     se8::DeviceBuffer ref_points, ref_normals; // used by se8_track_raycast
     se8::DeviceBuffer adj_poses, adj_partials, adj_out; // batched cost/gradient evaluation

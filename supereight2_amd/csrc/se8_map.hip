@@ -155,6 +155,7 @@ __global__ void k_frame_begin(MapView m, Iso T_arg, const float* T16_dev, FrameP
         m.counters[kCntUnique] = 0;
         m.counters[kCntMrNew] = 0;
         m.counters[kCntMrDown] = 0;
+        m.counters[kCntNodesStart] = m.counters[kCntNodes];
         for (int i = 0; i < 4; i++)
             m.counters[kCntMrScale + i] = 0;
         const Iso T = T16_dev ? iso_from16(T16_dev) : T_arg;
@@ -317,6 +318,260 @@ __global__ void k_alloc_rays(MapView m, Sensor sensor, const FramePoses* fp, con
                 if (at < cand_cap)
                     cand[at] = keys[j];
                 at++;
+            }
+        }
+    }
+}
+
+// Exclusive prefix over the wave of per-lane counts in [0, 16) and the wave's total, by ballots of
+// the count's bits.
+__device__ __forceinline__ void wave_prefix_small(int cnt, int& before, int& total)
+{
+    const uint64_t lt = (1ull << __lane_id()) - 1ull;
+    before = total = 0;
+#pragma unroll
+    for (int b = 0; b < 4; b++) {
+        const uint64_t mb = __ballot((cnt >> b) & 1);
+        before += __popcll(mb & lt) << b;
+        total += __popcll(mb) << b;
+    }
+}
+
+// Per lane, `total` consecutive positions of *counter for the wave (one atomic, by the first lane
+// with a non-zero count); returns this lane's first position.  Every lane of the wave must call it.
+__device__ __forceinline__ int wave_reserve(int cnt, int* counter)
+{
+    int before, total;
+    wave_prefix_small(cnt, before, total);
+    if (total == 0)
+        return 0;
+    const int leader = __ffsll((unsigned long long) __ballot(cnt > 0)) - 1;
+    int base = 0;
+    if (__lane_id() == leader)
+        base = atomicAdd(counter, total);
+    return __shfl(base, leader) + before;
+}
+
+constexpr int kClaimed = -2; // dense-table entry of an octant a thread is creating in this kernel
+
+// Creates the missing ancestors of a block this thread has just created, bottom-up (Octree::allocate
+// creates nodes top-down, octree_impl.hpp:171-199; the resulting node set is the same): each node is
+// claimed in its scale's dense table, so only one thread creates an octant, then takes a pool
+// index.  A node that exists, or that another thread has claimed, ends the walk: its ancestors exist
+// or are being created by that thread.  Child links are written by k_link_new, once every index of
+// the frame is final.
+__device__ void claim_ancestors(const MapView& m, const int c[3], uint64_t code)
+{
+    const int brick_scale = m.brick_shift + kBlockScale;
+    for (int s = kBlockScale + 1; s < m.max_scale; s++) {
+        int* e = m.node_dense + m.node_dense_off[s] + dense_index(m, c, s);
+        if (atomicCAS(e, -1, kClaimed) != -1)
+            return;
+        const int nn = atomicAdd(&m.counters[kCntNodes], 1);
+        if (nn >= m.max_nodes) {
+            atomicExch(&m.counters[kCntOverflow], 1);
+            atomicExch(e, -1);
+            return;
+        }
+        const uint64_t mask = (0x1ffffffffffffffull >> (3 * s)) << (3 * s);
+        m.node_key[nn] = ((code & mask) << kScaleOffset) | (uint64_t) s;
+        m.node_ts[nn] = -1;
+        if (s == brick_scale) {
+            const int br = atomicAdd(&m.counters[kCntBricks], 1);
+            if (br >= m.max_bricks)
+                atomicExch(&m.counters[kCntOverflow], 1);
+            else
+                m.brick_top[((c[2] >> s) * m.bricks_side + (c[1] >> s)) * m.bricks_side + (c[0] >> s)] = br;
+        }
+        atomicExch(e, nn);
+    }
+}
+
+// create-list + allocate-list for maps with the dense tables: the band march of RaycastCarver
+// (raycast_carver_impl.hpp:74-111), as k_alloc_rays samples it, with the allocation of
+// allocator::blocks / Octree::allocate (allocator_impl.hpp:68-147, octree_impl.hpp:171-199) done in
+// place: the first thread to reach an unallocated block claims its dense-table entry, creates the
+// block (slot, key, timestamp, an update-list entry marked new) and its missing ancestors.  The set
+// of blocks and nodes is that of the sorted-key path; slot and node numbering follow the schedule.
+// kCntCandidates counts the samples' block keys of unallocated blocks (consecutive repeats dropped).
+template<class Sensor>
+__global__ void k_carve_claim(MapView m, Sensor sensor, const FramePoses* fp, const float* depth, float band,
+                              int num_steps, int* update_list)
+{
+    const Iso T_WS = fp->T_WS;
+    const int x = blockIdx.x * blockDim.x + threadIdx.x;
+    const int y = blockIdx.y;
+    bool live = x < sensor.width();
+    float pos[3] = {0.f, 0.f, 0.f}, step[3] = {0.f, 0.f, 0.f};
+    if (live) {
+        const float d = depth[x + (size_t) y * sensor.width()];
+        if (d < sensor.near_plane() || d > (sensor.far_plane() + band * 0.5f))
+            live = false;
+        else {
+            float dir[3], pC[3], pW[3];
+            sensor.backproject(x, y, dir);
+            for (int i = 0; i < 3; i++)
+                pC[i] = d * dir[i];
+            iso_apply(T_WS, pC, pW);
+            float diff[3], rev[3];
+            for (int i = 0; i < 3; i++)
+                diff[i] = T_WS.t[i] - pW[i];
+            normalized3(diff, rev);
+            for (int i = 0; i < 3; i++) {
+                pos[i] = pW[i] - (band * 0.5f) * rev[i];
+                step[i] = (rev[i] * band) / (float) num_steps;
+            }
+        }
+    }
+    const int side = m.size >> 3;
+    constexpr int kBatch = 8;
+    uint64_t prev = ~0ull;
+    for (int s0 = 0; s0 < num_steps; s0 += kBatch) {
+        int vox[kBatch][3];
+        bool inb[kBatch];
+#pragma unroll
+        for (int j = 0; j < kBatch; j++) {
+            inb[j] = false;
+            vox[j][0] = vox[j][1] = vox[j][2] = 0;
+            if (live && s0 + j < num_steps) {
+                float pM[3];
+                for (int i = 0; i < 3; i++)
+                    pM[i] = pos[i] + m.t_MW[i];
+                if (pM[0] >= 0.f && pM[0] < m.dim && pM[1] >= 0.f && pM[1] < m.dim && pM[2] >= 0.f && pM[2] < m.dim) {
+                    const int wrap = m.size - 1; // see lookup_block
+                    vox[j][0] = ((int) (pM[0] / m.res)) & wrap;
+                    vox[j][1] = ((int) (pM[1] / m.res)) & wrap;
+                    vox[j][2] = ((int) (pM[2] / m.res)) & wrap;
+                    inb[j] = true;
+                }
+                for (int i = 0; i < 3; i++)
+                    pos[i] = pos[i] + step[i];
+            }
+        }
+        int slot[kBatch];
+        int64_t bi[kBatch];
+#pragma unroll
+        for (int j = 0; j < kBatch; j++) {
+            bi[j] = ((int64_t) (vox[j][2] >> 3) * side + (vox[j][1] >> 3)) * side + (vox[j][0] >> 3);
+            slot[j] = m.block_dense[bi[j]];
+        }
+        uint64_t keys[kBatch];
+        bool won[kBatch];
+        int cnt = 0, wins = 0;
+#pragma unroll
+        for (int j = 0; j < kBatch; j++) {
+            keys[j] = block_key_of_voxel(vox[j][0], vox[j][1], vox[j][2]);
+            won[j] = false;
+            if (inb[j] && slot[j] < 0) {
+                if (keys[j] != prev) {
+                    cnt++;
+                    // an entry already claimed in this kernel is being created by its claimant
+                    if (slot[j] == -1)
+                        won[j] = atomicCAS(&m.block_dense[bi[j]], -1, kClaimed) == -1;
+                }
+                prev = keys[j];
+            }
+            wins += won[j] ? 1 : 0;
+        }
+        const int cand0 = wave_reserve(cnt, &m.counters[kCntCandidates]);
+        (void) cand0;
+        // slots for the wave's new blocks, then update-list positions for those that fit the pool
+        const int sbase = wave_reserve(wins, &m.counters[kCntBlocks]);
+        int made = 0;
+#pragma unroll
+        for (int j = 0, k = 0; j < kBatch; j++)
+            if (won[j])
+                made += (sbase + k++ < m.max_blocks) ? 1 : 0;
+        const int ubase = wave_reserve(made, &m.counters[kCntUpdate]);
+        (void) wave_reserve(made, &m.counters[kCntUnique]);
+#pragma unroll
+        for (int j = 0, k = 0, u = 0; j < kBatch; j++) {
+            if (!won[j])
+                continue;
+            const int sl = sbase + k++;
+            if (sl >= m.max_blocks) {
+                atomicExch(&m.counters[kCntOverflow], 1);
+                atomicExch(&m.block_dense[bi[j]], -1);
+                continue;
+            }
+            m.block_key[sl] = keys[j];
+            m.block_ts[sl] = -1;
+            update_list[ubase + u++] = (int) ((uint32_t) sl | 0x80000000u);
+            atomicExch(&m.block_dense[bi[j]], sl);
+            const int c[3] = {vox[j][0] & ~7, vox[j][1] & ~7, vox[j][2] & ~7};
+            claim_ancestors(m, c, keys[j] >> kScaleOffset);
+        }
+    }
+}
+
+// The links of the octants k_carve_claim created: each new node becomes its parent's child (the root
+// for the top scale), each new block its scale-4 parent's child (block_parent), its brick's slot
+// entry and part of the AABB (octree_impl.hpp:267-273); the frustum part of the update list is the
+// rest of it.  Loops run over whole waves (the AABB is aggregated by ballots).
+__global__ void k_link_new(MapView m, const int* update_list)
+{
+    const int n_upd = m.counters[kCntUpdate];
+    if (blockIdx.x == 0 && threadIdx.x == 0)
+        m.counters[kCntFrustum] = n_upd - m.counters[kCntUnique];
+    const int stride = gridDim.x * blockDim.x;
+    const int n0 = m.counters[kCntNodesStart];
+    const int n1 = (int) min((int64_t) m.counters[kCntNodes], m.max_nodes);
+    for (int i = n0 + blockIdx.x * blockDim.x + threadIdx.x; i < n1; i += stride) {
+        const uint64_t key = m.node_key[i];
+        const int s = (int) (key & 0x1f);
+        const uint64_t code = key >> kScaleOffset;
+        int c[3];
+        key_to_coord(key, c);
+        const int parent = s + 1 >= m.max_scale ? 0 : m.node_dense[m.node_dense_off[s + 1] + dense_index(m, c, s + 1)];
+        if (parent >= 0)
+            m.node_child[parent * 8 + (int) ((code >> (3 * s)) & 7)] = i;
+    }
+    for (int base = blockIdx.x * blockDim.x; base < n_upd; base += stride) {
+        const int i = base + threadIdx.x;
+        bool made = false;
+        int c[3] = {0, 0, 0};
+        if (i < n_upd) {
+            const int entry = update_list[i];
+            if (entry < 0) {
+                made = true;
+                const int slot = entry & 0x7fffffff;
+                const uint64_t key = m.block_key[slot];
+                const uint64_t code = key >> kScaleOffset;
+                key_to_coord(key, c);
+                const int parent = m.max_scale > kBlockScale + 1
+                    ? m.node_dense[m.node_dense_off[kBlockScale + 1] + dense_index(m, c, kBlockScale + 1)]
+                    : 0;
+                m.block_parent[slot] = parent;
+                if (parent >= 0)
+                    m.node_child[parent * 8 + (int) ((code >> 9) & 7)] = slot;
+                const int bs = m.brick_shift + kBlockScale;
+                const int brick = m.brick_top[((c[2] >> bs) * m.bricks_side + (c[1] >> bs)) * m.bricks_side + (c[0] >> bs)];
+                const int msk = (1 << m.brick_shift) - 1;
+                const int bx = c[0] >> 3, by = c[1] >> 3, bz = c[2] >> 3;
+                if (brick >= 0)
+                    m.brick_slots[brick * 512 + (bx & msk) + ((by & msk) << 3) + ((bz & msk) << 6)] = slot;
+            }
+        }
+        if (__ballot(made)) {
+            const int lo[3] = {made ? c[0] : 0x7fffffff, made ? c[1] : 0x7fffffff, made ? c[2] : 0x7fffffff};
+            const int hi[3] = {made ? c[0] + 7 : -1, made ? c[1] + 7 : -1, made ? c[2] + 7 : -1};
+            int wlo[3], whi[3];
+#pragma unroll
+            for (int a = 0; a < 3; a++) {
+                wlo[a] = lo[a];
+                whi[a] = hi[a];
+#pragma unroll
+                for (int o = 32; o >= 1; o >>= 1) {
+                    wlo[a] = min(wlo[a], __shfl_xor(wlo[a], o));
+                    whi[a] = max(whi[a], __shfl_xor(whi[a], o));
+                }
+            }
+            if (__lane_id() == 0) {
+#pragma unroll
+                for (int a = 0; a < 3; a++) {
+                    atomicMin(&m.counters[kCntAabb + a], wlo[a]);
+                    atomicMax(&m.counters[kCntAabb + 3 + a], whi[a]);
+                }
             }
         }
     }
@@ -540,6 +795,11 @@ __device__ void build_level_one(const MapView& m, const uint64_t* keys, int i, i
     const uint64_t mask = (0x1ffffffffffffffull >> (3 * scale)) << (3 * scale);
     m.node_key[nn] = ((code & mask) << kScaleOffset) | (uint64_t) scale;
     m.node_ts[nn] = -1;
+    if (m.node_dense) {
+        int c[3];
+        key_to_coord(key, c);
+        m.node_dense[m.node_dense_off[scale] + dense_index(m, c, scale)] = nn;
+    }
     if (scale == brick_scale) {
         const int br = atomicAdd(&m.counters[kCntBricks], 1);
         if (br >= m.max_bricks) {
@@ -975,9 +1235,25 @@ extern "C" int se8_map_create(const se8_map_config* cfg, const se8_tsdf_config* 
     int rc = SE8_OK;
     rc = rc ? rc : dev_alloc(m->brick_top, sizeof(int) * bricks_side3);
     rc = rc ? rc : dev_alloc(m->brick_slots, sizeof(int) * 512 * v.max_bricks);
+    // Dense tables (maps of at most kDenseMaxSize voxels a side): one block slot per block position
+    // (a lookup is one load) and one node index per octant of every scale in [4, max_scale)
+    // (allocation claims octants in place).  They are an acceleration only: if either allocation
+    // fails, the map uses the brick index and the sorted-key allocation instead.
     const int64_t dense_side = size / 8;
-    if (size <= kDenseMaxSize)
-        rc = rc ? rc : dev_alloc(m->block_dense, sizeof(int) * dense_side * dense_side * dense_side);
+    int64_t node_dense_total = 0;
+    for (int sc = kBlockScale + 1; sc < p; sc++) {
+        v.node_dense_off[sc] = node_dense_total;
+        const int64_t sd = (int64_t) size >> sc;
+        node_dense_total += sd * sd * sd;
+    }
+    if (rc == SE8_OK && size <= kDenseMaxSize) {
+        if (dev_alloc(m->block_dense, sizeof(int) * dense_side * dense_side * dense_side) != SE8_OK
+            || dev_alloc(m->node_dense, sizeof(int) * std::max<int64_t>(node_dense_total, 1)) != SE8_OK) {
+            dev_free(m->block_dense);
+            dev_free(m->node_dense);
+            set_error("");
+        }
+    }
     rc = rc ? rc : dev_alloc(m->node_child, sizeof(int) * 8 * v.max_nodes);
     rc = rc ? rc : dev_alloc(m->node_key, sizeof(uint64_t) * v.max_nodes);
     rc = rc ? rc : dev_alloc(m->node_ts, sizeof(int) * v.max_nodes);
@@ -997,6 +1273,7 @@ extern "C" int se8_map_create(const se8_map_config* cfg, const se8_tsdf_config* 
     v.brick_top = (int*) m->brick_top.ptr;
     v.brick_slots = (int*) m->brick_slots.ptr;
     v.block_dense = (int*) m->block_dense.ptr;
+    v.node_dense = (int*) m->node_dense.ptr;
     v.node_child = (int*) m->node_child.ptr;
     v.node_key = (uint64_t*) m->node_key.ptr;
     v.node_ts = (int*) m->node_ts.ptr;
@@ -1009,8 +1286,10 @@ extern "C" int se8_map_create(const se8_map_config* cfg, const se8_tsdf_config* 
     hipStream_t s = nullptr;
     rc = rc ? rc : fill_i32(m->brick_top, bricks_side3, -1, s);
     rc = rc ? rc : fill_i32(m->brick_slots, 512 * v.max_bricks, -1, s);
-    if (m->block_dense.ptr)
+    if (m->block_dense.ptr) {
         rc = rc ? rc : fill_i32(m->block_dense, dense_side * dense_side * dense_side, -1, s);
+        rc = rc ? rc : fill_i32(m->node_dense, std::max<int64_t>(node_dense_total, 1), -1, s);
+    }
     rc = rc ? rc : fill_i32(m->node_child, 8 * v.max_nodes, -1, s);
     if (rc != SE8_OK) {
         se8_map_destroy(m);
@@ -1071,7 +1350,7 @@ extern "C" int se8_map_destroy(se8_map* m)
                             &m->block_key,    &m->block_parent, &m->block_ts,  &m->voxels,     &m->counters,
                             &m->update_list,  &m->candidates,   &m->unique_keys, &m->cur_node,
                             &m->frame_poses,  &m->mesh_offsets, &m->block_scale, &m->mr_new_scale,
-                            &m->block_dense};
+                            &m->block_dense,  &m->node_dense};
     for (DeviceBuffer* b : bufs)
         dev_free(*b);
     m->sorter.release();
@@ -1120,7 +1399,10 @@ static int integrate_impl(se8_map* m, int timestamp, const float* depth_dev, con
     const float band = 2.0f * m->view.res * m->tsdf.truncation_boundary_factor;
     const int num_steps = (int) std::ceil(band / (2.f * m->view.res));
     const size_t cand_cap = (size_t) sensor.width() * sensor.height() * num_steps;
-    int rc = ensure_candidates(m, cand_cap);
+    // maps with the dense block and node tables allocate in place (k_carve_claim + k_link_new);
+    // others sort the candidate keys and build the octree level by level
+    const bool claim = m->view.block_dense && m->view.node_dense;
+    int rc = claim ? SE8_OK : ensure_candidates(m, cand_cap);
     if (rc != SE8_OK)
         return rc;
 
@@ -1150,15 +1432,26 @@ static int integrate_impl(se8_map* m, int timestamp, const float* depth_dev, con
     if (timed)
         SE8_HIP_CHECK(hipEventRecord(m->ev[1], s));
     dim3 grid((sensor.width() + 63) / 64, sensor.height());
-    k_alloc_rays<Sensor><<<grid, 64, 0, s>>>(m->view, sensor, fp, depth_dev, band, num_steps,
-                                             (uint64_t*) m->candidates.ptr, (int64_t) cand_cap);
-    SE8_HIP_CHECK(hipGetLastError());
-    if (timed)
-        SE8_HIP_CHECK(hipEventRecord(m->ev[2], s));
-    rc = build_from_candidates(m, (uint64_t*) m->candidates.ptr, m->view.counters + kCntCandidates, (int) cand_cap,
-                               (int*) m->update_list.ptr, 1, s);
-    if (rc != SE8_OK)
-        return rc;
+    if (claim) {
+        k_carve_claim<Sensor><<<grid, 64, 0, s>>>(m->view, sensor, fp, depth_dev, band, num_steps,
+                                                  (int*) m->update_list.ptr);
+        SE8_HIP_CHECK(hipGetLastError());
+        if (timed)
+            SE8_HIP_CHECK(hipEventRecord(m->ev[2], s));
+        k_link_new<<<cus, 256, 0, s>>>(m->view, (const int*) m->update_list.ptr);
+        SE8_HIP_CHECK(hipGetLastError());
+    }
+    else {
+        k_alloc_rays<Sensor><<<grid, 64, 0, s>>>(m->view, sensor, fp, depth_dev, band, num_steps,
+                                                 (uint64_t*) m->candidates.ptr, (int64_t) cand_cap);
+        SE8_HIP_CHECK(hipGetLastError());
+        if (timed)
+            SE8_HIP_CHECK(hipEventRecord(m->ev[2], s));
+        rc = build_from_candidates(m, (uint64_t*) m->candidates.ptr, m->view.counters + kCntCandidates,
+                                   (int) cand_cap, (int*) m->update_list.ptr, 1, s);
+        if (rc != SE8_OK)
+            return rc;
+    }
     // ev[3] / ev[4] are stamped at the start and end of the update kernel itself
     const int* ulist = (const int*) m->update_list.ptr;
     if (is_multires(m)) {

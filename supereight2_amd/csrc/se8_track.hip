@@ -15,9 +15,12 @@
 // one-workgroup launch solves the last pass and applies checkPoseKernel (icp.cpp:227-246).
 //
 // compensated = 0: fp32 sums, fixed-order fp32 cross-workgroup sum, fp32 Cholesky (reference).
-// compensated = 1: fp32 per-thread sums over a thread's few pixels, compensated (TwoSum sum and
-// error pairs) combination across lanes and waves into each workgroup's partial, FP64
-// cross-workgroup sum and FP64 LDL^T solve.
+// compensated = 1: fp32 sums over a thread's few pixels and a fp32 pairwise tree within each wave,
+// compensated (TwoSum sum and error pairs) combination of the workgroup's waves into its partial,
+// FP64 cross-workgroup sum and FP64 LDL^T solve.
+//
+// A launch after its level has converged (the reference's break, icp.cpp:208-223) with nothing left
+// to solve only hands the state on, without waiting for the partial rows.
 #include <hip/hip_runtime.h>
 
 #include <cmath>
@@ -36,9 +39,16 @@ constexpr int kIcpMaxWg = 320;
 constexpr int kPartialStride = kIcpMaxWg * 64; // floats per partial-row buffer
 constexpr int kPyrThreads = 256;
 constexpr int kMaxLevels = SE8_TRACKER_MAX_LEVELS;
+// Compensated path: workgroup i adds its partial row, in 64-bit fixed point, into bin i % kBins of
+// the launch's round (rounds rotate over three buffers: launch k adds into round k % 3, reads round
+// (k + 2) % 3 and clears round (k + 1) % 3).  Integer addition is exact and associative, so the sum
+// does not depend on the order of the atomics, and the next launch reads kBins rows instead of one
+// row per workgroup.
+constexpr int kBins = 8;
+static_assert(kBins * 32 == 256, "one bin entry per thread of a 256-thread workgroup");
 // Level-0 tile side of a k_pyramid workgroup: a multiple of 2^(L-1) for every supported level count
 constexpr int kPyrTile = 32;
-static_assert((kPyrTile >> (kMaxLevels - 1)) >= 1, "every level needs a tile pixel");
+static_assert(kPyrTile == 32 && (kPyrTile >> (kMaxLevels - 1)) >= 1, "every level needs a tile pixel");
 
 // State handed from one Gauss-Newton launch to the next, double-buffered by launch index k:
 // launch k reads slot (k + 1) & 1 and its workgroup 0 writes slot k & 1.
@@ -75,7 +85,8 @@ struct IcpArgs {
     const float4* norm;    // level normal map, one float4 {x, y, z, 0} per pixel
     const float* ref_pts;  // full-res raycast points (w*h*3, the caller's layout)
     const float* ref_nrm;  // full-res raycast normals
-    float* partials;       // two buffers of kPartialStride floats, by launch parity
+    float* partials;       // two buffers of kPartialStride floats, by launch parity (plain fp32 path)
+    long long* bins;       // compensated path: three rounds of kBins x 32 fixed-point sums
     IcpShared* sh;
     Iso T_ref_inv;         // batched adjoint only; the tracker reads sh->T_ref_inv
     Cam cam;               // full-resolution sensor (projection into the reference maps)
@@ -105,6 +116,7 @@ struct PyrArgs {
     int neg_y;          // point_cloud_to_normal<true> (left-handed sensor frame)
     int tiles_x;        // level-0 tiles per row
     IcpShared* init;    // when not null, thread 0 writes the state that the first launch reads
+    long long* bins;    // when not null (compensated tracker), cleared for the first launch
     Iso T;              // start pose, unless T16_dev is set
     const float* T16_dev;
 };
@@ -153,16 +165,33 @@ __global__ __launch_bounds__(kPyrThreads) void k_pyramid(PyrArgs a)
         st->rows = 0;
         st->iterations = 0;
     }
+    if (a.bins && blockIdx.x == 0)
+        for (int i = threadIdx.x; i < 3 * kBins * 32; i += kPyrThreads)
+            a.bins[i] = 0;
     const int L = a.levels;
     const int S = 1 << (L - 1);                 // ring at level 0
     const int tx = blockIdx.x % a.tiles_x, ty = blockIdx.x / a.tiles_x;
     const int ox = tx * kPyrTile - S, oy = ty * kPyrTile - S; // level-0 region origin, a multiple of 2^(L-1)
     int side = kPyrTile + 2 * S;
     {
-        const int w0 = a.lv[0].w, h0 = a.lv[0].h;
-        for (int i = threadIdx.x; i < side * side; i += kPyrThreads) {
-            const int gx = ox + i % side, gy = oy + i / side;
-            pyr[i] = (gx >= 0 && gx < w0 && gy >= 0 && gy < h0) ? a.d0[gx + (size_t) gy * w0] : 0.f;
+        // every load of a batch is issued before any LDS store, so the region arrives in one round trip
+        // (two at most for deep pyramids) rather than one per element
+        constexpr int kBatch = 8;
+        const int w0 = a.lv[0].w, h0 = a.lv[0].h, n0 = side * side;
+        for (int base = 0; base < n0; base += kBatch * kPyrThreads) {
+            float v[kBatch];
+#pragma unroll
+            for (int j = 0; j < kBatch; j++) {
+                const int i = base + j * kPyrThreads + threadIdx.x;
+                const int gx = ox + i % side, gy = oy + i / side;
+                v[j] = (i < n0 && gx >= 0 && gx < w0 && gy >= 0 && gy < h0) ? a.d0[gx + (size_t) gy * w0] : 0.f;
+            }
+#pragma unroll
+            for (int j = 0; j < kBatch; j++) {
+                const int i = base + j * kPyrThreads + threadIdx.x;
+                if (i < n0)
+                    pyr[i] = v[j];
+            }
         }
     }
     int off = 0;
@@ -197,61 +226,55 @@ __global__ __launch_bounds__(kPyrThreads) void k_pyramid(PyrArgs a)
         }
     }
     __syncthreads();
-    // vertices and normals of the tile's pixels, all levels: level l has (kPyrTile >> l)^2 of them
-    int total = 0;
-    for (int l = 0; l < L; l++)
-        total += (kPyrTile >> l) * (kPyrTile >> l);
-    for (int i = threadIdx.x; i < total; i += kPyrThreads) {
-        int l = 0, j = i, loff = 0, lside = kPyrTile + 2 * S;
-        while (j >= (kPyrTile >> l) * (kPyrTile >> l)) {
-            j -= (kPyrTile >> l) * (kPyrTile >> l);
-            loff += lside * lside;
-            lside >>= 1;
-            l++;
-        }
-        const PyrLevel& P = a.lv[l];
+    // vertices and normals of the tile's pixels, level by level (the level is uniform in the
+    // workgroup, so its parameters are scalar loads): level l has (kPyrTile >> l)^2 of them
+    for (int l = 0, loff = 0, lside = kPyrTile + 2 * S; l < L; loff += lside * lside, lside >>= 1, l++) {
+        const PyrLevel P = a.lv[l];
+        const int lg = 5 - l; // log2(kPyrTile >> l)
         const int tside = kPyrTile >> l, ring = S >> l;
-        const int gx = (ox >> l) + ring + j % tside, gy = (oy >> l) + ring + j / tside;
-        if (gx >= P.w || gy >= P.h)
-            continue;
-        const float* D = pyr + loff;
-        const int rox = ox >> l, roy = oy >> l; // region origin at level l
-        auto point = [&](int x, int y, float p[3]) {
-            const float d = D[(x - rox) + (y - roy) * lside];
-            if (d > 0.f) {
-                const float dir[3] = {((float) x - P.cx) * P.inv_fx, ((float) y - P.cy) * P.inv_fy, 1.f};
-                p[0] = d * dir[0];
-                p[1] = d * dir[1];
-                p[2] = d * dir[2];
-            }
-            else
-                p[0] = p[1] = p[2] = 0.f;
-        };
-        float p[3];
-        point(gx, gy, p);
-        float n[3] = {0.f, 0.f, 0.f};
-        if (p[2] != 0.f) {
-            const int xl = max(gx - 1, 0), xr = min(gx + 1, P.w - 1);
-            const int ym = max(gy - 1, 0), yp = min(gy + 1, P.h - 1);
-            const int yu = a.neg_y ? ym : yp, yd = a.neg_y ? yp : ym;
-            float pl[3], pr[3], pu[3], pd[3];
-            point(xl, gy, pl);
-            point(xr, gy, pr);
-            point(gx, yu, pu);
-            point(gx, yd, pd);
-            if (pl[2] != 0.f && pr[2] != 0.f && pu[2] != 0.f && pd[2] != 0.f) {
-                float dx[3], dy[3], cr[3];
-                for (int k = 0; k < 3; k++) {
-                    dx[k] = pr[k] - pl[k];
-                    dy[k] = pu[k] - pd[k];
+        for (int j = threadIdx.x; j < tside * tside; j += kPyrThreads) {
+            const int gx = (ox >> l) + ring + (j & (tside - 1)), gy = (oy >> l) + ring + (j >> lg);
+            if (gx >= P.w || gy >= P.h)
+                continue;
+            const float* D = pyr + loff;
+            const int rox = ox >> l, roy = oy >> l; // region origin at level l
+            auto point = [&](int x, int y, float p[3]) {
+                const float d = D[(x - rox) + (y - roy) * lside];
+                if (d > 0.f) {
+                    const float dir[3] = {((float) x - P.cx) * P.inv_fx, ((float) y - P.cy) * P.inv_fy, 1.f};
+                    p[0] = d * dir[0];
+                    p[1] = d * dir[1];
+                    p[2] = d * dir[2];
                 }
-                cross3(dx, dy, cr);
-                normalized3(cr, n);
+                else
+                    p[0] = p[1] = p[2] = 0.f;
+            };
+            float p[3];
+            point(gx, gy, p);
+            float n[3] = {0.f, 0.f, 0.f};
+            if (p[2] != 0.f) {
+                const int xl = max(gx - 1, 0), xr = min(gx + 1, P.w - 1);
+                const int ym = max(gy - 1, 0), yp = min(gy + 1, P.h - 1);
+                const int yu = a.neg_y ? ym : yp, yd = a.neg_y ? yp : ym;
+                float pl[3], pr[3], pu[3], pd[3];
+                point(xl, gy, pl);
+                point(xr, gy, pr);
+                point(gx, yu, pu);
+                point(gx, yd, pd);
+                if (pl[2] != 0.f && pr[2] != 0.f && pu[2] != 0.f && pd[2] != 0.f) {
+                    float dx[3], dy[3], cr[3];
+                    for (int k = 0; k < 3; k++) {
+                        dx[k] = pr[k] - pl[k];
+                        dy[k] = pu[k] - pd[k];
+                    }
+                    cross3(dx, dy, cr);
+                    normalized3(cr, n);
+                }
             }
+            const size_t o = gx + (size_t) gy * P.w;
+            P.vert[o] = make_float4(p[0], p[1], p[2], 0.f);
+            P.norm[o] = make_float4(n[0], n[1], n[2], 0.f);
         }
-        const size_t o = gx + (size_t) gy * P.w;
-        P.vert[o] = make_float4(p[0], p[1], p[2], 0.f);
-        P.norm[o] = make_float4(n[0], n[1], n[2], 0.f);
     }
 }
 
@@ -377,10 +400,11 @@ __device__ __forceinline__ void two_sum(float a, float b, float& s, float& err)
     err = (a - (s - bb)) + (b - bb);
 }
 
-// 6x6 Cholesky in fp32 (icp.cpp:33-41): x = 0 if the factorisation fails
+// 6x6 Cholesky in fp32 (icp.cpp:33-41): x = 0 if the factorisation fails.  Each pivot's reciprocal
+// is taken once (6 divisions instead of 27 on the serial path of every Gauss-Newton launch).
 __device__ void solve_f32(const float* vals, float x[6])
 {
-    float A[6][6], L[6][6];
+    float A[6][6], L[6][6], inv[6];
     int c = 6;
     for (int i = 0; i < 6; i++)
         for (int j = i; j < 6; j++) {
@@ -400,11 +424,12 @@ __device__ void solve_f32(const float* vals, float x[6])
             return;
         }
         L[k][k] = sqrtf(s);
+        inv[k] = 1.f / L[k][k];
         for (int i = k + 1; i < 6; i++) {
             float v = A[i][k];
             for (int j = 0; j < k; j++)
                 v -= L[i][j] * L[k][j];
-            L[i][k] = v / L[k][k];
+            L[i][k] = v * inv[k];
         }
     }
     float y[6];
@@ -412,20 +437,20 @@ __device__ void solve_f32(const float* vals, float x[6])
         float v = vals[i];
         for (int j = 0; j < i; j++)
             v -= L[i][j] * y[j];
-        y[i] = v / L[i][i];
+        y[i] = v * inv[i];
     }
     for (int i = 5; i >= 0; i--) {
         float v = y[i];
         for (int j = i + 1; j < 6; j++)
             v -= L[j][i] * x[j];
-        x[i] = v / L[i][i];
+        x[i] = v * inv[i];
     }
 }
 
 // 6x6 LDL^T in FP64 (no pivoting: J^T J is symmetric positive semi-definite); x = 0 on failure
 __device__ void solve_f64(const double* vals, float x[6])
 {
-    double A[6][6], L[6][6], D[6];
+    double A[6][6], L[6][6], D[6], rD[6];
     int c = 6;
     for (int i = 0; i < 6; i++)
         for (int j = i; j < 6; j++) {
@@ -442,12 +467,13 @@ __device__ void solve_f64(const double* vals, float x[6])
             return;
         }
         D[j] = dj;
+        rD[j] = 1.0 / dj;
         L[j][j] = 1.0;
         for (int i = j + 1; i < 6; i++) {
             double v = A[i][j];
             for (int k = 0; k < j; k++)
                 v -= L[i][k] * L[j][k] * D[k];
-            L[i][j] = v / dj;
+            L[i][j] = v * rD[j];
         }
     }
     double y[6], z[6];
@@ -458,7 +484,7 @@ __device__ void solve_f64(const double* vals, float x[6])
         y[i] = v;
     }
     for (int i = 0; i < 6; i++)
-        y[i] /= D[i];
+        y[i] *= rD[i];
     for (int i = 5; i >= 0; i--) {
         double v = y[i];
         for (int k = i + 1; k < 6; k++)
@@ -660,33 +686,31 @@ __device__ __forceinline__ void load_tile(const IcpArgs& a, int tile, int n, flo
     }
 }
 
-// Solving the previous launch's pass is split in three so that its memory round trips overlap:
-// load_rows issues the loads of the partial rows (their count is known on the host, so they do not
-// wait for the slot), slot_word / put_slot_word bring the previous slot into LDS, rows_to_lds adds
-// each thread's rows and solve_lds adds the groups and solves.  Identical in every workgroup.  Thread t owns terms 4q..4q+3 (q = t & 7) of rows
-// b = g, g + 32, ... (g = t >> 3); it adds its rows in increasing b; then thread k < 32 adds the
-// 32 group sums of term k in increasing g.  The rows were written by the previous kernel, so plain
-// loads see them.
+// Solving the previous launch's pass is split so that its memory round trips overlap: load_rows
+// (plain path) or load_bins (compensated path) issues the loads of the previous launch's partial
+// sums without waiting, slot_word / put_slot_word bring the previous slot into LDS, rows_to_lds adds
+// each thread's rows, gather_sums adds the groups (or bins) and solve_step solves.  Identical in
+// every workgroup.  On the plain path thread t owns terms 4q..4q+3 (q = t & 7) of rows b = g,
+// g + 32, ... (g = t >> 3); it adds its rows in increasing b; then thread k < 32 adds the 32 group
+// sums of term k in increasing g.  The rows were written by the previous kernel, so plain loads see
+// them.
 constexpr int kRowsPerThread = kIcpMaxWg / 32;
 
-template<bool Comp>
 struct RowFrag {
     float4 sv[kRowsPerThread];
-    float4 cv[Comp ? kRowsPerThread : 1];
 };
 
-// Loads every row slot a thread may own (the buffers hold kIcpMaxWg rows, so all are in bounds)
-// without waiting; rows at or beyond nrows are masked when they are added.
-template<bool Comp>
-__device__ __forceinline__ void load_rows(const float* rows, RowFrag<Comp>& f)
+// Issues the loads of the rows a thread owns among the nrows rows written (nrows <= kIcpMaxWg, the
+// buffer's capacity) without waiting; the other entries are zero and are masked when they are added.
+__device__ __forceinline__ void load_rows(const float* rows, int nrows, RowFrag& f)
 {
     const int t = threadIdx.x, q = t & 7, g = t >> 3;
 #pragma unroll
     for (int i = 0; i < kRowsPerThread; i++) {
         const int b = g + 32 * i;
-        f.sv[i] = reinterpret_cast<const float4*>(rows + b * 64)[q];
-        if (Comp)
-            f.cv[i] = reinterpret_cast<const float4*>(rows + b * 64 + 32)[q];
+        f.sv[i] = make_float4(0.f, 0.f, 0.f, 0.f);
+        if (b < nrows)
+            f.sv[i] = reinterpret_cast<const float4*>(rows + b * 64)[q];
     }
 }
 
@@ -709,91 +733,112 @@ __device__ __forceinline__ void put_slot_word(IcpSlot& cur, uint32_t w)
         reinterpret_cast<uint32_t*>(&cur)[t] = w;
 }
 
-// Per-group partial sums of the previous launch's rows (LDS), between rows_to_lds and solve_lds.
+// Per-group partial sums of the previous launch's rows (LDS), between rows_to_lds and gather_sums.
 struct RowLds {
-    double d[32][33];
     float f[32][33];
 };
 
 // Adds this thread's rows (those below a.prev_rows) and stores the group sums in LDS.  Waits only
 // for the row loads, so it runs before any later loads are issued.  The caller synchronises.
-template<bool Comp>
-__device__ __forceinline__ void rows_to_lds(const IcpArgs& a, const RowFrag<Comp>& f, RowLds& r)
+__device__ __forceinline__ void rows_to_lds(const IcpArgs& a, const RowFrag& f, RowLds& r)
 {
     const int t = threadIdx.x, q = t & 7, g = t >> 3;
     const int nrows = a.prev_rows;
-    if (Comp) {
-        double acc[4] = {0.0, 0.0, 0.0, 0.0};
+    float acc[4] = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-        for (int i = 0; i < kRowsPerThread; i++) {
-            const bool in = g + 32 * i < nrows;
-            acc[0] += in ? (double) f.sv[i].x + (double) f.cv[i].x : 0.0;
-            acc[1] += in ? (double) f.sv[i].y + (double) f.cv[i].y : 0.0;
-            acc[2] += in ? (double) f.sv[i].z + (double) f.cv[i].z : 0.0;
-            acc[3] += in ? (double) f.sv[i].w + (double) f.cv[i].w : 0.0;
-        }
-        for (int k = 0; k < 4; k++)
-            r.d[g][4 * q + k] = acc[k];
+    for (int i = 0; i < kRowsPerThread; i++) {
+        const bool in = g + 32 * i < nrows;
+        acc[0] += in ? f.sv[i].x : 0.f;
+        acc[1] += in ? f.sv[i].y : 0.f;
+        acc[2] += in ? f.sv[i].z : 0.f;
+        acc[3] += in ? f.sv[i].w : 0.f;
     }
-    else {
-        float acc[4] = {0.f, 0.f, 0.f, 0.f};
-#pragma unroll
-        for (int i = 0; i < kRowsPerThread; i++) {
-            const bool in = g + 32 * i < nrows;
-            acc[0] += in ? f.sv[i].x : 0.f;
-            acc[1] += in ? f.sv[i].y : 0.f;
-            acc[2] += in ? f.sv[i].z : 0.f;
-            acc[3] += in ? f.sv[i].w : 0.f;
-        }
-        for (int k = 0; k < 4; k++)
-            r.f[g][4 * q + k] = acc[k];
-    }
+    for (int k = 0; k < 4; k++)
+        r.f[g][4 * q + k] = acc[k];
 }
 
-// If the previous launch ran its pixel pass (cur.did_pixels), add the group sums in a fixed order,
-// solve, apply T <- exp(x) T and raise that launch's level's convergence flag (icp.cpp:15-41,
-// 208-223).  LDS only; ends with a barrier.
-template<bool Comp>
-__device__ void solve_lds(const IcpArgs& a, const RowLds& r, IcpSlot& cur)
+// Fixed-point scale of term t of a row: 2^40 for e^2 and e J (t < 7), 2^24 for J^T J (t < 28), 1 for
+// the counts.  |sum| stays below 2^23 resp. 2^39 for any frame of up to 2^20 pixels at ranges up to
+// 100 m; out-of-range values (and NaN) saturate.
+__device__ __forceinline__ double fixed_scale(int t)
+{
+    return t < 7 ? 1099511627776.0 : (t < 28 ? 16777216.0 : 1.0);
+}
+
+__device__ __forceinline__ unsigned long long to_fixed(double v, int t)
+{
+    const double lim = 4.0e18;
+    const double x = fmin(fmax(v * fixed_scale(t), -lim), lim);
+    return (unsigned long long) (long long) rint(x);
+}
+
+// The previous launch's round, one entry per thread (bin t >> 5, term t & 31); workgroup 0 also
+// clears the round the next launch adds into.  Issues the load without waiting.
+__device__ __forceinline__ long long load_bins(const IcpArgs& a)
 {
     const int t = threadIdx.x;
-    if (cur.did_pixels) {
-        if (t < 32) {
-            if (Comp) {
-                double acc = r.d[0][t];
-                for (int j = 1; j < 32; j++)
-                    acc += r.d[j][t];
-                cur.sums_d[t] = acc;
-                cur.sums[t] = (float) acc;
-            }
-            else {
-                float acc = r.f[0][t];
-                for (int j = 1; j < 32; j++)
-                    acc += r.f[j][t];
-                cur.sums[t] = acc;
-                cur.sums_d[t] = (double) acc;
-            }
-        }
+    const long long v = a.bins[((a.k + 2) % 3) * (kBins * 32) + t];
+    if (blockIdx.x == 0)
+        a.bins[((a.k + 1) % 3) * (kBins * 32) + t] = 0;
+    return v;
+}
+
+// Sums of the previous launch's pass into cur.sums / cur.sums_d: the plain path adds the group sums
+// of the rows (rows_to_lds) in a fixed order in fp32, the compensated path adds the kBins fixed-point
+// entries exactly and converts the totals to FP64.  Ends with a barrier.
+template<bool Comp>
+__device__ void gather_sums(const RowLds& r, long long* lb, long long bv, IcpSlot& cur)
+{
+    const int t = threadIdx.x;
+    if (Comp) {
+        lb[t] = bv;
         __syncthreads();
-        if (t == 0) {
-            float x[6];
-            if (Comp)
-                solve_f64(cur.sums_d + 1, x);
-            else
-                solve_f32(cur.sums + 1, x);
-            const Iso Tn = iso_mul(se3_exp(x), pose_of(cur.T));
-            store_pose(Tn, cur.T);
-            const float xn = sqrtf(x[0] * x[0] + (x[1] * x[1] + (x[2] * x[2] + (x[3] * x[3] + (x[4] * x[4] + x[5] * x[5])))));
-            if (xn < a.icp_th)
-                cur.done[cur.level] = 1;
-            cur.iterations += 1;
+    }
+    if (t < 32) {
+        if (Comp) {
+            long long acc = 0;
+            for (int b = 0; b < kBins; b++)
+                acc += lb[b * 32 + t];
+            const double v = (double) acc / fixed_scale(t);
+            cur.sums_d[t] = v;
+            cur.sums[t] = (float) v;
+        }
+        else {
+            float acc = r.f[0][t];
+            for (int j = 1; j < 32; j++)
+                acc += r.f[j][t];
+            cur.sums[t] = acc;
+            cur.sums_d[t] = (double) acc;
         }
     }
     __syncthreads();
 }
 
+// Solves the system of cur.sums (plain: fp32 Cholesky; compensated: FP64 LDL^T), applies
+// T <- exp(x) T and raises the level's convergence flag (icp.cpp:15-41, 208-223).  Ends with a
+// barrier.
+template<bool Comp>
+__device__ void solve_step(const IcpArgs& a, IcpSlot& cur)
+{
+    if (threadIdx.x == 0) {
+        float x[6];
+        if (Comp)
+            solve_f64(cur.sums_d + 1, x);
+        else
+            solve_f32(cur.sums + 1, x);
+        const Iso Tn = iso_mul(se3_exp(x), pose_of(cur.T));
+        store_pose(Tn, cur.T);
+        const float xn = sqrtf(x[0] * x[0] + (x[1] * x[1] + (x[2] * x[2] + (x[3] * x[3] + (x[4] * x[4] + x[5] * x[5])))));
+        if (xn < a.icp_th)
+            cur.done[cur.level] = 1;
+        cur.iterations += 1;
+    }
+    __syncthreads();
+}
+
 // Launch k of the Gauss-Newton sequence of Tracker::track (tracker_impl.hpp:77-108): solve the
-// pass of launch k - 1 (load_rows, slot_word, rows_to_lds, solve_lds), then, unless this level has
+// pass of launch k - 1 (slot_word, load_rows + rows_to_lds or load_bins, gather_sums, solve_step),
+// then, unless this level has
 // converged, the pixel pass of iteration k at this level: trackKernel (icp_impl.hpp:16-102) fused with the 32-term
 // reduceKernel sums (icp.cpp:45-204) over tiles of PPT pixels per thread, a halving wave
 // reduction and an LDS reduction to one partial row per workgroup.  Workgroup 0 publishes the
@@ -804,23 +849,39 @@ __global__ __launch_bounds__(kIcpThreads) void k_icp_iter(IcpArgs a)
 {
     __shared__ IcpSlot cur;
     __shared__ RowLds rl;
+    __shared__ long long lb[Comp ? kBins * 32 : 1];
     __shared__ float red[kIcpThreads / 64][64];
     const IcpSlot* prev = &a.sh->slot[(a.k + 1) & 1];
     const int n = a.w * a.h;
-    // issue order = wait order: the slot word first, then the tile and the partial rows; the rows
+    // issue order = wait order: the slot word first, then the tile and the partial rows (or bins);
+    // the slot is stored (waiting for it alone) and read before the rows are awaited, and the rows
     // are added before any later load is issued, so that waiting for them waits for nothing else
     const uint32_t sw = slot_word(prev);
     float4 v4[PPT], n4[PPT];
     load_tile<PPT>(a, blockIdx.x, n, v4, n4);
-    {
-        RowFrag<Comp> frag;
-        load_rows<Comp>(a.partials + ((a.k + 1) & 1) * kPartialStride, frag);
-        put_slot_word(cur, sw);
-        rows_to_lds<Comp>(a, frag, rl);
-    }
+    RowFrag frag;
+    long long bv = 0;
+    if (Comp)
+        bv = load_bins(a);
+    else
+        load_rows(a.partials + ((a.k + 1) & 1) * kPartialStride, a.prev_rows, frag);
+    put_slot_word(cur, sw);
     __syncthreads();
-    solve_lds<Comp>(a, rl, cur);
-    const bool run = cur.done[a.level] == 0;
+    // a launch after this level converged with nothing left to solve only hands the slot on; it
+    // does not wait for the rows (their loads were issued ahead, in case they were needed)
+    const bool noop = cur.did_pixels == 0 && cur.done[a.level] != 0;
+    bool run = false;
+    if (!noop) {
+        if (cur.did_pixels) {
+            if (!Comp) {
+                rows_to_lds(a, frag, rl);
+                __syncthreads();
+            }
+            gather_sums<Comp>(rl, lb, bv, cur);
+            solve_step<Comp>(a, cur);
+        }
+        run = cur.done[a.level] == 0;
+    }
     if (run) {
         float s[32], c[32];
         for (int k = 0; k < 32; k++) {
@@ -850,13 +911,14 @@ __global__ __launch_bounds__(kIcpThreads) void k_icp_iter(IcpArgs a)
                 float v[32];
                 row_terms(res, e, J, v);
                 // a thread sums only the pixels of its tiles (PPT per tile; one tile per workgroup
-                // at 640x480), a few fp32 roundings; compensation (Comp) starts at the lane and wave
-                // combination below, where the sums grow
+                // at 640x480), a few fp32 roundings
                 for (int k = 0; k < 32; k++)
                     s[k] += v[k];
             }
         }
-        wave_reduce32<Comp>(s, c);
+        // within a wave a plain fp32 pairwise tree (6 levels); the compensated path starts at the
+        // combination of the workgroup's waves, then FP64 across workgroups
+        wave_reduce32<false>(s, c);
         const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
         if ((lane & 1) == 0) {
             red[wave][lane >> 1] = s[0];
@@ -868,9 +930,15 @@ __global__ __launch_bounds__(kIcpThreads) void k_icp_iter(IcpArgs a)
             float ss = red[0][k], cc = red[0][32 + k];
             for (int w = 1; w < kIcpThreads / 64; w++)
                 pair_add<Comp>(ss, cc, red[w][k], red[w][32 + k]);
-            float* row = a.partials + (a.k & 1) * kPartialStride + blockIdx.x * 64;
-            row[k] = ss;
-            row[32 + k] = cc;
+            if (Comp) {
+                unsigned long long* bin = reinterpret_cast<unsigned long long*>(a.bins)
+                    + (a.k % 3) * (kBins * 32) + (blockIdx.x % kBins) * 32;
+                atomicAdd(bin + k, to_fixed((double) ss + (double) cc, k));
+            }
+            else {
+                float* row = a.partials + (a.k & 1) * kPartialStride + blockIdx.x * 64;
+                row[k] = ss;
+            }
         }
     }
     if (blockIdx.x == 0) {
@@ -900,15 +968,24 @@ __global__ __launch_bounds__(kIcpThreads) void k_icp_finish(IcpArgs a)
 {
     __shared__ IcpSlot cur;
     __shared__ RowLds rl;
+    __shared__ long long lb[Comp ? kBins * 32 : 1];
     const uint32_t sw = slot_word(&a.sh->slot[(a.k + 1) & 1]);
-    {
-        RowFrag<Comp> frag;
-        load_rows<Comp>(a.partials + ((a.k + 1) & 1) * kPartialStride, frag);
-        put_slot_word(cur, sw);
-        rows_to_lds<Comp>(a, frag, rl);
-    }
+    RowFrag frag;
+    long long bv = 0;
+    if (Comp)
+        bv = load_bins(a);
+    else
+        load_rows(a.partials + ((a.k + 1) & 1) * kPartialStride, a.prev_rows, frag);
+    put_slot_word(cur, sw);
     __syncthreads();
-    solve_lds<Comp>(a, rl, cur);
+    if (cur.did_pixels) {
+        if (!Comp) {
+            rows_to_lds(a, frag, rl);
+            __syncthreads();
+        }
+        gather_sums<Comp>(rl, lb, bv, cur);
+        solve_step<Comp>(a, cur);
+    }
     if (threadIdx.x == 0) {
         IcpState* r = &a.sh->result;
         for (int i = 0; i < 12; i++)
@@ -933,7 +1010,7 @@ __global__ __launch_bounds__(kIcpThreads) void k_icp_finish(IcpArgs a)
 // Vertex and normal maps of the first `levels` pyramid levels in one launch; writes the state the
 // first Gauss-Newton launch reads (pose T) when init is not null.
 static void launch_pyramid(se8_tracker* tr, const float* depth, int levels, IcpShared* init, const Iso& T,
-                           const float* T16_dev, hipStream_t s)
+                           const float* T16_dev, hipStream_t s, long long* bins = nullptr)
 {
     const se8_pinhole& P = tr->sensor;
     PyrArgs pa{};
@@ -955,6 +1032,7 @@ static void launch_pyramid(se8_tracker* tr, const float* depth, int levels, IcpS
     pa.tiles_x = (P.width + kPyrTile - 1) / kPyrTile;
     const int tiles_y = (P.height + kPyrTile - 1) / kPyrTile;
     pa.init = init;
+    pa.bins = bins;
     pa.T = T;
     pa.T16_dev = T16_dev;
     k_pyramid<<<pa.tiles_x * tiles_y, kPyrThreads, pyramid_lds_bytes(levels), s>>>(pa);
@@ -990,8 +1068,10 @@ static int enqueue_track(se8_tracker* tr, const float* depth, const float* T16_h
     harvest_tracking(tr);
     if (timed)
         SE8_HIP_CHECK(hipEventRecord(tr->ev[0], s));
-    launch_pyramid(tr, depth, tr->cfg.num_levels, sh, T16_host ? iso_from16(T16_host) : Iso{}, T_in_dev, s);
+    long long* bins = tr->cfg.compensated ? (long long*) tr->bins.ptr : nullptr;
+    launch_pyramid(tr, depth, tr->cfg.num_levels, sh, T16_host ? iso_from16(T16_host) : Iso{}, T_in_dev, s, bins);
     IcpArgs a{};
+    a.bins = bins;
     a.ref_pts = ref_pts;
     a.ref_nrm = ref_nrm;
     a.partials = (float*) tr->partials.ptr;
@@ -1197,7 +1277,10 @@ extern "C" int se8_tracker_create(const se8_pinhole* sensor, const se8_tracker_c
         const size_t n = (size_t) cl.w * cl.h;
         rc = rc ? rc : dev_alloc(t->vert_lvl[l], sizeof(float4) * n);
         rc = rc ? rc : dev_alloc(t->norm_lvl[l], sizeof(float4) * n);
-        // smallest pixels-per-thread whose tiles fit one grid of at most kIcpMaxWg workgroups
+        // smallest pixels-per-thread whose tiles fit one grid of at most kIcpMaxWg workgroups: more,
+        // lighter workgroups keep each wave's row arithmetic short, which costs less than the extra
+        // partial rows the next launch reads (320x240 and 160x120 are faster in 300 and 75 workgroups
+        // of 256 pixels than in 75 and 19 of 1024)
         int ppt = 1;
         while (ppt < 4 && (n + kIcpThreads * ppt - 1) / (kIcpThreads * ppt) > (size_t) kIcpMaxWg)
             ppt *= 2;
@@ -1206,6 +1289,7 @@ extern "C" int se8_tracker_create(const se8_pinhole* sensor, const se8_tracker_c
         t->num_wg[l] = std::min(t->icp_tiles[l], kIcpMaxWg);
     }
     rc = rc ? rc : dev_alloc(t->partials, sizeof(float) * 2 * kPartialStride);
+    rc = rc ? rc : dev_alloc(t->bins, sizeof(long long) * 3 * kBins * 32);
     rc = rc ? rc : dev_alloc(t->state, sizeof(IcpShared));
     if (rc == SE8_OK && hipHostMalloc(&t->host_state, sizeof(IcpState), hipHostMallocDefault) != hipSuccess) {
         t->host_state = nullptr;
@@ -1232,6 +1316,7 @@ extern "C" int se8_tracker_destroy(se8_tracker* t)
         dev_free(t->norm_lvl[l]);
     }
     dev_free(t->partials);
+    dev_free(t->bins);
     dev_free(t->state);
     if (t->host_state)
         (void) hipHostFree(t->host_state);
