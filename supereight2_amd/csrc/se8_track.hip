@@ -46,9 +46,14 @@ constexpr int kMaxLevels = SE8_TRACKER_MAX_LEVELS;
 // row per workgroup.
 constexpr int kBins = 8;
 static_assert(kBins * 32 == 256, "one bin entry per thread of a 256-thread workgroup");
-// Level-0 tile side of a k_pyramid workgroup: a multiple of 2^(L-1) for every supported level count
-constexpr int kPyrTile = 32;
-static_assert(kPyrTile == 32 && (kPyrTile >> (kMaxLevels - 1)) >= 1, "every level needs a tile pixel");
+// Level-0 tile side of a k_pyramid workgroup: 16 (2^4), or 2^(L-1) for deeper pyramids, so that every
+// level keeps at least one tile pixel.  Small tiles give many light workgroups, which hide the latency
+// of the per-pixel division chains better than few heavy ones.
+constexpr int kPyrTileLog2Min = 4;
+inline int pyramid_tile_log2(int levels)
+{
+    return levels - 1 > kPyrTileLog2Min ? levels - 1 : kPyrTileLog2Min;
+}
 
 // State handed from one Gauss-Newton launch to the next, double-buffered by launch index k:
 // launch k reads slot (k + 1) & 1 and its workgroup 0 writes slot k & 1.
@@ -114,6 +119,7 @@ struct PyrArgs {
     PyrLevel lv[kMaxLevels];
     int levels;
     int neg_y;          // point_cloud_to_normal<true> (left-handed sensor frame)
+    int tile_log2;      // log2 of the level-0 tile side
     int tiles_x;        // level-0 tiles per row
     IcpShared* init;    // when not null, thread 0 writes the state that the first launch reads
     long long* bins;    // when not null (compensated tracker), cleared for the first launch
@@ -121,18 +127,18 @@ struct PyrArgs {
     const float* T16_dev;
 };
 
-// LDS floats of one k_pyramid workgroup: the level-0 region of side kPyrTile + 2 * 2^(L-1) (the
-// tile and a ring of 2^(L-1) pixels) and each coarser level's region of half the side.
+// LDS floats of one k_pyramid workgroup: the level-0 region of side tile + 2 * 2^(L-1) (the tile and
+// a ring of 2^(L-1) pixels) and each coarser level's region of half the side.
 inline size_t pyramid_lds_bytes(int levels)
 {
     size_t n = 0;
-    for (int l = 0, side = kPyrTile + (2 << (levels - 1)); l < levels; l++, side >>= 1)
+    for (int l = 0, side = (1 << pyramid_tile_log2(levels)) + (2 << (levels - 1)); l < levels; l++, side >>= 1)
         n += (size_t) side * side;
     return n * sizeof(float);
 }
 
 // The whole input side of Tracker::track (tracker_impl.hpp:50-75) in one launch.  A workgroup owns
-// an aligned kPyrTile x kPyrTile tile of the full-resolution image.  A pixel of level l is the
+// an aligned tile x tile square of the full-resolution image.  A pixel of level l is the
 // robust half-sample of an aligned 2^l x 2^l square of level 0 (half_sample_robust_image,
 // preprocessor.cpp:156-195: its index clamp never moves a pixel of the image), so the workgroup
 // stages its tile plus a ring of 2^(L-1) pixels in LDS, half-samples level by level in LDS (each
@@ -170,9 +176,10 @@ __global__ __launch_bounds__(kPyrThreads) void k_pyramid(PyrArgs a)
             a.bins[i] = 0;
     const int L = a.levels;
     const int S = 1 << (L - 1);                 // ring at level 0
+    const int tile = 1 << a.tile_log2;
     const int tx = blockIdx.x % a.tiles_x, ty = blockIdx.x / a.tiles_x;
-    const int ox = tx * kPyrTile - S, oy = ty * kPyrTile - S; // level-0 region origin, a multiple of 2^(L-1)
-    int side = kPyrTile + 2 * S;
+    const int ox = tx * tile - S, oy = ty * tile - S; // level-0 region origin, a multiple of 2^(L-1)
+    int side = tile + 2 * S;
     {
         // every load of a batch is issued before any LDS store, so the region arrives in one round trip
         // (two at most for deep pyramids) rather than one per element
@@ -227,11 +234,11 @@ __global__ __launch_bounds__(kPyrThreads) void k_pyramid(PyrArgs a)
     }
     __syncthreads();
     // vertices and normals of the tile's pixels, level by level (the level is uniform in the
-    // workgroup, so its parameters are scalar loads): level l has (kPyrTile >> l)^2 of them
-    for (int l = 0, loff = 0, lside = kPyrTile + 2 * S; l < L; loff += lside * lside, lside >>= 1, l++) {
+    // workgroup, so its parameters are scalar loads): level l has (tile >> l)^2 of them
+    for (int l = 0, loff = 0, lside = tile + 2 * S; l < L; loff += lside * lside, lside >>= 1, l++) {
         const PyrLevel P = a.lv[l];
-        const int lg = 5 - l; // log2(kPyrTile >> l)
-        const int tside = kPyrTile >> l, ring = S >> l;
+        const int lg = a.tile_log2 - l; // log2(tile >> l)
+        const int tside = tile >> l, ring = S >> l;
         for (int j = threadIdx.x; j < tside * tside; j += kPyrThreads) {
             const int gx = (ox >> l) + ring + (j & (tside - 1)), gy = (oy >> l) + ring + (j >> lg);
             if (gx >= P.w || gy >= P.h)
@@ -1029,8 +1036,10 @@ static void launch_pyramid(se8_tracker* tr, const float* depth, int levels, IcpS
     }
     pa.levels = levels;
     pa.neg_y = make_cam(P, 1, false).left_hand_frame;
-    pa.tiles_x = (P.width + kPyrTile - 1) / kPyrTile;
-    const int tiles_y = (P.height + kPyrTile - 1) / kPyrTile;
+    pa.tile_log2 = pyramid_tile_log2(levels);
+    const int tile = 1 << pa.tile_log2;
+    pa.tiles_x = (P.width + tile - 1) / tile;
+    const int tiles_y = (P.height + tile - 1) / tile;
     pa.init = init;
     pa.bins = bins;
     pa.T = T;
