@@ -144,6 +144,7 @@ def lib(precision: str = "f32") -> C.CDLL:
     L.or_get_data.argtypes = [P, R, R, R]
     L.or_integrate.argtypes = [P, C.c_int, C.POINTER(C.c_float), C.POINTER(Pinhole), R, C.POINTER(C.c_long)]
     L.or_raycast.argtypes = [P, C.POINTER(Pinhole), R, R, R, C.POINTER(C.c_int8)]
+    L.or_raycast_stats.argtypes = [P, C.POINTER(Pinhole), R, R, R, I]
     FP = C.POINTER(C.c_float)
     L.or_lidar_create.restype = P
     L.or_lidar_create.argtypes = [C.c_int, C.c_int, C.c_float, C.c_float, FP, FP]
@@ -332,6 +333,18 @@ class OracleMap:
         if bits:
             return p, n, s, int(out[0]), int(out[1]), mask[: n_blocks * 8].reshape(n_blocks, 8)
         return p, n, s, int(out[0]), int(out[1])
+
+    def raycast_stats(self, pin, T_WS):
+        """Per-pixel march statistics (H, W, 7): traversal iterations, samples in absent bricks, in
+        unallocated blocks of present bricks, at unobserved and at observed voxels, interpolations, and
+        of the samples in unallocated blocks of present bricks those whose 32-voxel octant is absent."""
+        T = np.ascontiguousarray(np.asarray(T_WS).reshape(16), dtype=self.dtype)
+        pts = np.zeros((pin.height, pin.width, 3), self.dtype)
+        nrm = np.zeros_like(pts)
+        st = np.zeros((pin.height, pin.width, 7), np.int32)
+        self.L.or_raycast_stats(self.h, C.byref(pin), _rp(T, self.precision), _rp(pts, self.precision),
+                                _rp(nrm, self.precision), st.ctypes.data_as(C.POINTER(C.c_int)))
+        return st
 
     def raycast(self, pin, T_WS):
         T = np.ascontiguousarray(np.asarray(T_WS).reshape(16), dtype=self.dtype)

@@ -785,6 +785,13 @@ typedef struct {
     long touch_blocks;
 } or_map;
 
+/* Per-ray march statistics of or_raycast_stats (test infrastructure): the ray being cast on this
+ * thread accumulates into 7 counters: [0] octree traversal iterations of the first-block search,
+ * [1] samples in a brick (scale-6 octant) that does not exist, [2] samples in an unallocated block
+ * of an existing brick, [3] samples in an allocated block at an unobserved voxel, [4] samples at an
+ * observed voxel, [5] interpolations, [6] the samples of [2] whose 32-voxel octant does not exist. */
+static __thread int* or_cur_stats;
+
 /* records a read of voxel i of block b (no-op unless recording) */
 static inline void touch_voxel(const or_map* m, const or_octant* b, int i)
 {
@@ -1930,6 +1937,8 @@ static void esvo_first_block(const or_map* m, const float o_W[3], const float d_
     float tcorner[3];
     float tc_max = 0.f;
     while (scale < OR_STACK) {
+        if (or_cur_stats)
+            or_cur_stats[0]++;
         for (int i = 0; i < 3; i++)
             tcorner[i] = pos[i] * tc[i] - tb[i];
         tc_max = fminf(fminf(tcorner[0], tcorner[1]), tcorner[2]);
@@ -2035,7 +2044,39 @@ static int raycast_ray_bracket(const or_map* m, const real o[3], const real d[3]
         return 0;
     for (; t < t_far; t += stepsize) {
         real tsdf, w;
-        or_get_data((void*) m, p, &tsdf, &w);
+        const int allocated = or_get_data((void*) m, p, &tsdf, &w);
+        if (or_cur_stats) {
+            int v[3];
+            int cat = 4;
+            if (!allocated) {
+                cat = 2;
+                if (map_point_to_voxel(m, p, v)) {
+                    /* descend to the brick octant (scale 6, or the root for smaller maps) */
+                    const or_octant* o = m->root;
+                    const int bsize = m->size < 64 ? m->size : 64;
+                    while (o && o->size > bsize) {
+                        const int half = o->size / 2;
+                        const int ci = ((v[0] & half) ? 1 : 0) | ((v[1] & half) ? 2 : 0) | ((v[2] & half) ? 4 : 0);
+                        o = o->child[ci];
+                    }
+                    if (!o)
+                        cat = 1;
+                    else {
+                        /* present brick: is the 32-voxel octant (scale 5) absent too? */
+                        while (o && o->size > 32) {
+                            const int half = o->size / 2;
+                            const int ci = ((v[0] & half) ? 1 : 0) | ((v[1] & half) ? 2 : 0) | ((v[2] & half) ? 4 : 0);
+                            o = o->child[ci];
+                        }
+                        if (!o)
+                            or_cur_stats[6]++;
+                    }
+                }
+            }
+            else if (w == (real) 0)
+                cat = 3;
+            or_cur_stats[cat]++;
+        }
         if (w == (real) 0) {
             stepsize = largestep;
             for (int i = 0; i < 3; i++)
@@ -2045,6 +2086,8 @@ static int raycast_ray_bracket(const or_map* m, const real o[3], const real d[3]
         f_tt = tsdf;
         if (f_tt <= (real) 0.1 && f_tt >= (real) -0.5) {
             real fi;
+            if (or_cur_stats)
+                or_cur_stats[5]++;
             if (or_get_field_interp((void*) m, p, &fi))
                 f_tt = fi;
         }
@@ -2069,8 +2112,24 @@ static int raycast_ray_bracket(const or_map* m, const real o[3], const real d[3]
     return 0;
 }
 
+static void raycast_impl(void* vm, const or_pinhole* pin, const real T_WS16[16], real* points, real* normals,
+                         signed char* scale, int* stats);
+
 OR_API void or_raycast(void* vm, const or_pinhole* pin, const real T_WS16[16], real* points, real* normals,
                        signed char* scale)
+{
+    raycast_impl(vm, pin, T_WS16, points, normals, scale, NULL);
+}
+
+/* or_raycast with the per-ray march statistics above (7 ints per pixel) */
+OR_API void or_raycast_stats(void* vm, const or_pinhole* pin, const real T_WS16[16], real* points, real* normals,
+                             int* stats)
+{
+    raycast_impl(vm, pin, T_WS16, points, normals, NULL, stats);
+}
+
+static void raycast_impl(void* vm, const or_pinhole* pin, const real T_WS16[16], real* points, real* normals,
+                         signed char* scale, int* stats)
 {
     const or_map* m = (const or_map*) vm;
     or_cam cam;
@@ -2086,7 +2145,10 @@ OR_API void or_raycast(void* vm, const or_pinhole* pin, const real T_WS16[16], r
             normalized3(dS, dn);
             lin_apply(&T_WS, dn, dW);
             real hit[3];
-            if (raycast_ray(m, T_WS.t, dW, cam_near_dist(&cam, dS), cam_far_dist(&cam, dS), hit)) {
+            or_cur_stats = stats ? stats + 7 * idx : NULL;
+            const int hit_ok = raycast_ray(m, T_WS.t, dW, cam_near_dist(&cam, dS), cam_far_dist(&cam, dS), hit);
+            or_cur_stats = NULL;
+            if (hit_ok) {
                 if (scale)
                     scale[idx] = 0;
                 memcpy(points + 3 * idx, hit, sizeof(real) * 3);

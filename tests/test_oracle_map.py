@@ -358,3 +358,18 @@ def test_raycast_touched_one_block_kat(tsdf):
     print("entry x", x_entry, "voxels read", np.nonzero(got)[0])
     assert B == 1 and V == len(xs)
     np.testing.assert_array_equal(got, want)
+
+
+def test_raycast_stats_count_the_march():
+    """The per-ray march statistics (used to analyse the raycast's latency chain) add up: a hit
+    ray's samples are the march's samples, the in-octant counters are a split of their parent
+    counters, and the one-block case of the read-set test takes exactly its 4 observed samples."""
+    m = O.OracleMap(32.0, 1.0)
+    m.allocate_blocks([[8, 8, 8]])
+    assert m.set_block_data([8, 8, 8], np.full(512, 0.25), np.ones(512)) == 0
+    T = np.array([[0, 0, 1, -20.0], [1, 0, 0, -4.3], [0, 1, 0, -4.7], [0, 0, 0, 1]], np.float32)
+    one = O.pinhole(1, 1, 1.0, 1.0, 0.0, 0.0, 0.4, 60.0)
+    st = m.raycast_stats(one, T)[0, 0]
+    # 4 observed samples in the block, then large steps through unallocated space out of the map
+    assert st[4] == 4 and st[3] == 0 and st[5] == 0
+    assert st[0] > 0 and st[6] <= st[2]
