@@ -237,6 +237,12 @@ int se8_allocate_blocks(se8_map* map, const int* voxel_coords, int64_t n, int on
 /* Write/read the 512 voxels (x fastest, BlockSingleRes::data, block_impl.hpp:45-65) of one block. */
 int se8_write_block(se8_map* map, const int block_coord[3], const float* tsdf512, const float* weight512);
 int se8_read_block(const se8_map* map, const int block_coord[3], float* tsdf512, float* weight512);
+/* The blocks the last integration updated (raycast_carver_impl.hpp:55-128: the frustum blocks and
+ * the blocks it allocated), the octants integrateDepth reports in updated_octants
+ * (map_integrator.hpp:78-81, impl/map_integrator_impl.hpp:77-80), as keys (code<<5|3) in no
+ * particular order.  keys NULL only counts; cap < the count returns SE8_ERR_CAPACITY with *out_n
+ * set.  Waits for the device. */
+int se8_read_update_list(const se8_map* map, uint64_t* keys, int64_t cap, int64_t* out_n);
 /* All allocated blocks sorted by Morton key (key_util_impl.hpp:74-101 layout code<<5|3), with
  * their timestamps; tsdf/weight (512 per block) may be NULL.  cap = capacity of the arrays. */
 int se8_dump_blocks(const se8_map* map, uint64_t* keys, int* timestamps, float* tsdf, float* weight, int64_t cap,

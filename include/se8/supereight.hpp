@@ -3,7 +3,9 @@
 //
 // Reference call sites (app/src/main.cpp:128, 137-147, 154-160) compile against these names:
 //   se::TSDFMap<se::Res::Single>          include/se/map/map.hpp:82-95, 621-622
+//   se::TSDFMap<se::Res::Multi>           include/se/map/map.hpp:623 (integration and structure)
 //   se::MapIntegrator<MapT>::integrateDepth include/se/integrator/map_integrator.hpp:66-81
+//   se::OctantBase (updated_octants)       include/se/map/octant/octant.hpp:19-120
 //   se::raycaster::raycast_volume          include/se/map/raycaster.hpp:56-64
 //   se::Tracker<MapT, SensorT>::track      include/se/tracker/tracker.hpp:42-73
 //   se::PinholeCamera, se::Image<T>        include/se/sensor/pinhole_camera.hpp:74-78,
@@ -16,22 +18,35 @@
 // Images live in host memory as in the reference; each call stages them through device buffers
 // owned by the object (or a per-thread scratch for raycast_volume).  Value types below provide the
 // subset of Eigen's interface the call sites use (x(), y(), z(), translation(), matrix data).
-// Device-side failures throw std::runtime_error carrying se8_last_error().
+// Device-side failures throw std::runtime_error carrying se8_last_error().  Calls that take a HIP
+// stream take it as se::Stream (default: the null stream), a type no other argument converts to.
 #pragma once
 
 #include <cmath>
 #include <cstddef>
 #include <cstdint>
 #include <cstring>
+#include <memory>
 #include <optional>
+#include <set>
 #include <stdexcept>
 #include <utility>
 #include <string>
+#include <type_traits>
+#include <unordered_map>
 #include <vector>
 
 #include "../se8.h"
 
 namespace se {
+
+// A HIP stream (hipStream_t) for the calls that accept one; explicit so that no pointer argument
+// converts to it by accident.
+struct Stream {
+    se8_stream s = nullptr;
+    Stream() = default;
+    explicit Stream(se8_stream st) : s(st) {}
+};
 
 namespace detail {
 inline void check(int rc, const char* what)
@@ -179,6 +194,40 @@ inline int save_mesh(const std::vector<FaceT>& mesh, const std::string& filename
 
 enum class Res { Single, Multi };
 
+// se::OctantBase (octant.hpp:19-120) as integrateDepth reports it in updated_octants: an
+// allocated block's coordinates in voxels (its vertex closest to the origin), its key (code << 5 |
+// scale, key_util_impl.hpp:74-101) and its timestamp when reported.  The map owns one handle per
+// block for its lifetime, so a pointer names the same block across integrations, as the
+// reference's octant pointers do.
+class OctantBase {
+    public:
+    explicit OctantBase(uint64_t key) : coord(decode(key, 0), decode(key, 1), decode(key, 2)), key_(key) {}
+    OctantBase(const OctantBase&) = delete;
+    OctantBase& operator=(const OctantBase&) = delete;
+
+    const Vector3i coord;
+    int timestamp = -1;
+    const bool is_block = true;
+
+    uint64_t getKey() const { return key_; }
+    int getScale() const { return (int) (key_ & 31u); }
+    int getSize() const { return 1 << getScale(); }
+
+    private:
+    static int decode(uint64_t key, int axis)
+    {
+        const uint64_t code = key >> 5;
+        int v = 0;
+        for (int b = 0; b < 21; b++)
+            v |= (int) ((code >> (3 * b + axis)) & 1u) << b;
+        return v;
+    }
+    uint64_t key_;
+};
+
+template<typename MapT>
+class MapIntegrator;
+
 // se::FieldData<Field::TSDF>::Config plus the storage precision and pool capacity
 struct TSDFDataConfig {
     float truncation_boundary_factor = 8.f;
@@ -263,11 +312,17 @@ inline int integrate(se8_map* m, int ts, const float* dev, const OusterLidar& s,
 }
 } // namespace detail
 
+// TSDFMap<Res::Single> and TSDFMap<Res::Multi> (map.hpp:621-623).  The multi-resolution map
+// (TSDFDataConfig::storage SE8_STORAGE_MULTIRES, set by the constructor) is integrated with a
+// pinhole camera and read through structure(), saveStructure() and se8_dump_blocks_multires on
+// handle(); the members that read the field (getData, getFieldInterp, getFieldGrad, mesh,
+// saveFieldSlices) and the raycaster and tracker are single-resolution only and do not compile
+// for it.
 template<Res ResT = Res::Single>
 class TSDFMap {
-    static_assert(ResT == Res::Single, "only the single-resolution TSDF map is provided");
-
     public:
+    static constexpr Res res = ResT;
+
     struct Config {
         Vector3f dim = Vector3f::Constant(10.f);
         float res = 0.1f;
@@ -281,9 +336,17 @@ class TSDFMap {
 
     TSDFMap(const Config& cfg, const TSDFDataConfig& data_cfg = TSDFDataConfig()) : res_(cfg.res)
     {
+        int storage = data_cfg.storage;
+        if (ResT == Res::Multi) {
+            if (storage != SE8_STORAGE_F32 && storage != SE8_STORAGE_MULTIRES)
+                throw std::invalid_argument("TSDFMap<Res::Multi>: the multi-resolution map stores FP32 data");
+            storage = SE8_STORAGE_MULTIRES;
+        }
+        else if (storage == SE8_STORAGE_MULTIRES) {
+            throw std::invalid_argument("TSDFMap<Res::Single>: use TSDFMap<Res::Multi> for a multi-resolution map");
+        }
         se8_map_config m{{cfg.dim.x(), cfg.dim.y(), cfg.dim.z()}, cfg.res, {cfg.t_MW.x(), cfg.t_MW.y(), cfg.t_MW.z()}};
-        se8_tsdf_config t{data_cfg.truncation_boundary_factor, data_cfg.max_weight, data_cfg.storage,
-                          data_cfg.max_blocks};
+        se8_tsdf_config t{data_cfg.truncation_boundary_factor, data_cfg.max_weight, storage, data_cfg.max_blocks};
         detail::check(se8_map_create(&m, &t, &h_), "se8_map_create");
         dim_ = cfg.dim;
     }
@@ -307,6 +370,7 @@ class TSDFMap {
     // getData (map_impl.hpp:68-86): {tsdf, weight}
     std::pair<float, float> getData(const Vector3f& p) const
     {
+        static_assert(ResT == Res::Single, "getData: single-resolution maps only");
         float t = 1.f, w = 0.f;
         detail::check(se8_get_data(h_, p.data(), 1, &t, &w), "se8_get_data");
         return {t, w};
@@ -314,6 +378,7 @@ class TSDFMap {
     // getFieldInterp (map_impl.hpp:154-172)
     std::optional<float> getFieldInterp(const Vector3f& p) const
     {
+        static_assert(ResT == Res::Single, "getFieldInterp: single-resolution maps only");
         float v = 0.f;
         uint8_t ok = 0;
         detail::check(se8_field_interp(h_, p.data(), 1, &v, &ok), "se8_field_interp");
@@ -322,6 +387,7 @@ class TSDFMap {
     // getFieldGrad (map_impl.hpp:239-268)
     std::optional<Vector3f> getFieldGrad(const Vector3f& p) const
     {
+        static_assert(ResT == Res::Single, "getFieldGrad: single-resolution maps only");
         Vector3f g;
         uint8_t ok = 0;
         detail::check(se8_field_grad(h_, p.data(), 1, g.v, &ok), "se8_field_grad");
@@ -334,6 +400,7 @@ class TSDFMap {
     // meshed on the device and copied back.
     SurfaceMesh mesh(const Isometry3f& T_OW = Isometry3f::Identity(), const int min_desired_scale = 0) const
     {
+        static_assert(ResT == Res::Single, "mesh: single-resolution maps only");
         (void) min_desired_scale;
         int64_t n = 0;
         detail::check(se8_mesh(h_, T_OW.data(), nullptr, 0, &n, nullptr), "se8_mesh");
@@ -402,6 +469,7 @@ class TSDFMap {
     int saveFieldSlices(const std::string& filename_x, const std::string& filename_y, const std::string& filename_z,
                         const Vector3f& point_W) const
     {
+        static_assert(ResT == Res::Single, "saveFieldSlices: single-resolution maps only");
         detail::check(se8_save_field_slices(h_, filename_x.c_str(), filename_y.c_str(), filename_z.c_str(),
                                             point_W.data()),
                       "se8_save_field_slices");
@@ -411,10 +479,32 @@ class TSDFMap {
     se8_map* handle() const { return h_; }
 
     private:
+    template<typename MapT>
+    friend class MapIntegrator;
+
+    // Inserts the handles of the blocks the last integration updated (se8_read_update_list) into
+    // out, stamped with the integration's timestamp.
+    void reportUpdated(int timestamp, std::set<const OctantBase*>& out)
+    {
+        int64_t n = 0;
+        detail::check(se8_read_update_list(h_, nullptr, 0, &n), "se8_read_update_list");
+        std::vector<uint64_t> keys((size_t) n);
+        if (n > 0)
+            detail::check(se8_read_update_list(h_, keys.data(), n, &n), "se8_read_update_list");
+        for (const uint64_t k : keys) {
+            std::unique_ptr<OctantBase>& h = handles_[k];
+            if (!h)
+                h = std::make_unique<OctantBase>(k);
+            h->timestamp = timestamp;
+            out.insert(h.get());
+        }
+    }
+
     se8_map* h_ = nullptr;
     float res_;
     Vector3f dim_;
     mutable detail::DeviceBuffer mesh_buf_;
+    std::unordered_map<uint64_t, std::unique_ptr<OctantBase>> handles_;
 };
 
 namespace preprocessor {
@@ -538,18 +628,33 @@ class MapIntegrator {
     public:
     explicit MapIntegrator(MapT& map) : map_(map) {}
 
-    // integrateDepth(timestamp, measurements) (map_integrator.hpp:78-81)
+    // integrateDepth(timestamp, measurements, updated_octants) (map_integrator.hpp:78-81): when
+    // updated_octants is non-null the handles of the blocks this integration updated (the frustum
+    // blocks and the blocks it allocated, map_integrator_impl.hpp:77-80) are inserted into it.
     template<typename SensorT>
-    void integrateDepth(int timestamp, const Measurements<SensorT>& m, se8_stream stream = nullptr)
+    void integrateDepth(int timestamp, const Measurements<SensorT>& m,
+                        std::set<const OctantBase*>* const updated_octants = nullptr)
     {
+        integrateDepth(timestamp, m, Stream(), updated_octants);
+    }
+
+    // The same on a HIP stream (returns once the integration has finished on it).
+    template<typename SensorT>
+    void integrateDepth(int timestamp, const Measurements<SensorT>& m, Stream stream,
+                        std::set<const OctantBase*>* const updated_octants = nullptr)
+    {
+        static_assert(MapT::res == Res::Single || std::is_same<SensorT, PinholeCamera>::value,
+                      "the multi-resolution map is integrated with a pinhole camera");
         const Image<float>& d = m.depth.image;
         const size_t bytes = sizeof(float) * d.size();
         void* dev = depth_.get(bytes);
-        detail::check(se8_copy(dev, d.data(), bytes, SE8_COPY_H2D, stream), "se8_copy");
+        detail::check(se8_copy(dev, d.data(), bytes, SE8_COPY_H2D, stream.s), "se8_copy");
         detail::check(detail::integrate(map_.handle(), timestamp, (const float*) dev, m.depth.sensor,
-                                        m.depth.T_WC.data(), stream),
+                                        m.depth.T_WC.data(), stream.s),
                       "integrateDepth");
-        detail::check(se8_stream_synchronize(stream), "se8_stream_synchronize");
+        detail::check(se8_stream_synchronize(stream.s), "se8_stream_synchronize");
+        if (updated_octants)
+            map_.reportUpdated(timestamp, *updated_octants);
     }
 
     private:
@@ -561,20 +666,21 @@ namespace raycaster {
 // raycast_volume(map, sensor, T_WS, points, normals, scale) (raycaster.hpp:56-64)
 template<typename MapT, typename SensorT>
 void raycast_volume(const MapT& map, const SensorT& sensor, const Isometry3f& T_WS, Image<Vector3f>& points_W,
-                    Image<Vector3f>& normals_W, Image<int8_t>& scale, se8_stream stream = nullptr)
+                    Image<Vector3f>& normals_W, Image<int8_t>& scale, Stream stream = Stream())
 {
+    static_assert(MapT::res == Res::Single, "raycast_volume: single-resolution maps only");
     thread_local detail::DeviceBuffer pts, nrm, scl;
     const size_t n = points_W.size();
     void* dp = pts.get(sizeof(float) * 3 * n);
     void* dn = nrm.get(sizeof(float) * 3 * n);
     void* ds = scl.get(n);
-    detail::check(se8_copy(ds, scale.data(), n, SE8_COPY_H2D, stream), "se8_copy");
-    detail::check(se8_raycast(map.handle(), &sensor.c(), T_WS.data(), (float*) dp, (float*) dn, (int8_t*) ds, stream),
+    detail::check(se8_copy(ds, scale.data(), n, SE8_COPY_H2D, stream.s), "se8_copy");
+    detail::check(se8_raycast(map.handle(), &sensor.c(), T_WS.data(), (float*) dp, (float*) dn, (int8_t*) ds, stream.s),
                   "se8_raycast");
-    detail::check(se8_copy(points_W.data(), dp, sizeof(float) * 3 * n, SE8_COPY_D2H, stream), "se8_copy");
-    detail::check(se8_copy(normals_W.data(), dn, sizeof(float) * 3 * n, SE8_COPY_D2H, stream), "se8_copy");
-    detail::check(se8_copy(scale.data(), ds, n, SE8_COPY_D2H, stream), "se8_copy");
-    detail::check(se8_stream_synchronize(stream), "se8_stream_synchronize");
+    detail::check(se8_copy(points_W.data(), dp, sizeof(float) * 3 * n, SE8_COPY_D2H, stream.s), "se8_copy");
+    detail::check(se8_copy(normals_W.data(), dn, sizeof(float) * 3 * n, SE8_COPY_D2H, stream.s), "se8_copy");
+    detail::check(se8_copy(scale.data(), ds, n, SE8_COPY_D2H, stream.s), "se8_copy");
+    detail::check(se8_stream_synchronize(stream.s), "se8_stream_synchronize");
 }
 } // namespace raycaster
 
@@ -591,6 +697,8 @@ struct TrackerConfig {
 template<typename MapT, typename SensorT>
 class Tracker {
     public:
+    static_assert(MapT::res == Res::Single, "se::Tracker: single-resolution maps only");
+
     Tracker(MapT& map, const SensorT& sensor, const TrackerConfig config = TrackerConfig())
         : map_(map), sensor_(sensor)
     {
@@ -614,38 +722,38 @@ class Tracker {
     ~Tracker() { se8_tracker_destroy(h_); }
 
     // track(depth, T_WS): raycasts the map at T_WS first (tracker_impl.hpp:17-26)
-    bool track(const Image<float>& depth, Isometry3f& T_WS, se8_stream stream = nullptr)
+    bool track(const Image<float>& depth, Isometry3f& T_WS, Stream stream = Stream())
     {
         void* dd = stage_depth(depth, stream);
         int ok = 0;
-        detail::check(se8_track_raycast(h_, map_.handle(), (const float*) dd, T_WS.data(), &ok, stream),
+        detail::check(se8_track_raycast(h_, map_.handle(), (const float*) dd, T_WS.data(), &ok, stream.s),
                       "se8_track_raycast");
         return ok != 0;
     }
 
     // track(depth, T_WS, points_W, normals_W) (tracker_impl.hpp:30-116)
     bool track(const Image<float>& depth, Isometry3f& T_WS, Image<Vector3f>& points_W, Image<Vector3f>& normals_W,
-               se8_stream stream = nullptr)
+               Stream stream = Stream())
     {
         void* dd = stage_depth(depth, stream);
         const size_t n = points_W.size();
         void* dp = pts_.get(sizeof(float) * 3 * n);
         void* dn = nrm_.get(sizeof(float) * 3 * n);
-        detail::check(se8_copy(dp, points_W.data(), sizeof(float) * 3 * n, SE8_COPY_H2D, stream), "se8_copy");
-        detail::check(se8_copy(dn, normals_W.data(), sizeof(float) * 3 * n, SE8_COPY_H2D, stream), "se8_copy");
+        detail::check(se8_copy(dp, points_W.data(), sizeof(float) * 3 * n, SE8_COPY_H2D, stream.s), "se8_copy");
+        detail::check(se8_copy(dn, normals_W.data(), sizeof(float) * 3 * n, SE8_COPY_H2D, stream.s), "se8_copy");
         int ok = 0;
         detail::check(se8_track(h_, (const float*) dd, T_WS.data(), (const float*) dp, (const float*) dn, &ok,
-                                nullptr, stream),
+                                nullptr, stream.s),
                       "se8_track");
         return ok != 0;
     }
 
     private:
-    void* stage_depth(const Image<float>& depth, se8_stream stream)
+    void* stage_depth(const Image<float>& depth, Stream stream)
     {
         const size_t bytes = sizeof(float) * depth.size();
         void* dd = depth_.get(bytes);
-        detail::check(se8_copy(dd, depth.data(), bytes, SE8_COPY_H2D, stream), "se8_copy");
+        detail::check(se8_copy(dd, depth.data(), bytes, SE8_COPY_H2D, stream.s), "se8_copy");
         return dd;
     }
 

@@ -112,6 +112,8 @@ def lib(precision: str = "f32") -> C.CDLL:
     L.or_map_num_nodes.restype = C.c_long
     L.or_map_num_nodes.argtypes = [P]
     L.or_map_dump_blocks.argtypes = [P, C.POINTER(U64), I, R, R]
+    L.or_map_last_update.restype = C.c_long
+    L.or_map_last_update.argtypes = [P, C.POINTER(U64)]
     L.or_map_dump_nodes.restype = C.c_long
     L.or_map_dump_nodes.argtypes = [P, C.POINTER(U64), C.POINTER(C.c_uint8), I]
     L.or_map_aabb.restype = C.c_int
@@ -367,6 +369,14 @@ class OracleMap:
         if with_data:
             return keys[:n], ts[:n], tsdf[:n], w[:n]
         return keys[:n], ts[:n], None, None
+
+    def update_list(self):
+        """Keys of the blocks the last integration updated (allocated + frustum: integrateDepth's
+        updated_octants, map_integrator_impl.hpp:77-80), sorted."""
+        n = self.L.or_map_last_update(self.h, C.POINTER(C.c_uint64)())
+        keys = np.zeros(max(n, 1), np.uint64)
+        self.L.or_map_last_update(self.h, _u64p(keys))
+        return np.sort(keys[:n])
 
     def dump_multires(self):
         """(keys, scales [N, 2] (current, minimum), data [N, 585, 2], past data [N, 585, 2]) sorted by key."""

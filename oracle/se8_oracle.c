@@ -783,6 +783,9 @@ typedef struct {
      * of the voxel's data through getData, the interpolation corners and the gradient samples */
     uint64_t* touch;
     long touch_blocks;
+    /* keys of the blocks the last integration updated (allocated + frustum), in update order */
+    uint64_t* last_update;
+    long n_last_update;
 } or_map;
 
 /* Per-ray march statistics of or_raycast_stats (test infrastructure): the ray being cast on this
@@ -872,6 +875,7 @@ OR_API void or_map_destroy(void* vm)
     free_octant(m->root);
     free(m->blocks);
     free(m->touch);
+    free(m->last_update);
     free(m);
 }
 
@@ -1084,6 +1088,15 @@ static int cmp_octant_key(const void* a, const void* b)
     const uint64_t kx = or_encode_code(x->coord[0], x->coord[1], x->coord[2]);
     const uint64_t ky = or_encode_code(y->coord[0], y->coord[1], y->coord[2]);
     return kx < ky ? -1 : (kx > ky ? 1 : 0);
+}
+
+/* Keys of the blocks the last integration updated; keys NULL only counts. */
+OR_API long or_map_last_update(void* vm, uint64_t* keys)
+{
+    or_map* m = (or_map*) vm;
+    if (keys && m->n_last_update > 0)
+        memcpy(keys, m->last_update, sizeof(uint64_t) * (size_t) m->n_last_update);
+    return m->n_last_update;
 }
 
 /* Blocks sorted by Morton key; tsdf/weight as 512-arrays per block (x fastest). */
@@ -1647,6 +1660,12 @@ static void integrate_sensor(or_map* m, int ts, const float* depth, const or_sen
         for (long i = 0; i < nb; i++)
             update_block(m, list[i], &T_CV, sen, depth, ts);
     }
+    /* the updated_octants integrateDepth reports (map_integrator_impl.hpp:77-80) */
+    free(m->last_update);
+    m->last_update = (uint64_t*) malloc(sizeof(uint64_t) * (nb > 0 ? nb : 1));
+    m->n_last_update = nb;
+    for (long i = 0; i < nb; i++)
+        or_encode_key(list[i]->coord, OR_BLOCK_SCALE, &m->last_update[i]);
     /* propagateTimeStampToRoot: only the blocks' direct parents are reached */
     for (long i = 0; i < nb; i++) {
         or_octant* p = list[i]->parent;

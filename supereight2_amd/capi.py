@@ -169,6 +169,7 @@ SIGNATURES = {
     "se8_allocate_blocks": (C.c_int, [_P, _I, C.c_int64, C.c_int, _I, C.POINTER(C.c_int64)]),
     "se8_write_block": (C.c_int, [_P, _I, _F, _F]),
     "se8_read_block": (C.c_int, [_P, _I, _F, _F]),
+    "se8_read_update_list": (C.c_int, [_P, C.POINTER(C.c_uint64), C.c_int64, C.POINTER(C.c_int64)]),
     "se8_dump_blocks": (C.c_int, [_P, C.POINTER(C.c_uint64), _I, _F, _F, C.c_int64, C.POINTER(C.c_int64)]),
     "se8_dump_blocks_multires": (C.c_int, [_P, C.POINTER(C.c_uint64), _I, _F, _F, C.c_int64, C.POINTER(C.c_int64)]),
     "se8_dump_nodes": (C.c_int, [_P, C.POINTER(C.c_uint64), C.POINTER(C.c_uint8), _I, C.c_int64, C.POINTER(C.c_int64)]),

@@ -1831,6 +1831,37 @@ extern "C" int se8_read_block(const se8_map* m, const int bc[3], float* tsdf512,
     return SE8_OK;
 }
 
+extern "C" int se8_read_update_list(const se8_map* m, uint64_t* keys, int64_t cap, int64_t* out_n)
+{
+    SE8_REQUIRE(m && out_n, "se8_read_update_list: null argument");
+    int host[kNumCounters];
+    int rc = read_counters(m, host, nullptr);
+    if (rc != SE8_OK)
+        return rc;
+    const int64_t n = std::min<int64_t>(host[kCntUpdate], m->view.max_blocks);
+    const int64_t nb = std::min<int64_t>(host[kCntBlocks], m->view.max_blocks);
+    *out_n = n;
+    if (!keys)
+        return SE8_OK;
+    if (cap < n) {
+        set_error("se8_read_update_list: capacity below the list length");
+        return SE8_ERR_CAPACITY;
+    }
+    std::vector<int> entries((size_t) n);
+    std::vector<uint64_t> block_keys((size_t) nb);
+    if (n > 0 && (hipMemcpy(entries.data(), m->update_list.ptr, sizeof(int) * n, hipMemcpyDeviceToHost) != hipSuccess
+                  || hipMemcpy(block_keys.data(), m->block_key.ptr, sizeof(uint64_t) * nb, hipMemcpyDeviceToHost)
+                         != hipSuccess)) {
+        set_error("se8_read_update_list: copy failed");
+        return SE8_ERR_HIP;
+    }
+    for (int64_t i = 0; i < n; i++) {
+        const int slot = entries[i] & 0x7fffffff;
+        keys[i] = slot < nb ? block_keys[slot] : 0;
+    }
+    return SE8_OK;
+}
+
 extern "C" int se8_dump_blocks(const se8_map* m, uint64_t* keys, int* timestamps, float* tsdf, float* weight,
                                int64_t cap, int64_t* out_n)
 {

@@ -191,6 +191,15 @@ class TSDFMap:
         capi.check(self.lib.se8_read_block(self.h, capi.iptr(bc), capi.fptr(t), capi.fptr(w)), "se8_read_block")
         return t, w
 
+    def update_list(self):
+        """Keys of the blocks the last integration updated (integrateDepth's updated_octants), sorted."""
+        n = C.c_int64()
+        capi.check(self.lib.se8_read_update_list(self.h, None, 0, C.byref(n)), "se8_read_update_list")
+        keys = np.zeros(max(n.value, 1), np.uint64)
+        capi.check(self.lib.se8_read_update_list(self.h, keys.ctypes.data_as(C.POINTER(C.c_uint64)), len(keys),
+                                                 C.byref(n)), "se8_read_update_list")
+        return np.sort(keys[: n.value])
+
     def dump_blocks(self, with_data=True):
         n = C.c_int64()
         capi.check(self.lib.se8_dump_blocks(self.h, None, None, None, None, 0, C.byref(n)), "se8_dump_blocks")

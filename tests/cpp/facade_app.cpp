@@ -3,11 +3,13 @@
 // depth of a room corner (three planes, so point-to-plane ICP is fully constrained) seen by a
 // camera that slides 1 cm per frame along x.
 // Prints one line per frame: frame, tracking ok, estimated x, ground-truth x, centre hit depth;
-// then integrates two scans of a spinning lidar through se::OusterLidar and prints the field at a
-// ground point and a wall point.
+// the blocks integrateDepth reported as updated at frames 4 and 5 (the second on se::Stream);
+// then integrates the first frame into a TSDFMap<Res::Multi>, and two scans of a spinning lidar
+// through se::OusterLidar and prints the field at a ground point and a wall point.
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <set>
 
 #include "se8/supereight.hpp"
 
@@ -95,7 +97,9 @@ int main(int argc, char** argv)
     const double R[3][3] = {{xr.x, yd.x, f.x}, {xr.y, yd.y, f.y}, {xr.z, yd.z, f.z}};
     const double lo[3] = {-3.0, -2.0, 0.0}, hi[3] = {3.0, 2.0, 3.0};
 
-    se::Isometry3f T_WS = se::Isometry3f::Identity();
+    se::Isometry3f T_WS = se::Isometry3f::Identity(), T_0;
+    se::Image<float> depth_0(640, 480);
+    std::set<const se::OctantBase*> updated_4, updated_5;
     int failures = 0;
     double worst = 0.0;
     for (int frame = 0; frame < 6; frame++) {
@@ -124,7 +128,17 @@ int main(int argc, char** argv)
                 T_WS(i, 3) = (float) p[i];
             }
         }
-        integrator.integrateDepth(frame, se::Measurements{se::Measurement{depth, sensor, T_WS}});
+        if (frame == 0) {
+            depth_0 = depth;
+            T_0 = T_WS;
+        }
+        if (frame == 4)
+            integrator.integrateDepth(frame, se::Measurements{se::Measurement{depth, sensor, T_WS}}, &updated_4);
+        else if (frame == 5)
+            integrator.integrateDepth(frame, se::Measurements{se::Measurement{depth, sensor, T_WS}}, se::Stream(),
+                                      &updated_5);
+        else
+            integrator.integrateDepth(frame, se::Measurements{se::Measurement{depth, sensor, T_WS}});
         se::raycaster::raycast_volume(map, sensor, T_WS, points, normals, scale);
         const se::Vector3f c = points(320, 240);
         const double dz = std::sqrt((c.x() - p[0]) * (c.x() - p[0]) + (c.y() - p[1]) * (c.y() - p[1])
@@ -136,6 +150,25 @@ int main(int argc, char** argv)
         if (!ok)
             failures++;
     }
+    // updated_octants: blocks (size 8, coordinates on the block grid) stamped with the frame; a block
+    // reported at both frames is the same handle
+    int bad = 0;
+    size_t shared = 0;
+    for (const se::OctantBase* o : updated_5) {
+        bad += !o->is_block || o->getSize() != 8 || o->timestamp != 5 || o->coord.x() % 8 || o->coord.y() % 8
+               || o->coord.z() % 8;
+        shared += updated_4.count(o);
+    }
+    std::printf("updated frame4 %zu frame5 %zu shared %zu bad %d\n", updated_4.size(), updated_5.size(), shared, bad);
+
+    // the multi-resolution map: frame 0 integrated, every block new and updated
+    se::TSDFMap<se::Res::Multi> mmap(se::Vector3f::Constant(10.24f), 0.02f);
+    se::MapIntegrator mint(mmap);
+    std::set<const se::OctantBase*> updated_multi;
+    mint.integrateDepth(0, se::Measurements{se::Measurement{depth_0, sensor, T_0}}, &updated_multi);
+    std::printf("multires blocks %lld updated %zu quads %zu\n", (long long) mmap.info().num_blocks,
+                updated_multi.size(), mmap.structure().size());
+
     const auto f_interp = map.getFieldInterp(se::Vector3f(points(320, 240).x(), points(320, 240).y(),
                                                           points(320, 240).z()));
     std::printf("interp %d %.5f blocks %lld worst_x_err %.6f\n", f_interp ? 1 : 0, f_interp ? *f_interp : 0.f,

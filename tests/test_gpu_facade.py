@@ -27,6 +27,13 @@ def test_facade_app_tracks_and_maps(tmp_path):
         assert abs(float(x) - float(gt)) < 2e-3
         assert abs(float(hd) - float(dd)) < 1e-2  # raycast hit at the measured surface
         assert sc == "0"
+    # updated_octants (map_integrator.hpp:78-81): handles persist, so blocks in both frames share one
+    m = re.search(r"updated frame4 (\d+) frame5 (\d+) shared (\d+) bad (\d+)", res.stdout)
+    assert m and int(m.group(1)) > 1000 and int(m.group(2)) > 1000 and m.group(4) == "0"
+    assert int(m.group(3)) > 0.5 * int(m.group(2))
+    # TSDFMap<Res::Multi>: the first integration allocates and updates every block
+    m = re.search(r"multires blocks (\d+) updated (\d+) quads (\d+)", res.stdout)
+    assert m and int(m.group(1)) > 100 and m.group(1) == m.group(2) and int(m.group(3)) == 6 * int(m.group(1))
     m = re.search(r"interp (\d) (\S+) blocks (\d+)", res.stdout)
     assert m and m.group(1) == "1" and abs(float(m.group(2))) < 0.2 and int(m.group(3)) > 1000
     # Map::mesh / saveMesh: the room's surfaces, a .ply written in the working directory

@@ -80,6 +80,37 @@ def test_tsdf_voxels_bit_exact(integrated):
     np.testing.assert_array_equal(gt, ot)
 
 
+def test_update_list_matches_oracle(integrated):
+    """integrateDepth's updated_octants (map_integrator_impl.hpp:77-80): the last integration's
+    frustum + new blocks, as a key set, on the dense-table (claim) allocation path."""
+    se, gmap, omap, _, _, stats = integrated
+    g, o = gmap.update_list(), omap.update_list()
+    print("updated blocks gpu/oracle", len(g), len(o), "oracle frustum + new", stats[-1]["frustum"] + stats[-1]["new"])
+    assert len(np.unique(g)) == len(g)
+    np.testing.assert_array_equal(g, o)
+    assert len(o) == stats[-1]["frustum"] + stats[-1]["new"]
+
+
+@pytest.mark.parametrize("dim", [10.24, 81.92])
+def test_update_list_every_frame(dim):
+    """The update list after each of 3 frames, on the claim path (10.24 m: dense block table) and the
+    sort path (81.92 m: brick index)."""
+    se = _se()
+    depth, poses = circle_frames(50, 3)
+    sensor, pin = _sensor(se), _pin()
+    gmap = se.TSDFMap(dim, 0.01)
+    integ = se.MapIntegrator(gmap)
+    omap = O.OracleMap(dim, 0.01)
+    for k in range(len(depth)):
+        print(f"map {dim} m: integrate frame {k}", flush=True)
+        integ.integrateDepth(k, torch.from_numpy(depth[k]).cuda(), sensor, poses[k])
+        st = omap.integrate(k, depth[k], pin, poses[k])
+        g, o = gmap.update_list(), omap.update_list()
+        assert len(o) == st["frustum"] + st["new"]
+        np.testing.assert_array_equal(g, o)
+    gmap.close()
+
+
 def test_aabb_matches(integrated):
     se, gmap, omap, _, _, _ = integrated
     gv, gmin, gmax = gmap.aabb()
