@@ -122,6 +122,8 @@ def lib(precision: str = "f32") -> C.CDLL:
     L.or_allocate_blocks.argtypes = [P, I, C.c_long, C.c_int, I]
     L.or_set_block_data.restype = C.c_int
     L.or_set_block_data.argtypes = [P, I, R, R]
+    L.or_set_multires_block.restype = C.c_int
+    L.or_set_multires_block.argtypes = [P, I, C.c_int, C.c_int, R, R]
     L.or_mesh.restype = C.c_long
     L.or_mesh.argtypes = [P, R, R, C.c_long]
     L.or_field_slice.argtypes = [P, C.c_int, C.c_int, R]
@@ -413,6 +415,15 @@ class OracleMap:
         t = np.ascontiguousarray(tsdf, dtype=self.dtype)
         w = np.ascontiguousarray(weight, dtype=self.dtype)
         return self.L.or_set_block_data(self.h, _ip(bc), _rp(t, self.precision), _rp(w, self.precision))
+
+    def set_multires_block(self, block_coord, cur_scale, min_scale, data, past):
+        """Sets an allocated block's multi-resolution state: data and past data [585, 2] (tsdf,
+        weight) in dump_multires's layout."""
+        bc = np.ascontiguousarray(block_coord, dtype=np.int32)
+        d = np.ascontiguousarray(data, dtype=self.dtype).reshape(585, 2)
+        p = np.ascontiguousarray(past, dtype=self.dtype).reshape(585, 2)
+        return self.L.or_set_multires_block(self.h, _ip(bc), int(cur_scale), int(min_scale), _rp(d, self.precision),
+                                            _rp(p, self.precision))
 
     def load_blocks(self, block_coords, tsdf, weight):
         """Allocates the given blocks and sets their 512 voxels each (e.g. a dump of another map)."""

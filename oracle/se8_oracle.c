@@ -1563,6 +1563,23 @@ static void mr_update_up(const or_map* m, or_octant* b, const or_iso* T_CV, cons
     }
 }
 
+/* Sets an allocated block's multi-resolution state (test infrastructure): current and minimum
+ * scale, 585 x {tsdf, weight} data and past data as or_map_dump_multires returns them. */
+OR_API int or_set_multires_block(void* vm, const int block_coord[3], int cur_scale, int min_scale, const real* data,
+                                 const real* past)
+{
+    or_map* m = (or_map*) vm;
+    or_octant* b = fetch_block(m, block_coord);
+    if (!b || !m->multires)
+        return -1;
+    mr_init(b);
+    b->cur_scale = cur_scale;
+    b->min_scale = min_scale;
+    memcpy(b->mr, data, sizeof(real) * 2 * 585);
+    memcpy(b->mr + 2 * 585, past, sizeof(real) * 2 * 585);
+    return 0;
+}
+
 OR_API void or_map_set_storage_f16(void* vm)
 {
     ((or_map*) vm)->storage_f16 = 1;

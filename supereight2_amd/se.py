@@ -36,6 +36,26 @@ def _dev_f32(t, shape=None):
     return C.c_void_p(t.data_ptr())
 
 
+def _dev_out(t, dtypes, numel, what):
+    """Device pointer of an optional output tensor the kernel writes numel elements of: None passes
+    through; otherwise a contiguous CUDA tensor of one of dtypes with exactly numel elements."""
+    if t is None:
+        return None
+    torch = _torch()
+    if not (isinstance(t, torch.Tensor) and t.is_cuda and t.is_contiguous()):
+        raise ValueError(f"{what} must be a contiguous CUDA tensor")
+    if t.dtype not in dtypes:
+        raise ValueError(f"{what} must have dtype {' or '.join(str(d) for d in dtypes)}, got {t.dtype}")
+    if t.numel() != numel:
+        raise ValueError(f"{what} must have {numel} elements, got {t.numel()}")
+    return C.c_void_p(t.data_ptr())
+
+
+def _index_dtypes():
+    torch = _torch()
+    return (torch.int64,) + ((torch.uint64,) if hasattr(torch, "uint64") else ())
+
+
 def _dev_pose(T):
     """Device pointer of a pose held in device memory (a float32 CUDA tensor of 16 values, row-major
     4x4), or None for a host pose."""
@@ -376,7 +396,7 @@ def downsample_depth(input_depth, output_depth, downsample_map=None, stream=None
     tensor) receives the index map the reference returns."""
     h_in, w_in = input_depth.shape
     h_out, w_out = output_depth.shape
-    mp = None if downsample_map is None else C.c_void_p(downsample_map.data_ptr())
+    mp = _dev_out(downsample_map, _index_dtypes(), h_out * w_out, "downsample_map")
     capi.check(capi.load().se8_downsample_depth(_dev_f32(input_depth), w_in, h_in, _dev_f32(output_depth), w_out,
                                                 h_out, mp, _stream_handle(stream)), "se8_downsample_depth")
     return downsample_map
@@ -390,7 +410,7 @@ def downsample_depth_u16(raw_depth, scale, divide, output_depth, downsample_map=
         raise ValueError("raw_depth must be a contiguous 16-bit CUDA tensor")
     h_in, w_in = raw_depth.shape
     h_out, w_out = output_depth.shape
-    mp = None if downsample_map is None else C.c_void_p(downsample_map.data_ptr())
+    mp = _dev_out(downsample_map, _index_dtypes(), h_out * w_out, "downsample_map")
     capi.check(capi.load().se8_downsample_depth_u16(C.c_void_p(raw_depth.data_ptr()), w_in, h_in, float(scale),
                                                     int(divide), _dev_f32(output_depth), w_out, h_out, mp,
                                                     _stream_handle(stream)), "se8_downsample_depth_u16")
@@ -487,7 +507,7 @@ class MapIntegrator:
 def raycast_volume(map_: TSDFMap, sensor: PinholeCamera, T_WS, points_W, normals_W, scale=None, stream=None):
     """se::raycaster::raycast_volume (raycaster.hpp:56-64) into device tensors [H, W, 3]."""
     shape = (sensor.height, sensor.width, 3)
-    sp = C.c_void_p(scale.data_ptr()) if scale is not None else None
+    sp = _dev_out(scale, (_torch().int8,), sensor.height * sensor.width, "scale")
     Td = _dev_pose(T_WS)
     if Td is not None:
         capi.check(map_.lib.se8_raycast_dev(map_.h, C.byref(sensor.c()), Td, _dev_f32(points_W, shape),
@@ -552,7 +572,7 @@ class Tracker:
         Ti, To = _dev_pose(T_in), _dev_pose(T_out)
         if Ti is None or To is None:
             raise ValueError("track_device takes device poses")
-        okp = C.c_void_p(ok.data_ptr()) if ok is not None else None
+        okp = _dev_out(ok, (_torch().int32,), 1, "ok")
         capi.check(self.lib.se8_track_dev(self.h, _dev_f32(depth, shape), Ti, To, _dev_f32(points_W, shape + (3,)),
                                           _dev_f32(normals_W, shape + (3,)), okp, _stream_handle(stream)),
                    "se8_track_dev")

@@ -179,3 +179,40 @@ def test_odd_image_size_pipeline_matches_oracle():
     np.testing.assert_allclose(T_g, T_o, atol=1e-4)
     tracker.close()
     gmap.close()
+
+
+def test_optional_outputs_are_validated():
+    """The optional output tensors (raycast scale, the downsample index map, track_device's ok) are
+    checked for device, dtype, contiguity and size before any kernel is launched."""
+    from supereight2_amd import se
+
+    sensor = _sensor(se)
+    H, W = CAM.height, CAM.width
+    gmap = se.TSDFMap(10.24, 0.01)
+    pts = torch.zeros((H, W, 3), dtype=torch.float32, device="cuda")
+    nrm = torch.zeros_like(pts)
+    T = np.eye(4, dtype=np.float32)
+    bad_scales = [torch.zeros((H, W), dtype=torch.int32, device="cuda"),   # wrong dtype
+                  torch.zeros((H, W - 1), dtype=torch.int8, device="cuda"),  # short
+                  torch.zeros((H, W), dtype=torch.int8),                     # host memory
+                  torch.zeros((W, 2 * H), dtype=torch.int8, device="cuda")[:, ::2]]  # not contiguous
+    for sc in bad_scales:
+        with pytest.raises(ValueError, match="scale"):
+            se.raycast_volume(gmap, sensor, T, pts, nrm, scale=sc)
+    se.raycast_volume(gmap, sensor, T, pts, nrm, scale=torch.zeros((H, W), dtype=torch.int8, device="cuda"))
+    depth = torch.ones((64, 64), dtype=torch.float32, device="cuda")
+    out = torch.zeros((32, 32), dtype=torch.float32, device="cuda")
+    for mp in (torch.zeros((32, 32), dtype=torch.int32, device="cuda"),
+               torch.zeros((32, 31), dtype=torch.int64, device="cuda")):
+        with pytest.raises(ValueError, match="downsample_map"):
+            se.downsample_depth(depth, out, mp)
+    tr = se.Tracker(gmap, sensor)
+    T_dev = torch.zeros((2, 16), dtype=torch.float32, device="cuda")
+    d = torch.zeros((H, W), dtype=torch.float32, device="cuda")
+    for ok in (torch.zeros(1, dtype=torch.int64, device="cuda"), torch.zeros(1, dtype=torch.int32),
+               torch.zeros(2, dtype=torch.int32, device="cuda")):
+        with pytest.raises(ValueError, match="ok"):
+            tr.track_device(d, T_dev[0], T_dev[1], pts, nrm, ok=ok)
+    torch.cuda.synchronize()
+    tr.close()
+    gmap.close()

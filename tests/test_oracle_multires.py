@@ -8,6 +8,15 @@
 - Up-propagation: after frames 40 apart at 4 mm, every coarser entry above a block's current scale is
   the mean of its valid children (float32, children in x-fastest order) with the rounded-up mean
   weight, its past data {mean, 0}.
+- Down-propagation and the coarse-scale update against closed forms (the oracle's block state set
+  directly, then a frame with no valid depth or a fronto-parallel wall):
+  a linear field at scale 2 initialises unobserved scale-1 children to the field at their sample
+  points; a constant parent delta moves every observed child by exactly that delta (clamped at -1)
+  with the weights added and capped; a wall at 4.5 m is sampled at the scale-1 positions
+  0.004 * (v + 0.5) - 5.12 with tsdf min(sdf / mu, 1).  Planted in the oracle, each of these
+  mistakes fails one of them: the delta's sign, the interpolation's sample offset, the initialised
+  weight, the handed-down past weight, the weight sum, the -1 clamp, the update's sample position
+  and its entry index.
 """
 import numpy as np
 import pytest
@@ -110,3 +119,151 @@ def test_integration_scale_known_answers():
         if abs(z - 3.15) > 0.01 and abs(z - 6.3) > 0.01:
             assert sc[i, 0] == (0 if z < 3.15 else (1 if z < 6.3 else 2)), (z, sc[i])
             assert sc[i, 1] == sc[i, 0]
+
+
+# --- closed forms for the down-propagation and the coarse-scale update ------------------------------
+# multires_tsdf_updater_impl.hpp:59-150 (down-propagation), :152-181 (the update at the block's
+# scale), propagator_impl.hpp:18-117.  A block 4.48 m in front of a camera at the world origin
+# (identity pose, 640 x 480, fx 525) at 4 mm voxels integrates at scale 1 (1.5 <= 4.48 / 525 /
+# 0.004 < 3 pixels per voxel, also with the 0.25 m hysteresis).
+
+RES = 0.004
+B0 = np.array([1280, 1280, 2400], np.int32)  # voxel coordinates; world (0, 0, 4.48) m at its corner
+OFF = {0: 0, 1: 512, 2: 576, 3: 584}
+
+
+def _entries(s):
+    """(entry index, local voxel coordinate of the sample's cell) for every entry of scale s."""
+    n, st = 8 >> s, 1 << s
+    return [(OFF[s] + x + y * n + z * n * n, np.array([x * st, y * st, z * st]))
+            for z in range(n) for y in range(n) for x in range(n)]
+
+
+def _linear(p):
+    """A linear field of voxel coordinates, within (-1, 1) over the blocks used here."""
+    q = np.asarray(p, np.float64) - (B0 + 4)
+    return 0.1 + 0.01 * q[..., 0] + 0.02 * q[..., 1] - 0.015 * q[..., 2]
+
+
+def _block_state(coord, cur_scale, fill):
+    """Data and past data with every entry of the scales >= cur_scale set by fill(entry, cell, s)."""
+    data = np.zeros((585, 2), np.float32)
+    data[:, 0] = 1.0
+    past = data.copy()
+    for s in range(cur_scale, 4):
+        for e, cell in _entries(s):
+            data[e] = fill(e, coord + cell, s)
+            past[e] = (data[e, 0], 0.0)
+    return data, past
+
+
+def _frame_without_depth(m, ts):
+    """A frame whose every pixel is invalid: each frustum block chooses its scale and propagates
+    down, no sample passes the depth test, then each block propagates up."""
+    m.integrate(ts, np.zeros((CAM.height, CAM.width), np.float32), _pin(), np.eye(4, dtype=np.float32))
+
+
+def _find(keys, coord):
+    k, _ = O.encode_key(coord, 3)
+    i = int(np.searchsorted(keys, k))
+    assert keys[i] == k
+    return i
+
+
+def test_linear_field_propagates_down_exactly():
+    """Children never observed are initialised by trilinear interpolation of the coarser scale at
+    their sample points (visitor::getFieldInterp).  Interpolation reproduces a linear field, so the
+    scale-1 entries equal the field at their sample points (cell corner + 0.5 * 2 voxels) and take
+    the parent's weight; up-propagation then returns the field at the coarse sample points, since
+    the mean of the 8 children of a linear field is its value at the parent's sample point."""
+    m = O.OracleMap(10.24, RES, multires=True)
+    blocks = [B0 + 8 * np.array([i, j, k]) for k in (-1, 0, 1) for j in (-1, 0, 1) for i in (-1, 0, 1)]
+    m.allocate_blocks(np.array(blocks))
+    for b in blocks:
+        data, past = _block_state(b, 2, lambda e, c, s: (_linear(c + 0.5 * (1 << s)), 5.0))
+        assert m.set_multires_block(b, 2, 2, data, past) == 0
+    _frame_without_depth(m, 1)
+    keys, sc, data, past = m.dump_multires()
+    i = _find(keys, B0)
+    assert tuple(sc[i]) == (1, 1)
+    for e, cell in _entries(1):
+        want = _linear(B0 + cell + 1.0)
+        assert abs(data[i, e, 0] - want) < 2e-6, (cell, data[i, e, 0], want)
+        assert data[i, e, 1] == 5.0 and past[i, e, 1] == 0.0 and past[i, e, 0] == data[i, e, 0]
+    for s in (2, 3):
+        for e, cell in _entries(s):
+            assert abs(data[i, e, 0] - _linear(B0 + cell + 0.5 * (1 << s))) < 2e-6
+            assert data[i, e, 1] == 5.0
+    assert (data[i, :512, 1] == 0).all()  # scale 0 stays unobserved
+
+
+def test_delta_propagation_preserves_a_constant_offset():
+    """Observed children take the parent's change since its last propagation (data - past data) as
+    a delta: a parent raised by the same delta everywhere raises every child by exactly that delta
+    (clamped at -1), adds the parent's past weight to the child's weight (capped at max_weight) and
+    hands the past weight down."""
+    m = O.OracleMap(10.24, RES, multires=True)
+    m.allocate_blocks(B0[None])
+    rng = np.random.default_rng(5)
+    delta = np.float32(-0.25)
+    g = (rng.integers(-3900, 3900, 64) / 4096.0).astype(np.float32)  # exact in float32, +- delta too
+    g[:4] = [-0.9, -0.8, -0.76, -0.74]  # the first three fall below -1 after the delta
+    wc = rng.integers(1, 60, 64).astype(np.float32)
+    wc[4:8] = 98.0
+    wp = np.float32(5.0)
+    p = (rng.integers(-2000, 2000, 8) / 4096.0).astype(np.float32)
+    data = np.zeros((585, 2), np.float32)
+    data[:, 0] = 1.0
+    past = data.copy()
+    data[512:576, 0], data[512:576, 1] = g, wc
+    data[576:584, 0], data[576:584, 1] = p, 7.0
+    past[576:584, 0], past[576:584, 1] = p - delta, wp
+    data[584], past[584] = (p.mean(), 7.0), (p.mean(), 0.0)
+    assert m.set_multires_block(B0, 2, 1, data, past) == 0
+    _frame_without_depth(m, 1)
+    keys, sc, d, ps = m.dump_multires()
+    i = _find(keys, B0)
+    assert tuple(sc[i]) == (1, 1)
+    want = np.maximum(g + delta, np.float32(-1))
+    np.testing.assert_array_equal(d[i, 512:576, 0], want)
+    assert (d[i, 512:515, 0] == -1).all() and d[i, 515, 0] > -1
+    np.testing.assert_array_equal(d[i, 512:576, 1], np.minimum(wc + wp, 100))
+    np.testing.assert_array_equal(ps[i, 512:576, 1], np.full(64, wp))
+
+
+def test_fronto_parallel_wall_at_scale_one():
+    """One frame of a wall at constant depth D = 4.5 m, camera at the origin looking along +z: every
+    block integrates at scale 1, one sample per 2 x 2 x 2 cell at the cell's voxel (x, y, z) offset by
+    half a voxel (T_CV's sample_offset_frac), i.e. p_C = 0.004 * (v + 0.5) - 5.12.  Its sdf is
+    (D - z) / z * |p_C|; the first update stores min(sdf / mu, 1) with weight 1 where sdf >= -mu and
+    leaves the entry unobserved behind that."""
+    D = 4.5
+    mu = 8 * RES
+    m = O.OracleMap(10.24, RES, multires=True)
+    depth = np.full((CAM.height, CAM.width), D, np.float32)
+    m.integrate(0, depth, _pin(), np.eye(4, dtype=np.float32))
+    keys, sc, data, past = m.dump_multires()
+    assert len(keys) > 1000
+    assert (sc == 1).all()
+    checked = behind = 0
+    rng = np.random.default_rng(1)
+    for i in rng.choice(len(keys), 300, replace=False):
+        bc = O.decode_key(int(keys[i]))[0]
+        for e, cell in _entries(1):
+            pc = RES * (np.asarray(bc, np.float64) + cell + 0.5) - 5.12
+            u, v = CAM.fx * pc[0] / pc[2] + CAM.cx, CAM.fy * pc[1] / pc[2] + CAM.cy
+            if not (0.5 < u < CAM.width - 1.5 and 0.5 < v < CAM.height - 1.5):
+                continue  # away from the image border, where rounding decides
+            sdf = (D - pc[2]) / pc[2] * np.linalg.norm(pc)
+            if abs(sdf + mu) < 1e-5:
+                continue
+            if sdf < -mu:
+                assert data[i, e, 1] == 0 and data[i, e, 0] == 1
+                behind += 1
+            else:
+                assert data[i, e, 1] == 1
+                assert abs(data[i, e, 0] - min(sdf / mu, 1.0)) < 5e-5, (bc, cell, data[i, e, 0], sdf / mu)
+            assert past[i, e, 1] == 1  # every sample that passed the pixel tests is counted
+            checked += 1
+        assert (data[i, :512, 1] == 0).all()
+    assert checked > 10000 and behind > 100
