@@ -15,6 +15,9 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <algorithm>
+#include <cmath>
+#include <cstring>
 #include <string>
 
 #include "../../include/se8.h"
@@ -163,9 +166,6 @@ struct Cam {
     int left_hand_frame;
 };
 
-// scaled = 1 reproduces PinholeCamera(sensor, dsf), which applies the downsampling formulas
-// even for dsf = 1.
-Cam make_cam(const se8_pinhole& p, int dsf, bool scaled);
 
 SE8_HD void cam_backproject(const Cam& c, float u, float v, float o[3])
 {
@@ -361,6 +361,78 @@ SE8_HD bool lidar_sphere_in_frustum(const LidarModel& L, const float c[3], float
     return true;
 }
 
+inline void plane_normal(const float p1[3], const float p2[3], const float p3[3], float out[3])
+{
+    float t1[3], t2[3], n[3];
+    for (int i = 0; i < 3; i++) {
+        t1[i] = p2[i] - p1[i];
+        t2[i] = p3[i] - p2[i];
+    }
+    cross3(t1, t2, n);
+    normalized3(n, out);
+}
+
+// scaled = 1 reproduces PinholeCamera(sensor, dsf), which applies the downsampling formulas
+// even for dsf = 1.
+inline Cam make_cam(const se8_pinhole& p, int dsf, bool scaled)
+{
+    Cam c;
+    const float s = (float) dsf;
+    c.w = (int) ((float) p.width / s);
+    c.h = (int) ((float) p.height / s);
+    if (!scaled) {
+        c.fx = p.fx;
+        c.fy = p.fy;
+        c.cx = p.cx;
+        c.cy = p.cy;
+    }
+    else {
+        c.fx = p.fx / s;
+        c.fy = p.fy / s;
+        c.cx = (p.cx + 0.5f) / s - 0.5f;
+        c.cy = (p.cy + 0.5f) / s - 0.5f;
+    }
+    c.inv_fx = 1.f / c.fx;
+    c.inv_fy = 1.f / c.fy;
+    c.near_plane = p.near_plane;
+    c.far_plane = p.far_plane;
+    c.left_hand_frame = (p.fx < 0.0f) ^ (p.fy < 0.0f);
+    float tl[3], tr[3], br[3], bl[3];
+    cam_backproject(c, 0.f, 0.f, tl);
+    cam_backproject(c, (float) c.w, 0.f, tr);
+    cam_backproject(c, (float) c.w, (float) c.h, br);
+    cam_backproject(c, 0.f, (float) c.h, bl);
+    float v[8][3]; // near TL TR BR BL, far TL TR BR BL
+    for (int i = 0; i < 3; i++) {
+        v[0][i] = c.near_plane * tl[i];
+        v[1][i] = c.near_plane * tr[i];
+        v[2][i] = c.near_plane * br[i];
+        v[3][i] = c.near_plane * bl[i];
+        v[4][i] = c.far_plane * tl[i];
+        v[5][i] = c.far_plane * tr[i];
+        v[6][i] = c.far_plane * br[i];
+        v[7][i] = c.far_plane * bl[i];
+    }
+    plane_normal(v[4], v[0], v[3], c.planes[0]); // left
+    plane_normal(v[1], v[5], v[6], c.planes[1]); // right
+    plane_normal(v[7], v[3], v[2], c.planes[2]); // bottom
+    plane_normal(v[5], v[1], v[0], c.planes[3]); // top
+    for (int i = 0; i < 4; i++)
+        c.planes[i][3] = 0.f;
+    const float nearp[4] = {0.f, 0.f, 1.f, -c.near_plane};
+    const float farp[4] = {0.f, 0.f, -1.f, c.far_plane};
+    for (int j = 0; j < 4; j++) {
+        c.planes[4][j] = nearp[j];
+        c.planes[5][j] = farp[j];
+    }
+    float sign = c.fx * c.fy;
+    sign = sign / fabsf(sign);
+    for (int i = 0; i < 4; i++)
+        for (int j = 0; j < 3; j++)
+            c.planes[i][j] *= sign;
+    return c;
+}
+
 // The sensor interface of the reference's templated integrator (SensorBase: backProject, project,
 // farDist, near_plane, measurementFromPoint, sphereInFrustum) for the two sensor types.
 struct PinholeSensor {
@@ -517,6 +589,53 @@ struct MapView {
     int* block_scale;   // multi-resolution maps: per block {current scale, minimum scale}, -1 = none
     int64_t max_blocks, max_nodes, max_bricks;
 };
+
+// Layout of a map (se8_map_create): the octree size is power_two_up(max(ceil(max dim / res), 16))
+// (octree_impl.hpp:14-19), bricks are octants of scale min(6, max_scale), the block pool holds
+// tsdf_config.max_blocks blocks (default: every block of the octree, at most 2^18), the node pool
+// 2 blocks + 64 nodes, the brick pool a quarter of the blocks + 64 bricks, and the dense node table
+// one entry per octant of every scale in [4, max_scale), at node_dense_off[s] (node_dense_total
+// entries in all).  Pointers are left null.  Returns false for an octree deeper than the 19 scales
+// a key holds.
+inline bool plan_map_view(const se8_map_config& cfg, const se8_tsdf_config& tcfg, MapView& v,
+                          int64_t& node_dense_total)
+{
+    const float maxdim = std::max(cfg.dim[0], std::max(cfg.dim[1], cfg.dim[2]));
+    int size = std::max((int) std::ceil(maxdim / cfg.res), 16);
+    const int p = (int) std::ceil(std::log2((float) size));
+    if (p > 19)
+        return false;
+    size = 1 << p;
+    std::memset(&v, 0, sizeof(v));
+    v.size = size;
+    v.max_scale = p;
+    const int brick_scale = std::min(6, p);
+    v.brick_shift = brick_scale - kBlockScale;
+    v.bricks_side = size >> brick_scale;
+    v.res = cfg.res;
+    v.dim = (float) size * cfg.res;
+    v.mu = cfg.res * tcfg.truncation_boundary_factor;
+    v.max_weight = tcfg.max_weight;
+    for (int i = 0; i < 3; i++)
+        v.t_MW[i] = cfg.t_MW[i];
+    v.storage = tcfg.storage;
+    int64_t max_blocks = tcfg.max_blocks;
+    if (max_blocks <= 0) {
+        const int64_t blocks_side = size / 8;
+        max_blocks = std::min<int64_t>(blocks_side * blocks_side * blocks_side, (int64_t) 1 << 18);
+    }
+    v.max_blocks = max_blocks;
+    v.max_nodes = 2 * max_blocks + 64;
+    const int64_t bricks_side3 = (int64_t) v.bricks_side * v.bricks_side * v.bricks_side;
+    v.max_bricks = std::min<int64_t>(bricks_side3, max_blocks / 4 + 64);
+    node_dense_total = 0;
+    for (int sc = kBlockScale + 1; sc < p; sc++) {
+        v.node_dense_off[sc] = node_dense_total;
+        const int64_t sd = (int64_t) size >> sc;
+        node_dense_total += sd * sd * sd;
+    }
+    return true;
+}
 
 // Per-integration poses derived from T_WS (in device memory, so that the pose itself may come
 // from device memory): the sensor pose, its inverse for the frustum test and T_CV of the updater.
@@ -753,6 +872,7 @@ struct DeviceBuffer {
 
 int dev_alloc(DeviceBuffer& b, size_t bytes);
 void dev_free(DeviceBuffer& b);
+int device_count_sm(); // compute units of the current device
 
 struct RadixSorter {
     DeviceBuffer keys_alt, tile_hist;
