@@ -373,3 +373,41 @@ def test_raycast_stats_count_the_march():
     # 4 observed samples in the block, then large steps through unallocated space out of the map
     assert st[4] == 4 and st[3] == 0 and st[5] == 0
     assert st[0] > 0 and st[6] <= st[2]
+
+
+def test_half_round_is_binary16_nearest_even():
+    """The oracle's FP16 storage rounding against numpy's float32 -> float16 conversion (IEEE
+    round to nearest, ties to even): special values, the subnormal range, ties, and random values
+    over the TSDF range and beyond."""
+    rng = np.random.default_rng(3)
+    special = np.array([0.0, -0.0, 1.0, -1.0, 0.5, 65504.0, 65520.0, -65520.0, 6.1e-5, 5.96e-8, 2.98e-8, 2.99e-8,
+                        1.0 + 2.0 ** -11, 1.0 + 3 * 2.0 ** -11, np.inf, -np.inf], np.float32)
+    vals = np.concatenate([special, rng.uniform(-1, 1, 3000).astype(np.float32),
+                           (rng.standard_normal(1000) * 1e-5).astype(np.float32),
+                           (rng.standard_normal(500) * 3e4).astype(np.float32)])
+    with np.errstate(over="ignore"):
+        want = vals.astype(np.float16).astype(np.float32)
+    np.testing.assert_array_equal(O.half_round(vals), want)
+    assert np.isnan(O.half_round(np.array([np.nan], np.float32)))[0]
+
+
+def test_fp16_storage_map_stays_within_rounding_of_fp32():
+    """Two frames into FP32 and FP16 oracle maps: the same blocks and weights, stored values on the
+    binary16 grid, and every tsdf within n_updates * 2^-12 of the FP32 map's (half an ulp of
+    [0.5, 1) per rounded write, shrunk by each later average)."""
+    from tests.helpers import CAM, spline_frames
+
+    depth, poses = spline_frames(2)
+    pin = O.pinhole(CAM.width, CAM.height, CAM.fx, CAM.fy, CAM.cx, CAM.cy, CAM.near_plane, CAM.far_plane)
+    m32, m16 = O.OracleMap(10.24, 0.01), O.OracleMap(10.24, 0.01, storage="f16")
+    for k in range(2):
+        m32.integrate(k, depth[k], pin, poses[k])
+        m16.integrate(k, depth[k], pin, poses[k])
+    k32, _, t32, w32 = m32.dump_blocks()
+    k16, _, t16, w16 = m16.dump_blocks()
+    np.testing.assert_array_equal(k32, k16)
+    np.testing.assert_array_equal(w32, w16)
+    obs = w16 > 0
+    np.testing.assert_array_equal(O.half_round(t16[obs][::97]), t16[obs][::97])
+    assert np.all(np.abs(t16[obs] - t32[obs]) <= w16[obs] * 2.0 ** -12)
+    assert np.abs(t16[obs] - t32[obs]).max() > 0
