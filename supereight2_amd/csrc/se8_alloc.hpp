@@ -37,6 +37,7 @@ __global__ void k_frame_begin(MapView m, Iso T_arg, const float* T16_dev, FrameP
         m.counters[kCntMrNew] = 0;
         m.counters[kCntMrDown] = 0;
         m.counters[kCntNodesStart] = m.counters[kCntNodes];
+        m.counters[kCntBlocksStart] = m.counters[kCntBlocks];
         for (int i = 0; i < 4; i++)
             m.counters[kCntMrScale + i] = 0;
         const Iso T = T16_dev ? iso_from16(T16_dev) : T_arg;
@@ -50,15 +51,18 @@ __global__ void k_frame_begin(MapView m, Iso T_arg, const float* T16_dev, FrameP
 // fetch-frustum: a block is listed iff its own sphere and those of all its ancestors below the
 // root pass sphereInFrustum (the DFS of iterator_impl.hpp:86-156 never prunes at the root).
 // ----------------------------------------------------------------------------------------------
+// The blocks allocated before this integration (slots below kCntBlocksStart; slots above it may be
+// in creation while this runs), `wg` of `n_wg` workgroups of 64-thread waves grid-striding over them.
 template<class Sensor>
-__global__ void k_frustum(MapView m, const FramePoses* fp, Sensor sensor, int* update_list)
+__device__ void frustum_part(const MapView& m, const FramePoses* fp, const Sensor& sensor, int* update_list, int wg,
+                             int n_wg)
 {
     // the block counter runs past the pool when an allocation overflows; only slots below the
     // capacity hold blocks
-    const int n = (int) min((int64_t) m.counters[kCntBlocks], m.max_blocks);
+    const int n = (int) min((int64_t) m.counters[kCntBlocksStart], m.max_blocks);
     const Iso T_SW = fp->T_SW;
     const float r_unit = sqrtf(3.0f) / 2.0f * m.res;
-    for (int base = blockIdx.x * blockDim.x; base < n; base += gridDim.x * blockDim.x) {
+    for (int base = wg * blockDim.x; base < n; base += n_wg * blockDim.x) {
         const int slot = base + threadIdx.x;
         bool in = false;
         if (slot < n) {
@@ -82,6 +86,12 @@ __global__ void k_frustum(MapView m, const FramePoses* fp, Sensor sensor, int* u
         if (in)
             update_list[pos] = slot;
     }
+}
+
+template<class Sensor>
+__global__ void k_frustum(MapView m, const FramePoses* fp, Sensor sensor, int* update_list)
+{
+    frustum_part(m, fp, sensor, update_list, blockIdx.x, gridDim.x);
     // the frustum part of the list is complete when this kernel ends; the carver kernel records
     // its size
 }
@@ -258,20 +268,13 @@ __device__ void claim_ancestors(const MapView& m, const int c[3], uint64_t code)
     }
 }
 
-// create-list + allocate-list for maps with the dense tables: the band march of RaycastCarver
-// (raycast_carver_impl.hpp:74-111), as k_alloc_rays samples it, with the allocation of
-// allocator::blocks / Octree::allocate (allocator_impl.hpp:68-147, octree_impl.hpp:171-199) done in
-// place: the first thread to reach an unallocated block claims its dense-table entry, creates the
-// block (slot, key, timestamp, an update-list entry marked new) and its missing ancestors.  The set
-// of blocks and nodes is that of the sorted-key path; slot and node numbering follow the schedule.
-// kCntCandidates counts the samples' block keys of unallocated blocks (consecutive repeats dropped).
+// The carver's band march for the pixels [64 * bx, +64) of row y (see k_frustum_carve).
 template<class Sensor>
-__global__ void k_carve_claim(MapView m, Sensor sensor, const FramePoses* fp, const float* depth, float band,
-                              int num_steps, int* update_list)
+__device__ void carve_part(const MapView& m, const Sensor& sensor, const FramePoses* fp, const float* depth, float band,
+                           int num_steps, int* update_list, int bx, int y)
 {
     const Iso T_WS = fp->T_WS;
-    const int x = blockIdx.x * blockDim.x + threadIdx.x;
-    const int y = blockIdx.y;
+    const int x = bx * blockDim.x + threadIdx.x;
     bool live = x < sensor.width();
     float pos[3] = {0.f, 0.f, 0.f}, step[3] = {0.f, 0.f, 0.f};
     if (live) {
@@ -375,19 +378,43 @@ __global__ void k_carve_claim(MapView m, Sensor sensor, const FramePoses* fp, co
     }
 }
 
-// The links of the octants k_carve_claim created: each new node becomes its parent's child (the root
-// for the top scale), each new block its scale-4 parent's child (block_parent), its brick's slot
-// entry and part of the AABB (octree_impl.hpp:267-273); the frustum part of the update list is the
-// rest of it.  Loops run over whole waves (the AABB is aggregated by ballots).
-__global__ void k_link_new(MapView m, const int* update_list)
+// create-list + allocate-list for maps with the dense tables: the band march of RaycastCarver
+// (raycast_carver_impl.hpp:74-111), as k_alloc_rays samples it, with the allocation of
+// allocator::blocks / Octree::allocate (allocator_impl.hpp:68-147, octree_impl.hpp:171-199) done in
+// place: the first thread to reach an unallocated block claims its dense-table entry, creates the
+// block (slot, key, timestamp, an update-list entry marked new) and its missing ancestors.  The set
+// of blocks and nodes is that of the sorted-key path; slot and node numbering follow the schedule.
+// kCntCandidates counts the samples' block keys of unallocated blocks (consecutive repeats dropped).
+// The first n_frustum_wg workgroups run fetch-frustum over the blocks that existed before, in the
+// same launch: their list entries and the carver's interleave (the new ones carry bit 31).
+// 64-thread workgroups: carver workgroup b - n_frustum_wg covers pixels
+// [64 * (b' % tiles_x), +64) of row b' / tiles_x.
+template<class Sensor>
+__global__ void k_frustum_carve(MapView m, Sensor sensor, const FramePoses* fp, const float* depth, float band,
+                                int num_steps, int* update_list, int n_frustum_wg, int tiles_x)
+{
+    const int b = blockIdx.x;
+    if (b < n_frustum_wg) {
+        frustum_part(m, fp, sensor, update_list, b, n_frustum_wg);
+        return;
+    }
+    const int c = b - n_frustum_wg;
+    carve_part(m, sensor, fp, depth, band, num_steps, update_list, c % tiles_x, c / tiles_x);
+}
+
+// The links of the octants the carver created: each new node becomes its parent's child (the root
+// for the top scale), each new block its scale-4 parent's child (block_parent, whose timestamp it
+// raises to ts when ts >= 0), its brick's slot entry and part of the AABB (octree_impl.hpp:267-273);
+// the frustum part of the update list is the rest of it.  Thread gt of gn threads in whole waves
+// (the AABB is aggregated by ballots).
+__device__ void link_part(const MapView& m, const int* update_list, int gt, int gn, int ts)
 {
     const int n_upd = m.counters[kCntUpdate];
-    if (blockIdx.x == 0 && threadIdx.x == 0)
+    if (gt == 0)
         m.counters[kCntFrustum] = n_upd - m.counters[kCntUnique];
-    const int stride = gridDim.x * blockDim.x;
     const int n0 = m.counters[kCntNodesStart];
     const int n1 = (int) min((int64_t) m.counters[kCntNodes], m.max_nodes);
-    for (int i = n0 + blockIdx.x * blockDim.x + threadIdx.x; i < n1; i += stride) {
+    for (int i = n0 + gt; i < n1; i += gn) {
         const uint64_t key = m.node_key[i];
         const int s = (int) (key & 0x1f);
         const uint64_t code = key >> kScaleOffset;
@@ -397,8 +424,9 @@ __global__ void k_link_new(MapView m, const int* update_list)
         if (parent >= 0)
             m.node_child[parent * 8 + (int) ((code >> (3 * s)) & 7)] = i;
     }
-    for (int base = blockIdx.x * blockDim.x; base < n_upd; base += stride) {
-        const int i = base + threadIdx.x;
+    const int lane0 = gt - (int) (threadIdx.x & 63);
+    for (int base = lane0; base < n_upd; base += gn) {
+        const int i = base + (int) (threadIdx.x & 63);
         bool made = false;
         int c[3] = {0, 0, 0};
         if (i < n_upd) {
@@ -413,8 +441,11 @@ __global__ void k_link_new(MapView m, const int* update_list)
                     ? m.node_dense[m.node_dense_off[kBlockScale + 1] + dense_index(m, c, kBlockScale + 1)]
                     : 0;
                 m.block_parent[slot] = parent;
-                if (parent >= 0)
+                if (parent >= 0) {
                     m.node_child[parent * 8 + (int) ((code >> 9) & 7)] = slot;
+                    if (ts >= 0)
+                        atomicMax(&m.node_ts[parent], ts);
+                }
                 const int bs = m.brick_shift + kBlockScale;
                 const int brick = m.brick_top[((c[2] >> bs) * m.bricks_side + (c[1] >> bs)) * m.bricks_side + (c[0] >> bs)];
                 const int msk = (1 << m.brick_shift) - 1;
@@ -446,6 +477,13 @@ __global__ void k_link_new(MapView m, const int* update_list)
             }
         }
     }
+}
+
+// link_part as its own launch (multi-resolution maps, whose updater reads block_parent); the update
+// kernel of a single-resolution map runs link_part itself
+__global__ void k_link_new(MapView m, const int* update_list)
+{
+    link_part(m, update_list, blockIdx.x * blockDim.x + threadIdx.x, gridDim.x * blockDim.x, -1);
 }
 
 // ----------------------------------------------------------------------------------------------
@@ -850,7 +888,7 @@ inline int build_from_candidates(se8_map* map, uint64_t* cand, const int* n_dev,
 // ev_frustum / ev_carve (may be null) are recorded after the frustum and the band march.
 template<class Sensor>
 int allocate_frame(se8_map* m, const Iso& T_arg, const float* T16_dev, const float* depth_dev, const Sensor& sensor,
-                   hipEvent_t ev_frustum, hipEvent_t ev_carve, hipStream_t s)
+                   hipEvent_t ev_frustum, hipEvent_t ev_carve, hipStream_t s, bool link_in_update = false)
 {
     const float band = 2.0f * m->view.res * m->tsdf.truncation_boundary_factor;
     const int num_steps = (int) std::ceil(band / (2.f * m->view.res));
@@ -860,21 +898,29 @@ int allocate_frame(se8_map* m, const Iso& T_arg, const float* T16_dev, const flo
     int* ulist = (int*) m->update_list.ptr;
     hipLaunchKernelGGL(k_frame_begin, dim3(1), dim3(64), 0, s, m->view, T_arg, T16_dev, fp);
     const int cus = device_count_sm();
+    if (claim) {
+        // fetch-frustum and the carver in one launch (ev_frustum then marks the same point as
+        // ev_carve's start)
+        if (ev_frustum)
+            SE8_HIP_CHECK(hipEventRecord(ev_frustum, s));
+        const int tiles_x = (sensor.width() + 63) / 64, n_frustum_wg = cus * 4;
+        hipLaunchKernelGGL(k_frustum_carve<Sensor>, dim3(n_frustum_wg + tiles_x * sensor.height()), dim3(64), 0, s,
+                           m->view, sensor, (const FramePoses*) fp, depth_dev, band, num_steps, ulist, n_frustum_wg,
+                           tiles_x);
+        SE8_HIP_CHECK(hipGetLastError());
+        if (ev_carve)
+            SE8_HIP_CHECK(hipEventRecord(ev_carve, s));
+        if (!link_in_update) {
+            hipLaunchKernelGGL(k_link_new, dim3(cus), dim3(256), 0, s, m->view, (const int*) ulist);
+            SE8_HIP_CHECK(hipGetLastError());
+        }
+        return SE8_OK;
+    }
     hipLaunchKernelGGL(k_frustum<Sensor>, dim3(cus * 4), dim3(256), 0, s, m->view, (const FramePoses*) fp, sensor,
                        ulist);
     if (ev_frustum)
         SE8_HIP_CHECK(hipEventRecord(ev_frustum, s));
     const dim3 grid((sensor.width() + 63) / 64, sensor.height());
-    if (claim) {
-        hipLaunchKernelGGL(k_carve_claim<Sensor>, grid, dim3(64), 0, s, m->view, sensor, (const FramePoses*) fp,
-                           depth_dev, band, num_steps, ulist);
-        SE8_HIP_CHECK(hipGetLastError());
-        if (ev_carve)
-            SE8_HIP_CHECK(hipEventRecord(ev_carve, s));
-        hipLaunchKernelGGL(k_link_new, dim3(cus), dim3(256), 0, s, m->view, (const int*) ulist);
-        SE8_HIP_CHECK(hipGetLastError());
-        return SE8_OK;
-    }
     hipLaunchKernelGGL(k_alloc_rays<Sensor>, grid, dim3(64), 0, s, m->view, sensor, (const FramePoses*) fp, depth_dev,
                        band, num_steps, (uint64_t*) m->candidates.ptr, (int64_t) cand_cap);
     SE8_HIP_CHECK(hipGetLastError());

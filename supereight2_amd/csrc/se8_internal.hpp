@@ -678,7 +678,8 @@ enum Counter {
     kCntMrDown = 15, // multi-resolution update: blocks moving to a finer scale
     kCntMrScale = 16, // 4 counters: multi-resolution update list entries at scale 0..3
     kCntNodesStart = 20, // node count at the start of the integration in flight (its nodes are new)
-    kNumCounters = 21
+    kCntBlocksStart = 21, // block count at the start of the integration in flight
+    kNumCounters = 22
 };
 
 // Wave-aggregated append: every lane with pred gets the next free position of *counter (one

@@ -118,10 +118,15 @@ constexpr int kUpdK = 2;       // blocks a workgroup updates together (their loa
 // projects, and the group's depth samples are all in flight together.
 // amdgpu_waves_per_eu(8): the scalar registers must stay within the budget of 8 waves per SIMD
 // (96 allocated), or the one resident wave of workgroups shrinks from 4 to 3 per CU.
+// With link set (maps with the dense tables), the kernel first links the octants the carver
+// created (link_part, which also stamps the new blocks' parents); nothing here reads those links,
+// so no workgroup waits for another.
 template<int Storage, class Sensor>
 __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(8, 8))) void k_update(
-    MapView m, const int* update_list, const FramePoses* fp, Sensor sensor, const float* depth, int ts)
+    MapView m, const int* update_list, const FramePoses* fp, Sensor sensor, const float* depth, int ts, int link)
 {
+    if (link)
+        link_part(m, update_list, blockIdx.x * blockDim.x + threadIdx.x, gridDim.x * blockDim.x, ts);
     __shared__ int s_entry[kUpdChunk];
     __shared__ uint64_t s_key[kUpdChunk];
     const Iso T_CV = fp->T_CV;
@@ -139,9 +144,11 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(8, 8))) voi
             const int slot = entry & 0x7fffffff;
             s_entry[t] = entry;
             s_key[t] = m.block_key[slot];
-            // block timestamp and one-level parent propagation (propagator_impl.hpp:146-197)
+            // block timestamp and one-level parent propagation (propagator_impl.hpp:146-197); a
+            // block new this frame has its parent stamped by link_part when the links are made here
             m.block_ts[slot] = ts;
-            atomicMax(&m.node_ts[m.block_parent[slot]], ts);
+            if (!(link && entry < 0))
+                atomicMax(&m.node_ts[m.block_parent[slot]], ts);
         }
         __syncthreads();
         VoxelRaw<Storage> cur[kUpdK];
@@ -506,8 +513,10 @@ static int integrate_impl(se8_map* m, int timestamp, const float* depth_dev, con
     FramePoses* fp = (FramePoses*) m->frame_poses.ptr;
     if (timed)
         SE8_HIP_CHECK(hipEventRecord(m->ev[0], s));
+    // single-resolution maps with the dense tables link the new octants inside the update kernel
+    const int link = m->view.block_dense && m->view.node_dense && !is_multires(m) ? 1 : 0;
     int rc = allocate_frame(m, T_arg, T16_dev, depth_dev, sensor, timed ? m->ev[1] : nullptr,
-                            timed ? m->ev[2] : nullptr, s);
+                            timed ? m->ev[2] : nullptr, s, link != 0);
     if (rc != SE8_OK)
         return rc;
     // ev[3] / ev[4] are stamped at the start and end of the update kernel itself
@@ -532,12 +541,12 @@ static int integrate_impl(se8_map* m, int timestamp, const float* depth_dev, con
         // round of workgroups waiting for the first to finish
         hipExtLaunchKernelGGL(k_update<SE8_STORAGE_F32, Sensor>,
                               dim3(resident_grid(k_update<SE8_STORAGE_F32, Sensor>, 512)), dim3(512), 0, s, ev_upd0,
-                              ev_upd1, 0, m->view, ulist, (const FramePoses*) fp, sensor, depth_dev, timestamp);
+                              ev_upd1, 0, m->view, ulist, (const FramePoses*) fp, sensor, depth_dev, timestamp, link);
     }
     else {
         hipExtLaunchKernelGGL(k_update<SE8_STORAGE_F16, Sensor>,
                               dim3(resident_grid(k_update<SE8_STORAGE_F16, Sensor>, 512)), dim3(512), 0, s, ev_upd0,
-                              ev_upd1, 0, m->view, ulist, (const FramePoses*) fp, sensor, depth_dev, timestamp);
+                              ev_upd1, 0, m->view, ulist, (const FramePoses*) fp, sensor, depth_dev, timestamp, link);
     }
     SE8_HIP_CHECK(hipGetLastError());
     if (!timed)

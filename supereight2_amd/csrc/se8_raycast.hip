@@ -354,69 +354,84 @@ __device__ bool field_grad_vox(const MapView& m, const float vf[3], float g[3])
 #pragma unroll
         for (int n = 0; n < 4; n++)
             hi[a][n] = (c[a][n] >> 3) != (c[a][0] >> 3);
-    int bs[8];
-#pragma unroll
-    for (int k = 0; k < 8; k++)
-        bs[k] = lookup_block_nb_t<(ST >> 1) != 0>(m, c[0][(k & 1) ? 3 : 0], c[1][(k & 2) ? 3 : 0], c[2][(k & 4) ? 3 : 0]);
+    // The 8 slots are separate values and the columns are expanded by a macro, not a lambda: an
+    // array element selected by a runtime condition, or a variable a lambda captures by reference,
+    // can be left in private (scratch) memory, and scratch use limits the kernel's occupancy.
+#define SE8_GRAD_SLOT(kk) \
+    lookup_block_nb_t<(ST >> 1) != 0>(m, c[0][((kk) & 1) ? 3 : 0], c[1][((kk) & 2) ? 3 : 0], c[2][((kk) & 4) ? 3 : 0])
+    const int bs0 = SE8_GRAD_SLOT(0), bs1 = SE8_GRAD_SLOT(1), bs2 = SE8_GRAD_SLOT(2), bs3 = SE8_GRAD_SLOT(3),
+              bs4 = SE8_GRAD_SLOT(4), bs5 = SE8_GRAD_SLOT(5), bs6 = SE8_GRAD_SLOT(6), bs7 = SE8_GRAD_SLOT(7);
+#undef SE8_GRAD_SLOT
     const int base_slot = lookup_block_nb_t<(ST >> 1) != 0>(m, base[0], base[1], base[2]);
     constexpr int kOuter = 0x9; // bit i set: index i is an outer layer
     float F[4][4][4];
-    bool ok = base_slot >= 0;
+    int ok = base_slot >= 0;
     // the voxels of one (y, z) column: all four x for an inner (y, z), the inner two otherwise
-    auto load_column = [&](int j, int k) {
-        const int lo = hi[2][k] ? (hi[1][j] ? bs[6] : bs[4]) : (hi[1][j] ? bs[2] : bs[0]);
-        const int up = hi[2][k] ? (hi[1][j] ? bs[7] : bs[5]) : (hi[1][j] ? bs[3] : bs[1]);
-        const bool inner = ((kOuter >> j) & 1) + ((kOuter >> k) & 1) == 0;
-#pragma unroll
-        for (int i = 0; i < 4; i++)
-            if (inner || !((kOuter >> i) & 1)) {
-                const int sl = hi[0][i] ? up : lo;
-                float w;
-                load_voxel_t<(ST & 1)>(m, max(sl, 0), (c[0][i] & 7) + ((c[1][j] & 7) << 3) + ((c[2][k] & 7) << 6),
-                                 F[i][j][k], w);
-                ok = ok && sl >= 0 && w != 0.f;
-            }
-    };
-    // three batches of loads (inner y and z, outer y, outer z) keep at most 16 voxel loads in
-    // flight, which holds the kernel under 96 VGPRs
-    load_column(1, 1);
-    load_column(2, 1);
-    load_column(1, 2);
-    load_column(2, 2);
-    __builtin_amdgcn_sched_barrier(0);
-    load_column(0, 1);
-    load_column(3, 1);
-    load_column(0, 2);
-    load_column(3, 2);
-    __builtin_amdgcn_sched_barrier(0);
-    load_column(1, 0);
-    load_column(2, 0);
-    load_column(1, 3);
-    load_column(2, 3);
-    if (!ok)
-        return false;
-    const float rx = 1.f - fac[0], ry = 1.f - fac[1], rz = 1.f - fac[2];
-#pragma unroll
-    for (int a = 0; a < 3; a++) {
-        float cd[8];
-#pragma unroll
-        for (int k = 0; k < 8; k++) {
-            const int ck[3] = {k & 1, (k >> 1) & 1, (k >> 2) & 1};
-            int ip[3], im[3];
-#pragma unroll
-            for (int i = 0; i < 3; i++) {
-                ip[i] = 1 + ck[i];
-                im[i] = 1 + ck[i];
-            }
-            ip[a] = 2 + ck[a];
-            im[a] = ck[a];
-            cd[k] = F[ip[0]][ip[1]][ip[2]] - F[im[0]][im[1]][im[2]];
-        }
-        const float v = ((cd[0] * rx + cd[1] * fac[0]) * ry + (cd[2] * rx + cd[3] * fac[0]) * fac[1]) * rz
-            + ((cd[4] * rx + cd[5] * fac[0]) * ry + (cd[6] * rx + cd[7] * fac[0]) * fac[1]) * fac[2];
-        g[a] = 0.5f * v;
+#define SE8_GRAD_COLUMN(j, k)                                                                              \
+    {                                                                                                      \
+        const int lo = hi[2][k] ? (hi[1][j] ? bs6 : bs4) : (hi[1][j] ? bs2 : bs0);                          \
+        const int up = hi[2][k] ? (hi[1][j] ? bs7 : bs5) : (hi[1][j] ? bs3 : bs1);                          \
+        const bool inner = ((kOuter >> (j)) & 1) + ((kOuter >> (k)) & 1) == 0;                             \
+        _Pragma("unroll") for (int i = 0; i < 4; i++) if (inner || !((kOuter >> i) & 1))                   \
+        {                                                                                                  \
+            const int sl = hi[0][i] ? up : lo;                                                             \
+            float w;                                                                                       \
+            load_voxel_t<(ST & 1)>(m, max(sl, 0), (c[0][i] & 7) + ((c[1][j] & 7) << 3) + ((c[2][k] & 7) << 6), \
+                                   F[i][j][k], w);                                                         \
+            ok &= (sl >= 0) & (w != 0.f);                                                                  \
+        }                                                                                                  \
     }
-    return true;
+    // Three batches of loads (inner y and z, outer y, outer z) keep at most 16 voxel loads in
+    // flight.  Each axis's blend is taken as soon as its voxels are in: x needs the first batch,
+    // y the second, z the third, so the voxels only one axis reads are released early.  The
+    // validity of each batch is folded in as it arrives, so its weights are not kept either.
+    const float rx = 1.f - fac[0], ry = 1.f - fac[1], rz = 1.f - fac[2];
+#define SE8_GRAD_AXIS(a)                                                                                   \
+    {                                                                                                      \
+        float cd[8];                                                                                       \
+        _Pragma("unroll") for (int k = 0; k < 8; k++)                                                      \
+        {                                                                                                  \
+            const int ck[3] = {k & 1, (k >> 1) & 1, (k >> 2) & 1};                                         \
+            int ip[3], im[3];                                                                              \
+            _Pragma("unroll") for (int i = 0; i < 3; i++)                                                  \
+            {                                                                                              \
+                ip[i] = 1 + ck[i];                                                                         \
+                im[i] = 1 + ck[i];                                                                         \
+            }                                                                                              \
+            ip[a] = 2 + ck[a];                                                                             \
+            im[a] = ck[a];                                                                                 \
+            cd[k] = F[ip[0]][ip[1]][ip[2]] - F[im[0]][im[1]][im[2]];                                       \
+        }                                                                                                  \
+        const float v = ((cd[0] * rx + cd[1] * fac[0]) * ry + (cd[2] * rx + cd[3] * fac[0]) * fac[1]) * rz \
+            + ((cd[4] * rx + cd[5] * fac[0]) * ry + (cd[6] * rx + cd[7] * fac[0]) * fac[1]) * fac[2];      \
+        g[a] = 0.5f * v;                                                                                   \
+    }
+    SE8_GRAD_COLUMN(1, 1)
+    SE8_GRAD_COLUMN(2, 1)
+    // FP32 voxels are two words each: the first batch is split in two so that its loads fit the
+    // registers of 5 resident waves per SIMD (every wave of a 640x480 frame)
+    if constexpr ((ST & 1) == SE8_STORAGE_F32)
+        asm volatile("" : "+v"(F[1][1][1]), "+v"(F[2][1][1]), "+v"(F[1][2][1]), "+v"(F[2][2][1]), "+v"(ok) : : "memory");
+    SE8_GRAD_COLUMN(1, 2)
+    SE8_GRAD_COLUMN(2, 2)
+    SE8_GRAD_AXIS(0)
+    // each axis is complete before the next batch of loads is issued (an empty asm statement that
+    // takes the axis's value and orders memory), so that only one batch is in flight at a time
+    asm volatile("" : "+v"(g[0]), "+v"(ok) : : "memory");
+    SE8_GRAD_COLUMN(0, 1)
+    SE8_GRAD_COLUMN(3, 1)
+    SE8_GRAD_COLUMN(0, 2)
+    SE8_GRAD_COLUMN(3, 2)
+    SE8_GRAD_AXIS(1)
+    asm volatile("" : "+v"(g[1]), "+v"(ok) : : "memory");
+    SE8_GRAD_COLUMN(1, 0)
+    SE8_GRAD_COLUMN(2, 0)
+    SE8_GRAD_COLUMN(1, 3)
+    SE8_GRAD_COLUMN(2, 3)
+    SE8_GRAD_AXIS(2)
+#undef SE8_GRAD_AXIS
+#undef SE8_GRAD_COLUMN
+    return ok != 0;
 }
 
 // raycast<TSDF> (raycaster_impl.hpp:456-533).  On a hit, bracket (if given) receives the ray
@@ -754,8 +769,8 @@ int launch_raycast(const se8_map* m, const Cam& cam, const Iso& T_WS, const floa
 {
     const dim3 grid((cam.w + 15) / 16, (cam.h + 7) / 8);
     auto* kr = pick_kernel(m->view, k_raycast<0>, k_raycast<1>, k_raycast<2>, k_raycast<3>);
-    hipExtLaunchKernelGGL(kr, grid, dim3(kRayThreads), (uint32_t) ray_stack_bytes(m->view), s, ev_start,
-                          ev_stop, 0, m->view, cam, T_WS, T16_dev, points, normals, scale);
+    hipExtLaunchKernelGGL(kr, grid, dim3(kRayThreads), (uint32_t) ray_stack_bytes(m->view), s, ev_start, ev_stop, 0,
+                          m->view, cam, T_WS, T16_dev, points, normals, scale);
     SE8_HIP_CHECK(hipGetLastError());
     return SE8_OK;
 }

@@ -407,8 +407,37 @@ __device__ __forceinline__ void two_sum(float a, float b, float& s, float& err)
     err = (a - (s - bb)) + (b - bb);
 }
 
-// 6x6 Cholesky in fp32 (icp.cpp:33-41): x = 0 if the factorisation fails.  Each pivot's reciprocal
-// is taken once (6 divisions instead of 27 on the serial path of every Gauss-Newton launch).
+// The solve and the exponential map run in one thread at the start of every Gauss-Newton launch,
+// so they sit on the launch chain's critical path.  They use the hardware reciprocal, reciprocal
+// square root and square root (about 1 ulp) instead of the correctly rounded sequences, and
+// polynomial sine and cosine for the small angles of a step: the tracked pose is compared with the
+// reference to a tolerance (the sums' order already differs), and these differ from the rounded
+// operations by an ulp.
+__device__ __forceinline__ float fast_rcp(float x)
+{
+    return __builtin_amdgcn_rcpf(x);
+}
+
+// sin and cos: Taylor polynomials to x^9 / x^10 on |x| <= pi/4 (truncation below 2e-9, under half
+// an ulp of the result), libm beyond
+__device__ __forceinline__ void sincos_step(float x, float& sn, float& cs)
+{
+    if (fabsf(x) <= 0.785f) {
+        const float x2 = x * x;
+        sn = x * __builtin_fmaf(x2, __builtin_fmaf(x2, __builtin_fmaf(x2, __builtin_fmaf(x2, 1.f / 362880.f, -1.f / 5040.f),
+                                                                      1.f / 120.f), -1.f / 6.f), 1.f);
+        cs = __builtin_fmaf(x2, __builtin_fmaf(x2, __builtin_fmaf(x2, __builtin_fmaf(x2, __builtin_fmaf(x2, -1.f / 3628800.f,
+                                                                                                         1.f / 40320.f),
+                                                                                     -1.f / 720.f), 1.f / 24.f), -0.5f), 1.f);
+    }
+    else {
+        sn = sinf(x);
+        cs = cosf(x);
+    }
+}
+
+// 6x6 Cholesky in fp32 (icp.cpp:33-41): x = 0 if the factorisation fails.  Only the reciprocal of
+// each pivot's square root is needed (one hardware rsq per column).
 __device__ void solve_f32(const float* vals, float x[6])
 {
     float A[6][6], L[6][6], inv[6];
@@ -430,8 +459,7 @@ __device__ void solve_f32(const float* vals, float x[6])
                 x[i] = 0.f;
             return;
         }
-        L[k][k] = sqrtf(s);
-        inv[k] = 1.f / L[k][k];
+        inv[k] = __builtin_amdgcn_rsqf(s);
         for (int i = k + 1; i < 6; i++) {
             float v = A[i][k];
             for (int j = 0; j < k; j++)
@@ -454,6 +482,14 @@ __device__ void solve_f32(const float* vals, float x[6])
     }
 }
 
+// reciprocal in FP64: the hardware estimate refined by two Newton steps (error below an ulp)
+__device__ __forceinline__ double rcp_f64(double d)
+{
+    double r = __builtin_amdgcn_rcp(d);
+    r = __builtin_fma(r, __builtin_fma(-d, r, 1.0), r);
+    return __builtin_fma(r, __builtin_fma(-d, r, 1.0), r);
+}
+
 // 6x6 LDL^T in FP64 (no pivoting: J^T J is symmetric positive semi-definite); x = 0 on failure
 __device__ void solve_f64(const double* vals, float x[6])
 {
@@ -474,7 +510,7 @@ __device__ void solve_f64(const double* vals, float x[6])
             return;
         }
         D[j] = dj;
-        rD[j] = 1.0 / dj;
+        rD[j] = rcp_f64(dj);
         L[j][j] = 1.0;
         for (int i = j + 1; i < 6; i++) {
             double v = A[i][j];
@@ -502,27 +538,30 @@ __device__ void solve_f64(const double* vals, float x[6])
         x[i] = (float) z[i];
 }
 
-// math::exp (math_util_impl.hpp:117-174) for the twist a = [v; w]
+// math::exp (math_util_impl.hpp:117-174) for the twist a = [v; w], with the reference's formulas
+// (quaternion of the half angle; V = I + c1 W + c2 W^2) over the fast operations above
 __device__ Iso se3_exp(const float a[6])
 {
     const float w[3] = {a[3], a[4], a[5]};
     const float theta_sq = dot3(w, w);
-    const float theta = sqrtf(theta_sq);
+    const float theta = __builtin_amdgcn_sqrtf(theta_sq);
     const float half_theta = 0.5f * theta;
-    float imag, re;
+    float imag, re, s_t = 0.f, c_t = 1.f;
     if (theta < 1e-10f) {
         const float t4 = theta_sq * theta_sq;
         imag = 0.5f - (1.0f / 48.0f) * theta_sq + (1.0f / 3840.0f) * t4;
         re = 1.f - 0.5f * theta_sq + (1.f / 384.f) * t4;
     }
     else {
-        imag = sinf(half_theta) / theta;
-        re = cosf(half_theta);
+        float s_h;
+        sincos_step(half_theta, s_h, re);
+        sincos_step(theta, s_t, c_t);
+        imag = s_h * fast_rcp(theta);
     }
     float q[4] = {re, imag * w[0], imag * w[1], imag * w[2]};
-    const float qn = sqrtf(q[0] * q[0] + (q[1] * q[1] + (q[2] * q[2] + q[3] * q[3])));
+    const float rq = __builtin_amdgcn_rsqf(q[0] * q[0] + (q[1] * q[1] + (q[2] * q[2] + q[3] * q[3])));
     for (int i = 0; i < 4; i++)
-        q[i] = q[i] / qn;
+        q[i] = q[i] * rq;
     const float qw = q[0], qx = q[1], qy = q[2], qz = q[3];
     const float tx = 2.f * qx, ty = 2.f * qy, tz = 2.f * qz;
     const float twx = tx * qw, twy = ty * qw, twz = tz * qw;
@@ -546,8 +585,9 @@ __device__ Iso se3_exp(const float a[6])
     }
     else {
         const float W[3][3] = {{0.f, -w[2], w[1]}, {w[2], 0.f, -w[0]}, {-w[1], w[0], 0.f}};
-        const float c1 = (1.f - cosf(theta)) / theta_sq;
-        const float c2 = (theta - sinf(theta)) / (theta_sq * theta);
+        const float r2 = fast_rcp(theta_sq);
+        const float c1 = (1.f - c_t) * r2;
+        const float c2 = (theta - s_t) * (r2 * fast_rcp(theta));
         for (int i = 0; i < 3; i++)
             for (int j = 0; j < 3; j++) {
                 const float w2 = W[i][0] * W[0][j] + (W[i][1] * W[1][j] + W[i][2] * W[2][j]);
