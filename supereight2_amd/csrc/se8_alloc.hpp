@@ -299,7 +299,7 @@ __device__ void carve_part(const MapView& m, const Sensor& sensor, const FramePo
     }
     const int side = m.size >> 3;
     constexpr int kBatch = 8;
-    uint64_t prev = ~0ull;
+    int prev = -1; // dense index of the previous unallocated sample's block (repeats are dropped)
     for (int s0 = 0; s0 < num_steps; s0 += kBatch) {
         int vox[kBatch][3];
         bool inb[kBatch];
@@ -322,28 +322,28 @@ __device__ void carve_part(const MapView& m, const Sensor& sensor, const FramePo
                     pos[i] = pos[i] + step[i];
             }
         }
+        // block positions as 32-bit dense indices (at most 512^3 on a dense map) and keys only for
+        // the blocks this thread creates: the fewer registers, the more of the carver's waves run
         int slot[kBatch];
-        int64_t bi[kBatch];
+        int bi[kBatch];
 #pragma unroll
         for (int j = 0; j < kBatch; j++) {
-            bi[j] = ((int64_t) (vox[j][2] >> 3) * side + (vox[j][1] >> 3)) * side + (vox[j][0] >> 3);
+            bi[j] = ((vox[j][2] >> 3) * side + (vox[j][1] >> 3)) * side + (vox[j][0] >> 3);
             slot[j] = m.block_dense[bi[j]];
         }
-        uint64_t keys[kBatch];
         bool won[kBatch];
         int cnt = 0, wins = 0;
 #pragma unroll
         for (int j = 0; j < kBatch; j++) {
-            keys[j] = block_key_of_voxel(vox[j][0], vox[j][1], vox[j][2]);
             won[j] = false;
             if (inb[j] && slot[j] < 0) {
-                if (keys[j] != prev) {
+                if (bi[j] != prev) {
                     cnt++;
                     // an entry already claimed in this kernel is being created by its claimant
                     if (slot[j] == -1)
                         won[j] = atomicCAS(&m.block_dense[bi[j]], -1, kClaimed) == -1;
                 }
-                prev = keys[j];
+                prev = bi[j];
             }
             wins += won[j] ? 1 : 0;
         }
@@ -368,12 +368,13 @@ __device__ void carve_part(const MapView& m, const Sensor& sensor, const FramePo
                 atomicExch(&m.block_dense[bi[j]], -1);
                 continue;
             }
-            m.block_key[sl] = keys[j];
+            const uint64_t key = block_key_of_voxel(vox[j][0], vox[j][1], vox[j][2]);
+            m.block_key[sl] = key;
             m.block_ts[sl] = -1;
             update_list[ubase + u++] = (int) ((uint32_t) sl | 0x80000000u);
             atomicExch(&m.block_dense[bi[j]], sl);
             const int c[3] = {vox[j][0] & ~7, vox[j][1] & ~7, vox[j][2] & ~7};
-            claim_ancestors(m, c, keys[j] >> kScaleOffset);
+            claim_ancestors(m, c, key >> kScaleOffset);
         }
     }
 }
@@ -903,7 +904,7 @@ int allocate_frame(se8_map* m, const Iso& T_arg, const float* T16_dev, const flo
         // ev_carve's start)
         if (ev_frustum)
             SE8_HIP_CHECK(hipEventRecord(ev_frustum, s));
-        const int tiles_x = (sensor.width() + 63) / 64, n_frustum_wg = cus * 4;
+        const int tiles_x = (sensor.width() + 63) / 64, n_frustum_wg = cus;
         hipLaunchKernelGGL(k_frustum_carve<Sensor>, dim3(n_frustum_wg + tiles_x * sensor.height()), dim3(64), 0, s,
                            m->view, sensor, (const FramePoses*) fp, depth_dev, band, num_steps, ulist, n_frustum_wg,
                            tiles_x);

@@ -324,6 +324,11 @@ inline T atomicCAS(T* p, T expected, T desired)
     return expected;
 }
 template<class T>
+inline T atomicOr(T* p, T v)
+{
+    return __atomic_fetch_or(p, v, __ATOMIC_SEQ_CST);
+}
+template<class T>
 inline T atomicMin(T* p, T v)
 {
     T old = __atomic_load_n(p, __ATOMIC_SEQ_CST);
