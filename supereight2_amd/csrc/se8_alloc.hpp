@@ -149,10 +149,12 @@ __global__ void k_alloc_rays(MapView m, Sensor sensor, const FramePoses* fp, con
                 for (int i = 0; i < 3; i++)
                     pM[i] = pos[i] + m.t_MW[i];
                 if (pM[0] >= 0.f && pM[0] < m.dim && pM[1] >= 0.f && pM[1] < m.dim && pM[2] >= 0.f && pM[2] < m.dim) {
+                    // pM / res as IEEE division gives it for pM >= 2^-60 (div_by_recip); below
+                    // that both truncate to voxel 0
                     const int wrap = m.size - 1; // see lookup_block
-                    vox[j][0] = ((int) (pM[0] / m.res)) & wrap;
-                    vox[j][1] = ((int) (pM[1] / m.res)) & wrap;
-                    vox[j][2] = ((int) (pM[2] / m.res)) & wrap;
+                    vox[j][0] = ((int) div_by_recip(pM[0], m.res, m.inv_res)) & wrap;
+                    vox[j][1] = ((int) div_by_recip(pM[1], m.res, m.inv_res)) & wrap;
+                    vox[j][2] = ((int) div_by_recip(pM[2], m.res, m.inv_res)) & wrap;
                     inb[j] = true;
                 }
                 for (int i = 0; i < 3; i++)
@@ -312,10 +314,12 @@ __device__ void carve_part(const MapView& m, const Sensor& sensor, const FramePo
                 for (int i = 0; i < 3; i++)
                     pM[i] = pos[i] + m.t_MW[i];
                 if (pM[0] >= 0.f && pM[0] < m.dim && pM[1] >= 0.f && pM[1] < m.dim && pM[2] >= 0.f && pM[2] < m.dim) {
+                    // pM / res as IEEE division gives it for pM >= 2^-60 (div_by_recip); below
+                    // that both truncate to voxel 0
                     const int wrap = m.size - 1; // see lookup_block
-                    vox[j][0] = ((int) (pM[0] / m.res)) & wrap;
-                    vox[j][1] = ((int) (pM[1] / m.res)) & wrap;
-                    vox[j][2] = ((int) (pM[2] / m.res)) & wrap;
+                    vox[j][0] = ((int) div_by_recip(pM[0], m.res, m.inv_res)) & wrap;
+                    vox[j][1] = ((int) div_by_recip(pM[1], m.res, m.inv_res)) & wrap;
+                    vox[j][2] = ((int) div_by_recip(pM[2], m.res, m.inv_res)) & wrap;
                     inb[j] = true;
                 }
                 for (int i = 0; i < 3; i++)

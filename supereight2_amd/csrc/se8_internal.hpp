@@ -130,6 +130,30 @@ SE8_HD float sqnorm3(const float v[3])
     return dot3(v, v);
 }
 
+// x / b rounded to nearest even, equal to IEEE division, from inv_b = 1 / b rounded to nearest.
+// q0 = x * inv_b lies within 1.5 ulp of x / b; the correction q0 + (x - b q0) inv_b, whose
+// residual one fma gives exactly, brings it within one ulp, and a second correction then gives the
+// correctly rounded quotient by Markstein's theorem (reciprocal within half an ulp of 1 / b,
+// quotient within one ulp: P. Markstein, "Computation of elementary functions on the IBM RISC
+// System/6000 processor", IBM J. Res. Dev. 34(1), 1990).  The residuals are exact while they stay
+// clear of underflow: 2^-60 <= |x| <= 2^60 with x / b a normal number (any map resolution between
+// 2^-40 and 2^40 m).  Five instructions instead of the division's twelve on the device.
+SE8_HD float div_by_recip(float x, float b, float inv_b)
+{
+    float q = x * inv_b;
+    q = fmaf(fmaf(-q, b, x), inv_b, q);
+    return fmaf(fmaf(-q, b, x), inv_b, q);
+}
+
+// x / b for any x: div_by_recip where it is exact, the division itself for smaller, zero and
+// non-finite x
+SE8_HD float div_rn(float x, float b, float inv_b)
+{
+    if (!(fabsf(x) >= 0x1p-60f && fabsf(x) <= 0x1p60f))
+        return x / b;
+    return div_by_recip(x, b, inv_b);
+}
+
 SE8_HD void normalized3(const float v[3], float o[3])
 {
     const float z = sqnorm3(v);
@@ -565,7 +589,7 @@ struct MapView {
     int brick_shift;  // log2(blocks per brick side): bricks are octants of scale 3 + brick_shift
     int bricks_side;  // size >> (3 + brick_shift)
     float res;
-    float inv_dim_unused;
+    float inv_res;    // 1 / res rounded to nearest (div_rn)
     float dim;        // size * res
     float mu;         // res * truncation factor
     float max_weight;
@@ -613,6 +637,7 @@ inline bool plan_map_view(const se8_map_config& cfg, const se8_tsdf_config& tcfg
     v.brick_shift = brick_scale - kBlockScale;
     v.bricks_side = size >> brick_scale;
     v.res = cfg.res;
+    v.inv_res = 1.0f / cfg.res;
     v.dim = (float) size * cfg.res;
     v.mu = cfg.res * tcfg.truncation_boundary_factor;
     v.max_weight = tcfg.max_weight;

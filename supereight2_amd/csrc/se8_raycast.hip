@@ -68,7 +68,7 @@ __device__ Span first_block(const MapView& m, const float oW[3], const float dW[
     for (int i = 0; i < 3; i++) {
         const float d = fabsf(dW[i]) < eps ? copysignf(eps, dW[i]) : dW[i];
         const float pM = oW[i] + m.t_MW[i];
-        const float o = (pM / m.res) / (float) m.size + 1.0f;
+        const float o = div_rn(pM, m.res, m.inv_res) / (float) m.size + 1.0f;
         tc[i] = -1.f * (1.0f / fabsf(d));
         tb[i] = tc[i] * o;
         if (d > 0.0f) {
@@ -192,32 +192,38 @@ __device__ Span first_block(const MapView& m, const float oW[3], const float dW[
     return s;
 }
 
-// pointToVoxel<Safe::On>: voxel of a world point, false outside the map
-__device__ __forceinline__ bool point_to_voxel(const MapView& m, const float p[3], int v[3])
-{
-    float pM[3];
-#pragma unroll
-    for (int i = 0; i < 3; i++)
-        pM[i] = p[i] + m.t_MW[i];
-    if (!(pM[0] >= 0.f && pM[0] < m.dim && pM[1] >= 0.f && pM[1] < m.dim && pM[2] >= 0.f && pM[2] < m.dim))
-        return false;
-#pragma unroll
-    for (int i = 0; i < 3; i++)
-        v[i] = (int) (pM[i] / m.res);
-    return true;
-}
-
+// pointToVoxel<Safe::On> before the truncation: the voxel position of a world point (point_M /
+// res, equal to IEEE division), false outside the map.  Inside the map 0 <= point_M < dim, so
+// div_by_recip is exact unless a coordinate is below 2^-60; the bounds are tested without branches
+// and the quotients taken for every lane.
 __device__ __forceinline__ bool point_to_voxel_f(const MapView& m, const float p[3], float v[3])
 {
     float pM[3];
 #pragma unroll
     for (int i = 0; i < 3; i++)
         pM[i] = p[i] + m.t_MW[i];
-    if (!(pM[0] >= 0.f && pM[0] < m.dim && pM[1] >= 0.f && pM[1] < m.dim && pM[2] >= 0.f && pM[2] < m.dim))
+    const bool in = (pM[0] >= 0.f) & (pM[0] < m.dim) & (pM[1] >= 0.f) & (pM[1] < m.dim) & (pM[2] >= 0.f)
+        & (pM[2] < m.dim);
+#pragma unroll
+    for (int i = 0; i < 3; i++)
+        v[i] = div_by_recip(pM[i], m.res, m.inv_res);
+    if (in && !(fminf(fminf(pM[0], pM[1]), pM[2]) >= 0x1p-60f)) {
+#pragma unroll
+        for (int i = 0; i < 3; i++)
+            v[i] = pM[i] / m.res;
+    }
+    return in;
+}
+
+// pointToVoxel<Safe::On>: voxel of a world point, false outside the map
+__device__ __forceinline__ bool point_to_voxel(const MapView& m, const float p[3], int v[3])
+{
+    float vf[3];
+    if (!point_to_voxel_f(m, p, vf))
         return false;
 #pragma unroll
     for (int i = 0; i < 3; i++)
-        v[i] = pM[i] / m.res;
+        v[i] = (int) vf[i];
     return true;
 }
 
@@ -261,28 +267,30 @@ __device__ __forceinline__ bool gather_corners(const MapView& m, const int base[
     return ok;
 }
 
-// getData<Safe::On> along a march: consecutive samples usually fall in the same block, so the
-// block slot of the previous sample is kept and the brick/slot loads are skipped when it repeats.
+// getData<Safe::On> along a march, from the sample's voxel position (point_to_voxel_f; in = inside
+// the map): consecutive samples usually fall in the same block, so the block slot of the previous
+// sample is kept and the brick/slot loads are skipped when its block coordinates repeat.
 struct SlotCache {
-    uint64_t key = ~0ull;
+    int bx = -1, by = -1, bz = -1;
     int slot = -1;
 };
 
 template <int ST>
-__device__ __forceinline__ void get_data_cached(const MapView& m, const float p[3], float& tsdf, float& weight,
-                                                SlotCache& cache)
+__device__ __forceinline__ void get_data_cached(const MapView& m, bool in, const float vf[3], float& tsdf,
+                                                float& weight, SlotCache& cache)
 {
-    int v[3];
-    if (!point_to_voxel(m, p, v)) {
+    if (!in) {
         tsdf = 1.f;
         weight = 0.f;
         return;
     }
+    const int v[3] = {(int) vf[0], (int) vf[1], (int) vf[2]};
     const int wrap = m.size - 1;
-    const uint64_t key = (uint64_t) ((v[0] & wrap) >> 3) | ((uint64_t) ((v[1] & wrap) >> 3) << 21)
-        | ((uint64_t) ((v[2] & wrap) >> 3) << 42);
-    if (key != cache.key) {
-        cache.key = key;
+    const int bx = (v[0] & wrap) >> 3, by = (v[1] & wrap) >> 3, bz = (v[2] & wrap) >> 3;
+    if (bx != cache.bx || by != cache.by || bz != cache.bz) {
+        cache.bx = bx;
+        cache.by = by;
+        cache.bz = bz;
         cache.slot = lookup_block_t<(ST >> 1) != 0>(m, v[0], v[1], v[2]);
     }
     if (cache.slot < 0) {
@@ -453,15 +461,15 @@ __device__ bool march(const MapView& m, const float o[3], const float d[3], floa
 #pragma unroll
     for (int i = 0; i < 3; i++)
         p[i] = o[i] + d[i] * t;
-    float f_t, w;
+    float f_t, w, vf[3];
     SlotCache cache;
-    get_data_cached<ST>(m, p, f_t, w, cache);
+    get_data_cached<ST>(m, point_to_voxel_f(m, p, vf), vf, f_t, w, cache);
     float f_tt = 0.f;
     if (!(f_t >= 0.f))
         return false;
     for (; t < t_far; t += stepsize) {
         float f_new;
-        get_data_cached<ST>(m, p, f_new, w, cache);
+        get_data_cached<ST>(m, point_to_voxel_f(m, p, vf), vf, f_new, w, cache);
         if (w == 0.f) {
             stepsize = largestep;
 #pragma unroll
@@ -470,13 +478,11 @@ __device__ bool march(const MapView& m, const float o[3], const float d[3], floa
             continue;
         }
         f_tt = f_new;
+        // a sample with a weight lies inside the map, and vf is its voxel position
         if (f_tt <= 0.1f && f_tt >= -0.5f) {
-            float vf[3];
-            if (point_to_voxel_f(m, p, vf)) {
-                float fi;
-                if (field_interp_vox<ST>(m, vf, fi))
-                    f_tt = fi;
-            }
+            float fi;
+            if (field_interp_vox<ST>(m, vf, fi))
+                f_tt = fi;
         }
         if (f_tt < 0.f)
             break;
@@ -529,12 +535,12 @@ __device__ bool interp_and_grad(const MapView& m, const float pW[3], float& f, f
     }
     f = ((d[0] * tc[0] + d[1] * t[0]) * tc[1] + (d[2] * tc[0] + d[3] * t[0]) * t[1]) * tc[2]
         + ((d[4] * tc[0] + d[5] * t[0]) * tc[1] + (d[6] * tc[0] + d[7] * t[0]) * t[1]) * t[2];
-    g[0] = (((d[1] - d[0]) * tc[1] + (d[3] - d[2]) * t[1]) * tc[2] + ((d[5] - d[4]) * tc[1] + (d[7] - d[6]) * t[1]) * t[2])
-        / m.res;
-    g[1] = (((d[2] - d[0]) * tc[0] + (d[3] - d[1]) * t[0]) * tc[2] + ((d[6] - d[4]) * tc[0] + (d[7] - d[5]) * t[0]) * t[2])
-        / m.res;
-    g[2] = (((d[4] - d[0]) * tc[0] + (d[5] - d[1]) * t[0]) * tc[1] + ((d[6] - d[2]) * tc[0] + (d[7] - d[3]) * t[0]) * t[1])
-        / m.res;
+    g[0] = div_rn(((d[1] - d[0]) * tc[1] + (d[3] - d[2]) * t[1]) * tc[2] + ((d[5] - d[4]) * tc[1] + (d[7] - d[6]) * t[1]) * t[2],
+                  m.res, m.inv_res);
+    g[1] = div_rn(((d[2] - d[0]) * tc[0] + (d[3] - d[1]) * t[0]) * tc[2] + ((d[6] - d[4]) * tc[0] + (d[7] - d[5]) * t[0]) * t[2],
+                  m.res, m.inv_res);
+    g[2] = div_rn(((d[4] - d[0]) * tc[0] + (d[5] - d[1]) * t[0]) * tc[1] + ((d[6] - d[2]) * tc[0] + (d[7] - d[3]) * t[0]) * t[1],
+                  m.res, m.inv_res);
     return true;
 }
 
@@ -678,7 +684,7 @@ __global__ __launch_bounds__(kRayThreads) void k_raycast(MapView m, Cam cam, Iso
         if (point_to_voxel_f(m, hit, vf) && field_grad_vox<ST>(m, vf, g)) {
 #pragma unroll
             for (int i = 0; i < 3; i++)
-                g[i] = g[i] / m.res;
+                g[i] = div_rn(g[i], m.res, m.inv_res);
             normalized3(g, nrm);
         }
     }
@@ -726,7 +732,7 @@ __global__ void k_query(MapView m, const float* pts, int n, int mode, float* out
         const bool ok = field_grad_vox<ST>(m, vf, g);
         valid[i] = ok;
         for (int k = 0; k < 3; k++)
-            out[3 * i + k] = ok ? g[k] / m.res : 0.f;
+            out[3 * i + k] = ok ? div_rn(g[k], m.res, m.inv_res) : 0.f;
     }
 }
 
