@@ -142,9 +142,10 @@ __device__ Span first_block(const MapView& m, const float oW[3], const float dW[
             for (int i = 0; i < 3; i++)
                 if (tcen[i] > t_min)
                     idx ^= 1 << i;
+            // pos + scale_exp2 * bit, as a select (adding 0 leaves pos unchanged)
 #pragma unroll
             for (int i = 0; i < 3; i++)
-                pos[i] = pos[i] + scale_exp2 * (float) ((idx >> i) & 1);
+                pos[i] = ((idx >> i) & 1) ? pos[i] + scale_exp2 : pos[i];
             t_max = tv_max;
         }
         else {
@@ -152,7 +153,7 @@ __device__ Span first_block(const MapView& m, const float oW[3], const float dW[
                 (tcorner[0] <= tc_max) | ((tcorner[1] <= tc_max) << 1) | ((tcorner[2] <= tc_max) << 2);
 #pragma unroll
             for (int i = 0; i < 3; i++)
-                pos[i] = pos[i] - scale_exp2 * (float) ((step_mask >> i) & 1);
+                pos[i] = ((step_mask >> i) & 1) ? pos[i] - scale_exp2 : pos[i];
             t_min = tc_max;
             idx ^= step_mask;
             if ((idx & step_mask) != 0) {
