@@ -192,7 +192,7 @@ def mesh_bench(gmap, torch, se, cpu):
 
 
 def run_pipeline(depth_dev, gt, n_frames, warmup, steps, storage, compensated, torch, se, sensor, timing="off",
-                 device_pose=True, keep_points=False, dump=False, mesh=None):
+                 device_pose=True, keep_points=False, dump=False, mesh=None, graph=False):
     """Runs the per-frame loop over all n_frames frames: frames [0, warmup) untimed, exactly `steps`
     frames [warmup, warmup + steps) timed, the rest of the sequence untimed after them (its time is
     kept separately for the whole-sequence rate).  With device_pose (the default), each frame's pose
@@ -201,7 +201,9 @@ def run_pipeline(depth_dev, gt, n_frames, warmup, steps, storage, compensated, t
     pose to the host, which passes it on).  `timing` is the library's stage-timing mode: "off" (no
     events, no host waits: the throughput figure), "stages" or "levels" (per-stage and per-pyramid-
     level device times from HIP events, accumulated over the timed frames).  keep_points keeps every
-    frame's raycast vertex map on the device; dump returns the final map's blocks."""
+    frame's raycast vertex map on the device; dump returns the final map's blocks.  With graph, frames
+    1.. are replays of one captured frame graph (se8_frame_graph_launch: the same kernels with the
+    same arguments, dispatched back to back); it needs device poses, timing off and no kept points."""
     H, W = sensor.height, sensor.width
     gmap = se.TSDFMap(10.24, 0.01, storage=storage)
     integ = se.MapIntegrator(gmap)
@@ -216,6 +218,12 @@ def run_pipeline(depth_dev, gt, n_frames, warmup, steps, storage, compensated, t
     T_dev = torch.zeros((n_frames, 16), dtype=torch.float32, device="cuda")
     T_dev[0] = torch.from_numpy(T.reshape(16))
     poses = []
+    fg = None
+    if graph:
+        assert device_pose and timing == "off" and not keep_points
+        # graph pair p is buffer 1 - p: launch k - 1 (frame k) raycasts into buffer k % 2 and tracks
+        # against buffer (k + 1) % 2, as the per-call loop below does
+        fg = se.FrameGraph(tracker, gmap, sensor, [pts[1], pts[0]], [nrm[1], nrm[0]])
     torch.cuda.synchronize()
     t0 = t1 = t2 = None
     for k in range(n_frames):
@@ -230,7 +238,9 @@ def run_pipeline(depth_dev, gt, n_frames, warmup, steps, storage, compensated, t
         pk = k if keep_points else k % 2
         pp = (k - 1) if keep_points else (k + 1) % 2
         cur, prev = k % 2, (k + 1) % 2
-        if device_pose:
+        if fg is not None and k > 0:
+            fg.launch(k, depth_dev[k], T_dev[k - 1], T_dev[k])
+        elif device_pose:
             if k > 0:
                 tracker.track_device(depth_dev[k], T_dev[k - 1], T_dev[k], pts[pp], nrm[prev])
             integ.integrateDepth(k, depth_dev[k], sensor, T_dev[k])
@@ -251,6 +261,8 @@ def run_pipeline(depth_dev, gt, n_frames, warmup, steps, storage, compensated, t
     info = gmap.info()
     mesh_out = mesh_bench(gmap, torch, se, mesh == "cpu") if mesh else None
     blocks = gmap.dump_blocks() if dump else None
+    if fg is not None:
+        fg.close()
     tracker.close()
     gmap.close()
     ni, nr, nt = max(st["n_integrations"], 1), max(st["n_raycasts"], 1), max(st["n_tracks"], 1)
@@ -659,17 +671,24 @@ def main():
     print("fp32 pipeline", file=sys.stderr, flush=True)
     r32 = run_pipeline(depth_dev, gt, n_frames, warmup, steps, "f32", False, torch, se, sensor, timing="off",
                        keep_points=fp64, dump=fp64, mesh="gpu" if args.no_cpu_baseline else "cpu")
+    print("fp32 pipeline, frame graph", file=sys.stderr, flush=True)
+    r32g = run_pipeline(depth_dev, gt, n_frames, warmup, steps, "f32", False, torch, se, sensor, timing="off",
+                        graph=True)
+    graph_equal32 = bool(np.array_equal(np.stack(r32g["poses"]), np.stack(r32["poses"])))
     ate32 = ate(r32["poses"], gt)
     print("fp32 pipeline, stage and level timing", file=sys.stderr, flush=True)
     r32t = run_pipeline(depth_dev, gt, n_frames, warmup, steps, "f32", False, torch, se, sensor, timing="levels")
     print("fp32 pipeline, host poses", file=sys.stderr, flush=True)
     r32h = run_pipeline(depth_dev, gt, n_frames, warmup, steps, "f32", False, torch, se, sensor, timing="off",
                         device_pose=False)
-    r16 = r16t = r16p = None
+    r16 = r16g = r16t = r16p = None
     if not args.no_fp16:
         print("fp16 pipeline", file=sys.stderr, flush=True)
         r16 = run_pipeline(depth_dev, gt, n_frames, warmup, steps, "f16", True, torch, se, sensor, timing="off",
                            keep_points=fp64, dump=fp64)
+        print("fp16 pipeline, frame graph", file=sys.stderr, flush=True)
+        r16g = run_pipeline(depth_dev, gt, n_frames, warmup, steps, "f16", True, torch, se, sensor, timing="off",
+                            graph=True)
         print("fp16 pipeline, stage and level timing", file=sys.stderr, flush=True)
         r16t = run_pipeline(depth_dev, gt, n_frames, warmup, steps, "f16", True, torch, se, sensor, timing="levels")
         print("fp16 storage, plain ICP", file=sys.stderr, flush=True)
@@ -721,7 +740,9 @@ def main():
         r16["update_blocks_per_frame"], r16["candidates_per_frame"] = r16t["update_blocks_per_frame"], \
             r16t["candidates_per_frame"]
         roof16 = rooflines(r16, fixture, cam, warmup, steps, 4)
-        fp16 = {"fps": r16["fps"], "sequence_fps": r16["sequence_fps"], "ate_m": ate(r16["poses"], gt),
+        fp16 = {"fps": r16["fps"], "sequence_fps": r16["sequence_fps"], "frame_graph_fps": r16g["fps"],
+                "frame_graph_poses_equal": bool(np.array_equal(np.stack(r16g["poses"]), np.stack(r16["poses"]))),
+                "ate_m": ate(r16["poses"], gt),
                 "ate_minus_fp32_m": ate(r16["poses"], gt) - ate32, "blocks": r16["blocks"],
                 "stage_ms": r16["stage_ms"], "track_level_us": [1000 * x for x in r16["track_level_ms"]],
                 "stage_delta_ms_vs_fp32": {k: r16["stage_ms"][k] - r32["stage_ms"][k] for k in r32["stage_ms"]},
@@ -753,6 +774,9 @@ def main():
                    "stage_note": "stage and level times from HIP events in a second run of the same frames; the "
                                  "timed run records no events",
                    "poses": "device-resident (track -> integrate -> raycast without host round trips)",
+                   "frame_graph_fps": r32g["fps"], "frame_graph_poses_equal": graph_equal32,
+                   "frame_graph_note": "the same loop with frames 1.. replayed from one captured frame graph "
+                                       "(se8_frame_graph_launch); value is the per-call loop",
                    "host_pose_fps": r32h["fps"], "host_pose_ate_m": ate(r32h["poses"], gt),
                    "update_blocks_per_frame": r32["update_blocks_per_frame"],
                    "candidates_per_frame": r32["candidates_per_frame"]},

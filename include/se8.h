@@ -45,6 +45,7 @@ typedef struct se8_map se8_map;
 typedef struct se8_tracker se8_tracker;
 typedef struct se8_lidar se8_lidar;
 typedef struct se8_reader se8_reader;
+typedef struct se8_frame_graph se8_frame_graph;
 
 /* se::Map::Config (include/se/map/map.hpp:42-71).  T_MW is a pure translation (the reference's
  * raycaster assumes no rotation, include/se/map/impl/raycaster_impl.hpp:465); the default
@@ -227,6 +228,27 @@ int se8_raycast_dev(const se8_map* map, const se8_pinhole* sensor, const float* 
                     float* normals_dev, int8_t* scale_dev, se8_stream stream);
 int se8_track_dev(se8_tracker* tracker, const float* depth_dev, const float* T_in_dev, float* T_out_dev,
                   const float* ref_points_dev, const float* ref_normals_dev, int* ok_dev, se8_stream stream);
+
+/* -------- frame graphs (no counterpart in the reference; for callers that run the loop of
+ * app/src/main.cpp:98-162 at device speed).  One frame of that loop -- Tracker::track from the
+ * previous pose against the previous raycast (tracker_impl.hpp:17-116), integrateDepth at the
+ * tracked pose (map_integrator_impl.hpp:52-83), raycast_volume at it (raycaster_impl.hpp:537-627)
+ * -- captured once as a HIP graph of kernel nodes and replayed per frame: results are identical to
+ * se8_track_dev + se8_integrate_depth_dev + se8_raycast_dev with the same arguments.
+ * se8_frame_graph_create captures it for one tracker, map (at most 4096 voxels a side,
+ * single-resolution) and sensor, with stage timing off, and two vertex/normal map pairs (W*H*3
+ * floats each): launch i (counting from 0) tracks against pair (i + 1) % 2, then raycasts into pair
+ * i % 2, so the first launch's reference is pair 1 (raycast it with se8_raycast_dev first).
+ * se8_frame_graph_launch replays one frame on the stream: depth_dev is W*H floats, T_in_dev the
+ * previous pose and T_out_dev the tracked pose (16 floats each, device memory, may be equal),
+ * ok_dev (may be NULL) the pose check.  The host returns without waiting for this frame; it waits
+ * for the frame two launches back before reusing its graph.  Other calls on the same map and
+ * tracker may be interleaved in stream order. -------- */
+int se8_frame_graph_create(se8_tracker* tracker, se8_map* map, const se8_pinhole* sensor, float* points0_dev,
+                           float* normals0_dev, float* points1_dev, float* normals1_dev, se8_frame_graph** out);
+int se8_frame_graph_launch(se8_frame_graph* graph, int timestamp, const float* depth_dev, const float* T_in_dev,
+                           float* T_out_dev, int* ok_dev, se8_stream stream);
+int se8_frame_graph_destroy(se8_frame_graph* graph);
 
 /* -------- map queries (host arrays) -------- */
 /* se::allocator::blocks(voxel_coords, octree, root, only_allocated)

@@ -16,7 +16,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 BUILD_DIR = os.path.join(HERE, "_build")
 LIB_PATH = os.path.join(HERE, "libse8.so")
-SOURCES = ["se8_map.hip", "se8_raycast.hip", "se8_track.hip", "se8_mesh.hip", "se8_preprocess.hip", "se8_io.hip", "se8_multires.hip", "se8_readers.cpp"]
+SOURCES = ["se8_map.hip", "se8_raycast.hip", "se8_track.hip", "se8_mesh.hip", "se8_preprocess.hip", "se8_io.hip", "se8_multires.hip", "se8_graph.hip", "se8_readers.cpp"]
 ARCH = os.environ.get("SE8_OFFLOAD_ARCH", "gfx950")
 
 HIP_FLAGS = [

@@ -166,6 +166,9 @@ SIGNATURES = {
     "se8_integrate_range_dev": (C.c_int, [_P, C.c_int, _P, _P, _P, _P]),
     "se8_raycast_dev": (C.c_int, [_P, C.POINTER(Pinhole), _P, _P, _P, _P, _P]),
     "se8_track_dev": (C.c_int, [_P, _P, _P, _P, _P, _P, _P, _P]),
+    "se8_frame_graph_create": (C.c_int, [_P, _P, C.POINTER(Pinhole), _P, _P, _P, _P, C.POINTER(_P)]),
+    "se8_frame_graph_launch": (C.c_int, [_P, C.c_int, _P, _P, _P, _P, _P]),
+    "se8_frame_graph_destroy": (C.c_int, [_P]),
     "se8_allocate_blocks": (C.c_int, [_P, _I, C.c_int64, C.c_int, _I, C.POINTER(C.c_int64)]),
     "se8_write_block": (C.c_int, [_P, _I, _F, _F]),
     "se8_read_block": (C.c_int, [_P, _I, _F, _F]),

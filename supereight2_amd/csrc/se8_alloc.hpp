@@ -902,6 +902,7 @@ int allocate_frame(se8_map* m, const Iso& T_arg, const float* T16_dev, const flo
     FramePoses* fp = (FramePoses*) m->frame_poses.ptr;
     int* ulist = (int*) m->update_list.ptr;
     hipLaunchKernelGGL(k_frame_begin, dim3(1), dim3(64), 0, s, m->view, T_arg, T16_dev, fp);
+    graph_note_launch(kNodeFrameBegin, s);
     const int cus = device_count_sm();
     if (claim) {
         // fetch-frustum and the carver in one launch (ev_frustum then marks the same point as
@@ -913,6 +914,7 @@ int allocate_frame(se8_map* m, const Iso& T_arg, const float* T16_dev, const flo
                            m->view, sensor, (const FramePoses*) fp, depth_dev, band, num_steps, ulist, n_frustum_wg,
                            tiles_x);
         SE8_HIP_CHECK(hipGetLastError());
+        graph_note_launch(kNodeFrustumCarve, s);
         if (ev_carve)
             SE8_HIP_CHECK(hipEventRecord(ev_carve, s));
         if (!link_in_update) {

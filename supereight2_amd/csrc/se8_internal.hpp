@@ -31,6 +31,22 @@ namespace se8 {
 // ----------------------------------------------------------------------------------------------
 void set_error(const std::string& msg);
 
+// ----------------------------------------------------------------------------------------------
+// frame graphs (se8_graph.hip): while a frame is being captured, the launches whose arguments change
+// from frame to frame (input depth, pose pointers, timestamp) report the graph node they became, so
+// that each replay can set those arguments.  A no-op unless this thread is capturing a frame graph.
+// ----------------------------------------------------------------------------------------------
+enum GraphNodeKind {
+    kNodePyramid,      // k_pyramid(PyrArgs): d0, T16_dev
+    kNodeIcpFinish,    // k_icp_finish(IcpArgs): T_out_dev, ok_dev
+    kNodeFrameBegin,   // k_frame_begin(m, T_arg, T16_dev, fp): argument 2
+    kNodeFrustumCarve, // k_frustum_carve(m, sensor, fp, depth, ...): argument 3
+    kNodeUpdate,       // k_update(m, list, fp, sensor, depth, ts, link): arguments 4 and 5
+    kNodeRaycast,      // k_raycast(m, cam, T_arg, T16_dev, ...): argument 3
+    kNumGraphNodes
+};
+void graph_note_launch(int kind, hipStream_t s);
+
 #define SE8_HIP_CHECK(expr)                                                                        \
     do {                                                                                           \
         hipError_t e_ = (expr);                                                                    \

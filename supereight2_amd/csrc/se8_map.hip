@@ -536,17 +536,20 @@ static int integrate_impl(se8_map* m, int timestamp, const float* depth_dev, con
             return SE8_ERR_INVALID;
         }
     }
-    else if (m->view.storage == SE8_STORAGE_F32) {
-        // one resident wave of workgroups: the grid-stride loop then never leaves a second, partial
-        // round of workgroups waiting for the first to finish
-        hipExtLaunchKernelGGL(k_update<SE8_STORAGE_F32, Sensor>,
-                              dim3(resident_grid(k_update<SE8_STORAGE_F32, Sensor>, 512)), dim3(512), 0, s, ev_upd0,
-                              ev_upd1, 0, m->view, ulist, (const FramePoses*) fp, sensor, depth_dev, timestamp, link);
-    }
     else {
-        hipExtLaunchKernelGGL(k_update<SE8_STORAGE_F16, Sensor>,
-                              dim3(resident_grid(k_update<SE8_STORAGE_F16, Sensor>, 512)), dim3(512), 0, s, ev_upd0,
-                              ev_upd1, 0, m->view, ulist, (const FramePoses*) fp, sensor, depth_dev, timestamp, link);
+        // one resident wave of workgroups: the grid-stride loop then never leaves a second, partial
+        // round of workgroups waiting for the first to finish.  Timed launches carry the stage
+        // events in the dispatch; untimed ones are plain launches (capturable in a frame graph).
+        auto* ku = m->view.storage == SE8_STORAGE_F32 ? k_update<SE8_STORAGE_F32, Sensor> : k_update<SE8_STORAGE_F16, Sensor>;
+        const dim3 grid(resident_grid(ku, 512));
+        if (timed)
+            hipExtLaunchKernelGGL(ku, grid, dim3(512), 0, s, ev_upd0, ev_upd1, 0, m->view, ulist,
+                                  (const FramePoses*) fp, sensor, depth_dev, timestamp, link);
+        else
+            hipLaunchKernelGGL(ku, grid, dim3(512), 0, s, m->view, ulist, (const FramePoses*) fp, sensor, depth_dev,
+                               timestamp, link);
+        SE8_HIP_CHECK(hipGetLastError());
+        graph_note_launch(kNodeUpdate, s);
     }
     SE8_HIP_CHECK(hipGetLastError());
     if (!timed)

@@ -776,9 +776,14 @@ int launch_raycast(const se8_map* m, const Cam& cam, const Iso& T_WS, const floa
 {
     const dim3 grid((cam.w + 15) / 16, (cam.h + 7) / 8);
     auto* kr = pick_kernel(m->view, k_raycast<0>, k_raycast<1>, k_raycast<2>, k_raycast<3>);
-    hipExtLaunchKernelGGL(kr, grid, dim3(kRayThreads), (uint32_t) ray_stack_bytes(m->view), s, ev_start, ev_stop, 0,
-                          m->view, cam, T_WS, T16_dev, points, normals, scale);
+    const uint32_t lds = (uint32_t) ray_stack_bytes(m->view);
+    if (ev_start || ev_stop)
+        hipExtLaunchKernelGGL(kr, grid, dim3(kRayThreads), lds, s, ev_start, ev_stop, 0, m->view, cam, T_WS, T16_dev,
+                              points, normals, scale);
+    else
+        hipLaunchKernelGGL(kr, grid, dim3(kRayThreads), lds, s, m->view, cam, T_WS, T16_dev, points, normals, scale);
     SE8_HIP_CHECK(hipGetLastError());
+    graph_note_launch(kNodeRaycast, s);
     return SE8_OK;
 }
 

@@ -1085,6 +1085,7 @@ static void launch_pyramid(se8_tracker* tr, const float* depth, int levels, IcpS
     pa.T = T;
     pa.T16_dev = T16_dev;
     k_pyramid<<<pa.tiles_x * tiles_y, kPyrThreads, pyramid_lds_bytes(levels), s>>>(pa);
+    graph_note_launch(kNodePyramid, s);
 }
 
 template<bool Comp>
@@ -1159,6 +1160,7 @@ static int enqueue_track(se8_tracker* tr, const float* depth, const float* T16_h
     else
         k_icp_finish<false><<<1, kIcpThreads, 0, s>>>(a);
     SE8_HIP_CHECK(hipGetLastError());
+    graph_note_launch(kNodeIcpFinish, s);
     if (timed) {
         SE8_HIP_CHECK(hipEventRecord(tr->ev[1], s));
         tr->track_pending = true;
@@ -1298,6 +1300,29 @@ static int noalloc_scratch_ptr(float** out)
     }
     *out = ptrs[dev];
     return SE8_OK;
+}
+
+// Frame graphs (se8_graph.hip): size of the argument struct of the pyramid launch (kNodePyramid)
+// or the final ICP launch (kNodeIcpFinish), and the addresses of its per-frame fields inside a copy
+// of it: the input depth and the start pose (d0, T16_dev), or the result's destinations (T_out_dev,
+// ok_dev).
+size_t graph_track_args_bytes(int kind)
+{
+    return kind == kNodePyramid ? sizeof(PyrArgs) : sizeof(IcpArgs);
+}
+
+void graph_track_args_fields(int kind, void* args, void** f0, void** f1)
+{
+    if (kind == kNodePyramid) {
+        PyrArgs* a = (PyrArgs*) args;
+        *f0 = &a->d0;
+        *f1 = &a->T16_dev;
+    }
+    else {
+        IcpArgs* a = (IcpArgs*) args;
+        *f0 = &a->T_out_dev;
+        *f1 = &a->ok_dev;
+    }
 }
 
 } // namespace se8

@@ -578,6 +578,42 @@ class Tracker:
                    "se8_track_dev")
 
 
+class FrameGraph:
+    """One frame of the tracking loop (Tracker::track, integrateDepth, raycast_volume) as a replayed
+    HIP graph (se8_frame_graph_*).  points/normals: two (H, W, 3) float32 CUDA tensors each; launch i
+    tracks against pair (i + 1) % 2 and raycasts into pair i % 2.  Stage timing must be off."""
+
+    def __init__(self, tracker: Tracker, map_: TSDFMap, sensor: PinholeCamera, points, normals):
+        shape = (sensor.height, sensor.width, 3)
+        self.lib = capi.load()
+        self.h = C.c_void_p()
+        self._keep = (points, normals)
+        capi.check(self.lib.se8_frame_graph_create(tracker.h, map_.h, C.byref(sensor.c()),
+                                                   _dev_f32(points[0], shape), _dev_f32(normals[0], shape),
+                                                   _dev_f32(points[1], shape), _dev_f32(normals[1], shape),
+                                                   C.byref(self.h)), "se8_frame_graph_create")
+        self.shape = (sensor.height, sensor.width)
+
+    def launch(self, timestamp, depth, T_in, T_out, ok=None, stream=None):
+        Ti, To = _dev_pose(T_in), _dev_pose(T_out)
+        if Ti is None or To is None:
+            raise ValueError("a frame graph takes device poses")
+        okp = _dev_out(ok, (_torch().int32,), 1, "ok")
+        capi.check(self.lib.se8_frame_graph_launch(self.h, int(timestamp), _dev_f32(depth, self.shape), Ti, To, okp,
+                                                   _stream_handle(stream)), "se8_frame_graph_launch")
+
+    def close(self):
+        if getattr(self, "h", None) and self.h.value:
+            self.lib.se8_frame_graph_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 def stage_times(map_: TSDFMap | None = None, tracker: Tracker | None = None) -> dict:
     out = capi.StageTimes()
     lib = capi.load()
