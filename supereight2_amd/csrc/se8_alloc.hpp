@@ -47,12 +47,45 @@ __global__ void k_frame_begin(MapView m, Iso T_arg, const float* T16_dev, FrameP
     }
 }
 
+// Exclusive prefix over the wave of per-lane counts in [0, 16) and the wave's total, by ballots of
+// the count's bits.
+__device__ __forceinline__ void wave_prefix_small(int cnt, int& before, int& total)
+{
+    const uint64_t lt = (1ull << __lane_id()) - 1ull;
+    before = total = 0;
+#pragma unroll
+    for (int b = 0; b < 4; b++) {
+        const uint64_t mb = __ballot((cnt >> b) & 1);
+        before += __popcll(mb & lt) << b;
+        total += __popcll(mb) << b;
+    }
+}
+
+// Per lane, `total` consecutive positions of *counter for the wave (one atomic, by the first lane
+// with a non-zero count); returns this lane's first position.  Every lane of the wave must call it.
+__device__ __forceinline__ int wave_reserve(int cnt, int* counter)
+{
+    int before, total;
+    wave_prefix_small(cnt, before, total);
+    if (total == 0)
+        return 0;
+    const int leader = __ffsll((unsigned long long) __ballot(cnt > 0)) - 1;
+    int base = 0;
+    if (__lane_id() == leader)
+        base = atomicAdd(counter, total);
+    return __shfl(base, leader) + before;
+}
+
 // ----------------------------------------------------------------------------------------------
 // fetch-frustum: a block is listed iff its own sphere and those of all its ancestors below the
 // root pass sphereInFrustum (the DFS of iterator_impl.hpp:86-156 never prunes at the root).
 // ----------------------------------------------------------------------------------------------
 // The blocks allocated before this integration (slots below kCntBlocksStart; slots above it may be
-// in creation while this runs), `wg` of `n_wg` workgroups of 64-thread waves grid-striding over them.
+// in creation while this runs), `wg` of `n_wg` workgroups of 64-thread waves grid-striding over them,
+// kFrustumBatch blocks per thread per round: the round's keys are loaded together and its blocks in
+// the frustum take their list positions with one atomic per wave.
+constexpr int kFrustumBatch = 4;
+
 template<class Sensor>
 __device__ void frustum_part(const MapView& m, const FramePoses* fp, const Sensor& sensor, int* update_list, int wg,
                              int n_wg)
@@ -62,29 +95,44 @@ __device__ void frustum_part(const MapView& m, const FramePoses* fp, const Senso
     const int n = (int) min((int64_t) m.counters[kCntBlocksStart], m.max_blocks);
     const Iso T_SW = fp->T_SW;
     const float r_unit = sqrtf(3.0f) / 2.0f * m.res;
-    for (int base = wg * blockDim.x; base < n; base += n_wg * blockDim.x) {
-        const int slot = base + threadIdx.x;
-        bool in = false;
-        if (slot < n) {
-            int c[3];
-            key_to_coord(m.block_key[slot], c);
-            in = true;
-            for (int s = kBlockScale; s < m.max_scale && in; s++) {
-                const int size = 1 << s;
-                float cW[3];
-                for (int i = 0; i < 3; i++) {
-                    const int oc = c[i] & ~(size - 1);
-                    const float pM = ((float) oc + (float) size * 0.5f) * m.res;
-                    cW[i] = pM + (-m.t_MW[i]);
-                }
-                float cS[3];
-                iso_apply(T_SW, cW, cS);
-                in = sensor.sphere_in_frustum(cS, r_unit * (float) size);
-            }
+    const int stride = n_wg * (int) blockDim.x;
+    // base is the same in every lane, so every lane of a wave takes the reservation together
+    for (int base = wg * (int) blockDim.x; base < n; base += kFrustumBatch * stride) {
+        const int slot0 = base + (int) threadIdx.x;
+        uint64_t key[kFrustumBatch];
+#pragma unroll
+        for (int j = 0; j < kFrustumBatch; j++) {
+            const int slot = slot0 + j * stride;
+            key[j] = slot < n ? m.block_key[slot] : 0;
         }
-        const int pos = wave_append_slot(in, &m.counters[kCntUpdate]);
-        if (in)
-            update_list[pos] = slot;
+        bool in[kFrustumBatch];
+        int cnt = 0;
+#pragma unroll
+        for (int j = 0; j < kFrustumBatch; j++) {
+            in[j] = slot0 + j * stride < n;
+            if (in[j]) {
+                int c[3];
+                key_to_coord(key[j], c);
+                for (int s = kBlockScale; s < m.max_scale && in[j]; s++) {
+                    const int size = 1 << s;
+                    float cW[3];
+                    for (int i = 0; i < 3; i++) {
+                        const int oc = c[i] & ~(size - 1);
+                        const float pM = ((float) oc + (float) size * 0.5f) * m.res;
+                        cW[i] = pM + (-m.t_MW[i]);
+                    }
+                    float cS[3];
+                    iso_apply(T_SW, cW, cS);
+                    in[j] = sensor.sphere_in_frustum(cS, r_unit * (float) size);
+                }
+            }
+            cnt += in[j] ? 1 : 0;
+        }
+        int pos = wave_reserve(cnt, &m.counters[kCntUpdate]);
+#pragma unroll
+        for (int j = 0; j < kFrustumBatch; j++)
+            if (in[j])
+                update_list[pos++] = slot0 + j * stride;
     }
 }
 
@@ -204,35 +252,6 @@ __global__ void k_alloc_rays(MapView m, Sensor sensor, const FramePoses* fp, con
             }
         }
     }
-}
-
-// Exclusive prefix over the wave of per-lane counts in [0, 16) and the wave's total, by ballots of
-// the count's bits.
-__device__ __forceinline__ void wave_prefix_small(int cnt, int& before, int& total)
-{
-    const uint64_t lt = (1ull << __lane_id()) - 1ull;
-    before = total = 0;
-#pragma unroll
-    for (int b = 0; b < 4; b++) {
-        const uint64_t mb = __ballot((cnt >> b) & 1);
-        before += __popcll(mb & lt) << b;
-        total += __popcll(mb) << b;
-    }
-}
-
-// Per lane, `total` consecutive positions of *counter for the wave (one atomic, by the first lane
-// with a non-zero count); returns this lane's first position.  Every lane of the wave must call it.
-__device__ __forceinline__ int wave_reserve(int cnt, int* counter)
-{
-    int before, total;
-    wave_prefix_small(cnt, before, total);
-    if (total == 0)
-        return 0;
-    const int leader = __ffsll((unsigned long long) __ballot(cnt > 0)) - 1;
-    int base = 0;
-    if (__lane_id() == leader)
-        base = atomicAdd(counter, total);
-    return __shfl(base, leader) + before;
 }
 
 constexpr int kClaimed = -2; // dense-table entry of an octant a thread is creating in this kernel

@@ -111,11 +111,43 @@ struct VoxelRaw<SE8_STORAGE_F16> {
 constexpr int kUpdChunk = 512; // blocks of a workgroup whose list entries are staged in LDS at once
 constexpr int kUpdK = 2;       // blocks a workgroup updates together (their loads in flight together)
 
+// What the update of one voxel needs from its position: the measurement along the sensor axis
+// (z for the pinhole, |p| for the lidar), the distance |p| and the depth sample at its pixel (ok =
+// the voxel is within range and projects into the image).
+struct VoxelSample {
+    float mz, nrm, d;
+    bool ok;
+};
+
+// Per voxel as singleres_tsdf_updater_impl.hpp:51-71: the voxel centre in the sensor frame, the far
+// test, the projection and the depth sample (issued, not awaited).  The load is issued on every lane
+// (pixel 0 where the voxel does not project) so that the pipeline's loads are straight-line code and
+// each wait covers only the loads it needs; `live` = false marks a lane with no block.
+template<class Sensor>
+__device__ __forceinline__ VoxelSample sample_voxel(const Iso& T_CV, const Sensor& sensor, const float* depth,
+                                                    uint64_t key, int vx, int vy, int vz, bool live)
+{
+    int c[3];
+    key_to_coord(key, c);
+    const float v[3] = {(float) (c[0] + vx), (float) (c[1] + vy), (float) (c[2] + vz)};
+    float pC[3];
+    iso_apply(T_CV, v, pC);
+    VoxelSample o;
+    o.nrm = sqrtf(sqnorm3(pC));
+    o.mz = sensor.measurement(pC);
+    float uv[2] = {0.f, 0.f};
+    o.ok = live && !sensor.beyond_far(pC, o.nrm) && sensor.project(pC, uv);
+    const int px = (int) (uv[0] + 0.5f), py = (int) (uv[1] + 0.5f);
+    o.d = depth[o.ok ? px + (size_t) py * sensor.width() : 0];
+    return o;
+}
+
 // Per voxel as singleres_tsdf_updater_impl.hpp:44-128.  A workgroup owns the list entries
 // blockIdx.x, blockIdx.x + gridDim.x, ...: it stages up to kUpdChunk of them (entry, key) in LDS
 // with one round of loads (stamping the block and parent timestamps at the same time), then walks
-// them kUpdK blocks at a time: the voxels of the next group are loaded while the current group
-// projects, and the group's depth samples are all in flight together.
+// them kUpdK blocks at a time as a two-stage pipeline: while a group is updated, the next group's
+// voxels and depth samples are already loading (its projections are computed first), so neither
+// load waits on the arithmetic it feeds.
 // amdgpu_waves_per_eu(8): the scalar registers must stay within the budget of 8 waves per SIMD
 // (96 allocated), or the one resident wave of workgroups shrinks from 4 to 3 per CU.
 // With link set (maps with the dense tables), the kernel first links the octants the carver
@@ -151,61 +183,48 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(8, 8))) voi
                 atomicMax(&m.node_ts[m.block_parent[slot]], ts);
         }
         __syncthreads();
-        VoxelRaw<Storage> cur[kUpdK];
-#pragma unroll
-        for (int k = 0; k < kUpdK; k++) {
-            cur[k].clear();
-            if (k < cnt && s_entry[k] >= 0)
-                cur[k].load(m, (size_t) s_entry[k] * 512 + t);
-        }
-        for (int i = 0; i < cnt; i += kUpdK) {
-            // the next group's voxels, issued before this group's depth samples
-            VoxelRaw<Storage> nxt[kUpdK];
+        // two register sets take turns: while the group in one is updated, the next group's voxels
+        // and depth samples load into the other (the loop body runs twice with the roles swapped,
+        // so no registers are copied between iterations)
+        // every lane issues every load (block slot 0 and pixel 0 stand in where there is nothing to
+        // read; update_group ignores those values), so the waits count exactly the loads they need
+        auto load_group = [&](int j, VoxelRaw<Storage> v[kUpdK], VoxelSample smp[kUpdK]) {
 #pragma unroll
             for (int k = 0; k < kUpdK; k++) {
-                nxt[k].clear();
-                const int j = i + kUpdK + k;
-                if (j < cnt && s_entry[j] >= 0)
-                    nxt[k].load(m, (size_t) s_entry[j] * 512 + t);
+                const int e = s_entry[min(j + k, cnt - 1)];
+                v[k].load(m, (size_t) (j + k < cnt && e >= 0 ? e : 0) * 512 + t);
             }
-            float pC[kUpdK][3], nrm[kUpdK], dval[kUpdK];
-            bool d_ok[kUpdK];
 #pragma unroll
-            for (int k = 0; k < kUpdK; k++) {
-                d_ok[k] = false;
-                dval[k] = 0.f;
-                nrm[k] = 0.f;
-                pC[k][0] = pC[k][1] = pC[k][2] = 0.f;
-                if (i + k < cnt) {
-                    int c[3];
-                    key_to_coord(s_key[i + k], c);
-                    const float v[3] = {(float) (c[0] + vx), (float) (c[1] + vy), (float) (c[2] + vz)};
-                    iso_apply(T_CV, v, pC[k]);
-                    nrm[k] = sqrtf(sqnorm3(pC[k]));
-                    float uv[2];
-                    if (!sensor.beyond_far(pC[k], nrm[k]) && sensor.project(pC[k], uv)) {
-                        const int px = (int) (uv[0] + 0.5f), py = (int) (uv[1] + 0.5f);
-                        dval[k] = depth[px + (size_t) py * sensor.width()];
-                        d_ok[k] = true;
-                    }
-                }
-            }
+            for (int k = 0; k < kUpdK; k++)
+                smp[k] = sample_voxel(T_CV, sensor, depth, s_key[min(j + k, cnt - 1)], vx, vy, vz, j + k < cnt);
+        };
+        auto update_group = [&](int i, VoxelRaw<Storage> v[kUpdK], const VoxelSample smp[kUpdK]) {
 #pragma unroll
             for (int k = 0; k < kUpdK; k++) {
                 if (i + k >= cnt)
                     break;
                 const int entry = s_entry[i + k];
+                // a block new this frame starts from FieldData<TSDF>{} (its slot was not read)
+                if (entry < 0)
+                    v[k].clear();
                 float tsdf, weight;
-                cur[k].get(tsdf, weight);
-                // measurementFromPoint: z for the pinhole, |p| for the lidar
-                const bool changed = tsdf_update_voxel(m, dval[k], d_ok[k], sensor.near_plane(),
-                                                       sensor.measurement(pC[k]), nrm[k], tsdf, weight);
+                v[k].get(tsdf, weight);
+                const bool changed = tsdf_update_voxel(m, smp[k].d, smp[k].ok, sensor.near_plane(), smp[k].mz,
+                                                       smp[k].nrm, tsdf, weight);
                 if (changed || entry < 0)
                     VoxelRaw<Storage>::store(m, (size_t) (entry & 0x7fffffff) * 512 + t, tsdf, weight);
             }
-#pragma unroll
-            for (int k = 0; k < kUpdK; k++)
-                cur[k] = nxt[k];
+        };
+        VoxelRaw<Storage> va[kUpdK], vb[kUpdK];
+        VoxelSample sa[kUpdK], sb[kUpdK];
+        load_group(0, va, sa);
+        for (int i = 0; i < cnt; i += 2 * kUpdK) {
+            load_group(i + kUpdK, vb, sb);
+            update_group(i, va, sa);
+            if (i + kUpdK >= cnt)
+                break;
+            load_group(i + 2 * kUpdK, va, sa);
+            update_group(i + kUpdK, vb, sb);
         }
     }
 }

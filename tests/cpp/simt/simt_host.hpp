@@ -298,6 +298,14 @@ inline T __shfl_xor(T x, int mask, int line = __builtin_LINE())
     return shfl_any(x, (int) __lane_id() ^ mask, line);
 }
 
+// lane l receives lane l - delta's value; lanes below delta keep their own (HIP's __shfl_up)
+template<class T>
+inline T __shfl_up(T x, unsigned delta, int line = __builtin_LINE())
+{
+    const int l = (int) __lane_id();
+    return shfl_any(x, l >= (int) delta ? l - (int) delta : l, line);
+}
+
 inline int __popcll(unsigned long long v)
 {
     return __builtin_popcountll(v);
