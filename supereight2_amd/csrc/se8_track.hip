@@ -407,8 +407,8 @@ __device__ __forceinline__ void two_sum(float a, float b, float& s, float& err)
     err = (a - (s - bb)) + (b - bb);
 }
 
-// The solve and the exponential map run in one thread at the start of every Gauss-Newton launch,
-// so they sit on the launch chain's critical path.  They use the hardware reciprocal, reciprocal
+// The solve and the exponential map run in one wave at the start of every Gauss-Newton launch, so
+// they sit on the launch chain's critical path.  They use the hardware reciprocal, reciprocal
 // square root and square root (about 1 ulp) instead of the correctly rounded sequences, and
 // polynomial sine and cosine for the small angles of a step: the tracked pose is compared with the
 // reference to a tolerance (the sums' order already differs), and these differ from the rounded
@@ -436,52 +436,6 @@ __device__ __forceinline__ void sincos_step(float x, float& sn, float& cs)
     }
 }
 
-// 6x6 Cholesky in fp32 (icp.cpp:33-41): x = 0 if the factorisation fails.  Only the reciprocal of
-// each pivot's square root is needed (one hardware rsq per column).
-__device__ void solve_f32(const float* vals, float x[6])
-{
-    float A[6][6], L[6][6], inv[6];
-    int c = 6;
-    for (int i = 0; i < 6; i++)
-        for (int j = i; j < 6; j++) {
-            A[i][j] = vals[c++];
-            A[j][i] = A[i][j];
-        }
-    for (int i = 0; i < 6; i++)
-        for (int j = 0; j < 6; j++)
-            L[i][j] = 0.f;
-    for (int k = 0; k < 6; k++) {
-        float s = A[k][k];
-        for (int j = 0; j < k; j++)
-            s -= L[k][j] * L[k][j];
-        if (!(s > 0.f)) {
-            for (int i = 0; i < 6; i++)
-                x[i] = 0.f;
-            return;
-        }
-        inv[k] = __builtin_amdgcn_rsqf(s);
-        for (int i = k + 1; i < 6; i++) {
-            float v = A[i][k];
-            for (int j = 0; j < k; j++)
-                v -= L[i][j] * L[k][j];
-            L[i][k] = v * inv[k];
-        }
-    }
-    float y[6];
-    for (int i = 0; i < 6; i++) {
-        float v = vals[i];
-        for (int j = 0; j < i; j++)
-            v -= L[i][j] * y[j];
-        y[i] = v * inv[i];
-    }
-    for (int i = 5; i >= 0; i--) {
-        float v = y[i];
-        for (int j = i + 1; j < 6; j++)
-            v -= L[j][i] * x[j];
-        x[i] = v * inv[i];
-    }
-}
-
 // reciprocal in FP64: the hardware estimate refined by two Newton steps (error below an ulp)
 __device__ __forceinline__ double rcp_f64(double d)
 {
@@ -490,64 +444,151 @@ __device__ __forceinline__ double rcp_f64(double d)
     return __builtin_fma(r, __builtin_fma(-d, r, 1.0), r);
 }
 
-// 6x6 LDL^T in FP64 (no pivoting: J^T J is symmetric positive semi-definite); x = 0 on failure
-__device__ void solve_f64(const double* vals, float x[6])
+// ---- the per-launch solve across the lanes of one wave ----
+// The 6x6 solve (icp.cpp:15-41), the exponential map (math::exp, math_util_impl.hpp:117-174) and the
+// pose product T <- exp(x) T (icp.cpp:208-223) sit on every Gauss-Newton launch's critical path.  Wave
+// 0 runs them with the rows of the system spread over its lanes: the factorisation is right-looking
+// (step k takes the pivot from lane k and updates the remaining rows in their own lanes), which
+// performs every entry's subtractions in the same order as a row-by-row factorisation; the
+// exponential map and the product give one output entry per lane.
+__device__ __forceinline__ float bcast_f(float v, int lane)
 {
-    double A[6][6], L[6][6], D[6], rD[6];
-    int c = 6;
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), lane));
+}
+
+__device__ __forceinline__ double bcast_d(double v, int lane)
+{
+    const unsigned long long u = (unsigned long long) __double_as_longlong(v);
+    const unsigned lo = (unsigned) __builtin_amdgcn_readlane((int) (unsigned) u, lane);
+    const unsigned hi = (unsigned) __builtin_amdgcn_readlane((int) (unsigned) (u >> 32), lane);
+    return __longlong_as_double((long long) (((unsigned long long) hi << 32) | lo));
+}
+
+// index of J^T J entry (i, j), i <= j, in the row-major upper triangle that follows the 6 J^T r terms:
+// row i starts at i * (11 - i) / 2 = 0, 5, 9, 12, 14, 15 (a byte table in one 64-bit constant)
+__device__ __forceinline__ int tri6(int i, int j)
+{
+    return (int) ((0x0F0E0C090500ull >> (8 * i)) & 0xff) + j;
+}
+
+// 6x6 Cholesky in fp32 (icp.cpp:33-41) across the lanes of one wave, every lane returning x; x = 0 if
+// the factorisation fails.  Lane r holds row r of A (lanes above 5 repeat row 5); only the reciprocal
+// of each pivot's square root is needed (one hardware rsq per column).
+__device__ void solve_f32_wave(const float* vals, float x[6])
+{
+    const int r = min((int) (threadIdx.x & 63), 5);
+    float A[6];
+#pragma unroll
+    for (int c = 0; c < 6; c++)
+        A[c] = vals[6 + tri6(min(r, c), max(r, c))];
+    float b = vals[r];
+    float inv[6], Lrow[6], Lcol[6][6]; // Lcol[k][j] = L[j][k] (j > k), the same in every lane
+    // straight-line code (a failed pivot is remembered, the result discarded at the end), so that the
+    // loads and broadcasts of the steps can be scheduled together
+    bool ok = true;
+#pragma unroll
+    for (int k = 0; k < 6; k++) {
+        const float d = bcast_f(A[k], k); // A[k][k] less its k subtractions
+        ok = ok && d > 0.f;
+        inv[k] = __builtin_amdgcn_rsqf(d);
+        const float lk = A[k] * inv[k]; // L[r][k] in lanes r > k
+        Lrow[k] = lk;
+#pragma unroll
+        for (int j = k + 1; j < 6; j++)
+            Lcol[k][j] = bcast_f(lk, j);
+#pragma unroll
+        for (int j = k + 1; j < 6; j++) {
+            const float u = A[j] - lk * Lcol[k][j];
+            A[j] = j <= r ? u : A[j];
+        }
+    }
+    float y[6];
+#pragma unroll
+    for (int k = 0; k < 6; k++) {
+        y[k] = bcast_f(b, k) * inv[k];
+        const float u = b - Lrow[k] * y[k];
+        b = r > k ? u : b;
+    }
+#pragma unroll
+    for (int i = 5; i >= 0; i--) {
+        float v = y[i];
+#pragma unroll
+        for (int j = i + 1; j < 6; j++)
+            v -= Lcol[i][j] * x[j];
+        x[i] = v * inv[i];
+    }
+#pragma unroll
     for (int i = 0; i < 6; i++)
-        for (int j = i; j < 6; j++) {
-            A[i][j] = vals[c++];
-            A[j][i] = A[i][j];
-        }
-    for (int j = 0; j < 6; j++) {
-        double dj = A[j][j];
-        for (int k = 0; k < j; k++)
-            dj -= L[j][k] * L[j][k] * D[k];
-        if (!(dj > 0.0)) {
-            for (int i = 0; i < 6; i++)
-                x[i] = 0.f;
-            return;
-        }
-        D[j] = dj;
-        rD[j] = rcp_f64(dj);
-        L[j][j] = 1.0;
-        for (int i = j + 1; i < 6; i++) {
-            double v = A[i][j];
-            for (int k = 0; k < j; k++)
-                v -= L[i][k] * L[j][k] * D[k];
-            L[i][j] = v * rD[j];
+        x[i] = ok ? x[i] : 0.f;
+}
+
+// 6x6 LDL^T in FP64 (no pivoting: J^T J is symmetric positive semi-definite) across the lanes of one
+// wave, as solve_f32_wave; x = 0 on failure.
+__device__ void solve_f64_wave(const double* vals, float x[6])
+{
+    const int r = min((int) (threadIdx.x & 63), 5);
+    double A[6];
+#pragma unroll
+    for (int c = 0; c < 6; c++)
+        A[c] = vals[6 + tri6(min(r, c), max(r, c))];
+    double b = vals[r];
+    double D[6], rD[6], Lrow[6], Lcol[6][6];
+    bool ok = true; // as in solve_f32_wave
+#pragma unroll
+    for (int k = 0; k < 6; k++) {
+        const double d = bcast_d(A[k], k);
+        ok = ok && d > 0.0;
+        D[k] = d;
+        rD[k] = rcp_f64(d);
+        const double lk = A[k] * rD[k];
+        Lrow[k] = lk;
+#pragma unroll
+        for (int j = k + 1; j < 6; j++)
+            Lcol[k][j] = bcast_d(lk, j);
+#pragma unroll
+        for (int j = k + 1; j < 6; j++) {
+            const double u = A[j] - lk * Lcol[k][j] * D[k];
+            A[j] = j <= r ? u : A[j];
         }
     }
     double y[6], z[6];
-    for (int i = 0; i < 6; i++) {
-        double v = vals[i];
-        for (int k = 0; k < i; k++)
-            v -= L[i][k] * y[k];
-        y[i] = v;
+#pragma unroll
+    for (int k = 0; k < 6; k++) {
+        y[k] = bcast_d(b, k);
+        const double u = b - Lrow[k] * y[k];
+        b = r > k ? u : b;
     }
+#pragma unroll
     for (int i = 0; i < 6; i++)
         y[i] *= rD[i];
+#pragma unroll
     for (int i = 5; i >= 0; i--) {
         double v = y[i];
+#pragma unroll
         for (int k = i + 1; k < 6; k++)
-            v -= L[k][i] * z[k];
+            v -= Lcol[i][k] * z[k];
         z[i] = v;
     }
+#pragma unroll
     for (int i = 0; i < 6; i++)
-        x[i] = (float) z[i];
+        x[i] = ok ? (float) z[i] : 0.f;
 }
 
-// math::exp (math_util_impl.hpp:117-174) for the twist a = [v; w], with the reference's formulas
-// (quaternion of the half angle; V = I + c1 W + c2 W^2) over the fast operations above
-__device__ Iso se3_exp(const float a[6])
+// T <- exp(a) T across the lanes of one wave, for the twist a = [v; w] with the reference's formulas
+// (math_util_impl.hpp:117-174: quaternion of the half angle; V = I + c1 W + c2 W^2): lane e < 12 writes
+// entry e of the row-major 3x4 pose (row e >> 2; column e & 3, the last one the translation).  Every
+// lane reads the old pose before any lane writes it (one wave, in order).
+__device__ void exp_update_wave(const float a[6], float* T12)
 {
+    const int e = (int) (threadIdx.x & 63);
+    const int i = min(e >> 2, 2), j = e & 3;
     const float w[3] = {a[3], a[4], a[5]};
     const float theta_sq = dot3(w, w);
     const float theta = __builtin_amdgcn_sqrtf(theta_sq);
     const float half_theta = 0.5f * theta;
     float imag, re, s_t = 0.f, c_t = 1.f;
-    if (theta < 1e-10f) {
+    const bool small = theta < 1e-10f;
+    if (small) {
         const float t4 = theta_sq * theta_sq;
         imag = 0.5f - (1.0f / 48.0f) * theta_sq + (1.0f / 3840.0f) * t4;
         re = 1.f - 0.5f * theta_sq + (1.f / 384.f) * t4;
@@ -560,43 +601,44 @@ __device__ Iso se3_exp(const float a[6])
     }
     float q[4] = {re, imag * w[0], imag * w[1], imag * w[2]};
     const float rq = __builtin_amdgcn_rsqf(q[0] * q[0] + (q[1] * q[1] + (q[2] * q[2] + q[3] * q[3])));
-    for (int i = 0; i < 4; i++)
-        q[i] = q[i] * rq;
+    for (int k = 0; k < 4; k++)
+        q[k] = q[k] * rq;
     const float qw = q[0], qx = q[1], qy = q[2], qz = q[3];
     const float tx = 2.f * qx, ty = 2.f * qy, tz = 2.f * qz;
     const float twx = tx * qw, twy = ty * qw, twz = tz * qw;
     const float txx = tx * qx, txy = ty * qx, txz = tz * qx;
     const float tyy = ty * qy, tyz = tz * qy, tzz = tz * qz;
-    Iso E;
-    E.R[0][0] = 1.f - (tyy + tzz);
-    E.R[0][1] = txy - twz;
-    E.R[0][2] = txz + twy;
-    E.R[1][0] = txy + twz;
-    E.R[1][1] = 1.f - (txx + tzz);
-    E.R[1][2] = tyz - twx;
-    E.R[2][0] = txz - twy;
-    E.R[2][1] = tyz + twx;
-    E.R[2][2] = 1.f - (txx + tyy);
-    float V[3][3];
-    if (theta < 1e-10f) {
-        for (int i = 0; i < 3; i++)
-            for (int j = 0; j < 3; j++)
-                V[i][j] = E.R[i][j];
+    // row i of E.R
+    float Ri[3];
+    Ri[0] = i == 0 ? 1.f - (tyy + tzz) : (i == 1 ? txy + twz : txz - twy);
+    Ri[1] = i == 0 ? txy - twz : (i == 1 ? 1.f - (txx + tzz) : tyz + twx);
+    Ri[2] = i == 0 ? txz + twy : (i == 1 ? tyz - twx : 1.f - (txx + tyy));
+    // E.t[i] = V[i] . v, V = R for small angles, else I + c1 W + c2 W^2
+    float Vi[3];
+    if (small) {
+        for (int c = 0; c < 3; c++)
+            Vi[c] = Ri[c];
     }
     else {
         const float W[3][3] = {{0.f, -w[2], w[1]}, {w[2], 0.f, -w[0]}, {-w[1], w[0], 0.f}};
+        const float Wi[3] = {i == 0 ? W[0][0] : (i == 1 ? W[1][0] : W[2][0]), i == 0 ? W[0][1] : (i == 1 ? W[1][1] : W[2][1]),
+                             i == 0 ? W[0][2] : (i == 1 ? W[1][2] : W[2][2])};
         const float r2 = fast_rcp(theta_sq);
         const float c1 = (1.f - c_t) * r2;
         const float c2 = (theta - s_t) * (r2 * fast_rcp(theta));
-        for (int i = 0; i < 3; i++)
-            for (int j = 0; j < 3; j++) {
-                const float w2 = W[i][0] * W[0][j] + (W[i][1] * W[1][j] + W[i][2] * W[2][j]);
-                V[i][j] = ((i == j ? 1.f : 0.f) + c1 * W[i][j]) + c2 * w2;
-            }
+        for (int c = 0; c < 3; c++) {
+            const float w2 = Wi[0] * W[0][c] + (Wi[1] * W[1][c] + Wi[2] * W[2][c]);
+            Vi[c] = ((i == c ? 1.f : 0.f) + c1 * Wi[c]) + c2 * w2;
+        }
     }
-    for (int i = 0; i < 3; i++)
-        E.t[i] = V[i][0] * a[0] + (V[i][1] * a[1] + V[i][2] * a[2]);
-    return E;
+    const float ti = Vi[0] * a[0] + (Vi[1] * a[1] + Vi[2] * a[2]);
+    // column j of the old pose (R column, or t for j = 3)
+    const float B0 = T12[j], B1 = T12[4 + j], B2 = T12[8 + j];
+    float out = Ri[0] * B0 + (Ri[1] * B1 + Ri[2] * B2);
+    if (j == 3)
+        out = out + ti;
+    if (e < 12)
+        T12[e] = out;
 }
 
 // checkPoseKernel (icp.cpp:227-246) on the last reduction: fail if the RMS residual exceeds 2 cm or
@@ -865,20 +907,21 @@ __device__ void gather_sums(const RowLds& r, long long* lb, long long bv, IcpSlo
 // T <- exp(x) T and raises the level's convergence flag (icp.cpp:15-41, 208-223).  Ends with a
 // barrier.
 template<bool Comp>
-__device__ void solve_step(const IcpArgs& a, IcpSlot& cur)
+__device__ __forceinline__ void solve_step(const IcpArgs& a, IcpSlot& cur)
 {
-    if (threadIdx.x == 0) {
+    if (threadIdx.x < 64) {
         float x[6];
         if (Comp)
-            solve_f64(cur.sums_d + 1, x);
+            solve_f64_wave(cur.sums_d + 1, x);
         else
-            solve_f32(cur.sums + 1, x);
-        const Iso Tn = iso_mul(se3_exp(x), pose_of(cur.T));
-        store_pose(Tn, cur.T);
-        const float xn = sqrtf(x[0] * x[0] + (x[1] * x[1] + (x[2] * x[2] + (x[3] * x[3] + (x[4] * x[4] + x[5] * x[5])))));
-        if (xn < a.icp_th)
-            cur.done[cur.level] = 1;
-        cur.iterations += 1;
+            solve_f32_wave(cur.sums + 1, x);
+        exp_update_wave(x, cur.T);
+        if (threadIdx.x == 0) {
+            const float xn = sqrtf(x[0] * x[0] + (x[1] * x[1] + (x[2] * x[2] + (x[3] * x[3] + (x[4] * x[4] + x[5] * x[5])))));
+            if (xn < a.icp_th)
+                cur.done[cur.level] = 1;
+            cur.iterations += 1;
+        }
     }
     __syncthreads();
 }
