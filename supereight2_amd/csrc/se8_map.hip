@@ -125,11 +125,9 @@ struct VoxelSample {
 // each wait covers only the loads it needs; `live` = false marks a lane with no block.
 template<class Sensor>
 __device__ __forceinline__ VoxelSample sample_voxel(const Iso& T_CV, const Sensor& sensor, const float* depth,
-                                                    uint64_t key, int vx, int vy, int vz, bool live)
+                                                    int4 c, int vx, int vy, int vz, bool live)
 {
-    int c[3];
-    key_to_coord(key, c);
-    const float v[3] = {(float) (c[0] + vx), (float) (c[1] + vy), (float) (c[2] + vz)};
+    const float v[3] = {(float) (c.x + vx), (float) (c.y + vy), (float) (c.z + vz)};
     float pC[3];
     iso_apply(T_CV, v, pC);
     VoxelSample o;
@@ -143,8 +141,9 @@ __device__ __forceinline__ VoxelSample sample_voxel(const Iso& T_CV, const Senso
 }
 
 // Per voxel as singleres_tsdf_updater_impl.hpp:44-128.  A workgroup owns the list entries
-// blockIdx.x, blockIdx.x + gridDim.x, ...: it stages up to kUpdChunk of them (entry, key) in LDS
-// with one round of loads (stamping the block and parent timestamps at the same time), then walks
+// blockIdx.x, blockIdx.x + gridDim.x, ...: it stages up to kUpdChunk of them (entry, block
+// coordinates, each key decoded once by the thread that loads it) in LDS with one round of loads
+// (stamping the block and parent timestamps at the same time), then walks
 // them kUpdK blocks at a time as a two-stage pipeline: while a group is updated, the next group's
 // voxels and depth samples are already loading (its projections are computed first), so neither
 // load waits on the arithmetic it feeds.
@@ -160,7 +159,7 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(8, 8))) voi
     if (link)
         link_part(m, update_list, blockIdx.x * blockDim.x + threadIdx.x, gridDim.x * blockDim.x, ts);
     __shared__ int s_entry[kUpdChunk];
-    __shared__ uint64_t s_key[kUpdChunk];
+    __shared__ int4 s_coord[kUpdChunk];
     const Iso T_CV = fp->T_CV;
     const int n = m.counters[kCntUpdate];
     const int t = threadIdx.x;
@@ -175,7 +174,9 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(8, 8))) voi
             const int entry = update_list[blockIdx.x + (size_t) (c0 + t) * G];
             const int slot = entry & 0x7fffffff;
             s_entry[t] = entry;
-            s_key[t] = m.block_key[slot];
+            int c[3];
+            key_to_coord(m.block_key[slot], c);
+            s_coord[t] = make_int4(c[0], c[1], c[2], 0);
             // block timestamp and one-level parent propagation (propagator_impl.hpp:146-197); a
             // block new this frame has its parent stamped by link_part when the links are made here
             m.block_ts[slot] = ts;
@@ -196,7 +197,7 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(8, 8))) voi
             }
 #pragma unroll
             for (int k = 0; k < kUpdK; k++)
-                smp[k] = sample_voxel(T_CV, sensor, depth, s_key[min(j + k, cnt - 1)], vx, vy, vz, j + k < cnt);
+                smp[k] = sample_voxel(T_CV, sensor, depth, s_coord[min(j + k, cnt - 1)], vx, vy, vz, j + k < cnt);
         };
         auto update_group = [&](int i, VoxelRaw<Storage> v[kUpdK], const VoxelSample smp[kUpdK]) {
 #pragma unroll
