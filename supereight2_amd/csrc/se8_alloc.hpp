@@ -354,20 +354,39 @@ __device__ void carve_part(const MapView& m, const Sensor& sensor, const FramePo
             bi[j] = ((vox[j][2] >> 3) * side + (vox[j][1] >> 3)) * side + (vox[j][0] >> 3);
             slot[j] = m.block_dense[bi[j]];
         }
-        bool won[kBatch];
-        int cnt = 0, wins = 0;
+        // the claims of the batch are issued together and awaited once: which samples claim depends
+        // only on the lookups (an entry already claimed in this kernel is being created by its
+        // claimant), so every compare-and-swap is in flight before any result is read
+        bool need[kBatch];
+        int cnt = 0;
 #pragma unroll
         for (int j = 0; j < kBatch; j++) {
-            won[j] = false;
+            need[j] = false;
             if (inb[j] && slot[j] < 0) {
                 if (bi[j] != prev) {
                     cnt++;
-                    // an entry already claimed in this kernel is being created by its claimant
-                    if (slot[j] == -1)
-                        won[j] = atomicCAS(&m.block_dense[bi[j]], -1, kClaimed) == -1;
+                    need[j] = slot[j] == -1;
                 }
                 prev = bi[j];
             }
+        }
+        int cas[kBatch];
+#pragma unroll
+        for (int j = 0; j < kBatch; j++) {
+            cas[j] = 0;
+            if (need[j])
+                cas[j] = atomicCAS(&m.block_dense[bi[j]], -1, kClaimed);
+        }
+        // every result in its own register past this point, so no result is tested (and awaited)
+        // inside the branch that issued it
+#pragma unroll
+        for (int j = 0; j < kBatch; j++)
+            SE8_REG_BARRIER(cas[j]);
+        bool won[kBatch];
+        int wins = 0;
+#pragma unroll
+        for (int j = 0; j < kBatch; j++) {
+            won[j] = need[j] && cas[j] == -1;
             wins += won[j] ? 1 : 0;
         }
         const int cand0 = wave_reserve(cnt, &m.counters[kCntCandidates]);

@@ -24,6 +24,14 @@
 
 #define SE8_HD __host__ __device__ __forceinline__
 
+// Pins v to a register of its own at this point of the program (an empty asm statement that reads
+// and writes it), so that its uses stay after this point: the results of atomics issued in separate
+// branches are then awaited together here, not each inside the branch that issued it.  The host
+// build of the allocation kernels (the CPU bounds checks) defines it away.
+#ifndef SE8_REG_BARRIER
+#define SE8_REG_BARRIER(v) asm volatile("" : "+v"(v))
+#endif
+
 namespace se8 {
 
 // ----------------------------------------------------------------------------------------------

@@ -40,6 +40,7 @@
 #define __shared__ static
 #define __launch_bounds__(...)
 #define __builtin_amdgcn_rcpf(x) (1.0f / (x))
+#define SE8_REG_BARRIER(v) ((void) (v))
 
 struct dim3 {
     unsigned int x, y, z;
