@@ -973,8 +973,7 @@ struct se8_tracker {
     se8_pinhole sensor;
     se8_tracker_config cfg;
     se8::DeviceBuffer vert_lvl[SE8_TRACKER_MAX_LEVELS], norm_lvl[SE8_TRACKER_MAX_LEVELS]; // float4 per pixel, per level
-    se8::DeviceBuffer partials; // per-workgroup reduction partials (plain path)
-    se8::DeviceBuffer bins;     // fixed-point partial sums (compensated path), cleared by each track call
+    se8::DeviceBuffer bins;     // fixed-point partial sums of the Gauss-Newton launches, cleared by each track call
     se8::DeviceBuffer state;    // counters, flags, final sums
     se8::DeviceBuffer ref_points, ref_normals; // used by se8_track_raycast
     se8::DeviceBuffer adj_poses, adj_partials, adj_out; // batched cost/gradient evaluation

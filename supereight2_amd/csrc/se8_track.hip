@@ -33,17 +33,16 @@
 namespace se8 {
 
 constexpr int kIcpThreads = 256;
-// Cap on the Gauss-Newton grid: a workgroup reads at most kIcpMaxWg / 32 partial rows per thread.
-// Levels larger than kIcpMaxWg tiles of kIcpThreads * 4 pixels loop over tiles.
+// Cap on the Gauss-Newton grid: levels larger than kIcpMaxWg tiles of kIcpThreads * 4 pixels loop
+// over tiles.
 constexpr int kIcpMaxWg = 320;
-constexpr int kPartialStride = kIcpMaxWg * 64; // floats per partial-row buffer
 constexpr int kPyrThreads = 256;
 constexpr int kMaxLevels = SE8_TRACKER_MAX_LEVELS;
-// Compensated path: workgroup i adds its partial row, in 64-bit fixed point, into bin i % kBins of
-// the launch's round (rounds rotate over three buffers: launch k adds into round k % 3, reads round
-// (k + 2) % 3 and clears round (k + 1) % 3).  Integer addition is exact and associative, so the sum
-// does not depend on the order of the atomics, and the next launch reads kBins rows instead of one
-// row per workgroup.
+// Hand-over between Gauss-Newton launches: workgroup i adds its partial row, in 64-bit fixed point,
+// into bin i % kBins of the launch's round (rounds rotate over three buffers: launch k adds into
+// round k % 3, reads round (k + 2) % 3 and clears round (k + 1) % 3).  Integer addition is exact and
+// associative, so the sum does not depend on the order of the atomics, and every workgroup of the
+// next launch reads kBins rows (one 64-bit word per thread) instead of one row per workgroup.
 constexpr int kBins = 8;
 static_assert(kBins * 32 == 256, "one bin entry per thread of a 256-thread workgroup");
 // Level-0 tile side of a k_pyramid workgroup: 16 (2^4), or 2^(L-1) for deeper pyramids, so that every
@@ -90,8 +89,7 @@ struct IcpArgs {
     const float4* norm;    // level normal map, one float4 {x, y, z, 0} per pixel
     const float* ref_pts;  // full-res raycast points (w*h*3, the caller's layout)
     const float* ref_nrm;  // full-res raycast normals
-    float* partials;       // two buffers of kPartialStride floats, by launch parity (plain fp32 path)
-    long long* bins;       // compensated path: three rounds of kBins x 32 fixed-point sums
+    long long* bins;       // three rounds of kBins x 32 fixed-point sums, by launch index mod 3
     IcpShared* sh;
     Iso T_ref_inv;         // batched adjoint only; the tracker reads sh->T_ref_inv
     Cam cam;               // full-resolution sensor (projection into the reference maps)
@@ -100,7 +98,6 @@ struct IcpArgs {
     float dist_th, normal_th, icp_th, track_th;
     int level;
     int k;                 // launch index within the call
-    int prev_rows;         // partial rows launch k - 1 wrote if it ran its pixel pass (its grid size)
     float* T_out_dev;      // k_icp_finish: the tracked pose also goes here (16 floats) when set
     int* ok_dev;           // k_icp_finish: the check result also goes here when set
 };
@@ -122,7 +119,7 @@ struct PyrArgs {
     int tile_log2;      // log2 of the level-0 tile side
     int tiles_x;        // level-0 tiles per row
     IcpShared* init;    // when not null, thread 0 writes the state that the first launch reads
-    long long* bins;    // when not null (compensated tracker), cleared for the first launch
+    long long* bins;    // when not null, the round the first launch reads is cleared
     Iso T;              // start pose, unless T16_dev is set
     const float* T16_dev;
 };
@@ -775,34 +772,10 @@ __device__ __forceinline__ void load_tile(const IcpArgs& a, int tile, int n, flo
     }
 }
 
-// Solving the previous launch's pass is split so that its memory round trips overlap: load_rows
-// (plain path) or load_bins (compensated path) issues the loads of the previous launch's partial
-// sums without waiting, slot_word / put_slot_word bring the previous slot into LDS, rows_to_lds adds
-// each thread's rows, gather_sums adds the groups (or bins) and solve_step solves.  Identical in
-// every workgroup.  On the plain path thread t owns terms 4q..4q+3 (q = t & 7) of rows b = g,
-// g + 32, ... (g = t >> 3); it adds its rows in increasing b; then thread k < 32 adds the 32 group
-// sums of term k in increasing g.  The rows were written by the previous kernel, so plain loads see
-// them.
-constexpr int kRowsPerThread = kIcpMaxWg / 32;
-
-struct RowFrag {
-    float4 sv[kRowsPerThread];
-};
-
-// Issues the loads of the rows a thread owns among the nrows rows written (nrows <= kIcpMaxWg, the
-// buffer's capacity) without waiting; the other entries are zero and are masked when they are added.
-__device__ __forceinline__ void load_rows(const float* rows, int nrows, RowFrag& f)
-{
-    const int t = threadIdx.x, q = t & 7, g = t >> 3;
-#pragma unroll
-    for (int i = 0; i < kRowsPerThread; i++) {
-        const int b = g + 32 * i;
-        f.sv[i] = make_float4(0.f, 0.f, 0.f, 0.f);
-        if (b < nrows)
-            f.sv[i] = reinterpret_cast<const float4*>(rows + b * 64)[q];
-    }
-}
-
+// Solving the previous launch's pass is split so that its memory round trips overlap: load_bins
+// issues the load of the previous launch's fixed-point sums without waiting, slot_word /
+// put_slot_word bring the previous slot into LDS, gather_sums adds the bins and solve_step solves.
+// Identical in every workgroup.
 // The previous launch's slot is copied to LDS as whole 32-bit words, one per thread, so the copy
 // costs a single load per thread: slot_word issues it, put_slot_word stores it (the caller
 // synchronises before reading `cur`).
@@ -820,30 +793,6 @@ __device__ __forceinline__ void put_slot_word(IcpSlot& cur, uint32_t w)
     const int t = threadIdx.x;
     if (t < kSlotWords)
         reinterpret_cast<uint32_t*>(&cur)[t] = w;
-}
-
-// Per-group partial sums of the previous launch's rows (LDS), between rows_to_lds and gather_sums.
-struct RowLds {
-    float f[32][33];
-};
-
-// Adds this thread's rows (those below a.prev_rows) and stores the group sums in LDS.  Waits only
-// for the row loads, so it runs before any later loads are issued.  The caller synchronises.
-__device__ __forceinline__ void rows_to_lds(const IcpArgs& a, const RowFrag& f, RowLds& r)
-{
-    const int t = threadIdx.x, q = t & 7, g = t >> 3;
-    const int nrows = a.prev_rows;
-    float acc[4] = {0.f, 0.f, 0.f, 0.f};
-#pragma unroll
-    for (int i = 0; i < kRowsPerThread; i++) {
-        const bool in = g + 32 * i < nrows;
-        acc[0] += in ? f.sv[i].x : 0.f;
-        acc[1] += in ? f.sv[i].y : 0.f;
-        acc[2] += in ? f.sv[i].z : 0.f;
-        acc[3] += in ? f.sv[i].w : 0.f;
-    }
-    for (int k = 0; k < 4; k++)
-        r.f[g][4 * q + k] = acc[k];
 }
 
 // Fixed-point scale of term t of a row: 2^40 for e^2 and e J (t < 7), 2^24 for J^T J (t < 28), 1 for
@@ -872,33 +821,21 @@ __device__ __forceinline__ long long load_bins(const IcpArgs& a)
     return v;
 }
 
-// Sums of the previous launch's pass into cur.sums / cur.sums_d: the plain path adds the group sums
-// of the rows (rows_to_lds) in a fixed order in fp32, the compensated path adds the kBins fixed-point
-// entries exactly and converts the totals to FP64.  Ends with a barrier.
-template<bool Comp>
-__device__ void gather_sums(const RowLds& r, long long* lb, long long bv, IcpSlot& cur)
+// Sums of the previous launch's pass into cur.sums / cur.sums_d: the kBins fixed-point entries of
+// each term added exactly, the total converted to FP64 (and rounded to fp32 for the plain solve).
+// Ends with a barrier.
+__device__ void gather_sums(long long* lb, long long bv, IcpSlot& cur)
 {
     const int t = threadIdx.x;
-    if (Comp) {
-        lb[t] = bv;
-        __syncthreads();
-    }
+    lb[t] = bv;
+    __syncthreads();
     if (t < 32) {
-        if (Comp) {
-            long long acc = 0;
-            for (int b = 0; b < kBins; b++)
-                acc += lb[b * 32 + t];
-            const double v = (double) acc / fixed_scale(t);
-            cur.sums_d[t] = v;
-            cur.sums[t] = (float) v;
-        }
-        else {
-            float acc = r.f[0][t];
-            for (int j = 1; j < 32; j++)
-                acc += r.f[j][t];
-            cur.sums[t] = acc;
-            cur.sums_d[t] = (double) acc;
-        }
+        long long acc = 0;
+        for (int b = 0; b < kBins; b++)
+            acc += lb[b * 32 + t];
+        const double v = (double) acc / fixed_scale(t);
+        cur.sums_d[t] = v;
+        cur.sums[t] = (float) v;
     }
     __syncthreads();
 }
@@ -927,47 +864,35 @@ __device__ __forceinline__ void solve_step(const IcpArgs& a, IcpSlot& cur)
 }
 
 // Launch k of the Gauss-Newton sequence of Tracker::track (tracker_impl.hpp:77-108): solve the
-// pass of launch k - 1 (slot_word, load_rows + rows_to_lds or load_bins, gather_sums, solve_step),
-// then, unless this level has
-// converged, the pixel pass of iteration k at this level: trackKernel (icp_impl.hpp:16-102) fused with the 32-term
-// reduceKernel sums (icp.cpp:45-204) over tiles of PPT pixels per thread, a halving wave
-// reduction and an LDS reduction to one partial row per workgroup.  Workgroup 0 publishes the
-// pose and flags for launch k + 1.  No workgroup waits on another, so no device-wide fence or
-// atomic is needed; the kernel boundary orders the hand-over.
+// pass of launch k - 1 (slot_word, load_bins, gather_sums, solve_step), then, unless this level has
+// converged, the pixel pass of iteration k at this level: trackKernel (icp_impl.hpp:16-102) fused
+// with the 32-term reduceKernel sums (icp.cpp:45-204) over tiles of PPT pixels per thread, a halving
+// wave reduction and an LDS reduction to one partial row per workgroup, added into the launch's
+// fixed-point bins.  Workgroup 0 publishes the pose and flags for launch k + 1.  No workgroup waits
+// on another, so no device-wide fence is needed; the kernel boundary orders the hand-over.
 template<bool Comp, int PPT>
 __global__ __launch_bounds__(kIcpThreads) void k_icp_iter(IcpArgs a)
 {
     __shared__ IcpSlot cur;
-    __shared__ RowLds rl;
-    __shared__ long long lb[Comp ? kBins * 32 : 1];
+    __shared__ long long lb[kBins * 32];
     __shared__ float red[kIcpThreads / 64][64];
     const IcpSlot* prev = &a.sh->slot[(a.k + 1) & 1];
     const int n = a.w * a.h;
-    // issue order = wait order: the slot word first, then the tile and the partial rows (or bins);
-    // the slot is stored (waiting for it alone) and read before the rows are awaited, and the rows
-    // are added before any later load is issued, so that waiting for them waits for nothing else
+    // issue order = wait order: the slot word first, then the tile and the bins; the slot is stored
+    // (waiting for it alone) and read before the bins are awaited
     const uint32_t sw = slot_word(prev);
     float4 v4[PPT], n4[PPT];
     load_tile<PPT>(a, blockIdx.x, n, v4, n4);
-    RowFrag frag;
-    long long bv = 0;
-    if (Comp)
-        bv = load_bins(a);
-    else
-        load_rows(a.partials + ((a.k + 1) & 1) * kPartialStride, a.prev_rows, frag);
+    const long long bv = load_bins(a);
     put_slot_word(cur, sw);
     __syncthreads();
     // a launch after this level converged with nothing left to solve only hands the slot on; it
-    // does not wait for the rows (their loads were issued ahead, in case they were needed)
+    // does not wait for the bins (their load was issued ahead, in case it was needed)
     const bool noop = cur.did_pixels == 0 && cur.done[a.level] != 0;
     bool run = false;
     if (!noop) {
         if (cur.did_pixels) {
-            if (!Comp) {
-                rows_to_lds(a, frag, rl);
-                __syncthreads();
-            }
-            gather_sums<Comp>(rl, lb, bv, cur);
+            gather_sums(lb, bv, cur);
             solve_step<Comp>(a, cur);
         }
         run = cur.done[a.level] == 0;
@@ -1020,15 +945,9 @@ __global__ __launch_bounds__(kIcpThreads) void k_icp_iter(IcpArgs a)
             float ss = red[0][k], cc = red[0][32 + k];
             for (int w = 1; w < kIcpThreads / 64; w++)
                 pair_add<Comp>(ss, cc, red[w][k], red[w][32 + k]);
-            if (Comp) {
-                unsigned long long* bin = reinterpret_cast<unsigned long long*>(a.bins)
-                    + (a.k % 3) * (kBins * 32) + (blockIdx.x % kBins) * 32;
-                atomicAdd(bin + k, to_fixed((double) ss + (double) cc, k));
-            }
-            else {
-                float* row = a.partials + (a.k & 1) * kPartialStride + blockIdx.x * 64;
-                row[k] = ss;
-            }
+            unsigned long long* bin = reinterpret_cast<unsigned long long*>(a.bins)
+                + (a.k % 3) * (kBins * 32) + (blockIdx.x % kBins) * 32;
+            atomicAdd(bin + k, to_fixed(Comp ? (double) ss + (double) cc : (double) ss, k));
         }
     }
     if (blockIdx.x == 0) {
@@ -1057,23 +976,13 @@ template<bool Comp>
 __global__ __launch_bounds__(kIcpThreads) void k_icp_finish(IcpArgs a)
 {
     __shared__ IcpSlot cur;
-    __shared__ RowLds rl;
-    __shared__ long long lb[Comp ? kBins * 32 : 1];
+    __shared__ long long lb[kBins * 32];
     const uint32_t sw = slot_word(&a.sh->slot[(a.k + 1) & 1]);
-    RowFrag frag;
-    long long bv = 0;
-    if (Comp)
-        bv = load_bins(a);
-    else
-        load_rows(a.partials + ((a.k + 1) & 1) * kPartialStride, a.prev_rows, frag);
+    const long long bv = load_bins(a);
     put_slot_word(cur, sw);
     __syncthreads();
     if (cur.did_pixels) {
-        if (!Comp) {
-            rows_to_lds(a, frag, rl);
-            __syncthreads();
-        }
-        gather_sums<Comp>(rl, lb, bv, cur);
+        gather_sums(lb, bv, cur);
         solve_step<Comp>(a, cur);
     }
     if (threadIdx.x == 0) {
@@ -1161,13 +1070,12 @@ static int enqueue_track(se8_tracker* tr, const float* depth, const float* T16_h
     harvest_tracking(tr);
     if (timed)
         SE8_HIP_CHECK(hipEventRecord(tr->ev[0], s));
-    long long* bins = tr->cfg.compensated ? (long long*) tr->bins.ptr : nullptr;
+    long long* bins = (long long*) tr->bins.ptr;
     launch_pyramid(tr, depth, tr->cfg.num_levels, sh, T16_host ? iso_from16(T16_host) : Iso{}, T_in_dev, s, bins);
     IcpArgs a{};
     a.bins = bins;
     a.ref_pts = ref_pts;
     a.ref_nrm = ref_nrm;
-    a.partials = (float*) tr->partials.ptr;
     a.sh = sh;
     a.cam = make_cam(P, 1, false);
     a.dist_th = tr->cfg.dist_threshold;
@@ -1191,7 +1099,6 @@ static int enqueue_track(se8_tracker* tr, const float* depth, const float* T16_h
                 launch_icp_iter<true>(tr->icp_ppt[level], nwg, a, s);
             else
                 launch_icp_iter<false>(tr->icp_ppt[level], nwg, a, s);
-            a.prev_rows = nwg;
         }
     }
     if (levels)
@@ -1405,7 +1312,6 @@ extern "C" int se8_tracker_create(const se8_pinhole* sensor, const se8_tracker_c
         t->icp_tiles[l] = (int) std::max<size_t>(1, (n + kIcpThreads * ppt - 1) / (kIcpThreads * ppt));
         t->num_wg[l] = std::min(t->icp_tiles[l], kIcpMaxWg);
     }
-    rc = rc ? rc : dev_alloc(t->partials, sizeof(float) * 2 * kPartialStride);
     rc = rc ? rc : dev_alloc(t->bins, sizeof(long long) * 3 * kBins * 32);
     rc = rc ? rc : dev_alloc(t->state, sizeof(IcpShared));
     if (rc == SE8_OK && hipHostMalloc(&t->host_state, sizeof(IcpState), hipHostMallocDefault) != hipSuccess) {
@@ -1432,7 +1338,6 @@ extern "C" int se8_tracker_destroy(se8_tracker* t)
         dev_free(t->vert_lvl[l]);
         dev_free(t->norm_lvl[l]);
     }
-    dev_free(t->partials);
     dev_free(t->bins);
     dev_free(t->state);
     if (t->host_state)
@@ -1538,7 +1443,6 @@ extern "C" int se8_icp_cost_grad_batched(se8_tracker* t, const float* depth_dev,
     a.norm = (const float4*) t->norm_lvl[0].ptr;
     a.ref_pts = ref_points_dev;
     a.ref_nrm = ref_normals_dev;
-    a.partials = nullptr;
     a.sh = nullptr;
     a.T_ref_inv = iso_inv(iso_from16(T_ref16));
     a.cam = make_cam(P, 1, false);
