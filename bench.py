@@ -551,6 +551,7 @@ def fp64_errors(depth_host, gt, runs, sensor, mu, log_every=50, raycast_every=10
       estimate, integrate, raycast): the poses of the two runs, every frame's vertex map and the
       final TSDF, which then also carry the two trajectories' difference."""
     from oracle import oracle as O
+    from supereight2_amd import synth
 
     O.set_threads(os.cpu_count() or 1)
     pin = O.pinhole(sensor.width, sensor.height, sensor.fx, sensor.fy, sensor.cx, sensor.cy, sensor.near_plane,
@@ -593,7 +594,7 @@ def fp64_errors(depth_host, gt, runs, sensor, mu, log_every=50, raycast_every=10
     for name, r in runs.items():
         pg = r["poses"]
         dt = [float(np.linalg.norm(np.asarray(a)[:3, 3] - b[:3, 3])) for a, b in zip(pg, poses64)]
-        dr = [float(np.degrees(np.arccos(np.clip((np.trace(np.asarray(a)[:3, :3].T @ b[:3, :3]) - 1) / 2, -1, 1))))
+        dr = [synth.rotation_angle_deg(np.asarray(a, np.float64)[:3, :3].T @ np.asarray(b, np.float64)[:3, :3])
               for a, b in zip(pg, poses64)]
         res[name]["own_tracking"] = dict(_tsdf_compare(r["blocks_dump"], dump64, mu), **ves[name].out(),
                                          pose_max_m=float(max(dt)), pose_max_deg=float(max(dr)),

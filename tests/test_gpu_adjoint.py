@@ -8,6 +8,7 @@ import numpy as np
 import pytest
 
 from oracle import oracle as O
+from supereight2_amd import synth
 from tests.helpers import CAM, circle_frames
 
 pytestmark = pytest.mark.gpu
@@ -159,7 +160,9 @@ def _gauss_newton(s, T, steps):
 
 
 def _rot_deg(A, B):
-    return np.degrees(np.arccos(np.clip((np.trace(A[:3, :3].T @ B[:3, :3]) - 1) / 2, -1, 1)))
+    # the angle of A^T B from its antisymmetric part (synth.rotation_angle_deg): the arccos of the
+    # trace cannot resolve 0.01 degrees between a float32 pose and a float64 one
+    return synth.rotation_angle_deg(np.asarray(A, np.float64)[:3, :3].T @ np.asarray(B, np.float64)[:3, :3])
 
 
 def test_batched_gauss_newton_refinement_converges(scene):

@@ -17,6 +17,7 @@ import numpy as np
 import pytest
 
 from oracle import oracle as O
+from supereight2_amd import synth
 
 pytestmark = pytest.mark.gpu
 
@@ -90,8 +91,7 @@ def _compare_maps(gmap, omap):
 
 
 def _rot_angle_deg(Ra, Rb):
-    c = (np.trace(Ra.T @ Rb) - 1) / 2
-    return float(np.degrees(np.arccos(np.clip(c, -1, 1))))
+    return synth.rotation_angle_deg(np.asarray(Ra, np.float64).T @ np.asarray(Rb, np.float64))
 
 
 def test_icl_sequence_end_to_end():
