@@ -42,9 +42,11 @@ constexpr int kMaxLevels = SE8_TRACKER_MAX_LEVELS;
 // into bin i % kBins of the launch's round (rounds rotate over three buffers: launch k adds into
 // round k % 3, reads round (k + 2) % 3 and clears round (k + 1) % 3).  Integer addition is exact and
 // associative, so the sum does not depend on the order of the atomics, and every workgroup of the
-// next launch reads kBins rows (one 64-bit word per thread) instead of one row per workgroup.
-constexpr int kBins = 8;
-static_assert(kBins * 32 == 256, "one bin entry per thread of a 256-thread workgroup");
+// next launch reads kBins rows (kBinWords 64-bit words per thread) instead of one row per
+// workgroup.  More bins spread the atomics of a launch over more addresses.
+constexpr int kBins = 16;
+constexpr int kBinWords = kBins * 32 / 256;
+static_assert(kBinWords * 256 == kBins * 32, "whole bin entries per thread of a 256-thread workgroup");
 // Level-0 tile side of a k_pyramid workgroup: 16 (2^4), or 2^(L-1) for deeper pyramids, so that every
 // level keeps at least one tile pixel.  Small tiles give many light workgroups, which hide the latency
 // of the per-pixel division chains better than few heavy ones.
@@ -810,28 +812,43 @@ __device__ __forceinline__ unsigned long long to_fixed(double v, int t)
     return (unsigned long long) (long long) rint(x);
 }
 
-// The previous launch's round, one entry per thread (bin t >> 5, term t & 31); workgroup 0 also
-// clears the round the next launch adds into.  Issues the load without waiting.
-__device__ __forceinline__ long long load_bins(const IcpArgs& a)
+// The previous launch's round, kBinWords entries per thread (entry t + 256 i: bin (t >> 5) + 8 i,
+// term t & 31); workgroup 0 also clears the round the next launch adds into.  Issues the loads
+// without waiting.
+struct BinWords {
+    long long v[kBinWords];
+};
+
+__device__ __forceinline__ BinWords load_bins(const IcpArgs& a)
 {
     const int t = threadIdx.x;
-    const long long v = a.bins[((a.k + 2) % 3) * (kBins * 32) + t];
-    if (blockIdx.x == 0)
-        a.bins[((a.k + 1) % 3) * (kBins * 32) + t] = 0;
-    return v;
+    BinWords w;
+#pragma unroll
+    for (int i = 0; i < kBinWords; i++)
+        w.v[i] = a.bins[((a.k + 2) % 3) * (kBins * 32) + t + 256 * i];
+    if (blockIdx.x == 0) {
+#pragma unroll
+        for (int i = 0; i < kBinWords; i++)
+            a.bins[((a.k + 1) % 3) * (kBins * 32) + t + 256 * i] = 0;
+    }
+    return w;
 }
 
 // Sums of the previous launch's pass into cur.sums / cur.sums_d: the kBins fixed-point entries of
 // each term added exactly, the total converted to FP64 (and rounded to fp32 for the plain solve).
 // Ends with a barrier.
-__device__ void gather_sums(long long* lb, long long bv, IcpSlot& cur)
+__device__ void gather_sums(long long* lb, const BinWords& bw, IcpSlot& cur)
 {
     const int t = threadIdx.x;
-    lb[t] = bv;
+    long long own = 0;
+#pragma unroll
+    for (int i = 0; i < kBinWords; i++)
+        own += bw.v[i];
+    lb[t] = own;
     __syncthreads();
     if (t < 32) {
         long long acc = 0;
-        for (int b = 0; b < kBins; b++)
+        for (int b = 0; b < 8; b++)
             acc += lb[b * 32 + t];
         const double v = (double) acc / fixed_scale(t);
         cur.sums_d[t] = v;
@@ -874,7 +891,7 @@ template<bool Comp, int PPT>
 __global__ __launch_bounds__(kIcpThreads) void k_icp_iter(IcpArgs a)
 {
     __shared__ IcpSlot cur;
-    __shared__ long long lb[kBins * 32];
+    __shared__ long long lb[256];
     __shared__ float red[kIcpThreads / 64][64];
     const IcpSlot* prev = &a.sh->slot[(a.k + 1) & 1];
     const int n = a.w * a.h;
@@ -883,7 +900,7 @@ __global__ __launch_bounds__(kIcpThreads) void k_icp_iter(IcpArgs a)
     const uint32_t sw = slot_word(prev);
     float4 v4[PPT], n4[PPT];
     load_tile<PPT>(a, blockIdx.x, n, v4, n4);
-    const long long bv = load_bins(a);
+    const BinWords bv = load_bins(a);
     put_slot_word(cur, sw);
     __syncthreads();
     // a launch after this level converged with nothing left to solve only hands the slot on; it
@@ -976,9 +993,9 @@ template<bool Comp>
 __global__ __launch_bounds__(kIcpThreads) void k_icp_finish(IcpArgs a)
 {
     __shared__ IcpSlot cur;
-    __shared__ long long lb[kBins * 32];
+    __shared__ long long lb[256];
     const uint32_t sw = slot_word(&a.sh->slot[(a.k + 1) & 1]);
-    const long long bv = load_bins(a);
+    const BinWords bv = load_bins(a);
     put_slot_word(cur, sw);
     __syncthreads();
     if (cur.did_pixels) {
