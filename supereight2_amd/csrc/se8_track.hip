@@ -895,17 +895,20 @@ __global__ __launch_bounds__(kIcpThreads) void k_icp_iter(IcpArgs a)
     __shared__ float red[kIcpThreads / 64][64];
     const IcpSlot* prev = &a.sh->slot[(a.k + 1) & 1];
     const int n = a.w * a.h;
-    // issue order = wait order: the slot word first, then the tile and the bins; the slot is stored
-    // (waiting for it alone) and read before the bins are awaited
+    // issue order = wait order: the slot word first, then the bins; the slot is stored (waiting for
+    // it alone) and read before the bins are awaited
     const uint32_t sw = slot_word(prev);
-    float4 v4[PPT], n4[PPT];
-    load_tile<PPT>(a, blockIdx.x, n, v4, n4);
     const BinWords bv = load_bins(a);
     put_slot_word(cur, sw);
     __syncthreads();
     // a launch after this level converged with nothing left to solve only hands the slot on; it
-    // does not wait for the bins (their load was issued ahead, in case it was needed)
+    // does not wait for the bins (their load was issued ahead, in case it was needed) and does not
+    // read its tile.  Any other launch issues its tile's loads now: they arrive while the previous
+    // pass is solved.
     const bool noop = cur.did_pixels == 0 && cur.done[a.level] != 0;
+    float4 v4[PPT], n4[PPT];
+    if (!noop)
+        load_tile<PPT>(a, blockIdx.x, n, v4, n4);
     bool run = false;
     if (!noop) {
         if (cur.did_pixels) {
