@@ -224,7 +224,9 @@ SE8_HD void cam_backproject(const Cam& c, float u, float v, float o[3])
 
 SE8_HD bool cam_project(const Cam& c, const float p[3], float uv[2])
 {
-    if (fabs((double) p[2]) < 1.0e-12)
+    // |z| < 1e-12 compared in double: 0x1.19799ap-40 is the least float above 1e-12, so the float
+    // comparison below selects exactly the same z
+    if (fabsf(p[2]) < 0x1.19799ap-40f)
         return false;
     const float rz = 1.f / p[2];
     const float x = p[0] * rz, y = p[1] * rz;
@@ -483,6 +485,20 @@ inline Cam make_cam(const se8_pinhole& p, int dsf, bool scaled)
 
 // The sensor interface of the reference's templated integrator (SensorBase: backProject, project,
 // farDist, near_plane, measurementFromPoint, sphereInFrustum) for the two sensor types.
+// sqrtf(x) for x >= 2^-96, +inf and NaN: the hardware root corrected by the signs of its two
+// neighbours' residuals, the correctly rounded sequence without the scaling of small inputs and the
+// selects for zero and infinity (below 2^-96 the residuals can underflow and the result is not the
+// rounded root).
+__device__ __forceinline__ float sqrt_rn_large(float x)
+{
+    const float s = __builtin_amdgcn_sqrtf(x);
+    const float s_dn = __int_as_float(__float_as_int(s) - 1);
+    const float s_up = __int_as_float(__float_as_int(s) + 1);
+    float r = fmaf(-s_dn, s, x) <= 0.f ? s_dn : s;
+    r = fmaf(-s_up, s, x) > 0.f ? s_up : r;
+    return r;
+}
+
 struct PinholeSensor {
     Cam c;
     SE8_HD int width() const { return c.w; }
@@ -493,11 +509,15 @@ struct PinholeSensor {
     SE8_HD bool project(const float p[3], float uv[2]) const { return cam_project(c, p, uv); }
     SE8_HD float far_dist(const float p[3]) const { return cam_far_dist(c, p); }
     // nrm > far_dist(p) with nrm = |p| (the updater's far test), without the two divisions in the
-    // common case: far_dist = far / (z / nrm) is bracketed by an estimate with a relative error
-    // below 1e-6 (a hardware reciprocal and three products), and only a point inside that bracket
-    // takes the exact path, so the result is that of the exact expression.
+    // common case.  For z >= +0 (sign bit clear), z / nrm <= 1 rounds to at most 1, so far_dist >= far
+    // and a point with nrm <= far is never beyond it.  Otherwise far_dist = far / (z / nrm) is
+    // bracketed by an estimate with a relative error below 1e-6 (a hardware reciprocal and three
+    // products), and only a point inside that bracket takes the exact path, so the result is that of
+    // the exact expression.
     __device__ bool beyond_far(const float p[3], float nrm) const
     {
+        if (__float_as_int(p[2]) >= 0 && nrm <= c.far_plane)
+            return false;
         if (p[2] < 0.f)
             return true; // z / nrm < 0, so far_dist < 0 <= nrm
         if (p[2] > 0.f && nrm > 0.f) {
@@ -512,6 +532,9 @@ struct PinholeSensor {
     }
     SE8_HD float measurement(const float p[3]) const { return p[2]; }
     SE8_HD bool sphere_in_frustum(const float ctr[3], float r) const { return cam_sphere_in_frustum(c, ctr, r); }
+    // project() rejects every point with |p|^2 < 2^-96 (then |z| < 2^-48 < 1e-12), so the updater's
+    // |p| only matters at or above 2^-96, where sqrt_rn_large is the rounded root
+    static constexpr bool kRejectsNearOrigin = true;
 };
 
 struct LidarSensor {
@@ -532,6 +555,7 @@ struct LidarSensor {
     __device__ bool beyond_far(const float*, float nrm) const { return nrm > L.far_plane; }
     SE8_HD float measurement(const float p[3]) const { return sqrtf(sqnorm3(p)); }
     SE8_HD bool sphere_in_frustum(const float ctr[3], float r) const { return lidar_sphere_in_frustum(L, ctr, r); }
+    static constexpr bool kRejectsNearOrigin = false;
 };
 
 // ----------------------------------------------------------------------------------------------

@@ -131,7 +131,7 @@ __device__ __forceinline__ VoxelSample sample_voxel(const Iso& T_CV, const Senso
     float pC[3];
     iso_apply(T_CV, v, pC);
     VoxelSample o;
-    o.nrm = sqrtf(sqnorm3(pC));
+    o.nrm = Sensor::kRejectsNearOrigin ? sqrt_rn_large(sqnorm3(pC)) : sqrtf(sqnorm3(pC));
     o.mz = sensor.measurement(pC);
     float uv[2] = {0.f, 0.f};
     o.ok = live && !sensor.beyond_far(pC, o.nrm) && sensor.project(pC, uv);

@@ -40,6 +40,7 @@
 #define __shared__ static
 #define __launch_bounds__(...)
 #define __builtin_amdgcn_rcpf(x) (1.0f / (x))
+#define __builtin_amdgcn_sqrtf(x) sqrtf(x)
 #define SE8_REG_BARRIER(v) ((void) (v))
 
 struct dim3 {
@@ -307,6 +308,18 @@ inline T __shfl_up(T x, unsigned delta, int line = __builtin_LINE())
     return shfl_any(x, l >= (int) delta ? l - (int) delta : l, line);
 }
 
+inline int __float_as_int(float f)
+{
+    int i;
+    std::memcpy(&i, &f, sizeof(i));
+    return i;
+}
+inline float __int_as_float(int i)
+{
+    float f;
+    std::memcpy(&f, &i, sizeof(f));
+    return f;
+}
 inline int __popcll(unsigned long long v)
 {
     return __builtin_popcountll(v);
