@@ -1004,6 +1004,7 @@ struct se8_tracker {
     int num_wg[SE8_TRACKER_MAX_LEVELS] = {};    // Gauss-Newton grid per level
     int icp_ppt[SE8_TRACKER_MAX_LEVELS] = {};   // pixels per thread per tile
     int icp_tiles[SE8_TRACKER_MAX_LEVELS] = {}; // tiles of 256 * ppt pixels
+    int adj_wg = 0;                             // batched adjoint grid over level 0
     void* host_state = nullptr; // pinned readback of the device tracking state
     bool track_pending = false;
     se8_stage_times times{};

@@ -33,9 +33,12 @@
 namespace se8 {
 
 constexpr int kIcpThreads = 256;
-// Cap on the Gauss-Newton grid: levels larger than kIcpMaxWg tiles of kIcpThreads * 4 pixels loop
-// over tiles.
+// Cap on the batched adjoint's grid (its partials, kIcpMaxWg per pose); levels larger than that many
+// tiles of kIcpThreads * 4 pixels loop over tiles.
 constexpr int kIcpMaxWg = 320;
+// Cap on a Gauss-Newton launch's grid: the workgroups hand over through the fixed-point bins, so a
+// larger grid costs the next launch nothing to read.
+constexpr int kIcpTrackMaxWg = 640;
 constexpr int kPyrThreads = 256;
 constexpr int kMaxLevels = SE8_TRACKER_MAX_LEVELS;
 // Hand-over between Gauss-Newton launches: workgroup i adds its partial row, in 64-bit fixed point,
@@ -1321,16 +1324,24 @@ extern "C" int se8_tracker_create(const se8_pinhole* sensor, const se8_tracker_c
         const size_t n = (size_t) cl.w * cl.h;
         rc = rc ? rc : dev_alloc(t->vert_lvl[l], sizeof(float4) * n);
         rc = rc ? rc : dev_alloc(t->norm_lvl[l], sizeof(float4) * n);
-        // smallest pixels-per-thread whose tiles fit one grid of at most kIcpMaxWg workgroups: more,
-        // lighter workgroups keep each wave's row arithmetic short, which costs less than the extra
-        // partial rows the next launch reads (320x240 and 160x120 are faster in 300 and 75 workgroups
-        // of 256 pixels than in 75 and 19 of 1024)
+        // smallest pixels-per-thread whose tiles fit one grid of at most kIcpTrackMaxWg workgroups:
+        // more, lighter workgroups keep each wave's row arithmetic short (320x240 and 160x120 are
+        // faster in 300 and 75 workgroups of 256 pixels than in 75 and 19 of 1024)
         int ppt = 1;
-        while (ppt < 4 && (n + kIcpThreads * ppt - 1) / (kIcpThreads * ppt) > (size_t) kIcpMaxWg)
+        while (ppt < 4 && (n + kIcpThreads * ppt - 1) / (kIcpThreads * ppt) > (size_t) kIcpTrackMaxWg)
             ppt *= 2;
         t->icp_ppt[l] = ppt;
         t->icp_tiles[l] = (int) std::max<size_t>(1, (n + kIcpThreads * ppt - 1) / (kIcpThreads * ppt));
-        t->num_wg[l] = std::min(t->icp_tiles[l], kIcpMaxWg);
+        t->num_wg[l] = std::min(t->icp_tiles[l], kIcpTrackMaxWg);
+    }
+    {
+        // the batched adjoint's grid over level 0: at most kIcpMaxWg workgroups (its scratch)
+        const size_t n = (size_t) t->sensor.width * t->sensor.height;
+        int ppt = 1;
+        while (ppt < 4 && (n + kIcpThreads * ppt - 1) / (kIcpThreads * ppt) > (size_t) kIcpMaxWg)
+            ppt *= 2;
+        t->adj_wg = (int) std::min<size_t>(std::max<size_t>(1, (n + kIcpThreads * ppt - 1) / (kIcpThreads * ppt)),
+                                           (size_t) kIcpMaxWg);
     }
     rc = rc ? rc : dev_alloc(t->bins, sizeof(long long) * 3 * kBins * 32);
     rc = rc ? rc : dev_alloc(t->state, sizeof(IcpShared));
@@ -1446,7 +1457,7 @@ extern "C" int se8_icp_cost_grad_batched(se8_tracker* t, const float* depth_dev,
     SE8_REQUIRE(n_poses > 0 && n_poses <= 4096, "se8_icp_cost_grad_batched: n_poses out of range");
     hipStream_t s = (hipStream_t) stream;
     const se8_pinhole& P = t->sensor;
-    const int n_wg = t->num_wg[0];
+    const int n_wg = t->adj_wg;
     int rc = dev_alloc(t->adj_poses, sizeof(float) * 12 * n_poses);
     rc = rc ? rc : dev_alloc(t->adj_partials, sizeof(float) * kAdjTerms * n_wg * n_poses);
     rc = rc ? rc : dev_alloc(t->adj_out, sizeof(double) * kAdjTerms * n_poses);
@@ -1503,7 +1514,7 @@ extern "C" int se8_icp_cost_grad_batched_noalloc(se8_tracker* t, const float* de
                 "se8_icp_cost_grad_batched_noalloc: null argument");
     SE8_REQUIRE(n_poses > 0 && n_poses <= kNoAllocMaxPoses,
                 "se8_icp_cost_grad_batched_noalloc: n_poses must be in [1, 64] (the size of the library's scratch)");
-    const int n_wg = t->num_wg[0];
+    const int n_wg = t->adj_wg;
     SE8_REQUIRE(n_wg > 0 && n_wg <= kIcpMaxWg, "se8_icp_cost_grad_batched_noalloc: tracker grid out of range");
     float* partials = nullptr;
     int rc = noalloc_scratch_ptr(&partials);
