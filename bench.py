@@ -192,7 +192,7 @@ def mesh_bench(gmap, torch, se, cpu):
 
 
 def run_pipeline(depth_dev, gt, n_frames, warmup, steps, storage, compensated, torch, se, sensor, timing="off",
-                 device_pose=True, keep_points=False, dump=False, mesh=None, graph=False):
+                 device_pose=True, keep_points=False, dump=False, mesh=None, graph=False, prepare=False):
     """Runs the per-frame loop over all n_frames frames: frames [0, warmup) untimed, exactly `steps`
     frames [warmup, warmup + steps) timed, the rest of the sequence untimed after them (its time is
     kept separately for the whole-sequence rate).  With device_pose (the default), each frame's pose
@@ -203,7 +203,9 @@ def run_pipeline(depth_dev, gt, n_frames, warmup, steps, storage, compensated, t
     level device times from HIP events, accumulated over the timed frames).  keep_points keeps every
     frame's raycast vertex map on the device; dump returns the final map's blocks.  With graph, frames
     1.. are replays of one captured frame graph (se8_frame_graph_launch: the same kernels with the
-    same arguments, dispatched back to back); it needs device poses, timing off and no kept points."""
+    same arguments, dispatched back to back); it needs device poses, timing off and no kept points.
+    With prepare (device poses), frame k's raycast also builds frame k+1's tracker pyramid in the same
+    launch (se8_raycast_dev_prepare_track), and frame k+1's track skips its own."""
     H, W = sensor.height, sensor.width
     gmap = se.TSDFMap(10.24, 0.01, storage=storage)
     integ = se.MapIntegrator(gmap)
@@ -244,7 +246,10 @@ def run_pipeline(depth_dev, gt, n_frames, warmup, steps, storage, compensated, t
             if k > 0:
                 tracker.track_device(depth_dev[k], T_dev[k - 1], T_dev[k], pts[pp], nrm[prev])
             integ.integrateDepth(k, depth_dev[k], sensor, T_dev[k])
-            se.raycast_volume(gmap, sensor, T_dev[k], pts[pk], nrm[cur])
+            if prepare and k + 1 < n_frames:
+                se.raycast_volume_prepare_track(gmap, sensor, T_dev[k], pts[pk], nrm[cur], tracker, depth_dev[k + 1])
+            else:
+                se.raycast_volume(gmap, sensor, T_dev[k], pts[pk], nrm[cur])
         else:
             if k > 0:
                 _, T, _ = tracker.track(depth_dev[k], T, pts[pp], nrm[prev])
@@ -671,7 +676,10 @@ def main():
 
     print("fp32 pipeline", file=sys.stderr, flush=True)
     r32 = run_pipeline(depth_dev, gt, n_frames, warmup, steps, "f32", False, torch, se, sensor, timing="off",
-                       keep_points=fp64, dump=fp64, mesh="gpu" if args.no_cpu_baseline else "cpu")
+                       keep_points=fp64, dump=fp64, mesh="gpu" if args.no_cpu_baseline else "cpu", prepare=True)
+    print("fp32 pipeline, pyramid in its own launch", file=sys.stderr, flush=True)
+    r32u = run_pipeline(depth_dev, gt, n_frames, warmup, steps, "f32", False, torch, se, sensor, timing="off")
+    unprepared_equal32 = bool(np.array_equal(np.stack(r32u["poses"]), np.stack(r32["poses"])))
     print("fp32 pipeline, frame graph", file=sys.stderr, flush=True)
     r32g = run_pipeline(depth_dev, gt, n_frames, warmup, steps, "f32", False, torch, se, sensor, timing="off",
                         graph=True)
@@ -686,7 +694,7 @@ def main():
     if not args.no_fp16:
         print("fp16 pipeline", file=sys.stderr, flush=True)
         r16 = run_pipeline(depth_dev, gt, n_frames, warmup, steps, "f16", True, torch, se, sensor, timing="off",
-                           keep_points=fp64, dump=fp64)
+                           keep_points=fp64, dump=fp64, prepare=True)
         print("fp16 pipeline, frame graph", file=sys.stderr, flush=True)
         r16g = run_pipeline(depth_dev, gt, n_frames, warmup, steps, "f16", True, torch, se, sensor, timing="off",
                             graph=True)
@@ -776,8 +784,13 @@ def main():
                                  "timed run records no events",
                    "poses": "device-resident (track -> integrate -> raycast without host round trips)",
                    "frame_graph_fps": r32g["fps"], "frame_graph_poses_equal": graph_equal32,
+                   "unprepared_fps": r32u["fps"], "unprepared_poses_equal": unprepared_equal32,
+                   "prepare_note": "value: each frame's raycast also builds the next frame's tracker pyramid in the "
+                                   "same launch (se8_raycast_dev_prepare_track); unprepared_fps: the same loop with "
+                                   "the pyramid as its own launch inside the track call",
                    "frame_graph_note": "the same loop with frames 1.. replayed from one captured frame graph "
-                                       "(se8_frame_graph_launch); value is the per-call loop",
+                                       "(se8_frame_graph_launch), the pyramid in the track's own launch; "
+                                       "value is the per-call loop",
                    "host_pose_fps": r32h["fps"], "host_pose_ate_m": ate(r32h["poses"], gt),
                    "update_blocks_per_frame": r32["update_blocks_per_frame"],
                    "candidates_per_frame": r32["candidates_per_frame"]},

@@ -228,6 +228,17 @@ int se8_raycast_dev(const se8_map* map, const se8_pinhole* sensor, const float* 
                     float* normals_dev, int8_t* scale_dev, se8_stream stream);
 int se8_track_dev(se8_tracker* tracker, const float* depth_dev, const float* T_in_dev, float* T_out_dev,
                   const float* ref_points_dev, const float* ref_normals_dev, int* ok_dev, se8_stream stream);
+/* se8_raycast_dev, and in the same launch the input side of the tracker's next call -- the pyramid
+ * of Tracker::track (tracker_impl.hpp:50-75) for next_depth_dev, starting from the pose at T_WS_dev
+ * -- for the loop of app/src/main.cpp:98-162, whose next track starts from the pose just raycast
+ * (no counterpart in the reference).  The next se8_track_dev on this tracker with depth_dev ==
+ * next_depth_dev and T_in_dev == T_WS_dev skips its own pyramid; results are identical to
+ * se8_raycast_dev followed by that se8_track_dev.  The depth image and the pose at T_WS_dev must
+ * not change in between; any other pyramid of the tracker (another track, a frame graph replay, a
+ * batched cost evaluation) discards the prepared one. */
+int se8_raycast_dev_prepare_track(const se8_map* map, const se8_pinhole* sensor, const float* T_WS_dev,
+                                  float* points_dev, float* normals_dev, int8_t* scale_dev, se8_tracker* tracker,
+                                  const float* next_depth_dev, se8_stream stream);
 
 /* -------- frame graphs (no counterpart in the reference; for callers that run the loop of
  * app/src/main.cpp:98-162 at device speed).  One frame of that loop -- Tracker::track from the

@@ -313,6 +313,7 @@ extern "C" int se8_frame_graph_launch(se8_frame_graph* g, int timestamp, const f
         return SE8_ERR_CAPACITY;
     }
     hipStream_t s = (hipStream_t) stream;
+    g->tr->prepared_depth = g->tr->prepared_T = nullptr; // the replay's pyramid overwrites a prepared one
     const int p = (int) (g->launches & 1);
     // the previous replay of this instantiation (two frames back) must have finished before its
     // arguments change

@@ -1014,6 +1014,10 @@ struct se8_tracker {
     // last launch of the call (index SE8_TRACKER_MAX_LEVELS); created on first use
     hipEvent_t ev_lvl[SE8_TRACKER_MAX_LEVELS + 1]{};
     bool lvl_pending = false;
+    // se8_raycast_dev_prepare_track built the pyramid of the next call for this depth and start pose
+    // (device pointers): a track call with the same two skips its own pyramid launch
+    const float* prepared_depth = nullptr;
+    const float* prepared_T = nullptr;
 };
 
 struct se8_lidar {

@@ -17,6 +17,7 @@
 #include <vector>
 
 #include "se8_internal.hpp"
+#include "se8_pyramid.hpp"
 
 namespace se8 {
 
@@ -30,7 +31,7 @@ K pick_kernel(const MapView& m, K f32_brick, K f16_brick, K f32_dense, K f16_den
 }
 
 constexpr int kStackDepth = 23;  // CAST_STACK_DEPTH (voxel_block_ray_iterator.hpp:15)
-constexpr int kMaxLevels = 16;   // log2(size / 8) for the largest supported octree (2^19)
+constexpr int kMaxOctreeLevels = 16;   // log2(size / 8) for the largest supported octree (2^19)
 constexpr int kRayThreads = 128; // 16 x 8 pixel tile
 
 // Dynamic LDS of a raycast workgroup: the traversal stack of every thread.
@@ -654,19 +655,19 @@ __global__ __launch_bounds__(kRayThreads) void k_raycast_depth_grad(MapView m, C
     valid[idx] = ok;
 }
 
+// The rays of the 16x8 pixel tile (bx, by); ray_stack is the workgroup's dynamic LDS.
 template <int ST>
-__global__ __launch_bounds__(kRayThreads) void k_raycast(MapView m, Cam cam, Iso T_arg, const float* T16_dev,
-                                                         float* points, float* normals, int8_t* scale_out)
+__device__ void raycast_tile(const MapView& m, const Cam& cam, const Iso& T_arg, const float* T16_dev, float* points,
+                             float* normals, int8_t* scale_out, int bx, int by, int* ray_stack)
 {
     // the pose comes from device memory when T16_dev is set (a pose a previous kernel produced)
     const Iso T_WS = T16_dev ? iso_from16(T16_dev) : T_arg;
     // traversal stack: (max_scale - 3) levels of {parent, t_max} per thread, sized at launch
-    extern __shared__ int ray_stack[];
     int* st_parent = ray_stack;
     float* st_tmax = reinterpret_cast<float*>(ray_stack + (m.max_scale - kBlockScale) * kRayThreads);
     const int t = threadIdx.x;
-    const int x = blockIdx.x * 16 + (t & 15);
-    const int y = blockIdx.y * 8 + (t >> 4);
+    const int x = bx * 16 + (t & 15);
+    const int y = by * 8 + (t >> 4);
     if (x >= cam.w || y >= cam.h)
         return;
     const size_t idx = x + (size_t) y * cam.w;
@@ -698,6 +699,31 @@ __global__ __launch_bounds__(kRayThreads) void k_raycast(MapView m, Cam cam, Iso
     normals[3 * idx + 0] = nrm[0];
     normals[3 * idx + 1] = nrm[1];
     normals[3 * idx + 2] = nrm[2];
+}
+
+template <int ST>
+__global__ __launch_bounds__(kRayThreads) void k_raycast(MapView m, Cam cam, Iso T_arg, const float* T16_dev,
+                                                         float* points, float* normals, int8_t* scale_out)
+{
+    extern __shared__ int ray_stack[];
+    raycast_tile<ST>(m, cam, T_arg, T16_dev, points, normals, scale_out, blockIdx.x, blockIdx.y, ray_stack);
+}
+
+// The raycast and, in the same launch, the pyramid of a tracker's next call (pyramid_tile over
+// 128-thread workgroups): workgroups below n_ray take the raycast's tiles (n_ray_x per row), the
+// rest the pyramid's.  The pyramid's workgroups are dispatched after the raycast's and run while
+// the raycast's slowest rays finish, on a device the raycast leaves mostly idle by then.
+template <int ST>
+__global__ __launch_bounds__(kRayThreads) void k_raycast_pyramid(MapView m, Cam cam, const float* T16_dev,
+                                                                 float* points, float* normals, int8_t* scale_out,
+                                                                 int n_ray_x, int n_ray, PyrArgs pa)
+{
+    extern __shared__ int lds[];
+    const int b = blockIdx.x;
+    if (b < n_ray)
+        raycast_tile<ST>(m, cam, Iso{}, T16_dev, points, normals, scale_out, b % n_ray_x, b / n_ray_x, lds);
+    else
+        pyramid_tile<kRayThreads>(pa, b - n_ray, reinterpret_cast<float*>(lds));
 }
 
 // ----------------------------------------------------------------------------------------------
@@ -797,7 +823,7 @@ extern "C" int se8_raycast(const se8_map* m, const se8_pinhole* sensor, const fl
     SE8_REQUIRE(!m || !is_multires(m), "se8_raycast: not available for a multi-resolution map");
     SE8_REQUIRE(m && sensor && T_WS16 && points_dev && normals_dev, "se8_raycast: null argument");
     SE8_REQUIRE(sensor->width > 0 && sensor->height > 0, "se8_raycast: empty image");
-    SE8_REQUIRE(m->view.max_scale - kBlockScale <= kMaxLevels, "se8_raycast: octree too deep");
+    SE8_REQUIRE(m->view.max_scale - kBlockScale <= kMaxOctreeLevels, "se8_raycast: octree too deep");
     hipStream_t s = (hipStream_t) stream;
     se8_map* mm = const_cast<se8_map*>(m);
     const Cam cam = make_cam(*sensor, 1, false);
@@ -815,7 +841,7 @@ extern "C" int se8_raycast_dev(const se8_map* m, const se8_pinhole* sensor, cons
     SE8_REQUIRE(!m || !is_multires(m), "se8_raycast_dev: not available for a multi-resolution map");
     SE8_REQUIRE(m && sensor && T_WS_dev && points_dev && normals_dev, "se8_raycast_dev: null argument");
     SE8_REQUIRE(sensor->width > 0 && sensor->height > 0, "se8_raycast_dev: empty image");
-    SE8_REQUIRE(m->view.max_scale - kBlockScale <= kMaxLevels, "se8_raycast_dev: octree too deep");
+    SE8_REQUIRE(m->view.max_scale - kBlockScale <= kMaxOctreeLevels, "se8_raycast_dev: octree too deep");
     hipStream_t s = (hipStream_t) stream;
     se8_map* mm = const_cast<se8_map*>(m);
     const Cam cam = make_cam(*sensor, 1, false);
@@ -825,6 +851,41 @@ extern "C" int se8_raycast_dev(const se8_map* m, const se8_pinhole* sensor, cons
                             timed ? mm->ev[5] : nullptr, timed ? mm->ev[6] : nullptr);
     mm->ray_pending = timed;
     return rc;
+}
+
+extern "C" int se8_raycast_dev_prepare_track(const se8_map* m, const se8_pinhole* sensor, const float* T_WS_dev,
+                                            float* points_dev, float* normals_dev, int8_t* scale_dev,
+                                            se8_tracker* tracker, const float* next_depth_dev, se8_stream stream)
+{
+    SE8_REQUIRE(!m || !is_multires(m), "se8_raycast_dev_prepare_track: not available for a multi-resolution map");
+    SE8_REQUIRE(m && sensor && T_WS_dev && points_dev && normals_dev && tracker && next_depth_dev,
+                "se8_raycast_dev_prepare_track: null argument");
+    SE8_REQUIRE(sensor->width > 0 && sensor->height > 0, "se8_raycast_dev_prepare_track: empty image");
+    SE8_REQUIRE(m->view.max_scale - kBlockScale <= kMaxOctreeLevels, "se8_raycast_dev_prepare_track: octree too deep");
+    hipStream_t s = (hipStream_t) stream;
+    se8_map* mm = const_cast<se8_map*>(m);
+    const Cam cam = make_cam(*sensor, 1, false);
+    const bool timed = mm->timing != SE8_TIMING_OFF;
+    harvest_raycast(mm, true);
+    PyrArgs pa{};
+    int n_pyr = 0;
+    int rc = tracker_prepare_pyramid(tracker, next_depth_dev, T_WS_dev, &pa, &n_pyr);
+    if (rc != SE8_OK)
+        return rc;
+    const int n_ray_x = (cam.w + 15) / 16, n_ray = n_ray_x * ((cam.h + 7) / 8);
+    auto* kr = pick_kernel(m->view, k_raycast_pyramid<0>, k_raycast_pyramid<1>, k_raycast_pyramid<2>,
+                           k_raycast_pyramid<3>);
+    const uint32_t lds = (uint32_t) std::max(ray_stack_bytes(m->view), pyramid_lds_bytes(pa.levels));
+    const dim3 grid((unsigned) (n_ray + n_pyr));
+    if (timed)
+        hipExtLaunchKernelGGL(kr, grid, dim3(kRayThreads), lds, s, mm->ev[5], mm->ev[6], 0, m->view, cam, T_WS_dev,
+                              points_dev, normals_dev, scale_dev, n_ray_x, n_ray, pa);
+    else
+        hipLaunchKernelGGL(kr, grid, dim3(kRayThreads), lds, s, m->view, cam, T_WS_dev, points_dev, normals_dev,
+                           scale_dev, n_ray_x, n_ray, pa);
+    SE8_HIP_CHECK(hipGetLastError());
+    mm->ray_pending = timed;
+    return SE8_OK;
 }
 
 extern "C" int se8_get_data(const se8_map* m, const float* pts, int64_t n, float* tsdf, float* weight)
@@ -879,7 +940,7 @@ extern "C" int se8_raycast_depth_grad_batched(const se8_map* m, const se8_pinhol
     SE8_REQUIRE(m && sensor && poses16 && depth_dev && grad_dev && valid_dev,
                 "se8_raycast_depth_grad_batched: null argument");
     SE8_REQUIRE(n_poses > 0 && n_poses <= 65535, "se8_raycast_depth_grad_batched: n_poses out of range");
-    SE8_REQUIRE(m->view.max_scale - kBlockScale <= kMaxLevels, "se8_raycast_depth_grad_batched: octree too deep");
+    SE8_REQUIRE(m->view.max_scale - kBlockScale <= kMaxOctreeLevels, "se8_raycast_depth_grad_batched: octree too deep");
     hipStream_t s = (hipStream_t) stream;
     const Cam cam = make_cam(*sensor, 1, false);
     std::vector<float> p12((size_t) 12 * n_poses);

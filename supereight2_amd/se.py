@@ -517,6 +517,22 @@ def raycast_volume(map_: TSDFMap, sensor: PinholeCamera, T_WS, points_W, normals
                                     _dev_f32(normals_W, shape), sp, _stream_handle(stream)), "se8_raycast")
 
 
+def raycast_volume_prepare_track(map_: TSDFMap, sensor: PinholeCamera, T_WS, points_W, normals_W, tracker,
+                                 next_depth, scale=None, stream=None):
+    """raycast_volume at a device pose, with the pyramid of the tracker's next call (next_depth, from
+    the same pose) in the same launch (se8_raycast_dev_prepare_track): the next track_device with
+    that depth tensor and T_in == T_WS skips its pyramid."""
+    shape = (sensor.height, sensor.width, 3)
+    sp = _dev_out(scale, (_torch().int8,), sensor.height * sensor.width, "scale")
+    Td = _dev_pose(T_WS)
+    if Td is None:
+        raise ValueError("raycast_volume_prepare_track takes a device pose")
+    capi.check(map_.lib.se8_raycast_dev_prepare_track(
+        map_.h, C.byref(sensor.c()), Td, _dev_f32(points_W, shape), _dev_f32(normals_W, shape), sp, tracker.h,
+        _dev_f32(next_depth, (tracker.sensor.height, tracker.sensor.width)), _stream_handle(stream)),
+        "se8_raycast_dev_prepare_track")
+
+
 class Tracker:
     """se::Tracker<MapT, SensorT> (tracker.hpp:35-84)."""
 
