@@ -1,26 +1,29 @@
 // Projective point-to-plane ICP of libse8.so: se::Tracker::track
 // (include/se/tracker/impl/tracker_impl.hpp:17-116).
 //
-// Per call: one launch builds the vertex and normal maps of all three pyramid levels (robust
-// half-sampling preprocessor.cpp:156-195, preprocessor_impl.hpp:18-40, preprocessor.cpp:109-152)
-// and initialises the device tracking state; then one launch per Gauss-Newton iteration.  Launch
-// k first solves the system of launch k - 1 in every workgroup (the same fixed-order reduction of
-// its partial rows, 6x6 solve, T <- exp(x) T and convergence test, icp.cpp:15-41, 208-223, so all
+// Per call: one launch builds the vertex and normal maps of all the pyramid levels (robust
+// half-sampling preprocessor.cpp:156-195, preprocessor_impl.hpp:18-40, preprocessor.cpp:109-152;
+// se8_pyramid.hpp) and initialises the device tracking state, unless a raycast launch already did
+// (se8_raycast_dev_prepare_track); then one launch per Gauss-Newton iteration.  Launch k first
+// solves the system of launch k - 1 in every workgroup (the exact sum of its fixed-point bins, 6x6
+// solve across one wave, T <- exp(x) T and convergence test, icp.cpp:15-41, 208-223, so all
 // workgroups agree bit for bit), then runs its own pixel pass: trackKernel (icp_impl.hpp:16-102)
 // fused with the 32-term sums of reduceKernel (icp.cpp:45-204), per-thread accumulation over a
 // tile of pixels whose loads are issued together, a halving wave shuffle and an LDS reduction to
-// one partial row per workgroup.  Workgroups never wait on each other: the kernel boundary hands
-// the rows and the pose to the next launch, so there is no device-wide fence or atomic.  Passes of
-// a converged level are skipped, so the host never waits inside the pyramid loop.  A final
-// one-workgroup launch solves the last pass and applies checkPoseKernel (icp.cpp:227-246).
+// one partial row per workgroup, added into the launch's fixed-point bins.  Workgroups never wait
+// on each other: the kernel boundary hands the sums and the pose to the next launch, so there is no
+// device-wide fence.  Passes of a converged level are skipped, so the host never waits inside the
+// pyramid loop.  A final one-workgroup launch solves the last pass and applies checkPoseKernel
+// (icp.cpp:227-246).
 //
-// compensated = 0: fp32 sums, fixed-order fp32 cross-workgroup sum, fp32 Cholesky (reference).
+// compensated = 0: fp32 sums within a workgroup, exact fixed-point sum across workgroups, fp32
+// Cholesky (the reference's precision).
 // compensated = 1: fp32 sums over a thread's few pixels and a fp32 pairwise tree within each wave,
 // compensated (TwoSum sum and error pairs) combination of the workgroup's waves into its partial,
-// FP64 cross-workgroup sum and FP64 LDL^T solve.
+// exact fixed-point sum across workgroups and FP64 LDL^T solve.
 //
 // A launch after its level has converged (the reference's break, icp.cpp:208-223) with nothing left
-// to solve only hands the state on, without waiting for the partial rows.
+// to solve only hands the state on, without waiting for the bins or reading its tile.
 #include <hip/hip_runtime.h>
 
 #include <cmath>
