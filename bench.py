@@ -205,7 +205,8 @@ def run_pipeline(depth_dev, gt, n_frames, warmup, steps, storage, compensated, t
     1.. are replays of one captured frame graph (se8_frame_graph_launch: the same kernels with the
     same arguments, dispatched back to back); it needs device poses, timing off and no kept points.
     With prepare (device poses), frame k's raycast also builds frame k+1's tracker pyramid in the same
-    launch (se8_raycast_dev_prepare_track), and frame k+1's track skips its own."""
+    launch (se8_raycast_dev_prepare_track) and frame k+1's track skips its own, and each track's last
+    launch also starts the frame's integration (se8_track_dev_prepare_integrate)."""
     H, W = sensor.height, sensor.width
     gmap = se.TSDFMap(10.24, 0.01, storage=storage)
     integ = se.MapIntegrator(gmap)
@@ -243,7 +244,10 @@ def run_pipeline(depth_dev, gt, n_frames, warmup, steps, storage, compensated, t
         if fg is not None and k > 0:
             fg.launch(k, depth_dev[k], T_dev[k - 1], T_dev[k])
         elif device_pose:
-            if k > 0:
+            if k > 0 and prepare:
+                se.track_device_prepare_integrate(tracker, depth_dev[k], T_dev[k - 1], T_dev[k], pts[pp], nrm[prev],
+                                                  gmap)
+            elif k > 0:
                 tracker.track_device(depth_dev[k], T_dev[k - 1], T_dev[k], pts[pp], nrm[prev])
             integ.integrateDepth(k, depth_dev[k], sensor, T_dev[k])
             if prepare and k + 1 < n_frames:
@@ -786,8 +790,9 @@ def main():
                    "frame_graph_fps": r32g["fps"], "frame_graph_poses_equal": graph_equal32,
                    "unprepared_fps": r32u["fps"], "unprepared_poses_equal": unprepared_equal32,
                    "prepare_note": "value: each frame's raycast also builds the next frame's tracker pyramid in the "
-                                   "same launch (se8_raycast_dev_prepare_track); unprepared_fps: the same loop with "
-                                   "the pyramid as its own launch inside the track call",
+                                   "same launch (se8_raycast_dev_prepare_track) and each track's last launch starts "
+                                   "the integration (se8_track_dev_prepare_integrate); unprepared_fps: the same loop "
+                                   "with the pyramid and the frame start as launches of their own",
                    "frame_graph_note": "the same loop with frames 1.. replayed from one captured frame graph "
                                        "(se8_frame_graph_launch), the pyramid in the track's own launch; "
                                        "value is the per-call loop",

@@ -236,6 +236,14 @@ int se8_track_dev(se8_tracker* tracker, const float* depth_dev, const float* T_i
  * se8_raycast_dev followed by that se8_track_dev.  The depth image and the pose at T_WS_dev must
  * not change in between; any other pyramid of the tracker (another track, a frame graph replay, a
  * batched cost evaluation) discards the prepared one. */
+/* se8_track_dev, whose last launch also starts the next integration of map at the tracked pose (the
+ * per-frame state that integrateDepth's first step sets up, map_integrator_impl.hpp:52-83): the next
+ * se8_integrate_depth_dev on map with T_WC_dev == T_out_dev skips that launch; results are
+ * identical to se8_track_dev followed by that integration (no counterpart in the reference).  Any
+ * other integration of map discards the prepared start. */
+int se8_track_dev_prepare_integrate(se8_tracker* tracker, const float* depth_dev, const float* T_in_dev,
+                                    float* T_out_dev, const float* ref_points_dev, const float* ref_normals_dev,
+                                    int* ok_dev, se8_map* map, se8_stream stream);
 int se8_raycast_dev_prepare_track(const se8_map* map, const se8_pinhole* sensor, const float* T_WS_dev,
                                   float* points_dev, float* normals_dev, int8_t* scale_dev, se8_tracker* tracker,
                                   const float* next_depth_dev, se8_stream stream);

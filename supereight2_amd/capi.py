@@ -166,6 +166,7 @@ SIGNATURES = {
     "se8_integrate_range_dev": (C.c_int, [_P, C.c_int, _P, _P, _P, _P]),
     "se8_raycast_dev": (C.c_int, [_P, C.POINTER(Pinhole), _P, _P, _P, _P, _P]),
     "se8_raycast_dev_prepare_track": (C.c_int, [_P, C.POINTER(Pinhole), _P, _P, _P, _P, _P, _P, _P]),
+    "se8_track_dev_prepare_integrate": (C.c_int, [_P, _P, _P, _P, _P, _P, _P, _P, _P]),
     "se8_track_dev": (C.c_int, [_P, _P, _P, _P, _P, _P, _P, _P]),
     "se8_frame_graph_create": (C.c_int, [_P, _P, C.POINTER(Pinhole), _P, _P, _P, _P, C.POINTER(_P)]),
     "se8_frame_graph_launch": (C.c_int, [_P, C.c_int, _P, _P, _P, _P, _P]),

@@ -29,22 +29,8 @@ namespace se8 {
 // read from device memory when T16_dev is set, else taken from the argument.
 __global__ void k_frame_begin(MapView m, Iso T_arg, const float* T16_dev, FramePoses* fp)
 {
-    if (threadIdx.x == 0) {
-        m.counters[kCntFrustum] = 0;
-        m.counters[kCntUpdate] = 0;
-        m.counters[kCntCandidates] = 0;
-        m.counters[kCntUnique] = 0;
-        m.counters[kCntMrNew] = 0;
-        m.counters[kCntMrDown] = 0;
-        m.counters[kCntNodesStart] = m.counters[kCntNodes];
-        m.counters[kCntBlocksStart] = m.counters[kCntBlocks];
-        for (int i = 0; i < 4; i++)
-            m.counters[kCntMrScale + i] = 0;
-        const Iso T = T16_dev ? iso_from16(T16_dev) : T_arg;
-        fp->T_WS = T;
-        fp->T_SW = iso_inv(T);
-        fp->T_CV = compute_T_CV(m, T);
-    }
+    if (threadIdx.x == 0)
+        frame_begin_state(m.counters, fp, m.t_MW, m.res, T16_dev ? iso_from16(T16_dev) : T_arg);
 }
 
 // Exclusive prefix over the wave of per-lane counts in [0, 16) and the wave's total, by ballots of
@@ -939,8 +925,14 @@ int allocate_frame(se8_map* m, const Iso& T_arg, const float* T16_dev, const flo
     const bool claim = m->view.block_dense && m->view.node_dense;
     FramePoses* fp = (FramePoses*) m->frame_poses.ptr;
     int* ulist = (int*) m->update_list.ptr;
-    hipLaunchKernelGGL(k_frame_begin, dim3(1), dim3(64), 0, s, m->view, T_arg, T16_dev, fp);
-    graph_note_launch(kNodeFrameBegin, s);
+    // a tracker that ended at this pose pointer already started the integration
+    // (se8_track_dev_prepare_integrate); otherwise k_frame_begin does
+    const bool prepared = T16_dev && T16_dev == m->prepared_T;
+    m->prepared_T = nullptr;
+    if (!prepared) {
+        hipLaunchKernelGGL(k_frame_begin, dim3(1), dim3(64), 0, s, m->view, T_arg, T16_dev, fp);
+        graph_note_launch(kNodeFrameBegin, s);
+    }
     const int cus = device_count_sm();
     if (claim) {
         // fetch-frustum and the carver in one launch (ev_frustum then marks the same point as

@@ -314,6 +314,7 @@ extern "C" int se8_frame_graph_launch(se8_frame_graph* g, int timestamp, const f
     }
     hipStream_t s = (hipStream_t) stream;
     g->tr->prepared_depth = g->tr->prepared_T = nullptr; // the replay's pyramid overwrites a prepared one
+    g->map->prepared_T = nullptr;                         // and its frame start the prepared integration
     const int p = (int) (g->launches & 1);
     // the previous replay of this instantiation (two frames back) must have finished before its
     // arguments change

@@ -719,23 +719,29 @@ struct FramePoses {
 };
 
 // T_CV = T_CW * T_WM * Scaling(res) * Translation(0.5) (singleres_tsdf_updater_impl.hpp:37-42)
-SE8_HD Iso compute_T_CV(const MapView& v, const Iso& T_WC)
+SE8_HD Iso compute_T_CV_from(const float t_MW[3], float res, const Iso& T_WC)
 {
     const Iso T_CW = iso_inv(T_WC);
     Iso T_WM;
     for (int i = 0; i < 3; i++) {
         for (int j = 0; j < 3; j++)
             T_WM.R[i][j] = i == j ? 1.f : 0.f;
-        T_WM.t[i] = -v.t_MW[i];
+        T_WM.t[i] = -t_MW[i];
     }
     Iso A = iso_mul(T_CW, T_WM);
     for (int i = 0; i < 3; i++)
         for (int j = 0; j < 3; j++)
-            A.R[i][j] = A.R[i][j] * v.res;
+            A.R[i][j] = A.R[i][j] * res;
     for (int i = 0; i < 3; i++)
         A.t[i] = A.t[i] + (A.R[i][0] * 0.5f + (A.R[i][1] * 0.5f + A.R[i][2] * 0.5f));
     return A;
 }
+
+SE8_HD Iso compute_T_CV(const MapView& v, const Iso& T_WC)
+{
+    return compute_T_CV_from(v.t_MW, v.res, T_WC);
+}
+
 
 enum Counter {
     kCntBlocks = 0,
@@ -754,6 +760,26 @@ enum Counter {
     kCntBlocksStart = 21, // block count at the start of the integration in flight
     kNumCounters = 22
 };
+
+// The start of an integration (one thread): clears the per-frame counters, records where the frame's
+// new nodes and blocks begin, and derives the frame's poses from T (k_frame_begin, or the tracker's
+// last launch when it prepares the integration that follows it).
+SE8_HD void frame_begin_state(int* counters, FramePoses* fp, const float t_MW[3], float res, const Iso& T)
+{
+    counters[kCntFrustum] = 0;
+    counters[kCntUpdate] = 0;
+    counters[kCntCandidates] = 0;
+    counters[kCntUnique] = 0;
+    counters[kCntMrNew] = 0;
+    counters[kCntMrDown] = 0;
+    counters[kCntNodesStart] = counters[kCntNodes];
+    counters[kCntBlocksStart] = counters[kCntBlocks];
+    for (int i = 0; i < 4; i++)
+        counters[kCntMrScale + i] = 0;
+    fp->T_WS = T;
+    fp->T_SW = iso_inv(T);
+    fp->T_CV = compute_T_CV_from(t_MW, res, T);
+}
 
 // Wave-aggregated append: every lane with pred gets the next free position of *counter (one
 // atomic per wave); -1 for the other lanes.
@@ -975,6 +1001,9 @@ struct se8_map {
     bool overflowed = false;       // a capacity overflow was reported: integration is refused from then on
     bool integ_pending = false;    // events ev[0..4] of the last integration not yet harvested
     bool ray_pending = false;      // events ev[5..6] of the last raycast not yet harvested
+    // se8_track_dev_prepare_integrate started the next integration at the pose at this device
+    // address (k_frame_begin's work): an integration at the same pose pointer skips that launch
+    const float* prepared_T = nullptr;
     se8_stage_times times{};
     hipEvent_t ev[8]{};
     int timing = SE8_TIMING_STAGES; // se8_set_stage_timing

@@ -60,6 +60,12 @@ struct IcpArgs {
     int k;                 // launch index within the call
     float* T_out_dev;      // k_icp_finish: the tracked pose also goes here (16 floats) when set
     int* ok_dev;           // k_icp_finish: the check result also goes here when set
+    // k_icp_finish, when fb_counters is set: the start of the integration that follows at the
+    // tracked pose (frame_begin_state on that map's counters and frame poses)
+    int* fb_counters;
+    FramePoses* fb_fp;
+    float fb_t_MW[3];
+    float fb_res;
 };
 
 // The pyramid over one 256-thread workgroup per tile (pyramid_tile, se8_pyramid.hpp).
@@ -808,6 +814,8 @@ __global__ __launch_bounds__(kIcpThreads) void k_icp_finish(IcpArgs a)
         }
         if (a.ok_dev)
             *a.ok_dev = r->ok;
+        if (a.fb_counters)
+            frame_begin_state(a.fb_counters, a.fb_fp, a.fb_t_MW, a.fb_res, iso_from16(r->T));
     }
 }
 
@@ -888,7 +896,8 @@ static void launch_icp_iter(int ppt, int grid, const IcpArgs& a, hipStream_t s)
 // final check.  The start pose is T16_host, or 16 floats at T_in_dev; the result lands in
 // IcpShared::result, and also at T_out_dev / ok_dev when they are set.  Nothing waits on the host.
 static int enqueue_track(se8_tracker* tr, const float* depth, const float* T16_host, const float* T_in_dev,
-                         float* T_out_dev, int* ok_dev, const float* ref_pts, const float* ref_nrm, hipStream_t s)
+                         float* T_out_dev, int* ok_dev, const float* ref_pts, const float* ref_nrm, hipStream_t s,
+                         se8_map* begin_map = nullptr)
 {
     const se8_pinhole& P = tr->sensor;
     IcpShared* sh = (IcpShared*) tr->state.ptr;
@@ -938,6 +947,14 @@ static int enqueue_track(se8_tracker* tr, const float* depth, const float* T16_h
         SE8_HIP_CHECK(hipEventRecord(tr->ev_lvl[kMaxLevels], s));
     a.T_out_dev = T_out_dev;
     a.ok_dev = ok_dev;
+    if (begin_map) {
+        // the finish kernel also starts begin_map's next integration at the tracked pose
+        a.fb_counters = begin_map->view.counters;
+        a.fb_fp = (FramePoses*) begin_map->frame_poses.ptr;
+        for (int i = 0; i < 3; i++)
+            a.fb_t_MW[i] = begin_map->view.t_MW[i];
+        a.fb_res = begin_map->view.res;
+    }
     if (tr->cfg.compensated)
         k_icp_finish<true><<<1, kIcpThreads, 0, s>>>(a);
     else
@@ -1212,6 +1229,20 @@ extern "C" int se8_track_dev(se8_tracker* t, const float* depth_dev, const float
                 "se8_track_dev: null argument");
     return enqueue_track(t, depth_dev, nullptr, T_in_dev, T_out_dev, ok_dev, ref_points_dev, ref_normals_dev,
                          (hipStream_t) stream);
+}
+
+extern "C" int se8_track_dev_prepare_integrate(se8_tracker* t, const float* depth_dev, const float* T_in_dev,
+                                               float* T_out_dev, const float* ref_points_dev,
+                                               const float* ref_normals_dev, int* ok_dev, se8_map* map,
+                                               se8_stream stream)
+{
+    SE8_REQUIRE(t && depth_dev && T_in_dev && T_out_dev && ref_points_dev && ref_normals_dev && map,
+                "se8_track_dev_prepare_integrate: null argument");
+    const int rc = enqueue_track(t, depth_dev, nullptr, T_in_dev, T_out_dev, ok_dev, ref_points_dev,
+                                 ref_normals_dev, (hipStream_t) stream, map);
+    if (rc == SE8_OK)
+        map->prepared_T = T_out_dev;
+    return rc;
 }
 
 extern "C" int se8_tracker_read_pyramid(const se8_tracker* t, int level, float* points, float* normals)

@@ -594,6 +594,20 @@ class Tracker:
                    "se8_track_dev")
 
 
+def track_device_prepare_integrate(tracker, depth, T_in, T_out, points_W, normals_W, map_, ok=None, stream=None):
+    """Tracker.track_device whose last launch also starts map_'s next integration at the tracked pose
+    (se8_track_dev_prepare_integrate): the next integrateDepth on map_ with the pose tensor T_out
+    skips that launch."""
+    shape = (tracker.sensor.height, tracker.sensor.width)
+    Ti, To = _dev_pose(T_in), _dev_pose(T_out)
+    if Ti is None or To is None:
+        raise ValueError("track_device_prepare_integrate takes device poses")
+    okp = _dev_out(ok, (_torch().int32,), 1, "ok")
+    capi.check(tracker.lib.se8_track_dev_prepare_integrate(
+        tracker.h, _dev_f32(depth, shape), Ti, To, _dev_f32(points_W, shape + (3,)), _dev_f32(normals_W, shape + (3,)),
+        okp, map_.h, _stream_handle(stream)), "se8_track_dev_prepare_integrate")
+
+
 class FrameGraph:
     """One frame of the tracking loop (Tracker::track, integrateDepth, raycast_volume) as a replayed
     HIP graph (se8_frame_graph_*).  points/normals: two (H, W, 3) float32 CUDA tensors each; launch i

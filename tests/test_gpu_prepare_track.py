@@ -1,9 +1,12 @@
-"""se8_raycast_dev_prepare_track against the per-call loop, through the C-ABI.
+"""se8_raycast_dev_prepare_track and se8_track_dev_prepare_integrate against the per-call loop,
+through the C-ABI.
 
-Raycasting frame k with the pyramid of frame k+1 in the same launch, then tracking frame k+1 (which
-skips its own pyramid), must give what se8_raycast_dev followed by se8_track_dev gives: every pose,
-check result and raycast bit-identical, and the final maps' blocks, weights and tsdf bit-identical.
-A track call whose depth or start pose differs from the prepared ones runs its own pyramid.
+Raycasting frame k with the pyramid of frame k+1 in the same launch, tracking frame k+1 (which
+skips its own pyramid) with the start of its integration in the tracker's last launch, then
+integrating it (which skips its frame start), must give what the separate calls give: every pose,
+check result and raycast bit-identical, and the final maps' blocks, weights and tsdf
+bit-identical.  A track call whose depth or start pose differs from the prepared ones runs its own
+pyramid.
 """
 import numpy as np
 import pytest
@@ -28,6 +31,7 @@ def frames():
 
 
 def _run(se, depth, gt, storage, compensated, fused, swap_at=None):
+    # fused: each raycast prepares the next track's pyramid and each track the next integration
     sensor = _sensor(se)
     gmap = se.TSDFMap(10.24, 0.01, storage=storage)
     tracker = se.Tracker(gmap, sensor, se.TrackerConfig(compensated=compensated))
@@ -45,7 +49,10 @@ def _run(se, depth, gt, storage, compensated, fused, swap_at=None):
         if k > 0:
             # at swap_at the prepared depth is replaced by a copy: the track must run its own pyramid
             d = depth[k].clone() if k == swap_at else depth[k]
-            tracker.track_device(d, T[k - 1], T[k], pts[prev], nrm[prev], ok=ok[k])
+            if fused:
+                se.track_device_prepare_integrate(tracker, d, T[k - 1], T[k], pts[prev], nrm[prev], gmap, ok=ok[k])
+            else:
+                tracker.track_device(d, T[k - 1], T[k], pts[prev], nrm[prev], ok=ok[k])
         integ.integrateDepth(k, depth[k], sensor, T[k])
         if fused and k + 1 < N_FRAMES:
             se.raycast_volume_prepare_track(gmap, sensor, T[k], pts[cur], nrm[cur], tracker, depth[k + 1])
